@@ -1,0 +1,707 @@
+/*
+ * impactz_oracle.c -- CPU restatement of the IMPACT-Z space-charge step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Each function
+ * cites the reference routine it restates; operation order and the REAL(4)
+ * literals of the reference are kept where they change FP64 results:
+ *   - update_CompDom pads by eps = 1.0/(N-3.0) evaluated in REAL(4)
+ *     (src/Appl/CompDom.f90:547-549);
+ *   - conv1st wraps the phase with pilc = 2.0*asin(1.0) in REAL(4)
+ *     (src/Appl/BeamBunch.f90:1268);
+ *   - scatter1 uses fpei = Clight*Clight*1.0e-7 with a REAL(4) 1.0e-7
+ *     (src/Appl/BeamBunch.f90:2071);
+ *   - diagnostic1 takes cube roots with the REAL(4) exponent 1.0/3.0
+ *     (src/Contrl/Output.f90:281).
+ * The only intentional deviation: with a deck current below 1e-10 A the
+ * reference still calls kick1 with the previous step's field
+ * (src/Contrl/AccSimulator.f90:1124,1417-1425); or_step skips that kick.
+ */
+#include "impactz_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Pts1(c, i) with 1-based component c, 0-based particle i. */
+#define P(c, i) pts[9 * (int64_t)(i) + (c) - 1]
+
+void or_physconst(double freq, or_const *c) {
+  /* src/DataStruct/PhysConst.f90:28-33 */
+  c->clight = 299792458.0;
+  c->pi = 2.0 * asin(1.0);
+  c->scxl = c->clight / (2.0 * c->pi * freq);
+  c->scfreq = freq;
+}
+
+/* ---------------------------------------------------------------- maps */
+
+/* intfunc1_Quadrupole, constant gradient (src/Appl/Quadrupole.f90:254-305). */
+static void quad_rhs(const or_const *c, double bgrad, double qmcc,
+                     const double *y, double *f) {
+  double gamma0 = -y[5];
+  double beta0 = sqrt(gamma0 * gamma0 - 1.0) / gamma0;
+  double gbet = sqrt((gamma0 - 1.0) * (gamma0 + 1.0));
+  double brho, s11, s33, s55;
+  f[0] = f[1] = f[2] = f[3] = 0.0;
+  f[4] = 1.0 / (beta0 * c->scxl);
+  f[5] = 0.0;
+  brho = gbet / c->clight / qmcc;
+  s11 = bgrad / brho * c->scxl;
+  s33 = -bgrad / brho * c->scxl;
+  s55 = 0.0;
+  f[6] = y[7] / c->scxl;
+  f[7] = -s11 * y[6];
+  f[8] = y[9] / c->scxl;
+  f[9] = -s11 * y[8];
+  f[10] = y[11] / c->scxl;
+  f[11] = -s33 * y[10];
+  f[12] = y[13] / c->scxl;
+  f[13] = -s33 * y[12];
+  f[14] = y[15] / c->scxl;
+  f[15] = -s55 * y[14];
+  f[16] = y[17] / c->scxl;
+  f[17] = -s55 * y[16];
+}
+
+void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
+                 double refpt[6], double charge, double mass, double xm[36]) {
+  /* maplinear_Quadrupole + rk6i_Quadrupole (src/Appl/Quadrupole.f90:120-252) */
+  double y[18], yt[18], f[18], a[18], b[18], cc[18], d[18], e[18], g[18],
+      o[18], p[18];
+  double h, gi, betai, ui, squi, sqi3, gf, betaf, uf, squf, sqf3;
+  double qmcc = charge / mass;
+  int i, k;
+  memset(y, 0, sizeof y);
+  y[4] = refpt[4];
+  y[5] = refpt[5];
+  y[6] = 1.0;
+  y[9] = 1.0;
+  y[10] = 1.0;
+  y[13] = 1.0;
+  y[14] = 1.0;
+  y[17] = 1.0;
+  h = tau / mapstp;
+  gi = -refpt[5];
+  betai = sqrt(gi * gi - 1.0) / gi;
+  ui = gi * betai;
+  squi = sqrt(ui);
+  sqi3 = squi * squi * squi;
+  for (i = 0; i < mapstp; i++) {
+    quad_rhs(c, grad, qmcc, y, f);
+    for (k = 0; k < 18; k++) { a[k] = h * f[k]; yt[k] = y[k] + a[k] / 9.0; }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      b[k] = h * f[k];
+      yt[k] = y[k] + (a[k] + 3.0 * b[k]) / 24.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      cc[k] = h * f[k];
+      yt[k] = y[k] + (a[k] - 3.0 * b[k] + 4.0 * cc[k]) / 6.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      d[k] = h * f[k];
+      yt[k] = y[k] + (278.0 * a[k] - 945.0 * b[k] + 840.0 * cc[k] + 99.0 * d[k]) / 544.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      e[k] = h * f[k];
+      yt[k] = y[k] + (-106.0 * a[k] + 273.0 * b[k] - 104.0 * cc[k] - 107.0 * d[k] +
+                      48.0 * e[k]) / 6.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      g[k] = h * f[k];
+      yt[k] = y[k] + (110974.0 * a[k] - 236799.0 * b[k] + 68376.0 * cc[k] +
+                      103803.0 * d[k] - 10240.0 * e[k] + 1926.0 * g[k]) / 45648.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      o[k] = h * f[k];
+      yt[k] = y[k] + (-101195.0 * a[k] + 222534.0 * b[k] - 71988.0 * cc[k] -
+                      26109.0 * d[k] - 2.0e4 * e[k] - 72.0 * g[k] + 22824.0 * o[k]) /
+                         25994.0;
+    }
+    quad_rhs(c, grad, qmcc, yt, f);
+    for (k = 0; k < 18; k++) {
+      p[k] = h * f[k];
+      y[k] = y[k] + (41.0 * a[k] + 216.0 * cc[k] + 27.0 * d[k] + 272.0 * e[k] +
+                     27.0 * g[k] + 216.0 * o[k] + 41.0 * p[k]) / 840.0;
+    }
+  }
+  refpt[4] = y[4];
+  refpt[5] = y[5];
+  gf = -refpt[5];
+  betaf = sqrt(gf * gf - 1.0) / gf;
+  uf = gf * betaf;
+  squf = sqrt(uf);
+  sqf3 = squf * squf * squf;
+  /* xm row-major: xm[6*(j-1)+(k-1)] = xm(j,k); dlti = dltf = thli = thlf = 0 */
+  for (k = 0; k < 36; k++) xm[k] = 0.0;
+  xm[0] = y[6] * squi / squf;
+  xm[6] = y[7] * squi * squf;
+  xm[1] = y[8] / (squi * squf);
+  xm[7] = y[9] * squf / squi;
+  xm[14] = y[10] * squi / squf;
+  xm[20] = y[11] * squi * squf;
+  xm[15] = y[12] / (squi * squf);
+  xm[21] = y[13] * squf / squi;
+  xm[28] = y[14] * sqi3 / sqf3;
+  xm[34] = y[15] * sqi3 * sqf3;
+  xm[29] = y[16] / (sqi3 * sqf3);
+  xm[35] = y[17] * sqf3 / sqi3;
+}
+
+void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
+             double *z, double tau, int bitype, double grad, int mapstp,
+             double charge, double mass) {
+  int64_t i;
+  if (bitype == 0) {
+    /* exact drift, src/Appl/BeamBunch.f90:276-298 */
+    double gam0 = -refpt[5];
+    double gambet0 = sqrt(gam0 * gam0 - 1.0);
+    double beta0 = gambet0 / gam0;
+    for (i = 0; i < n; i++) {
+      double tpx = P(2, i), tpy = P(4, i), tpt = P(6, i);
+      double d = tpt - gam0;
+      double tmph = sqrt(d * d - 1.0 - tpx * tpx - tpy * tpy);
+      P(1, i) = P(1, i) + tpx * tau / tmph / c->scxl;
+      P(3, i) = P(3, i) + tpy * tau / tmph / c->scxl;
+      P(5, i) = P(5, i) - (1.0 / beta0 + (tpt - gam0) / tmph) * tau / c->scxl;
+    }
+    refpt[4] = refpt[4] + tau / (c->scxl * beta0);
+  } else {
+    /* 6x6 map, src/Appl/BeamBunch.f90:308-320 */
+    double xm[36];
+    or_quad_map(c, tau, grad, mapstp, refpt, charge, mass, xm);
+    for (i = 0; i < n; i++) {
+      double t[6];
+      int j, k;
+      for (j = 0; j < 6; j++) {
+        t[j] = 0.0;
+        for (k = 0; k < 6; k++) t[j] = t[j] + P(k + 1, i) * xm[6 * j + k];
+      }
+      for (j = 0; j < 6; j++) P(j + 1, i) = t[j];
+    }
+  }
+  *z = *z + tau;
+}
+
+/* ------------------------------------------------------- frame changes */
+
+int64_t or_conv1st(const or_const *c, double *pts, int64_t n,
+                   const double refpt[6], double xrad, double yrad,
+                   double ptrange[6], int64_t *npt_total, double *current) {
+  const double pilc = (double)(2.0f * asinf(1.0f));
+  const double xl = c->scxl;
+  const double gam = -refpt[5];
+  const double rad = (xrad + yrad) / 2.0;
+  int64_t i0, i, ilost = 0;
+  int k;
+  for (k = 0; k < 3; k++) {
+    ptrange[2 * k] = (double)1.0e20f;
+    ptrange[2 * k + 1] = -(double)1.0e20f;
+  }
+  for (i0 = 0; i0 < n; i0++) {
+    int ntmp5;
+    double tmp5, rcpgammai, tmpbet, betai, tmpx, tmpy, tmp0, radtest;
+    i = i0 - ilost;
+    ntmp5 = (int)(P(5, i0) / pilc);
+    tmp5 = P(5, i0) - ntmp5 * pilc;
+    P(5, i0) = tmp5 - (ntmp5 % 2) * pilc;
+    rcpgammai = 1.0 / (-P(6, i0) + gam);
+    tmpbet = 1.0 - rcpgammai * rcpgammai * (1.0 + P(2, i0) * P(2, i0) + P(4, i0) * P(4, i0));
+    betai = sqrt(fabs(tmpbet));
+    tmpx = P(1, i0) * xl;
+    P(1, i) = (P(1, i0) - P(2, i0) * P(5, i0) * rcpgammai) * xl;
+    tmp0 = fmin(tmpx, P(1, i));
+    if (ptrange[0] > tmp0) ptrange[0] = tmp0;
+    tmp0 = fmax(tmpx, P(1, i));
+    if (ptrange[1] < tmp0) ptrange[1] = tmp0;
+    P(2, i) = P(2, i0);
+    tmpy = P(3, i0) * xl;
+    P(3, i) = (P(3, i0) - P(4, i0) * P(5, i0) * rcpgammai) * xl;
+    tmp0 = fmin(tmpy, P(3, i));
+    if (ptrange[2] > tmp0) ptrange[2] = tmp0;
+    tmp0 = fmax(tmpy, P(3, i));
+    if (ptrange[3] < tmp0) ptrange[3] = tmp0;
+    P(4, i) = P(4, i0);
+    P(5, i) = -gam * betai * P(5, i0) * xl;
+    if (ptrange[4] > P(5, i)) ptrange[4] = P(5, i);
+    if (ptrange[5] < P(5, i)) ptrange[5] = P(5, i);
+    P(6, i) = P(6, i0);
+    radtest = sqrt(P(1, i) * P(1, i) + P(3, i) * P(3, i));
+    if (radtest >= rad || tmpbet <= 0.0) ilost++;
+    P(7, i) = P(7, i0);
+    P(8, i) = P(8, i0);
+    P(9, i) = P(9, i0);
+  }
+  n -= ilost;
+  /* one process: nptot = nplc; Current = Current*nptot/Npt (:1657-1663) */
+  *current = *current * (double)n / (double)(*npt_total);
+  *npt_total = n;
+  return n;
+}
+
+void or_cvbkforth1st(const or_const *c, double *pts, int64_t n,
+                     const double refpt[6]) {
+  const double xl = c->scxl;
+  const double xk = 1.0 / xl;
+  const double gamma0 = -refpt[5];
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    double rcpgammai = 1.0 / (-P(6, i) + gamma0);
+    double betai = sqrt(1.0 - rcpgammai * rcpgammai *
+                                  (1.0 + P(2, i) * P(2, i) + P(4, i) * P(4, i)));
+    P(5, i) = P(5, i) * xk / (-gamma0 * betai);
+    P(1, i) = P(1, i) * xk + P(2, i) * P(5, i) * rcpgammai;
+    P(3, i) = P(3, i) * xk + P(4, i) * P(5, i) * rcpgammai;
+  }
+  for (i = 0; i < n; i++) {
+    double rcpgammai = 1.0 / (-P(6, i) + gamma0);
+    double betai = sqrt(1.0 - rcpgammai * rcpgammai *
+                                  (1.0 + P(2, i) * P(2, i) + P(4, i) * P(4, i)));
+    P(1, i) = P(1, i) * xl;
+    P(3, i) = P(3, i) * xl;
+    P(5, i) = -gamma0 * betai * P(5, i) * xl;
+  }
+}
+
+/* ------------------------------------------------------------ domain */
+
+void or_update_compdom(const double pr[6], int nx, int ny, int nz,
+                       double range[6], double msize[3]) {
+  const double epsx = (double)(1.0f / ((float)nx - 3.0f));
+  const double epsy = (double)(1.0f / ((float)ny - 3.0f));
+  const double epsz = (double)(1.0f / ((float)nz - 3.0f));
+  range[0] = pr[0] - epsx * (pr[1] - pr[0]);
+  range[1] = pr[1] + epsx * (pr[1] - pr[0]);
+  range[2] = pr[2] - epsy * (pr[3] - pr[2]);
+  range[3] = pr[3] + epsy * (pr[3] - pr[2]);
+  range[4] = pr[4] - epsz * (pr[5] - pr[4]);
+  range[5] = pr[5] + epsz * (pr[5] - pr[4]);
+  msize[0] = (range[1] - range[0]) / (nx - 1);
+  msize[1] = (range[3] - range[2]) / (ny - 1);
+  msize[2] = (range[5] - range[4]) / (nz - 1);
+}
+
+/* ----------------------------------------------------------- deposit */
+
+void or_deposit(const double *pts, int64_t n, int nx, int ny, int nz,
+                const double range[6], const double msize[3], double *rho) {
+  const double hxi = 1.0 / msize[0], hyi = 1.0 / msize[1], hzi = 1.0 / msize[2];
+  const double hx = msize[0], hy = msize[1], hz = msize[2];
+  const double xmin = range[0], ymin = range[2], zmin = range[4];
+  const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
+  int64_t i;
+  memset(rho, 0, sizeof(double) * (size_t)nx * ny * nz);
+  for (i = 0; i < n; i++) {
+    int ix = (int)((P(1, i) - xmin) * hxi + 1.0);
+    double ab = ((xmin - P(1, i)) + ix * hx) * hxi;
+    int jx = (int)((P(3, i) - ymin) * hyi + 1.0);
+    double cd = ((ymin - P(3, i)) + jx * hy) * hyi;
+    int kx = (int)((P(5, i) - zmin) * hzi + 1.0);
+    double ef = ((zmin - P(5, i)) + kx * hz) * hzi;
+    double w = P(8, i);
+    int64_t o;
+    if (ix < 1 || ix + 1 > nx || jx < 1 || jx + 1 > ny || kx < 1 || kx + 1 > nz)
+      continue;
+    o = (ix - 1) * sx + (jx - 1) * sy + (kx - 1) * sz;
+    rho[o] += ab * cd * ef * w;
+    rho[o + sy] += ab * (1.0 - cd) * ef * w;
+    rho[o + sy + sz] += ab * (1.0 - cd) * (1.0 - ef) * w;
+    rho[o + sz] += ab * cd * (1.0 - ef) * w;
+    rho[o + sx + sz] += (1.0 - ab) * cd * (1.0 - ef) * w;
+    rho[o + sx + sy + sz] += (1.0 - ab) * (1.0 - cd) * (1.0 - ef) * w;
+    rho[o + sx + sy] += (1.0 - ab) * (1.0 - cd) * ef * w;
+    rho[o + sx] += (1.0 - ab) * cd * ef * w;
+  }
+}
+
+/* ------------------------------------------------------ Green function */
+
+static double igf_antideriv(double x, double y, double z) {
+  /* src/Appl/Field.f90:401-415 */
+  double rr = sqrt(x * x + y * y + z * z);
+  double aa = -z * z * atan(x * y / (z * rr)) / 2.0;
+  double bb = -y * y * atan(x * z / (y * rr)) / 2.0;
+  double cc = -x * x * atan(y * z / (x * rr)) / 2.0;
+  double dd = y * z * log(x + rr);
+  double ee = x * z * log(y + rr);
+  double ff = x * y * log(z + rr);
+  return aa + bb + cc + dd + ee + ff;
+}
+
+void or_green_igf(int nx, int ny, int nz, double hx, double hy, double hz,
+                  double *g) {
+  const int tx = nx + 2, ty = ny + 2, tz = nz + 2;
+  double *t = (double *)malloc(sizeof(double) * (size_t)tx * ty * tz);
+  const double vol = hx * hy * hz;
+  int i, j, k;
+#define T(a, b, c) t[(a) + (int64_t)tx * ((b) + (int64_t)ty * (c))]
+  for (k = 0; k < tz; k++)
+    for (j = 0; j < ty; j++)
+      for (i = 0; i < tx; i++)
+        T(i, j, k) = igf_antideriv(i * hx - hx / 2.0, j * hy - hy / 2.0,
+                                   k * hz - hz / 2.0) / vol;
+  for (k = 0; k <= nz; k++)
+    for (j = 0; j <= ny; j++)
+      for (i = 0; i <= nx; i++)
+        g[i + (int64_t)(nx + 1) * (j + (int64_t)(ny + 1) * k)] =
+            T(i + 1, j + 1, k + 1) - T(i, j + 1, k + 1) - T(i + 1, j, k + 1) +
+            T(i, j, k + 1) - T(i + 1, j + 1, k) + T(i, j + 1, k) +
+            T(i + 1, j, k) - T(i, j, k);
+#undef T
+  free(t);
+}
+
+/* ------------------------------------------------------------- solver */
+
+/* In-place radix-2 complex FFT of length m (power of 2) on interleaved data,
+ * sign -1 forward, +1 backward, unnormalized. */
+static void fft1(double *re, double *im, int m, int sign) {
+  int i, j, k, len;
+  for (i = 1, j = 0; i < m; i++) {
+    int bit = m >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      double tr = re[i], ti = im[i];
+      re[i] = re[j]; im[i] = im[j];
+      re[j] = tr; im[j] = ti;
+    }
+  }
+  for (len = 2; len <= m; len <<= 1) {
+    int half = len >> 1;
+    for (k = 0; k < half; k++) {
+      double ang = sign * 2.0 * 3.14159265358979323846 * k / len;
+      double wr = cos(ang), wi = sin(ang);
+      for (i = k; i < m; i += len) {
+        double ur = re[i], ui = im[i];
+        double vr = re[i + half] * wr - im[i + half] * wi;
+        double vi = re[i + half] * wi + im[i + half] * wr;
+        re[i] = ur + vr; im[i] = ui + vi;
+        re[i + half] = ur - vr; im[i + half] = ui - vi;
+      }
+    }
+  }
+}
+
+static void fft3(double *re, double *im, int mx, int my, int mz, int sign) {
+  int mmax = mx > my ? mx : my;
+  double *lr, *li;
+  int64_t a, b;
+  mmax = mmax > mz ? mmax : mz;
+  lr = (double *)malloc(sizeof(double) * mmax);
+  li = (double *)malloc(sizeof(double) * mmax);
+  for (b = 0; b < (int64_t)my * mz; b++) fft1(re + b * mx, im + b * mx, mx, sign);
+  for (b = 0; b < mz; b++)
+    for (a = 0; a < mx; a++) {
+      int64_t base = a + (int64_t)mx * my * b;
+      int t;
+      for (t = 0; t < my; t++) { lr[t] = re[base + (int64_t)mx * t]; li[t] = im[base + (int64_t)mx * t]; }
+      fft1(lr, li, my, sign);
+      for (t = 0; t < my; t++) { re[base + (int64_t)mx * t] = lr[t]; im[base + (int64_t)mx * t] = li[t]; }
+    }
+  for (b = 0; b < (int64_t)mx * my; b++) {
+    int64_t s = (int64_t)mx * my;
+    int t;
+    for (t = 0; t < mz; t++) { lr[t] = re[b + s * t]; li[t] = im[b + s * t]; }
+    fft1(lr, li, mz, sign);
+    for (t = 0; t < mz; t++) { re[b + s * t] = lr[t]; im[b + s * t] = li[t]; }
+  }
+  free(lr);
+  free(li);
+}
+
+static int is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int or_solve_openbc(const double *rho, int nx, int ny, int nz, double hx,
+                    double hy, double hz, double *phi) {
+  const int mx = 2 * nx, my = 2 * ny, mz = 2 * nz;
+  const int64_t m = (int64_t)mx * my * mz;
+  double *g, *ar, *ai, *gr, *gi, scale;
+  int i, j, k;
+  if (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return -1;
+  g = (double *)malloc(sizeof(double) * (size_t)(nx + 1) * (ny + 1) * (nz + 1));
+  ar = (double *)calloc((size_t)m, sizeof(double));
+  ai = (double *)calloc((size_t)m, sizeof(double));
+  gr = (double *)malloc(sizeof(double) * (size_t)m);
+  gi = (double *)calloc((size_t)m, sizeof(double));
+  if (!g || !ar || !ai || !gr || !gi) return -2;
+  or_green_igf(nx, ny, nz, hx, hy, hz, g);
+  for (k = 0; k < nz; k++)
+    for (j = 0; j < ny; j++)
+      for (i = 0; i < nx; i++)
+        ar[i + (int64_t)mx * (j + (int64_t)my * k)] =
+            rho[i + (int64_t)nx * (j + (int64_t)ny * k)];
+  /* even extension: G(m) = g(min(m, M-m)) (src/Appl/Field.f90:446-503) */
+  for (k = 0; k < mz; k++)
+    for (j = 0; j < my; j++)
+      for (i = 0; i < mx; i++) {
+        int a = i <= nx ? i : mx - i, b = j <= ny ? j : my - j, c = k <= nz ? k : mz - k;
+        gr[i + (int64_t)mx * (j + (int64_t)my * k)] =
+            g[a + (int64_t)(nx + 1) * (b + (int64_t)(ny + 1) * c)];
+      }
+  fft3(ar, ai, mx, my, mz, -1);
+  fft3(gr, gi, mx, my, mz, -1);
+  for (int64_t t = 0; t < m; t++) {
+    double r = ar[t] * gr[t] - ai[t] * gi[t];
+    double s = ar[t] * gi[t] + ai[t] * gr[t];
+    ar[t] = r;
+    ai[t] = s;
+  }
+  fft3(ar, ai, mx, my, mz, +1);
+  scale = 1.0 / (double)m;
+  for (k = 0; k < nz; k++)
+    for (j = 0; j < ny; j++)
+      for (i = 0; i < nx; i++)
+        phi[i + (int64_t)nx * (j + (int64_t)ny * k)] =
+            ar[i + (int64_t)mx * (j + (int64_t)my * k)] * scale;
+  free(g); free(ar); free(ai); free(gr); free(gi);
+  return 0;
+}
+
+void or_solve_direct(const double *rho, int nx, int ny, int nz, double hx,
+                     double hy, double hz, double *phi) {
+  double *g = (double *)malloc(sizeof(double) * (size_t)(nx + 1) * (ny + 1) * (nz + 1));
+  int i, j, k, a, b, c;
+  or_green_igf(nx, ny, nz, hx, hy, hz, g);
+  for (k = 0; k < nz; k++)
+    for (j = 0; j < ny; j++)
+      for (i = 0; i < nx; i++) {
+        double s = 0.0;
+        for (c = 0; c < nz; c++)
+          for (b = 0; b < ny; b++)
+            for (a = 0; a < nx; a++)
+              s += rho[a + (int64_t)nx * (b + (int64_t)ny * c)] *
+                   g[abs(i - a) + (int64_t)(nx + 1) * (abs(j - b) + (int64_t)(ny + 1) * abs(k - c))];
+        phi[i + (int64_t)nx * (j + (int64_t)ny * k)] = s;
+      }
+  free(g);
+}
+
+/* ------------------------------------------------------- field + kick */
+
+void or_efield(const double *phi, int nx, int ny, int nz, const double msize[3],
+               double *ex, double *ey, double *ez) {
+  const double hxi = 1.0 / msize[0], hyi = 1.0 / msize[1], hzi = 1.0 / msize[2];
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  int i, j, k;
+#define F(a, b, c) phi[(a) + sy * (b) + sz * (c)]
+#define IDX(a, b, c) ((a) + sy * (b) + sz * (c))
+  for (k = 0; k < nz; k++)
+    for (j = 0; j < ny; j++) {
+      ex[IDX(0, j, k)] = hxi * (F(0, j, k) - F(1, j, k));
+      for (i = 1; i < nx - 1; i++)
+        ex[IDX(i, j, k)] = 0.5 * hxi * (F(i - 1, j, k) - F(i + 1, j, k));
+      ex[IDX(nx - 1, j, k)] = hxi * (F(nx - 2, j, k) - F(nx - 1, j, k));
+    }
+  for (k = 0; k < nz; k++) {
+    for (i = 0; i < nx; i++) ey[IDX(i, 0, k)] = hyi * (F(i, 0, k) - F(i, 1, k));
+    for (j = 1; j < ny - 1; j++)
+      for (i = 0; i < nx; i++)
+        ey[IDX(i, j, k)] = 0.5 * hyi * (F(i, j - 1, k) - F(i, j + 1, k));
+    for (i = 0; i < nx; i++)
+      ey[IDX(i, ny - 1, k)] = hyi * (F(i, ny - 2, k) - F(i, ny - 1, k));
+  }
+  for (j = 0; j < ny; j++)
+    for (i = 0; i < nx; i++) ez[IDX(i, j, 0)] = hzi * (F(i, j, 0) - F(i, j, 1));
+  for (k = 1; k < nz - 1; k++)
+    for (j = 0; j < ny; j++)
+      for (i = 0; i < nx; i++)
+        ez[IDX(i, j, k)] = 0.5 * hzi * (F(i, j, k - 1) - F(i, j, k + 1));
+  for (j = 0; j < ny; j++)
+    for (i = 0; i < nx; i++)
+      ez[IDX(i, j, nz - 1)] = hzi * (F(i, j, nz - 2) - F(i, j, nz - 1));
+#undef F
+#undef IDX
+}
+
+static double cic(const double *e, int64_t o, int64_t sy, int64_t sz, double ab,
+                  double cd, double ef) {
+  /* term order of src/Appl/BeamBunch.f90:2130-2137 */
+  return e[o] * ab * cd * ef + e[o + sy] * ab * (1.0 - cd) * ef +
+         e[o + sy + sz] * ab * (1.0 - cd) * (1.0 - ef) +
+         e[o + sz] * ab * cd * (1.0 - ef) +
+         e[o + 1 + sz] * (1.0 - ab) * cd * (1.0 - ef) +
+         e[o + 1 + sy + sz] * (1.0 - ab) * (1.0 - cd) * (1.0 - ef) +
+         e[o + 1 + sy] * (1.0 - ab) * (1.0 - cd) * ef +
+         e[o + 1] * (1.0 - ab) * cd * ef;
+}
+
+void or_scatter1(const or_const *c, double *pts, int64_t n, int nx, int ny,
+                 int nz, const double *ex, const double *ey, const double *ez,
+                 const double range[6], const double msize[3], double gam,
+                 double curr, double tau, double mass, double chrg) {
+  const double tmpscale = curr / c->scfreq;
+  const double twopi = 2.0 * c->pi;
+  const double fpei = c->clight * c->clight * (double)1.0e-7f;
+  const double xk = 1.0 / c->scxl;
+  const double bet = sqrt(gam * gam - 1.0) / gam;
+  const double bbyk = bet / xk;
+  const double gambet = gam * bet;
+  const double brho = gambet / (c->clight * chrg) * mass;
+  const double vz0 = bet * c->clight;
+  const double perv0 = 2.0 * curr * fpei / (brho * vz0 * vz0 * gam * gam);
+  const double xycon = 0.5 * perv0 * gambet * bbyk * twopi / tmpscale;
+  const double tcon = bet * xycon * gam * gam;
+  const double hxi = 1.0 / msize[0], hyi = 1.0 / msize[1], hzi = 1.0 / msize[2];
+  const double hx = msize[0], hy = msize[1], hz = msize[2];
+  const double xmin = range[0], ymin = range[2], zmin = range[4];
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  int64_t i;
+  (void)nz;
+  for (i = 0; i < n; i++) {
+    int ix = (int)((P(1, i) - xmin) * hxi + 1.0);
+    double ab = ((xmin - P(1, i)) + ix * hx) * hxi;
+    int jx = (int)((P(3, i) - ymin) * hyi + 1.0);
+    double cd = ((ymin - P(3, i)) + jx * hy) * hyi;
+    int kx = (int)((P(5, i) - zmin) * hzi + 1.0);
+    double ef = ((zmin - P(5, i)) + kx * hz) * hzi;
+    int64_t o = (ix - 1) + sy * (jx - 1) + sz * (kx - 1);
+    double exn = cic(ex, o, sy, sz, ab, cd, ef) * gam;
+    double eyn = cic(ey, o, sy, sz, ab, cd, ef) * gam;
+    double ezn = cic(ez, o, sy, sz, ab, cd, ef);
+    double rcpgammai = 1.0 / (-P(6, i) + gam);
+    double tmpsq = 1.0 - rcpgammai * rcpgammai * (1.0 + P(2, i) * P(2, i) + P(4, i) * P(4, i));
+    double betai = sqrt(tmpsq);
+    P(1, i) = P(1, i) * xk;
+    P(2, i) = P(2, i) + tau * xycon * exn;
+    P(3, i) = P(3, i) * xk;
+    P(4, i) = P(4, i) + tau * xycon * eyn;
+    P(5, i) = P(5, i) * xk / (-gam * betai);
+    P(6, i) = P(6, i) - tau * tcon * ezn;
+  }
+}
+
+void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+              const double *phi, const double range[6], const double msize[3],
+              const double refpt[6], double curr, double tau, double mass,
+              double chrg) {
+  size_t g = (size_t)nx * ny * nz;
+  double *e;
+  if (curr < (double)1.0e-20f) return;
+  e = (double *)malloc(3 * g * sizeof(double));
+  or_efield(phi, nx, ny, nz, msize, e, e + g, e + 2 * g);
+  or_scatter1(c, pts, n, nx, ny, nz, e, e + g, e + 2 * g, range, msize,
+              -refpt[5], curr, tau, mass, chrg);
+  free(e);
+}
+
+/* -------------------------------------------------------- diagnostics */
+
+void or_diag_sums(const double *pts, int64_t n, double s[37], double amax[6],
+                  double *r2max) {
+  int64_t i;
+  int j;
+  for (j = 0; j < 37; j++) s[j] = 0.0;
+  if (n > 0) {
+    for (j = 0; j < 6; j++) amax[j] = fabs(P(j + 1, 0));
+    *r2max = P(1, 0) * P(1, 0) + P(3, 0) * P(3, 0);
+  } else {
+    for (j = 0; j < 6; j++) amax[j] = 0.0;
+    *r2max = 0.0;
+  }
+  for (i = 0; i < n; i++) {
+    double x = P(1, i), px = P(2, i), y = P(3, i), py = P(4, i), z = P(5, i),
+           pz = P(6, i), r2;
+    s[0] += x;  s[6] += x * x;  s[15] += x * x * x;  s[16] += x * x * x * x;
+    s[12] += x * px;  s[27] += x * px * px;  s[28] += x * px * px * px;
+    s[29] += x * x * px;  s[30] += x * x * px * px;  s[31] += x * x * x * px;
+    s[1] += px;  s[7] += px * px;  s[17] += px * px * px;  s[18] += px * px * px * px;
+    s[2] += y;  s[8] += y * y;  s[19] += y * y * y;  s[20] += y * y * y * y;
+    s[13] += y * py;  s[32] += y * py * py;  s[33] += y * py * py * py;
+    s[34] += y * y * py;  s[35] += y * y * py * py;  s[36] += y * y * y * py;
+    s[3] += py;  s[9] += py * py;  s[21] += py * py * py;  s[22] += py * py * py * py;
+    s[4] += z;  s[10] += z * z;  s[23] += fabs(z * z * z);  s[24] += z * z * z * z;
+    s[14] += z * pz;
+    s[5] += pz;  s[11] += pz * pz;  s[25] += pz * pz * pz;  s[26] += pz * pz * pz * pz;
+    for (j = 0; j < 6; j++)
+      if (amax[j] < fabs(P(j + 1, i))) amax[j] = fabs(P(j + 1, i));
+    r2 = x * x + y * y;
+    if (*r2max < r2) *r2max = r2;
+  }
+}
+
+void or_diag_row(const or_const *c, const double t[37], const double gm[6],
+                 double r2max, int64_t npt, const double refpt[6], double mass,
+                 double out[41]) {
+  const double qmc = mass / 1.0e6, xl = c->scxl, xt = 180.0 / c->pi;
+  const double third = (double)(1.0f / 3.0f);
+  const double den1 = 1.0 / (double)npt;
+  const double gam = -refpt[5];
+  const double gambet = sqrt(gam * gam - 1.0);
+  const double bet = sqrt(gam * gam - 1.0) / gam;
+  const double energy = (gam - 1.0) * qmc;
+  double x0 = t[0] * den1, px0 = t[1] * den1, y0 = t[2] * den1, py0 = t[3] * den1;
+  double z0 = t[4] * den1, pz0 = t[5] * den1;
+  double sqx = t[6] * den1, sqpx = t[7] * den1, sqy = t[8] * den1;
+  double sqpy = t[9] * den1, sqz = t[10] * den1, sqpz = t[11] * den1;
+  double s1 = sqx - x0 * x0, s2 = sqpx - px0 * px0, s3 = sqy - y0 * y0;
+  double s4 = sqpy - py0 * py0, s5 = sqz - z0 * z0, s6 = sqpz - pz0 * pz0;
+  double xpx = t[12] * den1 - x0 * px0, ypy = t[13] * den1 - y0 * py0;
+  double zpz = t[14] * den1 - z0 * pz0;
+  double cubx = t[15] * den1, fthx = t[16] * den1, cubpx = t[17] * den1;
+  double fthpx = t[18] * den1, cuby = t[19] * den1, fthy = t[20] * den1;
+  double cubpy = t[21] * den1, fthpy = t[22] * den1, cubz = t[23] * den1;
+  double fthz = t[24] * den1, cubpz = t[25] * den1, fthpz = t[26] * den1;
+#define M3(c3, sq, m0) pow(fabs((c3) - 3 * (sq) * (m0) + 2 * (m0) * (m0) * (m0)), third)
+#define M4(f4, c3, sq, m0) \
+  sqrt(sqrt(fabs((f4) - 4 * (c3) * (m0) + 6 * (sq) * (m0) * (m0) - 3 * (m0) * (m0) * (m0) * (m0))))
+  double x03 = M3(cubx, sqx, x0), x04 = M4(fthx, cubx, sqx, x0);
+  double px03 = M3(cubpx, sqpx, px0), px04 = M4(fthpx, cubpx, sqpx, px0);
+  double y03 = M3(cuby, sqy, y0), y04 = M4(fthy, cuby, sqy, y0);
+  double py03 = M3(cubpy, sqpy, py0), py04 = M4(fthpy, cubpy, sqpy, py0);
+  double z03 = M3(cubz, sqz, z0), z04 = M4(fthz, cubz, sqz, z0);
+  double pz03 = M3(cubpz, sqpz, pz0), pz04 = M4(fthpz, cubpz, sqpz, pz0);
+#undef M3
+#undef M4
+  double epx = sqrt(fmax(s1 * s2 - xpx * xpx, 0.0));
+  double epy = sqrt(fmax(s3 * s4 - ypy * ypy, 0.0));
+  double epz = sqrt(fmax(s5 * s6 - zpz * zpz, 0.0));
+  double xrms = sqrt(s1), pxrms = sqrt(s2), yrms = sqrt(s3), pyrms = sqrt(s4);
+  double zrms = sqrt(s5), pzrms = sqrt(s6);
+  double v[41] = {
+      refpt[4], gam, energy, bet, sqrt(r2max) * xl,
+      x0 * xl, xrms * xl, px0 / gambet, pxrms / gambet, -xpx / epx, epx * xl,
+      y0 * xl, yrms * xl, py0 / gambet, pyrms / gambet, -ypy / epy, epy * xl,
+      z0 * xt, zrms * xt, pz0 * qmc, pzrms * qmc, -zpz / epz, epz * qmc * xt,
+      gm[0] * xl, gm[1] / gambet, gm[2] * xl, gm[3] / gambet, gm[4] * xt, gm[5] * qmc,
+      x03 * xl, px03 / gambet, y03 * xl, py03 / gambet, z03 * xt, pz03 * qmc,
+      x04 * xl, px04 / gambet, y04 * xl, py04 / gambet, z04 * xt, pz04 * qmc};
+  memcpy(out, v, sizeof v);
+}
+
+/* ------------------------------------------------------------ one step */
+
+int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
+                int nx, int ny, int nz, double tau1, int bitype, double grad,
+                int mapstp, double pipe_radius, double *rho_out,
+                double *phi_out) {
+  size_t g = (size_t)nx * ny * nz;
+  double *rho = rho_out ? rho_out : (double *)malloc(g * sizeof(double));
+  double *phi = phi_out ? phi_out : (double *)malloc(g * sizeof(double));
+  double ptrange[6], range[6], msize[3];
+  const double tau2 = 2.0 * tau1;
+  int rc = 0;
+  or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, grad, mapstp, st->charge,
+          st->mass);
+  if (st->bcurr >= (double)1.0e-10f) {
+    n = or_conv1st(c, pts, n, st->refpt, pipe_radius, pipe_radius, ptrange,
+                   &st->npt, &st->current);
+    or_update_compdom(ptrange, nx, ny, nz, range, msize);
+    or_deposit(pts, n, nx, ny, nz, range, msize, rho);
+    rc = or_solve_openbc(rho, nx, ny, nz, msize[0], msize[1], msize[2], phi);
+    or_cvbkforth1st(c, pts, n, st->refpt);
+    if (rc == 0)
+      or_kick1(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, st->current,
+               tau2, st->mass, st->charge);
+  }
+  or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, grad, mapstp, st->charge,
+          st->mass);
+  if (!rho_out) free(rho);
+  if (!phi_out) free(phi);
+  return rc == 0 ? n : -1;
+}

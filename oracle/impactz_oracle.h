@@ -1,0 +1,127 @@
+/*
+ * impactz_oracle.h -- CPU restatement of the IMPACT-Z space-charge step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker that the HIP library is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it.  Plain C99, FP64, single-threaded, following the
+ * reference routines cited per function.
+ *
+ * Particle arrays use the reference layout Pts1(9,N) (Fortran column-major:
+ * the 9 components of a particle are contiguous, pts[9*i + c]).
+ * Grid arrays use the reference layout rho(Nx,Ny,Nz), x fastest.
+ */
+#ifndef IMPACTZ_ORACLE_H
+#define IMPACTZ_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scalars the reference keeps in module globals (src/DataStruct/PhysConst.f90:25-35). */
+typedef struct {
+  double clight, pi, scxl, scfreq;
+} or_const;
+
+void or_physconst(double freq, or_const *c);
+
+/* Quadrupole linear map, file-ID-0 gradient mode: RK6 over mapstp steps
+ * (src/Appl/Quadrupole.f90:120-305).  refpt is updated as the reference does. */
+void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
+                 double refpt[6], double charge, double mass, double xm[36]);
+
+/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465) for bitype 0 (exact drift)
+ * and bitype 1 (quad map).  z is advanced by tau. */
+void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
+             double *z, double tau, int bitype, double grad, int mapstp,
+             double charge, double mass);
+
+/* conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352,1657-1663).
+ * Compacts lost particles in place; returns the new count; rescales *current. */
+int64_t or_conv1st(const or_const *c, double *pts, int64_t n,
+                   const double refpt[6], double xrad, double yrad,
+                   double ptrange[6], int64_t *npt_total, double *current);
+
+/* update_CompDom, Flagbc=1, one process (src/Appl/CompDom.f90:533-557,614-622). */
+void or_update_compdom(const double ptrange[6], int nx, int ny, int nz,
+                       double range[6], double msize[3]);
+
+/* deposit_BeamBunch (src/Appl/BeamBunch.f90:3678-3768). */
+void or_deposit(const double *pts, int64_t n, int nx, int ny, int nz,
+                const double range[6], const double msize[3], double *rho);
+
+/* greenf1tIntnew spatial part (src/Appl/Field.f90:384-428): the integrated
+ * Green function g(i,j,k), i<=nx, j<=ny, k<=nz, layout (nx+1,ny+1,nz+1). */
+void or_green_igf(int nx, int ny, int nz, double hx, double hy, double hz,
+                  double *g);
+
+/* openBC3D (src/Appl/Field.f90:239-331): phi = first octant of the cyclic
+ * convolution of zero-padded rho with the even-extended g on (2nx,2ny,2nz),
+ * computed with a radix-2 FFT.  nx, ny, nz must be powers of two. */
+int or_solve_openbc(const double *rho, int nx, int ny, int nz, double hx,
+                    double hy, double hz, double *phi);
+
+/* Same potential by direct summation, O(G^2): phi(i) = sum_j rho(j) g(|i-j|).
+ * Independent of any FFT; for tiny grids. */
+void or_solve_direct(const double *rho, int nx, int ny, int nz, double hx,
+                     double hy, double hz, double *phi);
+
+/* cvbkforth1st_BeamBunch (src/Appl/BeamBunch.f90:1667-1701). */
+void or_cvbkforth1st(const or_const *c, double *pts, int64_t n,
+                     const double refpt[6]);
+
+/* E = -grad phi as kick1_BeamBunch forms it, one process, Flagbc=1
+ * (src/Appl/BeamBunch.f90:1782-1799,1885-1899,1966-2005). */
+void or_efield(const double *phi, int nx, int ny, int nz, const double msize[3],
+               double *ex, double *ey, double *ez);
+
+/* scatter1_BeamBunch (src/Appl/BeamBunch.f90:2041-2181): gather + kick +
+ * 0th-order t->z back-transform. */
+void or_scatter1(const or_const *c, double *pts, int64_t n, int nx, int ny,
+                 int nz, const double *ex, const double *ey, const double *ez,
+                 const double range[6], const double msize[3], double gam,
+                 double curr, double tau, double mass, double chrg);
+
+/* kick1_BeamBunch = or_efield + or_scatter1 (skipped below 1e-20 A, :1731). */
+void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+              const double *phi, const double range[6], const double msize[3],
+              const double refpt[6], double curr, double tau, double mass,
+              double chrg);
+
+/* diagnostic1_Output sums (src/Contrl/Output.f90:119-245): 37 sums in the
+ * reference's tmplc order, 6 abs-maxima, max of x^2+y^2. */
+void or_diag_sums(const double *pts, int64_t n, double sums[37], double amax[6],
+                  double *r2max);
+
+/* The derived quantities of one diagnostic row (src/Contrl/Output.f90:247-339),
+ * each column scaled as the reference writes it:
+ * out[0..4]  : fort.18 columns after z (refpt5, gam, energy, bet, rmax)
+ * out[5..10] : fort.24 x0, xrms, px0, pxrms, -xpx/epx, epx
+ * out[11..16]: fort.25 (y), out[17..22]: fort.26 (z)
+ * out[23..28]: fort.27 maxima, out[29..34]: fort.29, out[35..40]: fort.30 */
+void or_diag_row(const or_const *c, const double sums[37], const double amax[6],
+                 double r2max, int64_t npt, const double refpt[6], double mass,
+                 double out[41]);
+
+/* One segment of the reference loop (src/Contrl/AccSimulator.f90:1099-1488)
+ * for a drift (bitype 0) or constant-gradient quad (bitype 1):
+ * map1(tau1) -> conv1st -> update_CompDom -> deposit -> solve -> cvbkforth1st
+ * -> kick1(2 tau1) -> map1(tau1).  Workspace is allocated internally.
+ * Returns the particle count after the step (<0 on error). */
+typedef struct {
+  double bcurr;                   /* deck current; below 1e-10 A no SC block */
+  double current, mass, charge;   /* BeamBunch%Current, %Mass, %Charge */
+  int64_t npt;                    /* BeamBunch%Npt */
+  double refpt[6];                /* BeamBunch%refptcl */
+  double z;
+} or_bunch_state;
+
+int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
+                int nx, int ny, int nz, double tau1, int bitype, double grad,
+                int mapstp, double pipe_radius, double *rho_out,
+                double *phi_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

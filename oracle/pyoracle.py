@@ -1,0 +1,247 @@
+"""ctypes view of the C oracle (liboracle_impactz.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP library is compared against.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load it.
+Particle arrays are (N, 9) float64 C-contiguous numpy arrays, i.e. the
+reference's Pts1(9, N) memory layout; grids are numpy arrays of shape
+(Nz, Ny, Nx) (x fastest, the reference's rho(Nx, Ny, Nz) memory layout).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_impactz.so")
+SRC_PATH = os.path.join(HERE, "impactz_oracle.c")
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+
+class OrConst(C.Structure):
+    _fields_ = [("clight", C.c_double), ("pi", C.c_double),
+                ("scxl", C.c_double), ("scfreq", C.c_double)]
+
+
+class OrState(C.Structure):
+    _fields_ = [("bcurr", C.c_double), ("current", C.c_double),
+                ("mass", C.c_double), ("charge", C.c_double),
+                ("npt", C.c_int64), ("refpt", C.c_double * 6),
+                ("z", C.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (no FMA contraction, like the reference's x86 build)."""
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH):
+        subprocess.check_call(["gcc", "-O2", "-std=c99", "-ffp-contract=off",
+                               "-fPIC", "-shared", SRC_PATH, "-o", LIB_PATH,
+                               "-lm"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_physconst.argtypes = [C.c_double, C.POINTER(OrConst)]
+        L.or_quad_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_double,
+                                  C.c_int, _dp, C.c_double, C.c_double, _dp]
+        L.or_map1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp,
+                              C.c_double, C.c_int, C.c_double, C.c_int,
+                              C.c_double, C.c_double]
+        L.or_conv1st.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp,
+                                 C.c_double, C.c_double, _dp, _i64p, _dp]
+        L.or_conv1st.restype = C.c_int64
+        L.or_update_compdom.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.or_deposit.argtypes = [_dp, C.c_int64, C.c_int, C.c_int, C.c_int,
+                                 _dp, _dp, _dp]
+        L.or_green_igf.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_double, _dp]
+        L.or_solve_openbc.argtypes = [_dp, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, _dp]
+        L.or_solve_openbc.restype = C.c_int
+        L.or_solve_direct.argtypes = [_dp, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, _dp]
+        L.or_cvbkforth1st.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp]
+        L.or_efield.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.or_kick1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.c_int,
+                               C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                               C.c_double, C.c_double, C.c_double, C.c_double]
+        L.or_diag_sums.argtypes = [_dp, C.c_int64, _dp, _dp, _dp]
+        L.or_diag_row.argtypes = [C.POINTER(OrConst), _dp, _dp, C.c_double,
+                                  C.c_int64, _dp, C.c_double, _dp]
+        L.or_step.argtypes = [C.POINTER(OrConst), _dp, C.c_int64,
+                              C.POINTER(OrState), C.c_int, C.c_int, C.c_int,
+                              C.c_double, C.c_int, C.c_double, C.c_int,
+                              C.c_double, _dp, _dp]
+        L.or_step.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def physconst(freq: float) -> OrConst:
+    c = OrConst()
+    lib().or_physconst(freq, C.byref(c))
+    return c
+
+
+def quad_map(c: OrConst, tau, grad, mapstp, refpt, charge, mass):
+    refpt = np.ascontiguousarray(refpt, dtype=np.float64).copy()
+    xm = np.zeros(36)
+    lib().or_quad_map(C.byref(c), tau, grad, mapstp, _p(refpt), charge, mass, _p(xm))
+    return xm.reshape(6, 6), refpt
+
+
+def map1(c, pts, refpt, z, tau, bitype, grad, mapstp, charge, mass):
+    zz = C.c_double(z)
+    lib().or_map1(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz),
+                  tau, bitype, grad, mapstp, charge, mass)
+    return zz.value
+
+
+def conv1st(c, pts, refpt, xrad, yrad, npt, current):
+    """In place on pts; returns (n_after, ptrange, npt, current)."""
+    pr = np.zeros(6)
+    npt_ = C.c_int64(npt)
+    cur = C.c_double(current)
+    n = lib().or_conv1st(C.byref(c), _p(pts), pts.shape[0], _p(refpt), xrad,
+                         yrad, _p(pr), C.byref(npt_), C.byref(cur))
+    return n, pr, npt_.value, cur.value
+
+
+def update_compdom(ptrange, grid):
+    rng, ms = np.zeros(6), np.zeros(3)
+    lib().or_update_compdom(_p(np.ascontiguousarray(ptrange, dtype=np.float64)),
+                            grid[0], grid[1], grid[2], _p(rng), _p(ms))
+    return rng, ms
+
+
+def deposit(pts, grid, rng, ms):
+    nx, ny, nz = grid
+    rho = np.zeros((nz, ny, nx))
+    lib().or_deposit(_p(pts), pts.shape[0], nx, ny, nz, _p(rng), _p(ms), _p(rho))
+    return rho
+
+
+def green_igf(grid, h):
+    nx, ny, nz = grid
+    g = np.zeros((nz + 1, ny + 1, nx + 1))
+    lib().or_green_igf(nx, ny, nz, h[0], h[1], h[2], _p(g))
+    return g
+
+
+def solve_openbc(rho, h):
+    nz, ny, nx = rho.shape
+    phi = np.zeros_like(rho)
+    rc = lib().or_solve_openbc(_p(np.ascontiguousarray(rho)), nx, ny, nz,
+                               h[0], h[1], h[2], _p(phi))
+    if rc != 0:
+        raise ValueError(f"or_solve_openbc failed ({rc})")
+    return phi
+
+
+def solve_direct(rho, h):
+    nz, ny, nx = rho.shape
+    phi = np.zeros_like(rho)
+    lib().or_solve_direct(_p(np.ascontiguousarray(rho)), nx, ny, nz,
+                          h[0], h[1], h[2], _p(phi))
+    return phi
+
+
+def cvbkforth1st(c, pts, refpt):
+    lib().or_cvbkforth1st(C.byref(c), _p(pts), pts.shape[0], _p(refpt))
+
+
+def efield(phi, ms):
+    nz, ny, nx = phi.shape
+    e = np.zeros((3,) + phi.shape)
+    lib().or_efield(_p(np.ascontiguousarray(phi)), nx, ny, nz, _p(ms),
+                    _p(e[0]), _p(e[1]), _p(e[2]))
+    return e
+
+
+def kick1(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg):
+    nz, ny, nx = phi.shape
+    lib().or_kick1(C.byref(c), _p(pts), pts.shape[0], nx, ny, nz,
+                   _p(np.ascontiguousarray(phi)), _p(rng), _p(ms), _p(refpt),
+                   curr, tau, mass, chrg)
+
+
+def diag_sums(pts):
+    s, m, r2 = np.zeros(37), np.zeros(6), C.c_double(0)
+    lib().or_diag_sums(_p(pts), pts.shape[0], _p(s), _p(m), C.byref(r2))
+    return s, m, r2.value
+
+
+def diag_row(c, sums, amax, r2max, npt, refpt, mass):
+    out = np.zeros(41)
+    lib().or_diag_row(C.byref(c), _p(np.ascontiguousarray(sums)),
+                      _p(np.ascontiguousarray(amax)), r2max, npt, _p(refpt),
+                      mass, _p(out))
+    return out
+
+
+class Run:
+    """The reference segment loop over a lattice, on the C oracle."""
+
+    def __init__(self, pts, beam, lattice, grid):
+        self.c = physconst(beam.freq)
+        self.pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
+        self.n = self.pts.shape[0]
+        self.grid = tuple(grid)
+        self.lattice = lattice
+        st = OrState()
+        st.bcurr = beam.current
+        st.current = beam.current
+        st.mass = beam.mass
+        st.charge = beam.charge
+        st.npt = self.n
+        for k in range(6):
+            st.refpt[k] = 0.0
+        st.refpt[4] = beam.phase0
+        st.refpt[5] = -(beam.kinetic / beam.mass + 1.0)
+        st.z = 0.0
+        self.st = st
+
+    def segments(self):
+        for e in self.lattice:
+            for _ in range(e.nseg):
+                yield e
+
+    def step(self, e, want_fields=False):
+        nx, ny, nz = self.grid
+        rho = np.zeros((nz, ny, nx)) if want_fields else None
+        phi = np.zeros((nz, ny, nx)) if want_fields else None
+        tau1 = 0.5 * e.length / e.nseg
+        n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
+                          nx, ny, nz, tau1, e.type, e.v1 if e.type == 1 else 0.0,
+                          e.mapstp, e.radius,
+                          _p(rho) if want_fields else None,
+                          _p(phi) if want_fields else None)
+        if n < 0:
+            raise RuntimeError("or_step failed")
+        self.n = n
+        return rho, phi
+
+    def particles(self):
+        return self.pts[: self.n]
+
+    def diag(self):
+        s, m, r2 = diag_sums(self.particles())
+        refpt = np.array(self.st.refpt[:])
+        return diag_row(self.c, s, m, r2, self.st.npt, refpt, self.st.mass)

@@ -1,0 +1,238 @@
+!----------------------------------------------------------------------
+! Dump harness for the IMPACT-Z reference (test infrastructure only).
+!
+! Links against the reference's own compiled modules and drives ONE
+! space-charge step sequence exactly in the order of the reference's
+! segment loop (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1,
+! Flagbc=1, Flagsubstep=0, Flagerr=0, drift and quadrupole elements only),
+! writing the intermediate quantities the reference never writes itself:
+! the particle array after every stage, the conv1st range, the updated
+! domain, the charge density and the potential.
+!
+! Input (list-directed, file named on the command line, default
+! harness.in):
+!   N Nx Ny Nz
+!   curr kinetic_eV mass_eV charge freq phase0
+!   nelem
+!   nelem lines:  length nseg mapstp type v1 v2 v3
+!                 (type 0: v1 = pipe radius; type 1: v1 = gradient T/m,
+!                  v2 = file ID, v3 = pipe radius)
+!   nsteps ndump
+!   ndump step indices (1-based) to dump
+!   particle file name (stream: int32 N, then Pts1(9,N) doubles)
+! Output: dump_<step>.bin (stream) per dumped step, fort.18/24-30 from
+! the reference's diagnostic1_Output every step.
+!----------------------------------------------------------------------
+      program ref_harness
+        use mpistub
+        use Pgrid2dclass
+        use CompDomclass
+        use FieldQuantclass
+        use BeamLineElemclass
+        use BeamBunchclass
+        use Timerclass
+        use Outputclass
+        use PhysConstclass
+        use NumConstclass
+        implicit none
+        include 'mpif.h'
+        type (Pgrid2d) :: grid2d
+        type (CompDom) :: Ageom
+        type (BeamBunch) :: Bpts
+        type (FieldQuant) :: Potential
+        type (DriftTube), target, dimension(:), allocatable :: drifts
+        type (Quadrupole), target, dimension(:), allocatable :: quads
+        type (BeamLineElem), dimension(:), allocatable :: Blnelem
+        integer :: N, Nx, Ny, Nz, nelem, nsteps, ndump, ierr, i, j, k
+        integer :: bnseg, bmpstp, bitype, ihlf, nstep, idr, iqr
+        integer :: Nplocal, Np, nchrg, u
+        integer, dimension(:), allocatable :: etype, enseg, empstp, dumpat
+        integer, dimension(1) :: nptlist0, idchg
+        double precision, dimension(1) :: qmcclist0
+        double precision, dimension(:), allocatable :: elen, ev1, ev2, ev3
+        double precision :: curr, kin, mass, chg, freq, ph0, gambet
+        double precision :: z, tau1, tau2, blength, piperad, piperad2
+        double precision, dimension(21) :: distparam
+        double precision, dimension(9) :: tmpdr
+        double precision, dimension(9) :: tmpquad
+        double precision, dimension(6) :: ptrange, range
+        double precision, dimension(3) :: msize
+        integer, dimension(3) :: lcgrid
+        double precision, allocatable, dimension(:,:,:) :: chgdens
+        character(len=256) :: ctl, pfile, dname
+        logical :: dodump
+        integer :: n32
+
+        ctl = 'harness.in'
+        if (command_argument_count() >= 1) call get_command_argument(1, ctl)
+        open(unit=11, file=trim(ctl), status='old')
+        read(11,*) N, Nx, Ny, Nz
+        read(11,*) curr, kin, mass, chg, freq, ph0
+        read(11,*) nelem
+        allocate(elen(nelem), enseg(nelem), empstp(nelem), etype(nelem))
+        allocate(ev1(nelem), ev2(nelem), ev3(nelem))
+        do i = 1, nelem
+          read(11,*) elen(i), enseg(i), empstp(i), etype(i), ev1(i), &
+                     ev2(i), ev3(i)
+        enddo
+        read(11,*) nsteps, ndump
+        allocate(dumpat(max(ndump,1)))
+        dumpat = -1
+        if (ndump > 0) read(11,*) dumpat(1:ndump)
+        read(11,'(a)') pfile
+        close(11)
+
+        call MPI_INIT(ierr)
+        call construct_Timer(0.0d0)
+        call construct_Pgrid2d(grid2d, MPI_COMM_WORLD, 1, 1)
+        call construct_PhysConst(freq)
+        distparam = 0.0d0
+        distparam(1) = 1.0d-3
+        distparam(2) = 1.0d-3
+        distparam(8) = 1.0d-3
+        distparam(9) = 1.0d-3
+        distparam(15) = 1.0d-1
+        distparam(16) = 1.0d-3
+        gambet = sqrt((kin/mass+1.0d0)**2-1.0d0)
+        call construct_CompDom(Ageom, distparam, 21, 19, Nx, Ny, Nz, &
+             grid2d, 1, 1, 1, 1.0d0, 1.0d0, 1.0d0, gambet)
+        call construct_BeamBunch(Bpts, curr, kin, mass, chg, N, ph0)
+        call construct_FieldQuant(Potential, Nx, Ny, Nz, Ageom, grid2d)
+
+        ! particles, exactly as readin_Dist leaves them (9 x N, in place)
+        open(unit=12, file=trim(pfile), access='stream', &
+             form='unformatted', status='old')
+        read(12) n32
+        if (n32 /= N) then
+          print*, "particle count mismatch", n32, N
+          stop
+        endif
+        allocate(Bpts%Pts1(9, N))
+        read(12) Bpts%Pts1
+        close(12)
+        Bpts%Nptlocal = N
+        Nplocal = N
+        Np = N
+
+        ! lattice: one target object per element, as init_AccSimulator does
+        allocate(drifts(nelem), quads(nelem), Blnelem(nelem))
+        idr = 0
+        iqr = 0
+        do i = 1, nelem
+          if (etype(i) == 0) then
+            idr = idr + 1
+            call construct_DriftTube(drifts(idr), enseg(i), empstp(i), &
+                 etype(i), elen(i))
+            tmpdr = 0.0d0
+            tmpdr(2) = ev1(i)
+            call setparam_DriftTube(drifts(idr), tmpdr(1:2))
+            Blnelem(i) = assign_BeamLineElem(drifts(idr))
+          else if (etype(i) == 1) then
+            iqr = iqr + 1
+            call construct_Quadrupole(quads(iqr), enseg(i), empstp(i), &
+                 etype(i), elen(i))
+            tmpquad = 0.0d0
+            tmpquad(2) = ev1(i)
+            tmpquad(3) = ev2(i)
+            tmpquad(4) = ev3(i)
+            call setparam_Quadrupole(quads(iqr), tmpquad)
+            Blnelem(i) = assign_BeamLineElem(quads(iqr))
+          else
+            print*, "harness supports drift (0) and quadrupole (1) only"
+            stop
+          endif
+        enddo
+
+        nchrg = 1
+        nptlist0(1) = N
+        qmcclist0(1) = Bpts%Pts1(7,1)
+        allocate(chgdens(1,1,1))
+        z = 0.0d0
+        nstep = 0
+        call diagnostic1_Output(z, Bpts, nchrg, nptlist0)
+
+        do i = 1, nelem
+          call getparam_BeamLineElem(Blnelem(i), blength, bnseg, bmpstp, &
+                                     bitype)
+          call getradius_BeamLineElem(Blnelem(i), piperad, piperad2)
+          tau1 = 0.0d0
+          if (bitype >= 0) tau1 = 0.5d0*blength/bnseg
+          tau2 = 2.0d0*tau1
+          ihlf = 0
+          do j = 1, bnseg
+            if (nstep >= nsteps) goto 900
+            dodump = any(dumpat(1:max(ndump,1)) == nstep+1)
+            if (dodump) then
+              write(dname, '(a,i0,a)') 'dump_', nstep+1, '.bin'
+              u = 20
+              open(unit=u, file=trim(dname), access='stream', &
+                   form='unformatted', status='replace')
+              call wpts(u)
+            endif
+            call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
+                                bnseg, j, ihlf)
+            if (dodump) call wpts(u)
+            call conv1st_BeamBunch(Bpts, tau2, Nplocal, Np, ptrange, &
+                                   1, 1.0d0, piperad, piperad2)
+            call chgupdate_BeamBunch(Bpts, nchrg, nptlist0, qmcclist0)
+            if (dodump) then
+              write(u) ptrange
+              write(u) Bpts%Current
+              call wpts(u)
+            endif
+            call update_CompDom(Ageom, ptrange, grid2d, 1)
+            call getlcmnum_CompDom(Ageom, lcgrid)
+            call setnpt_BeamBunch(Bpts, Nplocal)
+            call set_FieldQuant(Potential, Nx, Ny, Nz, Ageom, grid2d, 1, 1)
+            deallocate(chgdens)
+            allocate(chgdens(lcgrid(1), lcgrid(2), lcgrid(3)))
+            call charge_BeamBunch(Bpts, Nplocal, lcgrid(1), lcgrid(2), &
+                 lcgrid(3), Ageom, grid2d, chgdens, 1, 1.0d0)
+            if (dodump) then
+              call getrange_CompDom(Ageom, range)
+              call getmsize_CompDom(Ageom, msize)
+              write(u) range
+              write(u) msize
+              write(u) chgdens
+            endif
+            call update3O_FieldQuant(Potential, chgdens, Ageom, grid2d, &
+                 lcgrid(1), lcgrid(2), lcgrid(3), 1, 1, lcgrid(2), lcgrid(3))
+            if (dodump) write(u) Potential%FieldQ
+            call cvbkforth1st_BeamBunch(Bpts)
+            call setnpt_BeamBunch(Bpts, Nplocal)
+            if (dodump) call wpts(u)
+            call map2_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
+                 lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0)
+            if (dodump) call wpts(u)
+            call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
+                                bnseg, j, ihlf)
+            if (dodump) then
+              call wpts(u)
+              write(u) z
+              write(u) Bpts%refptcl
+              close(u)
+            endif
+            call diagnostic1_Output(z, Bpts, nchrg, nptlist0)
+            nstep = nstep + 1
+          enddo
+        enddo
+900     continue
+
+        ! final particle state of the run
+        open(unit=21, file='final.bin', access='stream', &
+             form='unformatted', status='replace')
+        call wpts(21)
+        write(21) z
+        write(21) Bpts%refptcl
+        write(21) Bpts%Current
+        close(21)
+
+      contains
+        subroutine wpts(uu)
+          integer, intent(in) :: uu
+          integer :: nn
+          nn = Bpts%Nptlocal
+          write(uu) nn
+          write(uu) Bpts%Pts1(1:9, 1:nn)
+        end subroutine wpts
+      end program ref_harness

@@ -1,0 +1,87 @@
+"""Write the golden fixtures under tests/golden/ from the compiled reference.
+
+Run from the repository root in a container that holds the reference build
+(``make -f oracle/Makefile.ref``):
+
+    python tests/golden/make_fixtures.py
+
+Every fixture is data only: the seeded input particles, the lattice and the
+quantities the reference harness (oracle/ref_harness.f90) dumped.  The
+reference's own tests hold no golden vectors, so these are the vectors.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "impact-z_amd"))
+sys.path.insert(0, ROOT)
+
+from impactz import beam as B  # noqa: E402
+from tests.golden import refdump as R  # noqa: E402
+
+STAGES = ("p0", "p_map1a", "p_conv1st", "p_cvbk", "p_kick", "p_map1b")
+
+
+def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
+            beam=B.Beam(), stages=STAGES):
+    pts = B.gaussian_bunch(n, seed=seed, beam=beam)
+    ref = R.run_harness(pts, beam, lattice, grid, nsteps, dump_steps)
+    out = {
+        "pts_in": pts,
+        "grid": np.array(grid),
+        "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                          beam.freq, beam.phase0]),
+        "lattice": np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2,
+                              e.v3] for e in lattice]),
+        "nsteps": np.array(nsteps),
+        "dump_steps": np.array(dump_steps),
+        "final": ref["final"],
+        "final_refpt": ref["final_refpt"],
+        "final_current": np.array(ref["final_current"]),
+        "final_z": np.array(ref["final_z"]),
+    }
+    for k in (18, 24, 25, 26, 27, 29, 30):
+        out[f"fort{k}"] = ref[f"fort{k}"]
+    for s in dump_steps:
+        d = ref["dumps"][s]
+        for key in stages:
+            out[f"s{s}_{key}"] = d[key]
+        for key in ("ptrange", "range", "msize", "rho", "phi", "refpt"):
+            out[f"s{s}_{key}"] = d[key]
+        out[f"s{s}_current"] = np.array(d["current"])
+        out[f"s{s}_z"] = np.array(d["z"])
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
+def main():
+    if not R.harness_available():
+        raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
+    meta = {}
+    # one FODO period on a cubic grid; dumps in QF (step 1) and the drift (step 5)
+    fixture("fodo_16cube", 1000, (16, 16, 16), B.fodo(1), 16, (1, 5))
+    meta["fodo_16cube"] = "1000 particles seed 1, 16^3, 1 FODO period, dumps at steps 1 (QF) and 5 (drift)"
+    # anisotropic grid: catches swapped axes in deposit, FFT and gather
+    fixture("fodo_aniso", 1000, (8, 16, 32), B.fodo(1), 6, (2,))
+    meta["fodo_aniso"] = "1000 particles seed 1, grid 8x16x32, dump at step 2"
+    # lossy pipe: conv1st drops particles and rescales the current
+    lossy = B.fodo(1, radius=0.0022)
+    fixture("fodo_lossy", 1000, (16, 16, 16), lossy, 4, (1, 2))
+    meta["fodo_lossy"] = "pipe radius 2.2 mm: particles lost in conv1st"
+    # moment trajectory over two periods (fort.24-30 every step), no dumps
+    fixture("fodo_traj", 2000, (16, 16, 32), B.fodo(2), 32, (), stages=())
+    meta["fodo_traj"] = "2000 particles, 16x16x32, 32 steps, diagnostics only"
+    with open(os.path.join(HERE, "fixtures.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
