@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Throughput of the IMPACT-Z space-charge step on one MI355X.
+
+Workload (the large single-bunch configuration): 50M macro-particles, an 80%
+6D Gaussian core + 20% 6D waterbag halo (2x core rms), on a 128x128x256 grid
+(256x256x512 doubled FFT), marched through the FODO lattice (QF/QD 0.1 m at
++-10 T/m, 0.4 m drifts, 4 space-charge steps per element).  A timed step is
+one full segment of the reference loop: half push, z->t + range, domain
+update, deposit, open-BC solve (Green function rebuilt), t->z, E field,
+gather + kick, half push and the 37 diagnostic moment sums.
+
+Prints ONE JSON line.  ``value`` is the FP64 form's particle-steps/s; the
+mixed form (FP32 deposit/FFT/gather, FP64 state/push/moments) is reported
+beside it under ``mixed``.  Inputs are resident in HBM before timing starts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "impact-z_amd"))
+
+from impactz import beam as B  # noqa: E402
+from impactz import hip  # noqa: E402
+from impactz.sim import Simulation  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kernel: str, n: int, g: int, form: str) -> float:
+    """Bytes a launch must move at minimum (particles SoA, grid once).
+
+    deposit : read t-frame x, y, z and the charge weight, write rho once
+    kick    : read and write the 6 phase-space rows, read Ex, Ey, Ez once
+    """
+    pe = 8 if form == "fp64" else 4
+    if kernel == "deposit":
+        return n * (3 * 8 + 8) + g * pe
+    if kernel == "kick":
+        return n * (6 * 8 * 2) + 3 * g * pe
+    raise ValueError(kernel)
+
+
+def run_form(pts, beam, lattice, grid, precision, steps, warmup):
+    form = "fp64" if precision == hip.FP64 else "mixed"
+    n = pts.shape[0]
+    sim = Simulation(pts, beam, lattice, grid, precision=precision)
+    segs = sim.segments()
+    log(f"{form}: warmup {warmup} steps")
+    for _ in range(warmup):
+        sim.step(next(segs), want_moments=True)
+    hip.timer_enable(True)
+    hip.timer_read(reset=True)
+    log(f"{form}: timing {steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(next(segs), want_moments=True)   # ends with a synchronising reduction
+    t1 = time.perf_counter()
+    timers = hip.timer_read(reset=True)
+    hip.timer_enable(False)
+    dt = t1 - t0
+    # second pass without kernel timers: the clock without event overhead
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step(next(segs), want_moments=True)
+    dt_clean = time.perf_counter() - t0
+    rows = sim.diag()
+    sim.close()
+    return {
+        "form": form,
+        "ms_per_step": 1e3 * dt_clean / steps,
+        "value": n * steps / dt_clean,
+        "ms_per_step_timed": 1e3 * dt / steps,
+        "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in timers.items() if v[1]},
+        "kernels_launches": {k: v[1] for k, v in timers.items() if v[1]},
+        "diag": rows,
+    }
+
+
+def roofline(res, n, g):
+    """Dominant particle kernel: algorithmic bytes / average HIP-event time."""
+    km = res["kernels_ms"]
+    kern = max(("deposit", "kick"), key=lambda k: km.get(k, 0.0))
+    t = km[kern] * 1e-3
+    byts = algorithmic_bytes(kern, n, g, res["form"])
+    ach = byts / t / 1e9
+    return {"kernel": kern, "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_launch": byts, "avg_ms": km[kern]}
+
+
+def cpu_baseline(grid, sample_n: int, steps=(1, 3)):
+    """The reference executable (built from source by oracle/Makefile.ref),
+    1 core, on a bounded sample of the same workload: ``sample_n`` particles
+    of the same beam on the same grid.  Per-step time is the difference of
+    two runs of steps[1] and steps[0] segments, so deck parsing and the
+    particle.in read cancel; the per-step fort.* writes are included."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ImpactZexe")
+    if not os.access(exe, os.X_OK):
+        return None
+    tmp = tempfile.mkdtemp(prefix="impz_cpu_")
+    try:
+        pts = B.core_halo_bunch(sample_n, seed=1)
+        B.write_particle_in(os.path.join(tmp, "particle.in"), pts)
+        times = []
+        for s in steps:
+            d = os.path.join(tmp, f"s{s}")
+            os.makedirs(d)
+            os.symlink(os.path.join(tmp, "particle.in"), os.path.join(d, "particle.in"))
+            B.write_deck(os.path.join(d, "ImpactZ.in"), sample_n, grid, B.Beam(),
+                         [B.Element(0.4, s, 1, 0, 1.0)])
+            t0 = time.perf_counter()
+            subprocess.run(f"ulimit -s unlimited; exec {exe}", shell=True, cwd=d,
+                           check=True, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL)
+            times.append(time.perf_counter() - t0)
+            log(f"cpu reference: {s} steps {times[-1]:.2f} s")
+        per_step = (times[1] - times[0]) / (steps[1] - steps[0])
+        return {"value": sample_n / per_step, "unit": "particle-steps/s", "cores": 1,
+                "kind": "reference",
+                "sample": (f"ImpactZexe (serial, amdflang -O3) on {sample_n} particles, "
+                           f"grid {grid[0]}x{grid[1]}x{grid[2]}; per-step time "
+                           f"{per_step:.3f} s = (t({steps[1]} steps) - t({steps[0]} step))"
+                           f"/{steps[1] - steps[0]}, fort.* writes included")}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=int, default=50_000_000)
+    ap.add_argument("--grid", default="128,128,256")
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--cpu-sample", type=int, default=200_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the multi-GPU path is a "
+              "design only (no --gpus above 1)", file=sys.stderr)
+        sys.exit(2)
+    grid = tuple(int(v) for v in args.grid.split(","))
+    n = args.particles
+    g = grid[0] * grid[1] * grid[2]
+    hip.init(0)
+    beam = B.Beam()
+    lattice = B.fodo(25)  # 25 periods = 400 steps
+    log(f"sampling {n} particles")
+    pts = B.core_halo_bunch(n, seed=1)
+    res = {}
+    for form in args.forms.split(","):
+        prec = hip.FP64 if form == "fp64" else hip.MIXED
+        res[form] = run_form(pts, beam, lattice, grid, prec, args.steps, args.warmup)
+        log(f"{form}: {res[form]['ms_per_step']:.3f} ms/step, "
+            f"{res[form]['value']:.3e} particle-steps/s, kernels {res[form]['kernels_ms']}")
+    del pts
+    main_form = "fp64" if "fp64" in res else next(iter(res))
+    r = res[main_form]
+    out = {
+        "metric": "particle-steps/s (50M, 128x128x256 grid) FP64 vs mixed; scan p50/p99 latency",
+        "value": r["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_form == "fp64" else "f32 grid / f64 state",
+        "data": "synthetic (seeded 6D Gaussian core + waterbag halo)",
+        "config": {"workload": "large single bunch: 50M particles, 128x128x256 grid, "
+                               "FODO lattice, one space-charge step per timed step",
+                   "particles": n, "grid": list(grid), "form": main_form},
+        "roofline": roofline(r, n, g),
+        "kernels_ms": r["kernels_ms"],
+    }
+    if "mixed" in res and main_form != "mixed":
+        m = res["mixed"]
+        rows64, rows32 = r["diag"], m["diag"]
+        out["mixed"] = {
+            "value": m["value"], "ms_per_step": m["ms_per_step"],
+            "speedup_vs_fp64": m["value"] / r["value"],
+            "roofline": roofline(m, n, g), "kernels_ms": m["kernels_ms"],
+        }
+    if not args.no_cpu:
+        cb = cpu_baseline(grid, args.cpu_sample)
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,710 @@
+// izh_api.hip -- the extern "C" entry points of libimpactz_hip.so.
+//
+// Host-side pieces of the step live here: the quadrupole map (RK6 on 18
+// variables, computed once per half step on the host as the reference does),
+// the domain update, the space-charge constants of scatter1 and the order of
+// calls of one segment.  Everything per particle or per grid cell runs in
+// the kernels of izh_particles.hip and izh_field.hip.
+#include <cmath>
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "izh_internal.hpp"
+
+namespace izh {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int field_alloc(izh_field *f);
+
+// ------------------------------------------------------------- timers
+namespace {
+struct PendingTimer {
+  int slot;
+  hipEvent_t a, b;
+};
+struct TimerState {
+  bool on = false;
+  std::vector<PendingTimer> pending;
+  PendingTimer open[IZH_T_COUNT];
+  bool is_open[IZH_T_COUNT] = {false};
+  double ms[IZH_T_COUNT] = {0};
+  int64_t n[IZH_T_COUNT] = {0};
+};
+TimerState &timers() {
+  static TimerState t;
+  return t;
+}
+}  // namespace
+
+bool timing_on() { return timers().on; }
+
+void timer_begin(int slot, hipStream_t s) {
+  TimerState &t = timers();
+  if (!t.on || slot < 0 || slot >= IZH_T_COUNT || t.is_open[slot]) return;
+  PendingTimer p{slot, nullptr, nullptr};
+  if (hipEventCreate(&p.a) != hipSuccess) return;
+  if (hipEventCreate(&p.b) != hipSuccess) {
+    (void)hipEventDestroy(p.a);
+    return;
+  }
+  (void)hipEventRecord(p.a, s);
+  t.open[slot] = p;
+  t.is_open[slot] = true;
+}
+
+void timer_end(int slot, hipStream_t s) {
+  TimerState &t = timers();
+  if (slot < 0 || slot >= IZH_T_COUNT || !t.is_open[slot]) return;
+  PendingTimer p = t.open[slot];
+  (void)hipEventRecord(p.b, s);
+  t.pending.push_back(p);
+  t.is_open[slot] = false;
+}
+
+static void timers_collect() {
+  TimerState &t = timers();
+  for (PendingTimer &p : t.pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(p.b) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      t.ms[p.slot] += ms;
+      t.n[p.slot] += 1;
+    }
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  t.pending.clear();
+}
+
+static void physconst(double freq, izh_const *c) {
+  // src/DataStruct/PhysConst.f90:28-33
+  c->clight = 299792458.0;
+  c->pi = 2.0 * std::asin(1.0);
+  c->scxl = c->clight / (2.0 * c->pi * freq);
+  c->scfreq = freq;
+}
+
+static int bunch_alloc_rows(izh_bunch *b, double **rows, int64_t cap) {
+  for (int r = 0; r < 9; r++) {
+    if (hipMalloc((void **)&rows[r], (size_t)(cap > 0 ? cap : 1) * sizeof(double)) !=
+        hipSuccess)
+      return fail(IZH_ERR_NOMEM, "device allocation of %lld particles failed",
+                  (long long)cap);
+  }
+  return IZH_OK;
+}
+
+static void free_rows(double **rows) {
+  for (int r = 0; r < 9; r++) {
+    if (rows[r]) (void)hipFree(rows[r]);
+    rows[r] = nullptr;
+  }
+}
+
+// Upload Pts1(9,n) (particle-major) into the SoA rows.
+static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
+  std::vector<double> row((size_t)(n > 0 ? n : 1));
+  b->wabs_sum = 0.0;
+  for (int r = 0; r < 9; r++) {
+    for (int64_t i = 0; i < n; i++) row[i] = pts9[9 * i + r];
+    if (r == 7)
+      for (int64_t i = 0; i < n; i++) b->wabs_sum += std::fabs(row[i]);
+    if (n > 0)
+      IZH_HIP(hipMemcpyAsync(b->s[r], row.data(), n * sizeof(double),
+                             hipMemcpyHostToDevice, b->stream));
+    IZH_HIP(hipStreamSynchronize(b->stream));
+  }
+  b->n = n;
+  return IZH_OK;
+}
+
+// REAL(4) padding of update_CompDom: eps = 1.0/(N-3.0) (src/Appl/CompDom.f90:547-549).
+static double eps_real4(int n) { return (double)(1.0f / ((float)n - 3.0f)); }
+
+// Space-charge constants of scatter1_BeamBunch (src/Appl/BeamBunch.f90:2065-2084),
+// with fpei = Clight*Clight*1.0e-7 using the REAL(4) literal 1.0e-7.
+static void sc_constants(const izh_bunch *b, double *xycon, double *tcon) {
+  const izh_const &c = b->c;
+  const double gam = -b->refpt[5];
+  const double curr = b->current, mass = b->mass, chrg = b->charge;
+  const double tmpscale = curr / c.scfreq;
+  const double twopi = 2.0 * c.pi;
+  const double fpei = c.clight * c.clight * (double)1.0e-7f;
+  const double xk = 1.0 / c.scxl;
+  const double bet = std::sqrt(gam * gam - 1.0) / gam;
+  const double bbyk = bet / xk;
+  const double gambet = gam * bet;
+  const double brho = gambet / (c.clight * chrg) * mass;
+  const double vz0 = bet * c.clight;
+  const double perv0 = 2.0 * curr * fpei / (brho * vz0 * vz0 * gam * gam);
+  *xycon = 0.5 * perv0 * gambet * bbyk * twopi / tmpscale;
+  *tcon = bet * (*xycon) * gam * gam;
+}
+
+}  // namespace izh
+
+using namespace izh;
+
+extern "C" {
+
+const char *izh_last_error(void) { return g_err.c_str(); }
+
+int izh_timer_enable(int on) {
+  timers_collect();
+  timers().on = on != 0;
+  return IZH_OK;
+}
+
+int izh_timer_read(double ms[IZH_T_COUNT], int64_t launches[IZH_T_COUNT],
+                   int reset) {
+  timers_collect();
+  TimerState &t = timers();
+  for (int k = 0; k < IZH_T_COUNT; k++) {
+    if (ms) ms[k] = t.ms[k];
+    if (launches) launches[k] = t.n[k];
+    if (reset) {
+      t.ms[k] = 0;
+      t.n[k] = 0;
+    }
+  }
+  return IZH_OK;
+}
+
+int izh_init(int device) {
+  int count = 0;
+  IZH_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(IZH_ERR_ARG, "device %d not present (%d devices)", device, count);
+  IZH_HIP(hipSetDevice(device));
+  IZH_HIP(hipFree(nullptr));
+  return IZH_OK;
+}
+
+int izh_physconst(double freq, izh_const *out) {
+  if (!out || !(freq > 0)) return fail(IZH_ERR_ARG, "izh_physconst: bad argument");
+  physconst(freq, out);
+  return IZH_OK;
+}
+
+int izh_bunch_create(const double *pts9, int64_t n, double current,
+                     double kinetic, double mass, double charge, double freq,
+                     double phase0, int precision, izh_bunch **out) {
+  if (!out || n < 0 || (n > 0 && !pts9) || !(mass > 0) || !(freq > 0) ||
+      (precision != IZH_FP64 && precision != IZH_MIXED))
+    return fail(IZH_ERR_ARG, "izh_bunch_create: bad argument");
+  *out = nullptr;
+  izh_bunch *b = new izh_bunch();
+  IZH_HIP(hipGetDevice(&b->device));
+  b->precision = precision;
+  b->current = current;
+  b->mass = mass;
+  b->charge = charge;
+  b->freq = freq;
+  b->npt = n;
+  // construct_BeamBunch: refptcl = phasini; refptcl(6) = -(W/m + 1)
+  for (int k = 0; k < 6; k++) b->refpt[k] = phase0;
+  b->refpt[5] = -(kinetic / mass + 1.0);
+  physconst(freq, &b->c);
+  b->cap = n;
+  int rc;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      (rc = bunch_alloc_rows(b, b->s, n)) != IZH_OK ||
+      hipMalloc((void **)&b->d_part, kMaxRedBlocks * kMoments * sizeof(double)) !=
+          hipSuccess ||
+      hipMalloc((void **)&b->d_out, (kMoments + 8) * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void **)&b->h_out, (kMoments + 8) * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess) {
+    izh_bunch_destroy(b);
+    return fail(IZH_ERR_NOMEM, "izh_bunch_create: allocation failed");
+  }
+  if ((rc = upload_pts(b, pts9, n)) != IZH_OK) {
+    izh_bunch_destroy(b);
+    return rc;
+  }
+  b->frame = kFrameZ;
+  *out = b;
+  return IZH_OK;
+}
+
+int izh_bunch_destroy(izh_bunch *b) {
+  if (!b) return IZH_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  free_rows(b->s);
+  free_rows(b->alt);
+  if (b->d_part) (void)hipFree(b->d_part);
+  if (b->d_out) (void)hipFree(b->d_out);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return IZH_OK;
+}
+
+int izh_bunch_download(izh_bunch *b, double *pts9, int64_t cap, int64_t *n) {
+  if (!b || !n || (b->n > 0 && !pts9))
+    return fail(IZH_ERR_ARG, "izh_bunch_download: bad argument");
+  if (cap < b->n)
+    return fail(IZH_ERR_ARG, "izh_bunch_download: %lld rows needed, %lld given",
+                (long long)b->n, (long long)cap);
+  std::vector<double> row((size_t)(b->n > 0 ? b->n : 1));
+  for (int r = 0; r < 9; r++) {
+    if (b->n > 0)
+      IZH_HIP(hipMemcpyAsync(row.data(), b->s[r], b->n * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
+    IZH_HIP(hipStreamSynchronize(b->stream));
+    for (int64_t i = 0; i < b->n; i++) pts9[9 * i + r] = row[i];
+  }
+  *n = b->n;
+  return IZH_OK;
+}
+
+int izh_bunch_upload(izh_bunch *b, const double *pts9, int64_t n) {
+  if (!b || n < 0 || (n > 0 && !pts9))
+    return fail(IZH_ERR_ARG, "izh_bunch_upload: bad argument");
+  if (n > b->cap) {
+    IZH_HIP(hipStreamSynchronize(b->stream));
+    free_rows(b->s);
+    free_rows(b->alt);
+    IZH_CHECK(bunch_alloc_rows(b, b->s, n));
+    b->cap = n;
+  }
+  IZH_CHECK(upload_pts(b, pts9, n));
+  b->frame = kFrameZ;
+  return IZH_OK;
+}
+
+int izh_bunch_get_info(const izh_bunch *b, izh_bunch_info *info) {
+  if (!b || !info) return fail(IZH_ERR_ARG, "izh_bunch_get_info: bad argument");
+  info->current = b->current;
+  info->mass = b->mass;
+  info->charge = b->charge;
+  info->npt = b->npt;
+  info->nptlocal = b->n;
+  for (int k = 0; k < 6; k++) info->refpt[k] = b->refpt[k];
+  info->freq = b->freq;
+  info->precision = b->precision;
+  return IZH_OK;
+}
+
+int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info) {
+  if (!b || !info) return fail(IZH_ERR_ARG, "izh_bunch_set_info: bad argument");
+  if (info->nptlocal != b->n)
+    return fail(IZH_ERR_ARG, "izh_bunch_set_info: nptlocal is set by upload");
+  b->current = info->current;
+  b->mass = info->mass;
+  b->charge = info->charge;
+  b->npt = info->npt;
+  for (int k = 0; k < 6; k++) b->refpt[k] = info->refpt[k];
+  return IZH_OK;
+}
+
+// ------------------------------------------------------- quadrupole map
+
+// intfunc1_Quadrupole, constant gradient (src/Appl/Quadrupole.f90:254-305):
+// reference particle phase rate and the three 2x2 linear ODE blocks.
+static void quad_rhs(const izh_const *c, double bgrad, double qmcc,
+                     const double y[18], double f[18]) {
+  const double gamma0 = -y[5];
+  const double beta0 = std::sqrt(gamma0 * gamma0 - 1.0) / gamma0;
+  const double gbet = std::sqrt((gamma0 - 1.0) * (gamma0 + 1.0));
+  const double brho = gbet / c->clight / qmcc;
+  const double s11 = bgrad / brho * c->scxl;
+  const double s33 = -bgrad / brho * c->scxl;
+  const double s55 = 0.0;
+  for (int k = 0; k < 4; k++) f[k] = 0.0;
+  f[4] = 1.0 / (beta0 * c->scxl);
+  f[5] = 0.0;
+  const double s[3] = {s11, s33, s55};
+  for (int blk = 0; blk < 3; blk++) {
+    // two columns (cosine- and sine-like) per plane: y' = p/Scxl, p' = -s y
+    for (int col = 0; col < 2; col++) {
+      const int o = 6 + 4 * blk + 2 * col;
+      f[o] = y[o + 1] / c->scxl;
+      f[o + 1] = -s[blk] * y[o];
+    }
+  }
+}
+
+int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
+                       int mapstp, double refpt[6], double charge, double mass,
+                       double xm[36]) {
+  if (!c || !refpt || !xm || mapstp < 1 || !(mass > 0))
+    return fail(IZH_ERR_ARG, "izh_quad_maplinear: bad argument");
+  // maplinear_Quadrupole + rk6i_Quadrupole (src/Appl/Quadrupole.f90:120-252):
+  // an 8-stage 6th-order Runge-Kutta over mapstp steps of the 18 variables.
+  static const double kA[7][7] = {
+      {1.0 / 9.0},
+      {1.0 / 24.0, 3.0 / 24.0},
+      {1.0 / 6.0, -3.0 / 6.0, 4.0 / 6.0},
+      {278.0 / 544.0, -945.0 / 544.0, 840.0 / 544.0, 99.0 / 544.0},
+      {-106.0 / 6.0, 273.0 / 6.0, -104.0 / 6.0, -107.0 / 6.0, 48.0 / 6.0},
+      {110974.0 / 45648.0, -236799.0 / 45648.0, 68376.0 / 45648.0,
+       103803.0 / 45648.0, -10240.0 / 45648.0, 1926.0 / 45648.0},
+      {-101195.0 / 25994.0, 222534.0 / 25994.0, -71988.0 / 25994.0,
+       -26109.0 / 25994.0, -20000.0 / 25994.0, -72.0 / 25994.0,
+       22824.0 / 25994.0}};
+  static const double kB[8] = {41.0 / 840.0, 0.0, 216.0 / 840.0, 27.0 / 840.0,
+                               272.0 / 840.0, 27.0 / 840.0, 216.0 / 840.0,
+                               41.0 / 840.0};
+  const double qmcc = charge / mass;
+  double y[18] = {0};
+  y[4] = refpt[4];
+  y[5] = refpt[5];
+  y[6] = y[9] = y[10] = y[13] = y[14] = y[17] = 1.0;
+  const double h = tau / mapstp;
+  const double gi = -refpt[5];
+  const double ui = gi * (std::sqrt(gi * gi - 1.0) / gi);
+  const double squi = std::sqrt(ui), sqi3 = squi * squi * squi;
+  double k[8][18], yt[18];
+  for (int step = 0; step < mapstp; step++) {
+    for (int st = 0; st < 8; st++) {
+      for (int v = 0; v < 18; v++) {
+        double acc = 0.0;
+        for (int q = 0; q < st; q++) acc += kA[st - 1][q] * k[q][v];
+        yt[v] = y[v] + acc;
+      }
+      quad_rhs(c, gradient, qmcc, st == 0 ? y : yt, k[st]);
+      for (int v = 0; v < 18; v++) k[st][v] *= h;
+    }
+    for (int v = 0; v < 18; v++) {
+      double acc = 0.0;
+      for (int st = 0; st < 8; st++) acc += kB[st] * k[st][v];
+      y[v] += acc;
+    }
+  }
+  refpt[4] = y[4];
+  refpt[5] = y[5];
+  const double gf = -refpt[5];
+  const double uf = gf * (std::sqrt(gf * gf - 1.0) / gf);
+  const double squf = std::sqrt(uf), sqf3 = squf * squf * squf;
+  for (int q = 0; q < 36; q++) xm[q] = 0.0;
+  // transverse blocks scale with sqrt(gamma beta), longitudinal with its cube
+  // (src/Appl/Quadrupole.f90:179-196, with dlt = thl = 0)
+  const double si[3] = {squi, squi, sqi3}, sf[3] = {squf, squf, sqf3};
+  for (int blk = 0; blk < 3; blk++) {
+    const int r = 2 * blk, o = 6 + 4 * blk;
+    xm[6 * r + r] = y[o] * si[blk] / sf[blk];
+    xm[6 * (r + 1) + r] = y[o + 1] * si[blk] * sf[blk];
+    xm[6 * r + r + 1] = y[o + 2] / (si[blk] * sf[blk]);
+    xm[6 * (r + 1) + r + 1] = y[o + 3] * sf[blk] / si[blk];
+  }
+  return IZH_OK;
+}
+
+// ------------------------------------------------------------- map push
+
+int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
+               double gradient, int mapstp) {
+  if (!b || !z) return fail(IZH_ERR_ARG, "izh_drift1: bad argument");
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_drift1: bunch is not in the z frame");
+  if (bitype == 0) {
+    // src/Appl/BeamBunch.f90:276-298
+    const double gam0 = -b->refpt[5];
+    const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
+    IZH_CHECK(launch_drift(b, tau, gam0, beta0));
+    b->refpt[4] = b->refpt[4] + tau / (b->c.scxl * beta0);
+  } else if (bitype == 1) {
+    double xm[36];
+    IZH_CHECK(izh_quad_maplinear(&b->c, tau, gradient, mapstp, b->refpt,
+                                 b->charge, b->mass, xm));
+    IZH_CHECK(launch_map_linear(b, xm));
+  } else {
+    return fail(IZH_ERR_ARG, "izh_drift1: bitype %d not supported (0 drift, 1 quad)",
+                bitype);
+  }
+  *z = *z + tau;
+  return IZH_OK;
+}
+
+int izh_map_linear(izh_bunch *b, const double xm[36]) {
+  if (!b || !xm) return fail(IZH_ERR_ARG, "izh_map_linear: bad argument");
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_map_linear: bunch is not in the z frame");
+  return launch_map_linear(b, xm);
+}
+
+// --------------------------------------------------------- conv1st
+
+int izh_conv1st(izh_bunch *b, double tau, double ptrange[6], int64_t *nplc,
+                int64_t *nptot, double xrad, double yrad, int flagbc) {
+  (void)tau;
+  if (!b) return fail(IZH_ERR_ARG, "izh_conv1st: bad argument");
+  if (flagbc != 1)
+    return fail(IZH_ERR_ARG, "izh_conv1st: only Flagbc=1 (3D open) is supported");
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_conv1st: bunch is not in the z frame");
+  double pr[6];
+  int64_t nlost = 0;
+  if (b->n > 0) {
+    IZH_CHECK(launch_conv1st(b, xrad, yrad, pr, &nlost));
+  } else {
+    // empty bunch: the reference's initial +-1e20 (REAL(4)) range survives
+    for (int k = 0; k < 3; k++) {
+      pr[2 * k] = (double)1.0e20f;
+      pr[2 * k + 1] = -(double)1.0e20f;
+    }
+  }
+  b->frame = kFrameT1;
+  if (nlost > 0) {
+    if (!b->alt[0]) IZH_CHECK(bunch_alloc_rows(b, b->alt, b->cap));
+    IZH_CHECK(launch_compact(b, xrad, yrad, b->n - nlost));
+  }
+  // one process: nptot = nplc; Current = Current*nptot/Npt (:1657-1663)
+  const int64_t nnew = b->n;
+  if (b->npt > 0) b->current = b->current * (double)nnew / (double)b->npt;
+  b->npt = nnew;
+  if (ptrange) std::memcpy(ptrange, pr, sizeof pr);
+  if (nplc) *nplc = nnew;
+  if (nptot) *nptot = nnew;
+  return IZH_OK;
+}
+
+int izh_update_compdom(izh_compdom *dom, const double pr[6], int flagbc) {
+  if (!dom || !pr) return fail(IZH_ERR_ARG, "izh_update_compdom: bad argument");
+  if (flagbc != 1)
+    return fail(IZH_ERR_ARG, "izh_update_compdom: only Flagbc=1 is supported");
+  if (dom->nx < 4 || dom->ny < 4 || dom->nz < 4)
+    return fail(IZH_ERR_ARG, "izh_update_compdom: mesh %dx%dx%d below 4",
+                dom->nx, dom->ny, dom->nz);
+  // src/Appl/CompDom.f90:547-557, 614-622
+  const int nn[3] = {dom->nx, dom->ny, dom->nz};
+  for (int a = 0; a < 3; a++) {
+    const double eps = eps_real4(nn[a]);
+    const double lo = pr[2 * a], hi = pr[2 * a + 1];
+    dom->range[2 * a] = lo - eps * (hi - lo);
+    dom->range[2 * a + 1] = hi + eps * (hi - lo);
+    dom->msize[a] = (dom->range[2 * a + 1] - dom->range[2 * a]) / (nn[a] - 1);
+  }
+  return IZH_OK;
+}
+
+// ------------------------------------------------------------ fields
+
+static bool pow2(int v) { return v >= 4 && (v & (v - 1)) == 0; }
+
+int izh_field_create(int nx, int ny, int nz, int precision, izh_field **out) {
+  if (!out) return fail(IZH_ERR_ARG, "izh_field_create: bad argument");
+  *out = nullptr;
+  if (!pow2(nx) || !pow2(ny) || !pow2(nz) || 2 * nx > 2048 || 2 * ny > 2048 ||
+      2 * nz > 2048)
+    return fail(IZH_ERR_ARG,
+                "izh_field_create: mesh %dx%dx%d must be powers of two in [4,1024]",
+                nx, ny, nz);
+  if (precision != IZH_FP64 && precision != IZH_MIXED)
+    return fail(IZH_ERR_ARG, "izh_field_create: bad precision %d", precision);
+  izh_field *f = new izh_field();
+  IZH_HIP(hipGetDevice(&f->device));
+  f->precision = precision;
+  f->nx = nx;
+  f->ny = ny;
+  f->nz = nz;
+  f->g = (int64_t)nx * ny * nz;
+  if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete f;
+    return fail(IZH_ERR_HIP, "izh_field_create: stream creation failed");
+  }
+  int rc = field_alloc(f);
+  if (rc != IZH_OK) {
+    izh_field_destroy(f);
+    return rc;
+  }
+  *out = f;
+  return IZH_OK;
+}
+
+int izh_field_destroy(izh_field *f) {
+  if (!f) return IZH_OK;
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  void *bufs[] = {f->rho_fx, f->rho, f->phi, f->e, f->spec_a, f->spec_b,
+                  f->green, f->green_t, f->corner, f->tw_x, f->tw_y, f->tw_z};
+  for (void *p : bufs)
+    if (p) (void)hipFree(p);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+  delete f;
+  return IZH_OK;
+}
+
+// Kernels of a bunch and a field run on the bunch's stream while they share
+// data: the field's stream waits for everything queued on the bunch's.
+static int join_streams(hipStream_t producer, hipStream_t consumer) {
+  if (producer == consumer) return IZH_OK;
+  hipEvent_t ev;
+  IZH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  IZH_HIP(hipEventRecord(ev, producer));
+  IZH_HIP(hipStreamWaitEvent(consumer, ev, 0));
+  IZH_HIP(hipEventDestroy(ev));
+  return IZH_OK;
+}
+
+static int check_dom(const izh_field *f, const izh_compdom *dom, const char *who) {
+  if (!dom) return fail(IZH_ERR_ARG, "%s: no domain", who);
+  if (dom->nx != f->nx || dom->ny != f->ny || dom->nz != f->nz)
+    return fail(IZH_ERR_ARG, "%s: domain mesh %dx%dx%d != field %dx%dx%d", who,
+                dom->nx, dom->ny, dom->nz, f->nx, f->ny, f->nz);
+  for (int a = 0; a < 3; a++)
+    if (!(dom->msize[a] > 0))
+      return fail(IZH_ERR_ARG, "%s: mesh size %g not positive", who, dom->msize[a]);
+  return IZH_OK;
+}
+
+int izh_charge(izh_bunch *b, const izh_compdom *dom, izh_field *f) {
+  if (!b || !f) return fail(IZH_ERR_ARG, "izh_charge: bad argument");
+  IZH_CHECK(check_dom(f, dom, "izh_charge"));
+  if (b->frame != kFrameT1)
+    return fail(IZH_ERR_STATE, "izh_charge: call izh_conv1st first");
+  if (b->precision != f->precision)
+    return fail(IZH_ERR_ARG, "izh_charge: bunch and field precision differ");
+  IZH_CHECK(join_streams(b->stream, f->stream));
+  IZH_CHECK(launch_deposit(b, dom, f));
+  IZH_CHECK(join_streams(f->stream, b->stream));
+  return IZH_OK;
+}
+
+int izh_update3O(izh_field *f, const izh_compdom *dom) {
+  if (!f) return fail(IZH_ERR_ARG, "izh_update3O: bad argument");
+  IZH_CHECK(check_dom(f, dom, "izh_update3O"));
+  if (!f->have_rho) return fail(IZH_ERR_STATE, "izh_update3O: no charge density");
+  IZH_CHECK(launch_solve(f, dom));
+  return IZH_OK;
+}
+
+int izh_cvbkforth1st(izh_bunch *b) {
+  if (!b) return fail(IZH_ERR_ARG, "izh_cvbkforth1st: bad argument");
+  if (b->frame != kFrameT1)
+    return fail(IZH_ERR_STATE, "izh_cvbkforth1st: bunch is not in the t frame");
+  IZH_CHECK(launch_cvbk(b));
+  b->frame = kFrameT0;
+  return IZH_OK;
+}
+
+int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
+              int flagbc) {
+  if (!b || !f) return fail(IZH_ERR_ARG, "izh_kick1: bad argument");
+  if (flagbc != 1) return fail(IZH_ERR_ARG, "izh_kick1: only Flagbc=1 is supported");
+  if (b->frame != kFrameT0)
+    return fail(IZH_ERR_STATE, "izh_kick1: call izh_cvbkforth1st first");
+  // src/Appl/BeamBunch.f90:1731: no kick below 1e-20 A; particles stay in
+  // the t frame exactly as the reference leaves them
+  if (b->current < (double)1.0e-20f) {
+    b->frame = kFrameZ;
+    return IZH_OK;
+  }
+  IZH_CHECK(check_dom(f, dom, "izh_kick1"));
+  if (!f->have_phi) return fail(IZH_ERR_STATE, "izh_kick1: no potential");
+  if (b->precision != f->precision)
+    return fail(IZH_ERR_ARG, "izh_kick1: bunch and field precision differ");
+  IZH_CHECK(launch_efield(f, dom));
+  IZH_CHECK(join_streams(f->stream, b->stream));
+  double xycon, tcon;
+  sc_constants(b, &xycon, &tcon);
+  IZH_CHECK(launch_gather_kick(b, f, dom, tau, xycon, tcon));
+  b->frame = kFrameZ;
+  return IZH_OK;
+}
+
+int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
+                    double *r2max) {
+  if (!b) return fail(IZH_ERR_ARG, "izh_diagnostic1: bad argument");
+  double m[kMoments];
+  if (b->n > 0) {
+    IZH_CHECK(launch_moments(b, m));
+  } else {
+    for (int k = 0; k < kMoments; k++) m[k] = 0.0;
+  }
+  if (sums) std::memcpy(sums, m, 37 * sizeof(double));
+  if (amax) std::memcpy(amax, m + 37, 6 * sizeof(double));
+  if (r2max) *r2max = m[43];
+  return IZH_OK;
+}
+
+int izh_field_download(izh_field *f, int which, double *host) {
+  if (!f || !host) return fail(IZH_ERR_ARG, "izh_field_download: bad argument");
+  const void *src;
+  switch (which) {
+    case IZH_RHO: src = f->rho; break;
+    case IZH_PHI: src = f->phi; break;
+    case IZH_EX: case IZH_EY: case IZH_EZ: {
+      size_t es = f->precision == IZH_FP64 ? 8 : 4;
+      src = (const char *)f->e + (size_t)(which - IZH_EX) * f->g * es;
+      break;
+    }
+    default: return fail(IZH_ERR_ARG, "izh_field_download: unknown buffer %d", which);
+  }
+  IZH_HIP(hipStreamSynchronize(f->stream));
+  // the grid holds physical values times rho_scale (1 in FP64); rho, phi and
+  // E are linear in the charge, so one factor restores physical units
+  const double un = 1.0 / f->rho_scale;
+  if (f->precision == IZH_FP64) {
+    IZH_HIP(hipMemcpy(host, src, f->g * sizeof(double), hipMemcpyDeviceToHost));
+    if (un != 1.0)
+      for (int64_t i = 0; i < f->g; i++) host[i] *= un;
+  } else {
+    std::vector<float> tmp((size_t)f->g);
+    IZH_HIP(hipMemcpy(tmp.data(), src, f->g * sizeof(float), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < f->g; i++) host[i] = (double)tmp[i] * un;
+  }
+  return IZH_OK;
+}
+
+int izh_field_upload_rho(izh_field *f, const double *rho) {
+  if (!f || !rho) return fail(IZH_ERR_ARG, "izh_field_upload_rho: bad argument");
+  IZH_HIP(hipStreamSynchronize(f->stream));
+  f->rho_scale = 1.0;
+  if (f->precision == IZH_FP64) {
+    IZH_HIP(hipMemcpy(f->rho, rho, f->g * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    double mx = 0.0;
+    for (int64_t i = 0; i < f->g; i++) mx = std::fmax(mx, std::fabs(rho[i]));
+    f->rho_scale = mx > 0 ? 1.0 / mx : 1.0;
+    std::vector<float> tmp((size_t)f->g);
+    for (int64_t i = 0; i < f->g; i++) tmp[i] = (float)(rho[i] * f->rho_scale);
+    IZH_HIP(hipMemcpy(f->rho, tmp.data(), f->g * sizeof(float), hipMemcpyHostToDevice));
+  }
+  f->have_rho = 1;
+  f->have_phi = 0;
+  return IZH_OK;
+}
+
+// ---------------------------------------------------------------- step
+
+int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+             double tau1, int bitype, double gradient, int mapstp, double xrad,
+             double yrad, double bcurr, double *moments) {
+  if (!b || !f || !dom || !z) return fail(IZH_ERR_ARG, "izh_step: bad argument");
+  const double tau2 = 2.0 * tau1;
+  // src/Contrl/AccSimulator.f90:1106-1110 first half push
+  IZH_CHECK(izh_drift1(b, z, tau1, bitype, gradient, mapstp));
+  // an empty bunch has nothing to deposit or kick (the reference would still
+  // build a domain from its +-1e20 initial range; nothing reads it)
+  if (bcurr >= (double)1.0e-10f && b->n > 0) {
+    double ptrange[6];
+    int64_t nplc, nptot;
+    IZH_CHECK(izh_conv1st(b, tau2, ptrange, &nplc, &nptot, xrad, yrad, 1));  // :1128
+    IZH_CHECK(izh_update_compdom(dom, ptrange, 1));                         // :1156
+    IZH_CHECK(izh_charge(b, dom, f));                                       // :1191
+    IZH_CHECK(izh_update3O(f, dom));                                        // :1263
+    IZH_CHECK(izh_cvbkforth1st(b));                                         // :1399
+    IZH_CHECK(izh_kick1(b, tau2, f, dom, 1));                               // :1425
+  }
+  IZH_CHECK(izh_drift1(b, z, tau1, bitype, gradient, mapstp));              // :1429
+  if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
+  return IZH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,313 @@
+"""ctypes binding of libimpactz_hip.so and a host-side mirror of the
+reference's BeamBunch / CompDom / FieldQuant call surface.
+
+Method names follow the reference procedures they replace
+(src/Appl/BeamBunch.f90, src/Appl/CompDom.f90, src/Appl/Field.f90,
+src/Contrl/Output.f90); errors raise ``IZHError`` where the reference would
+``print`` and ``stop``.  Nothing here computes physics: every call goes
+through the C ABI, and a missing library is an error, never a fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libimpactz_hip.so")
+
+FP64, MIXED = 0, 1
+RHO, PHI, EX, EY, EZ = range(5)
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+
+# every symbol of include/impactz_hip.h
+SYMBOLS = (
+    "izh_last_error", "izh_init", "izh_physconst", "izh_bunch_create",
+    "izh_bunch_destroy", "izh_bunch_download", "izh_bunch_upload",
+    "izh_bunch_get_info", "izh_bunch_set_info", "izh_quad_maplinear",
+    "izh_drift1", "izh_map_linear", "izh_conv1st", "izh_update_compdom",
+    "izh_field_create", "izh_field_destroy", "izh_charge", "izh_update3O",
+    "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
+    "izh_field_upload_rho", "izh_step", "izh_timer_enable", "izh_timer_read",
+)
+
+# timer slots (IZH_T_*), named after the reference's Timer accumulators
+TIMERS = ("push", "conv1st", "deposit", "green", "fft", "efield", "kick",
+          "cvbk", "diag")
+
+
+class IZHError(RuntimeError):
+    pass
+
+
+class Const(C.Structure):
+    _fields_ = [("clight", C.c_double), ("pi", C.c_double),
+                ("scxl", C.c_double), ("scfreq", C.c_double)]
+
+
+class BunchInfo(C.Structure):
+    _fields_ = [("current", C.c_double), ("mass", C.c_double),
+                ("charge", C.c_double), ("npt", C.c_int64),
+                ("nptlocal", C.c_int64), ("refpt", C.c_double * 6),
+                ("freq", C.c_double), ("precision", C.c_int)]
+
+
+class CompDomC(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("range", C.c_double * 6), ("msize", C.c_double * 3)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the library (raises if it was not built: no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IZHError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.izh_last_error.restype = C.c_char_p
+    L.izh_init.argtypes = [C.c_int]
+    L.izh_physconst.argtypes = [C.c_double, C.POINTER(Const)]
+    L.izh_bunch_create.argtypes = [_dp, C.c_int64, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.POINTER(vp)]
+    L.izh_bunch_destroy.argtypes = [vp]
+    L.izh_bunch_download.argtypes = [vp, _dp, C.c_int64, _i64p]
+    L.izh_bunch_upload.argtypes = [vp, _dp, C.c_int64]
+    L.izh_bunch_get_info.argtypes = [vp, C.POINTER(BunchInfo)]
+    L.izh_bunch_set_info.argtypes = [vp, C.POINTER(BunchInfo)]
+    L.izh_quad_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_double,
+                                     C.c_int, _dp, C.c_double, C.c_double, _dp]
+    L.izh_drift1.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_double, C.c_int]
+    L.izh_map_linear.argtypes = [vp, _dp]
+    L.izh_conv1st.argtypes = [vp, C.c_double, _dp, _i64p, _i64p, C.c_double,
+                              C.c_double, C.c_int]
+    L.izh_update_compdom.argtypes = [C.POINTER(CompDomC), _dp, C.c_int]
+    L.izh_field_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(vp)]
+    L.izh_field_destroy.argtypes = [vp]
+    L.izh_charge.argtypes = [vp, C.POINTER(CompDomC), vp]
+    L.izh_update3O.argtypes = [vp, C.POINTER(CompDomC)]
+    L.izh_cvbkforth1st.argtypes = [vp]
+    L.izh_kick1.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
+    L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
+    L.izh_field_download.argtypes = [vp, C.c_int, _dp]
+    L.izh_field_upload_rho.argtypes = [vp, _dp]
+    L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.c_double,
+                           C.c_int, C.c_double, C.c_int, C.c_double,
+                           C.c_double, C.c_double, _dp]
+    L.izh_timer_enable.argtypes = [C.c_int]
+    L.izh_timer_read.argtypes = [_dp, _i64p, C.c_int]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise IZHError(f"izh error {rc}: {lib().izh_last_error().decode()}")
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def init(device: int = 0) -> None:
+    check(lib().izh_init(device))
+
+
+def timer_enable(on: bool = True) -> None:
+    check(lib().izh_timer_enable(1 if on else 0))
+
+
+def timer_read(reset: bool = False) -> dict:
+    """{slot: (total ms, launches)} of the kernel timers (HIP events)."""
+    ms = np.zeros(len(TIMERS))
+    n = np.zeros(len(TIMERS), dtype=np.int64)
+    check(lib().izh_timer_read(_p(ms), n.ctypes.data_as(_i64p), 1 if reset else 0))
+    return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(TIMERS)}
+
+
+def physconst(freq: float) -> Const:
+    c = Const()
+    check(lib().izh_physconst(freq, C.byref(c)))
+    return c
+
+
+def quad_maplinear(c: Const, tau, gradient, mapstp, refpt, charge, mass):
+    """maplinear_Quadrupole (src/Appl/Quadrupole.f90:120-200); host only."""
+    r = np.ascontiguousarray(refpt, dtype=np.float64).copy()
+    xm = np.zeros(36)
+    check(lib().izh_quad_maplinear(C.byref(c), tau, gradient, mapstp, _p(r),
+                                   charge, mass, _p(xm)))
+    return xm.reshape(6, 6), r
+
+
+class CompDom:
+    """CompDom for one process (src/Appl/CompDom.f90:19-32)."""
+
+    def __init__(self, nx: int, ny: int, nz: int):
+        self.c = CompDomC()
+        self.c.nx, self.c.ny, self.c.nz = nx, ny, nz
+
+    def update(self, ptrange, flagbc: int = 1) -> None:
+        """update_CompDom (src/Appl/CompDom.f90:509-823)."""
+        pr = np.ascontiguousarray(ptrange, dtype=np.float64)
+        check(lib().izh_update_compdom(C.byref(self.c), _p(pr), flagbc))
+
+    @property
+    def range(self):
+        return np.array(self.c.range[:])
+
+    @property
+    def msize(self):
+        return np.array(self.c.msize[:])
+
+    def set(self, rng, msize) -> None:
+        for k in range(6):
+            self.c.range[k] = float(rng[k])
+        for k in range(3):
+            self.c.msize[k] = float(msize[k])
+
+
+class FieldQuant:
+    """Device-resident FieldQuant + solver workspace (src/Appl/Field.f90:19-100)."""
+
+    def __init__(self, nx: int, ny: int, nz: int, precision: int = FP64):
+        self.shape = (nz, ny, nx)
+        self.h = C.c_void_p()
+        check(lib().izh_field_create(nx, ny, nz, precision, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().izh_field_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update3O(self, dom: CompDom) -> None:
+        """update3O_FieldQuant (src/Appl/Field.f90:108-234)."""
+        check(lib().izh_update3O(self.h, C.byref(dom.c)))
+
+    def download(self, which: int) -> np.ndarray:
+        out = np.zeros(self.shape)
+        check(lib().izh_field_download(self.h, which, _p(out)))
+        return out
+
+    def upload_rho(self, rho: np.ndarray) -> None:
+        assert rho.shape == self.shape
+        check(lib().izh_field_upload_rho(self.h, _p(np.ascontiguousarray(rho, dtype=np.float64))))
+
+
+class BeamBunch:
+    """Device-resident BeamBunch (src/Appl/BeamBunch.f90:23-33)."""
+
+    def __init__(self, pts: np.ndarray, current, kinetic, mass, charge, freq,
+                 phase0=0.0, precision: int = FP64):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        assert pts.ndim == 2 and pts.shape[1] == 9
+        self.h = C.c_void_p()
+        check(lib().izh_bunch_create(_p(pts), pts.shape[0], current, kinetic,
+                                     mass, charge, freq, phase0, precision,
+                                     C.byref(self.h)))
+
+    @classmethod
+    def from_beam(cls, pts, beam, precision: int = FP64):
+        return cls(pts, beam.current, beam.kinetic, beam.mass, beam.charge,
+                   beam.freq, beam.phase0, precision)
+
+    def close(self):
+        if self.h:
+            lib().izh_bunch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> BunchInfo:
+        i = BunchInfo()
+        check(lib().izh_bunch_get_info(self.h, C.byref(i)))
+        return i
+
+    @property
+    def nptlocal(self) -> int:
+        return int(self.info().nptlocal)
+
+    @property
+    def refpt(self) -> np.ndarray:
+        return np.array(self.info().refpt[:])
+
+    @property
+    def current(self) -> float:
+        return float(self.info().current)
+
+    def getpts(self) -> np.ndarray:
+        """getpts_BeamBunch: the local particles as (N, 9) (Pts1 transposed view)."""
+        n = self.nptlocal
+        out = np.zeros((max(n, 1), 9))
+        nn = C.c_int64()
+        check(lib().izh_bunch_download(self.h, _p(out), out.shape[0], C.byref(nn)))
+        return out[: nn.value]
+
+    def setpts(self, pts: np.ndarray) -> None:
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        check(lib().izh_bunch_upload(self.h, _p(pts), pts.shape[0]))
+
+    # --- the reference's per-step procedures -------------------------------
+    def drift1(self, z: float, tau: float, bitype: int, gradient: float = 0.0,
+               mapstp: int = 1) -> float:
+        """map1_BeamBunch = drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465); returns z."""
+        zz = C.c_double(z)
+        check(lib().izh_drift1(self.h, C.byref(zz), tau, bitype, gradient, mapstp))
+        return zz.value
+
+    def conv1st(self, tau: float, xrad: float, yrad: float, flagbc: int = 1):
+        """conv1st_BeamBunch (src/Appl/BeamBunch.f90:1252-1665); returns (ptrange, nplc, nptot)."""
+        pr = np.zeros(6)
+        nplc, nptot = C.c_int64(), C.c_int64()
+        check(lib().izh_conv1st(self.h, tau, _p(pr), C.byref(nplc),
+                                C.byref(nptot), xrad, yrad, flagbc))
+        return pr, nplc.value, nptot.value
+
+    def charge(self, dom: CompDom, field: FieldQuant) -> None:
+        """charge_BeamBunch (src/Appl/BeamBunch.f90:3535-3630)."""
+        check(lib().izh_charge(self.h, C.byref(dom.c), field.h))
+
+    def cvbkforth1st(self) -> None:
+        """cvbkforth1st_BeamBunch (src/Appl/BeamBunch.f90:1667-1701)."""
+        check(lib().izh_cvbkforth1st(self.h))
+
+    def kick1(self, tau: float, field: FieldQuant, dom: CompDom, flagbc: int = 1) -> None:
+        """map2_BeamBunch = kick1_BeamBunch (src/Appl/BeamBunch.f90:1707-2038)."""
+        check(lib().izh_kick1(self.h, tau, field.h, C.byref(dom.c), flagbc))
+
+    def diagnostic1(self):
+        """diagnostic1_Output reductions (src/Contrl/Output.f90:119-245)."""
+        s, m, r2 = np.zeros(37), np.zeros(6), C.c_double()
+        check(lib().izh_diagnostic1(self.h, _p(s), _p(m), C.byref(r2)))
+        return s, m, r2.value
+
+    def step(self, field: FieldQuant, dom: CompDom, z: float, tau1: float,
+             bitype: int, gradient: float, mapstp: int, xrad: float,
+             yrad: float, bcurr: float, want_moments: bool = False):
+        """One segment of run_AccSimulator (src/Contrl/AccSimulator.f90:1099-1488)."""
+        zz = C.c_double(z)
+        mom = np.zeros(44) if want_moments else None
+        check(lib().izh_step(self.h, field.h, C.byref(dom.c), C.byref(zz), tau1,
+                             bitype, gradient, mapstp, xrad, yrad, bcurr,
+                             _p(mom) if want_moments else None))
+        return zz.value, mom

@@ -1,0 +1,128 @@
+"""Run a drift/quadrupole lattice on the GPU in the reference's segment order.
+
+This is the host loop of run_AccSimulator (src/Contrl/AccSimulator.f90:742-1504)
+restricted to Flagmap=1, Flagbc=1, Flagsubstep=0, Flagerr=0 and element types
+0 (drift) and 1 (constant-gradient quadrupole); the per-step procedures all
+run on the device through the C ABI.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import hip
+from .beam import Beam, Element
+
+CLIGHT = 299792458.0
+
+
+def diag_row(sums, amax, r2max, npt, refpt, mass, scxl, pi):
+    """Derived columns of diagnostic1_Output (src/Contrl/Output.f90:247-339).
+
+    Returns 41 values in the order written to fort.18 (5 after z), fort.24,
+    fort.25, fort.26, fort.27, fort.29 and fort.30 (6 each).
+    """
+    t = np.asarray(sums, dtype=np.float64)
+    qmc, xl, xt = mass / 1.0e6, scxl, 180.0 / pi
+    third = float(np.float32(1.0) / np.float32(3.0))  # REAL(4) exponent (:281)
+    den1 = 1.0 / float(npt)
+    gam = -refpt[5]
+    gambet = math.sqrt(gam * gam - 1.0)
+    bet = math.sqrt(gam * gam - 1.0) / gam
+    energy = (gam - 1.0) * qmc
+    m = t[:6] * den1                      # x0 px0 y0 py0 z0 pz0
+    sq = t[6:12] * den1                   # <x^2> ...
+    var = sq - m * m
+    cross = t[12:15] * den1 - m[0::2] * m[1::2]   # xpx, ypy, zpz
+    cub = t[[15, 17, 19, 21, 23, 25]] * den1
+    fth = t[[16, 18, 20, 22, 24, 26]] * den1
+    m3 = np.abs(cub - 3 * sq * m + 2 * m * m * m) ** third
+    m4 = np.sqrt(np.sqrt(np.abs(fth - 4 * cub * m + 6 * sq * m * m - 3 * m ** 4)))
+    rms = np.sqrt(var)
+    eps = np.sqrt(np.maximum(var[0::2] * var[1::2] - cross * cross, 0.0))
+    sc_pos = np.array([xl, xl, xt])       # x, y, z position scales
+    sc_mom = np.array([1 / gambet, 1 / gambet, qmc])
+    out = [refpt[4], gam, energy, bet, math.sqrt(r2max) * xl]
+    for a in range(3):
+        emit = eps[a] * (xl if a < 2 else qmc * xt)
+        out += [m[2 * a] * sc_pos[a], rms[2 * a] * sc_pos[a],
+                m[2 * a + 1] * sc_mom[a], rms[2 * a + 1] * sc_mom[a],
+                -cross[a] / eps[a] if eps[a] != 0 else np.nan, emit]
+    out += [amax[0] * xl, amax[1] / gambet, amax[2] * xl, amax[3] / gambet,
+            amax[4] * xt, amax[5] * qmc]
+    for mm in (m3, m4):
+        out += [mm[0] * xl, mm[1] / gambet, mm[2] * xl, mm[3] / gambet,
+                mm[4] * xt, mm[5] * qmc]
+    return np.array(out)
+
+
+class Simulation:
+    """A device-resident bunch + field marched through a lattice."""
+
+    def __init__(self, pts: np.ndarray, beam: Beam, lattice: list[Element],
+                 grid: tuple[int, int, int], precision: int = hip.FP64):
+        self.beam = beam
+        self.lattice = lattice
+        self.grid = tuple(grid)
+        self.const = hip.physconst(beam.freq)
+        self.bunch = hip.BeamBunch.from_beam(pts, beam, precision)
+        self.field = hip.FieldQuant(*self.grid, precision=precision)
+        self.dom = hip.CompDom(*self.grid)
+        self.z = 0.0
+        self.nstep = 0
+
+    def segments(self):
+        for e in self.lattice:
+            for _ in range(e.nseg):
+                yield e
+
+    def step(self, e: Element, want_moments: bool = True):
+        tau1 = 0.5 * e.length / e.nseg
+        grad = e.v1 if e.type == 1 else 0.0
+        self.z, mom = self.bunch.step(self.field, self.dom, self.z, tau1,
+                                      e.type, grad, e.mapstp, e.radius,
+                                      e.radius, self.beam.current, want_moments)
+        self.nstep += 1
+        return mom
+
+    def step_granular(self, e: Element):
+        """The same segment through the individual procedure entry points."""
+        tau1 = 0.5 * e.length / e.nseg
+        tau2 = 2.0 * tau1
+        grad = e.v1 if e.type == 1 else 0.0
+        b = self.bunch
+        self.z = b.drift1(self.z, tau1, e.type, grad, e.mapstp)
+        if self.beam.current >= float(np.float32(1.0e-10)):
+            ptrange, _, _ = b.conv1st(tau2, e.radius, e.radius)
+            self.dom.update(ptrange)
+            b.charge(self.dom, self.field)
+            self.field.update3O(self.dom)
+            b.cvbkforth1st()
+            b.kick1(tau2, self.field, self.dom)
+        self.z = b.drift1(self.z, tau1, e.type, grad, e.mapstp)
+        self.nstep += 1
+
+    def diag(self, mom=None):
+        if mom is None:
+            s, m, r2 = self.bunch.diagnostic1()
+        else:
+            s, m, r2 = mom[:37], mom[37:43], mom[43]
+        info = self.bunch.info()
+        return diag_row(s, m, r2, info.npt, np.array(info.refpt[:]),
+                        info.mass, self.const.scxl, self.const.pi)
+
+    def run(self, nsteps: int | None = None, every: int = 1):
+        """March through the lattice; returns the diagnostic rows (z first)."""
+        rows = [np.concatenate([[self.z], self.diag()])]
+        for k, e in enumerate(self.segments()):
+            if nsteps is not None and k >= nsteps:
+                break
+            mom = self.step(e, want_moments=(k + 1) % every == 0)
+            if mom is not None:
+                rows.append(np.concatenate([[self.z], self.diag(mom)]))
+        return np.array(rows)
+
+    def close(self):
+        self.bunch.close()
+        self.field.close()

@@ -1,0 +1,181 @@
+/*
+ * impactz_hip.h -- C ABI of the MI355X (gfx950) space-charge step of IMPACT-Z.
+ *
+ * One entry point per reference procedure on the step, so a Fortran host can
+ * bind each with iso_c_binding and keep the reference's call order
+ * (src/Contrl/AccSimulator.f90:1099-1488).  Conventions:
+ *   - every function returns int status: IZH_OK (0) or an IZH_ERR_* code;
+ *     izh_last_error() gives the message of the calling thread's last error;
+ *   - no globals carry physics: constants are passed or live in the handles;
+ *   - particle arrays crossing the ABI use the reference layout Pts1(9,N)
+ *     (9 contiguous doubles per particle: x/Scxl, gamma*beta_x, y/Scxl,
+ *     gamma*beta_y, phase, -delta gamma, q/mc^2, charge weight, id);
+ *   - grids crossing the ABI use the reference layout rho(Nx,Ny,Nz), x fastest;
+ *   - a bunch and its field stay resident on the GPU between calls; host
+ *     buffers are only read or written by the upload/download calls.
+ * Handles are not thread-safe; one host thread drives a handle at a time.
+ */
+#ifndef IMPACTZ_HIP_H
+#define IMPACTZ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IZH_OK 0
+#define IZH_ERR_ARG 1      /* bad argument or unsupported option */
+#define IZH_ERR_HIP 2      /* a HIP runtime call failed */
+#define IZH_ERR_NOMEM 3    /* device allocation failed */
+#define IZH_ERR_STATE 4    /* handle used in the wrong state */
+
+#define IZH_FP64 0         /* every stage in FP64 */
+#define IZH_MIXED 1        /* FP32 deposit/FFT/gather, FP64 push, state, moments */
+
+#define IZH_RHO 0          /* field buffers for izh_field_download */
+#define IZH_PHI 1
+#define IZH_EX 2
+#define IZH_EY 3
+#define IZH_EZ 4
+
+typedef struct izh_bunch izh_bunch;
+typedef struct izh_field izh_field;
+
+/* Constants the reference holds in PhysConst (src/DataStruct/PhysConst.f90:18-35). */
+typedef struct {
+  double clight, pi, scxl, scfreq;
+} izh_const;
+
+/* Host-visible bunch scalars: BeamBunch%Current, %Mass, %Charge, %Npt,
+ * %Nptlocal and %refptcl (src/Appl/BeamBunch.f90:23-33). */
+typedef struct {
+  double current, mass, charge;
+  int64_t npt, nptlocal;
+  double refpt[6];
+  double freq;
+  int precision;
+} izh_bunch_info;
+
+/* CompDom state for one process (src/Appl/CompDom.f90:19-32):
+ * Meshnum, SpatRange and Meshsize. */
+typedef struct {
+  int nx, ny, nz;
+  double range[6];
+  double msize[3];
+} izh_compdom;
+
+const char *izh_last_error(void);
+
+/* Select the device for the calling thread and warm up the runtime. */
+int izh_init(int device);
+
+/* construct_PhysConst (src/DataStruct/PhysConst.f90:25-35). */
+int izh_physconst(double freq, izh_const *out);
+
+/* construct_BeamBunch + readin_Dist (src/Appl/BeamBunch.f90:41-59,
+ * src/Appl/Distribution.f90:1666-1714): upload n particles (Pts1(9,n)) and
+ * keep them device resident in SoA form.  refptcl(1:5)=phase0,
+ * refptcl(6)=-(kinetic/mass+1).  precision: IZH_FP64 or IZH_MIXED. */
+int izh_bunch_create(const double *pts9, int64_t n, double current,
+                     double kinetic, double mass, double charge, double freq,
+                     double phase0, int precision, izh_bunch **out);
+int izh_bunch_destroy(izh_bunch *b);
+/* getpts_BeamBunch (src/Appl/BeamBunch.f90:3661-3668): copy the local
+ * particles back in the reference's order.  cap = rows of pts9. */
+int izh_bunch_download(izh_bunch *b, double *pts9, int64_t cap, int64_t *n);
+/* setpts_BeamBunch + setnpt_BeamBunch (src/Appl/BeamBunch.f90:3632-3656). */
+int izh_bunch_upload(izh_bunch *b, const double *pts9, int64_t n);
+int izh_bunch_get_info(const izh_bunch *b, izh_bunch_info *info);
+int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info);
+
+/* maplinear_Quadrupole for a constant gradient (file ID 0): the 6x6 linear
+ * map of a half step from RK6 integration (src/Appl/Quadrupole.f90:120-305).
+ * Host only.  xm is row-major, xm[6*(j-1)+(k-1)] = xm(j,k); refpt updated. */
+int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
+                       int mapstp, double refpt[6], double charge, double mass,
+                       double xm[36]);
+
+/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465) for bitype 0 (exact
+ * drift) and bitype 1 (constant-gradient quadrupole).  *z += tau. */
+int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
+               double gradient, int mapstp);
+/* The map-apply half of drift1_BeamBunch for any host-computed 6x6 map
+ * (src/Appl/BeamBunch.f90:308-320); refpt is the caller's to update. */
+int izh_map_linear(izh_bunch *b, const double xm[36]);
+
+/* conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1252-1665): z->t
+ * frame, phase wrap, range of 0th+1st order positions, loss test and
+ * order-preserving compaction, Current rescale.  ptrange may be NULL. */
+int izh_conv1st(izh_bunch *b, double tau, double ptrange[6], int64_t *nplc,
+                int64_t *nptot, double xrad, double yrad, int flagbc);
+
+/* update_CompDom, one process (src/Appl/CompDom.f90:509-823): new bounding
+ * box padded by REAL(4) 1/(N-3) of its extent, h = extent/(N-1). */
+int izh_update_compdom(izh_compdom *dom, const double ptrange[6], int flagbc);
+
+/* FieldQuant workspace for one mesh (src/Appl/Field.f90:19-100): charge
+ * density, potential, field and the doubled-grid FFT buffers. */
+int izh_field_create(int nx, int ny, int nz, int precision, izh_field **out);
+int izh_field_destroy(izh_field *f);
+
+/* charge_BeamBunch -> deposit_BeamBunch (src/Appl/BeamBunch.f90:3535-3768):
+ * deterministic CIC deposition of the t-frame particles into the field. */
+int izh_charge(izh_bunch *b, const izh_compdom *dom, izh_field *f);
+
+/* update3O_FieldQuant -> openBC3D (src/Appl/Field.f90:108-331): open-boundary
+ * Poisson solve with the integrated Green function on the doubled grid. */
+int izh_update3O(izh_field *f, const izh_compdom *dom);
+
+/* cvbkforth1st_BeamBunch (src/Appl/BeamBunch.f90:1667-1701). */
+int izh_cvbkforth1st(izh_bunch *b);
+
+/* kick1_BeamBunch (src/Appl/BeamBunch.f90:1707-2181), Flagbc=1: E from the
+ * potential, CIC gather, kick and 0th-order t->z back transform. */
+int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
+              int flagbc);
+
+/* diagnostic1_Output reductions (src/Contrl/Output.f90:119-245): the 37 sums
+ * in the reference's order, 6 abs-maxima, max of x^2+y^2. */
+int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
+                    double *r2max);
+
+/* Parity access to the field (host arrays of nx*ny*nz doubles). */
+int izh_field_download(izh_field *f, int which, double *host);
+int izh_field_upload_rho(izh_field *f, const double *rho);
+
+/* One segment of the reference loop for a drift (bitype 0) or quadrupole
+ * (bitype 1) element: drift1(tau1), conv1st, update_CompDom, charge,
+ * update3O, cvbkforth1st, kick1(2 tau1), drift1(tau1)
+ * (src/Contrl/AccSimulator.f90:1099-1488 with Flagmap=1, Flagbc=1,
+ * Flagsubstep=0).  moments (NULL or 44 doubles) receives the
+ * diagnostic1_Output sums of the state after the step: sums[37], amax[6],
+ * r2max.  bcurr is the deck current (no space charge below 1e-10 A). */
+int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+             double tau1, int bitype, double gradient, int mapstp,
+             double xrad, double yrad, double bcurr, double *moments);
+
+/* Per-phase kernel timers, the device counterpart of the reference's Timer
+ * accumulators (src/Func/Timer.f90:10-146; t_map1, t_rhofas, t_greenf,
+ * t_fft2dhpf, t_field, t_force/t_ntrslo, t_diag).  When enabled, each timed
+ * kernel launch is bracketed by HIP events on the stream it runs on.
+ * izh_timer_read returns, per slot, the summed milliseconds and the number
+ * of timed launches (both IZH_T_COUNT long) and optionally resets them. */
+#define IZH_T_PUSH 0       /* drift1 / map apply */
+#define IZH_T_CONV1ST 1    /* conv1st + range reduction */
+#define IZH_T_DEPOSIT 2    /* CIC deposition (charge_BeamBunch) */
+#define IZH_T_GREEN 3      /* integrated Green function + its transform */
+#define IZH_T_FFT 4        /* rho -> phi convolution passes */
+#define IZH_T_EFIELD 5     /* E = -grad phi */
+#define IZH_T_KICK 6       /* gather + kick + back transform (scatter1) */
+#define IZH_T_CVBK 7       /* cvbkforth1st */
+#define IZH_T_DIAG 8       /* diagnostic1 moment reductions */
+#define IZH_T_COUNT 9
+int izh_timer_enable(int on);
+int izh_timer_read(double ms[IZH_T_COUNT], int64_t launches[IZH_T_COUNT],
+                   int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMPACTZ_HIP_H */

@@ -167,6 +167,9 @@
               u = 20
               open(unit=u, file=trim(dname), access='stream', &
                    form='unformatted', status='replace')
+              write(u) z
+              write(u) Bpts%refptcl
+              write(u) Bpts%Current
               call wpts(u)
             endif
             call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &

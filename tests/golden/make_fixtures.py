@@ -52,9 +52,11 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
         d = ref["dumps"][s]
         for key in stages:
             out[f"s{s}_{key}"] = d[key]
-        for key in ("ptrange", "range", "msize", "rho", "phi", "refpt"):
+        for key in ("ptrange", "range", "msize", "rho", "phi", "refpt", "refpt0"):
             out[f"s{s}_{key}"] = d[key]
         out[f"s{s}_current"] = np.array(d["current"])
+        out[f"s{s}_current0"] = np.array(d["current0"])
+        out[f"s{s}_z0"] = np.array(d["z0"])
         out[f"s{s}_z"] = np.array(d["z"])
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **out)

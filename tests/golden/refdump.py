@@ -45,6 +45,9 @@ def read_dump(path: str, grid) -> dict:
     nx, ny, nz = grid
     r = _Reader(path)
     d = {}
+    d["z0"] = r.f64(1)[0]
+    d["refpt0"] = r.f64(6)
+    d["current0"] = r.f64(1)[0]
     d["p0"] = r.pts()
     d["p_map1a"] = r.pts()
     d["ptrange"] = r.f64(6)

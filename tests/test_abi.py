@@ -1,0 +1,79 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, and its host-only entry points (physical constants,
+quadrupole map, domain update, argument validation) match the reference."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, fixture_beam, load_golden, segment
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "impactz_hip.h")).read()
+    return sorted(set(re.findall(r"\b(izh_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from impactz import hip
+    lib = hip.lib()
+    declared = header_symbols()
+    assert len(declared) >= 20
+    for sym in declared:
+        assert hasattr(lib, sym), sym
+    assert sorted(hip.SYMBOLS) == declared
+
+
+def test_physconst_matches_oracle(oracle):
+    from impactz import hip
+    c = hip.physconst(324e6)
+    o = oracle.physconst(324e6)
+    assert (c.clight, c.pi, c.scxl, c.scfreq) == (o.clight, o.pi, o.scxl, o.scfreq)
+
+
+@pytest.mark.parametrize("name,step", [("fodo_16cube", 1), ("fodo_aniso", 2)])
+def test_quad_map_matches_reference(oracle, name, step):
+    """Host RK6 map of the library vs the oracle, and the reference's refpt."""
+    from impactz import hip
+    d = load_golden(name)
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    tau1 = 0.5 * e.length / e.nseg
+    c = hip.physconst(beam.freq)
+    refpt = d[f"s{step}_refpt0"]
+    xm, r1 = hip.quad_maplinear(c, tau1, e.v1, e.mapstp, refpt, beam.charge, beam.mass)
+    xo, r2 = oracle.quad_map(oracle.physconst(beam.freq), tau1, e.v1, e.mapstp,
+                             refpt, beam.charge, beam.mass)
+    assert np.allclose(xm, xo, rtol=1e-14, atol=1e-18)
+    assert r1[4] == pytest.approx(r2[4], rel=1e-15)
+    # two half steps of the quad move the reference phase as the reference did
+    _, r3 = hip.quad_maplinear(c, tau1, e.v1, e.mapstp, r1, beam.charge, beam.mass)
+    assert r3[4] == pytest.approx(float(d[f"s{step}_refpt"][4]), rel=1e-14)
+
+
+@pytest.mark.parametrize("name,step", [("fodo_16cube", 1), ("fodo_16cube", 5),
+                                       ("fodo_aniso", 2), ("fodo_lossy", 2)])
+def test_update_compdom_matches_reference(name, step):
+    from impactz import hip
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    dom = hip.CompDom(*grid)
+    dom.update(d[f"s{step}_ptrange"])
+    assert np.array_equal(dom.range, d[f"s{step}_range"])
+    assert np.array_equal(dom.msize, d[f"s{step}_msize"])
+
+
+def test_argument_errors_are_reported():
+    from impactz import hip
+    dom = hip.CompDom(3, 8, 8)
+    with pytest.raises(hip.IZHError, match="below 4"):
+        dom.update(np.array([-1, 1, -1, 1, -1, 1.0]))
+    dom = hip.CompDom(8, 8, 8)
+    with pytest.raises(hip.IZHError, match="Flagbc=1"):
+        dom.update(np.array([-1, 1, -1, 1, -1, 1.0]), flagbc=2)
+    with pytest.raises(hip.IZHError, match="powers of two"):
+        hip.FieldQuant(12, 8, 8)
+    c = hip.physconst(324e6)
+    with pytest.raises(hip.IZHError, match="bad argument"):
+        hip.quad_maplinear(c, 0.1, 1.0, 0, np.zeros(6), 1.0, 938e6)

@@ -1,0 +1,240 @@
+"""Pins the C oracle (the checker of the HIP path) before it is trusted.
+
+Two kinds of pins:
+  * golden vectors written by the compiled reference (tests/golden/*.npz via
+    oracle/ref_harness.f90): every stage of a step is fed the reference's own
+    input and must return the reference's output;
+  * mathematics that fixes a stage independently of the reference (direct
+    convolution, Gauss-Legendre cell averages of 1/r, closed-form quadrupole
+    and drift maps, CIC partition of unity, frame round trips).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import col_err, fixture_beam, fixture_lattice, load_golden, rel_err, segment
+
+CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
+         ("fodo_lossy", 1), ("fodo_lossy", 2)]
+
+
+@pytest.mark.parametrize("name,step", CASES)
+def test_oracle_stages_match_reference(oracle, name, step):
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    s = f"s{step}_"
+    c = oracle.physconst(beam.freq)
+    tau1 = 0.5 * e.length / e.nseg
+    grad = e.v1 if e.type == 1 else 0.0
+    # drift1 (first half)
+    pts = d[s + "p0"].copy()
+    refpt = d[s + "refpt0"].copy()
+    z = oracle.map1(c, pts, refpt, float(d[s + "z0"]), tau1, e.type, grad,
+                    e.mapstp, beam.charge, beam.mass)
+    assert col_err(pts, d[s + "p_map1a"]) < 1e-14
+    # conv1st
+    pts = d[s + "p_map1a"].copy()
+    npt0 = d[s + "p0"].shape[0]
+    n, pr, npt, cur = oracle.conv1st(c, pts, refpt, e.radius, e.radius, npt0,
+                                     float(d[s + "current0"]))
+    assert n == d[s + "p_conv1st"].shape[0] == npt
+    assert rel_err(pr, d[s + "ptrange"]) < 1e-15
+    assert cur == pytest.approx(float(d[s + "current"]), rel=1e-15)
+    assert col_err(pts[:n], d[s + "p_conv1st"]) < 1e-14
+    # update_CompDom
+    rng, ms = oracle.update_compdom(d[s + "ptrange"], grid)
+    assert np.array_equal(rng, d[s + "range"])
+    assert np.array_equal(ms, d[s + "msize"])
+    # deposit
+    rho = oracle.deposit(d[s + "p_conv1st"], grid, d[s + "range"], d[s + "msize"])
+    assert rel_err(rho, d[s + "rho"]) < 1e-14
+    # open-BC solve (FFT differs from the reference's FFTPACK by rounding only)
+    phi = oracle.solve_openbc(d[s + "rho"], d[s + "msize"])
+    assert rel_err(phi, d[s + "phi"]) < 1e-13
+    # cvbkforth1st
+    pts = d[s + "p_conv1st"].copy()
+    oracle.cvbkforth1st(c, pts, refpt)
+    assert col_err(pts, d[s + "p_cvbk"]) < 1e-14
+    # kick1
+    pts = d[s + "p_cvbk"].copy()
+    oracle.kick1(c, pts, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt,
+                 float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge)
+    assert col_err(pts, d[s + "p_kick"]) < 1e-13
+    # drift1 (second half)
+    pts = d[s + "p_kick"].copy()
+    z = oracle.map1(c, pts, refpt, z, tau1, e.type, grad, e.mapstp,
+                    beam.charge, beam.mass)
+    assert col_err(pts, d[s + "p_map1b"]) < 1e-14
+    assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
+    assert rel_err(refpt, d[s + "refpt"]) < 1e-15
+
+
+@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy"])
+def test_oracle_trajectory_matches_reference(oracle, name):
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
+    nsteps = int(d["nsteps"])
+    for k, e in enumerate(run.segments(), 1):
+        if k > nsteps:
+            break
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            # centroids are compared on the scale of the rms size
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+        assert row[0] == pytest.approx(d["fort18"][k, 1], rel=1e-14)
+    fin = d["final"]
+    assert run.n == fin.shape[0]
+    assert col_err(run.particles(), fin) < 1e-12
+    assert run.st.current == pytest.approx(float(d["final_current"]), rel=1e-14)
+
+
+# ------------------------------------------------------------------ maths
+
+def test_solve_fft_equals_direct_sum(oracle):
+    rng = np.random.default_rng(3)
+    for grid, h in (((4, 8, 8), (1.0, 0.7, 2.3)), ((8, 4, 16), (0.3, 0.5, 0.9))):
+        nx, ny, nz = grid
+        rho = rng.standard_normal((nz, ny, nx))
+        a = oracle.solve_openbc(rho, h)
+        b = oracle.solve_direct(rho, h)
+        assert rel_err(a, b) < 1e-13
+
+
+def _gl(n, a, b):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (b - a) * x + 0.5 * (b + a), 0.5 * (b - a) * w
+
+
+def test_igf_is_cell_average_of_inverse_r(oracle):
+    """g(i,j,k) = (1/V) * integral of 1/r over the cell centred at (ih, jh, kh)."""
+    h = (0.7, 1.1, 1.9)
+    g = oracle.green_igf((4, 4, 4), h)
+    for (i, j, k) in ((1, 0, 0), (3, 2, 1), (0, 4, 2), (2, 2, 2)):
+        xs, wx = _gl(24, (i - 0.5) * h[0], (i + 0.5) * h[0])
+        ys, wy = _gl(24, (j - 0.5) * h[1], (j + 0.5) * h[1])
+        zs, wz = _gl(24, (k - 0.5) * h[2], (k + 0.5) * h[2])
+        X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+        W = wx[:, None, None] * wy[None, :, None] * wz[None, None, :]
+        ref = np.sum(W / np.sqrt(X ** 2 + Y ** 2 + Z ** 2)) / (h[0] * h[1] * h[2])
+        assert g[k, j, i] == pytest.approx(ref, rel=1e-9)
+
+
+def test_igf_self_cell_unit_cube(oracle):
+    """Integral of 1/r over the unit cube about its centre (about 2.38008).
+
+    Written as 6 faces * (1/4) * integral over a face of 1/r (divergence of
+    r_hat/r over the cube), a smooth integrand for Gauss-Legendre.
+    """
+    ys, wy = _gl(64, -0.5, 0.5)
+    Y, Z = np.meshgrid(ys, ys, indexing="ij")
+    face = np.sum(wy[:, None] * wy[None, :] / np.sqrt(0.25 + Y ** 2 + Z ** 2))
+    exact = 6 * 0.25 * face
+    assert exact == pytest.approx(2.38008, abs=1e-5)
+    g = oracle.green_igf((4, 4, 4), (1.0, 1.0, 1.0))
+    assert g[0, 0, 0] == pytest.approx(exact, rel=1e-12)
+
+
+def test_quad_map_closed_form(oracle):
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    tau, grad = 0.05, 10.0
+    refpt = np.zeros(6)
+    refpt[5] = -b.gamma0
+    xm, r2 = oracle.quad_map(c, tau, grad, 10, refpt, b.charge, b.mass)
+    gb = b.gambet0
+    brho = gb * b.mass / (299792458.0 * b.charge)
+    k = math.sqrt(grad / brho)
+    L = tau
+    sx = c.scxl
+    # (x/Scxl, gamma beta x') transport of a focusing (x) and defocusing (y) plane
+    exp_x = np.array([[math.cos(k * L), math.sin(k * L) / (k * gb * sx)],
+                      [-k * gb * sx * math.sin(k * L), math.cos(k * L)]])
+    exp_y = np.array([[math.cosh(k * L), math.sinh(k * L) / (k * gb * sx)],
+                      [k * gb * sx * math.sinh(k * L), math.cosh(k * L)]])
+    assert np.allclose(xm[0:2, 0:2], exp_x, rtol=1e-11, atol=0)
+    assert np.allclose(xm[2:4, 2:4], exp_y, rtol=1e-11, atol=0)
+    assert xm[4, 5] == pytest.approx(L / (sx * gb ** 3), rel=1e-12)
+    assert xm[4, 4] == pytest.approx(1.0, abs=1e-15)
+    for blk in range(3):
+        m = xm[2 * blk:2 * blk + 2, 2 * blk:2 * blk + 2]
+        assert np.linalg.det(m) == pytest.approx(1.0, abs=1e-13)
+    assert r2[4] == pytest.approx(L / (sx * b.beta0), rel=1e-13)
+
+
+def test_drift_time_reversal(oracle):
+    from impactz.beam import Beam, gaussian_bunch
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    pts = gaussian_bunch(500, seed=5)
+    p = pts.copy()
+    refpt = np.zeros(6)
+    refpt[5] = -b.gamma0
+    z = oracle.map1(c, p, refpt, 0.0, 0.2, 0, 0.0, 1, b.charge, b.mass)
+    z = oracle.map1(c, p, refpt, z, -0.2, 0, 0.0, 1, b.charge, b.mass)
+    assert col_err(p, pts) < 1e-13
+    assert abs(z) < 1e-16 and abs(refpt[4]) < 1e-12
+
+
+def test_deposit_partition_of_unity(oracle):
+    rng = np.random.default_rng(1)
+    n = 2000
+    pts = np.zeros((n, 9))
+    pts[:, 0] = rng.uniform(-1, 1, n)
+    pts[:, 2] = rng.uniform(-2, 2, n)
+    pts[:, 4] = rng.uniform(-3, 3, n)
+    pts[:, 7] = rng.uniform(0.5, 1.5, n)
+    pr = np.array([-1, 1, -2, 2, -3, 3.0])
+    grid = (8, 16, 8)
+    rng_, ms = oracle.update_compdom(pr, grid)
+    rho = oracle.deposit(pts, grid, rng_, ms)
+    assert rho.sum() == pytest.approx(pts[:, 7].sum(), rel=1e-13)
+    # a single particle spreads exactly its trilinear weights
+    one = np.zeros((1, 9))
+    one[0, [0, 2, 4, 7]] = [0.1, -0.3, 0.7, 2.0]
+    rho1 = oracle.deposit(one, grid, rng_, ms)
+    fx = (0.1 - rng_[0]) / ms[0]
+    fy = (-0.3 - rng_[2]) / ms[1]
+    fz = (0.7 - rng_[4]) / ms[2]
+    i, j, k = int(fx), int(fy), int(fz)
+    ax, ay, az = fx - i, fy - j, fz - k
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                w = ((ax if di else 1 - ax) * (ay if dj else 1 - ay) *
+                     (az if dk else 1 - az) * 2.0)
+                assert rho1[k + dk, j + dj, i + di] == pytest.approx(w, rel=1e-12, abs=1e-15)
+    assert np.count_nonzero(rho1) == 8
+
+
+def test_frame_round_trip_with_zero_field(oracle):
+    """conv1st -> cvbkforth1st -> kick1 with phi = 0 restores the z-frame
+    particle (phase wrapped), the identity the reference's frame changes obey."""
+    from impactz.beam import Beam, gaussian_bunch
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    pts = gaussian_bunch(300, seed=9)
+    pts[:5, 4] += 2 * np.pi * np.array([1, -1, 2, 3, -2])
+    p = pts.copy()
+    refpt = np.zeros(6)
+    refpt[5] = -b.gamma0
+    n, pr, _, _ = oracle.conv1st(c, p, refpt, 1.0, 1.0, 300, 0.1)
+    grid = (8, 8, 8)
+    rng_, ms = oracle.update_compdom(pr, grid)
+    oracle.cvbkforth1st(c, p, refpt)
+    oracle.kick1(c, p, np.zeros((8, 8, 8)), rng_, ms, refpt, 0.1, 0.01, b.mass, b.charge)
+    want = pts.copy()
+    pilc = float(np.float32(np.pi))
+    nt = np.trunc(want[:, 4] / pilc)
+    want[:, 4] = want[:, 4] - nt * pilc - np.fmod(nt, 2) * pilc
+    assert col_err(p, want) < 1e-12
