@@ -102,39 +102,53 @@ def roofline(res, n, g):
             "traffic": None, "bytes_per_launch": byts, "avg_ms": km[kern]}
 
 
-def cpu_baseline(grid, sample_n: int, steps=(1, 3)):
-    """The reference executable (built from source by oracle/Makefile.ref),
-    1 core, on a bounded sample of the same workload: ``sample_n`` particles
-    of the same beam on the same grid.  Per-step time is the difference of
-    two runs of steps[1] and steps[0] segments, so deck parsing and the
-    particle.in read cancel; the per-step fort.* writes are included."""
+def _ref_step_time(exe, tmp, tag, n, grid, steps):
+    """Per-step wall time of the reference: (t(steps[1]) - t(steps[0])) / diff,
+    so deck parsing and the particle.in read cancel (fort.* writes stay in)."""
+    pts = B.core_halo_bunch(n, seed=1)
+    pin = os.path.join(tmp, f"{tag}_particle.in")
+    B.write_particle_in(pin, pts)
+    times = []
+    for s in steps:
+        d = os.path.join(tmp, f"{tag}_s{s}")
+        os.makedirs(d)
+        os.symlink(pin, os.path.join(d, "particle.in"))
+        B.write_deck(os.path.join(d, "ImpactZ.in"), n, grid, B.Beam(),
+                     [B.Element(0.4, s, 1, 0, 1.0)])
+        t0 = time.perf_counter()
+        subprocess.run(f"ulimit -s unlimited; exec {exe}", shell=True, cwd=d,
+                       check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu reference {tag}: n={n} grid={grid} {s} steps {times[-1]:.2f} s")
+    return (times[1] - times[0]) / (steps[1] - steps[0])
+
+
+def cpu_baseline(grid, n_full: int, n1=20_000, n2=400_000):
+    """The reference executable (built from source by oracle/Makefile.ref) on
+    one host core, on a bounded sample of the workload, scaled to it.  The
+    per-step time t(n) = a + n*b is measured on the workload's own grid at two
+    particle counts n1 < n2 (each as the difference of a 2-step and a 1-step
+    run, so start-up and particle.in parsing cancel):
+      b = (t(n2) - t(n1)) / (n2 - n1),  a = t(n1) - n1*b,
+      reported rate = n_full / (a + n_full*b)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ImpactZexe")
     if not os.access(exe, os.X_OK):
         return None
     tmp = tempfile.mkdtemp(prefix="impz_cpu_")
     try:
-        pts = B.core_halo_bunch(sample_n, seed=1)
-        B.write_particle_in(os.path.join(tmp, "particle.in"), pts)
-        times = []
-        for s in steps:
-            d = os.path.join(tmp, f"s{s}")
-            os.makedirs(d)
-            os.symlink(os.path.join(tmp, "particle.in"), os.path.join(d, "particle.in"))
-            B.write_deck(os.path.join(d, "ImpactZ.in"), sample_n, grid, B.Beam(),
-                         [B.Element(0.4, s, 1, 0, 1.0)])
-            t0 = time.perf_counter()
-            subprocess.run(f"ulimit -s unlimited; exec {exe}", shell=True, cwd=d,
-                           check=True, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL)
-            times.append(time.perf_counter() - t0)
-            log(f"cpu reference: {s} steps {times[-1]:.2f} s")
-        per_step = (times[1] - times[0]) / (steps[1] - steps[0])
-        return {"value": sample_n / per_step, "unit": "particle-steps/s", "cores": 1,
+        t1 = _ref_step_time(exe, tmp, "n1", n1, grid, (1, 2))
+        t2 = _ref_step_time(exe, tmp, "n2", n2, grid, (1, 2))
+        b = max((t2 - t1) / (n2 - n1), 0.0)
+        a = max(t1 - n1 * b, 0.0)
+        per_step = a + n_full * b
+        return {"value": n_full / per_step, "unit": "particle-steps/s", "cores": 1,
                 "kind": "reference",
-                "sample": (f"ImpactZexe (serial, amdflang -O3) on {sample_n} particles, "
-                           f"grid {grid[0]}x{grid[1]}x{grid[2]}; per-step time "
-                           f"{per_step:.3f} s = (t({steps[1]} steps) - t({steps[0]} step))"
-                           f"/{steps[1] - steps[0]}, fort.* writes included")}
+                "sample": (f"ImpactZexe (serial build, amdflang -O3, 1 core) on the "
+                           f"{grid[0]}x{grid[1]}x{grid[2]} grid with {n1} and {n2} "
+                           f"particles, 1 and 2 steps each: grid work a={a:.3f} s/step, "
+                           f"particle work b={b * 1e6:.3f} us/particle-step; rate = "
+                           f"{n_full}/(a + {n_full}*b), extrapolated to {n_full} "
+                           f"particles (fort.* writes included)")}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -147,7 +161,6 @@ def main():
     ap.add_argument("--particles", type=int, default=50_000_000)
     ap.add_argument("--grid", default="128,128,256")
     ap.add_argument("--forms", default="fp64,mixed")
-    ap.add_argument("--cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -199,7 +212,7 @@ def main():
             "roofline": roofline(m, n, g), "kernels_ms": m["kernels_ms"],
         }
     if not args.no_cpu:
-        cb = cpu_baseline(grid, args.cpu_sample)
+        cb = cpu_baseline(grid, n)
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None

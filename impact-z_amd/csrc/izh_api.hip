@@ -565,9 +565,9 @@ static int check_dom(const izh_field *f, const izh_compdom *dom, const char *who
 
 int izh_charge(izh_bunch *b, const izh_compdom *dom, izh_field *f) {
   if (!b || !f) return fail(IZH_ERR_ARG, "izh_charge: bad argument");
-  IZH_CHECK(check_dom(f, dom, "izh_charge"));
   if (b->frame != kFrameT1)
     return fail(IZH_ERR_STATE, "izh_charge: call izh_conv1st first");
+  IZH_CHECK(check_dom(f, dom, "izh_charge"));
   if (b->precision != f->precision)
     return fail(IZH_ERR_ARG, "izh_charge: bunch and field precision differ");
   IZH_CHECK(join_streams(b->stream, f->stream));
