@@ -41,59 +41,60 @@ def log(msg):
 
 
 def algorithmic_bytes(kernel: str, n: int, g: int, form: str) -> float:
-    """Bytes a launch must move at minimum (particles SoA, grid once).
+    """Bytes a launch must move at minimum (SoA particles, each grid once).
 
-    deposit : read t-frame x, y, z and the charge weight, write rho once
-    kick    : read and write the 6 phase-space rows, read Ex, Ey, Ez once
+    pass    : the fused particle pass -- read and write the 6 FP64 phase-space
+              rows, write the t-frame positions for the deposit (3 x 8 B FP64,
+              3 x 4 B mixed), read Ex, Ey, Ez once
+    deposit : read the t-frame positions, write rho once
     """
     pe = 8 if form == "fp64" else 4
+    if kernel == "pass":
+        return n * (6 * 8 * 2 + 3 * pe) + 3 * g * pe
     if kernel == "deposit":
-        return n * (3 * 8 + 8) + g * pe
-    if kernel == "kick":
-        return n * (6 * 8 * 2) + 3 * g * pe
+        return n * 3 * pe + g * pe
     raise ValueError(kernel)
 
 
 def run_form(pts, beam, lattice, grid, precision, steps, warmup):
+    """Warm up, then time `steps` space-charge steps run through izh_run
+    (each step: both half pushes, z->t, domain, deposit, Green function,
+    FFT solve, E, gather/kick and the 37 moment sums, all on the device)."""
     form = "fp64" if precision == hip.FP64 else "mixed"
     n = pts.shape[0]
     sim = Simulation(pts, beam, lattice, grid, precision=precision)
-    segs = sim.segments()
     log(f"{form}: warmup {warmup} steps")
-    for _ in range(warmup):
-        sim.step(next(segs), want_moments=True)
+    sim.run_fused(warmup, start=0)
     hip.timer_enable(True)
     hip.timer_read(reset=True)
-    log(f"{form}: timing {steps} steps")
+    log(f"{form}: timing {steps} steps with kernel timers")
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step(next(segs), want_moments=True)   # ends with a synchronising reduction
-    t1 = time.perf_counter()
+    sim.run_fused(steps, start=warmup)
+    dt_timed = time.perf_counter() - t0
     timers = hip.timer_read(reset=True)
     hip.timer_enable(False)
-    dt = t1 - t0
-    # second pass without kernel timers: the clock without event overhead
+    # the throughput clock: the same number of steps without kernel events
+    log(f"{form}: timing {steps} steps")
     t0 = time.perf_counter()
-    for _ in range(steps):
-        sim.step(next(segs), want_moments=True)
-    dt_clean = time.perf_counter() - t0
-    rows = sim.diag()
+    rec = sim.run_fused(steps, start=warmup + steps)
+    dt = time.perf_counter() - t0
+    last = sim.diag(rec[-1])
     sim.close()
     return {
         "form": form,
-        "ms_per_step": 1e3 * dt_clean / steps,
-        "value": n * steps / dt_clean,
-        "ms_per_step_timed": 1e3 * dt / steps,
+        "ms_per_step": 1e3 * dt / steps,
+        "value": n * steps / dt,
+        "ms_per_step_with_timers": 1e3 * dt_timed / steps,
         "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in timers.items() if v[1]},
         "kernels_launches": {k: v[1] for k, v in timers.items() if v[1]},
-        "diag": rows,
+        "diag": last,
     }
 
 
 def roofline(res, n, g):
     """Dominant particle kernel: algorithmic bytes / average HIP-event time."""
     km = res["kernels_ms"]
-    kern = max(("deposit", "kick"), key=lambda k: km.get(k, 0.0))
+    kern = max(("pass", "deposit"), key=lambda k: km.get(k, 0.0))
     t = km[kern] * 1e-3
     byts = algorithmic_bytes(kern, n, g, res["form"])
     ach = byts / t / 1e9
@@ -172,7 +173,7 @@ def main():
     g = grid[0] * grid[1] * grid[2]
     hip.init(0)
     beam = B.Beam()
-    lattice = B.fodo(25)  # 25 periods = 400 steps
+    lattice = B.fodo(25 * (1 + (2 * args.steps + args.warmup) // 400))  # >= 400 steps
     log(f"sampling {n} particles")
     pts = B.core_halo_bunch(n, seed=1)
     res = {}

@@ -6,6 +6,7 @@
 // calls of one segment.  Everything per particle or per grid cell runs in
 // the kernels of izh_particles.hip and izh_field.hip.
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -100,37 +101,98 @@ static void physconst(double freq, izh_const *c) {
   c->scfreq = freq;
 }
 
-static int bunch_alloc_rows(izh_bunch *b, double **rows, int64_t cap) {
-  for (int r = 0; r < 9; r++) {
-    if (hipMalloc((void **)&rows[r], (size_t)(cap > 0 ? cap : 1) * sizeof(double)) !=
-        hipSuccess)
-      return fail(IZH_ERR_NOMEM, "device allocation of %lld particles failed",
-                  (long long)cap);
-  }
+template <typename P>
+static int dmalloc(P **p, int64_t count) {
+  if (hipMalloc((void **)p, (size_t)(count > 0 ? count : 1) * sizeof(P)) != hipSuccess)
+    return fail(IZH_ERR_NOMEM, "device allocation of %lld x %zu bytes failed",
+                (long long)count, sizeof(P));
   return IZH_OK;
 }
 
-static void free_rows(double **rows) {
-  for (int r = 0; r < 9; r++) {
-    if (rows[r]) (void)hipFree(rows[r]);
-    rows[r] = nullptr;
-  }
+template <typename P>
+static void dfree(P *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
 }
 
-// Upload Pts1(9,n) (particle-major) into the SoA rows.
+// Moving rows (phase space + original index) for cap particles.
+static int bunch_alloc_live(izh_bunch *b, int64_t cap) {
+  for (int r = 0; r < 6; r++) IZH_CHECK(dmalloc(&b->r[r], cap));
+  IZH_CHECK(dmalloc(&b->orig, cap));
+  b->cap = cap;
+  return IZH_OK;
+}
+
+int bunch_alloc_alt(izh_bunch *b) {
+  for (int r = 0; r < 6; r++) IZH_CHECK(dmalloc(&b->r_alt[r], b->cap));
+  IZH_CHECK(dmalloc(&b->orig_alt, b->cap));
+  return IZH_OK;
+}
+
+static void bunch_free_all(izh_bunch *b) {
+  for (int r = 0; r < 6; r++) {
+    dfree(b->r[r]);
+    dfree(b->r_alt[r]);
+  }
+  dfree(b->orig);
+  dfree(b->orig_alt);
+  dfree(b->key);
+  dfree(b->key_alt);
+  dfree(b->tpos);
+  dfree(b->qm0);
+  dfree(b->w0);
+  dfree(b->id0);
+  dfree(b->d_scal);
+  for (int k = 0; k < 2; k++) dfree(b->hist[k]);
+  dfree(b->cursor);
+  dfree(b->chunks);
+  dfree(b->d_moments);
+  b->ntiles_cap = b->nchunks_cap = b->moments_cap = 0;
+  b->cap = 0;
+}
+
+// Upload Pts1(9,n) (particle-major): rows 1-6 into the moving SoA rows,
+// q/m, charge weight and id into the immutable rows, original index = i.
 static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
+  if (n > 0xfffffffeLL)
+    return fail(IZH_ERR_ARG, "bunch of %lld particles exceeds 2^32-2", (long long)n);
+  if (n > b->cap) {
+    IZH_HIP(hipStreamSynchronize(b->stream));
+    bunch_free_all(b);
+    IZH_CHECK(bunch_alloc_live(b, n));
+  }
+  dfree(b->qm0);
+  dfree(b->w0);
+  dfree(b->id0);
+  IZH_CHECK(dmalloc(&b->qm0, n));
+  IZH_CHECK(dmalloc(&b->w0, n));
+  IZH_CHECK(dmalloc(&b->id0, n));
   std::vector<double> row((size_t)(n > 0 ? n : 1));
   b->wabs_sum = 0.0;
+  b->w_uniform = 1;
+  b->w_value = n > 0 ? pts9[7] : 0.0;
+  double *dst[9] = {b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5],
+                    b->qm0, b->w0, b->id0};
   for (int r = 0; r < 9; r++) {
     for (int64_t i = 0; i < n; i++) row[i] = pts9[9 * i + r];
     if (r == 7)
-      for (int64_t i = 0; i < n; i++) b->wabs_sum += std::fabs(row[i]);
+      for (int64_t i = 0; i < n; i++) {
+        b->wabs_sum += std::fabs(row[i]);
+        if (row[i] != b->w_value) b->w_uniform = 0;
+      }
     if (n > 0)
-      IZH_HIP(hipMemcpyAsync(b->s[r], row.data(), n * sizeof(double),
+      IZH_HIP(hipMemcpyAsync(dst[r], row.data(), n * sizeof(double),
                              hipMemcpyHostToDevice, b->stream));
     IZH_HIP(hipStreamSynchronize(b->stream));
   }
+  std::vector<uint32_t> idx((size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) idx[i] = (uint32_t)i;
+  if (n > 0)
+    IZH_HIP(hipMemcpyAsync(b->orig, idx.data(), n * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, b->stream));
+  IZH_HIP(hipStreamSynchronize(b->stream));
   b->n = n;
+  b->n0 = n;
   return IZH_OK;
 }
 
@@ -221,17 +283,21 @@ int izh_bunch_create(const double *pts9, int64_t n, double current,
   for (int k = 0; k < 6; k++) b->refpt[k] = phase0;
   b->refpt[5] = -(kinetic / mass + 1.0);
   physconst(freq, &b->c);
-  b->cap = n;
-  int rc;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      (rc = bunch_alloc_rows(b, b->s, n)) != IZH_OK ||
-      hipMalloc((void **)&b->d_part, kMaxRedBlocks * kMoments * sizeof(double)) !=
-          hipSuccess ||
-      hipMalloc((void **)&b->d_out, (kMoments + 8) * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void **)&b->h_out, (kMoments + 8) * sizeof(double),
+  int rc = IZH_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(IZH_ERR_HIP, "izh_bunch_create: stream creation failed");
+  }
+  if ((rc = bunch_alloc_live(b, n)) != IZH_OK ||
+      (rc = dmalloc(&b->d_part, (int64_t)kMaxRedBlocks * kPartStride)) != IZH_OK ||
+      (rc = dmalloc(&b->d_out, (int64_t)kPartStride + 8)) != IZH_OK) {
+    izh_bunch_destroy(b);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&b->h_out, (kPartStride + 8) * sizeof(double),
                     hipHostMallocDefault) != hipSuccess) {
     izh_bunch_destroy(b);
-    return fail(IZH_ERR_NOMEM, "izh_bunch_create: allocation failed");
+    return fail(IZH_ERR_NOMEM, "izh_bunch_create: pinned allocation failed");
   }
   if ((rc = upload_pts(b, pts9, n)) != IZH_OK) {
     izh_bunch_destroy(b);
@@ -245,11 +311,11 @@ int izh_bunch_create(const double *pts9, int64_t n, double current,
 int izh_bunch_destroy(izh_bunch *b) {
   if (!b) return IZH_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  free_rows(b->s);
-  free_rows(b->alt);
-  if (b->d_part) (void)hipFree(b->d_part);
-  if (b->d_out) (void)hipFree(b->d_out);
+  bunch_free_all(b);
+  dfree(b->d_part);
+  dfree(b->d_out);
   if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_scal) (void)hipHostFree(b->h_scal);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return IZH_OK;
@@ -261,28 +327,46 @@ int izh_bunch_download(izh_bunch *b, double *pts9, int64_t cap, int64_t *n) {
   if (cap < b->n)
     return fail(IZH_ERR_ARG, "izh_bunch_download: %lld rows needed, %lld given",
                 (long long)b->n, (long long)cap);
-  std::vector<double> row((size_t)(b->n > 0 ? b->n : 1));
-  for (int r = 0; r < 9; r++) {
-    if (b->n > 0)
-      IZH_HIP(hipMemcpyAsync(row.data(), b->s[r], b->n * sizeof(double),
-                             hipMemcpyDeviceToHost, b->stream));
-    IZH_HIP(hipStreamSynchronize(b->stream));
-    for (int64_t i = 0; i < b->n; i++) pts9[9 * i + r] = row[i];
+  const int64_t nn = b->n, n0 = b->n0;
+  std::vector<uint32_t> orig((size_t)(nn > 0 ? nn : 1));
+  std::vector<double> rows((size_t)(6 * nn > 0 ? 6 * nn : 1));
+  std::vector<double> fixed((size_t)(3 * n0 > 0 ? 3 * n0 : 1));
+  for (int r = 0; r < 6 && nn > 0; r++)
+    IZH_HIP(hipMemcpyAsync(rows.data() + r * nn, b->r[r], nn * sizeof(double),
+                           hipMemcpyDeviceToHost, b->stream));
+  if (nn > 0)
+    IZH_HIP(hipMemcpyAsync(orig.data(), b->orig, nn * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, b->stream));
+  const double *fx[3] = {b->qm0, b->w0, b->id0};
+  for (int r = 0; r < 3 && n0 > 0; r++)
+    IZH_HIP(hipMemcpyAsync(fixed.data() + r * n0, fx[r], n0 * sizeof(double),
+                           hipMemcpyDeviceToHost, b->stream));
+  IZH_HIP(hipStreamSynchronize(b->stream));
+  // the reference keeps the surviving particles in their original order
+  // (order-preserving compaction, src/Appl/BeamBunch.f90:1290,1345-1347):
+  // place each row by its original index, then close the gaps
+  std::vector<int64_t> at((size_t)(n0 > 0 ? n0 : 1), -1);
+  for (int64_t i = 0; i < nn; i++) {
+    if (orig[i] >= (uint64_t)n0 || at[orig[i]] >= 0)
+      return fail(IZH_ERR_STATE, "izh_bunch_download: corrupt particle index");
+    at[orig[i]] = i;
   }
-  *n = b->n;
+  int64_t k = 0;
+  for (int64_t o = 0; o < n0; o++) {
+    const int64_t i = at[o];
+    if (i < 0) continue;
+    double *p = pts9 + 9 * k;
+    for (int r = 0; r < 6; r++) p[r] = rows[r * nn + i];
+    for (int r = 0; r < 3; r++) p[6 + r] = fixed[r * n0 + o];
+    k++;
+  }
+  *n = nn;
   return IZH_OK;
 }
 
 int izh_bunch_upload(izh_bunch *b, const double *pts9, int64_t n) {
   if (!b || n < 0 || (n > 0 && !pts9))
     return fail(IZH_ERR_ARG, "izh_bunch_upload: bad argument");
-  if (n > b->cap) {
-    IZH_HIP(hipStreamSynchronize(b->stream));
-    free_rows(b->s);
-    free_rows(b->alt);
-    IZH_CHECK(bunch_alloc_rows(b, b->s, n));
-    b->cap = n;
-  }
   IZH_CHECK(upload_pts(b, pts9, n));
   b->frame = kFrameZ;
   return IZH_OK;
@@ -462,7 +546,7 @@ int izh_conv1st(izh_bunch *b, double tau, double ptrange[6], int64_t *nplc,
   }
   b->frame = kFrameT1;
   if (nlost > 0) {
-    if (!b->alt[0]) IZH_CHECK(bunch_alloc_rows(b, b->alt, b->cap));
+    if (!b->r_alt[0]) IZH_CHECK(bunch_alloc_alt(b));
     IZH_CHECK(launch_compact(b, xrad, yrad, b->n - nlost));
   }
   // one process: nptot = nplc; Current = Current*nptot/Npt (:1657-1663)
@@ -506,6 +590,8 @@ int izh_field_create(int nx, int ny, int nz, int precision, izh_field **out) {
     return fail(IZH_ERR_ARG,
                 "izh_field_create: mesh %dx%dx%d must be powers of two in [4,1024]",
                 nx, ny, nz);
+  if ((int64_t)(nx + 2) * (ny + 2) * (nz + 2) > INT32_MAX)
+    return fail(IZH_ERR_ARG, "izh_field_create: mesh %dx%dx%d too large", nx, ny, nz);
   if (precision != IZH_FP64 && precision != IZH_MIXED)
     return fail(IZH_ERR_ARG, "izh_field_create: bad precision %d", precision);
   izh_field *f = new izh_field();
@@ -531,17 +617,19 @@ int izh_field_create(int nx, int ny, int nz, int precision, izh_field **out) {
 int izh_field_destroy(izh_field *f) {
   if (!f) return IZH_OK;
   if (f->stream) (void)hipStreamSynchronize(f->stream);
-  void *bufs[] = {f->rho_fx, f->rho, f->phi, f->e, f->spec_a, f->spec_b,
-                  f->green, f->green_t, f->corner, f->tw_x, f->tw_y, f->tw_z};
+  void *bufs[] = {f->rho_fx, f->rho,  f->phi,    f->e,    f->spec_a,
+                  f->spec_b, f->green, f->green_t, f->corner, f->tw_x,
+                  f->tw_y,   f->tw_z, f->d_dom,  f->chunks};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
+  if (f->h_dom) (void)hipHostFree(f->h_dom);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
   return IZH_OK;
 }
 
-// Kernels of a bunch and a field run on the bunch's stream while they share
-// data: the field's stream waits for everything queued on the bunch's.
+// Kernels of a bunch and a field run on their own streams while they share
+// data: the consumer's stream waits for everything queued on the producer's.
 static int join_streams(hipStream_t producer, hipStream_t consumer) {
   if (producer == consumer) return IZH_OK;
   hipEvent_t ev;
@@ -558,7 +646,7 @@ static int check_dom(const izh_field *f, const izh_compdom *dom, const char *who
     return fail(IZH_ERR_ARG, "%s: domain mesh %dx%dx%d != field %dx%dx%d", who,
                 dom->nx, dom->ny, dom->nz, f->nx, f->ny, f->nz);
   for (int a = 0; a < 3; a++)
-    if (!(dom->msize[a] > 0))
+    if (!(dom->msize[a] > 0) || !std::isfinite(dom->msize[a]))
       return fail(IZH_ERR_ARG, "%s: mesh size %g not positive", who, dom->msize[a]);
   return IZH_OK;
 }
@@ -570,8 +658,9 @@ int izh_charge(izh_bunch *b, const izh_compdom *dom, izh_field *f) {
   IZH_CHECK(check_dom(f, dom, "izh_charge"));
   if (b->precision != f->precision)
     return fail(IZH_ERR_ARG, "izh_charge: bunch and field precision differ");
+  IZH_CHECK(field_set_domain(f, dom));
   IZH_CHECK(join_streams(b->stream, f->stream));
-  IZH_CHECK(launch_deposit(b, dom, f));
+  IZH_CHECK(launch_deposit_granular(b, f));
   IZH_CHECK(join_streams(f->stream, b->stream));
   return IZH_OK;
 }
@@ -580,7 +669,8 @@ int izh_update3O(izh_field *f, const izh_compdom *dom) {
   if (!f) return fail(IZH_ERR_ARG, "izh_update3O: bad argument");
   IZH_CHECK(check_dom(f, dom, "izh_update3O"));
   if (!f->have_rho) return fail(IZH_ERR_STATE, "izh_update3O: no charge density");
-  IZH_CHECK(launch_solve(f, dom));
+  IZH_CHECK(field_set_domain(f, dom));
+  IZH_CHECK(launch_solve(f));
   return IZH_OK;
 }
 
@@ -599,8 +689,7 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
   if (flagbc != 1) return fail(IZH_ERR_ARG, "izh_kick1: only Flagbc=1 is supported");
   if (b->frame != kFrameT0)
     return fail(IZH_ERR_STATE, "izh_kick1: call izh_cvbkforth1st first");
-  // src/Appl/BeamBunch.f90:1731: no kick below 1e-20 A; particles stay in
-  // the t frame exactly as the reference leaves them
+  // src/Appl/BeamBunch.f90:1731: no kick below 1e-20 A
   if (b->current < (double)1.0e-20f) {
     b->frame = kFrameZ;
     return IZH_OK;
@@ -609,11 +698,13 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
   if (!f->have_phi) return fail(IZH_ERR_STATE, "izh_kick1: no potential");
   if (b->precision != f->precision)
     return fail(IZH_ERR_ARG, "izh_kick1: bunch and field precision differ");
-  IZH_CHECK(launch_efield(f, dom));
+  IZH_CHECK(field_set_domain(f, dom));
+  IZH_CHECK(launch_efield(f));
   IZH_CHECK(join_streams(f->stream, b->stream));
   double xycon, tcon;
   sc_constants(b, &xycon, &tcon);
-  IZH_CHECK(launch_gather_kick(b, f, dom, tau, xycon, tcon));
+  IZH_CHECK(launch_gather_kick(b, f, tau, xycon, tcon));
+  IZH_CHECK(join_streams(b->stream, f->stream));
   b->frame = kFrameZ;
   return IZH_OK;
 }
@@ -635,29 +726,24 @@ int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
 
 int izh_field_download(izh_field *f, int which, double *host) {
   if (!f || !host) return fail(IZH_ERR_ARG, "izh_field_download: bad argument");
-  const void *src;
-  switch (which) {
-    case IZH_RHO: src = f->rho; break;
-    case IZH_PHI: src = f->phi; break;
-    case IZH_EX: case IZH_EY: case IZH_EZ: {
-      size_t es = f->precision == IZH_FP64 ? 8 : 4;
-      src = (const char *)f->e + (size_t)(which - IZH_EX) * f->g * es;
-      break;
-    }
-    default: return fail(IZH_ERR_ARG, "izh_field_download: unknown buffer %d", which);
-  }
+  if (which < IZH_RHO || which > IZH_EZ)
+    return fail(IZH_ERR_ARG, "izh_field_download: unknown buffer %d", which);
   IZH_HIP(hipStreamSynchronize(f->stream));
   // the grid holds physical values times rho_scale (1 in FP64); rho, phi and
   // E are linear in the charge, so one factor restores physical units
   const double un = 1.0 / f->rho_scale;
-  if (f->precision == IZH_FP64) {
-    IZH_HIP(hipMemcpy(host, src, f->g * sizeof(double), hipMemcpyDeviceToHost));
-    if (un != 1.0)
-      for (int64_t i = 0; i < f->g; i++) host[i] *= un;
-  } else {
-    std::vector<float> tmp((size_t)f->g);
-    IZH_HIP(hipMemcpy(tmp.data(), src, f->g * sizeof(float), hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < f->g; i++) host[i] = (double)tmp[i] * un;
+  const bool fp64 = f->precision == IZH_FP64;
+  const size_t es = fp64 ? sizeof(double) : sizeof(float);
+  const int stride = which >= IZH_EX ? 4 : 1;          // E is interleaved
+  const int comp = which >= IZH_EX ? which - IZH_EX : 0;
+  const void *src = which == IZH_RHO ? f->rho : (which == IZH_PHI ? f->phi : f->e);
+  std::vector<char> tmp((size_t)f->g * stride * es);
+  IZH_HIP(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < f->g; i++) {
+    const int64_t k = i * stride + comp;
+    const double v = fp64 ? ((const double *)tmp.data())[k]
+                          : (double)((const float *)tmp.data())[k];
+    host[i] = v * un;
   }
   return IZH_OK;
 }
@@ -705,6 +791,36 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   IZH_CHECK(izh_drift1(b, z, tau1, bitype, gradient, mapstp));              // :1429
   if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
   return IZH_OK;
+}
+
+int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+            const izh_segment *segs, int64_t nseg, double bcurr,
+            double *moments) {
+  if (!b || !f || !dom || !z || (nseg > 0 && !segs) || nseg < 0)
+    return fail(IZH_ERR_ARG, "izh_run: bad argument");
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_run: bunch is not in the z frame");
+  if (dom->nx != f->nx || dom->ny != f->ny || dom->nz != f->nz)
+    return fail(IZH_ERR_ARG, "izh_run: domain mesh differs from the field's");
+  if (b->precision != f->precision)
+    return fail(IZH_ERR_ARG, "izh_run: bunch and field precision differ");
+  for (int64_t s = 0; s < nseg; s++)
+    if ((segs[s].bitype != 0 && segs[s].bitype != 1) || !(segs[s].tau1 >= 0) ||
+        (segs[s].bitype == 1 && segs[s].mapstp < 1))
+      return fail(IZH_ERR_ARG, "izh_run: segment %lld is not a drift or quadrupole",
+                  (long long)s);
+  if (nseg == 0) return IZH_OK;
+  // no space charge or nothing to carry it: the reference loop is maps only
+  if (bcurr < (double)1.0e-10f || b->n == 0) {
+    for (int64_t s = 0; s < nseg; s++) {
+      IZH_CHECK(izh_step(b, f, dom, z, segs[s].tau1, segs[s].bitype,
+                         segs[s].gradient, segs[s].mapstp, segs[s].xrad,
+                         segs[s].yrad, bcurr, moments ? moments + s * kRunRecord : nullptr));
+      if (moments) moments[s * kRunRecord + kMoments] = (double)b->npt;
+    }
+    return IZH_OK;
+  }
+  return run_steps(b, f, dom, z, segs, nseg, moments);
 }
 
 }  // extern "C"

@@ -17,6 +17,7 @@
 // (natural in, bit-reversed out) and decimation in time for the inverse of
 // the fused z pass (bit-reversed in, natural out), so no reorder pass is
 // needed between them.
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -120,17 +121,18 @@ template <typename T, int LOAD, int STORE, bool CONV>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(Lines p, const cx<T> *__restrict__ tw, int inverse) {
   __shared__ cx<T> buf[kFftPoints];
-  const int64_t nlines = p.nf * p.ns;
-  const int64_t line0 = (int64_t)blockIdx.x * p.B;
+  const int nf = (int)p.nf;
+  const int nlines = nf * (int)p.ns;
+  const int line0 = blockIdx.x * p.B;
   const int tot = p.B * p.L;
   // ---- load (zero padding / even extension / Hermitian completion)
   for (int t = threadIdx.x; t < tot; t += kFftThreads) {
     int lb, m;
     map_thread(t, p.se, p, lb, m);
-    int64_t line = line0 + lb;
+    int line = line0 + lb;
     cx<T> v = {(T)0, (T)0};
     if (line < nlines) {
-      int64_t f = line % p.nf, s = line / p.nf;
+      int f = line % nf, s = line / nf;
       int64_t base = f * p.sf + s * p.ss;
       if (LOAD == kLoadReal) {
         if (m < p.Lin) v.re = ((const T *)p.src)[base + m * p.se];
@@ -157,10 +159,10 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ tw, int inverse) {
     const T *G = (const T *)p.green;
     for (int t = threadIdx.x; t < tot; t += kFftThreads) {
       int lb = t >> p.log2L, q = t & (p.L - 1);
-      int64_t line = line0 + lb;
+      int line = line0 + lb;
       if (line < nlines) {
-        int64_t i = line % p.nf, j2 = line / p.nf;
-        int64_t jf = j2 <= p.my / 2 ? j2 : p.my - j2;
+        int i = line % nf, j2 = line / nf;
+        int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
         int kz = bitrev(q, p.log2L);
         int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
         T g = G[i + p.gkx * jf + p.gky * kf];
@@ -176,9 +178,9 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ tw, int inverse) {
   for (int t = threadIdx.x; t < totout; t += kFftThreads) {
     int lb, m;
     map_thread(t, p.de, p, lb, m);
-    int64_t line = line0 + lb;
+    int line = line0 + lb;
     if (line >= nlines || m >= p.Lout) continue;
-    int64_t f = line % p.nf, s = line / p.nf;
+    int f = line % nf, s = line / nf;
     int64_t o = f * p.df + s * p.ds + m * p.de;
     cx<T> v = CONV ? buf[lb * p.L + m] : buf[lb * p.L + bitrev(m, p.log2L)];
     if (STORE == kStoreCplx) {
@@ -190,16 +192,18 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ tw, int inverse) {
 }
 
 // ------------------------------------------------------ Green function
-// IGF antiderivative at cell corners (src/Appl/Field.f90:384-418), FP64.
+// IGF antiderivative at cell corners (src/Appl/Field.f90:384-418), FP64,
+// mesh sizes from the device domain.
 __global__ void k_igf_corner(double *__restrict__ out, int tx, int ty, int tz,
-                             double hx, double hy, double hz) {
-  int64_t n = (int64_t)tx * ty * tz;
+                             const DevDom *__restrict__ dom) {
+  const double hx = dom->msize[0], hy = dom->msize[1], hz = dom->msize[2];
   const double vol = hx * hy * hz;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(c % tx);
-    int j = (int)((c / tx) % ty);
-    int k = (int)(c / ((int64_t)tx * ty));
+  const int n = tx * ty * tz;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    int i = c % tx;
+    int j = (c / tx) % ty;
+    int k = c / (tx * ty);
     double x = i * hx - hx / 2.0, y = j * hy - hy / 2.0, z = k * hz - hz / 2.0;
     double rr = sqrt(x * x + y * y + z * z);
     double aa = -z * z * atan(x * y / (z * rr)) / 2.0;
@@ -218,13 +222,13 @@ __global__ void k_igf_diff(const double *__restrict__ t, T *__restrict__ g,
                            int nx, int ny, int nz) {
   const int tx = nx + 2, ty = ny + 2;
   const int gx = nx + 1, gy = ny + 1, gz = nz + 1;
-  int64_t n = (int64_t)gx * gy * gz;
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(c % gx);
-    int j = (int)((c / gx) % gy);
-    int k = (int)(c / ((int64_t)gx * gy));
-#define TT(a, b, cc) t[(a) + (int64_t)tx * ((b) + (int64_t)ty * (cc))]
+  const int n = gx * gy * gz;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    int i = c % gx;
+    int j = (c / gx) % gy;
+    int k = c / (gx * gy);
+#define TT(a, b, cc) t[(a) + tx * ((b) + ty * (cc))]
     double v = TT(i + 1, j + 1, k + 1) - TT(i, j + 1, k + 1) - TT(i + 1, j, k + 1) +
                TT(i, j, k + 1) - TT(i + 1, j + 1, k) + TT(i, j + 1, k) +
                TT(i + 1, j, k) - TT(i, j, k);
@@ -234,48 +238,166 @@ __global__ void k_igf_diff(const double *__restrict__ t, T *__restrict__ g,
 }
 
 // --------------------------------------------------------------- deposit
-struct Dom {
-  double xmin, ymin, zmin, hx, hy, hz, hxi, hyi, hzi;
-  int nx, ny, nz;
-};
-
 __global__ void k_zero_u64(unsigned long long *__restrict__ p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0ull;
 }
 
-// deposit_BeamBunch (src/Appl/BeamBunch.f90:3737-3766) in fixed point.
+// Lower-node cell (0-based) and CIC weights of deposit_BeamBunch
+// (src/Appl/BeamBunch.f90:3737-3744) in the position type P.
+template <typename P>
+struct Cic {
+  int i, j, k;
+  P ab, cd, ef;
+};
+
+template <typename P>
+__device__ inline Cic<P> cic_of(P x, P y, P z, P xmin, P ymin, P zmin, P hx, P hy,
+                                P hz, P hxi, P hyi, P hzi) {
+  Cic<P> c;
+  int ix = (int)((x - xmin) * hxi + (P)1);
+  c.ab = ((xmin - x) + ix * hx) * hxi;
+  int jx = (int)((y - ymin) * hyi + (P)1);
+  c.cd = ((ymin - y) + jx * hy) * hyi;
+  int kx = (int)((z - zmin) * hzi + (P)1);
+  c.ef = ((zmin - z) + kx * hz) * hzi;
+  c.i = ix - 1;
+  c.j = jx - 1;
+  c.k = kx - 1;
+  return c;
+}
+
+// Fixed-point contributions of one particle, in the reference's node order.
+template <typename P>
+__device__ inline void cic_contrib(const Cic<P> &c, double wq, long long v[8]) {
+  const P one = (P)1;
+  const P a = c.ab, b = c.cd, e = c.ef;
+  v[0] = __double2ll_rn((double)(a * b * e) * wq);
+  v[1] = __double2ll_rn((double)(a * (one - b) * e) * wq);
+  v[2] = __double2ll_rn((double)(a * (one - b) * (one - e)) * wq);
+  v[3] = __double2ll_rn((double)(a * b * (one - e)) * wq);
+  v[4] = __double2ll_rn((double)((one - a) * b * (one - e)) * wq);
+  v[5] = __double2ll_rn((double)((one - a) * (one - b) * (one - e)) * wq);
+  v[6] = __double2ll_rn((double)((one - a) * (one - b) * e) * wq);
+  v[7] = __double2ll_rn((double)((one - a) * b * e) * wq);
+}
+
+// Node offsets (di, dj, dk) in the order of cic_contrib.
+__constant__ int kNode[8][3] = {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {0, 0, 1},
+                                {1, 0, 1}, {1, 1, 1}, {1, 1, 0}, {1, 0, 0}};
+
+__device__ inline int wave_min_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off, 64));
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// deposit_BeamBunch (src/Appl/BeamBunch.f90:3678-3768) for one chunk of
+// consecutive particles per block.  The chunk's cells are bounded first; if
+// the box (plus the upper CIC node) fits kWindowCells, the block accumulates
+// in LDS and adds only the box to the grid, otherwise it adds straight to the
+// grid.  Values are fixed point (int64, scale 2^e), so the result is exact
+// and independent of the order of the additions.  Particles with a NaN x
+// (lost in the fused run) and nodes outside the grid are skipped, as the
+// reference skips them (:3746-3747).
+template <typename P>
 __global__ void __launch_bounds__(kBlock)
-k_deposit_fx(const double *__restrict__ X, const double *__restrict__ Y,
-             const double *__restrict__ Z, const double *__restrict__ W,
-             int64_t n, Dom d, double scale, unsigned long long *__restrict__ rho) {
-  const int64_t sy = d.nx, sz = (int64_t)d.nx * d.ny;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double x = X[i], y = Y[i], z = Z[i];
-    int ix = (int)((x - d.xmin) * d.hxi + 1.0);
-    double ab = ((d.xmin - x) + ix * d.hx) * d.hxi;
-    int jx = (int)((y - d.ymin) * d.hyi + 1.0);
-    double cd = ((d.ymin - y) + jx * d.hy) * d.hyi;
-    int kx = (int)((z - d.zmin) * d.hzi + 1.0);
-    double ef = ((d.zmin - z) + kx * d.hz) * d.hzi;
-    if (ix < 1 || ix + 1 > d.nx || jx < 1 || jx + 1 > d.ny || kx < 1 ||
-        kx + 1 > d.nz)
+k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
+                 const P *__restrict__ Z, const uint32_t *__restrict__ orig,
+                 const double *__restrict__ w0, double w_uniform, int uniform,
+                 const Chunk *__restrict__ chunks, const int64_t *__restrict__ d_nchunks,
+                 int64_t n_flat, const DevDom *__restrict__ dom, int nx, int ny,
+                 int nz, double scale, unsigned long long *__restrict__ rho,
+                 int64_t *__restrict__ overflow) {
+  __shared__ unsigned long long win[kWindowCells];
+  __shared__ int red[kBlock / 64][6];
+  __shared__ int box[6];
+  int64_t start, len;
+  if (chunks) {
+    if (blockIdx.x >= *d_nchunks) return;
+    start = chunks[blockIdx.x].start;
+    len = chunks[blockIdx.x].len;
+  } else {
+    start = (int64_t)blockIdx.x * kChunk;
+    if (start >= n_flat) return;
+    len = min((int64_t)kChunk, n_flat - start);
+  }
+  const P xmin = (P)dom->range[0], ymin = (P)dom->range[2], zmin = (P)dom->range[4];
+  const P hx = (P)dom->msize[0], hy = (P)dom->msize[1], hz = (P)dom->msize[2];
+  const P hxi = (P)dom->hinv[0], hyi = (P)dom->hinv[1], hzi = (P)dom->hinv[2];
+  // ---- bounding box of the chunk's lower nodes
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  for (int64_t t = threadIdx.x; t < len; t += kBlock) {
+    int64_t i = start + t;
+    P x = X[i];
+    if (!(x == x)) continue;
+    Cic<P> c = cic_of<P>(x, Y[i], Z[i], xmin, ymin, zmin, hx, hy, hz, hxi, hyi, hzi);
+    if (c.i < 0 || c.i + 1 >= nx || c.j < 0 || c.j + 1 >= ny || c.k < 0 ||
+        c.k + 1 >= nz)
       continue;
-    double w = W[i] * scale;
-    int64_t o = (ix - 1) + sy * (jx - 1) + sz * (kx - 1);
-    auto add = [&](int64_t idx, double c) {
-      atomicAdd(&rho[idx], (unsigned long long)__double2ll_rn(c));
-    };
-    add(o, ab * cd * ef * w);
-    add(o + sy, ab * (1.0 - cd) * ef * w);
-    add(o + sy + sz, ab * (1.0 - cd) * (1.0 - ef) * w);
-    add(o + sz, ab * cd * (1.0 - ef) * w);
-    add(o + 1 + sz, (1.0 - ab) * cd * (1.0 - ef) * w);
-    add(o + 1 + sy + sz, (1.0 - ab) * (1.0 - cd) * (1.0 - ef) * w);
-    add(o + 1 + sy, (1.0 - ab) * (1.0 - cd) * ef * w);
-    add(o + 1, (1.0 - ab) * cd * ef * w);
+    lo[0] = min(lo[0], c.i); lo[1] = min(lo[1], c.j); lo[2] = min(lo[2], c.k);
+    hi[0] = max(hi[0], c.i); hi[1] = max(hi[1], c.j); hi[2] = max(hi[2], c.k);
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int v6[6];
+  for (int a = 0; a < 3; a++) { v6[a] = wave_min_i(lo[a]); v6[3 + a] = wave_max_i(hi[a]); }
+  if (lane == 0)
+    for (int a = 0; a < 6; a++) red[wv][a] = v6[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int a = threadIdx.x, r = red[0][a];
+    for (int w = 1; w < kBlock / 64; w++) r = a < 3 ? min(r, red[w][a]) : max(r, red[w][a]);
+    box[a] = r;
+  }
+  __syncthreads();
+  if (box[0] == INT_MAX) return;  // nothing on the grid in this chunk
+  const int bx = box[0], by = box[1], bz = box[2];
+  const int wx = box[3] - bx + 2, wy = box[4] - by + 2, wz = box[5] - bz + 2;
+  const int64_t wcells = (int64_t)wx * wy * wz;
+  const bool local = wcells <= kWindowCells;
+  if (local) {
+    for (int c = threadIdx.x; c < wcells; c += kBlock) win[c] = 0ull;
+    __syncthreads();
+  } else if (threadIdx.x == 0 && overflow) {
+    atomicAdd((unsigned long long *)overflow, 1ull);
+  }
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  for (int64_t t = threadIdx.x; t < len; t += kBlock) {
+    int64_t i = start + t;
+    P x = X[i];
+    if (!(x == x)) continue;
+    Cic<P> c = cic_of<P>(x, Y[i], Z[i], xmin, ymin, zmin, hx, hy, hz, hxi, hyi, hzi);
+    if (c.i < 0 || c.i + 1 >= nx || c.j < 0 || c.j + 1 >= ny || c.k < 0 ||
+        c.k + 1 >= nz)
+      continue;
+    const double w = uniform ? w_uniform : w0[orig[i]];
+    long long v[8];
+    cic_contrib<P>(c, w * scale, v);
+    if (local) {
+      const int l0 = (c.i - bx) + wx * ((c.j - by) + wy * (c.k - bz));
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&win[l0 + kNode[q][0] + wx * (kNode[q][1] + wy * kNode[q][2])],
+                  (unsigned long long)v[q]);
+    } else {
+      const int64_t g0 = c.i + sy * c.j + sz * c.k;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&rho[g0 + kNode[q][0] + sy * kNode[q][1] + sz * kNode[q][2]],
+                  (unsigned long long)v[q]);
+    }
+  }
+  if (!local) return;
+  __syncthreads();
+  for (int c = threadIdx.x; c < wcells; c += kBlock) {
+    unsigned long long v = win[c];
+    if (v == 0ull) continue;
+    int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
+    atomicAdd(&rho[(bx + lx) + sy * (by + ly) + sz * (bz + lz)], v);
   }
 }
 
@@ -289,32 +411,32 @@ __global__ void k_fx_to_grid(const unsigned long long *__restrict__ fx,
 
 // ------------------------------------------------------------- E field
 // kick1_BeamBunch one-process Flagbc=1 stencils (src/Appl/BeamBunch.f90:
-// 1782-1794, 1885-1899, 1966-2005): central differences, one-sided at edges.
+// 1782-1794, 1885-1899, 1966-2005): central differences, one-sided at edges,
+// written interleaved (Ex, Ey, Ez, 0) for the 8-node gather.
 template <typename T>
-__global__ void k_efield(const T *__restrict__ phi, T *__restrict__ ex,
-                         T *__restrict__ ey, T *__restrict__ ez, int nx, int ny,
-                         int nz, double hxi, double hyi, double hzi) {
-  const int64_t sy = nx, sz = (int64_t)nx * ny, n = sz * nz;
-  const T hx = (T)hxi, hy = (T)hyi, hz = (T)hzi;
-  const T hx2 = (T)(0.5 * hxi), hy2 = (T)(0.5 * hyi), hz2 = (T)(0.5 * hzi);
-  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
-       c += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(c % nx);
-    int j = (int)((c / nx) % ny);
-    int k = (int)(c / sz);
-    T vx, vy, vz;
-    if (i == 0) vx = hx * (phi[c] - phi[c + 1]);
-    else if (i == nx - 1) vx = hx * (phi[c - 1] - phi[c]);
-    else vx = hx2 * (phi[c - 1] - phi[c + 1]);
-    if (j == 0) vy = hy * (phi[c] - phi[c + sy]);
-    else if (j == ny - 1) vy = hy * (phi[c - sy] - phi[c]);
-    else vy = hy2 * (phi[c - sy] - phi[c + sy]);
-    if (k == 0) vz = hz * (phi[c] - phi[c + sz]);
-    else if (k == nz - 1) vz = hz * (phi[c - sz] - phi[c]);
-    else vz = hz2 * (phi[c - sz] - phi[c + sz]);
-    ex[c] = vx;
-    ey[c] = vy;
-    ez[c] = vz;
+__global__ void k_efield(const T *__restrict__ phi, E4<T> *__restrict__ e,
+                         int nx, int ny, int nz, const DevDom *__restrict__ dom) {
+  const int sy = nx, sz = nx * ny, n = sz * nz;
+  const T hx = (T)dom->hinv[0], hy = (T)dom->hinv[1], hz = (T)dom->hinv[2];
+  const T hx2 = (T)(0.5 * dom->hinv[0]), hy2 = (T)(0.5 * dom->hinv[1]),
+          hz2 = (T)(0.5 * dom->hinv[2]);
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += gridDim.x * blockDim.x) {
+    int i = c % nx;
+    int j = (c / nx) % ny;
+    int k = c / sz;
+    E4<T> v;
+    if (i == 0) v.x = hx * (phi[c] - phi[c + 1]);
+    else if (i == nx - 1) v.x = hx * (phi[c - 1] - phi[c]);
+    else v.x = hx2 * (phi[c - 1] - phi[c + 1]);
+    if (j == 0) v.y = hy * (phi[c] - phi[c + sy]);
+    else if (j == ny - 1) v.y = hy * (phi[c - sy] - phi[c]);
+    else v.y = hy2 * (phi[c - sy] - phi[c + sy]);
+    if (k == 0) v.z = hz * (phi[c] - phi[c + sz]);
+    else if (k == nz - 1) v.z = hz * (phi[c - sz] - phi[c]);
+    else v.z = hz2 * (phi[c - sz] - phi[c + sz]);
+    v.w = (T)0;
+    e[c] = v;
   }
 }
 
@@ -338,17 +460,16 @@ int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
 }
 
 template <typename T>
-int solve_t(izh_field *f, const izh_compdom *dom) {
+int solve_t(izh_field *f) {
   const int nx = f->nx, ny = f->ny, nz = f->nz;
   const int mx = 2 * nx, my = 2 * ny, mz = 2 * nz, kx = nx + 1;
-  const double hx = dom->msize[0], hy = dom->msize[1], hz = dom->msize[2];
   // ---- G^: IGF corners, 8-corner difference, 3 even real transforms
   {
     ScopedTimer tm_(IZH_T_GREEN, f->stream);
-    int64_t nc = (int64_t)(nx + 2) * (ny + 2) * (nz + 2);
+    int nc = (nx + 2) * (ny + 2) * (nz + 2);
     hipLaunchKernelGGL(k_igf_corner, dim3(grid_for(nc)), dim3(kBlock), 0,
-                       f->stream, f->corner, nx + 2, ny + 2, nz + 2, hx, hy, hz);
-    int64_t ng = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+                       f->stream, f->corner, nx + 2, ny + 2, nz + 2, f->d_dom);
+    int ng = (nx + 1) * (ny + 1) * (nz + 1);
     hipLaunchKernelGGL(k_igf_diff<T>, dim3(grid_for(ng)), dim3(kBlock), 0,
                        f->stream, f->corner, (T *)f->green_t, nx, ny, nz);
     IZH_HIP(hipGetLastError());
@@ -376,8 +497,8 @@ int solve_t(izh_field *f, const izh_compdom *dom) {
     p.out_scale = 1.0 / ((double)mx * my * mz);
     IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false>(f, p, f->tw_z, false)));
   }
-  // ---- forward x: rho (nx,ny,nz) real -> A (kx, ny, nz)
   ScopedTimer tm_(IZH_T_FFT, f->stream);
+  // ---- forward x: rho (nx,ny,nz) real -> A (kx, ny, nz)
   Lines p{};
   p.src = f->rho; p.dst = f->spec_a;
   p.nf = ny; p.ns = nz;
@@ -445,6 +566,14 @@ int upload_twiddles(izh_field *f) {
 
 // ================================================================ launchers
 
+double fixed_point_scale(double wabs_sum) {
+  // every cell holds at most sum|w| (CIC weights of a particle sum to one),
+  // so 2^e * sum|w| <= 2^61 leaves headroom for the +-0.5 rounding per term
+  double wsum = wabs_sum > 0 ? wabs_sum : 1.0;
+  int e = 61 - (int)std::ceil(std::log2(wsum));
+  return std::ldexp(1.0, e);
+}
+
 int field_alloc(izh_field *f) {
   const size_t es = f->precision == IZH_FP64 ? sizeof(double) : sizeof(float);
   const int64_t g = f->g, kx = f->nx + 1;
@@ -452,38 +581,67 @@ int field_alloc(izh_field *f) {
   IZH_HIP(hipMalloc((void **)&f->rho_fx, g * sizeof(unsigned long long)));
   IZH_HIP(hipMalloc(&f->rho, g * es));
   IZH_HIP(hipMalloc(&f->phi, g * es));
-  IZH_HIP(hipMalloc(&f->e, 3 * g * es));
+  IZH_HIP(hipMalloc(&f->e, 4 * g * es));
   IZH_HIP(hipMalloc(&f->spec_a, kx * f->ny * f->nz * 2 * es));
   IZH_HIP(hipMalloc(&f->spec_b, kx * 2 * f->ny * f->nz * 2 * es));
   IZH_HIP(hipMalloc(&f->green, ghalf * es));
   IZH_HIP(hipMalloc(&f->green_t, ghalf * es));
   IZH_HIP(hipMalloc((void **)&f->corner,
                     (size_t)(f->nx + 2) * (f->ny + 2) * (f->nz + 2) * sizeof(double)));
+  IZH_HIP(hipMalloc((void **)&f->d_dom, sizeof(DevDom)));
+  IZH_HIP(hipHostMalloc((void **)&f->h_dom, sizeof(DevDom), hipHostMallocDefault));
+  IZH_HIP(hipMemset(f->d_dom, 0, sizeof(DevDom)));
   if (f->precision == IZH_FP64) return upload_twiddles<double>(f);
   return upload_twiddles<float>(f);
 }
 
-int launch_deposit(izh_bunch *b, const izh_compdom *dom, izh_field *f) {
-  Dom d;
-  d.xmin = dom->range[0]; d.ymin = dom->range[2]; d.zmin = dom->range[4];
-  d.hx = dom->msize[0]; d.hy = dom->msize[1]; d.hz = dom->msize[2];
-  d.hxi = 1.0 / d.hx; d.hyi = 1.0 / d.hy; d.hzi = 1.0 / d.hz;
-  d.nx = f->nx; d.ny = f->ny; d.nz = f->nz;
-  // every cell holds at most sum|w| (CIC weights of a particle sum to one),
-  // so 2^e * sum|w| < 2^61 leaves headroom for the +-0.5 rounding per term
-  double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
-  int e = 61 - (int)std::ceil(std::log2(wsum));
-  f->fixed_scale = std::ldexp(1.0, e);
+int field_set_domain(izh_field *f, const izh_compdom *dom) {
+  // the pinned staging buffer may still feed an earlier copy
+  IZH_HIP(hipStreamSynchronize(f->stream));
+  DevDom d{};
+  for (int k = 0; k < 6; k++) d.range[k] = dom->range[k];
+  for (int k = 0; k < 3; k++) {
+    d.msize[k] = dom->msize[k];
+    d.hinv[k] = 1.0 / dom->msize[k];
+  }
+  *f->h_dom = d;
+  IZH_HIP(hipMemcpyAsync(f->d_dom, f->h_dom, sizeof(DevDom), hipMemcpyHostToDevice,
+                         f->stream));
+  return IZH_OK;
+}
+
+// Zero the accumulator, deposit the chunks, convert to the grid type.
+// positions: 0 = the bunch's phase-space rows (t frame, FP64),
+//            1 = the fused run's t-frame position buffer (T).
+int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
+                   const int64_t *d_nchunks, int64_t max_chunks, int positions) {
+  f->fixed_scale = fixed_point_scale(b->wabs_sum);
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(f->g)), dim3(kBlock), 0, f->stream,
                      f->rho_fx, f->g);
-  {
+  if (max_chunks > 0) {
     ScopedTimer tm_(IZH_T_DEPOSIT, f->stream);
-    if (b->n > 0)
-      hipLaunchKernelGGL(k_deposit_fx, dim3(grid_for(b->n)), dim3(kBlock), 0,
-                         f->stream, b->s[0], b->s[2], b->s[4], b->s[7], b->n, d,
-                         f->fixed_scale, f->rho_fx);
+    int64_t *ovf = b->d_scal ? &b->d_scal->overflow : nullptr;
+    if (positions == 1 && f->precision == IZH_MIXED) {
+      const float *tp = (const float *)b->tpos;
+      hipLaunchKernelGGL(k_deposit_chunks<float>, dim3((unsigned)max_chunks),
+                         dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
+                         b->orig, b->w0, b->w_value, b->w_uniform, chunks,
+                         d_nchunks, b->n, f->d_dom, f->nx, f->ny, f->nz,
+                         f->fixed_scale, f->rho_fx, ovf);
+    } else {
+      const double *X = positions == 1 ? (const double *)b->tpos : b->r[0];
+      const double *Y = positions == 1 ? X + b->cap : b->r[2];
+      const double *Z = positions == 1 ? X + 2 * b->cap : b->r[4];
+      hipLaunchKernelGGL(k_deposit_chunks<double>, dim3((unsigned)max_chunks),
+                         dim3(kBlock), 0, f->stream, X, Y, Z, b->orig, b->w0,
+                         b->w_value, b->w_uniform, chunks, d_nchunks, b->n,
+                         f->d_dom, f->nx, f->ny, f->nz, f->fixed_scale, f->rho_fx,
+                         ovf);
+    }
+    IZH_HIP(hipGetLastError());
   }
   // FP64: the grid holds charge itself; mixed: charge / sum|w| (O(1) values)
+  double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
   f->rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
   const double inv = f->rho_scale / f->fixed_scale;
   if (f->precision == IZH_FP64)
@@ -498,29 +656,27 @@ int launch_deposit(izh_bunch *b, const izh_compdom *dom, izh_field *f) {
   return IZH_OK;
 }
 
-int launch_solve(izh_field *f, const izh_compdom *dom) {
-  int rc = f->precision == IZH_FP64 ? solve_t<double>(f, dom) : solve_t<float>(f, dom);
+int launch_deposit_granular(izh_bunch *b, izh_field *f) {
+  const int64_t nch = (b->n + kChunk - 1) / kChunk;
+  return launch_deposit(b, f, nullptr, nullptr, nch, 0);
+}
+
+int launch_solve(izh_field *f) {
+  int rc = f->precision == IZH_FP64 ? solve_t<double>(f) : solve_t<float>(f);
   if (rc == IZH_OK) f->have_phi = 1;
   return rc;
 }
 
-int launch_efield(izh_field *f, const izh_compdom *dom) {
-  const double hxi = 1.0 / dom->msize[0], hyi = 1.0 / dom->msize[1],
-               hzi = 1.0 / dom->msize[2];
-  {
-    ScopedTimer tm_(IZH_T_EFIELD, f->stream);
-    if (f->precision == IZH_FP64) {
-      double *e = (double *)f->e;
-      hipLaunchKernelGGL(k_efield<double>, dim3(grid_for(f->g)), dim3(kBlock), 0,
-                         f->stream, (const double *)f->phi, e, e + f->g,
-                         e + 2 * f->g, f->nx, f->ny, f->nz, hxi, hyi, hzi);
-    } else {
-      float *e = (float *)f->e;
-      hipLaunchKernelGGL(k_efield<float>, dim3(grid_for(f->g)), dim3(kBlock), 0,
-                         f->stream, (const float *)f->phi, e, e + f->g,
-                         e + 2 * f->g, f->nx, f->ny, f->nz, hxi, hyi, hzi);
-    }
-  }
+int launch_efield(izh_field *f) {
+  ScopedTimer tm_(IZH_T_EFIELD, f->stream);
+  if (f->precision == IZH_FP64)
+    hipLaunchKernelGGL(k_efield<double>, dim3(grid_for(f->g)), dim3(kBlock), 0,
+                       f->stream, (const double *)f->phi, (E4<double> *)f->e,
+                       f->nx, f->ny, f->nz, f->d_dom);
+  else
+    hipLaunchKernelGGL(k_efield<float>, dim3(grid_for(f->g)), dim3(kBlock), 0,
+                       f->stream, (const float *)f->phi, (E4<float> *)f->e, f->nx,
+                       f->ny, f->nz, f->d_dom);
   IZH_HIP(hipGetLastError());
   return IZH_OK;
 }

@@ -1,0 +1,655 @@
+// izh_fused.hip -- a run of space-charge steps as one device-resident pipeline.
+//
+// Per step s the particles make ONE streaming pass (k_pass):
+//   [step s-1] 0th-order t-frame position, 8-node gather of E(s-1), kick,
+//              back transform, second half push, diagnostic moment sums;
+//   [step s]   first half push, phase wrap, 1st-order z->t transform, range,
+//              loss test, tile key of the new position.
+// The pass writes each particle to its slot in the other ping-pong buffer,
+// grouped by the tile key computed one step earlier (a counting sort whose
+// histogram and scan were done by the previous k_finalize): neighbours in
+// memory stay neighbours in space, so the gather hits cache and the deposit
+// (izh_field.hip, k_deposit_chunks) accumulates each chunk in LDS.
+// k_finalize (one block) then reduces the pass's partials into the step's
+// moments, the new domain (update_CompDom), particle count, Current rescale
+// and kick constants, scans the new tile histogram into scatter cursors and
+// the next deposit's chunk list.  The host only launches kernels.
+//
+// Reference call order per segment (src/Contrl/AccSimulator.f90:1099-1488):
+// drift1 -> conv1st -> update_CompDom -> charge -> update3O -> cvbkforth1st ->
+// kick1 -> drift1 -> diagnostic1.  cvbkforth1st(conv1st(P)) is the 0th-order
+// transform of the wrapped z-frame state, so the pass gathers there directly.
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "izh_device.hpp"
+
+namespace izh {
+namespace {
+
+struct PassCfg {
+  const double *in[6];
+  double *out[6];
+  const uint32_t *in_orig;
+  uint32_t *out_orig;
+  const uint32_t *in_key;
+  uint32_t *out_key;
+  void *tpos;               // t-frame positions (P), 3 blocks of cap
+  int64_t cap;
+  const int64_t *d_nslots;  // input slots (device)
+  uint32_t *cursor;         // scatter cursor per tile of the input keys
+  uint32_t *hist_new;       // histogram of the output keys
+  const DevDom *dom;        // domain of step s-1: gather and key geometry
+  const void *e;            // E4<T> of step s-1
+  const DevScal *scal;      // current and kick constants of step s-1
+  DevScal *err;             // the same record, for the slot-overflow flag
+  int nx, ny, nz;
+  int tshift[3], tnx, tny;
+  double gam, xl, rad;      // -refpt(6), Scxl, pipe radius of step s
+  double tau2_prev;         // 2*tau1 of step s-1
+  double inv_rho_scale;     // undo the grid-charge scaling in the kick
+  int do_kick, do_moments, do_conv;
+  Push pb, pa;              // second half of step s-1, first half of step s
+  double *part;             // per block: kMoments moment values, kConvRed conv
+};
+
+// Slot of each participating lane among the lanes that share its key, with
+// one atomic per distinct key in the wavefront.
+__device__ inline uint32_t wave_rank_add(uint32_t *counters, uint32_t key,
+                                         bool take) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(take);
+  uint32_t slot = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lk = __shfl(key, leader, 64);
+    const unsigned long long same = __ballot(take && key == lk) & pending;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&counters[lk], (uint32_t)__popcll(same));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  return slot;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
+  const int64_t nslots = *c.d_nslots;
+  const double xk = 1.0 / c.xl;
+  const double xmin = c.dom->range[0], ymin = c.dom->range[2], zmin = c.dom->range[4];
+  const double hx = c.dom->msize[0], hy = c.dom->msize[1], hz = c.dom->msize[2];
+  const double hxi = c.dom->hinv[0], hyi = c.dom->hinv[1], hzi = c.dom->hinv[2];
+  const bool kick = c.do_kick && c.scal->current >= (double)1.0e-20f;
+  const double txy = c.tau2_prev * c.scal->xycon * c.inv_rho_scale;
+  const double ttc = c.tau2_prev * c.scal->tcon * c.inv_rho_scale;
+  const int64_t sy = c.nx, sz = (int64_t)c.nx * c.ny;
+  const E4<T> *E = (const E4<T> *)c.e;
+  T *tpx = (T *)c.tpos;
+  Mom mom;
+  mom_zero(mom);
+  double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
+  double nlost = 0.0, nalive = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nslots; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t kin = i < nslots ? c.in_key[i] : kDeadKey;
+    const bool valid = kin != kDeadKey;
+    double x = 0, px = 0, y = 0, py = 0, ph = 0, pt = 0;
+    uint32_t og = 0;
+    if (valid) {
+      x = c.in[0][i]; px = c.in[1][i]; y = c.in[2][i];
+      py = c.in[3][i]; ph = c.in[4][i]; pt = c.in[5][i];
+      og = c.in_orig[i];
+      nalive += 1.0;
+    }
+    if (valid && c.do_kick) {
+      if (kick) {
+        // cvbkforth1st + scatter1 (src/Appl/BeamBunch.f90:1667-1701,2041-2181)
+        double rcp = 1.0 / (-pt + c.gam);
+        double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
+        double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam * betai * ph * c.xl;
+        int ix = (int)((X0 - xmin) * hxi + 1.0);
+        double ab = ((xmin - X0) + ix * hx) * hxi;
+        int jx = (int)((Y0 - ymin) * hyi + 1.0);
+        double cd = ((ymin - Y0) + jx * hy) * hyi;
+        int kx = (int)((Z0 - zmin) * hzi + 1.0);
+        double ef = ((zmin - Z0) + kx * hz) * hzi;
+        ix = min(max(ix, 1), c.nx - 1);
+        jx = min(max(jx, 1), c.ny - 1);
+        kx = min(max(kx, 1), c.nz - 1);
+        int64_t o = (ix - 1) + sy * (jx - 1) + sz * (kx - 1);
+        T gx, gy, gz;
+        cic8_e4<T>(E, o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
+        double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
+        x = X0 * xk;
+        y = Y0 * xk;
+        ph = Z0 * xk / (-c.gam * betai);
+        px = px + txy * exn;
+        py = py + txy * eyn;
+        pt = pt - ttc * ezn;
+      }
+      apply_push(c.pb, x, px, y, py, ph, pt);
+      if (c.do_moments) mom_add(mom, x, px, y, py, ph, pt);
+    }
+    uint32_t knew = valid ? 0u : kDeadKey;
+    T tx = (T)NAN, ty = (T)NAN, tz = (T)NAN;
+    if (valid && c.do_conv) {
+      apply_push(c.pa, x, px, y, py, ph, pt);
+      // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
+      ph = wrap_phase(ph);
+      double rcp = 1.0 / (-pt + c.gam);
+      double tmpbet = 1.0 - rcp * rcp * (1.0 + px * px + py * py);
+      double betai = sqrt(fabs(tmpbet));
+      double tmpx = x * c.xl, xt = (x - px * ph * rcp) * c.xl;
+      double tmpy = y * c.xl, yt = (y - py * ph * rcp) * c.xl;
+      double zt = -c.gam * betai * ph * c.xl;
+      mn[0] = fmin(mn[0], fmin(tmpx, xt));
+      mxv[0] = fmax(mxv[0], fmax(tmpx, xt));
+      mn[1] = fmin(mn[1], fmin(tmpy, yt));
+      mxv[1] = fmax(mxv[1], fmax(tmpy, yt));
+      mn[2] = fmin(mn[2], zt);
+      mxv[2] = fmax(mxv[2], zt);
+      if (lost_t1(xt, yt, px, py, pt, c.gam, c.rad)) {
+        nlost += 1.0;
+        knew = kDeadKey;
+      } else {
+        int ci = min(max((int)((xt - xmin) * hxi), 0), c.nx - 1);
+        int cj = min(max((int)((yt - ymin) * hyi), 0), c.ny - 1);
+        int ck = min(max((int)((zt - zmin) * hzi), 0), c.nz - 1);
+        knew = (uint32_t)((ci >> c.tshift[0]) +
+                          c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
+        tx = (T)xt;
+        ty = (T)yt;
+        tz = (T)zt;
+      }
+    }
+    // counting-sort scatter by the key of the previous step, then the
+    // histogram of the new keys for the next one
+    const uint32_t slot = wave_rank_add(c.cursor, kin, valid);
+    if (c.do_conv) (void)wave_rank_add(c.hist_new, knew, knew != kDeadKey);
+    if (valid && slot >= (uint32_t)c.cap) {
+      atomicAdd((unsigned long long *)&c.err->err, 1ull);
+    } else if (valid) {
+      c.out[0][slot] = x; c.out[1][slot] = px; c.out[2][slot] = y;
+      c.out[3][slot] = py; c.out[4][slot] = ph; c.out[5][slot] = pt;
+      c.out_orig[slot] = og;
+      c.out_key[slot] = knew;
+      if (c.do_conv) {
+        tpx[slot] = tx;
+        tpx[c.cap + slot] = ty;
+        tpx[2 * c.cap + slot] = tz;
+      }
+    }
+  }
+  __shared__ double sh[kBlock / 64][kMoments];
+  double *out = c.part + (int64_t)blockIdx.x * kPartStride;
+  mom_block_store(mom, sh, out);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double v[kConvRed];
+  for (int k = 0; k < 3; k++) { v[k] = wave_min(mn[k]); v[3 + k] = wave_max(mxv[k]); }
+  v[6] = wave_sum(nlost);
+  v[7] = wave_sum(nalive);
+  if (lane == 0)
+    for (int k = 0; k < kConvRed; k++) sh[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < kConvRed) {
+    int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kBlock / 64; w++)
+      r = k < 3 ? fmin(r, sh[w][k]) : (k < 6 ? fmax(r, sh[w][k]) : r + sh[w][k]);
+    out[kMoments + k] = r;
+  }
+}
+
+// ------------------------------------------------------------ prologue
+// Initial tile keys: the 0th-order t-frame position of the uploaded z-frame
+// particles, binned on a domain that bounds them.
+__global__ void __launch_bounds__(kBlock)
+k_prologue_range(const double *__restrict__ X, const double *__restrict__ PX,
+                 const double *__restrict__ Y, const double *__restrict__ PY,
+                 const double *__restrict__ PH, const double *__restrict__ PT,
+                 int64_t n, double gam, double xl, double *__restrict__ part) {
+  double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double px = PX[i], py = PY[i];
+    double rcp = 1.0 / (-PT[i] + gam);
+    double betai = sqrt(fabs(1.0 - rcp * rcp * (1.0 + px * px + py * py)));
+    double p[3] = {X[i] * xl, Y[i] * xl, -gam * betai * wrap_phase(PH[i]) * xl};
+    for (int a = 0; a < 3; a++) { mn[a] = fmin(mn[a], p[a]); mx[a] = fmax(mx[a], p[a]); }
+  }
+  __shared__ double sh[kBlock / 64][6];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double v[6];
+  for (int a = 0; a < 3; a++) { v[a] = wave_min(mn[a]); v[3 + a] = wave_max(mx[a]); }
+  if (lane == 0)
+    for (int k = 0; k < 6; k++) sh[wv][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < kBlock / 64; w++) r = k < 3 ? fmin(r, sh[w][k]) : fmax(r, sh[w][k]);
+    part[blockIdx.x * 6 + k] = r;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_prologue_keys(const double *__restrict__ X, const double *__restrict__ PX,
+                const double *__restrict__ Y, const double *__restrict__ PY,
+                const double *__restrict__ PH, const double *__restrict__ PT,
+                int64_t n, double gam, double xl, const double *__restrict__ box,
+                int nx, int ny, int nz, int s0, int s1, int s2, int tnx, int tny,
+                uint32_t *__restrict__ key, uint32_t *__restrict__ hist) {
+  // box = reduced (min x, min y, min z, max x, max y, max z)
+  const double h[3] = {(box[3] - box[0]) / nx, (box[4] - box[1]) / ny,
+                       (box[5] - box[2]) / nz};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    uint32_t k = kDeadKey;
+    if (i < n) {
+      double px = PX[i], py = PY[i];
+      double rcp = 1.0 / (-PT[i] + gam);
+      double betai = sqrt(fabs(1.0 - rcp * rcp * (1.0 + px * px + py * py)));
+      double p[3] = {X[i] * xl, Y[i] * xl, -gam * betai * wrap_phase(PH[i]) * xl};
+      int ci[3];
+      const int nn[3] = {nx, ny, nz};
+      for (int a = 0; a < 3; a++) {
+        double u = h[a] > 0 ? (p[a] - box[a]) / h[a] : 0.0;
+        ci[a] = min(max((int)u, 0), nn[a] - 1);
+      }
+      k = (uint32_t)((ci[0] >> s0) + tnx * ((ci[1] >> s1) + tny * (ci[2] >> s2)));
+      key[i] = k;
+    }
+    (void)wave_rank_add(hist, k, k != kDeadKey);
+  }
+}
+
+// Provisional domain of the initial keys: the padded box of the bunch
+// (update_CompDom's padding, src/Appl/CompDom.f90:547-557,614-622).
+__global__ void k_prologue_dom(const double *__restrict__ box, DevDom *__restrict__ d,
+                               int nx, int ny, int nz) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nn[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; a++) {
+    const double eps = (double)(1.0f / ((float)nn[a] - 3.0f));
+    const double lo = box[a], hi = box[3 + a];
+    d->range[2 * a] = lo - eps * (hi - lo);
+    d->range[2 * a + 1] = hi + eps * (hi - lo);
+    d->msize[a] = (d->range[2 * a + 1] - d->range[2 * a]) / (nn[a] - 1);
+    d->hinv[a] = 1.0 / d->msize[a];
+    d->ptrange[2 * a] = lo;
+    d->ptrange[2 * a + 1] = hi;
+  }
+}
+
+// ------------------------------------------------------------ finalize
+struct FinCfg {
+  const double *part;
+  int nparts;
+  int mode;                 // 0 prologue, 1 step, 2 final
+  DevScal *scal;
+  DevDom *dom;
+  double *moments_out;      // kMoments doubles or null
+  const uint32_t *hist;     // histogram of the keys just written
+  uint32_t *cursor;         // -> exclusive offsets of hist
+  Chunk *chunks;            // -> deposit chunks of that grouping
+  int64_t *nchunks;         // -> their number
+  uint32_t *hist_zero;      // histogram the next pass fills
+  int ntiles;
+  int nx, ny, nz;
+  double gam, scfreq, clight, pi, scxl, mass, charge;
+  const double *prologue_box;
+};
+
+__global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
+  __shared__ double red[kPartStride];
+  __shared__ double sh[1024];
+  __shared__ uint32_t ssum[1024];
+  __shared__ uint32_t csum[1024];
+  const int tid = threadIdx.x;
+  // ---- reduce the pass partials (fixed tree order)
+  if (f.mode != 0) {
+    for (int q = 0; q < kPartStride; q++) {
+      const bool isconv = q >= kMoments;
+      const int cq = q - kMoments;
+      const int op = !isconv ? (q < 37 ? 0 : 2) : (cq < 3 ? 1 : (cq < 6 ? 2 : 0));
+      double acc = op == 1 ? 1e300 : (op == 2 ? -1e300 : 0.0);
+      for (int b = tid; b < f.nparts; b += blockDim.x) {
+        double v = f.part[(int64_t)b * kPartStride + q];
+        acc = op == 1 ? fmin(acc, v) : (op == 2 ? fmax(acc, v) : acc + v);
+      }
+      sh[tid] = acc;
+      __syncthreads();
+      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+          double a = sh[tid], b2 = sh[tid + s];
+          sh[tid] = op == 1 ? fmin(a, b2) : (op == 2 ? fmax(a, b2) : a + b2);
+        }
+        __syncthreads();
+      }
+      if (tid == 0) red[q] = sh[0];
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    DevScal &S = *f.scal;
+    if (f.mode != 0 && f.moments_out) {
+      for (int k = 0; k < kMoments; k++) f.moments_out[k] = red[k];
+      f.moments_out[kMoments] = red[kMoments + 7];  // Npt of those moments
+    }
+    if (f.mode == 1) {
+      const double *cv = red + kMoments;
+      const int64_t alive_in = (int64_t)cv[7];
+      const int64_t nnew = alive_in - (int64_t)cv[6];
+      S.nslots = alive_in;
+      S.nalive = nnew;
+      // conv1st: Current = Current*nptot/Npt (src/Appl/BeamBunch.f90:1657-1663)
+      if (S.npt > 0) S.current = S.current * (double)nnew / S.npt;
+      S.npt = (double)nnew;
+      if (nnew > 0) {
+        // update_CompDom, Flagbc=1 (src/Appl/CompDom.f90:547-557,614-622)
+        const double pr[6] = {cv[0], cv[3], cv[1], cv[4], cv[2], cv[5]};
+        const int nn[3] = {f.nx, f.ny, f.nz};
+        DevDom &D = *f.dom;
+        for (int a = 0; a < 3; a++) {
+          const double eps = (double)(1.0f / ((float)nn[a] - 3.0f));
+          const double lo = pr[2 * a], hi = pr[2 * a + 1];
+          D.range[2 * a] = lo - eps * (hi - lo);
+          D.range[2 * a + 1] = hi + eps * (hi - lo);
+          D.msize[a] = (D.range[2 * a + 1] - D.range[2 * a]) / (nn[a] - 1);
+          D.hinv[a] = 1.0 / D.msize[a];
+          D.ptrange[2 * a] = lo;
+          D.ptrange[2 * a + 1] = hi;
+        }
+      }
+      // scatter1 constants (src/Appl/BeamBunch.f90:2065-2084)
+      const double gam = f.gam, curr = S.current;
+      const double tmpscale = curr / f.scfreq;
+      const double twopi = 2.0 * f.pi;
+      const double fpei = f.clight * f.clight * (double)1.0e-7f;
+      const double xk = 1.0 / f.scxl;
+      const double bet = sqrt(gam * gam - 1.0) / gam;
+      const double bbyk = bet / xk;
+      const double gambet = gam * bet;
+      const double brho = gambet / (f.clight * f.charge) * f.mass;
+      const double vz0 = bet * f.clight;
+      const double perv0 = 2.0 * curr * fpei / (brho * vz0 * vz0 * gam * gam);
+      S.xycon = curr > 0 ? 0.5 * perv0 * gambet * bbyk * twopi / tmpscale : 0.0;
+      S.tcon = bet * S.xycon * gam * gam;
+    } else if (f.mode == 2) {
+      const int64_t alive_in = (int64_t)red[kMoments + 7];
+      S.nslots = alive_in;
+      S.nalive = alive_in;
+    } else {
+      // prologue: the input slots are the uploaded particles
+      S.nslots = S.nalive;
+    }
+  }
+  __syncthreads();
+  if (f.mode == 2) return;
+  // ---- exclusive scan of the tile histogram and the chunk list
+  const int per = (f.ntiles + blockDim.x - 1) / blockDim.x;
+  const int t0 = tid * per, t1 = min(t0 + per, f.ntiles);
+  uint32_t cnt = 0, nch = 0;
+  for (int t = t0; t < t1; t++) {
+    uint32_t h = f.hist[t];
+    cnt += h;
+    nch += (h + kChunk - 1) / kChunk;
+  }
+  ssum[tid] = cnt;
+  csum[tid] = nch;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    uint32_t a = tid >= off ? ssum[tid - off] : 0u;
+    uint32_t b2 = tid >= off ? csum[tid - off] : 0u;
+    __syncthreads();
+    ssum[tid] += a;
+    csum[tid] += b2;
+    __syncthreads();
+  }
+  uint32_t off = ssum[tid] - cnt, coff = csum[tid] - nch;
+  for (int t = t0; t < t1; t++) {
+    uint32_t h = f.hist[t];
+    f.cursor[t] = off;
+    for (uint32_t s = 0; s < h; s += kChunk) {
+      f.chunks[coff].start = off + s;
+      f.chunks[coff].len = min((uint32_t)kChunk, h - s);
+      coff++;
+    }
+    off += h;
+  }
+  if (tid == blockDim.x - 1) *f.nchunks = csum[tid];
+  for (int t = tid; t < f.ntiles; t += blockDim.x) f.hist_zero[t] = 0u;
+}
+
+__global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+int tile_shift(int n) {
+  int s = 0;
+  while ((1 << (s + 1)) <= n && s < kTileShift) s++;
+  return s;
+}
+
+}  // namespace
+
+int bunch_alloc_fused(izh_bunch *b, int ntiles, int64_t nseg) {
+  const size_t ps = b->precision == IZH_FP64 ? sizeof(double) : sizeof(float);
+  if (!b->key) {
+    IZH_HIP(hipMalloc((void **)&b->key, (size_t)b->cap * sizeof(uint32_t)));
+    IZH_HIP(hipMalloc((void **)&b->key_alt, (size_t)b->cap * sizeof(uint32_t)));
+    IZH_HIP(hipMalloc(&b->tpos, (size_t)3 * b->cap * ps));
+    IZH_HIP(hipMalloc((void **)&b->d_scal, sizeof(DevScal)));
+  }
+  if (!b->r_alt[0]) IZH_CHECK(bunch_alloc_alt(b));
+  if (!b->h_scal)
+    IZH_HIP(hipHostMalloc((void **)&b->h_scal, sizeof(DevScal), hipHostMallocDefault));
+  const int64_t nch = b->cap / kChunk + ntiles + 1;
+  if (ntiles > b->ntiles_cap || nch > b->nchunks_cap) {
+    for (int k = 0; k < 2; k++)
+      if (b->hist[k]) (void)hipFree(b->hist[k]);
+    if (b->cursor) (void)hipFree(b->cursor);
+    if (b->chunks) (void)hipFree(b->chunks);
+    for (int k = 0; k < 2; k++)
+      IZH_HIP(hipMalloc((void **)&b->hist[k], (size_t)ntiles * sizeof(uint32_t)));
+    IZH_HIP(hipMalloc((void **)&b->cursor, (size_t)ntiles * sizeof(uint32_t)));
+    IZH_HIP(hipMalloc((void **)&b->chunks, (size_t)2 * nch * sizeof(Chunk)));
+    b->ntiles_cap = ntiles;
+    b->nchunks_cap = nch;
+  }
+  if (nseg > b->moments_cap) {
+    if (b->d_moments) (void)hipFree(b->d_moments);
+    IZH_HIP(hipMalloc((void **)&b->d_moments, (size_t)nseg * kRunRecord * sizeof(double)));
+    b->moments_cap = nseg;
+  }
+  return IZH_OK;
+}
+
+// Host-side descriptors of a segment's half-step push; refpt and z follow the
+// reference's bookkeeping (drift1_BeamBunch, maplinear_Quadrupole).
+static int segment_push(izh_bunch *b, const izh_segment &sg, Push &p, double *z) {
+  p = Push{};
+  if (sg.bitype == 0) {
+    const double gam0 = -b->refpt[5];
+    const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
+    p.kind = 1;
+    p.tau = sg.tau1;
+    p.gam0 = gam0;
+    p.ibeta0 = 1.0 / beta0;
+    p.scxl = b->c.scxl;
+    b->refpt[4] = b->refpt[4] + sg.tau1 / (b->c.scxl * beta0);
+  } else if (sg.bitype == 1) {
+    p.kind = 2;
+    IZH_CHECK(izh_quad_maplinear(&b->c, sg.tau1, sg.gradient, sg.mapstp, b->refpt,
+                                 b->charge, b->mass, p.xm));
+  } else {
+    return fail(IZH_ERR_ARG, "izh_run: bitype %d not supported", sg.bitype);
+  }
+  *z = *z + sg.tau1;
+  return IZH_OK;
+}
+
+int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+              const izh_segment *segs, int64_t nseg, double *moments) {
+  const int nx = f->nx, ny = f->ny, nz = f->nz;
+  const int s0 = tile_shift(nx), s1 = tile_shift(ny), s2 = tile_shift(nz);
+  const int tnx = nx >> s0, tny = ny >> s1, tnz = nz >> s2;
+  const int ntiles = tnx * tny * tnz;
+  IZH_CHECK(bunch_alloc_fused(b, ntiles, nseg));
+  hipStream_t st = b->stream;
+  // everything runs on the bunch's stream, after work queued on the field's
+  {
+    hipEvent_t ev;
+    IZH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    IZH_HIP(hipEventRecord(ev, f->stream));
+    IZH_HIP(hipStreamWaitEvent(st, ev, 0));
+    IZH_HIP(hipEventDestroy(ev));
+  }
+  const hipStream_t fstream = f->stream;
+  f->stream = st;
+  const double gam = -b->refpt[5];
+  const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
+  const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
+  const int nb = grid_for(b->n);
+  int rc = IZH_OK;
+  // ---- device scalars
+  DevScal hs{};
+  hs.current = b->current;
+  hs.npt = (double)b->npt;
+  hs.nalive = b->n;
+  hs.nslots = b->n;
+  *b->h_scal = hs;
+  IZH_HIP(hipMemcpyAsync(b->d_scal, b->h_scal, sizeof hs, hipMemcpyHostToDevice, st));
+  // ---- prologue: initial keys, their histogram, cursors and chunks
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st,
+                     b->hist[0], (int64_t)ntiles);
+  hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                     b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                     b->c.scxl, b->d_part);
+  IZH_CHECK(launch_reduce(st, b->d_part, nb, 6, 0x7ull, 0x38ull, b->d_out));
+  hipLaunchKernelGGL(k_prologue_keys, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                     b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                     b->c.scxl, b->d_out, nx, ny, nz, s0, s1, s2, tnx, tny,
+                     b->key, b->hist[0]);
+  hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
+                     nx, ny, nz);
+  IZH_HIP(hipGetLastError());
+  FinCfg fc{};
+  fc.part = b->d_part;
+  fc.nparts = nb;
+  fc.scal = b->d_scal;
+  fc.dom = f->d_dom;
+  fc.ntiles = ntiles;
+  fc.nx = nx; fc.ny = ny; fc.nz = nz;
+  fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
+  fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
+  fc.mode = 0;
+  fc.hist = b->hist[0];
+  fc.cursor = b->cursor;
+  fc.chunks = b->chunks;
+  fc.nchunks = &b->d_scal->nchunks[0];
+  fc.hist_zero = b->hist[1];
+  {
+    ScopedTimer tm_(IZH_T_FINAL, st);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+  }
+  IZH_HIP(hipGetLastError());
+  // ---- the steps: pass s finishes step s-1 and starts step s
+  PassCfg pc{};
+  pc.cap = b->cap;
+  pc.d_nslots = &b->d_scal->nslots;
+  pc.cursor = b->cursor;
+  pc.dom = f->d_dom;
+  pc.e = f->e;
+  pc.scal = b->d_scal;
+  pc.err = b->d_scal;
+  pc.nx = nx; pc.ny = ny; pc.nz = nz;
+  pc.tshift[0] = s0; pc.tshift[1] = s1; pc.tshift[2] = s2;
+  pc.tnx = tnx; pc.tny = tny;
+  pc.gam = gam;
+  pc.xl = b->c.scxl;
+  pc.inv_rho_scale = 1.0 / rho_scale;
+  pc.tpos = b->tpos;
+  pc.part = b->d_part;
+  Push pb_prev{};
+  double tau2_prev = 0.0;
+  for (int64_t s = 0; s <= nseg && rc == IZH_OK; s++) {
+    const bool last = s == nseg;
+    Push pa{};
+    if (!last) IZH_CHECK(segment_push(b, segs[s], pa, z));  // first half of step s
+    for (int r = 0; r < 6; r++) { pc.in[r] = b->r[r]; pc.out[r] = b->r_alt[r]; }
+    pc.in_orig = b->orig;
+    pc.out_orig = b->orig_alt;
+    pc.in_key = b->key;
+    pc.out_key = b->key_alt;
+    pc.hist_new = b->hist[(s + 1) & 1];
+    pc.do_kick = s > 0;
+    pc.do_moments = s > 0;
+    pc.do_conv = !last;
+    pc.pb = pb_prev;
+    pc.pa = pa;
+    pc.tau2_prev = tau2_prev;
+    pc.rad = last ? 0.0 : 0.5 * (segs[s].xrad + segs[s].yrad);
+    {
+      ScopedTimer tm_(IZH_T_PASS, st);
+      if (f->precision == IZH_FP64)
+        hipLaunchKernelGGL((k_pass<double>), dim3(nb), dim3(kBlock), 0, st, pc);
+      else
+        hipLaunchKernelGGL((k_pass<float>), dim3(nb), dim3(kBlock), 0, st, pc);
+    }
+    IZH_HIP(hipGetLastError());
+    for (int r = 0; r < 6; r++) std::swap(b->r[r], b->r_alt[r]);
+    std::swap(b->orig, b->orig_alt);
+    std::swap(b->key, b->key_alt);
+    fc.mode = last ? 2 : 1;
+    fc.moments_out = s > 0 ? b->d_moments + (s - 1) * kRunRecord : nullptr;
+    fc.hist = b->hist[(s + 1) & 1];
+    fc.chunks = b->chunks + ((s + 1) & 1) * b->nchunks_cap;
+    fc.nchunks = &b->d_scal->nchunks[(s + 1) & 1];
+    fc.hist_zero = b->hist[s & 1];
+    {
+      ScopedTimer tm_(IZH_T_FINAL, st);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+    }
+    IZH_HIP(hipGetLastError());
+    if (last) break;
+    // charge, update3O and the E field of step s on the grouping just written
+    rc = launch_deposit(b, f, b->chunks + (s & 1) * b->nchunks_cap,
+                        &b->d_scal->nchunks[s & 1], b->nchunks_cap, 1);
+    if (rc == IZH_OK) rc = launch_solve(f);
+    if (rc == IZH_OK) rc = launch_efield(f);
+    // second half of step s: host bookkeeping now, applied by the next pass
+    if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, z);
+    tau2_prev = 2.0 * segs[s].tau1;
+  }
+  f->stream = fstream;
+  IZH_CHECK(rc);
+  // ---- results back to the host
+  IZH_HIP(hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(DevScal), hipMemcpyDeviceToHost, st));
+  IZH_HIP(hipMemcpyAsync(f->h_dom, f->d_dom, sizeof(DevDom), hipMemcpyDeviceToHost, st));
+  if (moments && nseg > 0)
+    IZH_HIP(hipMemcpyAsync(moments, b->d_moments, nseg * kRunRecord * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+  IZH_HIP(hipStreamSynchronize(st));
+  const DevScal out = *b->h_scal;
+  if (out.err != 0)
+    return fail(IZH_ERR_STATE, "izh_run: %lld particles had no sort slot",
+                (long long)out.err);
+  b->n = out.nslots;
+  b->npt = (int64_t)out.npt;
+  b->current = out.current;
+  for (int k = 0; k < 6; k++) dom->range[k] = f->h_dom->range[k];
+  for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
+  f->have_phi = 1;
+  return IZH_OK;
+}
+
+}  // namespace izh

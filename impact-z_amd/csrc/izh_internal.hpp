@@ -34,6 +34,13 @@ int fail(int code, const char *fmt, ...);
 constexpr int kBlock = 256;          // threads per block for particle passes
 constexpr int kMaxRedBlocks = 2048;  // fixed partial count => deterministic sums
 constexpr int kMoments = 44;         // 37 sums + 6 abs-max + r^2 max
+constexpr int kRunRecord = kMoments + 1;  // + Npt, per step of izh_run
+constexpr int kConvRed = 8;          // 3 min, 3 max, lost, alive
+constexpr int kPartStride = kMoments + kConvRed;
+constexpr int kChunk = 4096;         // particles per deposit chunk
+constexpr int kWindowCells = 4096;   // LDS deposit window (int64 cells)
+constexpr int kTileShift = 3;        // sort tiles of 8x8x8 cells
+constexpr uint32_t kDeadKey = 0xffffffffu;
 
 // Frame of the particle coordinates held by a bunch; the reference's call
 // order moves the bunch through these (src/Contrl/AccSimulator.f90:1110-1429).
@@ -43,29 +50,73 @@ enum Frame : int {
   kFrameT0 = 2,    // after cvbkforth1st: 0th-order t frame, metres
 };
 
+// Computational domain as the kernels read it (one per field, in device
+// memory, so a run of steps can update it without the host).
+struct DevDom {
+  double range[6];    // xmin, xmax, ymin, ymax, zmin, zmax (SpatRange)
+  double msize[3];    // hx, hy, hz (Meshsize)
+  double hinv[3];     // 1/h
+  double ptrange[6];  // the conv1st range it was built from
+};
+
+// Bunch scalars a run of steps keeps on the device.
+struct DevScal {
+  double current;     // BeamBunch%Current
+  double npt;         // BeamBunch%Npt (as a double for the rescale)
+  double xycon, tcon; // scatter1 constants of the current step
+  int64_t nalive;     // particles alive after the last conv1st
+  int64_t nslots;     // slots written by the last pass (alive + just lost)
+  int64_t nchunks[2]; // deposit chunks of the two groupings in flight
+  int64_t overflow;   // deposit chunks that fell back to global atomics
+  int64_t err;        // scatter slots out of range (must stay 0)
+};
+
+struct Chunk {
+  uint32_t start, len;
+};
+
 }  // namespace izh
 
-// Device-resident bunch in structure-of-arrays form.
+// Device-resident bunch in structure-of-arrays form.  The six phase-space
+// rows and the particle's original index move with the particle (sorting,
+// compaction); q/m, charge weight and id are immutable per particle and are
+// read through the original index (or held as one value when uniform).
 struct izh_bunch {
   int device = 0;
   int precision = IZH_FP64;
   hipStream_t stream = nullptr;
   int64_t cap = 0;       // allocated rows
-  int64_t n = 0;         // Nptlocal
+  int64_t n0 = 0;        // particles at upload (size of the immutable rows)
+  int64_t n = 0;         // Nptlocal: live rows in r[]
   int64_t npt = 0;       // Npt (one process: == n after conv1st)
   double current = 0, mass = 0, charge = 0, freq = 0;
   double refpt[6] = {0, 0, 0, 0, 0, 0};
   izh_const c{};
   int frame = izh::kFrameZ;
   double wabs_sum = 0;   // sum |Pts1(8,:)|, bounds every cell of the grid charge
-  // SoA state, Pts1 rows 1..9
-  double *s[9] = {nullptr};
-  double *alt[9] = {nullptr};   // compaction target, swapped with s
+  // moving rows: x, px, y, py, phi, pt and the original index
+  double *r[6] = {nullptr};
+  double *r_alt[6] = {nullptr};
+  uint32_t *orig = nullptr, *orig_alt = nullptr;
+  uint32_t *key = nullptr, *key_alt = nullptr;   // sort keys of the fused run
+  void *tpos = nullptr;  // t-frame positions for the deposit: 3*cap of T
+  // immutable rows, indexed by the original index
+  double *qm0 = nullptr, *w0 = nullptr, *id0 = nullptr;
+  int w_uniform = 0;
+  double w_value = 0;
   // reduction scratch
-  double *d_part = nullptr;     // kMaxRedBlocks * kMoments
-  int64_t *d_cnt = nullptr;     // kMaxRedBlocks + 1 (kept counts, then offsets)
-  double *d_out = nullptr;      // final reductions (kMoments + 8)
+  double *d_part = nullptr;     // kMaxRedBlocks * kPartStride
+  double *d_out = nullptr;      // final reductions (kPartStride + 8)
   double *h_out = nullptr;      // pinned mirror of d_out
+  // fused-run scratch
+  izh::DevScal *d_scal = nullptr;
+  izh::DevScal *h_scal = nullptr;          // pinned mirror
+  uint32_t *hist[2] = {nullptr, nullptr};   // tile histograms (ping-pong)
+  uint32_t *cursor = nullptr;               // scatter cursors per tile
+  izh::Chunk *chunks = nullptr;             // deposit work list
+  int64_t ntiles_cap = 0, nchunks_cap = 0;
+  double *d_moments = nullptr;              // per-step moments of a run
+  int64_t moments_cap = 0;
 };
 
 // Device-resident FieldQuant + solver workspace for one mesh.
@@ -78,19 +129,30 @@ struct izh_field {
   unsigned long long *rho_fx = nullptr;  // fixed-point deposit accumulator
   void *rho = nullptr;          // T[g]   charge density (T = double or float)
   void *phi = nullptr;          // T[g]   potential
-  void *e = nullptr;            // T[3g]  Ex, Ey, Ez
+  void *e = nullptr;            // T4[g]  (Ex, Ey, Ez, 0) interleaved
   void *spec_a = nullptr;       // complex T[(nx+1)*ny*nz]
   void *spec_b = nullptr;       // complex T[(nx+1)*2ny*nz]
   void *green = nullptr;        // T[(nx+1)*(ny+1)*(nz+1)] transformed Green fn
   void *green_t = nullptr;      // T[(nx+1)*(ny+1)*(nz+1)] work
   double *corner = nullptr;     // double[(nx+2)*(ny+2)*(nz+2)] IGF antiderivative
   void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;  // twiddles per length
+  izh::DevDom *d_dom = nullptr; // domain the kernels read
+  izh::DevDom *h_dom = nullptr; // pinned staging for host-set domains
+  izh::Chunk *chunks = nullptr; // uniform chunks for unsorted deposits
+  int64_t nchunks_cap = 0;
   double rho_scale = 1.0;       // grid value = physical charge * rho_scale
   int have_rho = 0, have_phi = 0;
   double fixed_scale = 0;       // 2^e used by the fixed-point deposit
 };
 
 namespace izh {
+
+// Element push of a half step, applied per particle inside the kernels.
+struct Push {
+  int kind;           // 0 none, 1 exact drift, 2 linear 6x6 map
+  double tau, gam0, ibeta0, scxl;
+  double xm[36];
+};
 
 // Launchers implemented in the kernel translation units.
 int launch_drift(izh_bunch *b, double tau, double gam0, double beta0);
@@ -100,11 +162,18 @@ int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
 int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep);
 int launch_cvbk(izh_bunch *b);
 int launch_moments(izh_bunch *b, double out[kMoments]);
-int launch_deposit(izh_bunch *b, const izh_compdom *dom, izh_field *f);
-int launch_solve(izh_field *f, const izh_compdom *dom);
-int launch_efield(izh_field *f, const izh_compdom *dom);
-int launch_gather_kick(izh_bunch *b, izh_field *f, const izh_compdom *dom,
-                       double tau, double xycon, double tcon);
+int field_set_domain(izh_field *f, const izh_compdom *dom);
+int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
+                   const int64_t *d_nchunks, int64_t max_chunks, int positions);
+int launch_deposit_granular(izh_bunch *b, izh_field *f);
+int launch_solve(izh_field *f);
+int launch_efield(izh_field *f);
+int launch_gather_kick(izh_bunch *b, izh_field *f, double tau, double xycon,
+                       double tcon);
+int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+              const izh_segment *segs, int64_t nseg, double *moments);
+double fixed_point_scale(double wabs_sum);
+int bunch_alloc_alt(izh_bunch *b);
 
 // Kernel timers (izh_timer_*): begin/end record events on the launch stream
 // when timing is enabled; both are no-ops otherwise.
@@ -126,9 +195,55 @@ inline int grid_for(int64_t n, int block = kBlock) {
   return (int)g;
 }
 
-inline int64_t blocks_for(int64_t n, int block = kBlock) {
-  int64_t g = (n + block - 1) / block;
-  return g < 1 ? 1 : g;
+// ------------------------------------------------------------ device math
+// REAL(4) pi of conv1st: pilc = 2.0*asin(1.0) (src/Appl/BeamBunch.f90:1268).
+__host__ __device__ inline double pilc_f32() { return (double)3.14159274101257324f; }
+
+// Phase wrap of conv1st (src/Appl/BeamBunch.f90:1296-1298).
+__device__ inline double wrap_phase(double p5) {
+  const double pilc = pilc_f32();
+  int ntmp5 = (int)(p5 / pilc);
+  double tmp5 = p5 - ntmp5 * pilc;
+  return tmp5 - (ntmp5 % 2) * pilc;
 }
+
+// Loss predicate of conv1st (src/Appl/BeamBunch.f90:1340-1343) on the
+// transformed coordinates.
+__device__ inline bool lost_t1(double xt, double yt, double px, double py,
+                               double pt, double gam, double rad) {
+  double rcp = 1.0 / (-pt + gam);
+  double tmpbet = 1.0 - rcp * rcp * (1.0 + px * px + py * py);
+  double radtest = sqrt(xt * xt + yt * yt);
+  return radtest >= rad || tmpbet <= 0.0;
+}
+
+// Half-step push of one particle (drift1_BeamBunch, src/Appl/BeamBunch.f90:
+// 276-320): exact drift or the 6x6 map product.
+__device__ inline void apply_push(const Push &p, double &x, double &px, double &y,
+                                  double &py, double &ph, double &pt) {
+  if (p.kind == 1) {
+    double d = pt - p.gam0;
+    double tmph = sqrt(d * d - 1.0 - px * px - py * py);
+    x = x + px * p.tau / tmph / p.scxl;
+    y = y + py * p.tau / tmph / p.scxl;
+    ph = ph - (p.ibeta0 + d / tmph) * p.tau / p.scxl;
+  } else if (p.kind == 2) {
+    const double v[6] = {x, px, y, py, ph, pt};
+    double t[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) s += v[k] * p.xm[6 * j + k];
+      t[j] = s;
+    }
+    x = t[0]; px = t[1]; y = t[2]; py = t[3]; ph = t[4]; pt = t[5];
+  }
+}
+
+template <typename T>
+struct alignas(sizeof(T) * 4) E4 {
+  T x, y, z, w;
+};
 
 }  // namespace izh

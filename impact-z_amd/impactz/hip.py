@@ -31,12 +31,13 @@ SYMBOLS = (
     "izh_drift1", "izh_map_linear", "izh_conv1st", "izh_update_compdom",
     "izh_field_create", "izh_field_destroy", "izh_charge", "izh_update3O",
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
-    "izh_field_upload_rho", "izh_step", "izh_timer_enable", "izh_timer_read",
+    "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
+    "izh_timer_read",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
 TIMERS = ("push", "conv1st", "deposit", "green", "fft", "efield", "kick",
-          "cvbk", "diag")
+          "cvbk", "diag", "pass", "final")
 
 
 class IZHError(RuntimeError):
@@ -53,6 +54,12 @@ class BunchInfo(C.Structure):
                 ("charge", C.c_double), ("npt", C.c_int64),
                 ("nptlocal", C.c_int64), ("refpt", C.c_double * 6),
                 ("freq", C.c_double), ("precision", C.c_int)]
+
+
+class Segment(C.Structure):
+    _fields_ = [("tau1", C.c_double), ("bitype", C.c_int),
+                ("gradient", C.c_double), ("mapstp", C.c_int),
+                ("xrad", C.c_double), ("yrad", C.c_double)]
 
 
 class CompDomC(C.Structure):
@@ -103,6 +110,8 @@ def lib() -> C.CDLL:
     L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.c_double,
                            C.c_int, C.c_double, C.c_int, C.c_double,
                            C.c_double, C.c_double, _dp]
+    L.izh_run.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
+                          C.c_int64, C.c_double, _dp]
     L.izh_timer_enable.argtypes = [C.c_int]
     L.izh_timer_read.argtypes = [_dp, _i64p, C.c_int]
     _lib = L
@@ -310,4 +319,21 @@ class BeamBunch:
         check(lib().izh_step(self.h, field.h, C.byref(dom.c), C.byref(zz), tau1,
                              bitype, gradient, mapstp, xrad, yrad, bcurr,
                              _p(mom) if want_moments else None))
+        return zz.value, mom
+
+    def run(self, field: FieldQuant, dom: CompDom, z: float, segments,
+            bcurr: float, want_moments: bool = True):
+        """izh_run: consecutive segments as one device-resident pipeline.
+
+        segments: iterable of (tau1, bitype, gradient, mapstp, xrad, yrad).
+        Returns (z, records[nseg, 45] or None): per step the 44 diagnostic
+        values of izh_step, then Npt."""
+        segs = list(segments)
+        arr = (Segment * max(len(segs), 1))()
+        for k, sg in enumerate(segs):
+            arr[k] = Segment(*sg)
+        zz = C.c_double(z)
+        mom = np.zeros((len(segs), 45)) if want_moments else None
+        check(lib().izh_run(self.h, field.h, C.byref(dom.c), C.byref(zz), arr,
+                            len(segs), bcurr, _p(mom) if want_moments else None))
         return zz.value, mom

@@ -103,14 +103,34 @@ class Simulation:
         self.z = b.drift1(self.z, tau1, e.type, grad, e.mapstp)
         self.nstep += 1
 
-    def diag(self, mom=None):
+    def segment_args(self, e: Element):
+        tau1 = 0.5 * e.length / e.nseg
+        return (tau1, e.type, e.v1 if e.type == 1 else 0.0, e.mapstp,
+                e.radius, e.radius)
+
+    def run_fused(self, nsteps: int, start: int = 0, want_moments: bool = True):
+        """Steps start..start+nsteps-1 of the lattice through izh_run.
+
+        Returns the per-step records (nsteps x 45: moments, then Npt) or None."""
+        segs = list(self.segments())[start:start + nsteps]
+        self.z, mom = self.bunch.run(self.field, self.dom, self.z,
+                                     [self.segment_args(e) for e in segs],
+                                     self.beam.current, want_moments)
+        self.nstep += len(segs)
+        return mom
+
+    def diag(self, mom=None, refpt=None):
+        """diagnostic1_Output row of the current state, or of a moments record
+        (44 values, or 45 with Npt as izh_run gives them)."""
         if mom is None:
             s, m, r2 = self.bunch.diagnostic1()
         else:
             s, m, r2 = mom[:37], mom[37:43], mom[43]
         info = self.bunch.info()
-        return diag_row(s, m, r2, info.npt, np.array(info.refpt[:]),
-                        info.mass, self.const.scxl, self.const.pi)
+        npt = int(mom[44]) if mom is not None and len(mom) > 44 else info.npt
+        rp = np.array(info.refpt[:]) if refpt is None else np.asarray(refpt)
+        return diag_row(s, m, r2, npt, rp, info.mass, self.const.scxl,
+                        self.const.pi)
 
     def run(self, nsteps: int | None = None, every: int = 1):
         """March through the lattice; returns the diagnostic rows (z first)."""

@@ -155,6 +155,30 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
              double tau1, int bitype, double gradient, int mapstp,
              double xrad, double yrad, double bcurr, double *moments);
 
+/* One segment of a drift/quadrupole lattice for izh_run. */
+typedef struct {
+  double tau1;        /* half step, L/(2*nseg) */
+  int bitype;         /* 0 drift, 1 constant-gradient quadrupole */
+  double gradient;    /* T/m, bitype 1 */
+  int mapstp;         /* RK6 steps of the quadrupole map */
+  double xrad, yrad;  /* pipe aperture of the loss test */
+} izh_segment;
+
+/* nseg consecutive izh_step calls (the element/segment loop of
+ * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504) run as one
+ * device-resident pipeline: each step's kick, second half push and moment
+ * sums are fused with the next step's first half push and z->t transform in
+ * one pass over the particles, which are re-sorted into 8x8x8-cell tiles in
+ * that same pass so the deposit accumulates in LDS; the domain update stays
+ * on the device, so the host does not wait between steps.  Results equal
+ * izh_step's to rounding.  moments (NULL or 45*nseg doubles) receives per
+ * step the 44 diagnostic1_Output values izh_step gives, then Npt.  After the call the
+ * bunch is in the z frame; its particle order is by tile (izh_bunch_download
+ * restores the reference's order). */
+int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+            const izh_segment *segs, int64_t nseg, double bcurr,
+            double *moments);
+
 /* Per-phase kernel timers, the device counterpart of the reference's Timer
  * accumulators (src/Func/Timer.f90:10-146; t_map1, t_rhofas, t_greenf,
  * t_fft2dhpf, t_field, t_force/t_ntrslo, t_diag).  When enabled, each timed
@@ -170,7 +194,9 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
 #define IZH_T_KICK 6       /* gather + kick + back transform (scatter1) */
 #define IZH_T_CVBK 7       /* cvbkforth1st */
 #define IZH_T_DIAG 8       /* diagnostic1 moment reductions */
-#define IZH_T_COUNT 9
+#define IZH_T_PASS 9       /* izh_run: fused kick/push/moments/z->t/sort pass */
+#define IZH_T_FINAL 10     /* izh_run: per-step reductions, domain, scans */
+#define IZH_T_COUNT 11
 int izh_timer_enable(int on);
 int izh_timer_read(double ms[IZH_T_COUNT], int64_t launches[IZH_T_COUNT],
                    int reset);
