@@ -823,4 +823,12 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   return run_steps(b, f, dom, z, segs, nseg, moments);
 }
 
+int izh_run_stats(const izh_bunch *b, int64_t stats[4]) {
+  if (!b || !stats) return fail(IZH_ERR_ARG, "izh_run_stats: bad argument");
+  for (int k = 0; k < 4; k++) stats[k] = b->run_stats[k];
+  return IZH_OK;
+}
+
+int izh_probe_copy(int64_t nbytes, double *ms) { return launch_probe_copy(nbytes, ms); }
+
 }  // extern "C"

@@ -646,6 +646,10 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   b->n = out.nslots;
   b->npt = (int64_t)out.npt;
   b->current = out.current;
+  b->run_stats[0] = out.nchunks[(nseg - 1) & 1];
+  b->run_stats[1] = out.overflow;
+  b->run_stats[2] = out.nalive;
+  b->run_stats[3] = ntiles;
   for (int k = 0; k < 6; k++) dom->range[k] = f->h_dom->range[k];
   for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
   f->have_phi = 1;

@@ -117,6 +117,7 @@ struct izh_bunch {
   int64_t ntiles_cap = 0, nchunks_cap = 0;
   double *d_moments = nullptr;              // per-step moments of a run
   int64_t moments_cap = 0;
+  int64_t run_stats[4] = {0, 0, 0, 0};      // izh_run_stats
 };
 
 // Device-resident FieldQuant + solver workspace for one mesh.
@@ -170,6 +171,7 @@ int launch_solve(izh_field *f);
 int launch_efield(izh_field *f);
 int launch_gather_kick(izh_bunch *b, izh_field *f, double tau, double xycon,
                        double tcon);
+int launch_probe_copy(int64_t nbytes, double *ms);
 int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
               const izh_segment *segs, int64_t nseg, double *moments);
 double fixed_point_scale(double wabs_sum);

@@ -275,9 +275,44 @@ k_moments(const double *__restrict__ X, const double *__restrict__ PX,
   mom_block_store(a, sh, part + (int64_t)blockIdx.x * kMoments);
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration: the reference
+// kernel for calibrating hardware byte counters.
+__global__ void __launch_bounds__(kBlock)
+k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
 
 // ================================================================ launchers
+
+int launch_probe_copy(int64_t nbytes, double *ms) {
+  const int64_t n = nbytes / 16;
+  if (n < 1) return fail(IZH_ERR_ARG, "izh_probe_copy: at least 16 bytes");
+  uint4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  IZH_HIP(hipMalloc((void **)&a, n * 16));
+  IZH_HIP(hipMalloc((void **)&b, n * 16));
+  IZH_HIP(hipMemset(a, 1, n * 16));
+  IZH_HIP(hipEventCreate(&e0));
+  IZH_HIP(hipEventCreate(&e1));
+  int64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  IZH_HIP(hipEventRecord(e0, 0));
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(kBlock), 0, 0, a, b, n);
+  IZH_HIP(hipEventRecord(e1, 0));
+  IZH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  IZH_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  IZH_HIP(hipEventDestroy(e0));
+  IZH_HIP(hipEventDestroy(e1));
+  IZH_HIP(hipFree(a));
+  IZH_HIP(hipFree(b));
+  return IZH_OK;
+}
 
 static int launch_push(izh_bunch *b, const Push &p) {
   if (b->n == 0) return IZH_OK;

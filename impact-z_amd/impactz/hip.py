@@ -32,7 +32,7 @@ SYMBOLS = (
     "izh_field_create", "izh_field_destroy", "izh_charge", "izh_update3O",
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
-    "izh_timer_read",
+    "izh_timer_read", "izh_run_stats", "izh_probe_copy",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
                            C.c_double, C.c_double, _dp]
     L.izh_run.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
                           C.c_int64, C.c_double, _dp]
+    L.izh_run_stats.argtypes = [vp, _i64p]
+    L.izh_probe_copy.argtypes = [C.c_int64, _dp]
     L.izh_timer_enable.argtypes = [C.c_int]
     L.izh_timer_read.argtypes = [_dp, _i64p, C.c_int]
     _lib = L
@@ -142,6 +144,13 @@ def timer_read(reset: bool = False) -> dict:
     n = np.zeros(len(TIMERS), dtype=np.int64)
     check(lib().izh_timer_read(_p(ms), n.ctypes.data_as(_i64p), 1 if reset else 0))
     return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(TIMERS)}
+
+
+def probe_copy(nbytes: int) -> float:
+    """Time (ms) of one plain device copy kernel of nbytes (counter calibration)."""
+    ms = C.c_double()
+    check(lib().izh_probe_copy(nbytes, C.byref(ms)))
+    return ms.value
 
 
 def physconst(freq: float) -> Const:
@@ -263,6 +272,13 @@ class BeamBunch:
     @property
     def current(self) -> float:
         return float(self.info().current)
+
+    def run_stats(self) -> dict:
+        """izh_run_stats: deposit chunk counts of the last izh_run."""
+        st = np.zeros(4, dtype=np.int64)
+        check(lib().izh_run_stats(self.h, st.ctypes.data_as(_i64p)))
+        return {"chunks": int(st[0]), "window_overflow": int(st[1]),
+                "alive": int(st[2]), "tiles": int(st[3])}
 
     def getpts(self) -> np.ndarray:
         """getpts_BeamBunch: the local particles as (N, 9) (Pts1 transposed view)."""

@@ -179,6 +179,17 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             const izh_segment *segs, int64_t nseg, double bcurr,
             double *moments);
 
+/* Counters of the last izh_run on this bunch (no reference counterpart;
+ * for tuning): stats[0] deposit chunks of the last step, stats[1] chunks
+ * whose cell box exceeded the LDS window over the whole run (they add to
+ * global memory directly), stats[2] particles alive, stats[3] sort tiles. */
+int izh_run_stats(const izh_bunch *b, int64_t stats[4]);
+
+/* Bandwidth probe (no reference counterpart): one launch of a plain copy
+ * kernel of nbytes between two device buffers, the calibration for
+ * hardware byte counters.  Returns the kernel time in *ms (HIP events). */
+int izh_probe_copy(int64_t nbytes, double *ms);
+
 /* Per-phase kernel timers, the device counterpart of the reference's Timer
  * accumulators (src/Func/Timer.f90:10-146; t_map1, t_rhofas, t_greenf,
  * t_fft2dhpf, t_field, t_force/t_ntrslo, t_diag).  When enabled, each timed
