@@ -116,24 +116,27 @@ static void dfree(P *&p) {
 }
 
 // Moving rows (phase space + original index) for cap particles.
+// The six rows share one allocation, row k at r[0] + k*cap, so a kernel can
+// address them from one base pointer.
 static int bunch_alloc_live(izh_bunch *b, int64_t cap) {
-  for (int r = 0; r < 6; r++) IZH_CHECK(dmalloc(&b->r[r], cap));
+  IZH_CHECK(dmalloc(&b->r[0], 6 * cap));
+  for (int r = 1; r < 6; r++) b->r[r] = b->r[0] + r * cap;
   IZH_CHECK(dmalloc(&b->orig, cap));
   b->cap = cap;
   return IZH_OK;
 }
 
 int bunch_alloc_alt(izh_bunch *b) {
-  for (int r = 0; r < 6; r++) IZH_CHECK(dmalloc(&b->r_alt[r], b->cap));
+  IZH_CHECK(dmalloc(&b->r_alt[0], 6 * b->cap));
+  for (int r = 1; r < 6; r++) b->r_alt[r] = b->r_alt[0] + r * b->cap;
   IZH_CHECK(dmalloc(&b->orig_alt, b->cap));
   return IZH_OK;
 }
 
 static void bunch_free_all(izh_bunch *b) {
-  for (int r = 0; r < 6; r++) {
-    dfree(b->r[r]);
-    dfree(b->r_alt[r]);
-  }
+  dfree(b->r[0]);
+  dfree(b->r_alt[0]);
+  for (int r = 1; r < 6; r++) b->r[r] = b->r_alt[r] = nullptr;
   dfree(b->orig);
   dfree(b->orig_alt);
   dfree(b->key);

@@ -86,6 +86,112 @@ __device__ inline void mom_block_store(const Mom &a, double (*sh)[kMoments],
 constexpr unsigned long long kMomentsMaxMask =
     ((1ull << kMoments) - 1ull) & ~((1ull << 37) - 1ull);
 
+// Exchange a double with the neighbouring lane of the pair (lane ^ 1) through
+// a DPP quad permutation [1,0,3,2]; every lane of the wave must be active.
+__device__ inline double swap_lane_pair(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int slo = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);
+  const int shi = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(shi, slo);
+}
+
+// The same diagnostic1_Output sums accumulated by lane pairs, with half the
+// registers of Mom: both lanes of a pair see both particles; the even lane
+// sums the x plane and the powers of phi, the odd lane the y plane and the
+// powers of pt.  Plane sums are u, u^2, u^3, u^4, u pu, u pu^2, u pu^3,
+// u^2 pu, u^2 pu^2, u^3 pu, pu, pu^2, pu^3, pu^4 with (u, pu) = (x, px) or
+// (y, py), matching the 14 x- and y-plane entries of Mom::s.
+struct PairMom {
+  double p[14];
+  double w[4];   // w, w^2, w^3 (|phi^3| for phi), w^4
+  double zp;     // phi*pt of the lane's own particle
+  double m[4];   // max |u|, |pu|, |w| over the pair; max x^2+y^2 (own)
+};
+
+__device__ inline void pmom_zero(PairMom &a) {
+#pragma unroll
+  for (int k = 0; k < 14; k++) a.p[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) a.w[k] = 0.0;
+  a.zp = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) a.m[k] = 0.0;
+}
+
+__device__ inline void pmom_one(PairMom &a, double u, double pu, double w, bool absw) {
+  const double u2 = u * u, pu2 = pu * pu, w2 = w * w;
+  double *p = a.p;
+  p[0] += u; p[1] += u2; p[2] += u2 * u; p[3] += u2 * u2;
+  p[4] += u * pu; p[5] += u * pu2; p[6] += u * pu2 * pu;
+  p[7] += u2 * pu; p[8] += u2 * pu2; p[9] += u2 * u * pu;
+  p[10] += pu; p[11] += pu2; p[12] += pu2 * pu; p[13] += pu2 * pu2;
+  const double w3 = w2 * w;
+  a.w[0] += w; a.w[1] += w2; a.w[2] += absw ? fabs(w3) : w3; a.w[3] += w2 * w2;
+  a.m[0] = fmax(a.m[0], fabs(u));
+  a.m[1] = fmax(a.m[1], fabs(pu));
+  a.m[2] = fmax(a.m[2], fabs(w));
+}
+
+// Add one particle per lane (zeros for a lane without one).  Call with every
+// lane of the wave active.
+__device__ inline void pmom_add(PairMom &a, double x, double px, double y, double py,
+                                double ph, double pt) {
+  const bool odd = threadIdx.x & 1;
+  const double q0 = swap_lane_pair(odd ? x : y);
+  const double q1 = swap_lane_pair(odd ? px : py);
+  const double q2 = swap_lane_pair(odd ? ph : pt);
+  pmom_one(a, odd ? y : x, odd ? py : px, odd ? pt : ph, !odd);
+  pmom_one(a, q0, q1, q2, !odd);
+  a.zp += ph * pt;
+  a.m[3] = fmax(a.m[3], x * x + y * y);
+}
+
+// Block-reduce PairMom into the kMoments values of Mom's layout at out.
+// sh: [blockDim/64][kMoments] in LDS.
+__device__ inline void pmom_block_store(const PairMom &a, double (*sh)[kMoments],
+                                        double *out) {
+  constexpr int kx[14] = {0, 6, 15, 16, 12, 27, 28, 29, 30, 31, 1, 7, 17, 18};
+  constexpr int ky[14] = {2, 8, 19, 20, 13, 32, 33, 34, 35, 36, 3, 9, 21, 22};
+  constexpr int kz[4] = {4, 10, 23, 24}, kp[4] = {5, 11, 25, 26};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const bool odd = lane & 1;
+#pragma unroll
+  for (int k = 0; k < 14; k++) {
+    const double e = wave_sum(odd ? 0.0 : a.p[k]);
+    const double o = wave_sum(odd ? a.p[k] : 0.0);
+    if (lane == 0) { sh[wv][kx[k]] = e; sh[wv][ky[k]] = o; }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const double e = wave_sum(odd ? 0.0 : a.w[k]);
+    const double o = wave_sum(odd ? a.w[k] : 0.0);
+    if (lane == 0) { sh[wv][kz[k]] = e; sh[wv][kp[k]] = o; }
+  }
+  {
+    const double v = wave_sum(a.zp);
+    if (lane == 0) sh[wv][14] = v;
+  }
+  constexpr int kme[3] = {37, 38, 41}, kmo[3] = {39, 40, 42};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double e = wave_max(odd ? 0.0 : a.m[k]);
+    const double o = wave_max(odd ? a.m[k] : 0.0);
+    if (lane == 0) { sh[wv][kme[k]] = e; sh[wv][kmo[k]] = o; }
+  }
+  {
+    const double v = wave_max(a.m[3]);
+    if (lane == 0) sh[wv][43] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < kMoments) {
+    const int k = threadIdx.x;
+    double r = sh[0][k];
+    for (int w = 1; w < nw; w++) r = (k < 37) ? r + sh[w][k] : fmax(r, sh[w][k]);
+    out[k] = r;
+  }
+}
+
 // 8-node CIC gather of interleaved (Ex, Ey, Ez, 0) at lower-node weights
 // ab, cd, ef, summed in the term order of scatter1_BeamBunch
 // (src/Appl/BeamBunch.f90:2130-2155).

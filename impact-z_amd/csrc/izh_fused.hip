@@ -29,14 +29,24 @@
 namespace izh {
 namespace {
 
+// One half-step push in the form the pass applies it.  The quadrupole map of
+// maplinear_Quadrupole is block diagonal (src/Appl/Quadrupole.f90:179-196),
+// so only its three 2x2 blocks travel.
+struct HalfPush {
+  double tau, gam0, ibeta0;  // exact drift
+  double m[12];              // (x,px), (y,py), (phi,pt) blocks, row major
+};
+
+enum PushKind : int { kPushNone = 0, kPushDrift = 1, kPushMap = 2 };
+
 struct PassCfg {
-  const double *in[6];
-  double *out[6];
+  const double *in;         // six rows of cap doubles, row k at in + k*cap
+  double *out;
   const uint32_t *in_orig;
   uint32_t *out_orig;
   const uint32_t *in_key;
   uint32_t *out_key;
-  void *tpos;               // t-frame positions (P), 3 blocks of cap
+  void *tpos;               // t-frame positions (T), 3 blocks of cap
   int64_t cap;
   const int64_t *d_nslots;  // input slots (device)
   uint32_t *cursor;         // scatter cursor per tile of the input keys
@@ -50,80 +60,156 @@ struct PassCfg {
   double gam, xl, rad;      // -refpt(6), Scxl, pipe radius of step s
   double tau2_prev;         // 2*tau1 of step s-1
   double inv_rho_scale;     // undo the grid-charge scaling in the kick
-  int do_kick, do_moments, do_conv;
-  Push pb, pa;              // second half of step s-1, first half of step s
+  HalfPush pb, pa;          // second half of step s-1, first half of step s
   double *part;             // per block: kMoments moment values, kConvRed conv
 };
 
-// Slot of each participating lane among the lanes that share its key, with
-// one atomic per distinct key in the wavefront.
-__device__ inline uint32_t wave_rank_add(uint32_t *counters, uint32_t key,
-                                         bool take) {
-  const int lane = threadIdx.x & 63;
-  unsigned long long pending = __ballot(take);
-  uint32_t slot = 0;
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const uint32_t lk = __shfl(key, leader, 64);
-    const unsigned long long same = __ballot(take && key == lk) & pending;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&counters[lk], (uint32_t)__popcll(same));
-    base = __shfl(base, leader, 64);
-    if ((same >> lane) & 1ull) slot = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-    pending &= ~same;
+// drift1_BeamBunch (src/Appl/BeamBunch.f90:276-320) for one particle.  The
+// map rows keep the reference's product order: the zero entries of the 6x6
+// product add nothing.
+template <int K>
+__device__ inline void half_push(const HalfPush &p, double xl, double &x, double &px,
+                                 double &y, double &py, double &ph, double &pt) {
+  if constexpr (K == kPushDrift) {
+    const double d = pt - p.gam0;
+    const double tmph = sqrt(d * d - 1.0 - px * px - py * py);
+    x = x + px * p.tau / tmph / xl;
+    y = y + py * p.tau / tmph / xl;
+    ph = ph - (p.ibeta0 + d / tmph) * p.tau / xl;
+  } else if constexpr (K == kPushMap) {
+    const double *m = p.m;
+    const double x1 = fma(px, m[1], x * m[0]), px1 = fma(px, m[3], x * m[2]);
+    const double y1 = fma(py, m[5], y * m[4]), py1 = fma(py, m[7], y * m[6]);
+    const double ph1 = fma(pt, m[9], ph * m[8]), pt1 = fma(pt, m[11], ph * m[10]);
+    x = x1; px = px1; y = y1; py = py1; ph = ph1; pt = pt1;
   }
-  return slot;
 }
 
-template <typename T>
+// Counting-sort slot of each participating lane, in two halves so the
+// atomics' latency overlaps other work: wave_rank_issue makes one atomic per
+// distinct key in the wavefront (all issued back to back), wave_rank_slot
+// later combines the leader's old counter value with the lane's rank.
+struct RankTicket {
+  uint32_t ret;   // the atomic's old value (leader lanes)
+  int leader;     // lane holding this lane's group atomic
+  uint32_t rank;  // position of this lane within its group
+};
+
+__device__ inline RankTicket wave_rank_issue(uint32_t *counters, uint32_t key, bool take) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(take);
+  RankTicket t{0u, lane, 0u};
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+    const unsigned long long same = __ballot(take && key == lk) & pending;
+    if (lane == leader) t.ret = atomicAdd(&counters[lk], (uint32_t)__popcll(same));
+    if ((same >> lane) & 1ull) {
+      t.leader = leader;
+      t.rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    }
+    pending &= ~same;
+  }
+  return t;
+}
+
+__device__ inline uint32_t wave_rank_slot(const RankTicket &t) {
+  return (uint32_t)__shfl((int)t.ret, t.leader, 64) + t.rank;
+}
+
+// Histogram update, one atomic per distinct key in the wavefront.
+__device__ inline void wave_count_add(uint32_t *counters, uint32_t key, bool take) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(take);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+    const unsigned long long same = __ballot(take && key == lk) & pending;
+    if (lane == leader) atomicAdd(&counters[lk], (uint32_t)__popcll(same));
+    pending &= ~same;
+  }
+}
+
+// Slot range of a block: contiguous, and the blocks one XCD receives (block
+// b runs on XCD b mod 8) cover adjacent ranges, so each XCD's L2 holds the
+// field of its own part of the sorted bunch.  gridDim.x is a multiple of 8.
+__device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
+  const int nb = gridDim.x;
+  const int vb = (int)(blockIdx.x & 7) * (nb >> 3) + (int)(blockIdx.x >> 3);
+  const int64_t per = ((n + nb - 1) / nb + kBlock - 1) / kBlock * kBlock;
+  lo = (int64_t)vb * per;
+  hi = lo + per < n ? lo + per : n;
+}
+
+// PB: push of the second half of step s-1 (kPushNone on the first pass, which
+// then has no kick and no moments); PA: first half of step s (kPushNone on
+// the last pass, which then has no z->t transform).
+template <typename T, int PB, int PA>
 __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
+  constexpr bool kKick = PB != kPushNone, kConv = PA != kPushNone;
   const int64_t nslots = *c.d_nslots;
+  int64_t lo, hi;
+  block_range(nslots, lo, hi);
+  const int64_t cap = c.cap;
   const double xk = 1.0 / c.xl;
   const double xmin = c.dom->range[0], ymin = c.dom->range[2], zmin = c.dom->range[4];
   const double hx = c.dom->msize[0], hy = c.dom->msize[1], hz = c.dom->msize[2];
   const double hxi = c.dom->hinv[0], hyi = c.dom->hinv[1], hzi = c.dom->hinv[2];
-  const bool kick = c.do_kick && c.scal->current >= (double)1.0e-20f;
+  const bool kick = kKick && c.scal->current >= (double)1.0e-20f;
   const double txy = c.tau2_prev * c.scal->xycon * c.inv_rho_scale;
   const double ttc = c.tau2_prev * c.scal->tcon * c.inv_rho_scale;
-  const int64_t sy = c.nx, sz = (int64_t)c.nx * c.ny;
+  const int sy = c.nx, sz = c.nx * c.ny;
   const E4<T> *E = (const E4<T> *)c.e;
   T *tpx = (T *)c.tpos;
-  Mom mom;
-  mom_zero(mom);
+  PairMom mom;
+  pmom_zero(mom);
   double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
-  double nlost = 0.0, nalive = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nslots; base += stride) {
-    const int64_t i = base + threadIdx.x;
-    const uint32_t kin = i < nslots ? c.in_key[i] : kDeadKey;
+  uint32_t nlost = 0, nalive = 0;
+  // slot i + kBlock is loaded while slot i is processed
+  int64_t i = lo + threadIdx.x;
+  uint32_t kin_n = kDeadKey, og_n = 0;
+  double r_n[6] = {0, 0, 0, 0, 0, 0};
+  if (i < hi) {
+    kin_n = c.in_key[i];
+    og_n = c.in_orig[i];
+#pragma unroll
+    for (int k = 0; k < 6; k++) r_n[k] = c.in[k * cap + i];
+  }
+  for (int64_t base = lo; base < hi; base += kBlock, i += kBlock) {
+    const uint32_t kin = kin_n, og = og_n;
+    double x = r_n[0], px = r_n[1], y = r_n[2], py = r_n[3], ph = r_n[4], pt = r_n[5];
     const bool valid = kin != kDeadKey;
-    double x = 0, px = 0, y = 0, py = 0, ph = 0, pt = 0;
-    uint32_t og = 0;
-    if (valid) {
-      x = c.in[0][i]; px = c.in[1][i]; y = c.in[2][i];
-      py = c.in[3][i]; ph = c.in[4][i]; pt = c.in[5][i];
-      og = c.in_orig[i];
-      nalive += 1.0;
+    nalive += valid ? 1u : 0u;
+    // counting-sort scatter by the key of the previous step
+    const RankTicket tk = wave_rank_issue(c.cursor, kin, valid);
+    if (i + kBlock < hi) {
+      const int64_t j = i + kBlock;
+      kin_n = c.in_key[j];
+      og_n = c.in_orig[j];
+#pragma unroll
+      for (int k = 0; k < 6; k++) r_n[k] = c.in[k * cap + j];
+    } else {
+      kin_n = kDeadKey;
     }
-    if (valid && c.do_kick) {
-      if (kick) {
+    if constexpr (kKick) {
+      if (valid && kick) {
         // cvbkforth1st + scatter1 (src/Appl/BeamBunch.f90:1667-1701,2041-2181)
-        double rcp = 1.0 / (-pt + c.gam);
-        double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
-        double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam * betai * ph * c.xl;
+        const double rcp = 1.0 / (-pt + c.gam);
+        const double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
+        const double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam * betai * ph * c.xl;
         int ix = (int)((X0 - xmin) * hxi + 1.0);
-        double ab = ((xmin - X0) + ix * hx) * hxi;
+        const double ab = ((xmin - X0) + ix * hx) * hxi;
         int jx = (int)((Y0 - ymin) * hyi + 1.0);
-        double cd = ((ymin - Y0) + jx * hy) * hyi;
+        const double cd = ((ymin - Y0) + jx * hy) * hyi;
         int kx = (int)((Z0 - zmin) * hzi + 1.0);
-        double ef = ((zmin - Z0) + kx * hz) * hzi;
+        const double ef = ((zmin - Z0) + kx * hz) * hzi;
         ix = min(max(ix, 1), c.nx - 1);
         jx = min(max(jx, 1), c.ny - 1);
         kx = min(max(kx, 1), c.nz - 1);
-        int64_t o = (ix - 1) + sy * (jx - 1) + sz * (kx - 1);
+        const int64_t o = (int64_t)(ix - 1) + (int64_t)sy * (jx - 1) + (int64_t)sz * (kx - 1);
         T gx, gy, gz;
         cic8_e4<T>(E, o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
-        double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
+        const double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
         x = X0 * xk;
         y = Y0 * xk;
         ph = Z0 * xk / (-c.gam * betai);
@@ -131,68 +217,80 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
         py = py + txy * eyn;
         pt = pt - ttc * ezn;
       }
-      apply_push(c.pb, x, px, y, py, ph, pt);
-      if (c.do_moments) mom_add(mom, x, px, y, py, ph, pt);
+      half_push<PB>(c.pb, c.xl, x, px, y, py, ph, pt);
+      // diagnostic1 sums of the state after step s-1 (all lanes take part)
+      pmom_add(mom, valid ? x : 0.0, valid ? px : 0.0, valid ? y : 0.0,
+               valid ? py : 0.0, valid ? ph : 0.0, valid ? pt : 0.0);
     }
     uint32_t knew = valid ? 0u : kDeadKey;
     T tx = (T)NAN, ty = (T)NAN, tz = (T)NAN;
-    if (valid && c.do_conv) {
-      apply_push(c.pa, x, px, y, py, ph, pt);
-      // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
-      ph = wrap_phase(ph);
-      double rcp = 1.0 / (-pt + c.gam);
-      double tmpbet = 1.0 - rcp * rcp * (1.0 + px * px + py * py);
-      double betai = sqrt(fabs(tmpbet));
-      double tmpx = x * c.xl, xt = (x - px * ph * rcp) * c.xl;
-      double tmpy = y * c.xl, yt = (y - py * ph * rcp) * c.xl;
-      double zt = -c.gam * betai * ph * c.xl;
-      mn[0] = fmin(mn[0], fmin(tmpx, xt));
-      mxv[0] = fmax(mxv[0], fmax(tmpx, xt));
-      mn[1] = fmin(mn[1], fmin(tmpy, yt));
-      mxv[1] = fmax(mxv[1], fmax(tmpy, yt));
-      mn[2] = fmin(mn[2], zt);
-      mxv[2] = fmax(mxv[2], zt);
-      if (lost_t1(xt, yt, px, py, pt, c.gam, c.rad)) {
-        nlost += 1.0;
-        knew = kDeadKey;
-      } else {
-        int ci = min(max((int)((xt - xmin) * hxi), 0), c.nx - 1);
-        int cj = min(max((int)((yt - ymin) * hyi), 0), c.ny - 1);
-        int ck = min(max((int)((zt - zmin) * hzi), 0), c.nz - 1);
-        knew = (uint32_t)((ci >> c.tshift[0]) +
-                          c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
-        tx = (T)xt;
-        ty = (T)yt;
-        tz = (T)zt;
+    if constexpr (kConv) {
+      if (valid) {
+        half_push<PA>(c.pa, c.xl, x, px, y, py, ph, pt);
+        // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
+        ph = wrap_phase(ph);
+        const double rcp = 1.0 / (-pt + c.gam);
+        const double tmpbet = 1.0 - rcp * rcp * (1.0 + px * px + py * py);
+        const double betai = sqrt(fabs(tmpbet));
+        const double tmpx = x * c.xl, xt = (x - px * ph * rcp) * c.xl;
+        const double tmpy = y * c.xl, yt = (y - py * ph * rcp) * c.xl;
+        const double zt = -c.gam * betai * ph * c.xl;
+        mn[0] = fmin(mn[0], fmin(tmpx, xt));
+        mxv[0] = fmax(mxv[0], fmax(tmpx, xt));
+        mn[1] = fmin(mn[1], fmin(tmpy, yt));
+        mxv[1] = fmax(mxv[1], fmax(tmpy, yt));
+        mn[2] = fmin(mn[2], zt);
+        mxv[2] = fmax(mxv[2], zt);
+        const double radtest = sqrt(xt * xt + yt * yt);
+        if (radtest >= c.rad || tmpbet <= 0.0) {
+          nlost++;
+          knew = kDeadKey;
+        } else {
+          const int ci = min(max((int)((xt - xmin) * hxi), 0), c.nx - 1);
+          const int cj = min(max((int)((yt - ymin) * hyi), 0), c.ny - 1);
+          const int ck = min(max((int)((zt - zmin) * hzi), 0), c.nz - 1);
+          knew = (uint32_t)((ci >> c.tshift[0]) +
+                            c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
+          tx = (T)xt;
+          ty = (T)yt;
+          tz = (T)zt;
+        }
       }
+      // histogram of the new keys for the next pass
+      wave_count_add(c.hist_new, knew, knew != kDeadKey);
     }
-    // counting-sort scatter by the key of the previous step, then the
-    // histogram of the new keys for the next one
-    const uint32_t slot = wave_rank_add(c.cursor, kin, valid);
-    if (c.do_conv) (void)wave_rank_add(c.hist_new, knew, knew != kDeadKey);
-    if (valid && slot >= (uint32_t)c.cap) {
+    const uint32_t slot = wave_rank_slot(tk);
+    if (valid && slot >= (uint32_t)cap) {
       atomicAdd((unsigned long long *)&c.err->err, 1ull);
     } else if (valid) {
-      c.out[0][slot] = x; c.out[1][slot] = px; c.out[2][slot] = y;
-      c.out[3][slot] = py; c.out[4][slot] = ph; c.out[5][slot] = pt;
+      c.out[slot] = x;
+      c.out[cap + slot] = px;
+      c.out[2 * cap + slot] = y;
+      c.out[3 * cap + slot] = py;
+      c.out[4 * cap + slot] = ph;
+      c.out[5 * cap + slot] = pt;
       c.out_orig[slot] = og;
       c.out_key[slot] = knew;
-      if (c.do_conv) {
+      if constexpr (kConv) {
         tpx[slot] = tx;
-        tpx[c.cap + slot] = ty;
-        tpx[2 * c.cap + slot] = tz;
+        tpx[cap + slot] = ty;
+        tpx[2 * cap + slot] = tz;
       }
     }
   }
   __shared__ double sh[kBlock / 64][kMoments];
   double *out = c.part + (int64_t)blockIdx.x * kPartStride;
-  mom_block_store(mom, sh, out);
+  if constexpr (kKick) {
+    pmom_block_store(mom, sh, out);
+  } else if (threadIdx.x < kMoments) {
+    out[threadIdx.x] = 0.0;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double v[kConvRed];
   for (int k = 0; k < 3; k++) { v[k] = wave_min(mn[k]); v[3 + k] = wave_max(mxv[k]); }
-  v[6] = wave_sum(nlost);
-  v[7] = wave_sum(nalive);
+  v[6] = wave_sum((double)nlost);
+  v[7] = wave_sum((double)nalive);
   if (lane == 0)
     for (int k = 0; k < kConvRed; k++) sh[wv][k] = v[k];
   __syncthreads();
@@ -203,6 +301,26 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
       r = k < 3 ? fmin(r, sh[w][k]) : (k < 6 ? fmax(r, sh[w][k]) : r + sh[w][k]);
     out[kMoments + k] = r;
   }
+}
+
+template <typename T>
+void launch_pass(int pb, int pa, int nb, hipStream_t st, const PassCfg &pc) {
+  const dim3 g(nb), b(kBlock);
+#define IZH_PASS(PB, PA) \
+  hipLaunchKernelGGL((k_pass<T, PB, PA>), g, b, 0, st, pc)
+  if (pb == kPushNone) {
+    if (pa == kPushDrift) IZH_PASS(kPushNone, kPushDrift);
+    else IZH_PASS(kPushNone, kPushMap);
+  } else if (pb == kPushDrift) {
+    if (pa == kPushNone) IZH_PASS(kPushDrift, kPushNone);
+    else if (pa == kPushDrift) IZH_PASS(kPushDrift, kPushDrift);
+    else IZH_PASS(kPushDrift, kPushMap);
+  } else {
+    if (pa == kPushNone) IZH_PASS(kPushMap, kPushNone);
+    else if (pa == kPushDrift) IZH_PASS(kPushMap, kPushDrift);
+    else IZH_PASS(kPushMap, kPushMap);
+  }
+#undef IZH_PASS
 }
 
 // ------------------------------------------------------------ prologue
@@ -265,7 +383,7 @@ k_prologue_keys(const double *__restrict__ X, const double *__restrict__ PX,
       k = (uint32_t)((ci[0] >> s0) + tnx * ((ci[1] >> s1) + tny * (ci[2] >> s2)));
       key[i] = k;
     }
-    (void)wave_rank_add(hist, k, k != kDeadKey);
+    wave_count_add(hist, k, k != kDeadKey);
   }
 }
 
@@ -308,33 +426,25 @@ struct FinCfg {
 
 __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
   __shared__ double red[kPartStride];
-  __shared__ double sh[1024];
   __shared__ uint32_t ssum[1024];
   __shared__ uint32_t csum[1024];
   const int tid = threadIdx.x;
-  // ---- reduce the pass partials (fixed tree order)
+  // ---- reduce the pass partials: one wavefront per quantity, fixed order
   if (f.mode != 0) {
-    for (int q = 0; q < kPartStride; q++) {
+    const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    for (int q = wv; q < kPartStride; q += nw) {
       const bool isconv = q >= kMoments;
       const int cq = q - kMoments;
       const int op = !isconv ? (q < 37 ? 0 : 2) : (cq < 3 ? 1 : (cq < 6 ? 2 : 0));
       double acc = op == 1 ? 1e300 : (op == 2 ? -1e300 : 0.0);
-      for (int b = tid; b < f.nparts; b += blockDim.x) {
-        double v = f.part[(int64_t)b * kPartStride + q];
+      for (int b = lane; b < f.nparts; b += 64) {
+        const double v = f.part[(int64_t)b * kPartStride + q];
         acc = op == 1 ? fmin(acc, v) : (op == 2 ? fmax(acc, v) : acc + v);
       }
-      sh[tid] = acc;
-      __syncthreads();
-      for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-          double a = sh[tid], b2 = sh[tid + s];
-          sh[tid] = op == 1 ? fmin(a, b2) : (op == 2 ? fmax(a, b2) : a + b2);
-        }
-        __syncthreads();
-      }
-      if (tid == 0) red[q] = sh[0];
-      __syncthreads();
+      acc = op == 1 ? wave_min(acc) : (op == 2 ? wave_max(acc) : wave_sum(acc));
+      if (lane == 0) red[q] = acc;
     }
+    __syncthreads();
   }
   if (tid == 0) {
     DevScal &S = *f.scal;
@@ -473,23 +583,35 @@ int bunch_alloc_fused(izh_bunch *b, int ntiles, int64_t nseg) {
   return IZH_OK;
 }
 
-// Host-side descriptors of a segment's half-step push; refpt and z follow the
+// Host-side descriptor of a segment's half-step push; refpt and z follow the
 // reference's bookkeeping (drift1_BeamBunch, maplinear_Quadrupole).
-static int segment_push(izh_bunch *b, const izh_segment &sg, Push &p, double *z) {
-  p = Push{};
+static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &kind,
+                        double *z) {
+  p = HalfPush{};
   if (sg.bitype == 0) {
     const double gam0 = -b->refpt[5];
     const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
-    p.kind = 1;
+    kind = kPushDrift;
     p.tau = sg.tau1;
     p.gam0 = gam0;
     p.ibeta0 = 1.0 / beta0;
-    p.scxl = b->c.scxl;
     b->refpt[4] = b->refpt[4] + sg.tau1 / (b->c.scxl * beta0);
   } else if (sg.bitype == 1) {
-    p.kind = 2;
+    kind = kPushMap;
+    double xm[36];
     IZH_CHECK(izh_quad_maplinear(&b->c, sg.tau1, sg.gradient, sg.mapstp, b->refpt,
-                                 b->charge, b->mass, p.xm));
+                                 b->charge, b->mass, xm));
+    for (int blk = 0; blk < 3; blk++) {
+      const int r = 2 * blk;
+      p.m[4 * blk + 0] = xm[6 * r + r];
+      p.m[4 * blk + 1] = xm[6 * r + r + 1];
+      p.m[4 * blk + 2] = xm[6 * (r + 1) + r];
+      p.m[4 * blk + 3] = xm[6 * (r + 1) + r + 1];
+    }
+    for (int j = 0; j < 6; j++)
+      for (int k = 0; k < 6; k++)
+        if (j / 2 != k / 2 && xm[6 * j + k] != 0.0)
+          return fail(IZH_ERR_STATE, "izh_run: quadrupole map is not block diagonal");
   } else {
     return fail(IZH_ERR_ARG, "izh_run: bitype %d not supported", sg.bitype);
   }
@@ -518,7 +640,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   const double gam = -b->refpt[5];
   const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
   const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
-  const int nb = grid_for(b->n);
+  const int nb = (grid_for(b->n) + 7) / 8 * 8;  // multiple of 8 for block_range
   int rc = IZH_OK;
   // ---- device scalars
   DevScal hs{};
@@ -579,21 +701,21 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   pc.inv_rho_scale = 1.0 / rho_scale;
   pc.tpos = b->tpos;
   pc.part = b->d_part;
-  Push pb_prev{};
+  HalfPush pb_prev{};
+  int kb_prev = kPushNone;
   double tau2_prev = 0.0;
   for (int64_t s = 0; s <= nseg && rc == IZH_OK; s++) {
     const bool last = s == nseg;
-    Push pa{};
-    if (!last) IZH_CHECK(segment_push(b, segs[s], pa, z));  // first half of step s
-    for (int r = 0; r < 6; r++) { pc.in[r] = b->r[r]; pc.out[r] = b->r_alt[r]; }
+    HalfPush pa{};
+    int ka = kPushNone;
+    if (!last) IZH_CHECK(segment_push(b, segs[s], pa, ka, z));  // first half of step s
+    pc.in = b->r[0];
+    pc.out = b->r_alt[0];
     pc.in_orig = b->orig;
     pc.out_orig = b->orig_alt;
     pc.in_key = b->key;
     pc.out_key = b->key_alt;
     pc.hist_new = b->hist[(s + 1) & 1];
-    pc.do_kick = s > 0;
-    pc.do_moments = s > 0;
-    pc.do_conv = !last;
     pc.pb = pb_prev;
     pc.pa = pa;
     pc.tau2_prev = tau2_prev;
@@ -601,9 +723,9 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     {
       ScopedTimer tm_(IZH_T_PASS, st);
       if (f->precision == IZH_FP64)
-        hipLaunchKernelGGL((k_pass<double>), dim3(nb), dim3(kBlock), 0, st, pc);
+        launch_pass<double>(kb_prev, ka, nb, st, pc);
       else
-        hipLaunchKernelGGL((k_pass<float>), dim3(nb), dim3(kBlock), 0, st, pc);
+        launch_pass<float>(kb_prev, ka, nb, st, pc);
     }
     IZH_HIP(hipGetLastError());
     for (int r = 0; r < 6; r++) std::swap(b->r[r], b->r_alt[r]);
@@ -627,7 +749,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
     // second half of step s: host bookkeeping now, applied by the next pass
-    if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, z);
+    if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, kb_prev, z);
     tau2_prev = 2.0 * segs[s].tau1;
   }
   f->stream = fstream;
