@@ -1,0 +1,369 @@
+!----------------------------------------------------------------------------
+! impactz_hip -- Fortran 90 binding of the C ABI in include/impactz_hip.h.
+!
+! The interfaces bind every entry point with iso_c_binding (handles are
+! type(c_ptr), scalars pass by value, arrays by reference).  The wrapper
+! subroutines below them keep the argument lists of the reference procedures
+! they replace, so a host loop written like run_AccSimulator
+! (src/Contrl/AccSimulator.f90:1099-1488) calls them in the same order, and
+! turn a non-zero status into the reference's reaction to an error: print
+! the message and stop.
+!----------------------------------------------------------------------------
+module impactz_hip
+  use iso_c_binding
+  implicit none
+  private :: c_strlen
+
+  integer(c_int), parameter :: IZH_OK = 0
+  integer(c_int), parameter :: IZH_FP64 = 0, IZH_MIXED = 1
+  integer(c_int), parameter :: IZH_RHO = 0, IZH_PHI = 1, IZH_EX = 2, &
+                               IZH_EY = 3, IZH_EZ = 4
+
+  ! PhysConst scalars (src/DataStruct/PhysConst.f90:18-35)
+  type, bind(C) :: izh_const
+    real(c_double) :: clight, pi, scxl, scfreq
+  end type izh_const
+
+  ! BeamBunch scalars (src/Appl/BeamBunch.f90:23-33)
+  type, bind(C) :: izh_bunch_info
+    real(c_double) :: current, mass, charge
+    integer(c_int64_t) :: npt, nptlocal
+    real(c_double) :: refpt(6)
+    real(c_double) :: freq
+    integer(c_int) :: precision
+  end type izh_bunch_info
+
+  ! CompDom for one process (src/Appl/CompDom.f90:19-32)
+  type, bind(C) :: izh_compdom
+    integer(c_int) :: nx, ny, nz
+    real(c_double) :: range(6)
+    real(c_double) :: msize(3)
+  end type izh_compdom
+
+  ! one segment of a drift/quadrupole lattice
+  type, bind(C) :: izh_segment
+    real(c_double) :: tau1
+    integer(c_int) :: bitype
+    real(c_double) :: gradient
+    integer(c_int) :: mapstp
+    real(c_double) :: xrad, yrad
+  end type izh_segment
+
+  interface
+    function izh_last_error() bind(C, name="izh_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    function izh_init(device) bind(C, name="izh_init") result(rc)
+      import :: c_int
+      integer(c_int), value :: device
+      integer(c_int) :: rc
+    end function
+    function izh_physconst(freq, c) bind(C, name="izh_physconst") result(rc)
+      import :: c_int, c_double, izh_const
+      real(c_double), value :: freq
+      type(izh_const) :: c
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_create(pts9, n, current, kinetic, mass, charge, freq, &
+                              phase0, precision, b) &
+        bind(C, name="izh_bunch_create") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      real(c_double), intent(in) :: pts9(*)
+      integer(c_int64_t), value :: n
+      real(c_double), value :: current, kinetic, mass, charge, freq, phase0
+      integer(c_int), value :: precision
+      type(c_ptr) :: b
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_destroy(b) bind(C, name="izh_bunch_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_download(b, pts9, cap, n) &
+        bind(C, name="izh_bunch_download") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double) :: pts9(*)
+      integer(c_int64_t), value :: cap
+      integer(c_int64_t) :: n
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_upload(b, pts9, n) bind(C, name="izh_bunch_upload") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), intent(in) :: pts9(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_get_info(b, info) bind(C, name="izh_bunch_get_info") result(rc)
+      import :: c_int, c_ptr, izh_bunch_info
+      type(c_ptr), value :: b
+      type(izh_bunch_info) :: info
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_set_info(b, info) bind(C, name="izh_bunch_set_info") result(rc)
+      import :: c_int, c_ptr, izh_bunch_info
+      type(c_ptr), value :: b
+      type(izh_bunch_info), intent(in) :: info
+      integer(c_int) :: rc
+    end function
+    function izh_quad_maplinear(c, tau, gradient, mapstp, refpt, charge, mass, xm) &
+        bind(C, name="izh_quad_maplinear") result(rc)
+      import :: c_int, c_double, izh_const
+      type(izh_const), intent(in) :: c
+      real(c_double), value :: tau, gradient
+      integer(c_int), value :: mapstp
+      real(c_double) :: refpt(6)
+      real(c_double), value :: charge, mass
+      real(c_double) :: xm(36)
+      integer(c_int) :: rc
+    end function
+    function izh_drift1(b, z, tau, bitype, gradient, mapstp) &
+        bind(C, name="izh_drift1") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double) :: z
+      real(c_double), value :: tau
+      integer(c_int), value :: bitype
+      real(c_double), value :: gradient
+      integer(c_int), value :: mapstp
+      integer(c_int) :: rc
+    end function
+    function izh_map_linear(b, xm) bind(C, name="izh_map_linear") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), intent(in) :: xm(36)
+      integer(c_int) :: rc
+    end function
+    function izh_conv1st(b, tau, ptrange, nplc, nptot, xrad, yrad, flagbc) &
+        bind(C, name="izh_conv1st") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double), value :: tau
+      real(c_double) :: ptrange(6)
+      integer(c_int64_t) :: nplc, nptot
+      real(c_double), value :: xrad, yrad
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
+    function izh_update_compdom(dom, ptrange, flagbc) &
+        bind(C, name="izh_update_compdom") result(rc)
+      import :: c_int, c_double, izh_compdom
+      type(izh_compdom) :: dom
+      real(c_double), intent(in) :: ptrange(6)
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
+    function izh_field_create(nx, ny, nz, precision, f) &
+        bind(C, name="izh_field_create") result(rc)
+      import :: c_int, c_ptr
+      integer(c_int), value :: nx, ny, nz, precision
+      type(c_ptr) :: f
+      integer(c_int) :: rc
+    end function
+    function izh_field_destroy(f) bind(C, name="izh_field_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: f
+      integer(c_int) :: rc
+    end function
+    function izh_charge(b, dom, f) bind(C, name="izh_charge") result(rc)
+      import :: c_int, c_ptr, izh_compdom
+      type(c_ptr), value :: b
+      type(izh_compdom), intent(in) :: dom
+      type(c_ptr), value :: f
+      integer(c_int) :: rc
+    end function
+    function izh_update3O(f, dom) bind(C, name="izh_update3O") result(rc)
+      import :: c_int, c_ptr, izh_compdom
+      type(c_ptr), value :: f
+      type(izh_compdom), intent(in) :: dom
+      integer(c_int) :: rc
+    end function
+    function izh_cvbkforth1st(b) bind(C, name="izh_cvbkforth1st") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: rc
+    end function
+    function izh_kick1(b, tau, f, dom, flagbc) bind(C, name="izh_kick1") result(rc)
+      import :: c_int, c_double, c_ptr, izh_compdom
+      type(c_ptr), value :: b
+      real(c_double), value :: tau
+      type(c_ptr), value :: f
+      type(izh_compdom), intent(in) :: dom
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
+    function izh_diagnostic1(b, sums, amax, r2max) &
+        bind(C, name="izh_diagnostic1") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double) :: sums(37), amax(6), r2max
+      integer(c_int) :: rc
+    end function
+    function izh_field_download(f, which, host) &
+        bind(C, name="izh_field_download") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      integer(c_int), value :: which
+      real(c_double) :: host(*)
+      integer(c_int) :: rc
+    end function
+    function izh_field_upload_rho(f, rho) bind(C, name="izh_field_upload_rho") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      real(c_double), intent(in) :: rho(*)
+      integer(c_int) :: rc
+    end function
+    function izh_step(b, f, dom, z, tau1, bitype, gradient, mapstp, xrad, yrad, &
+                      bcurr, moments) bind(C, name="izh_step") result(rc)
+      import :: c_int, c_double, c_ptr, izh_compdom
+      type(c_ptr), value :: b, f
+      type(izh_compdom) :: dom
+      real(c_double) :: z
+      real(c_double), value :: tau1
+      integer(c_int), value :: bitype
+      real(c_double), value :: gradient
+      integer(c_int), value :: mapstp
+      real(c_double), value :: xrad, yrad, bcurr
+      real(c_double) :: moments(44)
+      integer(c_int) :: rc
+    end function
+    function izh_run(b, f, dom, z, segs, nseg, bcurr, moments) &
+        bind(C, name="izh_run") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr, izh_compdom, izh_segment
+      type(c_ptr), value :: b, f
+      type(izh_compdom) :: dom
+      real(c_double) :: z
+      type(izh_segment), intent(in) :: segs(*)
+      integer(c_int64_t), value :: nseg
+      real(c_double), value :: bcurr
+      real(c_double) :: moments(*)
+      integer(c_int) :: rc
+    end function
+    function izh_run_stats(b, stats) bind(C, name="izh_run_stats") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int64_t) :: stats(4)
+      integer(c_int) :: rc
+    end function
+    function izh_probe_copy(nbytes, ms) bind(C, name="izh_probe_copy") result(rc)
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: nbytes
+      real(c_double) :: ms
+      integer(c_int) :: rc
+    end function
+    function izh_timer_enable(on) bind(C, name="izh_timer_enable") result(rc)
+      import :: c_int
+      integer(c_int), value :: on
+      integer(c_int) :: rc
+    end function
+    function izh_timer_read(ms, launches, reset) bind(C, name="izh_timer_read") result(rc)
+      import :: c_int, c_int64_t, c_double
+      real(c_double) :: ms(*)
+      integer(c_int64_t) :: launches(*)
+      integer(c_int), value :: reset
+      integer(c_int) :: rc
+    end function
+    function c_strlen(s) bind(C, name="strlen") result(n)
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: s
+      integer(c_size_t) :: n
+    end function
+  end interface
+
+contains
+
+  ! print + stop on a non-zero status, as the reference reacts to errors
+  subroutine izh_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    type(c_ptr) :: p
+    integer :: n, i
+    character(len=:), allocatable :: text
+    if (rc == IZH_OK) return
+    p = izh_last_error()
+    n = int(c_strlen(p))
+    call c_f_pointer(p, msg, [n])
+    allocate(character(len=n) :: text)
+    do i = 1, n
+      text(i:i) = msg(i)
+    end do
+    print *, 'impactz_hip: ', where, ' failed (status ', rc, '): ', text
+    stop 1
+  end subroutine izh_check
+
+  ! drift1_BeamBunch(this,beamln,z,tau,bitype,nseg,nst,ihlf)
+  ! (src/Appl/BeamBunch.f90:250-258): the element's data enter as bitype,
+  ! gradient and mapstp.
+  subroutine drift1_gpu(b, z, tau, bitype, gradient, mapstp)
+    type(c_ptr), intent(in) :: b
+    real(c_double), intent(inout) :: z
+    real(c_double), intent(in) :: tau, gradient
+    integer, intent(in) :: bitype, mapstp
+    call izh_check(izh_drift1(b, z, tau, int(bitype, c_int), gradient, &
+                              int(mapstp, c_int)), 'drift1')
+  end subroutine drift1_gpu
+
+  ! conv1st_BeamBunch(this,tau,nplc,nptot,ptrange,Flagbc,perd,xrad,yrad)
+  ! (src/Appl/BeamBunch.f90:1252-1260)
+  subroutine conv1st_gpu(b, tau, nplc, nptot, ptrange, flagbc, xrad, yrad)
+    type(c_ptr), intent(in) :: b
+    real(c_double), intent(in) :: tau, xrad, yrad
+    integer(c_int64_t), intent(out) :: nplc, nptot
+    real(c_double), intent(out) :: ptrange(6)
+    integer, intent(in) :: flagbc
+    call izh_check(izh_conv1st(b, tau, ptrange, nplc, nptot, xrad, yrad, &
+                               int(flagbc, c_int)), 'conv1st')
+  end subroutine conv1st_gpu
+
+  ! update_CompDom(this,ptrange,grid2d,Flagbc) (src/Appl/CompDom.f90:509-515)
+  subroutine update_compdom_gpu(dom, ptrange, flagbc)
+    type(izh_compdom), intent(inout) :: dom
+    real(c_double), intent(in) :: ptrange(6)
+    integer, intent(in) :: flagbc
+    call izh_check(izh_update_compdom(dom, ptrange, int(flagbc, c_int)), &
+                   'update_CompDom')
+  end subroutine update_compdom_gpu
+
+  ! charge_BeamBunch (src/Appl/BeamBunch.f90:3535-3544): the grid charge
+  ! stays in the device field f
+  subroutine charge_gpu(b, dom, f)
+    type(c_ptr), intent(in) :: b, f
+    type(izh_compdom), intent(in) :: dom
+    call izh_check(izh_charge(b, dom, f), 'charge')
+  end subroutine charge_gpu
+
+  ! update3O_FieldQuant (src/Appl/Field.f90:108-117)
+  subroutine update3O_gpu(f, dom)
+    type(c_ptr), intent(in) :: f
+    type(izh_compdom), intent(in) :: dom
+    call izh_check(izh_update3O(f, dom), 'update3O')
+  end subroutine update3O_gpu
+
+  ! cvbkforth1st_BeamBunch(this) (src/Appl/BeamBunch.f90:1667)
+  subroutine cvbkforth1st_gpu(b)
+    type(c_ptr), intent(in) :: b
+    call izh_check(izh_cvbkforth1st(b), 'cvbkforth1st')
+  end subroutine cvbkforth1st_gpu
+
+  ! kick1_BeamBunch(this,tau,innx,inny,innz,temppotent,ptsgeom,grid,Flagbc,perdlen)
+  ! (src/Appl/BeamBunch.f90:1707-1716): the potential is the device field f
+  subroutine kick1_gpu(b, tau, f, dom, flagbc)
+    type(c_ptr), intent(in) :: b, f
+    real(c_double), intent(in) :: tau
+    type(izh_compdom), intent(in) :: dom
+    integer, intent(in) :: flagbc
+    call izh_check(izh_kick1(b, tau, f, dom, int(flagbc, c_int)), 'kick1')
+  end subroutine kick1_gpu
+
+  ! the reductions of diagnostic1_Output (src/Contrl/Output.f90:119-245)
+  subroutine diagnostic1_sums_gpu(b, sums, amax, r2max)
+    type(c_ptr), intent(in) :: b
+    real(c_double), intent(out) :: sums(37), amax(6), r2max
+    call izh_check(izh_diagnostic1(b, sums, amax, r2max), 'diagnostic1')
+  end subroutine diagnostic1_sums_gpu
+
+end module impactz_hip

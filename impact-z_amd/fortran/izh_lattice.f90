@@ -1,0 +1,121 @@
+!----------------------------------------------------------------------------
+! izh_lattice -- a Fortran 90 host that marches a particle.in bunch through a
+! drift/quadrupole lattice on the GPU, calling the C ABI through the
+! impactz_hip module in the order of run_AccSimulator's segment loop
+! (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1, Flagbc=1):
+!   drift1(tau1) -> conv1st -> update_CompDom -> charge -> update3O ->
+!   cvbkforth1st -> kick1(2 tau1) -> drift1(tau1) -> diagnostic1
+! With fused = .true. the same segments go through izh_run in one call.
+!
+! Input: particle.in (readin_Dist format, src/Appl/Distribution.f90:1677-1705)
+! and the namelist file named by the first command-line argument.  Output:
+! one line per step with z, Npt and the 44 diagnostic1 reductions (37 sums,
+! 6 absolute maxima, max x^2+y^2) in the file named by the namelist.
+!----------------------------------------------------------------------------
+program izh_lattice
+  use iso_c_binding
+  use impactz_hip
+  implicit none
+  integer, parameter :: maxelem = 4096
+  integer :: nx = 16, ny = 16, nz = 32, precision = 0, device = 0
+  real(8) :: current = 0.1d0, kinetic = 150.0d6, mass = 938.272088d6
+  real(8) :: charge = 1.0d0, freq = 324.0d6, phase0 = 0.0d0
+  integer :: nelem = 0
+  real(8) :: elen(maxelem) = 0, grad(maxelem) = 0, radius(maxelem) = 1
+  integer :: nseg(maxelem) = 1, mapstp(maxelem) = 1, bitype(maxelem) = 0
+  logical :: fused = .false.
+  character(len=256) :: particles = 'particle.in', output = 'izh_moments.txt'
+  namelist /izh_run_nml/ nx, ny, nz, precision, device, current, kinetic, &
+      mass, charge, freq, phase0, nelem, elen, grad, radius, nseg, mapstp, &
+      bitype, fused, particles, output
+
+  character(len=256) :: nmlfile
+  real(c_double), allocatable :: pts(:, :), recs(:, :)
+  type(izh_segment), allocatable :: segs(:)
+  type(c_ptr) :: b, f
+  type(izh_compdom) :: dom
+  type(izh_bunch_info) :: info
+  real(c_double) :: z, tau1, tau2, ptrange(6), sums(37), amax(6), r2max
+  integer(c_int64_t) :: n, nplc, nptot, nstep, k
+  integer :: i, j, u, istep
+
+  call get_command_argument(1, nmlfile)
+  open(newunit=u, file=trim(nmlfile), status='old')
+  read(u, nml=izh_run_nml)
+  close(u)
+
+  ! readin_Dist: N, then one particle (9 values) per line
+  open(newunit=u, file=trim(particles), status='old')
+  read(u, *) n
+  allocate(pts(9, n))
+  do k = 1, n
+    read(u, *) pts(:, k)
+  end do
+  close(u)
+
+  call izh_check(izh_init(int(device, c_int)), 'init')
+  call izh_check(izh_bunch_create(pts, n, current, kinetic, mass, charge, freq, &
+                                  phase0, int(precision, c_int), b), 'bunch_create')
+  call izh_check(izh_field_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
+                                  int(precision, c_int), f), 'field_create')
+  dom%nx = nx; dom%ny = ny; dom%nz = nz
+  dom%range = 0; dom%msize = 0
+  z = 0.0d0
+
+  open(newunit=u, file=trim(output), status='replace')
+  if (.not. fused) then
+    istep = 0
+    do i = 1, nelem
+      tau1 = 0.5d0 * elen(i) / nseg(i)
+      tau2 = 2.0d0 * tau1
+      do j = 1, nseg(i)
+        call drift1_gpu(b, z, tau1, bitype(i), grad(i), mapstp(i))
+        if (current >= 1.0e-10) then
+          call conv1st_gpu(b, tau2, nplc, nptot, ptrange, 1, radius(i), radius(i))
+          call update_compdom_gpu(dom, ptrange, 1)
+          call charge_gpu(b, dom, f)
+          call update3O_gpu(f, dom)
+          call cvbkforth1st_gpu(b)
+          call kick1_gpu(b, tau2, f, dom, 1)
+        end if
+        call drift1_gpu(b, z, tau1, bitype(i), grad(i), mapstp(i))
+        call diagnostic1_sums_gpu(b, sums, amax, r2max)
+        call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
+        istep = istep + 1
+        write(u, '(I8,1X,ES25.17,1X,I12,44(1X,ES25.17))') istep, z, info%npt, &
+            sums, amax, r2max
+      end do
+    end do
+  else
+    nstep = sum(nseg(1:nelem))
+    allocate(segs(nstep), recs(45, nstep))
+    k = 0
+    do i = 1, nelem
+      do j = 1, nseg(i)
+        k = k + 1
+        segs(k)%tau1 = 0.5d0 * elen(i) / nseg(i)
+        segs(k)%bitype = int(bitype(i), c_int)
+        segs(k)%gradient = grad(i)
+        segs(k)%mapstp = int(mapstp(i), c_int)
+        segs(k)%xrad = radius(i)
+        segs(k)%yrad = radius(i)
+      end do
+    end do
+    call izh_check(izh_run(b, f, dom, z, segs, nstep, current, recs), 'run')
+    ! z after each step: the lattice's cumulative length
+    z = 0.0d0
+    k = 0
+    do i = 1, nelem
+      do j = 1, nseg(i)
+        k = k + 1
+        z = z + elen(i) / nseg(i)
+        write(u, '(I8,1X,ES25.17,1X,I12,44(1X,ES25.17))') k, z, &
+            int(recs(45, k), c_int64_t), recs(1:44, k)
+      end do
+    end do
+  end if
+  close(u)
+
+  call izh_check(izh_bunch_destroy(b), 'bunch_destroy')
+  call izh_check(izh_field_destroy(f), 'field_destroy')
+end program izh_lattice

@@ -29,6 +29,33 @@ HIP_FLAGS = [
 ]
 
 
+FC = os.environ.get("IZH_FC", "/opt/rocm/bin/amdflang")
+FORTRAN = os.path.join(PKG, "fortran")
+FORTRAN_EXE = os.path.join(FORTRAN, "izh_lattice")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """The Fortran 90 host (iso_c_binding module + lattice driver), linked
+    against the library with an rpath to it."""
+    objdir = os.path.join(PKG, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in ("impactz_hip.f90", "izh_lattice.f90"):
+        obj = os.path.join(objdir, src.replace(".f90", ".o"))
+        cmd = [FC, "-O2", "-module-dir", objdir, "-c", os.path.join(FORTRAN, src),
+               "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    cmd = [FC, "-O2", *objs, "-L", PKG, "-limpactz_hip", "-Wl,-rpath,$ORIGIN/..",
+           "-o", FORTRAN_EXE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return FORTRAN_EXE
+
+
 def build(verbose: bool = True, jobs: int = 4) -> str:
     """Compile every translation unit, then link the shared library."""
     objdir = os.path.join(PKG, "_build")
@@ -52,6 +79,8 @@ def build(verbose: bool = True, jobs: int = 4) -> str:
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
+    if os.access(FC, os.X_OK):
+        build_fortran(verbose)
     return LIB
 
 

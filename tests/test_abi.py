@@ -25,6 +25,19 @@ def test_library_exports_every_declared_symbol():
     assert sorted(hip.SYMBOLS) == declared
 
 
+def test_fortran_module_binds_every_declared_symbol():
+    """impact-z_amd/fortran/impactz_hip.f90 has a bind(C) interface for every
+    entry point of the header, and nothing else under that prefix."""
+    src = open(os.path.join(ROOT, "impact-z_amd", "fortran", "impactz_hip.f90")).read()
+    bound = sorted(set(re.findall(r'bind\(C,\s*name="(izh_[A-Za-z0-9_]+)"\)', src)))
+    assert bound == header_symbols()
+
+
+def test_fortran_host_is_built():
+    from impactz import _build
+    assert os.access(_build.FORTRAN_EXE, os.X_OK)
+
+
 def test_physconst_matches_oracle(oracle):
     from impactz import hip
     c = hip.physconst(324e6)

@@ -1,0 +1,66 @@
+"""The Fortran 90 host (impact-z_amd/fortran) against the reference.
+
+izh_lattice calls the C ABI through the iso_c_binding module in the order of
+run_AccSimulator's segment loop (or through izh_run); its per-step moment sums
+must reproduce the reference's fort.24/25/26 rows to 1e-10 relative."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import diag_row_err, fixture_beam, fixture_lattice, load_golden
+from test_gpu_parity import TOL, record
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "impact-z_amd", "fortran", "izh_lattice")
+
+
+def _namelist(path, d, fused, out):
+    bm = fixture_beam(d)
+    lat = fixture_lattice(d)
+    nx, ny, nz = (int(v) for v in d["grid"])
+    rows = [f"  nx={nx}, ny={ny}, nz={nz}, precision=0, device=0,",
+            f"  current={bm.current!r}, kinetic={bm.kinetic!r}, mass={bm.mass!r},",
+            f"  charge={bm.charge!r}, freq={bm.freq!r}, phase0={bm.phase0!r},",
+            f"  nelem={len(lat)}, fused={'.true.' if fused else '.false.'},",
+            f"  particles='{path}/particle.in', output='{out}',"]
+    for k, e in enumerate(lat, start=1):
+        grad = e.v1 if e.type == 1 else 0.0
+        rows.append(f"  elen({k})={e.length!r}, nseg({k})={e.nseg}, mapstp({k})={e.mapstp},"
+                    f" bitype({k})={e.type}, grad({k})={grad!r}, radius({k})={e.radius!r},")
+    with open(os.path.join(path, "run.nml"), "w") as f:
+        f.write("&izh_run_nml\n" + "\n".join(rows) + "\n/\n")
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_fortran_host_matches_reference(tmp_path, fused):
+    from impactz import hip
+    from impactz.beam import write_particle_in
+    from impactz.sim import diag_row
+    assert os.access(EXE, os.X_OK), "Fortran host not built (impactz._build.build)"
+    d = load_golden("fodo_traj")
+    write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
+    out = str(tmp_path / "moments.txt")
+    _namelist(str(tmp_path), d, fused, out)
+    print(f"fortran host fodo_traj fused={fused}", flush=True)
+    subprocess.run([EXE, str(tmp_path / "run.nml")], cwd=str(tmp_path), check=True,
+                   timeout=120)
+    rec = np.loadtxt(out, ndmin=2)
+    nsteps = int(d["nsteps"])
+    assert rec.shape == (nsteps, 47)
+    bm = fixture_beam(d)
+    c = hip.physconst(bm.freq)
+    refpt = np.zeros(6) - bm.gamma0
+    worst = 0.0
+    for k in range(nsteps):
+        row = diag_row(rec[k, 3:40], rec[k, 40:46], rec[k, 46], int(rec[k, 2]), refpt,
+                       bm.mass, c.scxl, c.pi)
+        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
+                              d["fort26"][k + 1, 1:]])
+        worst = max(worst, diag_row_err(row, ref, ref_is_fort=True))
+    assert abs(rec[-1, 1] - d["fort24"][nsteps, 0]) < 1e-12
+    record(test="fortran_host", case="fodo_traj", fused=fused, steps=nsteps, moments=worst)
+    assert worst < TOL
