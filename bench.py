@@ -56,29 +56,38 @@ def algorithmic_bytes(kernel: str, n: int, g: int, form: str) -> float:
     raise ValueError(kernel)
 
 
-def run_form(pts, beam, lattice, grid, precision, steps, warmup):
+def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps):
     """Warm up, then time `steps` space-charge steps run through izh_run
     (each step: both half pushes, z->t, domain, deposit, Green function,
-    FFT solve, E, gather/kick and the 37 moment sums, all on the device)."""
+    FFT solve, E, gather/kick and the 37 moment sums, all on the device).
+    Afterwards the same bunch continues untimed to `accuracy_steps`, so the
+    two forms can be compared step by step from the same start."""
     form = "fp64" if precision == hip.FP64 else "mixed"
     n = pts.shape[0]
     sim = Simulation(pts, beam, lattice, grid, precision=precision)
+    recs = []
     log(f"{form}: warmup {warmup} steps")
-    sim.run_fused(warmup, start=0)
+    recs.append(sim.run_fused(warmup, start=0))
     hip.timer_enable(True)
     hip.timer_read(reset=True)
     log(f"{form}: timing {steps} steps with kernel timers")
     t0 = time.perf_counter()
-    sim.run_fused(steps, start=warmup)
+    recs.append(sim.run_fused(steps, start=warmup))
     dt_timed = time.perf_counter() - t0
     timers = hip.timer_read(reset=True)
     hip.timer_enable(False)
     # the throughput clock: the same number of steps without kernel events
     log(f"{form}: timing {steps} steps")
     t0 = time.perf_counter()
-    rec = sim.run_fused(steps, start=warmup + steps)
+    recs.append(sim.run_fused(steps, start=warmup + steps))
     dt = time.perf_counter() - t0
-    last = sim.diag(rec[-1])
+    done = warmup + 2 * steps
+    if accuracy_steps > done:
+        log(f"{form}: continuing to step {accuracy_steps} (untimed, for the error report)")
+        recs.append(sim.run_fused(accuracy_steps - done, start=done))
+    rec = np.concatenate([r for r in recs if r is not None and len(r)])
+    rows = np.array([sim.diag(r) for r in rec])
+    efield = np.stack([sim.field.download(w) for w in (hip.EX, hip.EY, hip.EZ)])
     sim.close()
     return {
         "form": form,
@@ -87,8 +96,29 @@ def run_form(pts, beam, lattice, grid, precision, steps, warmup):
         "ms_per_step_with_timers": 1e3 * dt_timed / steps,
         "kernels_ms": {k: v[0] / max(v[1], 1) for k, v in timers.items() if v[1]},
         "kernels_launches": {k: v[1] for k, v in timers.items() if v[1]},
-        "diag": last,
+        "rows": rows,
+        "efield": efield,
     }
+
+
+# rms columns of a diagnostic row (impactz.sim.diag_row order): x, px, y, py,
+# z, pz rms of fort.24/25/26
+RMS_COLS = [6, 8, 12, 14, 18, 20]
+
+
+def mixed_error(r64, r32):
+    """The mixed form against FP64 from the same start: relative rms-moment
+    difference per step (max over the six rms columns) and the relative L2
+    difference of (Ex, Ey, Ez) after the last step."""
+    a, b = r64["rows"], r32["rows"]
+    k = min(len(a), len(b))
+    drift = np.max(np.abs(b[:k, RMS_COLS] - a[:k, RMS_COLS]) / np.abs(a[:k, RMS_COLS]), axis=1)
+    e64, e32 = r64["efield"], r32["efield"]
+    return {"steps": int(k),
+            "rms_rel_at_last_step": float(drift[-1]),
+            "rms_rel_max_over_steps": float(drift.max()),
+            "field_l2_rel_at_last_step": float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64)),
+            "target": "rms <= 1e-4 relative after 400 steps"}
 
 
 def roofline(res, n, g):
@@ -163,6 +193,9 @@ def main():
     ap.add_argument("--grid", default="128,128,256")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=400,
+                    help="steps both forms run (untimed past the timed ones) for the "
+                         "mixed-vs-FP64 error report")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the multi-GPU path is a "
@@ -173,13 +206,15 @@ def main():
     g = grid[0] * grid[1] * grid[2]
     hip.init(0)
     beam = B.Beam()
-    lattice = B.fodo(25 * (1 + (2 * args.steps + args.warmup) // 400))  # >= 400 steps
+    total = max(2 * args.steps + args.warmup, args.accuracy_steps)
+    lattice = B.fodo(25 * (1 + (total - 1) // 400))  # 16 steps per period
     log(f"sampling {n} particles")
     pts = B.core_halo_bunch(n, seed=1)
     res = {}
     for form in args.forms.split(","):
         prec = hip.FP64 if form == "fp64" else hip.MIXED
-        res[form] = run_form(pts, beam, lattice, grid, prec, args.steps, args.warmup)
+        res[form] = run_form(pts, beam, lattice, grid, prec, args.steps, args.warmup,
+                             args.accuracy_steps)
         log(f"{form}: {res[form]['ms_per_step']:.3f} ms/step, "
             f"{res[form]['value']:.3e} particle-steps/s, kernels {res[form]['kernels_ms']}")
     del pts
@@ -206,11 +241,11 @@ def main():
     }
     if "mixed" in res and main_form != "mixed":
         m = res["mixed"]
-        rows64, rows32 = r["diag"], m["diag"]
         out["mixed"] = {
             "value": m["value"], "ms_per_step": m["ms_per_step"],
             "speedup_vs_fp64": m["value"] / r["value"],
             "roofline": roofline(m, n, g), "kernels_ms": m["kernels_ms"],
+            "error_vs_fp64": mixed_error(r, m),
         }
     if not args.no_cpu:
         cb = cpu_baseline(grid, n)

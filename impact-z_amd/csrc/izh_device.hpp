@@ -146,10 +146,10 @@ __device__ inline void pmom_add(PairMom &a, double x, double px, double y, doubl
   a.m[3] = fmax(a.m[3], x * x + y * y);
 }
 
-// Block-reduce PairMom into the kMoments values of Mom's layout at out.
-// sh: [blockDim/64][kMoments] in LDS.
+// Block-reduce PairMom into the kMoments values of Mom's layout, value k at
+// out[k * ostride].  sh: [blockDim/64][kMoments] in LDS.
 __device__ inline void pmom_block_store(const PairMom &a, double (*sh)[kMoments],
-                                        double *out) {
+                                        double *out, int64_t ostride) {
   constexpr int kx[14] = {0, 6, 15, 16, 12, 27, 28, 29, 30, 31, 1, 7, 17, 18};
   constexpr int ky[14] = {2, 8, 19, 20, 13, 32, 33, 34, 35, 36, 3, 9, 21, 22};
   constexpr int kz[4] = {4, 10, 23, 24}, kp[4] = {5, 11, 25, 26};
@@ -188,16 +188,17 @@ __device__ inline void pmom_block_store(const PairMom &a, double (*sh)[kMoments]
     const int k = threadIdx.x;
     double r = sh[0][k];
     for (int w = 1; w < nw; w++) r = (k < 37) ? r + sh[w][k] : fmax(r, sh[w][k]);
-    out[k] = r;
+    out[k * ostride] = r;
   }
 }
 
-// 8-node CIC gather of interleaved (Ex, Ey, Ez, 0) at lower-node weights
-// ab, cd, ef, summed in the term order of scatter1_BeamBunch
-// (src/Appl/BeamBunch.f90:2130-2155).
-template <typename T>
-__device__ inline void cic8_e4(const E4<T> *__restrict__ e, int64_t o, int64_t sy,
-                               int64_t sz, T ab, T cd, T ef, T &gx, T &gy, T &gz) {
+// 8-node CIC gather at lower-node weights ab, cd, ef, summed in the term
+// order of scatter1_BeamBunch (src/Appl/BeamBunch.f90:2130-2155).  ld(q, x,
+// y, z) loads the field of node q; o is the lower node, sy and sz the node
+// strides in y and z.
+template <typename T, typename Load>
+__device__ inline void cic8(Load ld, int64_t o, int64_t sy, int64_t sz, T ab, T cd,
+                            T ef, T &gx, T &gy, T &gz) {
   const T one = (T)1;
   const int64_t idx[8] = {o, o + sy, o + sy + sz, o + sz, o + 1 + sz,
                           o + 1 + sy + sz, o + 1 + sy, o + 1};
@@ -212,14 +213,26 @@ __device__ inline void cic8_e4(const E4<T> *__restrict__ e, int64_t o, int64_t s
   T sx = (T)0, syy = (T)0, szz = (T)0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
-    E4<T> v = e[idx[c]];
-    sx += v.x * w[c];
-    syy += v.y * w[c];
-    szz += v.z * w[c];
+    T vx, vy, vz;
+    ld(idx[c], vx, vy, vz);
+    sx += vx * w[c];
+    syy += vy * w[c];
+    szz += vz * w[c];
   }
   gx = sx;
   gy = syy;
   gz = szz;
+}
+
+// The gather from interleaved (Ex, Ey, Ez, 0) in global memory.
+template <typename T>
+__device__ inline void cic8_e4(const E4<T> *__restrict__ e, int64_t o, int64_t sy,
+                               int64_t sz, T ab, T cd, T ef, T &gx, T &gy, T &gz) {
+  cic8<T>([e](int64_t q, T &x, T &y, T &z) {
+            const E4<T> v = e[q];
+            x = v.x; y = v.y; z = v.z;
+          },
+          o, sy, sz, ab, cd, ef, gx, gy, gz);
 }
 
 // Single-block reduction of per-block partials (nq <= 64 quantities each) in

@@ -61,45 +61,70 @@ __device__ inline int bitrev(int v, int bits) {
   return (int)(__brev((unsigned)v) >> (32 - bits));
 }
 
-// In-place radix-2 decimation in frequency: natural in, bit-reversed out.
-template <typename T>
-__device__ void fft_dif(cx<T> *buf, const Lines &p, const cx<T> *__restrict__ tw,
-                        bool inverse) {
-  const int L = p.L, nbf = p.B * (L >> 1);
-  for (int half = L >> 1; half >= 1; half >>= 1) {
-    const int tstride = L / (2 * half);
-    for (int t = threadIdx.x; t < nbf; t += kFftThreads) {
-      int lb = t / (L >> 1), j = t - lb * (L >> 1);
-      int k = j & (half - 1), g = j / half;
-      int i0 = lb * L + g * 2 * half + k, i1 = i0 + half;
-      cx<T> w = tw[k * tstride];
-      if (inverse) w.im = -w.im;
-      cx<T> u = buf[i0], v = buf[i1];
-      buf[i0] = {u.re + v.re, u.im + v.im};
-      buf[i1] = cmul<T>({u.re - v.re, u.im - v.im}, w);
+// In-place radix-2 decimation in frequency (natural in, bit-reversed out) and
+// decimation in time (bit-reversed in, natural out) over the B lines of a
+// block.  Up to three consecutive radix-2 stages run in registers on groups
+// of 8 elements: the elements a run of stages combines form closed groups
+// (stride = the run's smallest half), so each group is read from LDS once per
+// three stages.  Every element sees the same butterflies with the same
+// twiddles, in the same order, as the stage-by-stage transform.  tw: L/2
+// twiddles exp(-2 pi i k / L) in LDS.
+template <typename T, bool DIT, int K>
+__device__ inline void fft_run(cx<T> *buf, int L, int lines, const cx<T> *tw, bool inverse,
+                               int done) {
+  constexpr int span = 1 << K;
+  // DIF runs halves L/2 .. 1 downwards, DIT 1 .. L/2 upwards
+  const int s = DIT ? (1 << done) : (L >> (done + K));  // group stride
+  const int bsize = s * span;                          // block of a group
+  const int per_line = L >> K;
+  const int ngroups = lines * per_line;
+  for (int g = threadIdx.x; g < ngroups; g += kFftThreads) {
+    const int lb = g / per_line, q = g - lb * per_line;
+    const int blk = q / s, r = q - blk * s;
+    const int base = lb * L + blk * bsize + r;
+    cx<T> v[span];
+#pragma unroll
+    for (int j = 0; j < span; j++) v[j] = buf[base + j * s];
+#pragma unroll
+    for (int st = 0; st < K; st++) {
+      const int e = DIT ? st : K - 1 - st;  // pair distance 2^e in the group
+      const int tstride = L / (2 * (s << e));
+#pragma unroll
+      for (int j = 0; j < span; j++) {
+        if ((j >> e) & 1) continue;
+        const int k = r + (j & ((1 << e) - 1)) * s;
+        cx<T> w = tw[k * tstride];
+        if (inverse) w.im = -w.im;
+        const cx<T> u = v[j], x = v[j + (1 << e)];
+        if (DIT) {
+          const cx<T> y = cmul<T>(x, w);
+          v[j] = {u.re + y.re, u.im + y.im};
+          v[j + (1 << e)] = {u.re - y.re, u.im - y.im};
+        } else {
+          v[j] = {u.re + x.re, u.im + x.im};
+          v[j + (1 << e)] = cmul<T>({u.re - x.re, u.im - x.im}, w);
+        }
+      }
     }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < span; j++) buf[base + j * s] = v[j];
   }
+  __syncthreads();
 }
 
-// In-place radix-2 decimation in time: bit-reversed in, natural out.
-template <typename T>
-__device__ void fft_dit(cx<T> *buf, const Lines &p, const cx<T> *__restrict__ tw,
-                        bool inverse) {
-  const int L = p.L, nbf = p.B * (L >> 1);
-  for (int half = 1; half < L; half <<= 1) {
-    const int tstride = L / (2 * half);
-    for (int t = threadIdx.x; t < nbf; t += kFftThreads) {
-      int lb = t / (L >> 1), j = t - lb * (L >> 1);
-      int k = j & (half - 1), g = j / half;
-      int i0 = lb * L + g * 2 * half + k, i1 = i0 + half;
-      cx<T> w = tw[k * tstride];
-      if (inverse) w.im = -w.im;
-      cx<T> u = buf[i0], v = cmul<T>(buf[i1], w);
-      buf[i0] = {u.re + v.re, u.im + v.im};
-      buf[i1] = {u.re - v.re, u.im - v.im};
-    }
-    __syncthreads();
+template <typename T, bool DIT>
+__device__ void fft_stages(cx<T> *buf, int L, int log2L, int lines, const cx<T> *tw,
+                           bool inverse) {
+  int done = 0;
+  while (done < log2L) {
+    const int k = min(3, log2L - done);
+    if (k == 3)
+      fft_run<T, DIT, 3>(buf, L, lines, tw, inverse, done);
+    else if (k == 2)
+      fft_run<T, DIT, 2>(buf, L, lines, tw, inverse, done);
+    else
+      fft_run<T, DIT, 1>(buf, L, lines, tw, inverse, done);
+    done += k;
   }
 }
 
@@ -119,59 +144,100 @@ __device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &l
 
 template <typename T, int LOAD, int STORE, bool CONV>
 __global__ void __launch_bounds__(kFftThreads)
-k_fft_lines(Lines p, const cx<T> *__restrict__ tw, int inverse) {
+k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   __shared__ cx<T> buf[kFftPoints];
+  extern __shared__ __align__(16) unsigned char fft_dyn[];  // L/2 twiddles
+  cx<T> *tw = reinterpret_cast<cx<T> *>(fft_dyn);
+  // twiddles are written to LDS together with the line data below
+  constexpr int kTwPer = kFftPoints / 2 / kFftThreads;  // L/2 <= kFftPoints/2
+  cx<T> twv[kTwPer];
+#pragma unroll
+  for (int it = 0; it < kTwPer; it++) {
+    const int t = threadIdx.x + it * kFftThreads;
+    T re = (T)0, im = (T)0;
+    if (t < (p.L >> 1)) {
+      const cx<T> v = twg[t];
+      re = v.re; im = v.im;
+    }
+    twv[it] = {re, im};
+  }
   const int nf = (int)p.nf;
   const int nlines = nf * (int)p.ns;
   const int line0 = blockIdx.x * p.B;
-  const int tot = p.B * p.L;
-  // ---- load (zero padding / even extension / Hermitian completion)
-  for (int t = threadIdx.x; t < tot; t += kFftThreads) {
+  // ---- load (zero padding / even extension / Hermitian completion): all of
+  // a thread's loads are issued before the first LDS write
+  constexpr int kPer = kFftPoints / kFftThreads;  // B * L == kFftPoints
+  cx<T> ld[kPer];
+  int slot[kPer];
+#pragma unroll
+  for (int it = 0; it < kPer; it++) {
+    const int t = threadIdx.x + it * kFftThreads;
     int lb, m;
     map_thread(t, p.se, p, lb, m);
-    int line = line0 + lb;
-    cx<T> v = {(T)0, (T)0};
+    const int line = line0 + lb;
+    slot[it] = lb * p.L + m;
+    T re = (T)0, im = (T)0;
     if (line < nlines) {
-      int f = line % nf, s = line / nf;
-      int64_t base = f * p.sf + s * p.ss;
+      const int f = line % nf, s = line / nf;
+      const int64_t base = f * p.sf + s * p.ss;
       if (LOAD == kLoadReal) {
-        if (m < p.Lin) v.re = ((const T *)p.src)[base + m * p.se];
+        if (m < p.Lin) re = ((const T *)p.src)[base + m * p.se];
       } else if (LOAD == kLoadCplx) {
-        if (m < p.Lin) v = ((const cx<T> *)p.src)[base + m * p.se];
-      } else if (LOAD == kLoadEven) {
-        int mm = m <= (p.L >> 1) ? m : p.L - m;
-        v.re = ((const T *)p.src)[base + mm * p.se];
-      } else {  // kLoadHerm: first L/2+1 values stored
-        if (m <= (p.L >> 1)) {
-          v = ((const cx<T> *)p.src)[base + m * p.se];
-        } else {
-          v = ((const cx<T> *)p.src)[base + (p.L - m) * p.se];
-          v.im = -v.im;
+        if (m < p.Lin) {
+          const cx<T> v = ((const cx<T> *)p.src)[base + m * p.se];
+          re = v.re; im = v.im;
         }
+      } else if (LOAD == kLoadEven) {
+        const int mm = m <= (p.L >> 1) ? m : p.L - m;
+        re = ((const T *)p.src)[base + mm * p.se];
+      } else {  // kLoadHerm: first L/2+1 values stored
+        const bool lo = m <= (p.L >> 1);
+        const cx<T> v = ((const cx<T> *)p.src)[base + (lo ? m : p.L - m) * p.se];
+        re = v.re; im = lo ? v.im : -v.im;
       }
     }
-    buf[lb * p.L + m] = v;
+    ld[it] = {re, im};
+  }
+#pragma unroll
+  for (int it = 0; it < kPer; it++) buf[slot[it]] = ld[it];
+#pragma unroll
+  for (int it = 0; it < kTwPer; it++) {
+    const int t = threadIdx.x + it * kFftThreads;
+    if (t < (p.L >> 1)) tw[t] = twv[it];
   }
   __syncthreads();
-  fft_dif<T>(buf, p, tw, inverse != 0);
+  // fused z pass: fetch the line's G^ values now, so their latency hides
+  // behind the forward transform.  The forward spectrum is bit-reversed:
+  // position q holds frequency bitrev(q).
+  T gv[CONV ? kPer : 1];
   if (CONV) {
-    // forward z spectrum is bit-reversed: position q holds frequency bitrev(q)
     const T *G = (const T *)p.green;
-    for (int t = threadIdx.x; t < tot; t += kFftThreads) {
-      int lb = t >> p.log2L, q = t & (p.L - 1);
-      int line = line0 + lb;
+#pragma unroll
+    for (int it = 0; it < kPer; it++) {
+      const int t = threadIdx.x + it * kFftThreads;
+      const int lb = t >> p.log2L, q = t & (p.L - 1);
+      const int line = line0 + lb;
+      T g = (T)0;
       if (line < nlines) {
-        int i = line % nf, j2 = line / nf;
-        int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
-        int kz = bitrev(q, p.log2L);
-        int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
-        T g = G[i + p.gkx * jf + p.gky * kf];
-        cx<T> v = buf[t];
-        buf[t] = {v.re * g, v.im * g};
+        const int i = line % nf, j2 = line / nf;
+        const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
+        const int kz = bitrev(q, p.log2L);
+        const int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
+        g = G[i + p.gkx * jf + p.gky * kf];
       }
+      gv[it] = g;
+    }
+  }
+  fft_stages<T, false>(buf, p.L, p.log2L, p.B, tw, inverse != 0);
+  if (CONV) {
+#pragma unroll
+    for (int it = 0; it < kPer; it++) {
+      const int t = threadIdx.x + it * kFftThreads;
+      const cx<T> v = buf[t];
+      buf[t] = {v.re * gv[it], v.im * gv[it]};
     }
     __syncthreads();
-    fft_dit<T>(buf, p, tw, true);
+    fft_stages<T, true>(buf, p.L, p.log2L, p.B, tw, true);
   }
   // ---- store the kept part of each line
   const int totout = p.B * p.L;
@@ -309,23 +375,14 @@ __global__ void __launch_bounds__(kBlock)
 k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
                  const P *__restrict__ Z, const uint32_t *__restrict__ orig,
                  const double *__restrict__ w0, double w_uniform, int uniform,
-                 const Chunk *__restrict__ chunks, const int64_t *__restrict__ d_nchunks,
                  int64_t n_flat, const DevDom *__restrict__ dom, int nx, int ny,
-                 int nz, double scale, unsigned long long *__restrict__ rho,
-                 int64_t *__restrict__ overflow) {
+                 int nz, double scale, unsigned long long *__restrict__ rho) {
   __shared__ unsigned long long win[kWindowCells];
   __shared__ int red[kBlock / 64][6];
   __shared__ int box[6];
-  int64_t start, len;
-  if (chunks) {
-    if (blockIdx.x >= *d_nchunks) return;
-    start = chunks[blockIdx.x].start;
-    len = chunks[blockIdx.x].len;
-  } else {
-    start = (int64_t)blockIdx.x * kChunk;
-    if (start >= n_flat) return;
-    len = min((int64_t)kChunk, n_flat - start);
-  }
+  const int64_t start = (int64_t)blockIdx.x * kChunk;
+  if (start >= n_flat) return;
+  const int64_t len = min((int64_t)kChunk, n_flat - start);
   const P xmin = (P)dom->range[0], ymin = (P)dom->range[2], zmin = (P)dom->range[4];
   const P hx = (P)dom->msize[0], hy = (P)dom->msize[1], hz = (P)dom->msize[2];
   const P hxi = (P)dom->hinv[0], hyi = (P)dom->hinv[1], hzi = (P)dom->hinv[2];
@@ -362,8 +419,6 @@ k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
   if (local) {
     for (int c = threadIdx.x; c < wcells; c += kBlock) win[c] = 0ull;
     __syncthreads();
-  } else if (threadIdx.x == 0 && overflow) {
-    atomicAdd((unsigned long long *)overflow, 1ull);
   }
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   for (int64_t t = threadIdx.x; t < len; t += kBlock) {
@@ -398,6 +453,88 @@ k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
     if (v == 0ull) continue;
     int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
     atomicAdd(&rho[(bx + lx) + sy * (by + ly) + sz * (bz + lz)], v);
+  }
+}
+
+// The fused run's deposit.  Each block takes one chunk of the sorted bunch:
+// at most kChunk slots grouped under one sort tile (the tile of the
+// particles' previous position).  The block accumulates in an LDS window of
+// the tile's nodes widened by kTileMargin nodes on every side (plus the
+// upper CIC node), so no pass over the positions is needed to bound them;
+// particles that moved further add to the grid directly and are counted in
+// *spill.  Same fixed-point arithmetic and node order as k_deposit_chunks.
+constexpr int kTileMargin = 3;
+constexpr int kTileWinEdge = (1 << kTileShift) + 1 + 2 * kTileMargin;
+constexpr int kTileWinCells = kTileWinEdge * kTileWinEdge * kTileWinEdge;
+
+template <typename P>
+__global__ void __launch_bounds__(kBlock)
+k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
+                const P *__restrict__ Z, const uint32_t *__restrict__ orig,
+                const double *__restrict__ w0, double w_uniform, int uniform,
+                const Chunk *__restrict__ chunks, const int64_t *__restrict__ d_nchunks,
+                const DevDom *__restrict__ dom, int nx, int ny, int nz, int s0, int s1,
+                int s2, int tnx, int tny, double scale,
+                unsigned long long *__restrict__ rho, int64_t *__restrict__ spill) {
+  __shared__ unsigned long long win[kTileWinCells];
+  if (blockIdx.x >= *d_nchunks) return;
+  const Chunk ch = chunks[blockIdx.x];
+  const int tz = (int)(ch.tile / ((uint32_t)tnx * (uint32_t)tny));
+  const int rem = (int)(ch.tile % ((uint32_t)tnx * (uint32_t)tny));
+  const int ty = rem / tnx, tx = rem % tnx;
+  const int wx = min((1 << s0) + 1 + 2 * kTileMargin, nx);
+  const int wy = min((1 << s1) + 1 + 2 * kTileMargin, ny);
+  const int wz = min((1 << s2) + 1 + 2 * kTileMargin, nz);
+  const int bx = min(max((tx << s0) - kTileMargin, 0), nx - wx);
+  const int by = min(max((ty << s1) - kTileMargin, 0), ny - wy);
+  const int bz = min(max((tz << s2) - kTileMargin, 0), nz - wz);
+  const int wcells = wx * wy * wz;
+  for (int c = threadIdx.x; c < wcells; c += kBlock) win[c] = 0ull;
+  __syncthreads();
+  const P xmin = (P)dom->range[0], ymin = (P)dom->range[2], zmin = (P)dom->range[4];
+  const P hx = (P)dom->msize[0], hy = (P)dom->msize[1], hz = (P)dom->msize[2];
+  const P hxi = (P)dom->hinv[0], hyi = (P)dom->hinv[1], hzi = (P)dom->hinv[2];
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  uint32_t nspill = 0;
+  for (uint32_t t = threadIdx.x; t < ch.len; t += kBlock) {
+    const int64_t i = (int64_t)ch.start + t;
+    const P x = X[i];
+    if (!(x == x)) continue;
+    const Cic<P> c = cic_of<P>(x, Y[i], Z[i], xmin, ymin, zmin, hx, hy, hz, hxi, hyi, hzi);
+    if (c.i < 0 || c.i + 1 >= nx || c.j < 0 || c.j + 1 >= ny || c.k < 0 ||
+        c.k + 1 >= nz)
+      continue;
+    const double w = uniform ? w_uniform : w0[orig[i]];
+    long long v[8];
+    cic_contrib<P>(c, w * scale, v);
+    const int li = c.i - bx, lj = c.j - by, lk = c.k - bz;
+    if (li >= 0 && li + 1 < wx && lj >= 0 && lj + 1 < wy && lk >= 0 && lk + 1 < wz) {
+      const int l0 = li + wx * (lj + wy * lk);
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&win[l0 + kNode[q][0] + wx * (kNode[q][1] + wy * kNode[q][2])],
+                  (unsigned long long)v[q]);
+    } else {
+      const int64_t g0 = c.i + sy * c.j + sz * c.k;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        atomicAdd(&rho[g0 + kNode[q][0] + sy * kNode[q][1] + sz * kNode[q][2]],
+                  (unsigned long long)v[q]);
+      nspill++;
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < wcells; c += kBlock) {
+    const unsigned long long v = win[c];
+    if (v == 0ull) continue;
+    const int lx = c % wx, ly = (c / wx) % wy, lz = c / (wx * wy);
+    atomicAdd(&rho[(bx + lx) + sy * (by + ly) + sz * (bz + lz)], v);
+  }
+  if (spill) {
+    int tot = (int)nspill;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_down(tot, off, 64);
+    if ((threadIdx.x & 63) == 0 && tot > 0)
+      atomicAdd((unsigned long long *)spill, (unsigned long long)tot);
   }
 }
 
@@ -452,8 +589,9 @@ int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
   p.B = kFftPoints / p.L;
   int64_t nlines = p.nf * p.ns;
   int64_t nblk = (nlines + p.B - 1) / p.B;
+  const size_t tw_bytes = (size_t)(p.L / 2) * sizeof(cx<T>);
   hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV>), dim3((unsigned)nblk),
-                     dim3(kFftThreads), 0, f->stream, p, (const cx<T> *)tw,
+                     dim3(kFftThreads), tw_bytes, f->stream, p, (const cx<T> *)tw,
                      inverse ? 1 : 0);
   IZH_HIP(hipGetLastError());
   return IZH_OK;
@@ -610,33 +748,43 @@ int field_set_domain(izh_field *f, const izh_compdom *dom) {
   return IZH_OK;
 }
 
-// Zero the accumulator, deposit the chunks, convert to the grid type.
-// positions: 0 = the bunch's phase-space rows (t frame, FP64),
-//            1 = the fused run's t-frame position buffer (T).
+// Zero the accumulator, deposit, convert to the grid type.  With a chunk
+// list (the fused run) the positions are the t-frame buffer (T) of the
+// tile-sorted bunch; without one, the bunch's own rows (t frame, FP64) in
+// chunks of kChunk consecutive particles.
 int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
-                   const int64_t *d_nchunks, int64_t max_chunks, int positions) {
+                   const int64_t *d_nchunks, int64_t max_chunks) {
   f->fixed_scale = fixed_point_scale(b->wabs_sum);
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(f->g)), dim3(kBlock), 0, f->stream,
                      f->rho_fx, f->g);
   if (max_chunks > 0) {
     ScopedTimer tm_(IZH_T_DEPOSIT, f->stream);
-    int64_t *ovf = b->d_scal ? &b->d_scal->overflow : nullptr;
-    if (positions == 1 && f->precision == IZH_MIXED) {
-      const float *tp = (const float *)b->tpos;
-      hipLaunchKernelGGL(k_deposit_chunks<float>, dim3((unsigned)max_chunks),
-                         dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
-                         b->orig, b->w0, b->w_value, b->w_uniform, chunks,
-                         d_nchunks, b->n, f->d_dom, f->nx, f->ny, f->nz,
-                         f->fixed_scale, f->rho_fx, ovf);
+    if (chunks) {
+      // the fused run: chunks of the tile-sorted bunch, t positions in T
+      const int s0 = tile_shift(f->nx), s1 = tile_shift(f->ny), s2 = tile_shift(f->nz);
+      const int tnx = f->nx >> s0, tny = f->ny >> s1;
+      int64_t *spill = b->d_scal ? &b->d_scal->overflow : nullptr;
+      if (f->precision == IZH_MIXED) {
+        const float *tp = (const float *)b->tpos;
+        hipLaunchKernelGGL(k_deposit_tiles<float>, dim3((unsigned)max_chunks),
+                           dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
+                           b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
+                           f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
+                           f->fixed_scale, f->rho_fx, spill);
+      } else {
+        const double *tp = (const double *)b->tpos;
+        hipLaunchKernelGGL(k_deposit_tiles<double>, dim3((unsigned)max_chunks),
+                           dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
+                           b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
+                           f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
+                           f->fixed_scale, f->rho_fx, spill);
+      }
     } else {
-      const double *X = positions == 1 ? (const double *)b->tpos : b->r[0];
-      const double *Y = positions == 1 ? X + b->cap : b->r[2];
-      const double *Z = positions == 1 ? X + 2 * b->cap : b->r[4];
+      // one call's bunch in the t frame, in its own (unsorted) order
       hipLaunchKernelGGL(k_deposit_chunks<double>, dim3((unsigned)max_chunks),
-                         dim3(kBlock), 0, f->stream, X, Y, Z, b->orig, b->w0,
-                         b->w_value, b->w_uniform, chunks, d_nchunks, b->n,
-                         f->d_dom, f->nx, f->ny, f->nz, f->fixed_scale, f->rho_fx,
-                         ovf);
+                         dim3(kBlock), 0, f->stream, b->r[0], b->r[2], b->r[4],
+                         b->orig, b->w0, b->w_value, b->w_uniform, b->n, f->d_dom,
+                         f->nx, f->ny, f->nz, f->fixed_scale, f->rho_fx);
     }
     IZH_HIP(hipGetLastError());
   }
@@ -658,7 +806,7 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
 
 int launch_deposit_granular(izh_bunch *b, izh_field *f) {
   const int64_t nch = (b->n + kChunk - 1) / kChunk;
-  return launch_deposit(b, f, nullptr, nullptr, nch, 0);
+  return launch_deposit(b, f, nullptr, nullptr, nch);
 }
 
 int launch_solve(izh_field *f) {

@@ -61,7 +61,7 @@ struct PassCfg {
   double tau2_prev;         // 2*tau1 of step s-1
   double inv_rho_scale;     // undo the grid-charge scaling in the kick
   HalfPush pb, pa;          // second half of step s-1, first half of step s
-  double *part;             // per block: kMoments moment values, kConvRed conv
+  double *part;             // kMoments moment values, kConvRed conv, per block
 };
 
 // drift1_BeamBunch (src/Appl/BeamBunch.f90:276-320) for one particle.  The
@@ -141,11 +141,48 @@ __device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
   hi = lo + per < n ? lo + per : n;
 }
 
+// E of a kGBox^3-node neighbourhood of the block's current sort tile, staged
+// in LDS so the 8-node gather of the dense core reads LDS instead of the
+// vector L1 (whose tag rate bounds a gather over 8 scattered nodes).
+// Particles whose stencil leaves the box read global memory; both paths sum
+// the same values in the same order.
+constexpr int kGBox = 12;
+constexpr int kGBoxCells = kGBox * kGBox * kGBox;
+constexpr int kGBoxLow = 2;  // nodes below the tile's first node
+
+template <typename T> struct GatherLds;
+template <> struct GatherLds<double> {
+  double2 xy[kGBoxCells];
+  double z[kGBoxCells];
+};
+template <> struct GatherLds<float> {
+  float4 e[kGBoxCells];
+};
+
+__device__ inline void box_store(GatherLds<double> &g, int q, const E4<double> &v) {
+  g.xy[q] = make_double2(v.x, v.y);
+  g.z[q] = v.z;
+}
+__device__ inline void box_store(GatherLds<float> &g, int q, const E4<float> &v) {
+  g.e[q] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+__device__ inline void box_load(const GatherLds<double> &g, int64_t q, double &x,
+                                double &y, double &z) {
+  const double2 v = g.xy[q];
+  x = v.x; y = v.y; z = g.z[q];
+}
+__device__ inline void box_load(const GatherLds<float> &g, int64_t q, float &x,
+                                float &y, float &z) {
+  const float4 v = g.e[q];
+  x = v.x; y = v.y; z = v.z;
+}
+
 // PB: push of the second half of step s-1 (kPushNone on the first pass, which
 // then has no kick and no moments); PA: first half of step s (kPushNone on
 // the last pass, which then has no z->t transform).
 template <typename T, int PB, int PA>
-__global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_pass(PassCfg c) {
   constexpr bool kKick = PB != kPushNone, kConv = PA != kPushNone;
   const int64_t nslots = *c.d_nslots;
   int64_t lo, hi;
@@ -161,6 +198,10 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   const int sy = c.nx, sz = c.nx * c.ny;
   const E4<T> *E = (const E4<T> *)c.e;
   T *tpx = (T *)c.tpos;
+  __shared__ GatherLds<T> gbox;
+  const int bdx = min(kGBox, c.nx), bdy = min(kGBox, c.ny), bdz = min(kGBox, c.nz);
+  uint32_t box_key = kDeadKey;   // tile the staged box was built for
+  int ox = 0, oy = 0, oz = 0;    // its first node
   PairMom mom;
   pmom_zero(mom);
   double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
@@ -180,6 +221,27 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     double x = r_n[0], px = r_n[1], y = r_n[2], py = r_n[3], ph = r_n[4], pt = r_n[5];
     const bool valid = kin != kDeadKey;
     nalive += valid ? 1u : 0u;
+    if constexpr (kKick) {
+      // restage when the iteration's first slot starts another tile
+      const uint32_t k0 = c.in_key[base];
+      if (kick && k0 != kDeadKey && k0 != box_key) {
+        const uint32_t txy = (uint32_t)c.tnx * (uint32_t)c.tny;
+        const int tz = (int)(k0 / txy), rem = (int)(k0 % txy);
+        const int ty = rem / c.tnx, tx = rem % c.tnx;
+        ox = min(max((tx << c.tshift[0]) - kGBoxLow, 0), c.nx - bdx);
+        oy = min(max((ty << c.tshift[1]) - kGBoxLow, 0), c.ny - bdy);
+        oz = min(max((tz << c.tshift[2]) - kGBoxLow, 0), c.nz - bdz);
+        __syncthreads();  // the previous box is no longer read
+        const int bxy = bdx * bdy, nb = bxy * bdz;
+        for (int q = threadIdx.x; q < nb; q += kBlock) {
+          const int lz = q / bxy, r2 = q - lz * bxy, ly = r2 / bdx, lx = r2 - ly * bdx;
+          box_store(gbox, q, E[(int64_t)(ox + lx) + (int64_t)sy * (oy + ly) +
+                               (int64_t)sz * (oz + lz)]);
+        }
+        __syncthreads();
+        box_key = k0;
+      }
+    }
     // counting-sort scatter by the key of the previous step
     const RankTicket tk = wave_rank_issue(c.cursor, kin, valid);
     if (i + kBlock < hi) {
@@ -208,7 +270,15 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
         kx = min(max(kx, 1), c.nz - 1);
         const int64_t o = (int64_t)(ix - 1) + (int64_t)sy * (jx - 1) + (int64_t)sz * (kx - 1);
         T gx, gy, gz;
-        cic8_e4<T>(E, o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
+        const int li = ix - 1 - ox, lj = jx - 1 - oy, lk = kx - 1 - oz;
+        if (box_key != kDeadKey && li >= 0 && li + 1 < bdx && lj >= 0 && lj + 1 < bdy &&
+            lk >= 0 && lk + 1 < bdz) {
+          cic8<T>([](int64_t q, T &ex, T &ey, T &ez) { box_load(gbox, q, ex, ey, ez); },
+                  (int64_t)li + (int64_t)bdx * (lj + bdy * lk), bdx, bdx * bdy,
+                  (T)ab, (T)cd, (T)ef, gx, gy, gz);
+        } else {
+          cic8_e4<T>(E, o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
+        }
         const double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
         x = X0 * xk;
         y = Y0 * xk;
@@ -279,11 +349,12 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     }
   }
   __shared__ double sh[kBlock / 64][kMoments];
-  double *out = c.part + (int64_t)blockIdx.x * kPartStride;
+  // partials are quantity-major: value q of block b at part[q * kMaxRedBlocks + b]
+  double *out = c.part + blockIdx.x;
   if constexpr (kKick) {
-    pmom_block_store(mom, sh, out);
+    pmom_block_store(mom, sh, out, kMaxRedBlocks);
   } else if (threadIdx.x < kMoments) {
-    out[threadIdx.x] = 0.0;
+    out[(int64_t)threadIdx.x * kMaxRedBlocks] = 0.0;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -299,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     double r = sh[0][k];
     for (int w = 1; w < kBlock / 64; w++)
       r = k < 3 ? fmin(r, sh[w][k]) : (k < 6 ? fmax(r, sh[w][k]) : r + sh[w][k]);
-    out[kMoments + k] = r;
+    out[(int64_t)(kMoments + k) * kMaxRedBlocks] = r;
   }
 }
 
@@ -438,7 +509,7 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
       const int op = !isconv ? (q < 37 ? 0 : 2) : (cq < 3 ? 1 : (cq < 6 ? 2 : 0));
       double acc = op == 1 ? 1e300 : (op == 2 ? -1e300 : 0.0);
       for (int b = lane; b < f.nparts; b += 64) {
-        const double v = f.part[(int64_t)b * kPartStride + q];
+        const double v = f.part[(int64_t)q * kMaxRedBlocks + b];
         acc = op == 1 ? fmin(acc, v) : (op == 2 ? fmax(acc, v) : acc + v);
       }
       acc = op == 1 ? wave_min(acc) : (op == 2 ? wave_max(acc) : wave_sum(acc));
@@ -529,6 +600,7 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
     for (uint32_t s = 0; s < h; s += kChunk) {
       f.chunks[coff].start = off + s;
       f.chunks[coff].len = min((uint32_t)kChunk, h - s);
+      f.chunks[coff].tile = (uint32_t)t;
       coff++;
     }
     off += h;
@@ -541,12 +613,6 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = 0u;
-}
-
-int tile_shift(int n) {
-  int s = 0;
-  while ((1 << (s + 1)) <= n && s < kTileShift) s++;
-  return s;
 }
 
 }  // namespace
@@ -745,7 +811,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     if (last) break;
     // charge, update3O and the E field of step s on the grouping just written
     rc = launch_deposit(b, f, b->chunks + (s & 1) * b->nchunks_cap,
-                        &b->d_scal->nchunks[s & 1], b->nchunks_cap, 1);
+                        &b->d_scal->nchunks[s & 1], b->nchunks_cap);
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
     // second half of step s: host bookkeeping now, applied by the next pass

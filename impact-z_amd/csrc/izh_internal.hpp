@@ -67,13 +67,21 @@ struct DevScal {
   int64_t nalive;     // particles alive after the last conv1st
   int64_t nslots;     // slots written by the last pass (alive + just lost)
   int64_t nchunks[2]; // deposit chunks of the two groupings in flight
-  int64_t overflow;   // deposit chunks that fell back to global atomics
+  int64_t overflow;   // deposited particles whose nodes left their LDS window
   int64_t err;        // scatter slots out of range (must stay 0)
 };
 
+// A run of at most kChunk consecutive slots of one sort tile.
 struct Chunk {
-  uint32_t start, len;
+  uint32_t start, len, tile;
 };
+
+// Sort tiles are 2^s nodes per axis (s <= kTileShift, 2^s <= n).
+inline int tile_shift(int n) {
+  int s = 0;
+  while ((1 << (s + 1)) <= n && s < kTileShift) s++;
+  return s;
+}
 
 }  // namespace izh
 
@@ -165,7 +173,7 @@ int launch_cvbk(izh_bunch *b);
 int launch_moments(izh_bunch *b, double out[kMoments]);
 int field_set_domain(izh_field *f, const izh_compdom *dom);
 int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
-                   const int64_t *d_nchunks, int64_t max_chunks, int positions);
+                   const int64_t *d_nchunks, int64_t max_chunks);
 int launch_deposit_granular(izh_bunch *b, izh_field *f);
 int launch_solve(izh_field *f);
 int launch_efield(izh_field *f);

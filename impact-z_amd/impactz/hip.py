@@ -277,7 +277,7 @@ class BeamBunch:
         """izh_run_stats: deposit chunk counts of the last izh_run."""
         st = np.zeros(4, dtype=np.int64)
         check(lib().izh_run_stats(self.h, st.ctypes.data_as(_i64p)))
-        return {"chunks": int(st[0]), "window_overflow": int(st[1]),
+        return {"chunks": int(st[0]), "window_spill": int(st[1]),
                 "alive": int(st[2]), "tiles": int(st[3])}
 
     def getpts(self) -> np.ndarray:

@@ -180,9 +180,10 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             double *moments);
 
 /* Counters of the last izh_run on this bunch (no reference counterpart;
- * for tuning): stats[0] deposit chunks of the last step, stats[1] chunks
- * whose cell box exceeded the LDS window over the whole run (they add to
- * global memory directly), stats[2] particles alive, stats[3] sort tiles. */
+ * for tuning): stats[0] deposit chunks of the last step, stats[1] deposited
+ * particles over the whole run whose CIC nodes left their chunk's LDS window
+ * (they add to global memory directly), stats[2] particles alive,
+ * stats[3] sort tiles. */
 int izh_run_stats(const izh_bunch *b, int64_t stats[4]);
 
 /* Bandwidth probe (no reference counterpart): one launch of a plain copy
