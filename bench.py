@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -121,6 +122,28 @@ def mixed_error(r64, r32):
             "target": "rms <= 1e-4 relative after 400 steps"}
 
 
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profiled_traffic(kernel: str, form: str):
+    """HBM bytes per launch of a kernel as measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration) and
+    recorded in a profiles/*.md summary as a line
+    ``traffic_bytes: <kernel> <form> = <bytes>``; None if none is recorded."""
+    pat = re.compile(r"^traffic_bytes:\s+(\w+)\s+(\w+)\s+=\s+([0-9.eE+]+)")
+    if not os.path.isdir(PROFILES):
+        return None
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m and m.group(1) == kernel and m.group(2) == form:
+                    return float(m.group(3))
+    return None
+
+
 def roofline(res, n, g):
     """Dominant particle kernel: algorithmic bytes / average HIP-event time."""
     km = res["kernels_ms"]
@@ -130,7 +153,8 @@ def roofline(res, n, g):
     ach = byts / t / 1e9
     return {"kernel": kern, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": byts, "avg_ms": km[kern]}
+            "traffic": profiled_traffic(kern, res["form"]), "bytes_per_launch": byts,
+            "avg_ms": km[kern]}
 
 
 def _ref_step_time(exe, tmp, tag, n, grid, steps):
@@ -247,6 +271,9 @@ def main():
             "roofline": roofline(m, n, g), "kernels_ms": m["kernels_ms"],
             "error_vs_fp64": mixed_error(r, m),
         }
+    # one plain copy kernel of 1 GiB: the calibration of the byte counters
+    # when this run is profiled with rocprofv3 --pmc
+    out["copy_probe_ms_1GiB"] = hip.probe_copy(1 << 30)
     if not args.no_cpu:
         cb = cpu_baseline(grid, n)
         out["cpu_baseline"] = cb

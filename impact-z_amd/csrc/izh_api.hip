@@ -142,6 +142,7 @@ static void bunch_free_all(izh_bunch *b) {
   dfree(b->key);
   dfree(b->key_alt);
   dfree(b->tpos);
+  dfree(b->sink);
   dfree(b->qm0);
   dfree(b->w0);
   dfree(b->id0);

@@ -142,7 +142,13 @@ __device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &l
   }
 }
 
-template <typename T, int LOAD, int STORE, bool CONV>
+// PAIR: two real lines share one complex transform (line 2j in the real
+// part, line 2j+1 in the imaginary part).  Real even input (kLoadEven) gives
+// a real even spectrum each, so the parts separate directly; real input
+// (kLoadReal) is separated from Z(k) and conj Z(L-k) when stored; two
+// Hermitian spectra (kLoadHerm) are combined as X + iY, whose inverse has
+// the two real outputs as its parts.
+template <typename T, int LOAD, int STORE, bool CONV, bool PAIR>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   __shared__ cx<T> buf[kFftPoints];
@@ -177,7 +183,34 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     const int line = line0 + lb;
     slot[it] = lb * p.L + m;
     T re = (T)0, im = (T)0;
-    if (line < nlines) {
+    if (PAIR) {
+      const int l0 = 2 * line, l1 = l0 + 1;
+      const T *src = (const T *)p.src;
+      const cx<T> *csrc = (const cx<T> *)p.src;
+      if (l0 < nlines) {
+        const int64_t b0 = (l0 % nf) * p.sf + (l0 / nf) * p.ss;
+        const bool has1 = l1 < nlines;
+        const int64_t b1 = has1 ? (l1 % nf) * p.sf + (l1 / nf) * p.ss : b0;
+        if (LOAD == kLoadEven) {
+          const int mm = m <= (p.L >> 1) ? m : p.L - m;
+          re = src[b0 + mm * p.se];
+          im = has1 ? src[b1 + mm * p.se] : (T)0;
+        } else if (LOAD == kLoadReal) {
+          if (m < p.Lin) {
+            re = src[b0 + m * p.se];
+            im = has1 ? src[b1 + m * p.se] : (T)0;
+          }
+        } else {  // kLoadHerm: Z = X + iY of the completed spectra
+          const bool lo = m <= (p.L >> 1);
+          const int mm = lo ? m : p.L - m;
+          cx<T> x = csrc[b0 + mm * p.se];
+          cx<T> y = has1 ? csrc[b1 + mm * p.se] : cx<T>{(T)0, (T)0};
+          if (!lo) { x.im = -x.im; y.im = -y.im; }
+          re = x.re - y.im;
+          im = x.im + y.re;
+        }
+      }
+    } else if (line < nlines) {
       const int f = line % nf, s = line / nf;
       const int64_t base = f * p.sf + s * p.ss;
       if (LOAD == kLoadReal) {
@@ -244,8 +277,28 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   for (int t = threadIdx.x; t < totout; t += kFftThreads) {
     int lb, m;
     map_thread(t, p.de, p, lb, m);
+    if (m >= p.Lout) continue;
+    if (PAIR) {
+      const int l0 = 2 * (line0 + lb), l1 = l0 + 1;
+      if (l0 >= nlines) continue;
+      const bool has1 = l1 < nlines;
+      const int64_t o0 = (l0 % nf) * p.df + (l0 / nf) * p.ds + m * p.de;
+      const int64_t o1 = has1 ? (l1 % nf) * p.df + (l1 / nf) * p.ds + m * p.de : o0;
+      const cx<T> z = buf[lb * p.L + bitrev(m, p.log2L)];
+      if (STORE == kStoreReal) {
+        ((T *)p.dst)[o0] = (T)(z.re * p.out_scale);
+        if (has1) ((T *)p.dst)[o1] = (T)(z.im * p.out_scale);
+      } else {
+        // X(m) = (Z(m) + conj Z(L-m)) / 2,  Y(m) = (Z(m) - conj Z(L-m)) / 2i
+        const cx<T> w = buf[lb * p.L + bitrev((p.L - m) & (p.L - 1), p.log2L)];
+        const T h = (T)(0.5 * p.out_scale);
+        ((cx<T> *)p.dst)[o0] = {(z.re + w.re) * h, (z.im - w.im) * h};
+        if (has1) ((cx<T> *)p.dst)[o1] = {(z.im + w.im) * h, (w.re - z.re) * h};
+      }
+      continue;
+    }
     int line = line0 + lb;
-    if (line >= nlines || m >= p.Lout) continue;
+    if (line >= nlines) continue;
     int f = line % nf, s = line / nf;
     int64_t o = f * p.df + s * p.ds + m * p.de;
     cx<T> v = CONV ? buf[lb * p.L + m] : buf[lb * p.L + bitrev(m, p.log2L)];
@@ -496,31 +549,46 @@ k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
   const P hxi = (P)dom->hinv[0], hyi = (P)dom->hinv[1], hzi = (P)dom->hinv[2];
   const int64_t sy = nx, sz = (int64_t)nx * ny;
   uint32_t nspill = 0;
-  for (uint32_t t = threadIdx.x; t < ch.len; t += kBlock) {
-    const int64_t i = (int64_t)ch.start + t;
-    const P x = X[i];
-    if (!(x == x)) continue;
-    const Cic<P> c = cic_of<P>(x, Y[i], Z[i], xmin, ymin, zmin, hx, hy, hz, hxi, hyi, hzi);
-    if (c.i < 0 || c.i + 1 >= nx || c.j < 0 || c.j + 1 >= ny || c.k < 0 ||
-        c.k + 1 >= nz)
-      continue;
-    const double w = uniform ? w_uniform : w0[orig[i]];
-    long long v[8];
-    cic_contrib<P>(c, w * scale, v);
-    const int li = c.i - bx, lj = c.j - by, lk = c.k - bz;
-    if (li >= 0 && li + 1 < wx && lj >= 0 && lj + 1 < wy && lk >= 0 && lk + 1 < wz) {
-      const int l0 = li + wx * (lj + wy * lk);
+  // kDepBatch slots per thread per round: all their loads are issued first
+  constexpr int kDepBatch = 4;
+  for (uint32_t t0 = threadIdx.x; t0 < ch.len; t0 += kDepBatch * kBlock) {
+    P px[kDepBatch], py[kDepBatch], pz[kDepBatch];
+    double pw[kDepBatch];
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-        atomicAdd(&win[l0 + kNode[q][0] + wx * (kNode[q][1] + wy * kNode[q][2])],
-                  (unsigned long long)v[q]);
-    } else {
-      const int64_t g0 = c.i + sy * c.j + sz * c.k;
+    for (int u = 0; u < kDepBatch; u++) {
+      const uint32_t t = t0 + u * kBlock;
+      const int64_t i = (int64_t)ch.start + (t < ch.len ? t : ch.len - 1);
+      px[u] = t < ch.len ? X[i] : (P)NAN;
+      py[u] = Y[i];
+      pz[u] = Z[i];
+      pw[u] = uniform ? w_uniform : w0[orig[i]];
+    }
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-        atomicAdd(&rho[g0 + kNode[q][0] + sy * kNode[q][1] + sz * kNode[q][2]],
-                  (unsigned long long)v[q]);
-      nspill++;
+    for (int u = 0; u < kDepBatch; u++) {
+      const P x = px[u];
+      if (!(x == x)) continue;
+      const Cic<P> c = cic_of<P>(x, py[u], pz[u], xmin, ymin, zmin, hx, hy, hz, hxi,
+                                 hyi, hzi);
+      if (c.i < 0 || c.i + 1 >= nx || c.j < 0 || c.j + 1 >= ny || c.k < 0 ||
+          c.k + 1 >= nz)
+        continue;
+      long long v[8];
+      cic_contrib<P>(c, pw[u] * scale, v);
+      const int li = c.i - bx, lj = c.j - by, lk = c.k - bz;
+      if (li >= 0 && li + 1 < wx && lj >= 0 && lj + 1 < wy && lk >= 0 && lk + 1 < wz) {
+        const int l0 = li + wx * (lj + wy * lk);
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          atomicAdd(&win[l0 + kNode[q][0] + wx * (kNode[q][1] + wy * kNode[q][2])],
+                    (unsigned long long)v[q]);
+      } else {
+        const int64_t g0 = c.i + sy * c.j + sz * c.k;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          atomicAdd(&rho[g0 + kNode[q][0] + sy * kNode[q][1] + sz * kNode[q][2]],
+                    (unsigned long long)v[q]);
+        nspill++;
+      }
     }
   }
   __syncthreads();
@@ -583,14 +651,15 @@ int ilog2(int v) {
   return r;
 }
 
-template <typename T, int LOAD, int STORE, bool CONV>
+template <typename T, int LOAD, int STORE, bool CONV, bool PAIR = false>
 int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
   p.log2L = ilog2(p.L);
   p.B = kFftPoints / p.L;
-  int64_t nlines = p.nf * p.ns;
-  int64_t nblk = (nlines + p.B - 1) / p.B;
+  const int64_t nlines = p.nf * p.ns;
+  const int64_t nfft = PAIR ? (nlines + 1) / 2 : nlines;
+  const int64_t nblk = (nfft + p.B - 1) / p.B;
   const size_t tw_bytes = (size_t)(p.L / 2) * sizeof(cx<T>);
-  hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV>), dim3((unsigned)nblk),
+  hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR>), dim3((unsigned)nblk),
                      dim3(kFftThreads), tw_bytes, f->stream, p, (const cx<T> *)tw,
                      inverse ? 1 : 0);
   IZH_HIP(hipGetLastError());
@@ -618,14 +687,14 @@ int solve_t(izh_field *f) {
     p.sf = kx; p.ss = (int64_t)kx * (ny + 1); p.se = 1;
     p.df = kx; p.ds = (int64_t)kx * (ny + 1); p.de = 1;
     p.L = mx; p.Lin = p.L; p.Lout = nx + 1; p.out_scale = 1.0;
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false>(f, p, f->tw_x, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_x, false)));
     // y: lines (i<=nx, k<=nz); green -> green_t
     p.src = f->green; p.dst = f->green_t;
     p.nf = kx; p.ns = nz + 1;
     p.sf = 1; p.ss = (int64_t)kx * (ny + 1); p.se = kx;
     p.df = 1; p.ds = (int64_t)kx * (ny + 1); p.de = kx;
     p.L = my; p.Lin = p.L; p.Lout = ny + 1;
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false>(f, p, f->tw_y, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_y, false)));
     // z: lines (i<=nx, j<=ny); green_t -> green, scaled by 1/(mx my mz)
     p.src = f->green_t; p.dst = f->green;
     p.nf = kx; p.ns = ny + 1;
@@ -633,7 +702,7 @@ int solve_t(izh_field *f) {
     p.df = 1; p.ds = kx; p.de = (int64_t)kx * (ny + 1);
     p.L = mz; p.Lin = p.L; p.Lout = nz + 1;
     p.out_scale = 1.0 / ((double)mx * my * mz);
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false>(f, p, f->tw_z, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_z, false)));
   }
   ScopedTimer tm_(IZH_T_FFT, f->stream);
   // ---- forward x: rho (nx,ny,nz) real -> A (kx, ny, nz)
@@ -643,7 +712,7 @@ int solve_t(izh_field *f) {
   p.sf = nx; p.ss = (int64_t)nx * ny; p.se = 1;
   p.df = kx; p.ds = (int64_t)kx * ny; p.de = 1;
   p.L = mx; p.Lin = nx; p.Lout = kx; p.out_scale = 1.0;
-  IZH_CHECK((run_lines<T, kLoadReal, kStoreCplx, false>(f, p, f->tw_x, false)));
+  IZH_CHECK((run_lines<T, kLoadReal, kStoreCplx, false, true>(f, p, f->tw_x, false)));
   // ---- forward y: A -> B (kx, my, nz)
   p.src = f->spec_a; p.dst = f->spec_b;
   p.nf = kx; p.ns = nz;
@@ -673,7 +742,7 @@ int solve_t(izh_field *f) {
   p.sf = kx; p.ss = (int64_t)kx * ny; p.se = 1;
   p.df = nx; p.ds = (int64_t)nx * ny; p.de = 1;
   p.L = mx; p.Lin = mx; p.Lout = nx;
-  IZH_CHECK((run_lines<T, kLoadHerm, kStoreReal, false>(f, p, f->tw_x, true)));
+  IZH_CHECK((run_lines<T, kLoadHerm, kStoreReal, false, true>(f, p, f->tw_x, true)));
   return IZH_OK;
 }
 

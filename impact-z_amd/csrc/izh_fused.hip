@@ -62,6 +62,7 @@ struct PassCfg {
   double inv_rho_scale;     // undo the grid-charge scaling in the kick
   HalfPush pb, pa;          // second half of step s-1, first half of step s
   double *part;             // kMoments moment values, kConvRed conv, per block
+  void *sink;               // kSinkBytes for the stores of empty lanes
 };
 
 // drift1_BeamBunch (src/Appl/BeamBunch.f90:276-320) for one particle.  The
@@ -86,11 +87,12 @@ __device__ inline void half_push(const HalfPush &p, double xl, double &x, double
 }
 
 // Counting-sort slot of each participating lane, in two halves so the
-// atomics' latency overlaps other work: wave_rank_issue makes one atomic per
-// distinct key in the wavefront (all issued back to back), wave_rank_slot
-// later combines the leader's old counter value with the lane's rank.
+// atomics' latency overlaps other work.  wave_rank_issue groups the lanes by
+// key with ballots (no memory traffic), then makes ONE atomic instruction in
+// which each group's first lane adds the group's size; wave_rank_slot later
+// combines that old counter value with the lane's rank in its group.
 struct RankTicket {
-  uint32_t ret;   // the atomic's old value (leader lanes)
+  uint32_t ret;   // the atomic's old value (group leaders)
   int leader;     // lane holding this lane's group atomic
   uint32_t rank;  // position of this lane within its group
 };
@@ -99,17 +101,19 @@ __device__ inline RankTicket wave_rank_issue(uint32_t *counters, uint32_t key, b
   const int lane = threadIdx.x & 63;
   unsigned long long pending = __ballot(take);
   RankTicket t{0u, lane, 0u};
+  uint32_t count = 0;
   while (pending) {
     const int leader = __ffsll((long long)pending) - 1;
     const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
     const unsigned long long same = __ballot(take && key == lk) & pending;
-    if (lane == leader) t.ret = atomicAdd(&counters[lk], (uint32_t)__popcll(same));
     if ((same >> lane) & 1ull) {
       t.leader = leader;
       t.rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
     }
+    if (lane == leader) count = (uint32_t)__popcll(same);
     pending &= ~same;
   }
+  if (take && t.leader == lane) t.ret = atomicAdd(&counters[key], count);
   return t;
 }
 
@@ -117,17 +121,20 @@ __device__ inline uint32_t wave_rank_slot(const RankTicket &t) {
   return (uint32_t)__shfl((int)t.ret, t.leader, 64) + t.rank;
 }
 
-// Histogram update, one atomic per distinct key in the wavefront.
+// Histogram update: the lanes are grouped by key with ballots, then one
+// atomic instruction adds each group's size at its first lane.
 __device__ inline void wave_count_add(uint32_t *counters, uint32_t key, bool take) {
   const int lane = threadIdx.x & 63;
   unsigned long long pending = __ballot(take);
+  uint32_t count = 0;
   while (pending) {
     const int leader = __ffsll((long long)pending) - 1;
     const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
     const unsigned long long same = __ballot(take && key == lk) & pending;
-    if (lane == leader) atomicAdd(&counters[lk], (uint32_t)__popcll(same));
+    if (lane == leader) count = (uint32_t)__popcll(same);
     pending &= ~same;
   }
+  if (count) atomicAdd(&counters[key], count);
 }
 
 // Slot range of a block: contiguous, and the blocks one XCD receives (block
@@ -177,12 +184,55 @@ __device__ inline void box_load(const GatherLds<float> &g, int64_t q, float &x,
   x = v.x; y = v.y; z = v.z;
 }
 
+// One slot of the pass's input, read an iteration before it is processed.
+struct SlotIn {
+  uint32_t key, orig;
+  uint32_t k0;       // key of the iteration's first slot (block uniform)
+  double r[6];
+};
+
+// 8-node gather in two batches of four nodes: each batch's loads are issued
+// before any of them is used.  ld(q, x, y, z) loads node q.  Same weights and
+// summation order as cic8.
+template <typename T, typename Load>
+__device__ inline void gather_batched(Load ld, int64_t o, int64_t sy, int64_t sz, T ab,
+                                      T cd, T ef, T &gx, T &gy, T &gz) {
+  const T one = (T)1;
+  const int64_t idx[8] = {o, o + sy, o + sy + sz, o + sz, o + 1 + sz,
+                          o + 1 + sy + sz, o + 1 + sy, o + 1};
+  const T w[8] = {ab * cd * ef,
+                  ab * (one - cd) * ef,
+                  ab * (one - cd) * (one - ef),
+                  ab * cd * (one - ef),
+                  (one - ab) * cd * (one - ef),
+                  (one - ab) * (one - cd) * (one - ef),
+                  (one - ab) * (one - cd) * ef,
+                  (one - ab) * cd * ef};
+  T sx = (T)0, syy = (T)0, szz = (T)0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    T vx[4], vy[4], vz[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) ld(idx[4 * h + q], vx[q], vy[q], vz[q]);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      sx += vx[q] * w[4 * h + q];
+      syy += vy[q] * w[4 * h + q];
+      szz += vz[q] * w[4 * h + q];
+    }
+  }
+  gx = sx;
+  gy = syy;
+  gz = szz;
+}
+
 // PB: push of the second half of step s-1 (kPushNone on the first pass, which
 // then has no kick and no moments); PA: first half of step s (kPushNone on
-// the last pass, which then has no z->t transform).
+// the last pass, which then has no z->t transform).  The slot loop is
+// unrolled by two over two register sets, so the loads of the next slot
+// group land in the set not being worked on.
 template <typename T, int PB, int PA>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
-k_pass(PassCfg c) {
+__global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   constexpr bool kKick = PB != kPushNone, kConv = PA != kPushNone;
   const int64_t nslots = *c.d_nslots;
   int64_t lo, hi;
@@ -205,25 +255,46 @@ k_pass(PassCfg c) {
   PairMom mom;
   pmom_zero(mom);
   double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
-  uint32_t nlost = 0, nalive = 0;
-  // slot i + kBlock is loaded while slot i is processed
-  int64_t i = lo + threadIdx.x;
-  uint32_t kin_n = kDeadKey, og_n = 0;
-  double r_n[6] = {0, 0, 0, 0, 0, 0};
-  if (i < hi) {
-    kin_n = c.in_key[i];
-    og_n = c.in_orig[i];
+  uint32_t nlost = 0, nalive = 0, nerr = 0;
+
+  // Loads of the slot group starting at base (called while lo < hi).  Every
+  // lane loads (an index past the range is clamped to its last slot and the
+  // lane is marked empty), so no branch skips the loads and the waits for
+  // them and for the atomics before them can be counted exactly.
+  auto load = [&](int64_t base, SlotIn &sl) {
+    const int64_t j = base + threadIdx.x;
+    const int64_t jj = j < hi ? j : hi - 1;
+    // every lane loads the group's first key; the lane offset is a zero the
+    // compiler cannot see through, so the value stays in a vector register
+    // (no wait at the load) until the group is processed
+    const int zero = (int)__builtin_amdgcn_mbcnt_lo(0u, 0u);
+    const uint32_t k0 = c.in_key[(base < hi ? base : hi - 1) + zero];
+    const uint32_t key = c.in_key[jj];
+    const uint32_t og = c.in_orig[jj];
+    double r[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) r_n[k] = c.in[k * cap + i];
-  }
-  for (int64_t base = lo; base < hi; base += kBlock, i += kBlock) {
-    const uint32_t kin = kin_n, og = og_n;
-    double x = r_n[0], px = r_n[1], y = r_n[2], py = r_n[3], ph = r_n[4], pt = r_n[5];
+    for (int k = 0; k < 6; k++) r[k] = c.in[k * cap + jj];
+    const bool in = j < hi;
+    sl.k0 = base < hi ? k0 : kDeadKey;
+    sl.key = in ? key : kDeadKey;
+    sl.orig = in ? og : 0u;
+#pragma unroll
+    for (int k = 0; k < 6; k++) sl.r[k] = in ? r[k] : 0.0;
+  };
+
+  auto step = [&](SlotIn &sl, uint32_t k0, auto &&prefetch) {
+    const uint32_t kin = sl.key;
+    double &x = sl.r[0], &px = sl.r[1], &y = sl.r[2], &py = sl.r[3], &ph = sl.r[4],
+           &pt = sl.r[5];
     const bool valid = kin != kDeadKey;
     nalive += valid ? 1u : 0u;
+    // counting-sort scatter by the key of the previous step; the next slot
+    // group's loads are issued after the atomic, so waiting for its result
+    // does not wait for them
+    const RankTicket tk = wave_rank_issue(c.cursor, kin, valid);
+    prefetch();
     if constexpr (kKick) {
-      // restage when the iteration's first slot starts another tile
-      const uint32_t k0 = c.in_key[base];
+      // restage the gather box when the slot group starts another tile
       if (kick && k0 != kDeadKey && k0 != box_key) {
         const uint32_t txy = (uint32_t)c.tnx * (uint32_t)c.tny;
         const int tz = (int)(k0 / txy), rem = (int)(k0 % txy);
@@ -233,29 +304,30 @@ k_pass(PassCfg c) {
         oz = min(max((tz << c.tshift[2]) - kGBoxLow, 0), c.nz - bdz);
         __syncthreads();  // the previous box is no longer read
         const int bxy = bdx * bdy, nb = bxy * bdz;
-        for (int q = threadIdx.x; q < nb; q += kBlock) {
-          const int lz = q / bxy, r2 = q - lz * bxy, ly = r2 / bdx, lx = r2 - ly * bdx;
-          box_store(gbox, q, E[(int64_t)(ox + lx) + (int64_t)sy * (oy + ly) +
-                               (int64_t)sz * (oz + lz)]);
+        for (int q0 = threadIdx.x; q0 < nb; q0 += 2 * kBlock) {
+          E4<T> ev[2];
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const int q = q0 + u * kBlock;
+            E4<T> v{};
+            if (q < nb) {
+              const int lz = q / bxy, r2 = q - lz * bxy, ly = r2 / bdx, lx = r2 - ly * bdx;
+              v = E[(int64_t)(ox + lx) + (int64_t)sy * (oy + ly) + (int64_t)sz * (oz + lz)];
+            }
+            ev[u] = v;
+          }
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (q0 + u * kBlock < nb) box_store(gbox, q0 + u * kBlock, ev[u]);
         }
         __syncthreads();
         box_key = k0;
       }
-    }
-    // counting-sort scatter by the key of the previous step
-    const RankTicket tk = wave_rank_issue(c.cursor, kin, valid);
-    if (i + kBlock < hi) {
-      const int64_t j = i + kBlock;
-      kin_n = c.in_key[j];
-      og_n = c.in_orig[j];
-#pragma unroll
-      for (int k = 0; k < 6; k++) r_n[k] = c.in[k * cap + j];
-    } else {
-      kin_n = kDeadKey;
-    }
-    if constexpr (kKick) {
-      if (valid && kick) {
-        // cvbkforth1st + scatter1 (src/Appl/BeamBunch.f90:1667-1701,2041-2181)
+      if (kick) {
+        // cvbkforth1st + scatter1 (src/Appl/BeamBunch.f90:1667-1701,2041-2181).
+        // Computed on every lane (a lane without a particle holds zeros, which
+        // give an in-range node), so no branch separates the slot's loads from
+        // their first use; only valid lanes keep the result.
         const double rcp = 1.0 / (-pt + c.gam);
         const double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
         const double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam * betai * ph * c.xl;
@@ -268,24 +340,33 @@ k_pass(PassCfg c) {
         ix = min(max(ix, 1), c.nx - 1);
         jx = min(max(jx, 1), c.ny - 1);
         kx = min(max(kx, 1), c.nz - 1);
-        const int64_t o = (int64_t)(ix - 1) + (int64_t)sy * (jx - 1) + (int64_t)sz * (kx - 1);
-        T gx, gy, gz;
+        T gx = (T)0, gy = (T)0, gz = (T)0;
         const int li = ix - 1 - ox, lj = jx - 1 - oy, lk = kx - 1 - oz;
-        if (box_key != kDeadKey && li >= 0 && li + 1 < bdx && lj >= 0 && lj + 1 < bdy &&
-            lk >= 0 && lk + 1 < bdz) {
-          cic8<T>([](int64_t q, T &ex, T &ey, T &ez) { box_load(gbox, q, ex, ey, ez); },
-                  (int64_t)li + (int64_t)bdx * (lj + bdy * lk), bdx, bdx * bdy,
-                  (T)ab, (T)cd, (T)ef, gx, gy, gz);
-        } else {
-          cic8_e4<T>(E, o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
+        const bool inbox = box_key != kDeadKey && li >= 0 && li + 1 < bdx && lj >= 0 &&
+                           lj + 1 < bdy && lk >= 0 && lk + 1 < bdz;
+        if (valid && inbox) {
+          gather_batched<T>(
+              [](int64_t q, T &ex, T &ey, T &ez) { box_load(gbox, q, ex, ey, ez); },
+              (int64_t)li + (int64_t)bdx * (lj + bdy * lk), bdx, bdx * bdy, (T)ab, (T)cd,
+              (T)ef, gx, gy, gz);
+        } else if (valid) {
+          const int64_t o = (int64_t)(ix - 1) + (int64_t)sy * (jx - 1) + (int64_t)sz * (kx - 1);
+          gather_batched<T>(
+              [E](int64_t q, T &ex, T &ey, T &ez) {
+                const E4<T> v = E[q];
+                ex = v.x; ey = v.y; ez = v.z;
+              },
+              o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
         }
         const double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
-        x = X0 * xk;
-        y = Y0 * xk;
-        ph = Z0 * xk / (-c.gam * betai);
-        px = px + txy * exn;
-        py = py + txy * eyn;
-        pt = pt - ttc * ezn;
+        if (valid) {
+          x = X0 * xk;
+          y = Y0 * xk;
+          ph = Z0 * xk / (-c.gam * betai);
+          px = px + txy * exn;
+          py = py + txy * eyn;
+          pt = pt - ttc * ezn;
+        }
       }
       half_push<PB>(c.pb, c.xl, x, px, y, py, ph, pt);
       // diagnostic1 sums of the state after step s-1 (all lanes take part)
@@ -330,24 +411,48 @@ k_pass(PassCfg c) {
       wave_count_add(c.hist_new, knew, knew != kDeadKey);
     }
     const uint32_t slot = wave_rank_slot(tk);
-    if (valid && slot >= (uint32_t)cap) {
-      atomicAdd((unsigned long long *)&c.err->err, 1ull);
-    } else if (valid) {
-      c.out[slot] = x;
-      c.out[cap + slot] = px;
-      c.out[2 * cap + slot] = y;
-      c.out[3 * cap + slot] = py;
-      c.out[4 * cap + slot] = ph;
-      c.out[5 * cap + slot] = pt;
-      c.out_orig[slot] = og;
-      c.out_key[slot] = knew;
-      if constexpr (kConv) {
-        tpx[slot] = tx;
-        tpx[cap + slot] = ty;
-        tpx[2 * cap + slot] = tz;
-      }
+    // Every lane stores: a lane without a particle (or with a slot past the
+    // buffer, which must not happen and is counted) writes into the sink, so
+    // no branch skips the stores and later waits can be counted exactly.
+    const bool ok = valid && slot < (uint32_t)cap;
+    nerr += (valid && !ok) ? 1u : 0u;
+    const int lane = threadIdx.x & 63;
+    double *sd = (double *)c.sink + lane;
+    uint32_t *su = (uint32_t *)((double *)c.sink + 8 * 64) + lane;
+    T *st = (T *)((double *)c.sink + 10 * 64) + lane;
+    (ok ? c.out[slot] : sd[0]) = x;
+    (ok ? c.out[cap + slot] : sd[64]) = px;
+    (ok ? c.out[2 * cap + slot] : sd[128]) = y;
+    (ok ? c.out[3 * cap + slot] : sd[192]) = py;
+    (ok ? c.out[4 * cap + slot] : sd[256]) = ph;
+    (ok ? c.out[5 * cap + slot] : sd[320]) = pt;
+    (ok ? c.out_orig[slot] : su[0]) = sl.orig;
+    (ok ? c.out_key[slot] : su[64]) = knew;
+    if constexpr (kConv) {
+      (ok ? tpx[slot] : st[0]) = tx;
+      (ok ? tpx[cap + slot] : st[64]) = ty;
+      (ok ? tpx[2 * cap + slot] : st[128]) = tz;
+    }
+  };
+
+  // The first group is peeled off, so both paths into the loop end the same
+  // way (a group's stores after the next group's loads) and the wait at the
+  // top of the loop can be counted exactly.
+  SlotIn sa, sb;
+  if (lo < hi) {
+    load(lo, sa);
+    step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0),
+         [&] { load(lo + kBlock, sb); });
+    for (int64_t base = lo + kBlock; base < hi; base += 2 * kBlock) {
+      step(sb, (uint32_t)__builtin_amdgcn_readfirstlane((int)sb.k0),
+           [&] { load(base + kBlock, sa); });
+      if (base + kBlock >= hi) break;
+      step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0),
+           [&] { load(base + 2 * kBlock, sb); });
     }
   }
+
+  if (nerr) atomicAdd((unsigned long long *)&c.err->err, (unsigned long long)nerr);
   __shared__ double sh[kBlock / 64][kMoments];
   // partials are quantity-major: value q of block b at part[q * kMaxRedBlocks + b]
   double *out = c.part + blockIdx.x;
@@ -507,10 +612,21 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
       const bool isconv = q >= kMoments;
       const int cq = q - kMoments;
       const int op = !isconv ? (q < 37 ? 0 : 2) : (cq < 3 ? 1 : (cq < 6 ? 2 : 0));
-      double acc = op == 1 ? 1e300 : (op == 2 ? -1e300 : 0.0);
-      for (int b = lane; b < f.nparts; b += 64) {
-        const double v = f.part[(int64_t)q * kMaxRedBlocks + b];
-        acc = op == 1 ? fmin(acc, v) : (op == 2 ? fmax(acc, v) : acc + v);
+      const double ident = op == 1 ? 1e300 : (op == 2 ? -1e300 : 0.0);
+      double acc = ident;
+      // eight loads in flight per lane, combined in slot order
+      constexpr int kU = 8;
+      const double *src = f.part + (int64_t)q * kMaxRedBlocks;
+      for (int b0 = lane; b0 < f.nparts; b0 += 64 * kU) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          const int b = b0 + 64 * u;
+          v[u] = b < f.nparts ? src[b] : ident;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+          acc = op == 1 ? fmin(acc, v[u]) : (op == 2 ? fmax(acc, v[u]) : acc + v[u]);
       }
       acc = op == 1 ? wave_min(acc) : (op == 2 ? wave_max(acc) : wave_sum(acc));
       if (lane == 0) red[q] = acc;
@@ -623,6 +739,7 @@ int bunch_alloc_fused(izh_bunch *b, int ntiles, int64_t nseg) {
     IZH_HIP(hipMalloc((void **)&b->key, (size_t)b->cap * sizeof(uint32_t)));
     IZH_HIP(hipMalloc((void **)&b->key_alt, (size_t)b->cap * sizeof(uint32_t)));
     IZH_HIP(hipMalloc(&b->tpos, (size_t)3 * b->cap * ps));
+    IZH_HIP(hipMalloc(&b->sink, kSinkBytes));
     IZH_HIP(hipMalloc((void **)&b->d_scal, sizeof(DevScal)));
   }
   if (!b->r_alt[0]) IZH_CHECK(bunch_alloc_alt(b));
@@ -767,6 +884,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   pc.inv_rho_scale = 1.0 / rho_scale;
   pc.tpos = b->tpos;
   pc.part = b->d_part;
+  pc.sink = b->sink;
   HalfPush pb_prev{};
   int kb_prev = kPushNone;
   double tau2_prev = 0.0;

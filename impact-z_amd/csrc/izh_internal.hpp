@@ -37,10 +37,11 @@ constexpr int kMoments = 44;         // 37 sums + 6 abs-max + r^2 max
 constexpr int kRunRecord = kMoments + 1;  // + Npt, per step of izh_run
 constexpr int kConvRed = 8;          // 3 min, 3 max, lost, alive
 constexpr int kPartStride = kMoments + kConvRed;
-constexpr int kChunk = 4096;         // particles per deposit chunk
+constexpr int kChunk = 8192;         // particles per deposit chunk
 constexpr int kWindowCells = 4096;   // LDS deposit window (int64 cells)
 constexpr int kTileShift = 3;        // sort tiles of 8x8x8 cells
 constexpr uint32_t kDeadKey = 0xffffffffu;
+constexpr int kSinkBytes = 64 * 8 * 16;  // one 8-byte word per lane, 16 rows
 
 // Frame of the particle coordinates held by a bunch; the reference's call
 // order moves the bunch through these (src/Contrl/AccSimulator.f90:1110-1429).
@@ -108,6 +109,7 @@ struct izh_bunch {
   uint32_t *orig = nullptr, *orig_alt = nullptr;
   uint32_t *key = nullptr, *key_alt = nullptr;   // sort keys of the fused run
   void *tpos = nullptr;  // t-frame positions for the deposit: 3*cap of T
+  void *sink = nullptr;  // kSinkBytes the fused pass's empty lanes store into
   // immutable rows, indexed by the original index
   double *qm0 = nullptr, *w0 = nullptr, *id0 = nullptr;
   int w_uniform = 0;
