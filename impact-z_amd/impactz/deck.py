@@ -1,0 +1,266 @@
+"""The host deck path: read an ``ImpactZ.in`` deck and its ``particle.in``,
+march the bunch through the lattice on the GPU, and write the reference's
+per-step diagnostic files.
+
+The deck reader follows in1_Input / in2_Input (src/Contrl/Input.f90:37-384):
+records are whitespace separated, lines whose first non-blank character is
+``!`` are comments, Fortran ``d`` exponents are accepted and a lattice line
+ends at ``/``.  The element table follows the lattice branch of
+run_AccSimulator (src/Contrl/AccSimulator.f90:237-536, 742-1504) for the
+elements this build runs: drifts (type 0) and constant-gradient quadrupoles
+(type 1 with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270).  Zero-length
+diagnostic elements that do not change the particles (types -2 to -8) are
+skipped; any other element type is refused with an error, as are decks
+outside Flagmap=1, Flagbc=1, Flagdist=19.
+
+Output files are those of diagnostic1_Output (src/Contrl/Output.f90:341-352):
+fort.18, fort.24-27, fort.29 and fort.30, one row at z = 0 and one after each
+space-charge step, values separated by blanks.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import hip
+from .beam import Beam, Element
+from .sim import Simulation
+
+# Zero-length elements that only write files in the reference
+# (src/Contrl/AccSimulator.f90:883-922); they leave the bunch unchanged.
+DIAGNOSTIC_TYPES = {-2: "phase-space output", -3: "accumulated density output",
+                    -4: "1D density output", -5: "2D density output",
+                    -6: "3D density output", -7: "particle dump",
+                    -8: "slice output"}
+
+
+class DeckError(ValueError):
+    pass
+
+
+@dataclass
+class DeckElement:
+    length: float
+    nseg: int
+    mapstp: int
+    type: int
+    values: list[float] = field(default_factory=list)
+
+
+@dataclass
+class Deck:
+    npcol: int
+    nprow: int
+    dim: int
+    np: int
+    flagmap: int
+    flagerr: int
+    flagdiag: int
+    nx: int
+    ny: int
+    nz: int
+    flagbc: int
+    xrad: float
+    yrad: float
+    perdlen: float
+    flagdist: int
+    rstartflg: int
+    flagsbstp: int
+    nchrg: int
+    nptlist: list[int]
+    currlist: list[float]
+    qmcclist: list[float]
+    distparam: list[float]
+    bcurr: float
+    bkenergy: float
+    bmass: float
+    bcharge: float
+    bfreq: float
+    phsini: float
+    elements: list[DeckElement]
+
+    @property
+    def grid(self) -> tuple[int, int, int]:
+        return (self.nx, self.ny, self.nz)
+
+    def beam(self) -> Beam:
+        return Beam(current=self.bcurr, kinetic=self.bkenergy, mass=self.bmass,
+                    charge=self.bcharge, freq=self.bfreq, phase0=self.phsini)
+
+
+def _number(tok: str) -> float:
+    return float(tok.replace("d", "e").replace("D", "e"))
+
+
+def _records(text: str):
+    for line in text.splitlines():
+        s = line.strip()
+        if not s or s.startswith("!"):
+            continue
+        yield s
+
+
+def read_deck(path: str) -> Deck:
+    """Parse an ImpactZ.in deck (in1_Input/in2_Input, src/Contrl/Input.f90)."""
+    with open(path) as f:
+        recs = list(_records(f.read()))
+    it = iter(recs)
+
+    def nxt(kind: str) -> list[str]:
+        try:
+            return next(it).split("/")[0].split()
+        except StopIteration:
+            raise DeckError(f"{path}: deck ends before the {kind} record") from None
+
+    try:
+        r = nxt("processor layout")
+        npcol, nprow = int(r[0]), int(r[1])
+        r = nxt("Dim/Np/Flagmap/Flagerr/Flagdiag")
+        dim, npart, flagmap, flagerr, flagdiag = (int(v) for v in r[:5])
+        r = nxt("mesh")
+        nx, ny, nz, flagbc = (int(v) for v in r[:4])
+        xrad, yrad, perdlen = (_number(v) for v in r[4:7])
+        r = nxt("distribution flags")
+        flagdist, rstartflg, flagsbstp, nchrg = (int(v) for v in r[:4])
+        nptlist = [int(v) for v in nxt("particle list")[:nchrg]]
+        currlist = [_number(v) for v in nxt("current list")[:nchrg]]
+        qmcclist = [_number(v) for v in nxt("q/m list")[:nchrg]]
+        distparam = []
+        for _ in range(3):
+            distparam += [_number(v) for v in nxt("distribution parameters")[:7]]
+        r = nxt("beam")
+        bcurr, bkenergy, bmass, bcharge, bfreq, phsini = (_number(v) for v in r[:6])
+    except (ValueError, IndexError) as e:
+        raise DeckError(f"{path}: malformed deck header ({e})") from None
+    elements = []
+    for line in it:
+        toks = line.split("/")[0].split()
+        if not toks:
+            continue
+        try:
+            elements.append(DeckElement(_number(toks[0]), int(_number(toks[1])),
+                                        int(_number(toks[2])), int(_number(toks[3])),
+                                        [_number(v) for v in toks[4:]]))
+        except (ValueError, IndexError):
+            raise DeckError(f"{path}: malformed lattice line: {line}") from None
+    return Deck(npcol, nprow, dim, npart, flagmap, flagerr, flagdiag, nx, ny, nz,
+                flagbc, xrad, yrad, perdlen, flagdist, rstartflg, flagsbstp, nchrg,
+                nptlist, currlist, qmcclist, distparam, bcurr, bkenergy, bmass,
+                bcharge, bfreq, phsini, elements)
+
+
+def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
+    """The element table of the deck as Elements of this build, and notes on
+    the diagnostic elements skipped.  Raises DeckError for anything the GPU
+    step does not implement."""
+    if deck.flagmap != 1:
+        raise DeckError(f"Flagmap={deck.flagmap}: only the linear map integrator "
+                        "(Flagmap=1) is on this path")
+    if deck.flagbc != 1:
+        raise DeckError(f"Flagbc={deck.flagbc}: only open boundaries (Flagbc=1)")
+    if deck.flagsbstp != 0:
+        raise DeckError("Flagsbstp != 0 (sub-cycling) is not on this path")
+    if deck.nchrg != 1:
+        raise DeckError(f"{deck.nchrg} charge states: one is supported")
+    out, notes = [], []
+    for k, e in enumerate(deck.elements, start=1):
+        v = e.values + [0.0] * 3
+        if e.type == 0:
+            out.append(Element(e.length, e.nseg, e.mapstp, 0, v[0]))
+        elif e.type == 1:
+            if v[1] > 1.0e-5:
+                raise DeckError(f"element {k}: quadrupole with field file ID {v[1]:g} "
+                                "(fringe field from file) is not on this path")
+            out.append(Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2]))
+        elif e.type in DIAGNOSTIC_TYPES and e.length == 0.0:
+            notes.append(f"element {k}: type {e.type} ({DIAGNOSTIC_TYPES[e.type]}) "
+                         "skipped: it writes files only")
+        else:
+            raise DeckError(f"element {k}: type {e.type} is not on this path "
+                            "(drift 0 and constant-gradient quadrupole 1 are)")
+    return out, notes
+
+
+def read_particle_in(path: str) -> np.ndarray:
+    """particle.in of readin_Dist (src/Appl/Distribution.f90:1677-1705): the
+    count, then 9 values per particle (Fortran d exponents accepted)."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        body = f.read()
+    if "d" in body or "D" in body:
+        body = body.replace("d", "e").replace("D", "e")
+    vals = np.array(body.split(), dtype=np.float64) if n else np.zeros(0)
+    if vals.size != 9 * n:
+        raise DeckError(f"{path}: expected {n} particles of 9 values, got {vals.size} values")
+    return vals.reshape(n, 9)
+
+
+FORT_FILES = (18, 24, 25, 26, 27, 29, 30)
+
+
+def write_fort(outdir: str, rows: np.ndarray) -> None:
+    """fort.18/24-27/29/30 of diagnostic1_Output from diagnostic rows
+    (z, then the 41 values of impactz.sim.diag_row)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    cols = {18: slice(1, 6), 24: slice(6, 12), 25: slice(12, 18), 26: slice(18, 24),
+            27: slice(24, 30), 29: slice(30, 36), 30: slice(36, 42)}
+    for unit in FORT_FILES:
+        with open(os.path.join(outdir, f"fort.{unit}"), "w") as f:
+            for r in rows:
+                vals = [r[0], *r[cols[unit]]]
+                f.write(" ".join(repr(float(x)) for x in vals) + "\n")
+
+
+def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
+             outdir: str | None = None, nsteps: int | None = None) -> np.ndarray:
+    """Run ``workdir/ImpactZ.in`` with ``workdir/particle.in`` and write the
+    diagnostic files into ``outdir`` (default: workdir).  fused=True runs the
+    lattice through izh_run.  Returns the diagnostic rows."""
+    deck = read_deck(os.path.join(workdir, "ImpactZ.in"))
+    if deck.flagdist != 19:
+        raise DeckError(f"Flagdist={deck.flagdist}: inputs come from particle.in "
+                        "(Flagdist=19) on this path")
+    lattice, notes = lattice_of(deck)
+    for n in notes:
+        print(f"[impactz.deck] {n}", flush=True)
+    pts = read_particle_in(os.path.join(workdir, "particle.in"))
+    sim = Simulation(pts, deck.beam(), lattice, deck.grid, precision=precision)
+    try:
+        if fused:
+            total = sum(e.nseg for e in lattice)
+            total = total if nsteps is None else min(total, nsteps)
+            rows = [np.concatenate([[sim.z], sim.diag()])]
+            rec = sim.run_fused(total)
+            # z after each step as drift1_BeamBunch advances it: tau1 twice
+            z = 0.0
+            segs = list(sim.segments())[:total]
+            for k in range(total):
+                tau1 = 0.5 * segs[k].length / segs[k].nseg
+                z = (z + tau1) + tau1
+                rows.append(np.concatenate([[z], sim.diag(rec[k])]))
+            rows = np.array(rows)
+        else:
+            rows = sim.run(nsteps)
+    finally:
+        sim.close()
+    write_fort(outdir or workdir, rows)
+    return rows
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("workdir", help="directory holding ImpactZ.in and particle.in")
+    ap.add_argument("--mixed", action="store_true", help="mixed-precision form")
+    ap.add_argument("--fused", action="store_true", help="run through izh_run")
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+    hip.init(args.device)
+    run_deck(args.workdir, hip.MIXED if args.mixed else hip.FP64, args.fused)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,132 @@
+"""The deck path on the CPU: parsing ImpactZ.in records the way in1_Input /
+in2_Input read them (src/Contrl/Input.f90:37-384), the element table of this
+build, and the refusals for what is not on the path."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import fixture_beam, fixture_lattice, load_golden
+
+
+def _write(path, text):
+    with open(path, "w") as f:
+        f.write(text)
+
+
+HEADER = """!
+! comment lines start with '!'
+1 1
+!  Dim Np Flagmap Flagerr Flagdiag
+6 3 1 0 1
+  16 16 32 1 0.02 0.03 0.1
+19 0 0 1
+3
+0.1
+1.0657889726d-09
+0.001 0.001 0.0 1.0 1.0 0.0 0.0
+0.001 0.001 0.0 1.0 1.0 0.0 0.0
+0.1 0.001 0.0 1.0 1.0 0.0 0.0
+0.1 1.5d8 938.272088d6 1.0 324.0e6 0.0
+"""
+
+
+def test_deck_records_and_lattice(tmp_path):
+    from impactz.deck import lattice_of, read_deck
+    deck = HEADER + """0.0 0 201 -8 101.0 / slice output
+0.4 4 10 0 1.0 /
+0.1 4 10 1 10.0 -1.0 0.15 0.0 0.0 0.0 0.0 0.0 /
+! a comment inside the lattice
+0.0 0 1001 -2 1.0 /
+0.4 2 10 0 0.5 /
+"""
+    p = tmp_path / "ImpactZ.in"
+    _write(p, deck)
+    d = read_deck(str(p))
+    assert (d.npcol, d.nprow, d.dim, d.np, d.flagmap, d.flagerr, d.flagdiag) == (1, 1, 6, 3, 1, 0, 1)
+    assert d.grid == (16, 16, 32) and d.flagbc == 1
+    assert (d.xrad, d.yrad, d.perdlen) == (0.02, 0.03, 0.1)
+    assert (d.flagdist, d.rstartflg, d.flagsbstp, d.nchrg) == (19, 0, 0, 1)
+    assert d.qmcclist == [1.0657889726e-09]
+    assert len(d.distparam) == 21
+    assert (d.bcurr, d.bkenergy, d.bmass, d.bcharge, d.bfreq, d.phsini) == \
+        (0.1, 1.5e8, 938.272088e6, 1.0, 324.0e6, 0.0)
+    assert [e.type for e in d.elements] == [-8, 0, 1, -2, 0]
+    lat, notes = lattice_of(d)
+    assert len(notes) == 2
+    assert [(e.length, e.nseg, e.mapstp, e.type) for e in lat] == \
+        [(0.4, 4, 10, 0), (0.1, 4, 10, 1), (0.4, 2, 10, 0)]
+    assert lat[0].radius == 1.0 and lat[2].radius == 0.5
+    assert lat[1].v1 == 10.0 and lat[1].radius == 0.15
+
+
+def test_deck_written_for_the_reference_reads_back(tmp_path):
+    """The deck the fixtures were run with (impactz.beam.write_deck) parses
+    to the same beam and lattice."""
+    from impactz.beam import write_deck
+    from impactz.deck import lattice_of, read_deck
+    d = load_golden("fodo_traj")
+    bm, lat = fixture_beam(d), fixture_lattice(d)
+    grid = tuple(int(v) for v in d["grid"])
+    p = tmp_path / "ImpactZ.in"
+    write_deck(str(p), d["pts_in"].shape[0], grid, bm, lat)
+    deck = read_deck(str(p))
+    assert deck.grid == grid
+    b2 = deck.beam()
+    assert (b2.current, b2.kinetic, b2.mass, b2.charge, b2.freq, b2.phase0) == \
+        (bm.current, bm.kinetic, bm.mass, bm.charge, bm.freq, bm.phase0)
+    lat2, notes = lattice_of(deck)
+    assert not notes
+    assert [(e.length, e.nseg, e.mapstp, e.type, e.v1, e.radius) for e in lat2] == \
+        [(e.length, e.nseg, e.mapstp, e.type, e.v1, e.radius) for e in lat]
+
+
+@pytest.mark.parametrize("line, what", [
+    ("0.1 4 10 101 2503370.0 3.24e8 222 852.0 0.0065 /", "type 101"),
+    ("0.1 4 10 1 10.0 2.0 0.15 /", "field file ID"),
+    ("0.0 0 10 -1 1.0 /", "type -1"),
+    ("0.1 4 10 4 0.02 0.0 150 0.03 /", "type 4"),
+])
+def test_deck_refuses_elements_off_the_path(tmp_path, line, what):
+    from impactz.deck import DeckError, lattice_of, read_deck
+    p = tmp_path / "ImpactZ.in"
+    _write(p, HEADER + line + "\n")
+    with pytest.raises(DeckError, match=what):
+        lattice_of(read_deck(str(p)))
+
+
+def test_deck_refuses_other_integrators_and_boundaries(tmp_path):
+    from impactz.deck import DeckError, lattice_of, read_deck
+    for old, new, what in [("6 3 1 0 1", "6 3 2 0 1", "Flagmap=2"),
+                           ("16 16 32 1 0.02", "16 16 32 2 0.02", "Flagbc=2")]:
+        p = tmp_path / "ImpactZ.in"
+        _write(p, HEADER.replace(old, new) + "0.4 4 10 0 1.0 /\n")
+        with pytest.raises(DeckError, match=what):
+            lattice_of(read_deck(str(p)))
+
+
+def test_particle_in_round_trip_and_fortran_exponents(tmp_path):
+    from impactz.beam import gaussian_bunch, write_particle_in
+    from impactz.deck import DeckError, read_particle_in
+    pts = gaussian_bunch(50, seed=2)
+    p = tmp_path / "particle.in"
+    write_particle_in(str(p), pts)
+    assert np.array_equal(read_particle_in(str(p)), pts)
+    _write(p, "1\n1.0D-3 2.0d0 3 4 5 6 7 8 9\n")
+    assert read_particle_in(str(p))[0, 0] == 1.0e-3
+    _write(p, "2\n1 2 3 4 5 6 7 8 9\n")
+    with pytest.raises(DeckError):
+        read_particle_in(str(p))
+
+
+def test_fort_writer_columns(tmp_path):
+    from impactz.deck import write_fort
+    rows = np.arange(2 * 42, dtype=np.float64).reshape(2, 42)
+    write_fort(str(tmp_path), rows)
+    f24 = np.loadtxt(tmp_path / "fort.24")
+    assert f24.shape == (2, 7)
+    assert np.array_equal(f24[:, 0], rows[:, 0]) and np.array_equal(f24[:, 1:], rows[:, 6:12])
+    f18 = np.loadtxt(tmp_path / "fort.18")
+    assert np.array_equal(f18[:, 1:], rows[:, 1:6])
+    f30 = np.loadtxt(tmp_path / "fort.30")
+    assert np.array_equal(f30[:, 1:], rows[:, 36:42])
