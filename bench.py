@@ -82,6 +82,8 @@ def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps)
     t0 = time.perf_counter()
     recs.append(sim.run_fused(steps, start=warmup + steps))
     dt = time.perf_counter() - t0
+    # deposit chunks of the last step and LDS-window spills summed over the run
+    dep = sim.bunch.run_stats()
     done = warmup + 2 * steps
     if accuracy_steps > done:
         log(f"{form}: continuing to step {accuracy_steps} (untimed, for the error report)")
@@ -99,6 +101,9 @@ def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps)
         "kernels_launches": {k: v[1] for k, v in timers.items() if v[1]},
         "rows": rows,
         "efield": efield,
+        "deposit": {"chunks": dep["chunks"], "window_spill": dep["window_spill"],
+                    "spill_per_particle_step":
+                    dep["window_spill"] / max(dep["alive"] * steps, 1)},
     }
 
 
@@ -208,7 +213,83 @@ def cpu_baseline(grid, n_full: int, n1=20_000, n2=400_000):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    f = sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+    if not f:
+        raise SystemExit(f"no *{suffix} under {d}")
+    with open(f[0]) as fh:
+        return list(csv.DictReader(fh))
+
+
+def _short(name):
+    for junk in ("void ", "izh::(anonymous namespace)::", "izh::"):
+        name = name.replace(junk, "")
+    return name.split("(")[0]
+
+
+def summarize_profiles(stats_dir, fetch_dir, write_dir, form):
+    """Markdown summary of three rocprofv3 runs of this script: a kernel trace,
+    a FETCH_SIZE pass and a WRITE_SIZE pass (each --output-format csv).
+    Byte counters are scaled by what they read for k_copy16, the 1 GiB copy
+    of ``copy_probe_ms_1GiB``.  The fused pass's traffic is averaged over its
+    launches after the first one of the process (that one scatters the
+    bunch out of its unsorted upload order)."""
+    agg = {}
+    for r in _csv_rows(stats_dir, "kernel_trace.csv"):
+        agg.setdefault(_short(r["Kernel_Name"]), []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    out = ["| kernel | launches | avg ms | total ms |", "|---|---|---|---|"]
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        out.append(f"| {k} | {len(v)} | {sum(v) / len(v):.4f} | {sum(v):.3f} |")
+
+    def counter(d, cname):
+        per = {}
+        for r in _csv_rows(d, "counter_collection.csv"):
+            if r["Counter_Name"] == cname:
+                key = (int(r["Dispatch_Id"]), _short(r["Kernel_Name"]))
+                per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+        return sorted(per.items())
+
+    fe = dict(counter(fetch_dir, "FETCH_SIZE"))
+    wr = dict(counter(write_dir, "WRITE_SIZE"))
+    copy_f = next(v for (_, k), v in fe.items() if k == "k_copy16")
+    copy_w = next(v for (_, k), v in wr.items() if k == "k_copy16")
+    cf, cw = (1 << 20) / copy_f, (1 << 20) / copy_w  # KiB the copy moves: 1 GiB
+    out += ["", f"Calibration: the 1 GiB copy reads FETCH_SIZE {copy_f:.0f} KiB "
+            f"(factor {cf:.3f}) and WRITE_SIZE {copy_w:.0f} KiB (factor {cw:.3f}).", "",
+            "| kernel | launches | FETCH KiB avg | WRITE KiB avg | HBM bytes/launch |",
+            "|---|---|---|---|---|"]
+    # the two runs make the same dispatches in the same order
+    both = [(d, k, fe[(d, k)], wr[(d, k)]) for (d, k) in sorted(fe) if (d, k) in wr]
+    byk = {}
+    for d, k, fv, wv in both:
+        byk.setdefault(k, []).append((fv, wv))
+    for k, v in sorted(byk.items()):
+        f_ = sum(a for a, _ in v) / len(v)
+        w_ = sum(b for _, b in v) / len(v)
+        out.append(f"| {k} | {len(v)} | {f_:.0f} | {w_:.0f} | {(f_ * cf + w_ * cw) * 1024:.4g} |")
+
+    def hbm(prefix, skip_first):
+        v = [(fv * cf + wv * cw) * 1024 for d, k, fv, wv in both if k.startswith(prefix)]
+        v = v[1:] if skip_first else v
+        return sum(v) / len(v) if v else None
+
+    out.append("")
+    for name, prefix, skip in (("pass", "k_pass", True), ("deposit", "k_deposit", False)):
+        b_ = hbm(prefix, skip)
+        if b_ is not None:
+            out.append(f"traffic_bytes: {name} {form} = {b_:.4g}")
+    return "\n".join(out)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
+        if len(sys.argv) != 6:
+            raise SystemExit("usage: bench.py --summarize-profiles STATS FETCH WRITE FORM")
+        print(summarize_profiles(*sys.argv[2:6]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -262,6 +343,7 @@ def main():
                    "particles": n, "grid": list(grid), "form": main_form},
         "roofline": roofline(r, n, g),
         "kernels_ms": r["kernels_ms"],
+        "deposit": r["deposit"],
     }
     if "mixed" in res and main_form != "mixed":
         m = res["mixed"]

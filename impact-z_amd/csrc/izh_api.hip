@@ -158,6 +158,7 @@ static void bunch_free_all(izh_bunch *b) {
 // Upload Pts1(9,n) (particle-major): rows 1-6 into the moving SoA rows,
 // q/m, charge weight and id into the immutable rows, original index = i.
 static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
+  touch_particles(b);
   if (n > 0xfffffffeLL)
     return fail(IZH_ERR_ARG, "bunch of %lld particles exceeds 2^32-2", (long long)n);
   if (n > b->cap) {
@@ -391,6 +392,7 @@ int izh_bunch_get_info(const izh_bunch *b, izh_bunch_info *info) {
 
 int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info) {
   if (!b || !info) return fail(IZH_ERR_ARG, "izh_bunch_set_info: bad argument");
+  touch_particles(b);
   if (info->nptlocal != b->n)
     return fail(IZH_ERR_ARG, "izh_bunch_set_info: nptlocal is set by upload");
   b->current = info->current;

@@ -409,6 +409,22 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
       }
       // histogram of the new keys for the next pass
       wave_count_add(c.hist_new, knew, knew != kDeadKey);
+    } else {
+      // last pass: the tile of the 0th-order t-frame position the next run
+      // starts from (k_prologue_keys' position, on the current domain), so
+      // the next run needs no prologue
+      if (valid) {
+        const double rcp = 1.0 / (-pt + c.gam);
+        const double betai = sqrt(fabs(1.0 - rcp * rcp * (1.0 + px * px + py * py)));
+        const double X0 = x * c.xl, Y0 = y * c.xl;
+        const double Z0 = -c.gam * betai * wrap_phase(ph) * c.xl;
+        const int ci = min(max((int)((X0 - xmin) * hxi), 0), c.nx - 1);
+        const int cj = min(max((int)((Y0 - ymin) * hyi), 0), c.ny - 1);
+        const int ck = min(max((int)((Z0 - zmin) * hzi), 0), c.nz - 1);
+        knew = (uint32_t)((ci >> c.tshift[0]) +
+                          c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
+      }
+      wave_count_add(c.hist_new, knew, knew != kDeadKey);
     }
     const uint32_t slot = wave_rank_slot(tk);
     // Every lane stores: a lane without a particle (or with a slot past the
@@ -688,8 +704,8 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
     }
   }
   __syncthreads();
-  if (f.mode == 2) return;
-  // ---- exclusive scan of the tile histogram and the chunk list
+  // ---- exclusive scan of the tile histogram and the chunk list (after the
+  // last pass: of the keys the next run starts from)
   const int per = (f.ntiles + blockDim.x - 1) / blockDim.x;
   const int t0 = tid * per, t1 = min(t0 + per, f.ntiles);
   uint32_t cnt = 0, nch = 0;
@@ -722,6 +738,7 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
     off += h;
   }
   if (tid == blockDim.x - 1) *f.nchunks = csum[tid];
+  __syncthreads();  // hist_zero may be hist itself
   for (int t = tid; t < f.ntiles; t += blockDim.x) f.hist_zero[t] = 0u;
 }
 
@@ -825,8 +842,18 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
   const int nb = (grid_for(b->n) + 7) / 8 * 8;  // multiple of 8 for block_range
   int rc = IZH_OK;
+  // The previous run on this bunch and field left sort keys, their scan and
+  // the first deposit chunks (in chunk slot b->keys_po): start from them.
+  const bool warm = b->keys_valid && b->keys_field == f && b->keys_grid[0] == nx &&
+                    b->keys_grid[1] == ny && b->keys_grid[2] == nz;
+  const int po = warm ? b->keys_po : 0;  // chunk slot of deposit(s) = (s + po) & 1
+  b->keys_valid = 0;
   // ---- device scalars
   DevScal hs{};
+  if (warm) {
+    hs.nchunks[0] = b->h_scal->nchunks[0];
+    hs.nchunks[1] = b->h_scal->nchunks[1];
+  }
   hs.current = b->current;
   hs.npt = (double)b->npt;
   hs.nalive = b->n;
@@ -835,7 +862,18 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   IZH_HIP(hipMemcpyAsync(b->d_scal, b->h_scal, sizeof hs, hipMemcpyHostToDevice, st));
   // ---- prologue: initial keys, their histogram, cursors and chunks
   hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st,
-                     b->hist[0], (int64_t)ntiles);
+                     b->hist[warm ? 1 : 0], (int64_t)ntiles);
+  FinCfg fc{};
+  fc.part = b->d_part;
+  fc.nparts = nb;
+  fc.scal = b->d_scal;
+  fc.dom = f->d_dom;
+  fc.ntiles = ntiles;
+  fc.nx = nx; fc.ny = ny; fc.nz = nz;
+  fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
+  fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
+  fc.cursor = b->cursor;
+  if (!warm) {
   hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
                      b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
                      b->c.scxl, b->d_part);
@@ -847,18 +885,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
                      nx, ny, nz);
   IZH_HIP(hipGetLastError());
-  FinCfg fc{};
-  fc.part = b->d_part;
-  fc.nparts = nb;
-  fc.scal = b->d_scal;
-  fc.dom = f->d_dom;
-  fc.ntiles = ntiles;
-  fc.nx = nx; fc.ny = ny; fc.nz = nz;
-  fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
-  fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
   fc.mode = 0;
   fc.hist = b->hist[0];
-  fc.cursor = b->cursor;
   fc.chunks = b->chunks;
   fc.nchunks = &b->d_scal->nchunks[0];
   fc.hist_zero = b->hist[1];
@@ -866,6 +894,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     ScopedTimer tm_(IZH_T_FINAL, st);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
   }
+  }  // !warm
   IZH_HIP(hipGetLastError());
   // ---- the steps: pass s finishes step s-1 and starts step s
   PassCfg pc{};
@@ -918,8 +947,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     fc.mode = last ? 2 : 1;
     fc.moments_out = s > 0 ? b->d_moments + (s - 1) * kRunRecord : nullptr;
     fc.hist = b->hist[(s + 1) & 1];
-    fc.chunks = b->chunks + ((s + 1) & 1) * b->nchunks_cap;
-    fc.nchunks = &b->d_scal->nchunks[(s + 1) & 1];
+    fc.chunks = b->chunks + ((s + 1 + po) & 1) * b->nchunks_cap;
+    fc.nchunks = &b->d_scal->nchunks[(s + 1 + po) & 1];
     fc.hist_zero = b->hist[s & 1];
     {
       ScopedTimer tm_(IZH_T_FINAL, st);
@@ -928,8 +957,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     IZH_HIP(hipGetLastError());
     if (last) break;
     // charge, update3O and the E field of step s on the grouping just written
-    rc = launch_deposit(b, f, b->chunks + (s & 1) * b->nchunks_cap,
-                        &b->d_scal->nchunks[s & 1], b->nchunks_cap);
+    rc = launch_deposit(b, f, b->chunks + ((s + po) & 1) * b->nchunks_cap,
+                        &b->d_scal->nchunks[(s + po) & 1], b->nchunks_cap);
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
     // second half of step s: host bookkeeping now, applied by the next pass
@@ -952,13 +981,17 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   b->n = out.nslots;
   b->npt = (int64_t)out.npt;
   b->current = out.current;
-  b->run_stats[0] = out.nchunks[(nseg - 1) & 1];
+  b->run_stats[0] = out.nchunks[(nseg + 1 + po) & 1];
   b->run_stats[1] = out.overflow;
   b->run_stats[2] = out.nalive;
   b->run_stats[3] = ntiles;
   for (int k = 0; k < 6; k++) dom->range[k] = f->h_dom->range[k];
   for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
   f->have_phi = 1;
+  b->keys_valid = 1;
+  b->keys_po = (int)((nseg + 1 + po) & 1);
+  b->keys_field = f;
+  b->keys_grid[0] = nx; b->keys_grid[1] = ny; b->keys_grid[2] = nz;
   return IZH_OK;
 }
 

@@ -128,6 +128,12 @@ struct izh_bunch {
   double *d_moments = nullptr;              // per-step moments of a run
   int64_t moments_cap = 0;
   int64_t run_stats[4] = {0, 0, 0, 0};      // izh_run_stats
+  // Sort keys, their scan (cursor) and first deposit chunks left by the last
+  // izh_run for the next one; any other change to the particles clears this.
+  int keys_valid = 0;
+  const izh_field *keys_field = nullptr;
+  int keys_grid[3] = {0, 0, 0};
+  int keys_po = 0;  // chunk slot holding the next run's first deposit grouping
 };
 
 // Device-resident FieldQuant + solver workspace for one mesh.
@@ -164,6 +170,10 @@ struct Push {
   double tau, gam0, ibeta0, scxl;
   double xm[36];
 };
+
+// Called by every operation that changes the particles or their order: the
+// sort state a run of steps leaves for the next one no longer applies.
+inline void touch_particles(izh_bunch *b) { b->keys_valid = 0; }
 
 // Launchers implemented in the kernel translation units.
 int launch_drift(izh_bunch *b, double tau, double gam0, double beta0);

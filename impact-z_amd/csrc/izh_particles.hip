@@ -324,6 +324,7 @@ static int launch_push(izh_bunch *b, const Push &p) {
 }
 
 int launch_drift(izh_bunch *b, double tau, double gam0, double beta0) {
+  touch_particles(b);
   Push p{};
   p.kind = 1;
   p.tau = tau;
@@ -334,6 +335,7 @@ int launch_drift(izh_bunch *b, double tau, double gam0, double beta0) {
 }
 
 int launch_map_linear(izh_bunch *b, const double xm[36]) {
+  touch_particles(b);
   Push p{};
   p.kind = 2;
   std::memcpy(p.xm, xm, sizeof p.xm);
@@ -342,6 +344,7 @@ int launch_map_linear(izh_bunch *b, const double xm[36]) {
 
 int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
                    int64_t *nlost) {
+  touch_particles(b);
   const double gam = -b->refpt[5];
   const double rad = (xrad + yrad) / 2.0;
   const int nb = grid_for(b->n);
@@ -365,6 +368,7 @@ int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
 }
 
 int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep) {
+  touch_particles(b);
   const double gam = -b->refpt[5];
   const double rad = (xrad + yrad) / 2.0;
   const int64_t nchunks = (b->n + kCompactChunk - 1) / kCompactChunk;
@@ -396,6 +400,7 @@ int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep) {
 }
 
 int launch_cvbk(izh_bunch *b) {
+  touch_particles(b);
   if (b->n == 0) return IZH_OK;
   ScopedTimer tm_(IZH_T_CVBK, b->stream);
   hipLaunchKernelGGL(k_cvbk, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream,
@@ -407,6 +412,7 @@ int launch_cvbk(izh_bunch *b) {
 
 int launch_gather_kick(izh_bunch *b, izh_field *f, double tau, double xycon,
                        double tcon) {
+  touch_particles(b);
   if (b->n == 0) return IZH_OK;
   const double gam = -b->refpt[5];
   const double xk = 1.0 / b->c.scxl;

@@ -180,7 +180,8 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             double *moments);
 
 /* Counters of the last izh_run on this bunch (no reference counterpart;
- * for tuning): stats[0] deposit chunks of the last step, stats[1] deposited
+ * for tuning): stats[0] deposit chunks of the grouping the run ends with
+ * (the one the next izh_run on this bunch starts from), stats[1] deposited
  * particles over the whole run whose CIC nodes left their chunk's LDS window
  * (they add to global memory directly), stats[2] particles alive,
  * stats[3] sort tiles. */
