@@ -236,13 +236,19 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, form):
     of ``copy_probe_ms_1GiB``.  The fused pass's traffic is averaged over its
     launches after the first one of the process (that one scatters the
     bunch out of its unsorted upload order)."""
-    agg = {}
-    for r in _csv_rows(stats_dir, "kernel_trace.csv"):
-        agg.setdefault(_short(r["Kernel_Name"]), []).append(
-            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    agg, seq = {}, []
+    for r in sorted(_csv_rows(stats_dir, "kernel_trace.csv"),
+                    key=lambda r: int(r["Start_Timestamp"])):
+        k, ms = _short(r["Kernel_Name"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+        agg.setdefault(k, []).append(ms)
+        seq.append((k, ms))
     out = ["| kernel | launches | avg ms | total ms |", "|---|---|---|---|"]
     for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
         out.append(f"| {k} | {len(v)} | {sum(v) / len(v):.4f} | {sum(v):.3f} |")
+    steady = [ms for k, ms in seq if k.startswith("k_pass")][1:]
+    if steady:
+        out += ["", f"Fused pass, all kinds, launches after the first: {len(steady)}, "
+                f"average {sum(steady) / len(steady):.4f} ms."]
 
     def counter(d, cname):
         per = {}

@@ -511,11 +511,13 @@ k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
 
 // The fused run's deposit.  Each block takes one chunk of the sorted bunch:
 // at most kChunk slots grouped under one sort tile (the tile of the
-// particles' previous position).  The block accumulates in an LDS window of
-// the tile's nodes widened by kTileMargin nodes on every side (plus the
-// upper CIC node), so no pass over the positions is needed to bound them;
-// particles that moved further add to the grid directly and are counted in
-// *spill.  Same fixed-point arithmetic and node order as k_deposit_chunks.
+// particles' previous position, in the grid *kgrid that key was computed
+// in).  The block accumulates in an LDS window of the tile's nodes, mapped
+// into this step's grid and widened by kTileMargin nodes on every side (plus
+// the upper CIC node), so no pass over the positions is needed to bound
+// them; particles that moved further add to the grid directly and are
+// counted in *spill.  Same fixed-point arithmetic and node order as
+// k_deposit_chunks.
 constexpr int kTileMargin = 3;
 constexpr int kTileWinEdge = (1 << kTileShift) + 1 + 2 * kTileMargin;
 constexpr int kTileWinCells = kTileWinEdge * kTileWinEdge * kTileWinEdge;
@@ -526,8 +528,8 @@ k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
                 const P *__restrict__ Z, const uint32_t *__restrict__ orig,
                 const double *__restrict__ w0, double w_uniform, int uniform,
                 const Chunk *__restrict__ chunks, const int64_t *__restrict__ d_nchunks,
-                const DevDom *__restrict__ dom, int nx, int ny, int nz, int s0, int s1,
-                int s2, int tnx, int tny, double scale,
+                const KeyGrid *__restrict__ kgrid, const DevDom *__restrict__ dom, int nx,
+                int ny, int nz, int s0, int s1, int s2, int tnx, int tny, double scale,
                 unsigned long long *__restrict__ rho, int64_t *__restrict__ spill) {
   __shared__ unsigned long long win[kTileWinCells];
   if (blockIdx.x >= *d_nchunks) return;
@@ -538,9 +540,13 @@ k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
   const int wx = min((1 << s0) + 1 + 2 * kTileMargin, nx);
   const int wy = min((1 << s1) + 1 + 2 * kTileMargin, ny);
   const int wz = min((1 << s2) + 1 + 2 * kTileMargin, nz);
-  const int bx = min(max((tx << s0) - kTileMargin, 0), nx - wx);
-  const int by = min(max((ty << s1) - kTileMargin, 0), ny - wy);
-  const int bz = min(max((tz << s2) - kTileMargin, 0), nz - wz);
+  const KeyGrid kg = *kgrid;
+  const int bx = min(max(tile_first_node(tx << s0, kg.lo[0], kg.h[0], dom->range[0],
+                                         dom->hinv[0]) - kTileMargin, 0), nx - wx);
+  const int by = min(max(tile_first_node(ty << s1, kg.lo[1], kg.h[1], dom->range[2],
+                                         dom->hinv[1]) - kTileMargin, 0), ny - wy);
+  const int bz = min(max(tile_first_node(tz << s2, kg.lo[2], kg.h[2], dom->range[4],
+                                         dom->hinv[2]) - kTileMargin, 0), nz - wz);
   const int wcells = wx * wy * wz;
   for (int c = threadIdx.x; c < wcells; c += kBlock) win[c] = 0ull;
   __syncthreads();
@@ -821,7 +827,7 @@ int field_set_domain(izh_field *f, const izh_compdom *dom) {
 // list (the fused run) the positions are the t-frame buffer (T) of the
 // tile-sorted bunch; without one, the bunch's own rows (t frame, FP64) in
 // chunks of kChunk consecutive particles.
-int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
+int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGrid *kgrid,
                    const int64_t *d_nchunks, int64_t max_chunks) {
   f->fixed_scale = fixed_point_scale(b->wabs_sum);
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(f->g)), dim3(kBlock), 0, f->stream,
@@ -830,6 +836,7 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
     ScopedTimer tm_(IZH_T_DEPOSIT, f->stream);
     if (chunks) {
       // the fused run: chunks of the tile-sorted bunch, t positions in T
+      if (!kgrid) return fail(IZH_ERR_ARG, "deposit: chunk list without its key grid");
       const int s0 = tile_shift(f->nx), s1 = tile_shift(f->ny), s2 = tile_shift(f->nz);
       const int tnx = f->nx >> s0, tny = f->ny >> s1;
       int64_t *spill = b->d_scal ? &b->d_scal->overflow : nullptr;
@@ -838,14 +845,14 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
         hipLaunchKernelGGL(k_deposit_tiles<float>, dim3((unsigned)max_chunks),
                            dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
                            b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
-                           f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
+                           kgrid, f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
                            f->fixed_scale, f->rho_fx, spill);
       } else {
         const double *tp = (const double *)b->tpos;
         hipLaunchKernelGGL(k_deposit_tiles<double>, dim3((unsigned)max_chunks),
                            dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
                            b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
-                           f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
+                           kgrid, f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
                            f->fixed_scale, f->rho_fx, spill);
       }
     } else {
@@ -875,7 +882,7 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
 
 int launch_deposit_granular(izh_bunch *b, izh_field *f) {
   const int64_t nch = (b->n + kChunk - 1) / kChunk;
-  return launch_deposit(b, f, nullptr, nullptr, nch);
+  return launch_deposit(b, f, nullptr, nullptr, nullptr, nch);
 }
 
 int launch_solve(izh_field *f) {

@@ -33,8 +33,8 @@ namespace {
 // maplinear_Quadrupole is block diagonal (src/Appl/Quadrupole.f90:179-196),
 // so only its three 2x2 blocks travel.
 struct HalfPush {
-  double tau, gam0, ibeta0;  // exact drift
-  double m[12];              // (x,px), (y,py), (phi,pt) blocks, row major
+  double tau_xl, gam0, phase;  // exact drift: tau/Scxl, gamma0, tau/(Scxl beta0)
+  double m[12];                // (x,px), (y,py), (phi,pt) blocks, row major
 };
 
 enum PushKind : int { kPushNone = 0, kPushDrift = 1, kPushMap = 2 };
@@ -63,20 +63,25 @@ struct PassCfg {
   HalfPush pb, pa;          // second half of step s-1, first half of step s
   double *part;             // kMoments moment values, kConvRed conv, per block
   void *sink;               // kSinkBytes for the stores of empty lanes
+  const Chunk *in_chunks;   // slot ranges of the input grouping, by tile
+  const int64_t *in_nchunks;
+  const KeyGrid *kgrid;     // grid the tiles of in_chunks were computed in
 };
 
 // drift1_BeamBunch (src/Appl/BeamBunch.f90:276-320) for one particle.  The
 // map rows keep the reference's product order: the zero entries of the 6x6
 // product add nothing.
 template <int K>
-__device__ inline void half_push(const HalfPush &p, double xl, double &x, double &px,
+__device__ inline void half_push(const HalfPush &p, double &x, double &px,
                                  double &y, double &py, double &ph, double &pt) {
   if constexpr (K == kPushDrift) {
+    // drift1_BeamBunch bitype 0 (src/Appl/BeamBunch.f90:290-297) with the
+    // common factor (tau/Scxl)/tmph taken once: one division instead of six
     const double d = pt - p.gam0;
-    const double tmph = sqrt(d * d - 1.0 - px * px - py * py);
-    x = x + px * p.tau / tmph / xl;
-    y = y + py * p.tau / tmph / xl;
-    ph = ph - (p.ibeta0 + d / tmph) * p.tau / xl;
+    const double r = p.tau_xl / sqrt(d * d - 1.0 - px * px - py * py);
+    x = fma(px, r, x);
+    y = fma(py, r, y);
+    ph = ph - fma(d, r, p.phase);
   } else if constexpr (K == kPushMap) {
     const double *m = p.m;
     const double x1 = fma(px, m[1], x * m[0]), px1 = fma(px, m[3], x * m[2]);
@@ -187,7 +192,7 @@ __device__ inline void box_load(const GatherLds<float> &g, int64_t q, float &x,
 // One slot of the pass's input, read an iteration before it is processed.
 struct SlotIn {
   uint32_t key, orig;
-  uint32_t k0;       // key of the iteration's first slot (block uniform)
+  uint32_t k0;       // sort tile of the group's first slot (block uniform)
   double r[6];
 };
 
@@ -256,6 +261,29 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   pmom_zero(mom);
   double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
   uint32_t nlost = 0, nalive = 0, nerr = 0;
+  // The input slots are grouped by the tiles of the keys the previous pass
+  // scattered by; in_chunks lists those slot ranges in slot order.  The
+  // gather box follows the tile of each group's first slot: the block walks
+  // the list forward from the chunk holding its first slot.
+  // The list is not written during the pass, so it is read through the
+  // constant address space: scalar loads, which no vector-memory wait counts.
+  using ConstChunk = __attribute__((address_space(4))) const Chunk;
+  const ConstChunk *chs = (const ConstChunk *)c.in_chunks;
+  int64_t nch = 0, ci = 0, nxt_start = INT64_MAX;
+  uint32_t cur_tile = kDeadKey;
+  if constexpr (kKick) {
+    nch = *(__attribute__((address_space(4))) const int64_t *)c.in_nchunks;
+    int64_t a = 0, b = nch;  // last chunk starting at or before lo
+    while (b - a > 1) {
+      const int64_t m = (a + b) >> 1;
+      if ((int64_t)chs[m].start <= lo) a = m; else b = m;
+    }
+    ci = a;
+    if (nch > 0) {
+      cur_tile = chs[ci].tile;
+      nxt_start = ci + 1 < nch ? (int64_t)chs[ci + 1].start : INT64_MAX;
+    }
+  }
 
   // Loads of the slot group starting at base (called while lo < hi).  Every
   // lane loads (an index past the range is clamped to its last slot and the
@@ -264,18 +292,25 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   auto load = [&](int64_t base, SlotIn &sl) {
     const int64_t j = base + threadIdx.x;
     const int64_t jj = j < hi ? j : hi - 1;
-    // every lane loads the group's first key; the lane offset is a zero the
-    // compiler cannot see through, so the value stays in a vector register
-    // (no wait at the load) until the group is processed
-    const int zero = (int)__builtin_amdgcn_mbcnt_lo(0u, 0u);
-    const uint32_t k0 = c.in_key[(base < hi ? base : hi - 1) + zero];
+    uint32_t k0 = kDeadKey;
+    if constexpr (kKick) {
+      if (base < hi) {
+        // the next chunk's start is held, so only a crossing loads
+        while (nxt_start <= base) {
+          ci++;
+          cur_tile = chs[ci].tile;
+          nxt_start = ci + 1 < nch ? (int64_t)chs[ci + 1].start : INT64_MAX;
+        }
+        k0 = cur_tile;
+      }
+    }
     const uint32_t key = c.in_key[jj];
     const uint32_t og = c.in_orig[jj];
     double r[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) r[k] = c.in[k * cap + jj];
     const bool in = j < hi;
-    sl.k0 = base < hi ? k0 : kDeadKey;
+    sl.k0 = k0;
     sl.key = in ? key : kDeadKey;
     sl.orig = in ? og : 0u;
 #pragma unroll
@@ -299,9 +334,14 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
         const uint32_t txy = (uint32_t)c.tnx * (uint32_t)c.tny;
         const int tz = (int)(k0 / txy), rem = (int)(k0 % txy);
         const int ty = rem / c.tnx, tx = rem % c.tnx;
-        ox = min(max((tx << c.tshift[0]) - kGBoxLow, 0), c.nx - bdx);
-        oy = min(max((ty << c.tshift[1]) - kGBoxLow, 0), c.ny - bdy);
-        oz = min(max((tz << c.tshift[2]) - kGBoxLow, 0), c.nz - bdz);
+        // the tile's first node in the grid of this E field
+        const KeyGrid &kg = *c.kgrid;
+        ox = min(max(tile_first_node(tx << c.tshift[0], kg.lo[0], kg.h[0], xmin, hxi) -
+                     kGBoxLow, 0), c.nx - bdx);
+        oy = min(max(tile_first_node(ty << c.tshift[1], kg.lo[1], kg.h[1], ymin, hyi) -
+                     kGBoxLow, 0), c.ny - bdy);
+        oz = min(max(tile_first_node(tz << c.tshift[2], kg.lo[2], kg.h[2], zmin, hzi) -
+                     kGBoxLow, 0), c.nz - bdz);
         __syncthreads();  // the previous box is no longer read
         const int bxy = bdx * bdy, nb = bxy * bdz;
         for (int q0 = threadIdx.x; q0 < nb; q0 += 2 * kBlock) {
@@ -368,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
           pt = pt - ttc * ezn;
         }
       }
-      half_push<PB>(c.pb, c.xl, x, px, y, py, ph, pt);
+      half_push<PB>(c.pb, x, px, y, py, ph, pt);
       // diagnostic1 sums of the state after step s-1 (all lanes take part)
       pmom_add(mom, valid ? x : 0.0, valid ? px : 0.0, valid ? y : 0.0,
                valid ? py : 0.0, valid ? ph : 0.0, valid ? pt : 0.0);
@@ -377,7 +417,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     T tx = (T)NAN, ty = (T)NAN, tz = (T)NAN;
     if constexpr (kConv) {
       if (valid) {
-        half_push<PA>(c.pa, c.xl, x, px, y, py, ph, pt);
+        half_push<PA>(c.pa, x, px, y, py, ph, pt);
         // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
         ph = wrap_phase(ph);
         const double rcp = 1.0 / (-pt + c.gam);
@@ -610,10 +650,11 @@ struct FinCfg {
   Chunk *chunks;            // -> deposit chunks of that grouping
   int64_t *nchunks;         // -> their number
   uint32_t *hist_zero;      // histogram the next pass fills
+  int kg;                   // grouping whose keys were just written (dom->kgrid)
   int ntiles;
   int nx, ny, nz;
   double gam, scfreq, clight, pi, scxl, mass, charge;
-  const double *prologue_box;
+  const double *prologue_box;  // mode 0: the box k_prologue_keys used
 };
 
 __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
@@ -651,6 +692,17 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
   }
   if (tid == 0) {
     DevScal &S = *f.scal;
+    // the keys the pass just wrote were computed in the domain as it stands
+    // before this step's update; k_prologue_keys used the unpadded box
+    {
+      DevDom &D = *f.dom;
+      const int nn[3] = {f.nx, f.ny, f.nz};
+      for (int a = 0; a < 3; a++) {
+        const double *pb = f.prologue_box;
+        D.kgrid[f.kg].lo[a] = f.mode == 0 ? pb[a] : D.range[2 * a];
+        D.kgrid[f.kg].h[a] = f.mode == 0 ? (pb[3 + a] - pb[a]) / nn[a] : D.msize[a];
+      }
+    }
     if (f.mode != 0 && f.moments_out) {
       for (int k = 0; k < kMoments; k++) f.moments_out[k] = red[k];
       f.moments_out[kMoments] = red[kMoments + 7];  // Npt of those moments
@@ -792,9 +844,9 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
     const double gam0 = -b->refpt[5];
     const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
     kind = kPushDrift;
-    p.tau = sg.tau1;
+    p.tau_xl = sg.tau1 / b->c.scxl;
     p.gam0 = gam0;
-    p.ibeta0 = 1.0 / beta0;
+    p.phase = sg.tau1 / (b->c.scxl * beta0);
     b->refpt[4] = b->refpt[4] + sg.tau1 / (b->c.scxl * beta0);
   } else if (sg.bitype == 1) {
     kind = kPushMap;
@@ -886,6 +938,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
                      nx, ny, nz);
   IZH_HIP(hipGetLastError());
   fc.mode = 0;
+  fc.kg = 0;
+  fc.prologue_box = b->d_out;
   fc.hist = b->hist[0];
   fc.chunks = b->chunks;
   fc.nchunks = &b->d_scal->nchunks[0];
@@ -929,6 +983,11 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     pc.in_key = b->key;
     pc.out_key = b->key_alt;
     pc.hist_new = b->hist[(s + 1) & 1];
+    // the input slots: the output of pass s-1, grouped by the keys of pass
+    // s-2 (or of the prologue), which deposit s-1 used too
+    pc.in_chunks = b->chunks + ((s + 1 + po) & 1) * b->nchunks_cap;
+    pc.in_nchunks = &b->d_scal->nchunks[(s + 1 + po) & 1];
+    pc.kgrid = &f->d_dom->kgrid[(s + 1 + po) & 1];
     pc.pb = pb_prev;
     pc.pa = pa;
     pc.tau2_prev = tau2_prev;
@@ -945,6 +1004,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     std::swap(b->orig, b->orig_alt);
     std::swap(b->key, b->key_alt);
     fc.mode = last ? 2 : 1;
+    fc.kg = (int)((s + 1 + po) & 1);
     fc.moments_out = s > 0 ? b->d_moments + (s - 1) * kRunRecord : nullptr;
     fc.hist = b->hist[(s + 1) & 1];
     fc.chunks = b->chunks + ((s + 1 + po) & 1) * b->nchunks_cap;
@@ -958,7 +1018,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     if (last) break;
     // charge, update3O and the E field of step s on the grouping just written
     rc = launch_deposit(b, f, b->chunks + ((s + po) & 1) * b->nchunks_cap,
-                        &b->d_scal->nchunks[(s + po) & 1], b->nchunks_cap);
+                        &f->d_dom->kgrid[(s + po) & 1], &b->d_scal->nchunks[(s + po) & 1],
+                        b->nchunks_cap);
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
     // second half of step s: host bookkeeping now, applied by the next pass

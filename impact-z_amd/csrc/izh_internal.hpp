@@ -37,7 +37,7 @@ constexpr int kMoments = 44;         // 37 sums + 6 abs-max + r^2 max
 constexpr int kRunRecord = kMoments + 1;  // + Npt, per step of izh_run
 constexpr int kConvRed = 8;          // 3 min, 3 max, lost, alive
 constexpr int kPartStride = kMoments + kConvRed;
-constexpr int kChunk = 8192;         // particles per deposit chunk
+constexpr int kChunk = 16384;        // particles per deposit chunk
 constexpr int kWindowCells = 4096;   // LDS deposit window (int64 cells)
 constexpr int kTileShift = 3;        // sort tiles of 8x8x8 cells
 constexpr uint32_t kDeadKey = 0xffffffffu;
@@ -53,12 +53,28 @@ enum Frame : int {
 
 // Computational domain as the kernels read it (one per field, in device
 // memory, so a run of steps can update it without the host).
+// The grid a grouping of sort keys was computed in: lower corner and cell
+// size per axis.  Key tile t of an axis starts at node t << shift of it.
+struct KeyGrid {
+  double lo[3], h[3];
+};
+
 struct DevDom {
   double range[6];    // xmin, xmax, ymin, ymax, zmin, zmax (SpatRange)
   double msize[3];    // hx, hy, hz (Meshsize)
   double hinv[3];     // 1/h
   double ptrange[6];  // the conv1st range it was built from
+  KeyGrid kgrid[2];   // grid of the keys of each of the two groupings in flight
 };
+
+// First node, in the grid with lower corner lo and 1/h hinv, of the key tile
+// that starts at node t0 of key grid (klo, kh).  The domain moves and
+// rescales between steps, so a tile's nodes shift by a few cells.
+__device__ inline int tile_first_node(int t0, double klo, double kh, double lo,
+                                      double hinv) {
+  const double u = floor((klo + (double)t0 * kh - lo) * hinv);
+  return (int)fmin(fmax(u, -1.0e6), 1.0e6);  // callers clamp to the grid
+}
 
 // Bunch scalars a run of steps keeps on the device.
 struct DevScal {
@@ -184,7 +200,7 @@ int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep);
 int launch_cvbk(izh_bunch *b);
 int launch_moments(izh_bunch *b, double out[kMoments]);
 int field_set_domain(izh_field *f, const izh_compdom *dom);
-int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks,
+int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGrid *kgrid,
                    const int64_t *d_nchunks, int64_t max_chunks);
 int launch_deposit_granular(izh_bunch *b, izh_field *f);
 int launch_solve(izh_field *f);
