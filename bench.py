@@ -183,32 +183,36 @@ def _ref_step_time(exe, tmp, tag, n, grid, steps):
     return (times[1] - times[0]) / (steps[1] - steps[0])
 
 
-def cpu_baseline(grid, n_full: int, n1=20_000, n2=400_000):
+def cpu_baseline(grid, n_full: int, n1=20_000, n2=1_000_000, small=(32, 32, 32)):
     """The reference executable (built from source by oracle/Makefile.ref) on
     one host core, on a bounded sample of the workload, scaled to it.  The
-    per-step time t(n) = a + n*b is measured on the workload's own grid at two
-    particle counts n1 < n2 (each as the difference of a 2-step and a 1-step
-    run, so start-up and particle.in parsing cancel):
-      b = (t(n2) - t(n1)) / (n2 - n1),  a = t(n1) - n1*b,
+    per-step time is t(n) = a + n*b: a, the grid work (Green function, FFTs),
+    depends on the grid; b, the work per particle, does not.  Each time is
+    the difference of a long and a short run, so start-up and particle.in
+    parsing cancel:
+      b from n1 and n2 particles on a small grid (where the particle work
+        dominates): b = (t_small(n2) - t_small(n1)) / (n2 - n1);
+      a from n1 particles on the workload's grid: a = t(n1) - n1*b;
       reported rate = n_full / (a + n_full*b)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ImpactZexe")
     if not os.access(exe, os.X_OK):
         return None
     tmp = tempfile.mkdtemp(prefix="impz_cpu_")
     try:
-        t1 = _ref_step_time(exe, tmp, "n1", n1, grid, (1, 2))
-        t2 = _ref_step_time(exe, tmp, "n2", n2, grid, (1, 2))
-        b = max((t2 - t1) / (n2 - n1), 0.0)
+        s_lo = _ref_step_time(exe, tmp, "small_n1", n1, small, (1, 5))
+        s_hi = _ref_step_time(exe, tmp, "small_n2", n2, small, (1, 5))
+        b = max((s_hi - s_lo) / (n2 - n1), 0.0)
+        t1 = _ref_step_time(exe, tmp, "full_n1", n1, grid, (1, 2))
         a = max(t1 - n1 * b, 0.0)
         per_step = a + n_full * b
         return {"value": n_full / per_step, "unit": "particle-steps/s", "cores": 1,
                 "kind": "reference",
-                "sample": (f"ImpactZexe (serial build, amdflang -O3, 1 core) on the "
-                           f"{grid[0]}x{grid[1]}x{grid[2]} grid with {n1} and {n2} "
-                           f"particles, 1 and 2 steps each: grid work a={a:.3f} s/step, "
-                           f"particle work b={b * 1e6:.3f} us/particle-step; rate = "
-                           f"{n_full}/(a + {n_full}*b), extrapolated to {n_full} "
-                           f"particles (fort.* writes included)")}
+                "sample": (f"ImpactZexe (serial build, amdflang -O3, 1 core): particle work "
+                           f"b={b * 1e6:.3f} us/particle-step from {n1} and {n2} particles "
+                           f"on a {small[0]}x{small[1]}x{small[2]} grid (1 and 5 steps), grid "
+                           f"work a={a:.3f} s/step from {n1} particles on the "
+                           f"{grid[0]}x{grid[1]}x{grid[2]} grid (1 and 2 steps); rate = "
+                           f"{n_full}/(a + {n_full}*b) (fort.* writes included)")}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 

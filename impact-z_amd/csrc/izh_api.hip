@@ -403,40 +403,66 @@ int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info) {
   return IZH_OK;
 }
 
-// ------------------------------------------------------- quadrupole map
+// ------------------------------------------------------- element maps
 
-// intfunc1_Quadrupole, constant gradient (src/Appl/Quadrupole.f90:254-305):
-// reference particle phase rate and the three 2x2 linear ODE blocks.
-static void quad_rhs(const izh_const *c, double bgrad, double qmcc,
-                     const double y[18], double f[18]) {
+// The linear ODE the element maps share: the reference particle's phase rate
+// (y[4]) and energy (y[5]), the cosine- and sine-like columns of three 2x2
+// systems q' = p/Scxl, p' = -s q (y[6..17]) and, for the solenoid, the 2x2
+// rotation of the Larmor frame (y[18..21]).  Spring constants:
+//   quadrupole  s = (G/Brho, -G/Brho, 0)*Scxl  (src/Appl/Quadrupole.f90:283-291)
+//   ConstFoc    s = (kx0^2, ky0^2, kz0^2)*Scxl (src/Appl/ConstFoc.f90:276-281)
+//   solenoid    s = ((B/2Brho)^2, (B/2Brho)^2, 0)*Scxl, rotation rate
+//               -B/(2 Brho)                    (src/Appl/Sol.f90:298-318)
+struct LinearElement {
+  int type;      // 1 quadrupole (constant gradient), 2 constant focusing, 3 solenoid
+  double p[3];   // 1: gradient; 2: kx0^2, ky0^2, kz0^2; 3: Bz0
+  int nvar() const { return type == 3 ? 22 : 18; }
+};
+
+static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc,
+                       const double *y, double *f) {
   const double gamma0 = -y[5];
   const double beta0 = std::sqrt(gamma0 * gamma0 - 1.0) / gamma0;
   const double gbet = std::sqrt((gamma0 - 1.0) * (gamma0 + 1.0));
-  const double brho = gbet / c->clight / qmcc;
-  const double s11 = bgrad / brho * c->scxl;
-  const double s33 = -bgrad / brho * c->scxl;
-  const double s55 = 0.0;
   for (int k = 0; k < 4; k++) f[k] = 0.0;
   f[4] = 1.0 / (beta0 * c->scxl);
   f[5] = 0.0;
-  const double s[3] = {s11, s33, s55};
+  double s[3], rot = 0.0;
+  if (e.type == 1) {
+    const double brho = gbet / c->clight / qmcc;
+    s[0] = e.p[0] / brho * c->scxl;
+    s[1] = -e.p[0] / brho * c->scxl;
+    s[2] = 0.0;
+  } else if (e.type == 2) {
+    for (int a = 0; a < 3; a++) s[a] = e.p[a] * c->scxl;
+  } else {
+    const double brho = gbet / c->clight / qmcc;
+    const double k = 0.5 * e.p[0] / brho;
+    s[0] = s[1] = (k * k) * c->scxl;
+    s[2] = 0.0;
+    rot = -e.p[0] / brho / 2;
+  }
   for (int blk = 0; blk < 3; blk++) {
-    // two columns (cosine- and sine-like) per plane: y' = p/Scxl, p' = -s y
     for (int col = 0; col < 2; col++) {
       const int o = 6 + 4 * blk + 2 * col;
       f[o] = y[o + 1] / c->scxl;
       f[o + 1] = -s[blk] * y[o];
     }
   }
+  if (e.type == 3) {
+    f[18] = -rot * y[20];
+    f[19] = -rot * y[21];
+    f[20] = rot * y[18];
+    f[21] = rot * y[19];
+  }
 }
 
-int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
-                       int mapstp, double refpt[6], double charge, double mass,
-                       double xm[36]) {
-  if (!c || !refpt || !xm || mapstp < 1 || !(mass > 0))
-    return fail(IZH_ERR_ARG, "izh_quad_maplinear: bad argument");
-  // maplinear_Quadrupole + rk6i_Quadrupole (src/Appl/Quadrupole.f90:120-252):
-  // an 8-stage 6th-order Runge-Kutta over mapstp steps of the 18 variables.
+// maplinear_Quadrupole / _ConstFoc / _Sol with rk6i_* (src/Appl/Quadrupole.f90:
+// 120-252, ConstFoc.f90:116-243, Sol.f90:120-268): an 8-stage 6th-order
+// Runge-Kutta over mapstp steps, then the 6x6 map with the energy scaling
+// of each block (dlt = thl = 0).
+static void linear_map(const izh_const *c, const LinearElement &e, double tau, int mapstp,
+                       double refpt[6], double charge, double mass, double xm[36]) {
   static const double kA[7][7] = {
       {1.0 / 9.0},
       {1.0 / 24.0, 3.0 / 24.0},
@@ -451,27 +477,30 @@ int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
   static const double kB[8] = {41.0 / 840.0, 0.0, 216.0 / 840.0, 27.0 / 840.0,
                                272.0 / 840.0, 27.0 / 840.0, 216.0 / 840.0,
                                41.0 / 840.0};
+  constexpr int kMaxVar = 22;
+  const int nv = e.nvar();
   const double qmcc = charge / mass;
-  double y[18] = {0};
+  double y[kMaxVar] = {0};
   y[4] = refpt[4];
   y[5] = refpt[5];
   y[6] = y[9] = y[10] = y[13] = y[14] = y[17] = 1.0;
+  y[18] = y[21] = 1.0;
   const double h = tau / mapstp;
   const double gi = -refpt[5];
   const double ui = gi * (std::sqrt(gi * gi - 1.0) / gi);
   const double squi = std::sqrt(ui), sqi3 = squi * squi * squi;
-  double k[8][18], yt[18];
+  double k[8][kMaxVar] = {{0}}, yt[kMaxVar] = {0};
   for (int step = 0; step < mapstp; step++) {
     for (int st = 0; st < 8; st++) {
-      for (int v = 0; v < 18; v++) {
+      for (int v = 0; v < nv; v++) {
         double acc = 0.0;
         for (int q = 0; q < st; q++) acc += kA[st - 1][q] * k[q][v];
         yt[v] = y[v] + acc;
       }
-      quad_rhs(c, gradient, qmcc, st == 0 ? y : yt, k[st]);
-      for (int v = 0; v < 18; v++) k[st][v] *= h;
+      linear_rhs(c, e, qmcc, st == 0 ? y : yt, k[st]);
+      for (int v = 0; v < nv; v++) k[st][v] *= h;
     }
-    for (int v = 0; v < 18; v++) {
+    for (int v = 0; v < nv; v++) {
       double acc = 0.0;
       for (int st = 0; st < 8; st++) acc += kB[st] * k[st][v];
       y[v] += acc;
@@ -483,43 +512,114 @@ int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
   const double uf = gf * (std::sqrt(gf * gf - 1.0) / gf);
   const double squf = std::sqrt(uf), sqf3 = squf * squf * squf;
   for (int q = 0; q < 36; q++) xm[q] = 0.0;
-  // transverse blocks scale with sqrt(gamma beta), longitudinal with its cube
-  // (src/Appl/Quadrupole.f90:179-196, with dlt = thl = 0)
-  const double si[3] = {squi, squi, sqi3}, sf[3] = {squf, squf, sqf3};
-  for (int blk = 0; blk < 3; blk++) {
-    const int r = 2 * blk, o = 6 + 4 * blk;
-    xm[6 * r + r] = y[o] * si[blk] / sf[blk];
-    xm[6 * (r + 1) + r] = y[o + 1] * si[blk] * sf[blk];
-    xm[6 * r + r + 1] = y[o + 2] / (si[blk] * sf[blk]);
-    xm[6 * (r + 1) + r + 1] = y[o + 3] * sf[blk] / si[blk];
+  // one 2x2 block from ODE columns o..o+3, scaled by w (the solenoid's
+  // rotation entry; 1 otherwise), at rows r and columns cc
+  auto block = [&](int r, int cc, int o, double si, double sf, double w, bool scaled) {
+    if (scaled) {
+      xm[6 * r + cc] = w * (y[o] * si / sf);
+      xm[6 * (r + 1) + cc] = w * (y[o + 1] * si * sf);
+      xm[6 * r + cc + 1] = w * y[o + 2] / (si * sf);
+      xm[6 * (r + 1) + cc + 1] = w * y[o + 3] * sf / si;
+    } else {
+      xm[6 * r + cc] = y[o] * si / sf;
+      xm[6 * (r + 1) + cc] = y[o + 1] * si * sf;
+      xm[6 * r + cc + 1] = y[o + 2] / (si * sf);
+      xm[6 * (r + 1) + cc + 1] = y[o + 3] * sf / si;
+    }
+  };
+  if (e.type == 3) {
+    // src/Appl/Sol.f90:187-207: the x and y blocks times the rotation
+    block(0, 0, 6, squi, squf, y[18], true);
+    block(0, 2, 6, squi, squf, y[19], true);
+    block(2, 0, 10, squi, squf, y[20], true);
+    block(2, 2, 10, squi, squf, y[21], true);
+  } else {
+    block(0, 0, 6, squi, squf, 1.0, false);
+    block(2, 2, 10, squi, squf, 1.0, false);
   }
+  // the longitudinal block scales with the cube (src/Appl/Quadrupole.f90:191-196)
+  block(4, 4, 14, sqi3, sqf3, 1.0, false);
+}
+
+// The linear element of a deck line's values, or an error message.
+static const char *linear_element(int bitype, const double v[4], LinearElement &e) {
+  e = LinearElement{};
+  e.type = bitype;
+  if (bitype == 1) {
+    // a field profile from a file (file ID > 1e-5) is not on this path
+    if (v[1] > 1.0e-5) return "quadrupole fringe field from a file";
+    e.p[0] = v[0];
+  } else if (bitype == 2) {
+    for (int a = 0; a < 3; a++) e.p[a] = v[a];
+  } else if (bitype == 3) {
+    e.p[0] = v[0];
+  } else {
+    return "no linear map for this element type";
+  }
+  return nullptr;
+}
+
+int izh_element_maplinear(const izh_const *c, double tau, int bitype, const double v[4],
+                          int mapstp, double refpt[6], double charge, double mass,
+                          double xm[36]) {
+  if (!c || !v || !refpt || !xm || mapstp < 1 || !(mass > 0))
+    return fail(IZH_ERR_ARG, "izh_element_maplinear: bad argument");
+  LinearElement e;
+  if (const char *why = linear_element(bitype, v, e))
+    return fail(IZH_ERR_ARG, "izh_element_maplinear: bitype %d: %s", bitype, why);
+  linear_map(c, e, tau, mapstp, refpt, charge, mass, xm);
   return IZH_OK;
+}
+
+int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
+                       int mapstp, double refpt[6], double charge, double mass,
+                       double xm[36]) {
+  if (!c || !refpt || !xm || mapstp < 1 || !(mass > 0))
+    return fail(IZH_ERR_ARG, "izh_quad_maplinear: bad argument");
+  const double v[4] = {gradient, 0.0, 0.0, 0.0};
+  return izh_element_maplinear(c, tau, 1, v, mapstp, refpt, charge, mass, xm);
 }
 
 // ------------------------------------------------------------- map push
 
-int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
-               double gradient, int mapstp) {
-  if (!b || !z) return fail(IZH_ERR_ARG, "izh_drift1: bad argument");
+int izh_map1(izh_bunch *b, double *z, double tau, int bitype, const double v[4],
+             int mapstp) {
+  if (!b || !z || !v) return fail(IZH_ERR_ARG, "izh_map1: bad argument");
   if (b->frame != kFrameZ)
-    return fail(IZH_ERR_STATE, "izh_drift1: bunch is not in the z frame");
-  if (bitype == 0) {
+    return fail(IZH_ERR_STATE, "izh_map1: bunch is not in the z frame");
+  const char *why = nullptr;
+  const int kind = map1_kind(bitype, v, &why);
+  if (kind < 0) return fail(IZH_ERR_ARG, "izh_map1: bitype %d: %s", bitype, why);
+  if (kind == kMap1Drift) {
     // src/Appl/BeamBunch.f90:276-298
     const double gam0 = -b->refpt[5];
     const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
     IZH_CHECK(launch_drift(b, tau, gam0, beta0));
     b->refpt[4] = b->refpt[4] + tau / (b->c.scxl * beta0);
-  } else if (bitype == 1) {
-    double xm[36];
-    IZH_CHECK(izh_quad_maplinear(&b->c, tau, gradient, mapstp, b->refpt,
-                                 b->charge, b->mass, xm));
-    IZH_CHECK(launch_map_linear(b, xm));
+  } else if (kind == kMap1QuadK) {
+    // transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607)
+    const double r6 = b->refpt[5];
+    const double gambet0 = std::sqrt(r6 * r6 - 1.0);
+    const double beta0 = std::sqrt(1.0 - 1.0 / (r6 * r6));
+    IZH_CHECK(launch_quad_kmode(b, tau, v[0], b->charge / b->mass, gambet0, beta0));
+    b->refpt[4] = b->refpt[4] + tau / beta0 / b->c.scxl;
   } else {
-    return fail(IZH_ERR_ARG, "izh_drift1: bitype %d not supported (0 drift, 1 quad)",
-                bitype);
+    double xm[36];
+    IZH_CHECK(izh_element_maplinear(&b->c, tau, bitype, v, mapstp, b->refpt, b->charge,
+                                    b->mass, xm));
+    IZH_CHECK(launch_map_linear(b, xm));
   }
   *z = *z + tau;
   return IZH_OK;
+}
+
+int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
+               double gradient, int mapstp) {
+  if (bitype != 0 && bitype != 1)
+    return fail(IZH_ERR_ARG, "izh_drift1: bitype %d not supported (0 drift, 1 quad; "
+                "izh_map1 takes the other elements)", bitype);
+  const double v[4] = {gradient, 0.0, 0.0, 0.0};
+  return izh_map1(b, z, tau, bitype, v, mapstp);
 }
 
 int izh_map_linear(izh_bunch *b, const double xm[36]) {
@@ -776,25 +876,24 @@ int izh_field_upload_rho(izh_field *f, const double *rho) {
 // ---------------------------------------------------------------- step
 
 int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
-             double tau1, int bitype, double gradient, int mapstp, double xrad,
-             double yrad, double bcurr, double *moments) {
-  if (!b || !f || !dom || !z) return fail(IZH_ERR_ARG, "izh_step: bad argument");
-  const double tau2 = 2.0 * tau1;
+             const izh_segment *seg, double bcurr, double *moments) {
+  if (!b || !f || !dom || !z || !seg) return fail(IZH_ERR_ARG, "izh_step: bad argument");
+  const double tau1 = seg->tau1, tau2 = 2.0 * tau1;
   // src/Contrl/AccSimulator.f90:1106-1110 first half push
-  IZH_CHECK(izh_drift1(b, z, tau1, bitype, gradient, mapstp));
+  IZH_CHECK(izh_map1(b, z, tau1, seg->bitype, seg->v, seg->mapstp));
   // an empty bunch has nothing to deposit or kick (the reference would still
   // build a domain from its +-1e20 initial range; nothing reads it)
   if (bcurr >= (double)1.0e-10f && b->n > 0) {
     double ptrange[6];
     int64_t nplc, nptot;
-    IZH_CHECK(izh_conv1st(b, tau2, ptrange, &nplc, &nptot, xrad, yrad, 1));  // :1128
+    IZH_CHECK(izh_conv1st(b, tau2, ptrange, &nplc, &nptot, seg->xrad, seg->yrad, 1));
     IZH_CHECK(izh_update_compdom(dom, ptrange, 1));                         // :1156
     IZH_CHECK(izh_charge(b, dom, f));                                       // :1191
     IZH_CHECK(izh_update3O(f, dom));                                        // :1263
     IZH_CHECK(izh_cvbkforth1st(b));                                         // :1399
     IZH_CHECK(izh_kick1(b, tau2, f, dom, 1));                               // :1425
   }
-  IZH_CHECK(izh_drift1(b, z, tau1, bitype, gradient, mapstp));              // :1429
+  IZH_CHECK(izh_map1(b, z, tau1, seg->bitype, seg->v, seg->mapstp));       // :1429
   if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
   return IZH_OK;
 }
@@ -810,18 +909,25 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     return fail(IZH_ERR_ARG, "izh_run: domain mesh differs from the field's");
   if (b->precision != f->precision)
     return fail(IZH_ERR_ARG, "izh_run: bunch and field precision differ");
-  for (int64_t s = 0; s < nseg; s++)
-    if ((segs[s].bitype != 0 && segs[s].bitype != 1) || !(segs[s].tau1 >= 0) ||
-        (segs[s].bitype == 1 && segs[s].mapstp < 1))
-      return fail(IZH_ERR_ARG, "izh_run: segment %lld is not a drift or quadrupole",
-                  (long long)s);
+  for (int64_t s = 0; s < nseg; s++) {
+    const izh_segment &sg = segs[s];
+    const char *why = nullptr;
+    const int kind = map1_kind(sg.bitype, sg.v, &why);
+    if (kind < 0)
+      return fail(IZH_ERR_ARG, "izh_run: segment %lld: bitype %d: %s", (long long)s,
+                  sg.bitype, why);
+    if (!(sg.tau1 >= 0) || (kind == kMap1Linear && sg.mapstp < 1))
+      return fail(IZH_ERR_ARG, "izh_run: segment %lld: bad step or map steps", (long long)s);
+    if (kind == kMap1QuadK && bcurr >= (double)1.0e-10f)
+      return fail(IZH_ERR_ARG, "izh_run: segment %lld is a K-mode quadrupole (chromatic "
+                  "map): run it through izh_step", (long long)s);
+  }
   if (nseg == 0) return IZH_OK;
   // no space charge or nothing to carry it: the reference loop is maps only
   if (bcurr < (double)1.0e-10f || b->n == 0) {
     for (int64_t s = 0; s < nseg; s++) {
-      IZH_CHECK(izh_step(b, f, dom, z, segs[s].tau1, segs[s].bitype,
-                         segs[s].gradient, segs[s].mapstp, segs[s].xrad,
-                         segs[s].yrad, bcurr, moments ? moments + s * kRunRecord : nullptr));
+      IZH_CHECK(izh_step(b, f, dom, z, &segs[s], bcurr,
+                         moments ? moments + s * kRunRecord : nullptr));
       if (moments) moments[s * kRunRecord + kMoments] = (double)b->npt;
     }
     return IZH_OK;

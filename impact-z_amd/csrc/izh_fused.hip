@@ -34,10 +34,14 @@ namespace {
 // so only its three 2x2 blocks travel.
 struct HalfPush {
   double tau_xl, gam0, phase;  // exact drift: tau/Scxl, gamma0, tau/(Scxl beta0)
-  double m[12];                // (x,px), (y,py), (phi,pt) blocks, row major
+  // kPushMap: the (x,px), (y,py), (phi,pt) 2x2 blocks, row major (m[0..11]);
+  // kPushMapC: the 4x4 transverse block, row major (m[0..15]), then the
+  // (phi,pt) block (m[16..19])
+  double m[20];
 };
 
-enum PushKind : int { kPushNone = 0, kPushDrift = 1, kPushMap = 2 };
+// kPushMapC is a map whose transverse planes are coupled (the solenoid).
+enum PushKind : int { kPushNone = 0, kPushDrift = 1, kPushMap = 2, kPushMapC = 3 };
 
 struct PassCfg {
   const double *in;         // six rows of cap doubles, row k at in + k*cap
@@ -88,6 +92,14 @@ __device__ inline void half_push(const HalfPush &p, double &x, double &px,
     const double y1 = fma(py, m[5], y * m[4]), py1 = fma(py, m[7], y * m[6]);
     const double ph1 = fma(pt, m[9], ph * m[8]), pt1 = fma(pt, m[11], ph * m[10]);
     x = x1; px = px1; y = y1; py = py1; ph = ph1; pt = pt1;
+  } else if constexpr (K == kPushMapC) {
+    const double *m = p.m;
+    double t[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      t[j] = fma(py, m[4 * j + 3], fma(y, m[4 * j + 2], fma(px, m[4 * j + 1], x * m[4 * j])));
+    const double ph1 = fma(pt, m[17], ph * m[16]), pt1 = fma(pt, m[19], ph * m[18]);
+    x = t[0]; px = t[1]; y = t[2]; py = t[3]; ph = ph1; pt = pt1;
   }
 }
 
@@ -535,24 +547,27 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   }
 }
 
+template <typename T, int PB>
+void launch_pass_pa(int pa, int nb, hipStream_t st, const PassCfg &pc) {
+  const dim3 g(nb), b(kBlock);
+  switch (pa) {
+    case kPushNone:
+      if constexpr (PB != kPushNone) hipLaunchKernelGGL((k_pass<T, PB, kPushNone>), g, b, 0, st, pc);
+      break;
+    case kPushDrift: hipLaunchKernelGGL((k_pass<T, PB, kPushDrift>), g, b, 0, st, pc); break;
+    case kPushMap: hipLaunchKernelGGL((k_pass<T, PB, kPushMap>), g, b, 0, st, pc); break;
+    default: hipLaunchKernelGGL((k_pass<T, PB, kPushMapC>), g, b, 0, st, pc); break;
+  }
+}
+
 template <typename T>
 void launch_pass(int pb, int pa, int nb, hipStream_t st, const PassCfg &pc) {
-  const dim3 g(nb), b(kBlock);
-#define IZH_PASS(PB, PA) \
-  hipLaunchKernelGGL((k_pass<T, PB, PA>), g, b, 0, st, pc)
-  if (pb == kPushNone) {
-    if (pa == kPushDrift) IZH_PASS(kPushNone, kPushDrift);
-    else IZH_PASS(kPushNone, kPushMap);
-  } else if (pb == kPushDrift) {
-    if (pa == kPushNone) IZH_PASS(kPushDrift, kPushNone);
-    else if (pa == kPushDrift) IZH_PASS(kPushDrift, kPushDrift);
-    else IZH_PASS(kPushDrift, kPushMap);
-  } else {
-    if (pa == kPushNone) IZH_PASS(kPushMap, kPushNone);
-    else if (pa == kPushDrift) IZH_PASS(kPushMap, kPushDrift);
-    else IZH_PASS(kPushMap, kPushMap);
+  switch (pb) {
+    case kPushNone: launch_pass_pa<T, kPushNone>(pa, nb, st, pc); break;
+    case kPushDrift: launch_pass_pa<T, kPushDrift>(pa, nb, st, pc); break;
+    case kPushMap: launch_pass_pa<T, kPushMap>(pa, nb, st, pc); break;
+    default: launch_pass_pa<T, kPushMapC>(pa, nb, st, pc); break;
   }
-#undef IZH_PASS
 }
 
 // ------------------------------------------------------------ prologue
@@ -836,11 +851,13 @@ int bunch_alloc_fused(izh_bunch *b, int ntiles, int64_t nseg) {
 }
 
 // Host-side descriptor of a segment's half-step push; refpt and z follow the
-// reference's bookkeeping (drift1_BeamBunch, maplinear_Quadrupole).
+// reference's bookkeeping (drift1_BeamBunch and the element's maplinear).
 static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &kind,
                         double *z) {
   p = HalfPush{};
-  if (sg.bitype == 0) {
+  const char *why = nullptr;
+  const int m1 = map1_kind(sg.bitype, sg.v, &why);
+  if (m1 == kMap1Drift) {
     const double gam0 = -b->refpt[5];
     const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
     kind = kPushDrift;
@@ -848,24 +865,40 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
     p.gam0 = gam0;
     p.phase = sg.tau1 / (b->c.scxl * beta0);
     b->refpt[4] = b->refpt[4] + sg.tau1 / (b->c.scxl * beta0);
-  } else if (sg.bitype == 1) {
-    kind = kPushMap;
+  } else if (m1 == kMap1Linear) {
     double xm[36];
-    IZH_CHECK(izh_quad_maplinear(&b->c, sg.tau1, sg.gradient, sg.mapstp, b->refpt,
-                                 b->charge, b->mass, xm));
-    for (int blk = 0; blk < 3; blk++) {
-      const int r = 2 * blk;
-      p.m[4 * blk + 0] = xm[6 * r + r];
-      p.m[4 * blk + 1] = xm[6 * r + r + 1];
-      p.m[4 * blk + 2] = xm[6 * (r + 1) + r];
-      p.m[4 * blk + 3] = xm[6 * (r + 1) + r + 1];
-    }
+    IZH_CHECK(izh_element_maplinear(&b->c, sg.tau1, sg.bitype, sg.v, sg.mapstp, b->refpt,
+                                    b->charge, b->mass, xm));
+    // the longitudinal plane is never coupled to the transverse ones
     for (int j = 0; j < 6; j++)
       for (int k = 0; k < 6; k++)
-        if (j / 2 != k / 2 && xm[6 * j + k] != 0.0)
-          return fail(IZH_ERR_STATE, "izh_run: quadrupole map is not block diagonal");
+        if ((j < 4) != (k < 4) && xm[6 * j + k] != 0.0)
+          return fail(IZH_ERR_STATE, "izh_run: element map couples phi/pt to x/y");
+    bool coupled = false;
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++)
+        if (j / 2 != k / 2 && xm[6 * j + k] != 0.0) coupled = true;
+    if (coupled) {
+      kind = kPushMapC;
+      for (int j = 0; j < 4; j++)
+        for (int k = 0; k < 4; k++) p.m[4 * j + k] = xm[6 * j + k];
+      p.m[16] = xm[6 * 4 + 4];
+      p.m[17] = xm[6 * 4 + 5];
+      p.m[18] = xm[6 * 5 + 4];
+      p.m[19] = xm[6 * 5 + 5];
+    } else {
+      kind = kPushMap;
+      for (int blk = 0; blk < 3; blk++) {
+        const int r = 2 * blk;
+        p.m[4 * blk + 0] = xm[6 * r + r];
+        p.m[4 * blk + 1] = xm[6 * r + r + 1];
+        p.m[4 * blk + 2] = xm[6 * (r + 1) + r];
+        p.m[4 * blk + 3] = xm[6 * (r + 1) + r + 1];
+      }
+    }
   } else {
-    return fail(IZH_ERR_ARG, "izh_run: bitype %d not supported", sg.bitype);
+    return fail(IZH_ERR_ARG, "izh_run: bitype %d: %s", sg.bitype,
+                why ? why : "the K-mode quadrupole runs through izh_step");
   }
   *z = *z + sg.tau1;
   return IZH_OK;

@@ -193,6 +193,32 @@ inline void touch_particles(izh_bunch *b) { b->keys_valid = 0; }
 
 // Launchers implemented in the kernel translation units.
 int launch_drift(izh_bunch *b, double tau, double gam0, double beta0);
+int launch_quad_kmode(izh_bunch *b, double tau, double kq, double qmass, double gambet0,
+                      double beta0);
+// The model drift1_BeamBunch applies to an element (src/Appl/BeamBunch.f90:
+// 276-449): kMap1Drift, kMap1Linear (a host 6x6 map) or kMap1QuadK; -1 with
+// *why set for the branches not on this path.
+enum { kMap1Drift = 0, kMap1Linear = 1, kMap1QuadK = 2 };
+inline int map1_kind(int bitype, const double v[4], const char **why) {
+  *why = nullptr;
+  if (bitype == 0) return kMap1Drift;
+  if (bitype == 1) {
+    // src/Appl/BeamBunch.f90:299-307: the file-ID slot selects the model
+    if (v[1] < -10.0) { *why = "undulator transfer map"; return -1; }
+    if (v[1] < -1.0e-6) return kMap1QuadK;
+    if (v[1] > 1.0e-5) { *why = "quadrupole fringe field from a file"; return -1; }
+    return kMap1Linear;
+  }
+  if (bitype == 2 || bitype == 3) {
+    // Param(5) < 0 selects the ideal RF cavity model (:324-330)
+    if (v[3] < 0.0) { *why = "Param(5) < 0 (ideal RF cavity branch)"; return -1; }
+    return kMap1Linear;
+  }
+  *why = "element type not on this path (0 drift, 1 quadrupole, 2 constant "
+         "focusing, 3 solenoid)";
+  return -1;
+}
+
 int launch_map_linear(izh_bunch *b, const double xm[36]);
 int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
                    int64_t *nlost);

@@ -63,6 +63,53 @@ k_push(double *__restrict__ x, double *__restrict__ px, double *__restrict__ y,
   }
 }
 
+// ------------------------------------------------------- K-mode quadrupole
+// transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607): each particle's
+// own strength kstr = (q/m)/(q/m)_0 * gambet0/|gamma beta| * K gives a
+// cos/sin (focusing) and cosh/sinh (defocusing) pair of 2x2 maps; the phase
+// slips by (1/beta_z - 1/beta0) tau/Scxl.  In place, one particle per lane.
+__global__ void __launch_bounds__(kBlock)
+k_quad_kmode(double *__restrict__ X, double *__restrict__ PX, double *__restrict__ Y,
+             double *__restrict__ PY, double *__restrict__ PH, const double *__restrict__ PT,
+             const uint32_t *__restrict__ orig, const double *__restrict__ qm0, int64_t n,
+             double tau, double kq, double qmass, double gam0, double gambet0, double beta0,
+             double scxl) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = X[i], px = PX[i], y = Y[i], py = PY[i];
+    const double gam = gam0 - PT[i];
+    double gambetz = sqrt(gam * gam - 1.0 - px * px - py * py);
+    const double betaz = gambetz / gam;
+    gambetz = sqrt(gam * gam - 1.0);
+    const double kstr = qm0[orig[i]] / qmass * gambet0 / gambetz * kq;
+    const double rtkstr = sqrt(fabs(kstr));
+    const double rtkstrzz = rtkstr * tau;
+    double xm11, xm12, xm21, xm22, xm33, xm34, xm43, xm44;
+    if (kstr != 0.0) {
+      const double co = cos(rtkstrzz), si = sin(rtkstrzz);
+      const double t = exp(rtkstrzz);
+      const double ch = (t * t + 1.0) / (t + t), sh = (t * t - 1.0) / (t + t);
+      // focusing plane: x when kstr > 0, y when kstr < 0
+      const double fc = co, fs = si, dc = ch, ds = sh;
+      if (kstr > 0.0) {
+        xm11 = fc; xm12 = fs / rtkstr; xm21 = -rtkstr * fs; xm22 = fc;
+        xm33 = dc; xm34 = ds / rtkstr; xm43 = ds * rtkstr; xm44 = dc;
+      } else {
+        xm11 = dc; xm12 = ds / rtkstr; xm21 = rtkstr * ds; xm22 = dc;
+        xm33 = fc; xm34 = fs / rtkstr; xm43 = -fs * rtkstr; xm44 = fc;
+      }
+    } else {
+      xm11 = 1.0; xm12 = tau; xm21 = 0.0; xm22 = 1.0;
+      xm33 = 1.0; xm34 = tau; xm43 = 0.0; xm44 = 1.0;
+    }
+    X[i] = xm11 * x + xm12 * px / gambetz / scxl;
+    PX[i] = gambetz * scxl * xm21 * x + xm22 * px;
+    Y[i] = xm33 * y + xm34 * py / gambetz / scxl;
+    PY[i] = gambetz * scxl * xm43 * y + xm44 * py;
+    PH[i] = PH[i] + (1.0 / betaz - 1.0 / beta0) * tau / scxl;
+  }
+}
+
 // -------------------------------------------------------------- conv1st
 // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352), in place.
 // Partials per block: 3 minima, 3 maxima, lost count.
@@ -332,6 +379,18 @@ int launch_drift(izh_bunch *b, double tau, double gam0, double beta0) {
   p.ibeta0 = 1.0 / beta0;
   p.scxl = b->c.scxl;
   return launch_push(b, p);
+}
+
+int launch_quad_kmode(izh_bunch *b, double tau, double kq, double qmass, double gambet0,
+                      double beta0) {
+  touch_particles(b);
+  if (b->n == 0) return IZH_OK;
+  ScopedTimer tm_(IZH_T_PUSH, b->stream);
+  hipLaunchKernelGGL(k_quad_kmode, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream,
+                     b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->orig, b->qm0,
+                     b->n, tau, kq, qmass, -b->refpt[5], gambet0, beta0, b->c.scxl);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
 }
 
 int launch_map_linear(izh_bunch *b, const double xm[36]) {

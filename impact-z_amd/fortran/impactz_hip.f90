@@ -40,12 +40,13 @@ module impactz_hip
     real(c_double) :: msize(3)
   end type izh_compdom
 
-  ! one segment of a drift/quadrupole lattice
+  ! one segment of a lattice: the element's half step, type, map steps,
+  ! values v1..v4 (its Param(2:5)) and pipe aperture
   type, bind(C) :: izh_segment
     real(c_double) :: tau1
     integer(c_int) :: bitype
-    real(c_double) :: gradient
     integer(c_int) :: mapstp
+    real(c_double) :: v(4)
     real(c_double) :: xrad, yrad
   end type izh_segment
 
@@ -118,6 +119,29 @@ module impactz_hip
       real(c_double) :: refpt(6)
       real(c_double), value :: charge, mass
       real(c_double) :: xm(36)
+      integer(c_int) :: rc
+    end function
+    function izh_element_maplinear(c, tau, bitype, v, mapstp, refpt, charge, mass, xm) &
+        bind(C, name="izh_element_maplinear") result(rc)
+      import :: c_int, c_double, izh_const
+      type(izh_const), intent(in) :: c
+      real(c_double), value :: tau
+      integer(c_int), value :: bitype
+      real(c_double), intent(in) :: v(4)
+      integer(c_int), value :: mapstp
+      real(c_double) :: refpt(6)
+      real(c_double), value :: charge, mass
+      real(c_double) :: xm(36)
+      integer(c_int) :: rc
+    end function
+    function izh_map1(b, z, tau, bitype, v, mapstp) bind(C, name="izh_map1") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double) :: z
+      real(c_double), value :: tau
+      integer(c_int), value :: bitype
+      real(c_double), intent(in) :: v(4)
+      integer(c_int), value :: mapstp
       integer(c_int) :: rc
     end function
     function izh_drift1(b, z, tau, bitype, gradient, mapstp) &
@@ -216,17 +240,14 @@ module impactz_hip
       real(c_double), intent(in) :: rho(*)
       integer(c_int) :: rc
     end function
-    function izh_step(b, f, dom, z, tau1, bitype, gradient, mapstp, xrad, yrad, &
-                      bcurr, moments) bind(C, name="izh_step") result(rc)
-      import :: c_int, c_double, c_ptr, izh_compdom
+    function izh_step(b, f, dom, z, seg, bcurr, moments) &
+        bind(C, name="izh_step") result(rc)
+      import :: c_int, c_double, c_ptr, izh_compdom, izh_segment
       type(c_ptr), value :: b, f
       type(izh_compdom) :: dom
       real(c_double) :: z
-      real(c_double), value :: tau1
-      integer(c_int), value :: bitype
-      real(c_double), value :: gradient
-      integer(c_int), value :: mapstp
-      real(c_double), value :: xrad, yrad, bcurr
+      type(izh_segment), intent(in) :: seg
+      real(c_double), value :: bcurr
       real(c_double) :: moments(44)
       integer(c_int) :: rc
     end function
@@ -297,14 +318,14 @@ contains
 
   ! drift1_BeamBunch(this,beamln,z,tau,bitype,nseg,nst,ihlf)
   ! (src/Appl/BeamBunch.f90:250-258): the element's data enter as bitype,
-  ! gradient and mapstp.
-  subroutine drift1_gpu(b, z, tau, bitype, gradient, mapstp)
+  ! its values v1..v4 (Param(2:5)) and mapstp.
+  subroutine drift1_gpu(b, z, tau, bitype, v, mapstp)
     type(c_ptr), intent(in) :: b
     real(c_double), intent(inout) :: z
-    real(c_double), intent(in) :: tau, gradient
+    real(c_double), intent(in) :: tau, v(4)
     integer, intent(in) :: bitype, mapstp
-    call izh_check(izh_drift1(b, z, tau, int(bitype, c_int), gradient, &
-                              int(mapstp, c_int)), 'drift1')
+    call izh_check(izh_map1(b, z, tau, int(bitype, c_int), v, int(mapstp, c_int)), &
+                   'drift1')
   end subroutine drift1_gpu
 
   ! conv1st_BeamBunch(this,tau,nplc,nptot,ptrange,Flagbc,perd,xrad,yrad)

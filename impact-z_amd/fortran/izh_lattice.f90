@@ -1,6 +1,7 @@
 !----------------------------------------------------------------------------
 ! izh_lattice -- a Fortran 90 host that marches a particle.in bunch through a
-! drift/quadrupole lattice on the GPU, calling the C ABI through the
+! lattice of drifts, quadrupoles, constant-focusing elements and solenoids
+! on the GPU, calling the C ABI through the
 ! impactz_hip module in the order of run_AccSimulator's segment loop
 ! (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1, Flagbc=1):
 !   drift1(tau1) -> conv1st -> update_CompDom -> charge -> update3O ->
@@ -8,7 +9,8 @@
 ! With fused = .true. the same segments go through izh_run in one call.
 !
 ! Input: particle.in (readin_Dist format, src/Appl/Distribution.f90:1677-1705)
-! and the namelist file named by the first command-line argument.  Output:
+! and the namelist file named by the first command-line argument; val(1:4,i)
+! holds element i's deck values v1..v4 (its Param(2:5)).  Output:
 ! one line per step with z, Npt and the 44 diagnostic1 reductions (37 sums,
 ! 6 absolute maxima, max x^2+y^2) in the file named by the namelist.
 !----------------------------------------------------------------------------
@@ -21,12 +23,12 @@ program izh_lattice
   real(8) :: current = 0.1d0, kinetic = 150.0d6, mass = 938.272088d6
   real(8) :: charge = 1.0d0, freq = 324.0d6, phase0 = 0.0d0
   integer :: nelem = 0
-  real(8) :: elen(maxelem) = 0, grad(maxelem) = 0, radius(maxelem) = 1
+  real(8) :: elen(maxelem) = 0, val(4, maxelem) = 0, radius(maxelem) = 1
   integer :: nseg(maxelem) = 1, mapstp(maxelem) = 1, bitype(maxelem) = 0
   logical :: fused = .false.
   character(len=256) :: particles = 'particle.in', output = 'izh_moments.txt'
   namelist /izh_run_nml/ nx, ny, nz, precision, device, current, kinetic, &
-      mass, charge, freq, phase0, nelem, elen, grad, radius, nseg, mapstp, &
+      mass, charge, freq, phase0, nelem, elen, val, radius, nseg, mapstp, &
       bitype, fused, particles, output
 
   character(len=256) :: nmlfile
@@ -69,7 +71,7 @@ program izh_lattice
       tau1 = 0.5d0 * elen(i) / nseg(i)
       tau2 = 2.0d0 * tau1
       do j = 1, nseg(i)
-        call drift1_gpu(b, z, tau1, bitype(i), grad(i), mapstp(i))
+        call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
         if (current >= 1.0e-10) then
           call conv1st_gpu(b, tau2, nplc, nptot, ptrange, 1, radius(i), radius(i))
           call update_compdom_gpu(dom, ptrange, 1)
@@ -78,7 +80,7 @@ program izh_lattice
           call cvbkforth1st_gpu(b)
           call kick1_gpu(b, tau2, f, dom, 1)
         end if
-        call drift1_gpu(b, z, tau1, bitype(i), grad(i), mapstp(i))
+        call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
         call diagnostic1_sums_gpu(b, sums, amax, r2max)
         call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
         istep = istep + 1
@@ -95,7 +97,7 @@ program izh_lattice
         k = k + 1
         segs(k)%tau1 = 0.5d0 * elen(i) / nseg(i)
         segs(k)%bitype = int(bitype(i), c_int)
-        segs(k)%gradient = grad(i)
+        segs(k)%v = val(:, i)
         segs(k)%mapstp = int(mapstp(i), c_int)
         segs(k)%xrad = radius(i)
         segs(k)%yrad = radius(i)

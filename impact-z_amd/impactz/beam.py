@@ -60,10 +60,15 @@ class Beam:
 
 @dataclasses.dataclass(frozen=True)
 class Element:
-    """One lattice line ``L nseg mapstp type v1 v2 v3 /`` (src/Contrl/Input.f90:307-314).
+    """One lattice line ``L nseg mapstp type v1 v2 v3 v4 /`` (src/Contrl/Input.f90:307-314;
+    the values become the element's Param(2:5), src/Contrl/AccSimulator.f90:262-320).
 
-    type 0: drift, v1 = pipe radius.  type 1: quadrupole, v1 = gradient [T/m],
-    v2 = file ID (0: constant gradient), v3 = pipe radius.
+    type 0: drift, v1 = pipe radius.
+    type 1: quadrupole, v1 = gradient [T/m] (or K [1/m^2] when v2 < 0: the
+            K-mode map), v2 = file ID (0: constant gradient), v3 = pipe radius.
+    type 2: constant focusing, v1..v3 = kx0^2, ky0^2, kz0^2 [1/m^2],
+            v4 = pipe radius.
+    type 3: solenoid, v1 = Bz0 [T], v2 = file ID, v3 = pipe radius.
     """
 
     length: float
@@ -73,11 +78,16 @@ class Element:
     v1: float
     v2: float = 0.0
     v3: float = 0.0
+    v4: float = 0.0
 
     @property
     def radius(self) -> float:
         # getradius_BeamLineElem (src/Appl/BeamLineElem.f90:500-552)
-        return self.v1 if self.type == 0 else self.v3
+        return {0: self.v1, 2: self.v4}.get(self.type, self.v3)
+
+    @property
+    def values(self) -> list[float]:
+        return [self.v1, self.v2, self.v3, self.v4]
 
 
 def fodo(periods: int, gradient: float = 10.0, quad_len: float = 0.1,
@@ -201,6 +211,6 @@ def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
             lines.append(f"{e.length!r} {e.nseg} {e.mapstp} 0 {e.v1!r} /")
         else:
             lines.append(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} {e.v1!r} "
-                         f"{e.v2!r} {e.v3!r} 0.0 0.0 0.0 0.0 0.0 /")
+                         f"{e.v2!r} {e.v3!r} {e.v4!r} 0.0 0.0 0.0 0.0 /")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -7,8 +7,10 @@ records are whitespace separated, lines whose first non-blank character is
 ``!`` are comments, Fortran ``d`` exponents are accepted and a lattice line
 ends at ``/``.  The element table follows the lattice branch of
 run_AccSimulator (src/Contrl/AccSimulator.f90:237-536, 742-1504) for the
-elements this build runs: drifts (type 0) and constant-gradient quadrupoles
-(type 1 with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270).  Zero-length
+elements this build runs: drifts (type 0), quadrupoles (type 1: constant
+gradient with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270, or given by
+K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
+(type 2) and solenoids (type 3).  Zero-length
 diagnostic elements that do not change the particles (types -2 to -8) are
 skipped; any other element type is refused with an error, as are decks
 outside Flagmap=1, Flagbc=1, Flagdist=19.
@@ -167,20 +169,30 @@ def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
         raise DeckError(f"{deck.nchrg} charge states: one is supported")
     out, notes = [], []
     for k, e in enumerate(deck.elements, start=1):
-        v = e.values + [0.0] * 3
+        v = e.values + [0.0] * 4
         if e.type == 0:
             out.append(Element(e.length, e.nseg, e.mapstp, 0, v[0]))
         elif e.type == 1:
             if v[1] > 1.0e-5:
                 raise DeckError(f"element {k}: quadrupole with field file ID {v[1]:g} "
                                 "(fringe field from file) is not on this path")
-            out.append(Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2]))
+            if v[1] < -10.0:
+                raise DeckError(f"element {k}: undulator map (file ID {v[1]:g}) is not "
+                                "on this path")
+            out.append(Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2], v[3]))
+        elif e.type in (2, 3):
+            # drift1_BeamBunch takes its ideal-RF branch when Param(5) < 0
+            # (src/Appl/BeamBunch.f90:324-330)
+            if v[3] < 0.0:
+                raise DeckError(f"element {k}: type {e.type} with Param(5) = {v[3]:g} < 0 "
+                                "(ideal RF cavity model) is not on this path")
+            out.append(Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3]))
         elif e.type in DIAGNOSTIC_TYPES and e.length == 0.0:
             notes.append(f"element {k}: type {e.type} ({DIAGNOSTIC_TYPES[e.type]}) "
                          "skipped: it writes files only")
         else:
             raise DeckError(f"element {k}: type {e.type} is not on this path "
-                            "(drift 0 and constant-gradient quadrupole 1 are)")
+                            "(drift 0, quadrupole 1, constant focusing 2 and solenoid 3 are)")
     return out, notes
 
 

@@ -32,7 +32,8 @@ SYMBOLS = (
     "izh_field_create", "izh_field_destroy", "izh_charge", "izh_update3O",
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
-    "izh_timer_read", "izh_run_stats", "izh_probe_copy",
+    "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
+    "izh_map1",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -57,9 +58,30 @@ class BunchInfo(C.Structure):
 
 
 class Segment(C.Structure):
-    _fields_ = [("tau1", C.c_double), ("bitype", C.c_int),
-                ("gradient", C.c_double), ("mapstp", C.c_int),
-                ("xrad", C.c_double), ("yrad", C.c_double)]
+    """izh_segment: an element's half step tau1, type, map steps, values
+    v1..v4 (the deck line's, i.e. Param(2:5)) and pipe aperture."""
+    _fields_ = [("tau1", C.c_double), ("bitype", C.c_int), ("mapstp", C.c_int),
+                ("v", C.c_double * 4), ("xrad", C.c_double), ("yrad", C.c_double)]
+
+
+def values4(v) -> np.ndarray:
+    """The four element values izh_map1 takes (a lone number: a gradient)."""
+    out = np.zeros(4)
+    if np.isscalar(v):
+        out[0] = float(v)
+    else:
+        vals = np.asarray(v, dtype=np.float64).ravel()[:4]
+        out[: vals.size] = vals
+    return out
+
+
+def segment(tau1: float, bitype: int, v, mapstp: int, xrad: float, yrad: float) -> Segment:
+    sg = Segment()
+    sg.tau1, sg.bitype, sg.mapstp = tau1, bitype, mapstp
+    for k, x in enumerate(values4(v)):
+        sg.v[k] = x
+    sg.xrad, sg.yrad = xrad, yrad
+    return sg
 
 
 class CompDomC(C.Structure):
@@ -93,6 +115,9 @@ def lib() -> C.CDLL:
     L.izh_quad_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_double,
                                      C.c_int, _dp, C.c_double, C.c_double, _dp]
     L.izh_drift1.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_double, C.c_int]
+    L.izh_element_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_int, _dp,
+                                        C.c_int, _dp, C.c_double, C.c_double, _dp]
+    L.izh_map1.argtypes = [vp, _dp, C.c_double, C.c_int, _dp, C.c_int]
     L.izh_map_linear.argtypes = [vp, _dp]
     L.izh_conv1st.argtypes = [vp, C.c_double, _dp, _i64p, _i64p, C.c_double,
                               C.c_double, C.c_int]
@@ -107,9 +132,8 @@ def lib() -> C.CDLL:
     L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
     L.izh_field_download.argtypes = [vp, C.c_int, _dp]
     L.izh_field_upload_rho.argtypes = [vp, _dp]
-    L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.c_double,
-                           C.c_int, C.c_double, C.c_int, C.c_double,
-                           C.c_double, C.c_double, _dp]
+    L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
+                           C.c_double, _dp]
     L.izh_run.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
                           C.c_int64, C.c_double, _dp]
     L.izh_run_stats.argtypes = [vp, _i64p]
@@ -165,6 +189,17 @@ def quad_maplinear(c: Const, tau, gradient, mapstp, refpt, charge, mass):
     xm = np.zeros(36)
     check(lib().izh_quad_maplinear(C.byref(c), tau, gradient, mapstp, _p(r),
                                    charge, mass, _p(xm)))
+    return xm.reshape(6, 6), r
+
+
+def element_maplinear(c: Const, tau, bitype, v, mapstp, refpt, charge, mass):
+    """maplinear_BeamLineElem (src/Appl/BeamLineElem.f90:776-825) for bitypes
+    1-3; host only.  Returns (xm 6x6, refpt after)."""
+    r = np.ascontiguousarray(refpt, dtype=np.float64).copy()
+    xm = np.zeros(36)
+    vv = values4(v)
+    check(lib().izh_element_maplinear(C.byref(c), tau, bitype, _p(vv), mapstp, _p(r),
+                                      charge, mass, _p(xm)))
     return xm.reshape(6, 6), r
 
 
@@ -295,9 +330,18 @@ class BeamBunch:
     # --- the reference's per-step procedures -------------------------------
     def drift1(self, z: float, tau: float, bitype: int, gradient: float = 0.0,
                mapstp: int = 1) -> float:
-        """map1_BeamBunch = drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465); returns z."""
+        """map1_BeamBunch = drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465) for a
+        drift or constant-gradient quadrupole; returns z."""
         zz = C.c_double(z)
         check(lib().izh_drift1(self.h, C.byref(zz), tau, bitype, gradient, mapstp))
+        return zz.value
+
+    def map1(self, z: float, tau: float, bitype: int, v, mapstp: int = 1) -> float:
+        """drift1_BeamBunch for every element izh_map1 takes; v = the element's
+        values v1..v4.  Returns z."""
+        zz = C.c_double(z)
+        vv = values4(v)
+        check(lib().izh_map1(self.h, C.byref(zz), tau, bitype, _p(vv), mapstp))
         return zz.value
 
     def conv1st(self, tau: float, xrad: float, yrad: float, flagbc: int = 1):
@@ -326,28 +370,26 @@ class BeamBunch:
         check(lib().izh_diagnostic1(self.h, _p(s), _p(m), C.byref(r2)))
         return s, m, r2.value
 
-    def step(self, field: FieldQuant, dom: CompDom, z: float, tau1: float,
-             bitype: int, gradient: float, mapstp: int, xrad: float,
-             yrad: float, bcurr: float, want_moments: bool = False):
+    def step(self, field: FieldQuant, dom: CompDom, z: float, seg: Segment,
+             bcurr: float, want_moments: bool = False):
         """One segment of run_AccSimulator (src/Contrl/AccSimulator.f90:1099-1488)."""
         zz = C.c_double(z)
         mom = np.zeros(44) if want_moments else None
-        check(lib().izh_step(self.h, field.h, C.byref(dom.c), C.byref(zz), tau1,
-                             bitype, gradient, mapstp, xrad, yrad, bcurr,
-                             _p(mom) if want_moments else None))
+        check(lib().izh_step(self.h, field.h, C.byref(dom.c), C.byref(zz), C.byref(seg),
+                             bcurr, _p(mom) if want_moments else None))
         return zz.value, mom
 
     def run(self, field: FieldQuant, dom: CompDom, z: float, segments,
             bcurr: float, want_moments: bool = True):
         """izh_run: consecutive segments as one device-resident pipeline.
 
-        segments: iterable of (tau1, bitype, gradient, mapstp, xrad, yrad).
+        segments: iterable of Segment (see segment()).
         Returns (z, records[nseg, 45] or None): per step the 44 diagnostic
         values of izh_step, then Npt."""
         segs = list(segments)
         arr = (Segment * max(len(segs), 1))()
         for k, sg in enumerate(segs):
-            arr[k] = Segment(*sg)
+            arr[k] = sg if isinstance(sg, Segment) else segment(*sg)
         zz = C.c_double(z)
         mom = np.zeros((len(segs), 45)) if want_moments else None
         check(lib().izh_run(self.h, field.h, C.byref(dom.c), C.byref(zz), arr,

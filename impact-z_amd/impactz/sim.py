@@ -1,8 +1,8 @@
-"""Run a drift/quadrupole lattice on the GPU in the reference's segment order.
+"""Run a linear-element lattice on the GPU in the reference's segment order.
 
 This is the host loop of run_AccSimulator (src/Contrl/AccSimulator.f90:742-1504)
 restricted to Flagmap=1, Flagbc=1, Flagsubstep=0, Flagerr=0 and element types
-0 (drift) and 1 (constant-gradient quadrupole); the per-step procedures all
+0 (drift), 1 (quadrupole), 2 (constant focusing) and 3 (solenoid); the per-step procedures all
 run on the device through the C ABI.
 """
 from __future__ import annotations
@@ -78,11 +78,8 @@ class Simulation:
                 yield e
 
     def step(self, e: Element, want_moments: bool = True):
-        tau1 = 0.5 * e.length / e.nseg
-        grad = e.v1 if e.type == 1 else 0.0
-        self.z, mom = self.bunch.step(self.field, self.dom, self.z, tau1,
-                                      e.type, grad, e.mapstp, e.radius,
-                                      e.radius, self.beam.current, want_moments)
+        self.z, mom = self.bunch.step(self.field, self.dom, self.z, self.segment_args(e),
+                                      self.beam.current, want_moments)
         self.nstep += 1
         return mom
 
@@ -90,9 +87,8 @@ class Simulation:
         """The same segment through the individual procedure entry points."""
         tau1 = 0.5 * e.length / e.nseg
         tau2 = 2.0 * tau1
-        grad = e.v1 if e.type == 1 else 0.0
         b = self.bunch
-        self.z = b.drift1(self.z, tau1, e.type, grad, e.mapstp)
+        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp)
         if self.beam.current >= float(np.float32(1.0e-10)):
             ptrange, _, _ = b.conv1st(tau2, e.radius, e.radius)
             self.dom.update(ptrange)
@@ -100,13 +96,13 @@ class Simulation:
             self.field.update3O(self.dom)
             b.cvbkforth1st()
             b.kick1(tau2, self.field, self.dom)
-        self.z = b.drift1(self.z, tau1, e.type, grad, e.mapstp)
+        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp)
         self.nstep += 1
 
-    def segment_args(self, e: Element):
+    def segment_args(self, e: Element) -> hip.Segment:
+        """The izh_segment of one space-charge step of element e."""
         tau1 = 0.5 * e.length / e.nseg
-        return (tau1, e.type, e.v1 if e.type == 1 else 0.0, e.mapstp,
-                e.radius, e.radius)
+        return hip.segment(tau1, e.type, e.values, e.mapstp, e.radius, e.radius)
 
     def run_fused(self, nsteps: int, start: int = 0, want_moments: bool = True):
         """Steps start..start+nsteps-1 of the lattice through izh_run.

@@ -89,15 +89,43 @@ int izh_bunch_upload(izh_bunch *b, const double *pts9, int64_t n);
 int izh_bunch_get_info(const izh_bunch *b, izh_bunch_info *info);
 int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info);
 
-/* maplinear_Quadrupole for a constant gradient (file ID 0): the 6x6 linear
- * map of a half step from RK6 integration (src/Appl/Quadrupole.f90:120-305).
- * Host only.  xm is row-major, xm[6*(j-1)+(k-1)] = xm(j,k); refpt updated. */
+/* Element values.  v[0..3] are the deck line's v1..v4, which become the
+ * element's Param(2:5) (src/Contrl/AccSimulator.f90:262-320):
+ *   bitype 0 drift:              pipe radius
+ *   bitype 1 quadrupole:         gradient T/m (K in 1/m^2 when v[1] < -1e-6),
+ *                                file ID, pipe radius
+ *   bitype 2 constant focusing:  kx0^2, ky0^2, kz0^2 (1/m^2), pipe radius
+ *   bitype 3 solenoid:           Bz0 (T), file ID, pipe radius, x offset */
+
+/* maplinear_BeamLineElem (src/Appl/BeamLineElem.f90:776-825) for a
+ * constant-gradient quadrupole (bitype 1, file ID <= 1e-5), constant
+ * focusing (2) and a solenoid (3): the 6x6 linear map of a step tau from
+ * RK6 integration over mapstp steps (maplinear_Quadrupole, _ConstFoc, _Sol:
+ * src/Appl/Quadrupole.f90:120-305, src/Appl/ConstFoc.f90:116-294,
+ * src/Appl/Sol.f90:120-321).  Host only.  xm is row-major,
+ * xm[6*(j-1)+(k-1)] = xm(j,k); refpt is updated. */
+int izh_element_maplinear(const izh_const *c, double tau, int bitype,
+                          const double v[4], int mapstp, double refpt[6],
+                          double charge, double mass, double xm[36]);
+
+/* The quadrupole case of izh_element_maplinear, v = {gradient, 0, 0, 0}
+ * (maplinear_Quadrupole, src/Appl/Quadrupole.f90:120-305). */
 int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
                        int mapstp, double refpt[6], double charge, double mass,
                        double xm[36]);
 
-/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465) for bitype 0 (exact
- * drift) and bitype 1 (constant-gradient quadrupole).  *z += tau. */
+/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465): the exact drift
+ * (bitype 0); a quadrupole (1) through its linear map, or through the
+ * chromatic K-mode map transfmapK_Quadrupole (src/Appl/Quadrupole.f90:
+ * 509-607) when v[1] < -1e-6; constant focusing (2) and a solenoid (3)
+ * through their linear maps.  *z += tau.  Refused with IZH_ERR_ARG: the
+ * undulator map (bitype 1, v[1] < -10), quadrupole fields from a file
+ * (v[1] > 1e-5), the ideal RF-cavity branch (bitype 2 or 3 with
+ * Param(5) = v[3] < 0) and other element types. */
+int izh_map1(izh_bunch *b, double *z, double tau, int bitype, const double v[4],
+             int mapstp);
+
+/* izh_map1 for bitype 0 and 1 with v = {gradient, 0, 0, 0}. */
 int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
                double gradient, int mapstp);
 /* The map-apply half of drift1_BeamBunch for any host-computed 6x6 map
@@ -144,28 +172,28 @@ int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
 int izh_field_download(izh_field *f, int which, double *host);
 int izh_field_upload_rho(izh_field *f, const double *rho);
 
-/* One segment of the reference loop for a drift (bitype 0) or quadrupole
- * (bitype 1) element: drift1(tau1), conv1st, update_CompDom, charge,
- * update3O, cvbkforth1st, kick1(2 tau1), drift1(tau1)
- * (src/Contrl/AccSimulator.f90:1099-1488 with Flagmap=1, Flagbc=1,
- * Flagsubstep=0).  moments (NULL or 44 doubles) receives the
- * diagnostic1_Output sums of the state after the step: sums[37], amax[6],
- * r2max.  bcurr is the deck current (no space charge below 1e-10 A). */
-int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
-             double tau1, int bitype, double gradient, int mapstp,
-             double xrad, double yrad, double bcurr, double *moments);
-
-/* One segment of a drift/quadrupole lattice for izh_run. */
+/* One segment of a lattice: an element's half step and values. */
 typedef struct {
   double tau1;        /* half step, L/(2*nseg) */
-  int bitype;         /* 0 drift, 1 constant-gradient quadrupole */
-  double gradient;    /* T/m, bitype 1 */
-  int mapstp;         /* RK6 steps of the quadrupole map */
+  int bitype;         /* 0 drift, 1 quadrupole, 2 constant focusing, 3 solenoid */
+  int mapstp;         /* RK6 steps of the element's linear map */
+  double v[4];        /* the element's values, as for izh_map1 */
   double xrad, yrad;  /* pipe aperture of the loss test */
 } izh_segment;
 
+/* One segment of the reference loop: map1(tau1), conv1st, update_CompDom,
+ * charge, update3O, cvbkforth1st, kick1(2 tau1), map1(tau1)
+ * (src/Contrl/AccSimulator.f90:1099-1488 with Flagmap=1, Flagbc=1,
+ * Flagsubstep=0), for every element izh_map1 takes.  moments (NULL or 44
+ * doubles) receives the diagnostic1_Output sums of the state after the step:
+ * sums[37], amax[6], r2max.  bcurr is the deck current (no space charge
+ * below 1e-10 A). */
+int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+             const izh_segment *seg, double bcurr, double *moments);
+
 /* nseg consecutive izh_step calls (the element/segment loop of
- * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504) run as one
+ * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504; every element of
+ * izh_map1 except the K-mode quadrupole, which runs through izh_step) run as one
  * device-resident pipeline: each step's kick, second half push and moment
  * sums are fused with the next step's first half push and z->t transform in
  * one pass over the particles, which are re-sorted into 8x8x8-cell tiles in

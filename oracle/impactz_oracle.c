@@ -35,20 +35,41 @@ void or_physconst(double freq, or_const *c) {
 
 /* ---------------------------------------------------------------- maps */
 
-/* intfunc1_Quadrupole, constant gradient (src/Appl/Quadrupole.f90:254-305). */
-static void quad_rhs(const or_const *c, double bgrad, double qmcc,
-                     const double *y, double *f) {
+/* The linear ODE of the element maps: the reference particle's phase and
+ * energy (y1..y6), cosine- and sine-like solutions of three 2x2 systems
+ * (y7..y18) and, for the solenoid, its rotation pair (y19..y22).
+ *   quadrupole: intfunc1_Quadrupole, constant gradient
+ *               (src/Appl/Quadrupole.f90:254-305), p[0] = gradient;
+ *   ConstFoc:   intfunc1_ConstFoc (src/Appl/ConstFoc.f90:245-293),
+ *               p = kx0^2, ky0^2, kz0^2;
+ *   solenoid:   intfunc1_Sol (src/Appl/Sol.f90:271-321), p[0] = Bz0. */
+static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
+                    const double *y, double *f) {
   double gamma0 = -y[5];
   double beta0 = sqrt(gamma0 * gamma0 - 1.0) / gamma0;
   double gbet = sqrt((gamma0 - 1.0) * (gamma0 + 1.0));
-  double brho, s11, s33, s55;
+  double brho, s11, s33, s55, c2alpha = 0.0;
   f[0] = f[1] = f[2] = f[3] = 0.0;
   f[4] = 1.0 / (beta0 * c->scxl);
   f[5] = 0.0;
-  brho = gbet / c->clight / qmcc;
-  s11 = bgrad / brho * c->scxl;
-  s33 = -bgrad / brho * c->scxl;
-  s55 = 0.0;
+  if (type == 1) {
+    brho = gbet / c->clight / qmcc;
+    s11 = pv[0] / brho * c->scxl;
+    s33 = -pv[0] / brho * c->scxl;
+    s55 = 0.0;
+  } else if (type == 2) {
+    s11 = pv[0] * c->scxl;
+    s33 = pv[1] * c->scxl;
+    s55 = pv[2] * c->scxl;
+  } else {
+    double k;
+    brho = gbet / c->clight / qmcc;
+    k = 0.5 * pv[0] / brho;
+    s11 = (k * k) * c->scxl;
+    s33 = (k * k) * c->scxl;
+    s55 = 0.0;
+    c2alpha = -pv[0] / brho / 2;
+  }
   f[6] = y[7] / c->scxl;
   f[7] = -s11 * y[6];
   f[8] = y[9] / c->scxl;
@@ -61,17 +82,31 @@ static void quad_rhs(const or_const *c, double bgrad, double qmcc,
   f[15] = -s55 * y[14];
   f[16] = y[17] / c->scxl;
   f[17] = -s55 * y[16];
+  if (type == 3) {
+    f[18] = -c2alpha * y[20];
+    f[19] = -c2alpha * y[21];
+    f[20] = c2alpha * y[18];
+    f[21] = c2alpha * y[19];
+  }
 }
 
-void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
-                 double refpt[6], double charge, double mass, double xm[36]) {
-  /* maplinear_Quadrupole + rk6i_Quadrupole (src/Appl/Quadrupole.f90:120-252) */
-  double y[18], yt[18], f[18], a[18], b[18], cc[18], d[18], e[18], g[18],
-      o[18], p[18];
+int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
+                   int mapstp, double refpt[6], double charge, double mass,
+                   double xm[36]) {
+  /* maplinear_Quadrupole / _ConstFoc / _Sol with rk6i_* (src/Appl/
+   * Quadrupole.f90:120-252, ConstFoc.f90:116-243, Sol.f90:120-268) */
+  enum { NV = 22 };
+  double y[NV], yt[NV], f[NV], a[NV], b[NV], cc[NV], d[NV], e[NV], g[NV], o[NV],
+      p[NV];
   double h, gi, betai, ui, squi, sqi3, gf, betaf, uf, squf, sqf3;
   double qmcc = charge / mass;
-  int i, k;
+  int i, k, nv;
+  if (bitype < 1 || bitype > 3 || mapstp < 1) return -1;
+  /* a quadrupole's field from a file (file ID > 1e-5) is not restated */
+  if (bitype == 1 && v[1] > 1.0e-5) return -1;
+  nv = bitype == 3 ? 22 : 18;
   memset(y, 0, sizeof y);
+  memset(f, 0, sizeof f);
   y[4] = refpt[4];
   y[5] = refpt[5];
   y[6] = 1.0;
@@ -80,6 +115,8 @@ void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
   y[13] = 1.0;
   y[14] = 1.0;
   y[17] = 1.0;
+  y[18] = 1.0;
+  y[21] = 1.0;
   h = tau / mapstp;
   gi = -refpt[5];
   betai = sqrt(gi * gi - 1.0) / gi;
@@ -87,44 +124,44 @@ void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
   squi = sqrt(ui);
   sqi3 = squi * squi * squi;
   for (i = 0; i < mapstp; i++) {
-    quad_rhs(c, grad, qmcc, y, f);
-    for (k = 0; k < 18; k++) { a[k] = h * f[k]; yt[k] = y[k] + a[k] / 9.0; }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, y, f);
+    for (k = 0; k < nv; k++) { a[k] = h * f[k]; yt[k] = y[k] + a[k] / 9.0; }
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       b[k] = h * f[k];
       yt[k] = y[k] + (a[k] + 3.0 * b[k]) / 24.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       cc[k] = h * f[k];
       yt[k] = y[k] + (a[k] - 3.0 * b[k] + 4.0 * cc[k]) / 6.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       d[k] = h * f[k];
       yt[k] = y[k] + (278.0 * a[k] - 945.0 * b[k] + 840.0 * cc[k] + 99.0 * d[k]) / 544.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       e[k] = h * f[k];
       yt[k] = y[k] + (-106.0 * a[k] + 273.0 * b[k] - 104.0 * cc[k] - 107.0 * d[k] +
                       48.0 * e[k]) / 6.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       g[k] = h * f[k];
       yt[k] = y[k] + (110974.0 * a[k] - 236799.0 * b[k] + 68376.0 * cc[k] +
                       103803.0 * d[k] - 10240.0 * e[k] + 1926.0 * g[k]) / 45648.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       o[k] = h * f[k];
       yt[k] = y[k] + (-101195.0 * a[k] + 222534.0 * b[k] - 71988.0 * cc[k] -
                       26109.0 * d[k] - 2.0e4 * e[k] - 72.0 * g[k] + 22824.0 * o[k]) /
                          25994.0;
     }
-    quad_rhs(c, grad, qmcc, yt, f);
-    for (k = 0; k < 18; k++) {
+    lin_rhs(c, bitype, v, qmcc, yt, f);
+    for (k = 0; k < nv; k++) {
       p[k] = h * f[k];
       y[k] = y[k] + (41.0 * a[k] + 216.0 * cc[k] + 27.0 * d[k] + 272.0 * e[k] +
                      27.0 * g[k] + 216.0 * o[k] + 41.0 * p[k]) / 840.0;
@@ -139,23 +176,118 @@ void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
   sqf3 = squf * squf * squf;
   /* xm row-major: xm[6*(j-1)+(k-1)] = xm(j,k); dlti = dltf = thli = thlf = 0 */
   for (k = 0; k < 36; k++) xm[k] = 0.0;
-  xm[0] = y[6] * squi / squf;
-  xm[6] = y[7] * squi * squf;
-  xm[1] = y[8] / (squi * squf);
-  xm[7] = y[9] * squf / squi;
-  xm[14] = y[10] * squi / squf;
-  xm[20] = y[11] * squi * squf;
-  xm[15] = y[12] / (squi * squf);
-  xm[21] = y[13] * squf / squi;
+  if (bitype == 3) {
+    /* src/Appl/Sol.f90:187-207: each transverse block times the rotation */
+    xm[0] = y[18] * (y[6] * squi / squf);
+    xm[6] = y[18] * (y[7] * squi * squf);
+    xm[1] = y[18] * y[8] / (squi * squf);
+    xm[7] = y[18] * y[9] * squf / squi;
+    xm[2] = y[19] * (y[6] * squi / squf);
+    xm[8] = y[19] * (y[7] * squi * squf);
+    xm[3] = y[19] * y[8] / (squi * squf);
+    xm[9] = y[19] * y[9] * squf / squi;
+    xm[12] = y[20] * (y[10] * squi / squf);
+    xm[18] = y[20] * (y[11] * squi * squf);
+    xm[13] = y[20] * y[12] / (squi * squf);
+    xm[19] = y[20] * y[13] * squf / squi;
+    xm[14] = y[21] * (y[10] * squi / squf);
+    xm[20] = y[21] * (y[11] * squi * squf);
+    xm[15] = y[21] * y[12] / (squi * squf);
+    xm[21] = y[21] * y[13] * squf / squi;
+  } else {
+    xm[0] = y[6] * squi / squf;
+    xm[6] = y[7] * squi * squf;
+    xm[1] = y[8] / (squi * squf);
+    xm[7] = y[9] * squf / squi;
+    xm[14] = y[10] * squi / squf;
+    xm[20] = y[11] * squi * squf;
+    xm[15] = y[12] / (squi * squf);
+    xm[21] = y[13] * squf / squi;
+  }
   xm[28] = y[14] * sqi3 / sqf3;
   xm[34] = y[15] * sqi3 * sqf3;
   xm[29] = y[16] / (sqi3 * sqf3);
   xm[35] = y[17] * sqf3 / sqi3;
+  return 0;
 }
 
-void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
-             double *z, double tau, int bitype, double grad, int mapstp,
-             double charge, double mass) {
+void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
+                 double refpt[6], double charge, double mass, double xm[36]) {
+  const double v[4] = {grad, 0.0, 0.0, 0.0};
+  or_element_map(c, tau, 1, v, mapstp, refpt, charge, mass, xm);
+}
+
+/* transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607): the chromatic
+ * map of a quadrupole given by K = G/Brho, per particle. */
+static void quad_kmode(const or_const *c, double *pts, int64_t n, double refpt[6],
+                       double tau, double kq, double qmass) {
+  double gambet0 = sqrt(refpt[5] * refpt[5] - 1.0);
+  double beta0 = sqrt(1.0 - 1.0 / (refpt[5] * refpt[5]));
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    double gam = -refpt[5] - P(6, i);
+    double gambetz = sqrt(gam * gam - 1.0 - P(2, i) * P(2, i) - P(4, i) * P(4, i));
+    double betaz = gambetz / gam;
+    double kstr, rtkstr, rtkstrzz, cs, ss, t;
+    double xm11, xm12, xm21, xm22, xm33, xm34, xm43, xm44, tmp[4];
+    gambetz = sqrt(gam * gam - 1.0);
+    kstr = P(7, i) / qmass * gambet0 / gambetz * kq;
+    rtkstr = sqrt(fabs(kstr));
+    rtkstrzz = rtkstr * tau;
+    if (kstr > 0.0) {
+      cs = cos(rtkstrzz);
+      ss = sin(rtkstrzz);
+      xm11 = cs;
+      xm12 = ss / rtkstr;
+      xm21 = -rtkstr * ss;
+      xm22 = cs;
+      t = exp(rtkstrzz);
+      cs = (t * t + 1.0) / (t + t);
+      ss = (t * t - 1.0) / (t + t);
+      xm33 = cs;
+      xm34 = ss / rtkstr;
+      xm43 = ss * rtkstr;
+      xm44 = cs;
+    } else if (kstr < 0.0) {
+      t = exp(rtkstrzz);
+      cs = (t * t + 1.0) / (t + t);
+      ss = (t * t - 1.0) / (t + t);
+      xm11 = cs;
+      xm12 = ss / rtkstr;
+      xm21 = rtkstr * ss;
+      xm22 = cs;
+      cs = cos(rtkstrzz);
+      ss = sin(rtkstrzz);
+      xm33 = cs;
+      xm34 = ss / rtkstr;
+      xm43 = -ss * rtkstr;
+      xm44 = cs;
+    } else {
+      xm11 = 1.0;
+      xm12 = tau;
+      xm21 = 0.0;
+      xm22 = 1.0;
+      xm33 = 1.0;
+      xm34 = tau;
+      xm43 = 0.0;
+      xm44 = 1.0;
+    }
+    tmp[0] = xm11 * P(1, i) + xm12 * P(2, i) / gambetz / c->scxl;
+    tmp[1] = gambetz * c->scxl * xm21 * P(1, i) + xm22 * P(2, i);
+    tmp[2] = xm33 * P(3, i) + xm34 * P(4, i) / gambetz / c->scxl;
+    tmp[3] = gambetz * c->scxl * xm43 * P(3, i) + xm44 * P(4, i);
+    P(5, i) = P(5, i) + (1.0 / betaz - 1.0 / beta0) * tau / c->scxl;
+    P(1, i) = tmp[0];
+    P(2, i) = tmp[1];
+    P(3, i) = tmp[2];
+    P(4, i) = tmp[3];
+  }
+  refpt[4] = refpt[4] + tau / beta0 / c->scxl;
+}
+
+int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
+            double *z, double tau, int bitype, const double v[4], int mapstp,
+            double charge, double mass) {
   int64_t i;
   if (bitype == 0) {
     /* exact drift, src/Appl/BeamBunch.f90:276-298 */
@@ -171,10 +303,16 @@ void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
       P(5, i) = P(5, i) - (1.0 / beta0 + (tpt - gam0) / tmph) * tau / c->scxl;
     }
     refpt[4] = refpt[4] + tau / (c->scxl * beta0);
+  } else if (bitype == 1 && v[1] < -10.0) {
+    return -1;  /* undulator map (src/Appl/BeamBunch.f90:302-304): not restated */
+  } else if (bitype == 1 && v[1] < -1.0e-6) {
+    quad_kmode(c, pts, n, refpt, tau, v[0], charge / mass);  /* :305-307 */
   } else {
-    /* 6x6 map, src/Appl/BeamBunch.f90:308-320 */
+    /* 6x6 map, src/Appl/BeamBunch.f90:308-320, 436-449 */
     double xm[36];
-    or_quad_map(c, tau, grad, mapstp, refpt, charge, mass, xm);
+    if (bitype >= 2 && v[3] < 0.0) return -1;  /* ideal RF cavity model (:325-329) */
+    if (or_element_map(c, tau, bitype, v, mapstp, refpt, charge, mass, xm) != 0)
+      return -1;
     for (i = 0; i < n; i++) {
       double t[6];
       int j, k;
@@ -186,6 +324,7 @@ void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
     }
   }
   *z = *z + tau;
+  return 0;
 }
 
 /* ------------------------------------------------------- frame changes */
@@ -677,7 +816,7 @@ void or_diag_row(const or_const *c, const double t[37], const double gm[6],
 /* ------------------------------------------------------------ one step */
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                int nx, int ny, int nz, double tau1, int bitype, double grad,
+                int nx, int ny, int nz, double tau1, int bitype, const double v[4],
                 int mapstp, double pipe_radius, double *rho_out,
                 double *phi_out) {
   size_t g = (size_t)nx * ny * nz;
@@ -685,10 +824,9 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
   double *phi = phi_out ? phi_out : (double *)malloc(g * sizeof(double));
   double ptrange[6], range[6], msize[3];
   const double tau2 = 2.0 * tau1;
-  int rc = 0;
-  or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, grad, mapstp, st->charge,
-          st->mass);
-  if (st->bcurr >= (double)1.0e-10f) {
+  int rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, v, mapstp, st->charge,
+                   st->mass);
+  if (rc == 0 && st->bcurr >= (double)1.0e-10f) {
     n = or_conv1st(c, pts, n, st->refpt, pipe_radius, pipe_radius, ptrange,
                    &st->npt, &st->current);
     or_update_compdom(ptrange, nx, ny, nz, range, msize);
@@ -699,8 +837,9 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
       or_kick1(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, st->current,
                tau2, st->mass, st->charge);
   }
-  or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, grad, mapstp, st->charge,
-          st->mass);
+  if (rc == 0)
+    rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, v, mapstp, st->charge,
+                 st->mass);
   if (!rho_out) free(rho);
   if (!phi_out) free(phi);
   return rc == 0 ? n : -1;

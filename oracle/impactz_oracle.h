@@ -25,16 +25,33 @@ typedef struct {
 
 void or_physconst(double freq, or_const *c);
 
-/* Quadrupole linear map, file-ID-0 gradient mode: RK6 over mapstp steps
- * (src/Appl/Quadrupole.f90:120-305).  refpt is updated as the reference does. */
+/* Element values: v[0..3] = the deck line's v1..v4, i.e. the element's
+ * Param(2:5) (src/Contrl/AccSimulator.f90:262-320).
+ *   bitype 1 quadrupole: gradient T/m (K in 1/m^2 when v[1] < -1e-6),
+ *            file ID, pipe radius;
+ *   bitype 2 constant focusing: kx0^2, ky0^2, kz0^2, pipe radius;
+ *   bitype 3 solenoid: Bz0, file ID, pipe radius, x misalignment. */
+
+/* The element's 6x6 linear map: RK6 over mapstp steps of maplinear_
+ * Quadrupole (constant gradient), _ConstFoc or _Sol (src/Appl/Quadrupole.f90:
+ * 120-305, ConstFoc.f90:116-294, Sol.f90:120-321).  refpt is updated as the
+ * reference does.  Returns -1 for anything else. */
+int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
+                   int mapstp, double refpt[6], double charge, double mass,
+                   double xm[36]);
+
+/* The quadrupole case of or_element_map with v = {grad, 0, 0, 0}. */
 void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
                  double refpt[6], double charge, double mass, double xm[36]);
 
-/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465) for bitype 0 (exact drift)
- * and bitype 1 (quad map).  z is advanced by tau. */
-void or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
-             double *z, double tau, int bitype, double grad, int mapstp,
-             double charge, double mass);
+/* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465): bitype 0 exact drift,
+ * bitype 1 quadrupole (the K-mode chromatic map transfmapK_Quadrupole,
+ * src/Appl/Quadrupole.f90:509-607, when v[1] < -1e-6), bitypes 2 and 3
+ * through their linear maps.  z is advanced by tau.  Returns -1 for the
+ * branches not restated (undulator, ideal RF cavity, field files). */
+int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
+            double *z, double tau, int bitype, const double v[4], int mapstp,
+            double charge, double mass);
 
 /* conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352,1657-1663).
  * Compacts lost particles in place; returns the new count; rescales *current. */
@@ -104,7 +121,7 @@ void or_diag_row(const or_const *c, const double sums[37], const double amax[6],
                  double out[41]);
 
 /* One segment of the reference loop (src/Contrl/AccSimulator.f90:1099-1488)
- * for a drift (bitype 0) or constant-gradient quad (bitype 1):
+ * for an element or_map1 supports:
  * map1(tau1) -> conv1st -> update_CompDom -> deposit -> solve -> cvbkforth1st
  * -> kick1(2 tau1) -> map1(tau1).  Workspace is allocated internally.
  * Returns the particle count after the step (<0 on error). */
@@ -117,7 +134,7 @@ typedef struct {
 } or_bunch_state;
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                int nx, int ny, int nz, double tau1, int bitype, double grad,
+                int nx, int ny, int nz, double tau1, int bitype, const double v[4],
                 int mapstp, double pipe_radius, double *rho_out,
                 double *phi_out);
 

@@ -56,9 +56,13 @@ def lib() -> C.CDLL:
         L.or_physconst.argtypes = [C.c_double, C.POINTER(OrConst)]
         L.or_quad_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_double,
                                   C.c_int, _dp, C.c_double, C.c_double, _dp]
+        L.or_element_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_int, _dp,
+                                     C.c_int, _dp, C.c_double, C.c_double, _dp]
+        L.or_element_map.restype = C.c_int
         L.or_map1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp,
-                              C.c_double, C.c_int, C.c_double, C.c_int,
+                              C.c_double, C.c_int, _dp, C.c_int,
                               C.c_double, C.c_double]
+        L.or_map1.restype = C.c_int
         L.or_conv1st.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp,
                                  C.c_double, C.c_double, _dp, _i64p, _dp]
         L.or_conv1st.restype = C.c_int64
@@ -82,7 +86,7 @@ def lib() -> C.CDLL:
                                   C.c_int64, _dp, C.c_double, _dp]
         L.or_step.argtypes = [C.POINTER(OrConst), _dp, C.c_int64,
                               C.POINTER(OrState), C.c_int, C.c_int, C.c_int,
-                              C.c_double, C.c_int, C.c_double, C.c_int,
+                              C.c_double, C.c_int, _dp, C.c_int,
                               C.c_double, _dp, _dp]
         L.or_step.restype = C.c_int64
         _lib = L
@@ -107,10 +111,34 @@ def quad_map(c: OrConst, tau, grad, mapstp, refpt, charge, mass):
     return xm.reshape(6, 6), refpt
 
 
-def map1(c, pts, refpt, z, tau, bitype, grad, mapstp, charge, mass):
+def element_values(v) -> np.ndarray:
+    """v1..v4 of an element (a lone number is a quadrupole gradient)."""
+    if np.isscalar(v):
+        return np.array([float(v), 0.0, 0.0, 0.0])
+    out = np.zeros(4)
+    vals = np.asarray(v, dtype=np.float64).ravel()[:4]
+    out[: vals.size] = vals
+    return out
+
+
+def element_map(c, tau, bitype, v, mapstp, refpt, charge, mass):
+    refpt = np.ascontiguousarray(refpt, dtype=np.float64).copy()
+    xm = np.zeros(36)
+    vv = element_values(v)
+    if lib().or_element_map(C.byref(c), tau, bitype, _p(vv), mapstp, _p(refpt),
+                            charge, mass, _p(xm)) != 0:
+        raise ValueError(f"no linear map for element type {bitype}")
+    return xm.reshape(6, 6), refpt
+
+
+def map1(c, pts, refpt, z, tau, bitype, v, mapstp, charge, mass):
+    """drift1_BeamBunch in place on pts; v = the element's v1..v4 (or a
+    quadrupole gradient).  Returns the new z."""
     zz = C.c_double(z)
-    lib().or_map1(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz),
-                  tau, bitype, grad, mapstp, charge, mass)
+    vv = element_values(v)
+    if lib().or_map1(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz),
+                     tau, bitype, _p(vv), mapstp, charge, mass) != 0:
+        raise ValueError(f"element type {bitype} with values {vv} is not restated")
     return zz.value
 
 
@@ -228,9 +256,9 @@ class Run:
         rho = np.zeros((nz, ny, nx)) if want_fields else None
         phi = np.zeros((nz, ny, nx)) if want_fields else None
         tau1 = 0.5 * e.length / e.nseg
+        vv = element_values([e.v1, e.v2, e.v3, e.v4])
         n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
-                          nx, ny, nz, tau1, e.type, e.v1 if e.type == 1 else 0.0,
-                          e.mapstp, e.radius,
+                          nx, ny, nz, tau1, e.type, _p(vv), e.mapstp, e.radius,
                           _p(rho) if want_fields else None,
                           _p(phi) if want_fields else None)
         if n < 0:

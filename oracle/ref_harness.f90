@@ -4,7 +4,8 @@
 ! Links against the reference's own compiled modules and drives ONE
 ! space-charge step sequence exactly in the order of the reference's
 ! segment loop (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1,
-! Flagbc=1, Flagsubstep=0, Flagerr=0, drift and quadrupole elements only),
+! Flagbc=1, Flagsubstep=0, Flagerr=0; drift, quadrupole, constant-focusing
+! and solenoid elements),
 ! writing the intermediate quantities the reference never writes itself:
 ! the particle array after every stage, the conv1st range, the updated
 ! domain, the charge density and the potential.
@@ -14,9 +15,12 @@
 !   N Nx Ny Nz
 !   curr kinetic_eV mass_eV charge freq phase0
 !   nelem
-!   nelem lines:  length nseg mapstp type v1 v2 v3
-!                 (type 0: v1 = pipe radius; type 1: v1 = gradient T/m,
-!                  v2 = file ID, v3 = pipe radius)
+!   nelem lines:  length nseg mapstp type v1 v2 v3 v4
+!                 (the deck line's values, src/Contrl/AccSimulator.f90:
+!                  262-320: type 0: v1 = pipe radius; type 1: v1 = gradient
+!                  T/m (or K, 1/m^2, when v2 < 0), v2 = file ID, v3 = pipe
+!                  radius; type 2: v1..v3 = kx0^2, ky0^2, kz0^2, v4 = pipe
+!                  radius; type 3: v1 = Bz0, v2 = file ID, v3 = pipe radius)
 !   nsteps ndump
 !   ndump step indices (1-based) to dump
 !   particle file name (stream: int32 N, then Pts1(9,N) doubles)
@@ -42,19 +46,22 @@
         type (FieldQuant) :: Potential
         type (DriftTube), target, dimension(:), allocatable :: drifts
         type (Quadrupole), target, dimension(:), allocatable :: quads
+        type (ConstFoc), target, dimension(:), allocatable :: cfs
+        type (Sol), target, dimension(:), allocatable :: sols
         type (BeamLineElem), dimension(:), allocatable :: Blnelem
         integer :: N, Nx, Ny, Nz, nelem, nsteps, ndump, ierr, i, j, k
-        integer :: bnseg, bmpstp, bitype, ihlf, nstep, idr, iqr
+        integer :: bnseg, bmpstp, bitype, ihlf, nstep, idr, iqr, icf, isl
         integer :: Nplocal, Np, nchrg, u
         integer, dimension(:), allocatable :: etype, enseg, empstp, dumpat
         integer, dimension(1) :: nptlist0, idchg
         double precision, dimension(1) :: qmcclist0
-        double precision, dimension(:), allocatable :: elen, ev1, ev2, ev3
+        double precision, dimension(:), allocatable :: elen, ev1, ev2, ev3, ev4
         double precision :: curr, kin, mass, chg, freq, ph0, gambet
         double precision :: z, tau1, tau2, blength, piperad, piperad2
         double precision, dimension(21) :: distparam
         double precision, dimension(9) :: tmpdr
         double precision, dimension(9) :: tmpquad
+        double precision, dimension(5) :: tmpcf
         double precision, dimension(6) :: ptrange, range
         double precision, dimension(3) :: msize
         integer, dimension(3) :: lcgrid
@@ -70,10 +77,10 @@
         read(11,*) curr, kin, mass, chg, freq, ph0
         read(11,*) nelem
         allocate(elen(nelem), enseg(nelem), empstp(nelem), etype(nelem))
-        allocate(ev1(nelem), ev2(nelem), ev3(nelem))
+        allocate(ev1(nelem), ev2(nelem), ev3(nelem), ev4(nelem))
         do i = 1, nelem
           read(11,*) elen(i), enseg(i), empstp(i), etype(i), ev1(i), &
-                     ev2(i), ev3(i)
+                     ev2(i), ev3(i), ev4(i)
         enddo
         read(11,*) nsteps, ndump
         allocate(dumpat(max(ndump,1)))
@@ -116,8 +123,11 @@
 
         ! lattice: one target object per element, as init_AccSimulator does
         allocate(drifts(nelem), quads(nelem), Blnelem(nelem))
+        allocate(cfs(nelem), sols(nelem))
         idr = 0
         iqr = 0
+        icf = 0
+        isl = 0
         do i = 1, nelem
           if (etype(i) == 0) then
             idr = idr + 1
@@ -137,8 +147,30 @@
             tmpquad(4) = ev3(i)
             call setparam_Quadrupole(quads(iqr), tmpquad)
             Blnelem(i) = assign_BeamLineElem(quads(iqr))
+          else if (etype(i) == 2) then
+            icf = icf + 1
+            call construct_ConstFoc(cfs(icf), enseg(i), empstp(i), &
+                 etype(i), elen(i))
+            tmpcf = 0.0d0
+            tmpcf(2) = ev1(i)
+            tmpcf(3) = ev2(i)
+            tmpcf(4) = ev3(i)
+            tmpcf(5) = ev4(i)
+            call setparam_ConstFoc(cfs(icf), tmpcf)
+            Blnelem(i) = assign_BeamLineElem(cfs(icf))
+          else if (etype(i) == 3) then
+            isl = isl + 1
+            call construct_Sol(sols(isl), enseg(i), empstp(i), &
+                 etype(i), elen(i))
+            tmpquad = 0.0d0
+            tmpquad(2) = ev1(i)
+            tmpquad(3) = ev2(i)
+            tmpquad(4) = ev3(i)
+            tmpquad(5) = ev4(i)
+            call setparam_Sol(sols(isl), tmpquad)
+            Blnelem(i) = assign_BeamLineElem(sols(isl))
           else
-            print*, "harness supports drift (0) and quadrupole (1) only"
+            print*, "harness supports element types 0 to 3 only"
             stop
           endif
         enddo

@@ -29,8 +29,8 @@ def fixture_beam(d):
 
 def fixture_lattice(d):
     from impactz.beam import Element
-    return [Element(float(r[0]), int(r[1]), int(r[2]), int(r[3]), float(r[4]),
-                    float(r[5]), float(r[6])) for r in d["lattice"]]
+    return [Element(float(r[0]), int(r[1]), int(r[2]), int(r[3]),
+                    *(float(v) for v in r[4:8])) for r in d["lattice"]]
 
 
 def segment(d, step):

@@ -38,7 +38,7 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
         "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
                           beam.freq, beam.phase0]),
         "lattice": np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2,
-                              e.v3] for e in lattice]),
+                              e.v3, e.v4] for e in lattice]),
         "nsteps": np.array(nsteps),
         "dump_steps": np.array(dump_steps),
         "final": ref["final"],
@@ -64,6 +64,22 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
     return out
 
 
+def cf_sol_lattice():
+    """Constant focusing (src/Appl/ConstFoc.f90) and solenoid (src/Appl/Sol.f90)."""
+    return [B.Element(0.1, 4, 10, 2, 30.0, 20.0, 5.0, 1.0),
+            B.Element(0.2, 4, 10, 0, 1.0),
+            B.Element(0.1, 4, 10, 3, 1.5, 0.0, 1.0),
+            B.Element(0.2, 4, 10, 0, 1.0)]
+
+
+def kquad_lattice():
+    """Quadrupoles given by K (file ID < 0: transfmapK_Quadrupole)."""
+    return [B.Element(0.1, 4, 10, 1, 5.4, -1.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0),
+            B.Element(0.1, 4, 10, 1, -5.4, -1.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0)]
+
+
 def main():
     if not R.harness_available():
         raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
@@ -81,6 +97,12 @@ def main():
     # moment trajectory over two periods (fort.24-30 every step), no dumps
     fixture("fodo_traj", 2000, (16, 16, 32), B.fodo(2), 32, (), stages=())
     meta["fodo_traj"] = "2000 particles, 16x16x32, 32 steps, diagnostics only"
+    # the other linear elements: constant focusing, solenoid, K-mode quads
+    fixture("cf_sol", 1000, (16, 16, 16), cf_sol_lattice(), 16, (2, 10))
+    meta["cf_sol"] = ("constant focusing (type 2), drift, solenoid (type 3), drift; "
+                      "dumps at steps 2 (constant focusing) and 10 (solenoid)")
+    fixture("kquad", 1000, (16, 16, 16), kquad_lattice(), 16, (3,))
+    meta["kquad"] = "FODO of K-mode quadrupoles (file ID -1, K = +-5.4 1/m^2), dump at step 3"
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(meta, f, indent=1)
 

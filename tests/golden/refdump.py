@@ -95,7 +95,7 @@ def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
         f.write(f"{len(lattice)}\n")
         for e in lattice:
             f.write(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} {e.v1!r} "
-                    f"{e.v2!r} {e.v3!r}\n")
+                    f"{e.v2!r} {e.v3!r} {e.v4!r}\n")
         f.write(f"{nsteps} {len(dump_steps)}\n")
         if dump_steps:
             f.write(" ".join(str(s) for s in dump_steps) + "\n")

@@ -65,6 +65,37 @@ def test_quad_map_matches_reference(oracle, name, step):
     assert r3[4] == pytest.approx(float(d[f"s{step}_refpt"][4]), rel=1e-14)
 
 
+@pytest.mark.parametrize("bitype,v", [(1, [10.0, 0.0, 1.0, 0.0]), (1, [-7.5, 0.0, 1.0, 0.0]),
+                                      (2, [30.0, 20.0, 5.0, 1.0]), (2, [-4.0, 9.0, 0.0, 1.0]),
+                                      (3, [1.5, 0.0, 1.0, 0.0]), (3, [-0.8, 0.0, 1.0, 0.0])])
+def test_element_maplinear_matches_oracle(oracle, bitype, v):
+    """Host element maps of the library (quadrupole, constant focusing,
+    solenoid) against the oracle, itself pinned to the compiled reference."""
+    from impactz import beam as B
+    from impactz import hip
+    bm = B.Beam()
+    refpt = np.zeros(6)
+    refpt[5] = -bm.gamma0
+    xm, r1 = hip.element_maplinear(hip.physconst(bm.freq), 0.0125, bitype, v, 10, refpt,
+                                   bm.charge, bm.mass)
+    xo, r2 = oracle.element_map(oracle.physconst(bm.freq), 0.0125, bitype, v, 10, refpt,
+                                bm.charge, bm.mass)
+    assert np.allclose(xm, xo, rtol=1e-14, atol=1e-18)
+    assert np.array_equal(xm == 0.0, xo == 0.0)
+    assert r1[4] == pytest.approx(r2[4], rel=1e-15)
+
+
+def test_element_maplinear_refuses_other_elements():
+    from impactz import hip
+    c = hip.physconst(324e6)
+    r = np.zeros(6)
+    r[5] = -1.16
+    with pytest.raises(hip.IZHError, match="file"):
+        hip.element_maplinear(c, 0.01, 1, [1.0, 1.0, 0.0, 0.0], 10, r, 1.0, 938e6)
+    with pytest.raises(hip.IZHError, match="no linear map"):
+        hip.element_maplinear(c, 0.01, 4, [1.0, 0.0, 0.0, 0.0], 10, r, 1.0, 938e6)
+
+
 @pytest.mark.parametrize("name,step", [("fodo_16cube", 1), ("fodo_16cube", 5),
                                        ("fodo_aniso", 2), ("fodo_lossy", 2)])
 def test_update_compdom_matches_reference(name, step):

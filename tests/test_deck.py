@@ -84,6 +84,8 @@ def test_deck_written_for_the_reference_reads_back(tmp_path):
 @pytest.mark.parametrize("line, what", [
     ("0.1 4 10 101 2503370.0 3.24e8 222 852.0 0.0065 /", "type 101"),
     ("0.1 4 10 1 10.0 2.0 0.15 /", "field file ID"),
+    ("0.1 4 10 1 10.0 -20.0 0.15 /", "undulator"),
+    ("0.1 4 10 2 30.0 20.0 5.0 -1.0 /", "ideal RF"),
     ("0.0 0 10 -1 1.0 /", "type -1"),
     ("0.1 4 10 4 0.02 0.0 150 0.03 /", "type 4"),
 ])
@@ -130,3 +132,19 @@ def test_fort_writer_columns(tmp_path):
     assert np.array_equal(f18[:, 1:], rows[:, 1:6])
     f30 = np.loadtxt(tmp_path / "fort.30")
     assert np.array_equal(f30[:, 1:], rows[:, 36:42])
+
+
+def test_deck_linear_elements(tmp_path):
+    """Constant focusing, solenoid and K-mode quadrupole lines become
+    elements with their four values and the reference's pipe radius."""
+    from impactz.deck import lattice_of, read_deck
+    p = tmp_path / "ImpactZ.in"
+    _write(p, HEADER + "0.1 4 10 2 30.0 20.0 5.0 0.02 /\n"
+               "0.1 4 10 3 1.5 0.0 0.03 0.0 0.0 /\n"
+               "0.1 4 10 1 5.4 -1.0 0.04 /\n")
+    lat, notes = lattice_of(read_deck(str(p)))
+    assert [e.type for e in lat] == [2, 3, 1]
+    assert lat[0].values == [30.0, 20.0, 5.0, 0.02] and lat[0].radius == 0.02
+    assert lat[1].values[:3] == [1.5, 0.0, 0.03] and lat[1].radius == 0.03
+    assert lat[2].values[:3] == [5.4, -1.0, 0.04] and lat[2].radius == 0.04
+    assert notes == []

@@ -12,17 +12,18 @@ from test_gpu_parity import TOL, record
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_deck_run_matches_reference_fort_files(tmp_path, gpu, fused):
+@pytest.mark.parametrize("name,fused", [("fodo_traj", False), ("fodo_traj", True),
+                                        ("kquad", False), ("cf_sol", True)])
+def test_deck_run_matches_reference_fort_files(tmp_path, gpu, name, fused):
     from impactz.beam import write_deck, write_particle_in
     from impactz.deck import run_deck
-    d = load_golden("fodo_traj")
+    d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
     write_deck(str(tmp_path / "ImpactZ.in"), d["pts_in"].shape[0], grid,
                fixture_beam(d), fixture_lattice(d))
     write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
     nsteps = int(d["nsteps"])
-    print(f"deck run fodo_traj fused={fused}", flush=True)
+    print(f"deck run {name} fused={fused}", flush=True)
     run_deck(str(tmp_path), fused=fused, nsteps=nsteps)
     f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (18, 24, 25, 26)}
     assert f[24].shape == (nsteps + 1, 7)
@@ -32,5 +33,5 @@ def test_deck_run_matches_reference_fort_files(tmp_path, gpu, fused):
         ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
         worst = max(worst, diag_row_err(got, ref, ref_is_fort=True))
     assert np.max(np.abs(f[24][:, 0] - d["fort24"][: nsteps + 1, 0])) < 1e-12
-    record(test="deck_run", case="fodo_traj", fused=fused, steps=nsteps, moments=worst)
+    record(test="deck_run", case=name, fused=fused, steps=nsteps, moments=worst)
     assert worst < TOL

@@ -28,24 +28,25 @@ def _namelist(path, d, fused, out):
             f"  nelem={len(lat)}, fused={'.true.' if fused else '.false.'},",
             f"  particles='{path}/particle.in', output='{out}',"]
     for k, e in enumerate(lat, start=1):
-        grad = e.v1 if e.type == 1 else 0.0
+        vals = ", ".join(f"val({j},{k})={v!r}" for j, v in enumerate(e.values, start=1))
         rows.append(f"  elen({k})={e.length!r}, nseg({k})={e.nseg}, mapstp({k})={e.mapstp},"
-                    f" bitype({k})={e.type}, grad({k})={grad!r}, radius({k})={e.radius!r},")
+                    f" bitype({k})={e.type}, {vals}, radius({k})={e.radius!r},")
     with open(os.path.join(path, "run.nml"), "w") as f:
         f.write("&izh_run_nml\n" + "\n".join(rows) + "\n/\n")
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_fortran_host_matches_reference(tmp_path, fused):
+@pytest.mark.parametrize("name,fused", [("fodo_traj", False), ("fodo_traj", True),
+                                        ("cf_sol", False), ("cf_sol", True)])
+def test_fortran_host_matches_reference(tmp_path, name, fused):
     from impactz import hip
     from impactz.beam import write_particle_in
     from impactz.sim import diag_row
     assert os.access(EXE, os.X_OK), "Fortran host not built (impactz._build.build)"
-    d = load_golden("fodo_traj")
+    d = load_golden(name)
     write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
     out = str(tmp_path / "moments.txt")
     _namelist(str(tmp_path), d, fused, out)
-    print(f"fortran host fodo_traj fused={fused}", flush=True)
+    print(f"fortran host {name} fused={fused}", flush=True)
     subprocess.run([EXE, str(tmp_path / "run.nml")], cwd=str(tmp_path), check=True,
                    timeout=120)
     rec = np.loadtxt(out, ndmin=2)
@@ -62,5 +63,5 @@ def test_fortran_host_matches_reference(tmp_path, fused):
                               d["fort26"][k + 1, 1:]])
         worst = max(worst, diag_row_err(row, ref, ref_is_fort=True))
     assert abs(rec[-1, 1] - d["fort24"][nsteps, 0]) < 1e-12
-    record(test="fortran_host", case="fodo_traj", fused=fused, steps=nsteps, moments=worst)
+    record(test="fortran_host", case=name, fused=fused, steps=nsteps, moments=worst)
     assert worst < TOL

@@ -39,7 +39,8 @@ def test_fused_matches_stepwise_100k_32cube(gpu):
     assert b.bunch.nptlocal == a.bunch.nptlocal == 100_000
 
 
-@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_aniso", "fodo_lossy"])
+@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_aniso", "fodo_lossy",
+                                  "cf_sol"])
 def test_fused_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)
@@ -108,3 +109,19 @@ def test_fused_mixed_precision_error(gpu):
     fl2 = float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64))
     record(test="fused_mixed_vs_fp64", steps=32, rms=worst, field_l2=fl2)
     assert worst < 1e-4 and fl2 < 1e-3
+
+
+def test_fused_refuses_kmode_quadrupole(gpu):
+    """The chromatic K-mode map is not a fused push kind: izh_run says so and
+    leaves the bunch as it was; izh_step runs it."""
+    from impactz import beam as B
+    from impactz.sim import Simulation
+    bm = B.Beam()
+    pts = B.gaussian_bunch(4000, seed=2)
+    lat = [B.Element(0.1, 4, 10, 1, 5.4, -1.0, 1.0)]
+    sim = Simulation(pts, bm, lat, (16, 16, 16))
+    with pytest.raises(gpu.IZHError, match="K-mode"):
+        sim.run_fused(4)
+    assert col_err(sim.bunch.getpts(), pts) == 0.0
+    rows = sim.run(nsteps=4)
+    assert rows.shape[0] == 5

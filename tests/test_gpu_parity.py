@@ -31,7 +31,8 @@ def record(**kw):
 
 
 CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
-         ("fodo_lossy", 1), ("fodo_lossy", 2)]
+         ("fodo_lossy", 1), ("fodo_lossy", 2), ("cf_sol", 2), ("cf_sol", 10),
+         ("kquad", 3)]
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -44,7 +45,6 @@ def test_gpu_stages_match_reference(gpu, name, step):
     e = segment(d, step)
     s = f"s{step}_"
     tau1 = 0.5 * e.length / e.nseg
-    grad = e.v1 if e.type == 1 else 0.0
     b = hip.BeamBunch.from_beam(d[s + "p0"], beam, hip.FP64)
     info = b.info()
     for k in range(6):
@@ -54,7 +54,7 @@ def test_gpu_stages_match_reference(gpu, name, step):
     field = hip.FieldQuant(*grid)
     dom = hip.CompDom(*grid)
     err = {}
-    z = b.drift1(float(d[s + "z0"]), tau1, e.type, grad, e.mapstp)
+    z = b.map1(float(d[s + "z0"]), tau1, e.type, e.values, e.mapstp)
     err["map1a"] = col_err(b.getpts(), d[s + "p_map1a"])
     pr, nplc, nptot = b.conv1st(2 * tau1, e.radius, e.radius)
     assert nplc == nptot == d[s + "p_conv1st"].shape[0]
@@ -72,7 +72,7 @@ def test_gpu_stages_match_reference(gpu, name, step):
     err["cvbk"] = col_err(b.getpts(), d[s + "p_cvbk"])
     b.kick1(2 * tau1, field, dom)
     err["kick"] = col_err(b.getpts(), d[s + "p_kick"])
-    z = b.drift1(z, tau1, e.type, grad, e.mapstp)
+    z = b.map1(z, tau1, e.type, e.values, e.mapstp)
     err["map1b"] = col_err(b.getpts(), d[s + "p_map1b"])
     err["refpt"] = rel_err(b.refpt, d[s + "refpt"])
     assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
@@ -99,7 +99,8 @@ def test_gpu_solver_matches_direct_convolution(gpu, oracle):
         assert e < 1e-12
 
 
-@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy"])
+@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy", "cf_sol",
+                                  "kquad"])
 def test_gpu_trajectory_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)
@@ -207,12 +208,12 @@ def test_gpu_edge_cases(gpu):
     b.cvbkforth1st()
     b.kick1(0.02, f, dom)          # zero current: no kick, back to the z frame
     # an empty bunch can be stepped and gives zero moments
-    z, mom = b.step(f, dom, z, 0.05, 0, 0.0, 1, 1.0, 1.0, bm.current, True)
+    z, mom = b.step(f, dom, z, hip.segment(0.05, 0, 1.0, 1, 1.0, 1.0), bm.current, True)
     assert np.all(mom == 0)
     # a single particle has a degenerate domain: reported, not computed
     one = hip.BeamBunch.from_beam(pts[:1], bm)
     with pytest.raises(hip.IZHError, match="mesh size"):
-        one.step(f, dom, 0.0, 0.05, 0, 0.0, 1, 1.0, 1.0, bm.current)
+        one.step(f, dom, 0.0, hip.segment(0.05, 0, 1.0, 1, 1.0, 1.0), bm.current)
     # stage order is enforced
     c = hip.BeamBunch.from_beam(pts, bm)
     with pytest.raises(hip.IZHError, match="conv1st"):
@@ -221,3 +222,20 @@ def test_gpu_edge_cases(gpu):
         c.conv1st(0.02, 1.0, 1.0)
         c.drift1(0.0, 0.01, 0)
     del ctypes
+
+
+def test_gpu_unsupported_elements_are_refused(gpu):
+    """drift1_BeamBunch branches off this path fail with a message, before
+    touching the particles."""
+    from impactz import beam as B
+    hip = gpu
+    bm = B.Beam()
+    pts = B.gaussian_bunch(1000, seed=2)
+    b = hip.BeamBunch.from_beam(pts, bm)
+    for bt, v, what in ((1, [1.0, -20.0, 1.0], "undulator"),
+                        (1, [1.0, 2.0, 1.0], "file"),
+                        (2, [1.0, 1.0, 1.0, -1.0], "ideal RF"),
+                        (4, [0.0], "not on this path")):
+        with pytest.raises(hip.IZHError, match=what):
+            b.map1(0.0, 0.01, bt, v, 10)
+    assert col_err(b.getpts(), pts) == 0.0

@@ -16,7 +16,8 @@ import pytest
 from conftest import col_err, fixture_beam, fixture_lattice, load_golden, rel_err, segment
 
 CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
-         ("fodo_lossy", 1), ("fodo_lossy", 2)]
+         ("fodo_lossy", 1), ("fodo_lossy", 2), ("cf_sol", 2), ("cf_sol", 10),
+         ("kquad", 3)]
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -28,11 +29,10 @@ def test_oracle_stages_match_reference(oracle, name, step):
     s = f"s{step}_"
     c = oracle.physconst(beam.freq)
     tau1 = 0.5 * e.length / e.nseg
-    grad = e.v1 if e.type == 1 else 0.0
     # drift1 (first half)
     pts = d[s + "p0"].copy()
     refpt = d[s + "refpt0"].copy()
-    z = oracle.map1(c, pts, refpt, float(d[s + "z0"]), tau1, e.type, grad,
+    z = oracle.map1(c, pts, refpt, float(d[s + "z0"]), tau1, e.type, e.values,
                     e.mapstp, beam.charge, beam.mass)
     assert col_err(pts, d[s + "p_map1a"]) < 1e-14
     # conv1st
@@ -65,14 +65,15 @@ def test_oracle_stages_match_reference(oracle, name, step):
     assert col_err(pts, d[s + "p_kick"]) < 1e-13
     # drift1 (second half)
     pts = d[s + "p_kick"].copy()
-    z = oracle.map1(c, pts, refpt, z, tau1, e.type, grad, e.mapstp,
+    z = oracle.map1(c, pts, refpt, z, tau1, e.type, e.values, e.mapstp,
                     beam.charge, beam.mass)
     assert col_err(pts, d[s + "p_map1b"]) < 1e-14
     assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
     assert rel_err(refpt, d[s + "refpt"]) < 1e-15
 
 
-@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy"])
+@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy", "cf_sol",
+                                  "kquad"])
 def test_oracle_trajectory_matches_reference(oracle, name):
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
@@ -170,6 +171,90 @@ def test_quad_map_closed_form(oracle):
         m = xm[2 * blk:2 * blk + 2, 2 * blk:2 * blk + 2]
         assert np.linalg.det(m) == pytest.approx(1.0, abs=1e-13)
     assert r2[4] == pytest.approx(L / (sx * b.beta0), rel=1e-13)
+
+
+def _blk(k, L, gb, sx, p=1):
+    """Transport of (q/Scxl, (gamma beta)^p q') through a constant spring k^2."""
+    if k > 0:
+        return np.array([[math.cos(k * L), math.sin(k * L) / (k * gb ** p * sx)],
+                         [-k * gb ** p * sx * math.sin(k * L), math.cos(k * L)]])
+    return np.array([[1.0, L / (gb ** p * sx)], [0.0, 1.0]])
+
+
+def test_constfoc_map_closed_form(oracle):
+    """Constant focusing (src/Appl/ConstFoc.f90): three harmonic blocks."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    tau, k2 = 0.05, (30.0, 20.0, 5.0)
+    refpt = np.zeros(6)
+    refpt[5] = -b.gamma0
+    xm, r2 = oracle.element_map(c, tau, 2, [*k2, 1.0], 10, refpt, b.charge, b.mass)
+    gb, sx = b.gambet0, c.scxl
+    assert np.allclose(xm[0:2, 0:2], _blk(math.sqrt(k2[0]), tau, gb, sx), rtol=1e-11, atol=0)
+    assert np.allclose(xm[2:4, 2:4], _blk(math.sqrt(k2[1]), tau, gb, sx), rtol=1e-11, atol=0)
+    # the longitudinal block scales with (gamma beta)^3
+    assert np.allclose(xm[4:6, 4:6], _blk(math.sqrt(k2[2]), tau, gb, sx, 3), rtol=1e-11,
+                       atol=0)
+    assert np.count_nonzero(xm[0:2, 2:6]) == 0 and np.count_nonzero(xm[2:4, 4:6]) == 0
+    assert r2[4] == pytest.approx(tau / (sx * b.beta0), rel=1e-13)
+
+
+def test_solenoid_map_closed_form(oracle):
+    """The solenoid map of src/Appl/Sol.f90: both transverse planes focused
+    by (B/2Brho)^2, times the rotation by -B/(2 Brho) * tau."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    tau, b0 = 0.05, 1.5
+    refpt = np.zeros(6)
+    refpt[5] = -b.gamma0
+    xm, _ = oracle.element_map(c, tau, 3, [b0, 0.0, 1.0, 0.0], 10, refpt, b.charge,
+                               b.mass)
+    gb, sx = b.gambet0, c.scxl
+    brho = gb * b.mass / (299792458.0 * b.charge)
+    f = _blk(0.5 * b0 / brho, tau, gb, sx)
+    a = -b0 / brho / 2 * tau
+    rot = np.array([[math.cos(a), -math.sin(a)], [math.sin(a), math.cos(a)]])
+    assert np.allclose(xm[0:4, 0:4], np.kron(rot, f), rtol=1e-11, atol=1e-15)
+    assert np.allclose(xm[4:6, 4:6], _blk(0.0, tau, gb, sx, 3), rtol=1e-12, atol=0)
+
+
+def test_kmode_quad_equals_gradient_map_on_energy(oracle):
+    """transfmapK_Quadrupole and the RK6 map of maplinear_Quadrupole are two
+    routines of the reference; for on-energy particles (pt = 0) with K = G/Brho
+    they must give the same transverse transport."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    rng = np.random.default_rng(4)
+    pts = np.zeros((200, 9))
+    pts[:, 0] = rng.normal(0, 1e-3, 200) / c.scxl
+    pts[:, 2] = rng.normal(0, 1e-3, 200) / c.scxl
+    pts[:, 6] = b.charge / b.mass
+    brho = b.gambet0 * b.mass / (299792458.0 * b.charge)
+    for kq in (5.4, -5.4):
+        p1, p2 = pts.copy(), pts.copy()
+        r1 = np.zeros(6); r1[5] = -b.gamma0
+        r2 = r1.copy()
+        oracle.map1(c, p1, r1, 0.0, 0.05, 1, [kq, -1.0, 1.0, 0.0], 10, b.charge, b.mass)
+        oracle.map1(c, p2, r2, 0.0, 0.05, 1, [kq * brho, 0.0, 1.0, 0.0], 10, b.charge,
+                    b.mass)
+        assert col_err(p1[:, :4], p2[:, :4]) < 1e-11
+        assert np.all(np.abs(p1[:, 4]) < 1e-15)  # 1/beta_z - 1/beta0 rounds to 1e-16
+        assert r1[4] == pytest.approx(r2[4], rel=1e-13)
+
+
+def test_unsupported_branches_are_refused(oracle):
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    p = np.zeros((1, 9))
+    r = np.zeros(6); r[5] = -b.gamma0
+    for bt, v in ((1, [1.0, -20.0, 1.0, 0.0]), (1, [1.0, 1.0, 1.0, 0.0]),
+                  (2, [1.0, 1.0, 1.0, -1.0]), (4, [0.0] * 4)):
+        with pytest.raises(ValueError):
+            oracle.map1(c, p, r, 0.0, 0.05, bt, v, 10, b.charge, b.mass)
 
 
 def test_drift_time_reversal(oracle):
