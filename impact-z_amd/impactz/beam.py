@@ -188,21 +188,28 @@ def write_particle_in(path: str, pts: np.ndarray) -> None:
 
 
 def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
-               lattice: list[Element], flagdiag: int = 1) -> None:
-    """An ``ImpactZ.in`` for Flagmap=1, Flagbc=1, Flagdist=19 (src/Contrl/Input.f90:62-314)."""
+               lattice: list[Element], flagdiag: int = 1, rstartflg: int = 0,
+               flagdist: int = 19, distparam=None) -> None:
+    """An ``ImpactZ.in`` for Flagmap=1, Flagbc=1 (src/Contrl/Input.f90:62-314):
+    particles from particle.in (Flagdist=19) or a binary fort.N (199, the file
+    number in distparam[1]); rstartflg=1 restarts from a -7 checkpoint."""
     nx, ny, nz = grid
+    dp = list(distparam) if distparam is not None else [
+        0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
+        0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
+        0.1, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0]
     qm = beam.charge / beam.mass
     lines = [
         "1 1",
         f"6 {n} 1 0 {flagdiag}",
         f"{nx} {ny} {nz} 1 1.0 1.0 1.0",
-        "19 0 0 1",
+        f"{flagdist} {rstartflg} 0 1",
         f"{n}",
         f"{beam.current!r}",
         f"{qm!r}",
-        "0.001 0.001 0.0 1.0 1.0 0.0 0.0",
-        "0.001 0.001 0.0 1.0 1.0 0.0 0.0",
-        "0.1 0.001 0.0 1.0 1.0 0.0 0.0",
+        " ".join(repr(float(v)) for v in dp[0:7]),
+        " ".join(repr(float(v)) for v in dp[7:14]),
+        " ".join(repr(float(v)) for v in dp[14:21]),
         f"{beam.current!r} {beam.kinetic!r} {beam.mass!r} {beam.charge!r} "
         f"{beam.freq!r} {beam.phase0!r}",
     ]

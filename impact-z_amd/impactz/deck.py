@@ -28,6 +28,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import hip
+from . import particle_io as PIO
 from .beam import Beam, Element
 from .sim import Simulation
 
@@ -154,10 +155,23 @@ def read_deck(path: str) -> Deck:
                 bcharge, bfreq, phsini, elements)
 
 
-def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
-    """The element table of the deck as Elements of this build, and notes on
-    the diagnostic elements skipped.  Raises DeckError for anything the GPU
-    step does not implement."""
+@dataclass
+class FileAction:
+    """A zero-length element that writes a particle file at its place in the
+    lattice (src/Contrl/AccSimulator.f90:883-911): -2 phase dump
+    (phase_Output, sample period = v1) or -7 checkpoint (outpoint_Output);
+    the file is fort.<mapstp>."""
+    index: int          # 1-based element index in the deck
+    type: int
+    unit: int
+    value: float = 0.0
+
+
+def program_of(deck: Deck) -> tuple[list, list[str]]:
+    """The deck's element table as the sequence this build runs: (deck index,
+    Element or FileAction) in deck order, and notes on the diagnostic
+    elements skipped.  Raises DeckError for anything the GPU path does not
+    implement."""
     if deck.flagmap != 1:
         raise DeckError(f"Flagmap={deck.flagmap}: only the linear map integrator "
                         "(Flagmap=1) is on this path")
@@ -171,7 +185,7 @@ def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
     for k, e in enumerate(deck.elements, start=1):
         v = e.values + [0.0] * 4
         if e.type == 0:
-            out.append(Element(e.length, e.nseg, e.mapstp, 0, v[0]))
+            out.append((k, Element(e.length, e.nseg, e.mapstp, 0, v[0])))
         elif e.type == 1:
             if v[1] > 1.0e-5:
                 raise DeckError(f"element {k}: quadrupole with field file ID {v[1]:g} "
@@ -179,20 +193,37 @@ def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
             if v[1] < -10.0:
                 raise DeckError(f"element {k}: undulator map (file ID {v[1]:g}) is not "
                                 "on this path")
-            out.append(Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2], v[3]))
+            out.append((k, Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2], v[3])))
         elif e.type in (2, 3):
             # drift1_BeamBunch takes its ideal-RF branch when Param(5) < 0
             # (src/Appl/BeamBunch.f90:324-330)
             if v[3] < 0.0:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {v[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")
-            out.append(Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3]))
-        elif e.type in DIAGNOSTIC_TYPES and e.length == 0.0:
-            notes.append(f"element {k}: type {e.type} ({DIAGNOSTIC_TYPES[e.type]}) "
-                         "skipped: it writes files only")
+            out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3])))
+        elif e.type in DIAGNOSTIC_TYPES and e.length == 0.0 and e.nseg == 0:
+            if e.type in (-2, -7):
+                out.append((k, FileAction(k, e.type, e.mapstp, v[0])))
+            else:
+                notes.append(f"element {k}: type {e.type} ({DIAGNOSTIC_TYPES[e.type]}) "
+                             "skipped: it writes files only")
         else:
             raise DeckError(f"element {k}: type {e.type} is not on this path "
                             "(drift 0, quadrupole 1, constant focusing 2 and solenoid 3 are)")
+    return out, notes
+
+
+def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
+    """The element table of the deck as Elements of this build, and notes on
+    the file-writing elements left out of it."""
+    prog, notes = program_of(deck)
+    out = []
+    for _, p in prog:
+        if isinstance(p, FileAction):
+            notes.append(f"element {p.index}: type {p.type} "
+                         f"({DIAGNOSTIC_TYPES[p.type]}) writes fort.{p.unit}")
+        else:
+            out.append(p)
     return out, notes
 
 
@@ -226,40 +257,114 @@ def write_fort(outdir: str, rows: np.ndarray) -> None:
                 f.write(" ".join(repr(float(x)) for x in vals) + "\n")
 
 
+def initial_particles(deck: Deck, workdir: str) -> np.ndarray:
+    """The initial particles: particle.in (Flagdist=19, readin_Dist) or the
+    binary fort.N of readPar_Dist (Flagdist=199, N = distparam(2))."""
+    if deck.flagdist == 19:
+        return read_particle_in(os.path.join(workdir, "particle.in"))
+    if deck.flagdist == 199:
+        unit = int(deck.distparam[1] + 0.001)
+        return PIO.read_binary_particles(os.path.join(workdir, f"fort.{unit}"),
+                                         deck.distparam)
+    raise DeckError(f"Flagdist={deck.flagdist}: inputs come from particle.in "
+                    "(Flagdist=19) or a binary fort.N (199) on this path")
+
+
 def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
              outdir: str | None = None, nsteps: int | None = None) -> np.ndarray:
-    """Run ``workdir/ImpactZ.in`` with ``workdir/particle.in`` and write the
-    diagnostic files into ``outdir`` (default: workdir).  fused=True runs the
-    lattice through izh_run.  Returns the diagnostic rows."""
+    """Run ``workdir/ImpactZ.in`` and write the diagnostic files into
+    ``outdir`` (default: workdir).  fused=True runs each stretch of
+    space-charge segments between file-writing elements through izh_run.
+    Rstartflg=1 restarts from the -7 checkpoint fort.<mapstp of the last -7
+    element> at the element after the one that wrote it, as
+    run_AccSimulator does (src/Contrl/AccSimulator.f90:558-561,674-742).
+    Returns the diagnostic rows."""
     deck = read_deck(os.path.join(workdir, "ImpactZ.in"))
-    if deck.flagdist != 19:
-        raise DeckError(f"Flagdist={deck.flagdist}: inputs come from particle.in "
-                        "(Flagdist=19) on this path")
-    lattice, notes = lattice_of(deck)
+    prog, notes = program_of(deck)
     for n in notes:
         print(f"[impactz.deck] {n}", flush=True)
-    pts = read_particle_in(os.path.join(workdir, "particle.in"))
-    sim = Simulation(pts, deck.beam(), lattice, deck.grid, precision=precision)
+    out = outdir or workdir
+    # counters of run_AccSimulator that the checkpoint records
+    st = {"z": 0.0, "i": 0, "j": 0, "ibal": 0, "nstep": 0}
+    ck = None
+    if deck.rstartflg == 1:
+        units = [p.unit for _, p in prog if isinstance(p, FileAction) and p.type == -7]
+        if not units:
+            raise DeckError("Rstartflg=1 needs a -7 element naming the checkpoint file")
+        ck = PIO.read_checkpoint(os.path.join(workdir, f"fort.{units[-1]}"))
+        pts = ck.pts
+        st.update(z=ck.z, i=ck.inb, j=ck.jstp, ibal=ck.ibal, nstep=ck.nstep)
+    else:
+        pts = initial_particles(deck, workdir)
+    elements = [p for _, p in prog if isinstance(p, Element)]
+    sim = Simulation(pts, deck.beam(), elements, deck.grid, precision=precision)
     try:
-        if fused:
-            total = sum(e.nseg for e in lattice)
-            total = total if nsteps is None else min(total, nsteps)
-            rows = [np.concatenate([[sim.z], sim.diag()])]
-            rec = sim.run_fused(total)
-            # z after each step as drift1_BeamBunch advances it: tau1 twice
-            z = 0.0
-            segs = list(sim.segments())[:total]
-            for k in range(total):
-                tau1 = 0.5 * segs[k].length / segs[k].nseg
-                z = (z + tau1) + tau1
-                rows.append(np.concatenate([[z], sim.diag(rec[k])]))
-            rows = np.array(rows)
-        else:
-            rows = sim.run(nsteps)
+        if ck is not None:
+            # inpoint_Output: the particles, refptcl and z of the checkpoint;
+            # Current stays the deck's and Npt is the particle count
+            sim.bunch.set_refpt(ck.refpt)
+            sim.z = ck.z
+        rows = [np.concatenate([[sim.z], sim.diag()])]
+        sc = deck.bcurr >= float(np.float32(1.0e-10))
+        budget = nsteps
+        pending = []   # (element, tau1) of segments not yet run
+
+        def flush():
+            if not pending:
+                return
+            if fused:
+                z = sim.z
+                recs = sim.run_elements([e for e, _ in pending])
+                for (e, tau1), rec in zip(pending, recs):
+                    z = (z + tau1) + tau1  # drift1_BeamBunch advances z by tau1 twice
+                    rows.append(np.concatenate([[z], sim.diag(rec)]))
+            else:
+                for e, _ in pending:
+                    mom = sim.step(e)
+                    rows.append(np.concatenate([[sim.z], sim.diag(mom)]))
+            pending.clear()
+
+        for index, p in prog:
+            if index <= st["i"]:
+                continue      # a restart resumes after the checkpoint's element
+            if isinstance(p, FileAction):
+                flush()
+                _file_action(p, sim, out, st, deck)
+                st["i"], st["j"] = index, 1   # its empty segment loop leaves j = 1
+                continue
+            for _ in range(p.nseg):
+                if budget is not None and budget <= 0:
+                    break
+                pending.append((p, 0.5 * p.length / p.nseg))
+                st["nstep"] += 1
+                st["ibal"] += 1 if sc else 0
+                if budget is not None:
+                    budget -= 1
+            st["i"], st["j"] = index, (p.nseg + 1 if p.nseg >= 1 else 1)
+        flush()
+        rows = np.array(rows)
     finally:
         sim.close()
-    write_fort(outdir or workdir, rows)
+    write_fort(out, rows)
     return rows
+
+
+def _file_action(p: FileAction, sim: Simulation, outdir: str, st: dict, deck: Deck):
+    pts = sim.bunch.getpts()
+    refpt = sim.bunch.refpt
+    if p.type == -2:
+        # integerSamplePeriod = realSamplePeriod (conversion truncates)
+        PIO.write_phase(os.path.join(outdir, f"fort.{p.unit}"), pts, int(p.value),
+                        sim.const.scxl, float(refpt[5]))
+    else:
+        rng, ms = sim.dom.range, sim.dom.msize
+        nx, ny, nz = deck.grid
+        ck = PIO.Checkpoint(z=sim.z, inb=p.index, jstp=st["j"], ibal=st["ibal"],
+                            nstep=st["nstep"], msize=ms, range=rng,
+                            lctabnm=np.array([nz, ny]),
+                            lctabrg=np.array([rng[4], rng[5], rng[2], rng[3]]),
+                            refpt=refpt, pts=pts)
+        PIO.write_checkpoint(os.path.join(outdir, f"fort.{p.unit}"), ck)
 
 
 def main(argv=None) -> int:

@@ -323,6 +323,13 @@ class BeamBunch:
         check(lib().izh_bunch_download(self.h, _p(out), out.shape[0], C.byref(nn)))
         return out[: nn.value]
 
+    def set_refpt(self, refpt) -> None:
+        """refptcl(6) of the bunch (izh_bunch_set_info)."""
+        i = self.info()
+        for k in range(6):
+            i.refpt[k] = float(refpt[k])
+        check(lib().izh_bunch_set_info(self.h, C.byref(i)))
+
     def setpts(self, pts: np.ndarray) -> None:
         pts = np.ascontiguousarray(pts, dtype=np.float64)
         check(lib().izh_bunch_upload(self.h, _p(pts), pts.shape[0]))

@@ -115,6 +115,16 @@ class Simulation:
         self.nstep += len(segs)
         return mom
 
+    def run_elements(self, elements, want_moments: bool = True):
+        """One space-charge step per entry of ``elements`` through izh_run;
+        returns the per-step records (45 values: moments, then Npt)."""
+        elements = list(elements)
+        self.z, mom = self.bunch.run(self.field, self.dom, self.z,
+                                     [self.segment_args(e) for e in elements],
+                                     self.beam.current, want_moments)
+        self.nstep += len(elements)
+        return mom
+
     def diag(self, mom=None, refpt=None):
         """diagnostic1_Output row of the current state, or of a moments record
         (44 values, or 45 with Npt as izh_run gives them)."""

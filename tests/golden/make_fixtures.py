@@ -13,7 +13,10 @@ from __future__ import annotations
 
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -80,6 +83,73 @@ def kquad_lattice():
             B.Element(0.4, 4, 10, 0, 1.0)]
 
 
+def io_lattice():
+    """FODO with a phase-space dump (-2, file 60) and a checkpoint (-7, file
+    50) between the first drift and the defocusing quadrupole."""
+    return [B.Element(0.1, 4, 10, 1, 10.0, 0.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0),
+            B.Element(0.0, 0, 60, -2, 1.0),
+            B.Element(0.0, 0, 50, -7, 0.0),
+            B.Element(0.1, 4, 10, 1, -10.0, 0.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0)]
+
+
+def _run_exe(d):
+    subprocess.run(f"ulimit -s unlimited; exec {R.EXE}", shell=True, cwd=d, check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def io_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
+    """Particle I/O of the reference executable itself (ImpactZexe built from
+    source): a run that writes a -2 phase dump and a -7 checkpoint, the run
+    restarted from that checkpoint (Rstartflg=1), and a run whose particles
+    come from a binary fort.40 with the coordinate offsets of readPar_Dist
+    (Flagdist=199)."""
+    from impactz import particle_io as P
+    pts = B.gaussian_bunch(n, seed=7, beam=beam)
+    lat = io_lattice()
+    tmp = tempfile.mkdtemp(prefix="impz_io_")
+    out = {"pts_in": pts, "grid": np.array(grid),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0]),
+           "lattice": np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2, e.v3,
+                                 e.v4] for e in lat])}
+    a = os.path.join(tmp, "a")
+    os.makedirs(a)
+    B.write_particle_in(os.path.join(a, "particle.in"), pts)
+    B.write_deck(os.path.join(a, "ImpactZ.in"), n, grid, beam, lat)
+    _run_exe(a)
+    with open(os.path.join(a, "fort.50"), "rb") as f:
+        out["ckpt50"] = np.frombuffer(f.read(), dtype=np.uint8)
+    out["phase60"] = np.loadtxt(os.path.join(a, "fort.60"), ndmin=2)
+    for k in (18, 24, 25, 26):
+        out[f"a_fort{k}"] = R.read_fort(os.path.join(a, f"fort.{k}"))
+    b = os.path.join(tmp, "b")
+    os.makedirs(b)
+    shutil.copy(os.path.join(a, "fort.50"), os.path.join(b, "fort.50"))
+    B.write_deck(os.path.join(b, "ImpactZ.in"), n, grid, beam, lat, rstartflg=1)
+    _run_exe(b)
+    for k in (18, 24, 25, 26):
+        out[f"b_fort{k}"] = R.read_fort(os.path.join(b, f"fort.{k}"))
+    c = os.path.join(tmp, "c")
+    os.makedirs(c)
+    dp = [0.001, 40.0, 0.0, 1.0, 1.0, 2.0e-3, -1.0e-4,
+          0.001, 0.001, 0.0, 1.0, 1.0, -3.0e-3, 2.0e-4,
+          0.1, 0.001, 0.0, 1.0, 1.0, 0.05, 1.0e-4]
+    P.write_binary_particles(os.path.join(c, "fort.40"), pts)
+    B.write_deck(os.path.join(c, "ImpactZ.in"), n, grid, beam, lat, flagdist=199,
+                 distparam=dp)
+    _run_exe(c)
+    out["c_distparam"] = np.array(dp)
+    for k in (18, 24, 25, 26):
+        out[f"c_fort{k}"] = R.read_fort(os.path.join(c, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "io_ckpt.npz")
+    np.savez_compressed(path, **out)
+    print(f"io_ckpt: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
 def main():
     if not R.harness_available():
         raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
@@ -103,6 +173,9 @@ def main():
                       "dumps at steps 2 (constant focusing) and 10 (solenoid)")
     fixture("kquad", 1000, (16, 16, 16), kquad_lattice(), 16, (3,))
     meta["kquad"] = "FODO of K-mode quadrupoles (file ID -1, K = +-5.4 1/m^2), dump at step 3"
+    io_fixture()
+    meta["io_ckpt"] = ("ImpactZexe runs: -2 phase dump and -7 checkpoint after 8 steps, "
+                       "the run restarted from that checkpoint, and Flagdist=199 binary input")
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(meta, f, indent=1)
 
