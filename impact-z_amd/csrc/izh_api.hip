@@ -815,6 +815,99 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
   return IZH_OK;
 }
 
+// ------------------------------------------------ Lorentz integrator
+
+int izh_drift2(izh_bunch *b, double *z, double tau) {
+  if (!b || !z) return fail(IZH_ERR_ARG, "izh_drift2: bad argument");
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_drift2: bunch is not in the z frame");
+  // src/Appl/BeamBunch.f90:481-504
+  const double gamma0 = -b->refpt[5];
+  const double beta0 = std::sqrt(1.0 - 1.0 / gamma0 / gamma0);
+  IZH_CHECK(launch_drift2(b, tau, gamma0, beta0));
+  b->refpt[4] = b->refpt[4] + 0.5 * tau / (b->c.scxl * beta0);
+  *z = *z + 0.5 * tau;
+  return IZH_OK;
+}
+
+int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
+              const izh_compdom *dom, int flagbc) {
+  if (!b || !e) return fail(IZH_ERR_ARG, "izh_kick2: bad argument");
+  if (flagbc != 1) return fail(IZH_ERR_ARG, "izh_kick2: only Flagbc=1 is supported");
+  if (const char *why = elem_field_unsupported(e->bitype, e->v))
+    return fail(IZH_ERR_ARG, "izh_kick2: bitype %d: %s", e->bitype, why);
+  // scatter2 after cvbkforth1st (0th-order t frame, current > 0); scatter20
+  // on a z-frame bunch (src/Appl/BeamBunch.f90:2231,2487-2499)
+  const bool sc = b->frame == kFrameT0 && b->current > 0.0;
+  if (b->frame != kFrameZ && b->frame != kFrameT0)
+    return fail(IZH_ERR_STATE, "izh_kick2: bunch is neither in the z frame nor after "
+                "izh_cvbkforth1st");
+  if (sc) {
+    if (!f) return fail(IZH_ERR_ARG, "izh_kick2: space charge needs the field");
+    IZH_CHECK(check_dom(f, dom, "izh_kick2"));
+    if (!f->have_phi) return fail(IZH_ERR_STATE, "izh_kick2: no potential");
+    if (b->precision != f->precision)
+      return fail(IZH_ERR_ARG, "izh_kick2: bunch and field precision differ");
+  }
+  Kick2 k{};
+  k.e.type = e->bitype;
+  for (int a = 0; a < 4; a++) k.e.v[a] = e->v[a];
+  k.e.length = e->length;
+  k.e.zedge = e->zedge;
+  k.gam = -b->refpt[5];
+  k.clight = b->c.clight;
+  k.xl = b->c.scxl;
+  k.xk = 1.0 / b->c.scxl;
+  k.beta0_c = std::sqrt(k.gam * k.gam - 1.0) / k.gam / b->c.clight;
+  k.tg = b->refpt[4];
+  k.z = z;
+  k.tau = tau;
+  k.qmcc = b->charge / b->mass;
+  const double pos0[4] = {0.0, 0.0, z, k.tg};
+  double f0[6];
+  elem_field(k.e, k.xl, k.clight, pos0, f0);
+  k.ez0 = f0[2];
+  k.tmpgamma0 = k.gam + 0.5 * tau * k.qmcc * k.ez0;
+  if (sc) {
+    IZH_CHECK(field_set_domain(f, dom));
+    IZH_CHECK(launch_efield(f));
+    IZH_CHECK(join_streams(f->stream, b->stream));
+    // Clight^2 * 1e-7 (REAL(4)) is 1/(4 pi eps0); the grid holds charge * rho_scale
+    k.tmpscale = b->c.clight * b->c.clight * (double)1.0e-7f / f->rho_scale;
+    // -beta0/Clight*tmpscale*eyn: the scaled factor enters through tmpscale
+    IZH_CHECK(launch_kick2(b, f, k, true));
+    IZH_CHECK(join_streams(b->stream, f->stream));
+  } else {
+    IZH_CHECK(launch_kick2(b, f, k, false));
+  }
+  b->refpt[5] = -(k.gam + tau * k.qmcc * k.ez0);
+  b->frame = kFrameZ;
+  return IZH_OK;
+}
+
+int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double tau1,
+              const izh_element *e, double xrad, double yrad, double bcurr,
+              double *moments) {
+  if (!b || !f || !dom || !z || !e) return fail(IZH_ERR_ARG, "izh_step2: bad argument");
+  if (const char *why = elem_field_unsupported(e->bitype, e->v))
+    return fail(IZH_ERR_ARG, "izh_step2: bitype %d: %s", e->bitype, why);
+  const double tau2 = 2.0 * tau1;
+  IZH_CHECK(izh_drift2(b, z, tau2));                                        // :1112
+  if (bcurr >= (double)1.0e-10f && b->n > 0) {
+    double ptrange[6];
+    int64_t nplc, nptot;
+    IZH_CHECK(izh_conv1st(b, tau2, ptrange, &nplc, &nptot, xrad, yrad, 1));
+    IZH_CHECK(izh_update_compdom(dom, ptrange, 1));
+    IZH_CHECK(izh_charge(b, dom, f));
+    IZH_CHECK(izh_update3O(f, dom));
+    IZH_CHECK(izh_cvbkforth1st(b));
+  }
+  IZH_CHECK(izh_kick2(b, *z, tau2, e, f, dom, 1));                          // :1452
+  IZH_CHECK(izh_drift2(b, z, tau2));                                        // :1456
+  if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
+  return IZH_OK;
+}
+
 int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
                     double *r2max) {
   if (!b) return fail(IZH_ERR_ARG, "izh_diagnostic1: bad argument");

@@ -219,6 +219,102 @@ inline int map1_kind(int bitype, const double v[4], const char **why) {
   return -1;
 }
 
+// ---------------------------------------- nonlinear integrator (Flagmap=2)
+
+// An element's analytic field for the Lorentz integrator: getfld_DriftTube,
+// getfld_Quadrupole (file ID <= 0), getfld_ConstFoc and getfld_Sol
+// (src/Appl/DriftTube.f90:290-305, Quadrupole.f90:369-402, ConstFoc.f90:
+// 297-322, Sol.f90:388-435).  pos = (x, y in m, z, t); f = E (V/m), B (T).
+struct ElemField {
+  int type;
+  double v[4];
+  double length, zedge;
+};
+
+__host__ __device__ inline void elem_field(const ElemField &e, double scxl, double clight,
+                                           const double pos[4], double f[6]) {
+  for (int k = 0; k < 6; k++) f[k] = 0.0;
+  if (e.type == 1) {
+    f[3] = e.v[0] * pos[1];
+    f[4] = e.v[0] * pos[0];
+  } else if (e.type == 2) {
+    // the reference's placeholders: qmcc = gambetz = 1
+    const double qmcc = 1.0, pz = 1.0;
+    f[2] = e.v[2] * pos[3] * pz * pz * pz * scxl / qmcc;
+    f[3] = -e.v[0] * pos[1] * pz / qmcc / clight;
+    f[4] = e.v[1] * pos[0] * pz / qmcc / clight;
+  } else if (e.type == 3) {
+    // uniform Bz with linear ramps over 2*Param(4) at both ends
+    const double x02 = 2 * e.v[2], zz = pos[2] - e.zedge, b0 = e.v[0];
+    if (zz >= 0.0 && zz < x02) {
+      const double bb = b0 / x02;
+      f[3] = -bb / 2 * pos[0];
+      f[4] = -bb / 2 * pos[1];
+      f[5] = bb * zz;
+    } else if (zz >= x02 && zz <= e.length - x02) {
+      f[5] = b0;
+    } else if (zz > e.length - x02 && zz <= e.length) {
+      const double bb = -b0 / x02, zshift = e.length - x02;
+      f[3] = -bb / 2 * pos[0];
+      f[4] = -bb / 2 * pos[1];
+      f[5] = b0 + bb * (zz - zshift);
+    }
+  }
+}
+
+// The element types with an analytic field on this path, or a message.
+inline const char *elem_field_unsupported(int bitype, const double v[4]) {
+  if (bitype == 0 || bitype == 2 || bitype == 3) return nullptr;
+  if (bitype == 1)
+    return v[1] > 0.0 ? "quadrupole field profile from a file (file ID > 0)" : nullptr;
+  return "element type not on this path (0 drift, 1 quadrupole, 2 constant focusing, "
+         "3 solenoid)";
+}
+
+// Implicit Lorentz push of scatter2/scatter20 (src/Appl/BeamBunch.f90:
+// 3026-3066): the momenta solve a 3x3 system with the longitudinal momentum
+// averaged over the step, in two iterations.
+__device__ inline void lorentz_push(double &px, double &py, double &pt, double q, double ex,
+                                    double ey, double ez, double bx, double by, double bz,
+                                    double tau, double clight, double tmpgamma0,
+                                    double gam, double qmcc, double ez0) {
+  const double tmp12 = -tau * clight * bz * q / 2;
+  const double tmp13 = tau * ex * q / 2;
+  const double tmp23 = tau * ey * q / 2;
+  const double tmp1 = tmpgamma0 * ex * tau * q;
+  const double tmp2 = clight * by * tau * q;
+  const double tmp3 = tmpgamma0 * ey * tau * q;
+  const double tmp4 = clight * bx * tau * q;
+  const double tmp5 = (ez0 * qmcc - ez * q) * tau;
+  double pz = sqrt((gam - pt) * (gam - pt) - 1.0 - px * px - py * py);
+  const double p1 = px, p2 = py, p3 = pt;
+  const double tmpgamma00 = gam + tau * qmcc * ez0;
+  for (int ii = 0; ii < 2; ii++) {
+    const double a12 = tmp12 / pz, a13 = tmp13 / pz, a23 = tmp23 / pz;
+    const double b1 = p1 - a12 * p2 - a13 * p3 + tmp1 / pz - tmp2;
+    const double b2 = a12 * p1 + p2 - a23 * p3 + tmp3 / pz + tmp4;
+    const double b3 = -a13 * p1 - a23 * p2 + p3 + tmp5;
+    const double det = 1 + a12 * a12 - a13 * a13 - a23 * a23;
+    const double s11 = 1 - a23 * a23, s12 = -a12 + a13 * a23, s13 = a12 * a23 - a13;
+    const double s21 = a12 + a13 * a23, s22 = 1 - a13 * a13, s23 = -a23 - a13 * a12;
+    const double s31 = -a12 * a23 - a13, s32 = -a23 + a12 * a13, s33 = 1 + a12 * a12;
+    px = (s11 * b1 + s12 * b2 + s13 * b3) / det;
+    py = (s21 * b1 + s22 * b2 + s23 * b3) / det;
+    pt = (s31 * b1 + s32 * b2 + s33 * b3) / det;
+    const double tmpsq2 = (tmpgamma00 - pt) * (tmpgamma00 - pt) - 1.0 - px * px - py * py;
+    if (tmpsq2 > 0.0) pz = 0.5 * pz + 0.5 * sqrt(tmpsq2);
+  }
+}
+
+// Constants of one kick2 call, set on the host.
+struct Kick2 {
+  ElemField e;
+  double gam, beta0_c, xk, xl, clight, tg, z, tau, tmpscale, tmpgamma0, qmcc, ez0;
+};
+
+int launch_drift2(izh_bunch *b, double tau, double gamma0, double beta0);
+int launch_kick2(izh_bunch *b, izh_field *f, const Kick2 &k, bool space_charge);
+
 int launch_map_linear(izh_bunch *b, const double xm[36]);
 int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
                    int64_t *nlost);

@@ -307,6 +307,78 @@ k_gather_kick(double *__restrict__ x, double *__restrict__ px,
   }
 }
 
+// ------------------------------------------------- Lorentz integrator
+// drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2.
+__global__ void __launch_bounds__(kBlock)
+k_drift2(double *__restrict__ X, const double *__restrict__ PX, double *__restrict__ Y,
+         const double *__restrict__ PY, double *__restrict__ PH, const double *__restrict__ PT,
+         int64_t n, double tau, double gamma0, double beta0, double xl) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double px = PX[i], py = PY[i], g = gamma0 - PT[i];
+    const double pz = sqrt(g * g - 1.0 - px * px - py * py);
+    X[i] = X[i] + 0.5 * tau * px / (xl * pz);
+    Y[i] = Y[i] + 0.5 * tau * py / (xl * pz);
+    PH[i] = PH[i] + 0.5 * tau * (g / (xl * pz) - 1.0 / (beta0 * xl));
+  }
+}
+
+// scatter2_BeamBunch (src/Appl/BeamBunch.f90:2726-3067) with SC: the space-
+// charge E gathered at the 0th-order t-frame position (scaled back by
+// tmpscale), the back transform to the z frame, the element's field at the
+// particle and the implicit Lorentz push.  Without SC it is scatter20
+// (:3436-3527): the element's field only, on z-frame particles.
+template <typename T, bool SC>
+__global__ void __launch_bounds__(kBlock)
+k_kick2(double *__restrict__ X, double *__restrict__ PX, double *__restrict__ Y,
+        double *__restrict__ PY, double *__restrict__ PH, double *__restrict__ PT,
+        const uint32_t *__restrict__ orig, const double *__restrict__ qm0, int64_t n,
+        const E4<T> *__restrict__ e, const DevDom *__restrict__ dom, int nx, int ny, int nz,
+        Kick2 k) {
+  const int64_t sy = nx, sz = (int64_t)nx * ny;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double x = X[i], y = Y[i], ph = PH[i], px = PX[i], py = PY[i], pt = PT[i];
+    double exn = 0.0, eyn = 0.0, ezn = 0.0;
+    if constexpr (SC) {
+      const double xmin = dom->range[0], ymin = dom->range[2], zmin = dom->range[4];
+      const double hx = dom->msize[0], hy = dom->msize[1], hz = dom->msize[2];
+      const double hxi = dom->hinv[0], hyi = dom->hinv[1], hzi = dom->hinv[2];
+      int ix = (int)((x - xmin) * hxi + 1.0);
+      const double ab = ((xmin - x) + ix * hx) * hxi;
+      int jx = (int)((y - ymin) * hyi + 1.0);
+      const double cd = ((ymin - y) + jx * hy) * hyi;
+      int kx = (int)((ph - zmin) * hzi + 1.0);
+      const double ef = ((zmin - ph) + kx * hz) * hzi;
+      ix = min(max(ix, 1), nx - 1);
+      jx = min(max(jx, 1), ny - 1);
+      kx = min(max(kx, 1), nz - 1);
+      T gx, gy, gz;
+      cic8_e4<T>(e, (ix - 1) + sy * (jx - 1) + sz * (kx - 1), sy, sz, (T)ab, (T)cd, (T)ef,
+                 gx, gy, gz);
+      exn = (double)gx * k.gam;
+      eyn = (double)gy * k.gam;
+      ezn = (double)gz;
+      const double rcp = 1.0 / (-pt + k.gam);
+      const double betai = sqrt(1.0 - rcp * rcp * (1 + px * px + py * py));
+      ph = ph * k.xk / (-k.gam * betai);
+      x = x * k.xk;
+      y = y * k.xk;
+    }
+    const double pos[4] = {x * k.xl, y * k.xl, k.z, ph + k.tg};
+    double f[6];
+    elem_field(k.e, k.xl, k.clight, pos, f);
+    const double ex = k.tmpscale * exn + f[0], ey = k.tmpscale * eyn + f[1];
+    const double ez = k.tmpscale * ezn + f[2];
+    const double bx = -k.beta0_c * k.tmpscale * eyn + f[3];
+    const double by = k.beta0_c * k.tmpscale * exn + f[4];
+    lorentz_push(px, py, pt, qm0[orig[i]], ex, ey, ez, bx, by, f[5], k.tau, k.clight,
+                 k.tmpgamma0, k.gam, k.qmcc, k.ez0);
+    X[i] = x; Y[i] = y; PH[i] = ph;
+    PX[i] = px; PY[i] = py; PT[i] = pt;
+  }
+}
+
 // ----------------------------------------------------------------- moments
 __global__ void __launch_bounds__(kBlock)
 k_moments(const double *__restrict__ X, const double *__restrict__ PX,
@@ -488,6 +560,41 @@ int launch_gather_kick(izh_bunch *b, izh_field *f, double tau, double xycon,
                        0, b->stream, b->r[0], b->r[1], b->r[2], b->r[3], b->r[4],
                        b->r[5], b->n, (const E4<float> *)f->e, f->d_dom, f->nx,
                        f->ny, f->nz, gam, xk, txy, ttc);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+int launch_drift2(izh_bunch *b, double tau, double gamma0, double beta0) {
+  touch_particles(b);
+  if (b->n == 0) return IZH_OK;
+  ScopedTimer tm_(IZH_T_PUSH, b->stream);
+  hipLaunchKernelGGL(k_drift2, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream, b->r[0],
+                     b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, tau, gamma0, beta0,
+                     b->c.scxl);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+int launch_kick2(izh_bunch *b, izh_field *f, const Kick2 &k, bool space_charge) {
+  touch_particles(b);
+  if (b->n == 0) return IZH_OK;
+  ScopedTimer tm_(IZH_T_KICK, b->stream);
+  if (!space_charge) {
+    hipLaunchKernelGGL((k_kick2<double, false>), dim3(grid_for(b->n)), dim3(kBlock), 0,
+                       b->stream, b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5],
+                       b->orig, b->qm0, b->n, (const E4<double> *)nullptr, nullptr, 2, 2, 2,
+                       k);
+  } else if (f->precision == IZH_FP64) {
+    hipLaunchKernelGGL((k_kick2<double, true>), dim3(grid_for(b->n)), dim3(kBlock), 0,
+                       b->stream, b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5],
+                       b->orig, b->qm0, b->n, (const E4<double> *)f->e, f->d_dom, f->nx,
+                       f->ny, f->nz, k);
+  } else {
+    hipLaunchKernelGGL((k_kick2<float, true>), dim3(grid_for(b->n)), dim3(kBlock), 0,
+                       b->stream, b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5],
+                       b->orig, b->qm0, b->n, (const E4<float> *)f->e, f->d_dom, f->nx,
+                       f->ny, f->nz, k);
+  }
   IZH_HIP(hipGetLastError());
   return IZH_OK;
 }

@@ -50,6 +50,15 @@ module impactz_hip
     real(c_double) :: xrad, yrad
   end type izh_segment
 
+  ! an element for the Lorentz integrator (Flagmap=2): type, values v1..v4,
+  ! length and entrance z
+  type, bind(C) :: izh_element
+    integer(c_int) :: bitype
+    real(c_double) :: v(4)
+    real(c_double) :: length
+    real(c_double) :: zedge
+  end type izh_element
+
   interface
     function izh_last_error() bind(C, name="izh_last_error") result(p)
       import :: c_ptr
@@ -142,6 +151,34 @@ module impactz_hip
       integer(c_int), value :: bitype
       real(c_double), intent(in) :: v(4)
       integer(c_int), value :: mapstp
+      integer(c_int) :: rc
+    end function
+    function izh_drift2(b, z, tau) bind(C, name="izh_drift2") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      real(c_double) :: z
+      real(c_double), value :: tau
+      integer(c_int) :: rc
+    end function
+    function izh_kick2(b, z, tau, e, f, dom, flagbc) bind(C, name="izh_kick2") result(rc)
+      import :: c_int, c_double, c_ptr, izh_element, izh_compdom
+      type(c_ptr), value :: b, f
+      real(c_double), value :: z, tau
+      type(izh_element), intent(in) :: e
+      type(izh_compdom), intent(in) :: dom
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
+    function izh_step2(b, f, dom, z, tau1, e, xrad, yrad, bcurr, moments) &
+        bind(C, name="izh_step2") result(rc)
+      import :: c_int, c_double, c_ptr, izh_element, izh_compdom
+      type(c_ptr), value :: b, f
+      type(izh_compdom) :: dom
+      real(c_double) :: z
+      real(c_double), value :: tau1
+      type(izh_element), intent(in) :: e
+      real(c_double), value :: xrad, yrad, bcurr
+      real(c_double) :: moments(44)
       integer(c_int) :: rc
     end function
     function izh_drift1(b, z, tau, bitype, gradient, mapstp) &

@@ -33,7 +33,7 @@ SYMBOLS = (
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
     "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
-    "izh_map1",
+    "izh_map1", "izh_drift2", "izh_kick2", "izh_step2",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -62,6 +62,22 @@ class Segment(C.Structure):
     v1..v4 (the deck line's, i.e. Param(2:5)) and pipe aperture."""
     _fields_ = [("tau1", C.c_double), ("bitype", C.c_int), ("mapstp", C.c_int),
                 ("v", C.c_double * 4), ("xrad", C.c_double), ("yrad", C.c_double)]
+
+
+class ElementC(C.Structure):
+    """izh_element: type, values v1..v4, length and entrance z of an element
+    for the Lorentz integrator."""
+    _fields_ = [("bitype", C.c_int), ("v", C.c_double * 4), ("length", C.c_double),
+                ("zedge", C.c_double)]
+
+
+def element(bitype: int, v, length: float, zedge: float) -> ElementC:
+    e = ElementC()
+    e.bitype = bitype
+    for k, x in enumerate(values4(v)):
+        e.v[k] = x
+    e.length, e.zedge = length, zedge
+    return e
 
 
 def values4(v) -> np.ndarray:
@@ -118,6 +134,11 @@ def lib() -> C.CDLL:
     L.izh_element_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_int, _dp,
                                         C.c_int, _dp, C.c_double, C.c_double, _dp]
     L.izh_map1.argtypes = [vp, _dp, C.c_double, C.c_int, _dp, C.c_int]
+    L.izh_drift2.argtypes = [vp, _dp, C.c_double]
+    L.izh_kick2.argtypes = [vp, C.c_double, C.c_double, C.POINTER(ElementC), vp,
+                            C.POINTER(CompDomC), C.c_int]
+    L.izh_step2.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.c_double,
+                            C.POINTER(ElementC), C.c_double, C.c_double, C.c_double, _dp]
     L.izh_map_linear.argtypes = [vp, _dp]
     L.izh_conv1st.argtypes = [vp, C.c_double, _dp, _i64p, _i64p, C.c_double,
                               C.c_double, C.c_int]
@@ -350,6 +371,29 @@ class BeamBunch:
         vv = values4(v)
         check(lib().izh_map1(self.h, C.byref(zz), tau, bitype, _p(vv), mapstp))
         return zz.value
+
+    def drift2(self, z: float, tau: float) -> float:
+        """drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2; returns z."""
+        zz = C.c_double(z)
+        check(lib().izh_drift2(self.h, C.byref(zz), tau))
+        return zz.value
+
+    def kick2(self, z: float, tau: float, elem: ElementC, field: FieldQuant | None,
+              dom: CompDom | None, flagbc: int = 1) -> None:
+        """kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508)."""
+        check(lib().izh_kick2(self.h, z, tau, C.byref(elem), field.h if field else None,
+                              C.byref(dom.c) if dom else None, flagbc))
+
+    def step2(self, field: FieldQuant, dom: CompDom, z: float, tau1: float,
+              elem: ElementC, xrad: float, yrad: float, bcurr: float,
+              want_moments: bool = False):
+        """One Flagmap=2 segment of run_AccSimulator."""
+        zz = C.c_double(z)
+        mom = np.zeros(44) if want_moments else None
+        check(lib().izh_step2(self.h, field.h, C.byref(dom.c), C.byref(zz), tau1,
+                              C.byref(elem), xrad, yrad, bcurr,
+                              _p(mom) if want_moments else None))
+        return zz.value, mom
 
     def conv1st(self, tau: float, xrad: float, yrad: float, flagbc: int = 1):
         """conv1st_BeamBunch (src/Appl/BeamBunch.f90:1252-1665); returns (ptrange, nplc, nptot)."""

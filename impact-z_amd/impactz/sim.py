@@ -61,8 +61,11 @@ class Simulation:
     """A device-resident bunch + field marched through a lattice."""
 
     def __init__(self, pts: np.ndarray, beam: Beam, lattice: list[Element],
-                 grid: tuple[int, int, int], precision: int = hip.FP64):
+                 grid: tuple[int, int, int], precision: int = hip.FP64, flagmap: int = 1):
         self.beam = beam
+        self.flagmap = flagmap
+        self._left = 0      # segments left in the current element (Flagmap=2)
+        self.zedge = 0.0    # z of the current element's entrance
         self.lattice = lattice
         self.grid = tuple(grid)
         self.const = hip.physconst(beam.freq)
@@ -78,8 +81,19 @@ class Simulation:
                 yield e
 
     def step(self, e: Element, want_moments: bool = True):
-        self.z, mom = self.bunch.step(self.field, self.dom, self.z, self.segment_args(e),
-                                      self.beam.current, want_moments)
+        if self.flagmap == 2:
+            if self._left == 0:
+                # the element's entrance (src/Contrl/AccSimulator.f90:1025-1026)
+                self._left, self.zedge = e.nseg, self.z
+            self._left -= 1
+            self.z, mom = self.bunch.step2(
+                self.field, self.dom, self.z, 0.5 * e.length / e.nseg,
+                hip.element(e.type, e.values, e.length, self.zedge), e.radius, e.radius,
+                self.beam.current, want_moments)
+        else:
+            self.z, mom = self.bunch.step(self.field, self.dom, self.z,
+                                          self.segment_args(e), self.beam.current,
+                                          want_moments)
         self.nstep += 1
         return mom
 
@@ -108,6 +122,9 @@ class Simulation:
         """Steps start..start+nsteps-1 of the lattice through izh_run.
 
         Returns the per-step records (nsteps x 45: moments, then Npt) or None."""
+        if self.flagmap != 1:
+            raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
+                               "Flagmap=2 runs step by step")
         segs = list(self.segments())[start:start + nsteps]
         self.z, mom = self.bunch.run(self.field, self.dom, self.z,
                                      [self.segment_args(e) for e in segs],
@@ -118,6 +135,9 @@ class Simulation:
     def run_elements(self, elements, want_moments: bool = True):
         """One space-charge step per entry of ``elements`` through izh_run;
         returns the per-step records (45 values: moments, then Npt)."""
+        if self.flagmap != 1:
+            raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
+                               "Flagmap=2 runs step by step")
         elements = list(elements)
         self.z, mom = self.bunch.run(self.field, self.dom, self.z,
                                      [self.segment_args(e) for e in elements],

@@ -191,6 +191,41 @@ typedef struct {
 int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
              const izh_segment *seg, double bcurr, double *moments);
 
+/* ---- The nonlinear Lorentz integrator (Flagmap=2) ---- */
+
+/* An element for the Lorentz integrator: its type and values (as for
+ * izh_map1), its length and the z of its entrance (the Param(1) "zedge"
+ * run_AccSimulator sets, src/Contrl/AccSimulator.f90:1025-1026).  Fields:
+ * none for a drift, B = G (y, x, 0) for a quadrupole (file ID <= 0), the
+ * constant-focusing and solenoid fields of getfld_ConstFoc / getfld_Sol. */
+typedef struct {
+  int bitype;
+  double v[4];
+  double length;
+  double zedge;
+} izh_element;
+
+/* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2;
+ * *z and refpt(5) advance by tau/2. */
+int izh_drift2(izh_bunch *b, double *z, double tau);
+
+/* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508), Flagbc=1, at the
+ * lattice position z: after izh_cvbkforth1st with current, the space-charge
+ * field of the last izh_update3O plus the element's field (scatter2); on a
+ * z-frame bunch the element's field alone (scatter20); each applied by the
+ * implicit Lorentz push.  refpt(6) follows the on-axis Ez.  The bunch ends
+ * in the z frame. */
+int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
+              const izh_compdom *dom, int flagbc);
+
+/* One segment of the reference loop with Flagmap=2: drift2(2 tau1),
+ * conv1st, update_CompDom, charge, update3O, cvbkforth1st, kick2(2 tau1),
+ * drift2(2 tau1) (src/Contrl/AccSimulator.f90:1099-1488).  Arguments and
+ * moments as for izh_step. */
+int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double tau1,
+              const izh_element *e, double xrad, double yrad, double bcurr,
+              double *moments);
+
 /* nseg consecutive izh_step calls (the element/segment loop of
  * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504; every element of
  * izh_map1 except the K-mode quadrupole, which runs through izh_step) run as one

@@ -729,6 +729,213 @@ void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
   free(e);
 }
 
+/* ----------------------------------------- nonlinear integrator (Flagmap=2) */
+
+void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], double *z,
+               double tau) {
+  /* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509) */
+  const double xl = c->scxl;
+  const double gamma0 = -refpt[5];
+  const double beta0 = sqrt(1.0 - 1.0 / gamma0 / gamma0);
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    double pz = sqrt((gamma0 - P(6, i)) * (gamma0 - P(6, i)) - 1.0 - P(2, i) * P(2, i) -
+                     P(4, i) * P(4, i));
+    P(1, i) = P(1, i) + 0.5 * tau * P(2, i) / (xl * pz);
+    P(3, i) = P(3, i) + 0.5 * tau * P(4, i) / (xl * pz);
+    P(5, i) = P(5, i) + 0.5 * tau * ((gamma0 - P(6, i)) / (xl * pz) - 1.0 / (beta0 * xl));
+  }
+  refpt[4] = refpt[4] + 0.5 * tau / (xl * beta0);
+  *z = *z + 0.5 * tau;
+}
+
+/* getfld_BeamLineElem for the analytic fields (src/Appl/BeamLineElem.f90:
+ * 827-870): E (V/m) and B (T) at pos = (x, y, z, t).  Returns -1 for the
+ * elements not restated. */
+int or_elem_field(const or_const *c, int bitype, const double v[4], double length,
+                  double zedge, const double pos[4], double f[6]) {
+  int k;
+  for (k = 0; k < 6; k++) f[k] = 0.0;
+  if (bitype == 0) return 0;  /* getfld_DriftTube: no field */
+  if (bitype == 1) {
+    /* getfld_Quadrupole (src/Appl/Quadrupole.f90:369-402), file ID <= 0 */
+    if (v[1] > 0.0) return -1;
+    f[3] = v[0] * pos[1];
+    f[4] = v[0] * pos[0];
+    return 0;
+  }
+  if (bitype == 2) {
+    /* getfld_ConstFoc (src/Appl/ConstFoc.f90:297-322): qmcc = gambetz = 1 */
+    const double qmcc = 1.0, pz = 1.0;
+    f[2] = v[2] * pos[3] * pz * pz * pz * c->scxl / qmcc;
+    f[3] = -v[0] * pos[1] * pz / qmcc / c->clight;
+    f[4] = v[1] * pos[0] * pz / qmcc / c->clight;
+    return 0;
+  }
+  if (bitype == 3) {
+    /* getfld_Sol (src/Appl/Sol.f90:388-435): linear ramps over 2*Param(4) */
+    const double x02 = 2 * v[2];
+    const double zz = pos[2] - zedge;
+    const double b0 = v[0];
+    if (zz >= 0.0 && zz < x02) {
+      const double bb = b0 / x02;
+      f[3] = -bb / 2 * pos[0];
+      f[4] = -bb / 2 * pos[1];
+      f[5] = bb * zz;
+    } else if (zz >= x02 && zz <= length - x02) {
+      f[5] = b0;
+    } else if (zz > length - x02 && zz <= length) {
+      const double bb = -b0 / x02;
+      const double zshift = length - x02;
+      f[3] = -bb / 2 * pos[0];
+      f[4] = -bb / 2 * pos[1];
+      f[5] = b0 + bb * (zz - zshift);
+    }
+    return 0;
+  }
+  return -1;
+}
+
+/* The implicit Lorentz push of scatter2/scatter20 (src/Appl/BeamBunch.f90:
+ * 3026-3066): two iterations with the longitudinal momentum averaged. */
+static void lorentz_push(double *p, double ex, double ey, double ez, double bx, double by,
+                         double bz, double tau, double clight, double tmpgamma0,
+                         double gam, double qmcc, double ez0) {
+  const double q = p[6];
+  const double tmp12 = -tau * clight * bz * q / 2;
+  const double tmp13 = tau * ex * q / 2;
+  const double tmp23 = tau * ey * q / 2;
+  const double tmp1 = tmpgamma0 * ex * tau * q;
+  const double tmp2 = clight * by * tau * q;
+  const double tmp3 = tmpgamma0 * ey * tau * q;
+  const double tmp4 = clight * bx * tau * q;
+  const double tmp5 = (ez0 * qmcc - ez * q) * tau;
+  double pz = sqrt((gam - p[5]) * (gam - p[5]) - 1.0 - p[1] * p[1] - p[3] * p[3]);
+  const double p1 = p[1], p2 = p[3], p3 = p[5];
+  int ii;
+  for (ii = 0; ii < 2; ii++) {
+    const double a12 = tmp12 / pz, a13 = tmp13 / pz, a23 = tmp23 / pz;
+    const double b1 = p1 - a12 * p2 - a13 * p3 + tmp1 / pz - tmp2;
+    const double b2 = a12 * p1 + p2 - a23 * p3 + tmp3 / pz + tmp4;
+    const double b3 = -a13 * p1 - a23 * p2 + p3 + tmp5;
+    const double det = 1 + a12 * a12 - a13 * a13 - a23 * a23;
+    const double s11 = 1 - a23 * a23, s12 = -a12 + a13 * a23, s13 = a12 * a23 - a13;
+    const double s21 = a12 + a13 * a23, s22 = 1 - a13 * a13, s23 = -a23 - a13 * a12;
+    const double s31 = -a12 * a23 - a13, s32 = -a23 + a12 * a13, s33 = 1 + a12 * a12;
+    double tmpgamma00, tmpsq2;
+    p[1] = (s11 * b1 + s12 * b2 + s13 * b3) / det;
+    p[3] = (s21 * b1 + s22 * b2 + s23 * b3) / det;
+    p[5] = (s31 * b1 + s32 * b2 + s33 * b3) / det;
+    tmpgamma00 = gam + tau * qmcc * ez0;
+    tmpsq2 = (tmpgamma00 - p[5]) * (tmpgamma00 - p[5]) - 1.0 - p[1] * p[1] - p[3] * p[3];
+    if (tmpsq2 > 0.0) pz = 0.5 * pz + 0.5 * sqrt(tmpsq2);
+  }
+}
+
+int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+             const double *phi, const double range[6], const double msize[3],
+             double refpt[6], double curr, double tau, double mass, double chrg,
+             int bitype, const double v[4], double length, double zedge, double z) {
+  /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508) with scatter2
+   * (:2510-3071) or, without current, scatter20 (:3075-3531), for the
+   * elements with analytic fields */
+  const double xk = 1.0 / c->scxl, xl = c->scxl;
+  const double tmpscale = c->clight * c->clight * (double)1.0e-7f;
+  const double tg = refpt[4];
+  double gam = -refpt[5];
+  const double beta0 = sqrt(gam * gam - 1.0) / gam;
+  const double qmcc = chrg / mass;
+  double pos[4] = {0.0, 0.0, z, tg}, fld[6];
+  double ez0, tmpgamma0;
+  int64_t i;
+  if (or_elem_field(c, bitype, v, length, zedge, pos, fld) != 0) return -1;
+  ez0 = fld[2];
+  tmpgamma0 = gam + 0.5 * tau * qmcc * ez0;
+  if (curr > 0.0) {
+    size_t g = (size_t)nx * ny * nz;
+    double *e = (double *)malloc(3 * g * sizeof(double));
+    const double *ex = e, *ey = e + g, *ezg = e + 2 * g;
+    const double hxi = 1.0 / msize[0], hyi = 1.0 / msize[1], hzi = 1.0 / msize[2];
+    const double hx = msize[0], hy = msize[1], hz = msize[2];
+    const double xmin = range[0], ymin = range[2], zmin = range[4];
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    or_efield(phi, nx, ny, nz, msize, e, e + g, e + 2 * g);
+    for (i = 0; i < n; i++) {
+      int ix = (int)((P(1, i) - xmin) * hxi + 1.0);
+      double ab = ((xmin - P(1, i)) + ix * hx) * hxi;
+      int jx = (int)((P(3, i) - ymin) * hyi + 1.0);
+      double cd = ((ymin - P(3, i)) + jx * hy) * hyi;
+      int kx = (int)((P(5, i) - zmin) * hzi + 1.0);
+      double ef = ((zmin - P(5, i)) + kx * hz) * hzi;
+      int64_t o = (ix - 1) + sy * (jx - 1) + sz * (kx - 1);
+      double exn = cic(ex, o, sy, sz, ab, cd, ef) * gam;
+      double eyn = cic(ey, o, sy, sz, ab, cd, ef) * gam;
+      double ezn = cic(ezg, o, sy, sz, ab, cd, ef);
+      double rcpgammai = 1.0 / (-P(6, i) + gam);
+      double betai = sqrt(1.0 - rcpgammai * rcpgammai *
+                                    (1 + P(2, i) * P(2, i) + P(4, i) * P(4, i)));
+      P(5, i) = P(5, i) * xk / (-gam * betai);
+      P(1, i) = P(1, i) * xk;
+      P(3, i) = P(3, i) * xk;
+      pos[0] = P(1, i) * xl;
+      pos[1] = P(3, i) * xl;
+      pos[2] = z;
+      pos[3] = P(5, i) + tg;
+      or_elem_field(c, bitype, v, length, zedge, pos, fld);
+      lorentz_push(&P(1, i), tmpscale * exn + fld[0], tmpscale * eyn + fld[1],
+                   tmpscale * ezn + fld[2], -beta0 / c->clight * tmpscale * eyn + fld[3],
+                   beta0 / c->clight * tmpscale * exn + fld[4], fld[5], tau, c->clight,
+                   tmpgamma0, gam, qmcc, ez0);
+    }
+    free(e);
+  } else {
+    for (i = 0; i < n; i++) {
+      pos[0] = P(1, i) * xl;
+      pos[1] = P(3, i) * xl;
+      pos[2] = z;
+      pos[3] = P(5, i) + tg;
+      or_elem_field(c, bitype, v, length, zedge, pos, fld);
+      lorentz_push(&P(1, i), fld[0], fld[1], fld[2], fld[3], fld[4], fld[5], tau,
+                   c->clight, tmpgamma0, gam, qmcc, ez0);
+    }
+  }
+  gam = gam + tau * qmcc * ez0;
+  refpt[5] = -gam;
+  return 0;
+}
+
+int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
+                 int nx, int ny, int nz, double tau1, int bitype, const double v[4],
+                 double length, double zedge, double pipe_radius, double *rho_out,
+                 double *phi_out) {
+  /* one segment of the reference loop with Flagmap=2
+   * (src/Contrl/AccSimulator.f90:1106-1113,1446-1458) */
+  size_t g = (size_t)nx * ny * nz;
+  double *rho = rho_out ? rho_out : (double *)calloc(g, sizeof(double));
+  double *phi = phi_out ? phi_out : (double *)calloc(g, sizeof(double));
+  double ptrange[6], range[6] = {0, 0, 0, 0, 0, 0}, msize[3] = {1, 1, 1};
+  const double tau2 = 2.0 * tau1;
+  int rc = 0;
+  double curr = 0.0;
+  or_drift2(c, pts, n, st->refpt, &st->z, tau2);
+  if (st->bcurr >= (double)1.0e-10f) {
+    n = or_conv1st(c, pts, n, st->refpt, pipe_radius, pipe_radius, ptrange,
+                   &st->npt, &st->current);
+    or_update_compdom(ptrange, nx, ny, nz, range, msize);
+    or_deposit(pts, n, nx, ny, nz, range, msize, rho);
+    rc = or_solve_openbc(rho, nx, ny, nz, msize[0], msize[1], msize[2], phi);
+    or_cvbkforth1st(c, pts, n, st->refpt);
+    curr = st->current;
+  }
+  if (rc == 0)
+    rc = or_kick2(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, curr, tau2,
+                  st->mass, st->charge, bitype, v, length, zedge, st->z);
+  if (rc == 0) or_drift2(c, pts, n, st->refpt, &st->z, tau2);
+  if (!rho_out) free(rho);
+  if (!phi_out) free(phi);
+  return rc == 0 ? n : -1;
+}
+
 /* -------------------------------------------------------- diagnostics */
 
 void or_diag_sums(const double *pts, int64_t n, double s[37], double amax[6],

@@ -105,6 +105,30 @@ void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
               const double refpt[6], double curr, double tau, double mass,
               double chrg);
 
+/* ---- the nonlinear Lorentz integrator (Flagmap=2) ---- */
+
+/* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2;
+ * z and refpt(5) advance. */
+void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], double *z,
+               double tau);
+
+/* getfld_BeamLineElem (src/Appl/BeamLineElem.f90:827-870) for the elements
+ * with analytic fields: drift (none), quadrupole (file ID <= 0: B = G (y, x, 0)),
+ * constant focusing and solenoid (with its linear end ramps).  pos = (x, y
+ * in m, z, t); f = (Ex, Ey, Ez, Bx, By, Bz).  zedge is the element's entrance
+ * z.  Returns -1 for anything else. */
+int or_elem_field(const or_const *c, int bitype, const double v[4], double length,
+                  double zedge, const double pos[4], double f[6]);
+
+/* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508): the space-charge E of
+ * phi (when curr > 0, scatter2) plus the element's field, applied by the
+ * implicit Lorentz push; refpt(6) follows the on-axis Ez.  Particles in the
+ * 0th-order t frame (curr > 0) or the z frame (scatter20). */
+int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+             const double *phi, const double range[6], const double msize[3],
+             double refpt[6], double curr, double tau, double mass, double chrg,
+             int bitype, const double v[4], double length, double zedge, double z);
+
 /* diagnostic1_Output sums (src/Contrl/Output.f90:119-245): 37 sums in the
  * reference's tmplc order, 6 abs-maxima, max of x^2+y^2. */
 void or_diag_sums(const double *pts, int64_t n, double sums[37], double amax[6],
@@ -137,6 +161,13 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                 int nx, int ny, int nz, double tau1, int bitype, const double v[4],
                 int mapstp, double pipe_radius, double *rho_out,
                 double *phi_out);
+
+/* One Flagmap=2 segment: drift2(2 tau1) -> conv1st -> update_CompDom ->
+ * deposit -> solve -> cvbkforth1st -> kick2(2 tau1) -> drift2(2 tau1). */
+int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
+                 int nx, int ny, int nz, double tau1, int bitype, const double v[4],
+                 double length, double zedge, double pipe_radius, double *rho_out,
+                 double *phi_out);
 
 #ifdef __cplusplus
 }

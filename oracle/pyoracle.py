@@ -89,6 +89,19 @@ def lib() -> C.CDLL:
                               C.c_double, C.c_int, _dp, C.c_int,
                               C.c_double, _dp, _dp]
         L.or_step.restype = C.c_int64
+        L.or_drift2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double]
+        L.or_elem_field.argtypes = [C.POINTER(OrConst), C.c_int, _dp, C.c_double,
+                                    C.c_double, _dp, _dp]
+        L.or_elem_field.restype = C.c_int
+        L.or_kick2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.c_int, C.c_int,
+                               C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_int, _dp, C.c_double,
+                               C.c_double, C.c_double]
+        L.or_kick2.restype = C.c_int
+        L.or_step2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.POINTER(OrState),
+                               C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
+                               C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.or_step2.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -140,6 +153,33 @@ def map1(c, pts, refpt, z, tau, bitype, v, mapstp, charge, mass):
                      tau, bitype, _p(vv), mapstp, charge, mass) != 0:
         raise ValueError(f"element type {bitype} with values {vv} is not restated")
     return zz.value
+
+
+def drift2(c, pts, refpt, z, tau):
+    """drift2_BeamBunch in place; returns the new z."""
+    zz = C.c_double(z)
+    lib().or_drift2(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz), tau)
+    return zz.value
+
+
+def elem_field(c, bitype, v, length, zedge, pos):
+    f = np.zeros(6)
+    vv = element_values(v)
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    if lib().or_elem_field(C.byref(c), bitype, _p(vv), length, zedge, _p(p), _p(f)) != 0:
+        raise ValueError(f"no analytic field for element type {bitype}")
+    return f
+
+
+def kick2(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, bitype, v, length, zedge, z):
+    """kick2_BeamBunch in place on pts (and refpt(6))."""
+    nz, ny, nx = phi.shape
+    vv = element_values(v)
+    if lib().or_kick2(C.byref(c), _p(pts), pts.shape[0], nx, ny, nz, _p(phi),
+                      _p(np.ascontiguousarray(rng, dtype=np.float64)),
+                      _p(np.ascontiguousarray(ms, dtype=np.float64)), _p(refpt), curr, tau,
+                      mass, chrg, bitype, _p(vv), length, zedge, z) != 0:
+        raise ValueError(f"no analytic field for element type {bitype}")
 
 
 def conv1st(c, pts, refpt, xrad, yrad, npt, current):
@@ -227,7 +267,10 @@ def diag_row(c, sums, amax, r2max, npt, refpt, mass):
 class Run:
     """The reference segment loop over a lattice, on the C oracle."""
 
-    def __init__(self, pts, beam, lattice, grid):
+    def __init__(self, pts, beam, lattice, grid, flagmap: int = 1):
+        self.flagmap = flagmap
+        self.zedge = 0.0
+        self._left = 0   # segments left in the current element
         self.c = physconst(beam.freq)
         self.pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
         self.n = self.pts.shape[0]
@@ -257,10 +300,20 @@ class Run:
         phi = np.zeros((nz, ny, nx)) if want_fields else None
         tau1 = 0.5 * e.length / e.nseg
         vv = element_values([e.v1, e.v2, e.v3, e.v4])
-        n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
-                          nx, ny, nz, tau1, e.type, _p(vv), e.mapstp, e.radius,
-                          _p(rho) if want_fields else None,
-                          _p(phi) if want_fields else None)
+        if self.flagmap == 1:
+            n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
+                              nx, ny, nz, tau1, e.type, _p(vv), e.mapstp, e.radius,
+                              _p(rho) if want_fields else None,
+                              _p(phi) if want_fields else None)
+        else:
+            if self._left == 0:
+                # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
+                self._left, self.zedge = e.nseg, self.st.z
+            self._left -= 1
+            n = lib().or_step2(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
+                               nx, ny, nz, tau1, e.type, _p(vv), e.length, self.zedge,
+                               e.radius, _p(rho) if want_fields else None,
+                               _p(phi) if want_fields else None)
         if n < 0:
             raise RuntimeError("or_step failed")
         self.n = n

@@ -14,6 +14,7 @@
 ! harness.in):
 !   N Nx Ny Nz
 !   curr kinetic_eV mass_eV charge freq phase0
+!   flagmap (1 linear maps, 2 nonlinear Lorentz integrator)
 !   nelem
 !   nelem lines:  length nseg mapstp type v1 v2 v3 v4
 !                 (the deck line's values, src/Contrl/AccSimulator.f90:
@@ -68,13 +69,14 @@
         double precision, allocatable, dimension(:,:,:) :: chgdens
         character(len=256) :: ctl, pfile, dname
         logical :: dodump
-        integer :: n32
+        integer :: n32, flagmap
 
         ctl = 'harness.in'
         if (command_argument_count() >= 1) call get_command_argument(1, ctl)
         open(unit=11, file=trim(ctl), status='old')
         read(11,*) N, Nx, Ny, Nz
         read(11,*) curr, kin, mass, chg, freq, ph0
+        read(11,*) flagmap
         read(11,*) nelem
         allocate(elen(nelem), enseg(nelem), empstp(nelem), etype(nelem))
         allocate(ev1(nelem), ev2(nelem), ev3(nelem), ev4(nelem))
@@ -187,6 +189,8 @@
           call getparam_BeamLineElem(Blnelem(i), blength, bnseg, bmpstp, &
                                      bitype)
           call getradius_BeamLineElem(Blnelem(i), piperad, piperad2)
+          ! the element's entrance z (src/Contrl/AccSimulator.f90:1025-1026)
+          call setparam_BeamLineElem(Blnelem(i), 1, z)
           tau1 = 0.0d0
           if (bitype >= 0) tau1 = 0.5d0*blength/bnseg
           tau2 = 2.0d0*tau1
@@ -204,8 +208,12 @@
               write(u) Bpts%Current
               call wpts(u)
             endif
-            call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
-                                bnseg, j, ihlf)
+            if (flagmap == 1) then
+              call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
+                                  bnseg, j, ihlf)
+            else
+              call map1_BeamBunch(Bpts, z, tau2)
+            endif
             if (dodump) call wpts(u)
             call conv1st_BeamBunch(Bpts, tau2, Nplocal, Np, ptrange, &
                                    1, 1.0d0, piperad, piperad2)
@@ -236,11 +244,20 @@
             call cvbkforth1st_BeamBunch(Bpts)
             call setnpt_BeamBunch(Bpts, Nplocal)
             if (dodump) call wpts(u)
-            call map2_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
-                 lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0)
+            if (flagmap == 1) then
+              call map2_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
+                   lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0)
+            else
+              call map2_BeamBunch(Bpts, Blnelem(i), z, tau2, lcgrid(1), &
+                   lcgrid(2), lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 0)
+            endif
             if (dodump) call wpts(u)
-            call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
-                                bnseg, j, ihlf)
+            if (flagmap == 1) then
+              call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
+                                  bnseg, j, ihlf)
+            else
+              call map1_BeamBunch(Bpts, z, tau2)
+            endif
             if (dodump) then
               call wpts(u)
               write(u) z

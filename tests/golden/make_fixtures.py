@@ -32,10 +32,11 @@ STAGES = ("p0", "p_map1a", "p_conv1st", "p_cvbk", "p_kick", "p_map1b")
 
 
 def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
-            beam=B.Beam(), stages=STAGES):
+            beam=B.Beam(), stages=STAGES, flagmap=1):
     pts = B.gaussian_bunch(n, seed=seed, beam=beam)
-    ref = R.run_harness(pts, beam, lattice, grid, nsteps, dump_steps)
+    ref = R.run_harness(pts, beam, lattice, grid, nsteps, dump_steps, flagmap=flagmap)
     out = {
+        "flagmap": np.array(flagmap),
         "pts_in": pts,
         "grid": np.array(grid),
         "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
@@ -72,6 +73,16 @@ def cf_sol_lattice():
     return [B.Element(0.1, 4, 10, 2, 30.0, 20.0, 5.0, 1.0),
             B.Element(0.2, 4, 10, 0, 1.0),
             B.Element(0.1, 4, 10, 3, 1.5, 0.0, 1.0),
+            B.Element(0.2, 4, 10, 0, 1.0)]
+
+
+def nl_cf_sol_lattice():
+    """Constant focusing and a solenoid for the Lorentz integrator; the
+    solenoid's Param(4) (its pipe radius, 10 mm) also sets the length of its
+    field ramps in getfld_Sol (2 x 10 mm)."""
+    return [B.Element(0.1, 4, 10, 2, 30.0, 20.0, 5.0, 1.0),
+            B.Element(0.2, 4, 10, 0, 1.0),
+            B.Element(0.1, 4, 10, 3, 1.5, 0.0, 0.01),
             B.Element(0.2, 4, 10, 0, 1.0)]
 
 
@@ -173,6 +184,13 @@ def main():
                       "dumps at steps 2 (constant focusing) and 10 (solenoid)")
     fixture("kquad", 1000, (16, 16, 16), kquad_lattice(), 16, (3,))
     meta["kquad"] = "FODO of K-mode quadrupoles (file ID -1, K = +-5.4 1/m^2), dump at step 3"
+    # the nonlinear Lorentz integrator (Flagmap=2)
+    fixture("nl_fodo", 1000, (16, 16, 16), B.fodo(1), 16, (1, 5), flagmap=2)
+    meta["nl_fodo"] = "Flagmap=2 through one FODO period, dumps at steps 1 (QF) and 5 (drift)"
+    fixture("nl_cf_sol", 1000, (16, 16, 16), nl_cf_sol_lattice(), 16, (2, 10, 11),
+            flagmap=2)
+    meta["nl_cf_sol"] = ("Flagmap=2: constant focusing, drift, solenoid with 20 mm end "
+                         "ramps, drift; dumps at steps 2, 10 and 11")
     io_fixture()
     meta["io_ckpt"] = ("ImpactZexe runs: -2 phase dump and -7 checkpoint after 8 steps, "
                        "the run restarted from that checkpoint, and Flagdist=199 binary input")

@@ -80,7 +80,7 @@ def read_fort(path: str) -> np.ndarray:
 
 
 def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
-                dump_steps=(), workdir: str | None = None) -> dict:
+                dump_steps=(), workdir: str | None = None, flagmap: int = 1) -> dict:
     """Run the reference on ``pts`` through ``lattice``; return dumps and diagnostics."""
     nx, ny, nz = grid
     n = pts.shape[0]
@@ -92,6 +92,7 @@ def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
         f.write(f"{n} {nx} {ny} {nz}\n")
         f.write(f"{beam.current!r} {beam.kinetic!r} {beam.mass!r} "
                 f"{beam.charge!r} {beam.freq!r} {beam.phase0!r}\n")
+        f.write(f"{flagmap}\n")
         f.write(f"{len(lattice)}\n")
         for e in lattice:
             f.write(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} {e.v1!r} "

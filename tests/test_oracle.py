@@ -323,3 +323,124 @@ def test_frame_round_trip_with_zero_field(oracle):
     nt = np.trunc(want[:, 4] / pilc)
     want[:, 4] = want[:, 4] - nt * pilc - np.fmod(nt, 2) * pilc
     assert col_err(p, want) < 1e-12
+
+
+# ------------------------------------------ nonlinear integrator (Flagmap=2)
+
+NL_CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
+            ("nl_cf_sol", 11)]
+
+
+def _zedge(d, step):
+    """Entrance z of the element holding the 1-based step."""
+    from conftest import fixture_lattice
+    z, k = 0.0, 0
+    for e in fixture_lattice(d):
+        if k + e.nseg >= step:
+            return z
+        k += e.nseg
+        z += e.length
+    raise ValueError(step)
+
+
+@pytest.mark.parametrize("name,step", NL_CASES)
+def test_oracle_nonlinear_stages_match_reference(oracle, name, step):
+    d = load_golden(name)
+    assert int(d["flagmap"]) == 2
+    grid = tuple(int(v) for v in d["grid"])
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    s = f"s{step}_"
+    c = oracle.physconst(beam.freq)
+    tau1 = 0.5 * e.length / e.nseg
+    pts = d[s + "p0"].copy()
+    refpt = d[s + "refpt0"].copy()
+    z = oracle.drift2(c, pts, refpt, float(d[s + "z0"]), 2 * tau1)
+    assert col_err(pts, d[s + "p_map1a"]) < 1e-14
+    pts = d[s + "p_cvbk"].copy()
+    refpt_k = refpt.copy()
+    oracle.kick2(c, pts, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt_k,
+                 float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge, e.type,
+                 e.values, e.length, _zedge(d, step), z)
+    assert col_err(pts, d[s + "p_kick"]) < 1e-13
+    pts = d[s + "p_kick"].copy()
+    z = oracle.drift2(c, pts, refpt_k, z, 2 * tau1)
+    assert col_err(pts, d[s + "p_map1b"]) < 1e-14
+    assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
+    assert rel_err(refpt_k, d[s + "refpt"]) < 1e-15
+
+
+@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol"])
+def test_oracle_nonlinear_trajectory_matches_reference(oracle, name):
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid, flagmap=2)
+    nsteps = int(d["nsteps"])
+    for k, e in enumerate(run.segments(), 1):
+        if k > nsteps:
+            break
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+    assert col_err(run.particles(), d["final"]) < 1e-12
+
+
+def test_drift2_equals_exact_drift(oracle):
+    """drift2(2 tau) and the exact drift1(tau) are the same map."""
+    from impactz.beam import Beam, gaussian_bunch
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    pts = gaussian_bunch(500, seed=8)
+    p1, p2 = pts.copy(), pts.copy()
+    r1 = np.zeros(6); r1[5] = -b.gamma0
+    r2 = r1.copy()
+    z1 = oracle.map1(c, p1, r1, 0.0, 0.05, 0, 0.0, 1, b.charge, b.mass)
+    z2 = oracle.drift2(c, p2, r2, 0.0, 0.1)
+    assert col_err(p1, p2) < 1e-13 and z1 == z2
+    assert r1[4] == pytest.approx(r2[4], rel=1e-14)
+
+
+def test_kick2_lorentz_push_properties(oracle):
+    """Without space charge: no field leaves the particles alone; a pure
+    magnetic field (quadrupole, solenoid) keeps |p|; the quadrupole's
+    transverse kick matches the linear map for a short step."""
+    from impactz.beam import Beam, gaussian_bunch
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    pts = gaussian_bunch(400, seed=9)
+    refpt = np.zeros(6); refpt[5] = -b.gamma0
+    phi = np.zeros((4, 4, 4))
+    rng, ms = np.zeros(6), np.ones(3)
+    p = pts.copy()
+    oracle.kick2(c, p, phi, rng, ms, refpt.copy(), 0.0, 0.01, b.mass, b.charge, 0,
+                 [1.0], 0.1, 0.0, 0.05)
+    assert np.array_equal(p, pts)
+    gam0 = b.gamma0
+
+    def pmag(q):
+        return np.sqrt((gam0 - q[:, 5]) ** 2 - 1.0)  # |gamma beta| from the energy
+
+    for bt, v in ((1, [10.0, 0.0, 1.0, 0.0]), (3, [1.5, 0.0, 0.01, 0.0])):
+        p = pts.copy()
+        oracle.kick2(c, p, phi, rng, ms, refpt.copy(), 0.0, 0.01, b.mass, b.charge, bt,
+                     v, 0.1, 0.0, 0.05)
+        # the energy coordinate is unchanged by a magnetic field
+        assert np.max(np.abs(p[:, 5] - pts[:, 5])) < 1e-15
+        assert np.allclose(pmag(p), pmag(pts), rtol=1e-15)
+    # quadrupole: dpx = -G/Brho * x * (gamma beta) * tau for a short step
+    p = pts.copy()
+    g, tau = 10.0, 1e-4
+    oracle.kick2(c, p, phi, rng, ms, refpt.copy(), 0.0, tau, b.mass, b.charge, 1,
+                 [g, 0.0, 1.0, 0.0], 0.1, 0.0, 0.05)
+    qm = b.charge / b.mass
+    brho = b.gambet0 * b.mass / (299792458.0 * b.charge)
+    dpx = (p[:, 1] - pts[:, 1])
+    want = -g * (pts[:, 0] * c.scxl) * tau / brho * b.gambet0
+    assert np.allclose(dpx, want, rtol=2e-2, atol=1e-12)
+    del qm
