@@ -417,6 +417,31 @@ contains
     call izh_check(izh_kick1(b, tau, f, dom, int(flagbc, c_int)), 'kick1')
   end subroutine kick1_gpu
 
+  ! drift2_BeamBunch(this,z,tau) (src/Appl/BeamBunch.f90:467-509), the
+  ! Lorentz integrator's drift (Flagmap=2)
+  subroutine drift2_gpu(b, z, tau)
+    type(c_ptr), intent(in) :: b
+    real(c_double), intent(inout) :: z
+    real(c_double), intent(in) :: tau
+    call izh_check(izh_drift2(b, z, tau), 'drift2')
+  end subroutine drift2_gpu
+
+  ! kick2_BeamBunch(this,beamelem,z,tau,...) (src/Appl/BeamBunch.f90:
+  ! 2187-2508): space-charge field plus the element's field, Lorentz push;
+  ! zedge is the z of the element's entrance
+  subroutine kick2_gpu(b, z, tau, bitype, v, elen, zedge, f, dom, flagbc)
+    type(c_ptr), intent(in) :: b, f
+    real(c_double), intent(in) :: z, tau, v(4), elen, zedge
+    integer, intent(in) :: bitype, flagbc
+    type(izh_compdom), intent(in) :: dom
+    type(izh_element) :: e
+    e%bitype = int(bitype, c_int)
+    e%v = v
+    e%length = elen
+    e%zedge = zedge
+    call izh_check(izh_kick2(b, z, tau, e, f, dom, int(flagbc, c_int)), 'kick2')
+  end subroutine kick2_gpu
+
   ! the reductions of diagnostic1_Output (src/Contrl/Output.f90:119-245)
   subroutine diagnostic1_sums_gpu(b, sums, amax, r2max)
     type(c_ptr), intent(in) :: b

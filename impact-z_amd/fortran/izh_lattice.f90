@@ -3,16 +3,20 @@
 ! lattice of drifts, quadrupoles, constant-focusing elements and solenoids
 ! on the GPU, calling the C ABI through the
 ! impactz_hip module in the order of run_AccSimulator's segment loop
-! (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1, Flagbc=1):
+! (src/Contrl/AccSimulator.f90:1099-1488, Flagbc=1).  Flagmap=1:
 !   drift1(tau1) -> conv1st -> update_CompDom -> charge -> update3O ->
 !   cvbkforth1st -> kick1(2 tau1) -> drift1(tau1) -> diagnostic1
-! With fused = .true. the same segments go through izh_run in one call.
+! Flagmap=2 (the Lorentz integrator): drift2(2 tau1), the same space-charge
+! stages, kick2(2 tau1) with the element's field, drift2(2 tau1).
+! With fused = .true. the Flagmap=1 segments go through izh_run in one call.
 !
 ! Input: particle.in (readin_Dist format, src/Appl/Distribution.f90:1677-1705)
 ! and the namelist file named by the first command-line argument; val(1:4,i)
 ! holds element i's deck values v1..v4 (its Param(2:5)).  Output:
-! one line per step with z, Npt and the 44 diagnostic1 reductions (37 sums,
-! 6 absolute maxima, max x^2+y^2) in the file named by the namelist.
+! one line per step with z, Npt, the 44 diagnostic1 reductions (37 sums,
+! 6 absolute maxima, max x^2+y^2) and the reference particle's gamma
+! (-refptcl(6), which the Lorentz integrator changes in a longitudinal field)
+! in the file named by the namelist.
 !----------------------------------------------------------------------------
 program izh_lattice
   use iso_c_binding
@@ -26,10 +30,11 @@ program izh_lattice
   real(8) :: elen(maxelem) = 0, val(4, maxelem) = 0, radius(maxelem) = 1
   integer :: nseg(maxelem) = 1, mapstp(maxelem) = 1, bitype(maxelem) = 0
   logical :: fused = .false.
+  integer :: flagmap = 1
   character(len=256) :: particles = 'particle.in', output = 'izh_moments.txt'
   namelist /izh_run_nml/ nx, ny, nz, precision, device, current, kinetic, &
       mass, charge, freq, phase0, nelem, elen, val, radius, nseg, mapstp, &
-      bitype, fused, particles, output
+      bitype, fused, flagmap, particles, output
 
   character(len=256) :: nmlfile
   real(c_double), allocatable :: pts(:, :), recs(:, :)
@@ -37,7 +42,7 @@ program izh_lattice
   type(c_ptr) :: b, f
   type(izh_compdom) :: dom
   type(izh_bunch_info) :: info
-  real(c_double) :: z, tau1, tau2, ptrange(6), sums(37), amax(6), r2max
+  real(c_double) :: z, zedge, tau1, tau2, ptrange(6), sums(37), amax(6), r2max
   integer(c_int64_t) :: n, nplc, nptot, nstep, k
   integer :: i, j, u, istep
 
@@ -64,28 +69,46 @@ program izh_lattice
   dom%range = 0; dom%msize = 0
   z = 0.0d0
 
+  if (flagmap /= 1 .and. flagmap /= 2) then
+    print *, 'izh_lattice: flagmap must be 1 or 2'
+    stop 1
+  end if
+  if (fused .and. flagmap /= 1) then
+    print *, 'izh_lattice: izh_run fuses Flagmap=1 segments only'
+    stop 1
+  end if
   open(newunit=u, file=trim(output), status='replace')
   if (.not. fused) then
     istep = 0
     do i = 1, nelem
       tau1 = 0.5d0 * elen(i) / nseg(i)
       tau2 = 2.0d0 * tau1
+      zedge = z
       do j = 1, nseg(i)
-        call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+        if (flagmap == 1) then
+          call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+        else
+          call drift2_gpu(b, z, tau2)
+        end if
         if (current >= 1.0e-10) then
           call conv1st_gpu(b, tau2, nplc, nptot, ptrange, 1, radius(i), radius(i))
           call update_compdom_gpu(dom, ptrange, 1)
           call charge_gpu(b, dom, f)
           call update3O_gpu(f, dom)
           call cvbkforth1st_gpu(b)
-          call kick1_gpu(b, tau2, f, dom, 1)
         end if
-        call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+        if (flagmap == 1) then
+          if (current >= 1.0e-10) call kick1_gpu(b, tau2, f, dom, 1)
+          call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+        else
+          call kick2_gpu(b, z, tau2, bitype(i), val(:, i), elen(i), zedge, f, dom, 1)
+          call drift2_gpu(b, z, tau2)
+        end if
         call diagnostic1_sums_gpu(b, sums, amax, r2max)
         call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
         istep = istep + 1
-        write(u, '(I8,1X,ES25.17,1X,I12,44(1X,ES25.17))') istep, z, info%npt, &
-            sums, amax, r2max
+        write(u, '(I8,1X,ES25.17,1X,I12,45(1X,ES25.17))') istep, z, info%npt, &
+            sums, amax, r2max, -info%refpt(6)
       end do
     end do
   else
@@ -104,6 +127,8 @@ program izh_lattice
       end do
     end do
     call izh_check(izh_run(b, f, dom, z, segs, nstep, current, recs), 'run')
+    ! the linear maps leave refptcl(6) alone
+    call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
     ! z after each step: the lattice's cumulative length
     z = 0.0d0
     k = 0
@@ -111,8 +136,8 @@ program izh_lattice
       do j = 1, nseg(i)
         k = k + 1
         z = z + elen(i) / nseg(i)
-        write(u, '(I8,1X,ES25.17,1X,I12,44(1X,ES25.17))') k, z, &
-            int(recs(45, k), c_int64_t), recs(1:44, k)
+        write(u, '(I8,1X,ES25.17,1X,I12,45(1X,ES25.17))') k, z, &
+            int(recs(45, k), c_int64_t), recs(1:44, k), -info%refpt(6)
       end do
     end do
   end if

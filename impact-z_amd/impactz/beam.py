@@ -189,10 +189,11 @@ def write_particle_in(path: str, pts: np.ndarray) -> None:
 
 def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
                lattice: list[Element], flagdiag: int = 1, rstartflg: int = 0,
-               flagdist: int = 19, distparam=None) -> None:
-    """An ``ImpactZ.in`` for Flagmap=1, Flagbc=1 (src/Contrl/Input.f90:62-314):
-    particles from particle.in (Flagdist=19) or a binary fort.N (199, the file
-    number in distparam[1]); rstartflg=1 restarts from a -7 checkpoint."""
+               flagdist: int = 19, distparam=None, flagmap: int = 1) -> None:
+    """An ``ImpactZ.in`` for Flagbc=1 (src/Contrl/Input.f90:62-314) and the
+    integrator ``flagmap`` (1 linear map, 2 nonlinear Lorentz): particles
+    from particle.in (Flagdist=19) or a binary fort.N (199, the file number in
+    distparam[1]); rstartflg=1 restarts from a -7 checkpoint."""
     nx, ny, nz = grid
     dp = list(distparam) if distparam is not None else [
         0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
@@ -201,7 +202,7 @@ def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
     qm = beam.charge / beam.mass
     lines = [
         "1 1",
-        f"6 {n} 1 0 {flagdiag}",
+        f"6 {n} {flagmap} 0 {flagdiag}",
         f"{nx} {ny} {nz} 1 1.0 1.0 1.0",
         f"{flagdist} {rstartflg} 0 1",
         f"{n}",

@@ -10,10 +10,13 @@ run_AccSimulator (src/Contrl/AccSimulator.f90:237-536, 742-1504) for the
 elements this build runs: drifts (type 0), quadrupoles (type 1: constant
 gradient with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270, or given by
 K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
-(type 2) and solenoids (type 3).  Zero-length
-diagnostic elements that do not change the particles (types -2 to -8) are
-skipped; any other element type is refused with an error, as are decks
-outside Flagmap=1, Flagbc=1, Flagdist=19.
+(type 2) and solenoids (type 3), under the linear map integrator (Flagmap=1)
+or the nonlinear Lorentz integrator (Flagmap=2); a type -25 element switches
+the integrator to its mapstp for the elements after it
+(src/Contrl/AccSimulator.f90:807-809).  Zero-length diagnostic elements that
+do not change the particles (types -3 to -6, -8) are skipped; -2 and -7
+write particle files; any other element type is refused with an error, as are
+decks outside Flagbc=1 and Flagdist=19/199.
 
 Output files are those of diagnostic1_Output (src/Contrl/Output.f90:341-352):
 fort.18, fort.24-27, fort.29 and fort.30, one row at z = 0 and one after each
@@ -167,14 +170,25 @@ class FileAction:
     value: float = 0.0
 
 
+@dataclass
+class MapSwitch:
+    """A type -25 element: Flagmap = its mapstp from here on
+    (src/Contrl/AccSimulator.f90:807-809)."""
+    index: int
+    flagmap: int
+
+
 def program_of(deck: Deck) -> tuple[list, list[str]]:
     """The deck's element table as the sequence this build runs: (deck index,
-    Element or FileAction) in deck order, and notes on the diagnostic
-    elements skipped.  Raises DeckError for anything the GPU path does not
-    implement."""
-    if deck.flagmap != 1:
-        raise DeckError(f"Flagmap={deck.flagmap}: only the linear map integrator "
-                        "(Flagmap=1) is on this path")
+    Element, FileAction or MapSwitch) in deck order, and notes on the
+    diagnostic elements skipped.  Raises DeckError for anything the GPU path
+    does not implement under the integrator in effect at that element."""
+    def integrator(flagmap, where):
+        if flagmap not in (1, 2):
+            raise DeckError(f"{where}Flagmap={flagmap}: the linear map (1) and nonlinear "
+                            "Lorentz (2) integrators are on this path")
+        return flagmap
+    flagmap = integrator(deck.flagmap, "")
     if deck.flagbc != 1:
         raise DeckError(f"Flagbc={deck.flagbc}: only open boundaries (Flagbc=1)")
     if deck.flagsbstp != 0:
@@ -187,20 +201,28 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
         if e.type == 0:
             out.append((k, Element(e.length, e.nseg, e.mapstp, 0, v[0])))
         elif e.type == 1:
-            if v[1] > 1.0e-5:
+            # map1 reads the field file when its ID > 1e-5; getfld_Quadrupole
+            # (Flagmap=2) when it is > 0 (src/Appl/Quadrupole.f90:379-400)
+            if v[1] > (1.0e-5 if flagmap == 1 else 0.0):
                 raise DeckError(f"element {k}: quadrupole with field file ID {v[1]:g} "
                                 "(fringe field from file) is not on this path")
-            if v[1] < -10.0:
+            if flagmap == 1 and v[1] < -10.0:
                 raise DeckError(f"element {k}: undulator map (file ID {v[1]:g}) is not "
                                 "on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, 1, v[0], v[1], v[2], v[3])))
         elif e.type in (2, 3):
             # drift1_BeamBunch takes its ideal-RF branch when Param(5) < 0
-            # (src/Appl/BeamBunch.f90:324-330)
-            if v[3] < 0.0:
+            # (src/Appl/BeamBunch.f90:324-330); Flagmap=2 does not call it
+            if flagmap == 1 and v[3] < 0.0:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {v[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3])))
+        elif e.type == -25:
+            if e.nseg != 0:
+                raise DeckError(f"element {k}: a type -25 element with {e.nseg} segments "
+                                "(zero-length space-charge steps) is not on this path")
+            flagmap = integrator(e.mapstp, f"element {k}: ")
+            out.append((k, MapSwitch(k, flagmap)))
         elif e.type in DIAGNOSTIC_TYPES and e.length == 0.0 and e.nseg == 0:
             if e.type in (-2, -7):
                 out.append((k, FileAction(k, e.type, e.mapstp, v[0])))
@@ -222,6 +244,8 @@ def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
         if isinstance(p, FileAction):
             notes.append(f"element {p.index}: type {p.type} "
                          f"({DIAGNOSTIC_TYPES[p.type]}) writes fort.{p.unit}")
+        elif isinstance(p, MapSwitch):
+            notes.append(f"element {p.index}: type -25 switches to Flagmap={p.flagmap}")
         else:
             out.append(p)
     return out, notes
@@ -274,7 +298,8 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
              outdir: str | None = None, nsteps: int | None = None) -> np.ndarray:
     """Run ``workdir/ImpactZ.in`` and write the diagnostic files into
     ``outdir`` (default: workdir).  fused=True runs each stretch of
-    space-charge segments between file-writing elements through izh_run.
+    linear-map (Flagmap=1) segments between file-writing elements through
+    izh_run; Flagmap=2 segments run step by step.
     Rstartflg=1 restarts from the -7 checkpoint fort.<mapstp of the last -7
     element> at the element after the one that wrote it, as
     run_AccSimulator does (src/Contrl/AccSimulator.f90:558-561,674-742).
@@ -297,7 +322,8 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
     else:
         pts = initial_particles(deck, workdir)
     elements = [p for _, p in prog if isinstance(p, Element)]
-    sim = Simulation(pts, deck.beam(), elements, deck.grid, precision=precision)
+    sim = Simulation(pts, deck.beam(), elements, deck.grid, precision=precision,
+                     flagmap=deck.flagmap)
     try:
         if ck is not None:
             # inpoint_Output: the particles, refptcl and z of the checkpoint;
@@ -312,7 +338,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
         def flush():
             if not pending:
                 return
-            if fused:
+            if fused and sim.flagmap == 1:
                 z = sim.z
                 recs = sim.run_elements([e for e, _ in pending])
                 for (e, tau1), rec in zip(pending, recs):
@@ -326,7 +352,14 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
 
         for index, p in prog:
             if index <= st["i"]:
-                continue      # a restart resumes after the checkpoint's element
+                # a restart resumes after the checkpoint's element, with the
+                # deck's Flagmap (src/Contrl/AccSimulator.f90:742)
+                continue
+            if isinstance(p, MapSwitch):
+                flush()
+                sim.flagmap = p.flagmap
+                st["i"], st["j"] = index, 1
+                continue
             if isinstance(p, FileAction):
                 flush()
                 _file_action(p, sim, out, st, deck)

@@ -161,6 +161,46 @@ def io_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
     return out
 
 
+def switch_lattice():
+    """Linear maps through a FODO half-period, a type -25 element switching
+    to the Lorentz integrator (Flagmap = its mapstp, 2), then constant
+    focusing, a solenoid with 10 mm field ramps and drifts."""
+    return [B.Element(0.1, 4, 10, 1, 10.0, 0.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0),
+            B.Element(0.0, 0, 2, -25, 0.0),
+            B.Element(0.1, 4, 10, 2, 30.0, 20.0, 5.0, 1.0),
+            B.Element(0.2, 4, 10, 0, 1.0),
+            B.Element(0.1, 4, 10, 3, 1.5, 0.0, 0.01),
+            B.Element(0.2, 4, 10, 0, 1.0)]
+
+
+def exe_nonlinear_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
+    """Runs of the reference executable (ImpactZexe built from source) with
+    the Lorentz integrator: a deck that starts with Flagmap=1 and switches
+    to Flagmap=2 at a -25 element, and a Flagmap=2 deck of the FODO period."""
+    pts = B.gaussian_bunch(n, seed=3, beam=beam)
+    tmp = tempfile.mkdtemp(prefix="impz_nl_")
+    out = {"pts_in": pts, "grid": np.array(grid),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0])}
+    for tag, lat, flagmap in (("sw", switch_lattice(), 1), ("nl", B.fodo(1), 2)):
+        d = os.path.join(tmp, tag)
+        os.makedirs(d)
+        B.write_particle_in(os.path.join(d, "particle.in"), pts)
+        B.write_deck(os.path.join(d, "ImpactZ.in"), n, grid, beam, lat, flagmap=flagmap)
+        _run_exe(d)
+        out[f"{tag}_flagmap"] = np.array(flagmap)
+        out[f"{tag}_lattice"] = np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2,
+                                           e.v3, e.v4] for e in lat])
+        for k in (18, 24, 25, 26):
+            out[f"{tag}_fort{k}"] = R.read_fort(os.path.join(d, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "exe_nonlinear.npz")
+    np.savez_compressed(path, **out)
+    print(f"exe_nonlinear: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
 def main():
     if not R.harness_available():
         raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
@@ -194,6 +234,10 @@ def main():
     io_fixture()
     meta["io_ckpt"] = ("ImpactZexe runs: -2 phase dump and -7 checkpoint after 8 steps, "
                        "the run restarted from that checkpoint, and Flagdist=199 binary input")
+    exe_nonlinear_fixture()
+    meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
+                             "switched to 2 by a -25 element before constant focusing and "
+                             "a solenoid, and a Flagmap=2 FODO period")
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump(meta, f, indent=1)
 

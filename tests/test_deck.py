@@ -99,7 +99,7 @@ def test_deck_refuses_elements_off_the_path(tmp_path, line, what):
 
 def test_deck_refuses_other_integrators_and_boundaries(tmp_path):
     from impactz.deck import DeckError, lattice_of, read_deck
-    for old, new, what in [("6 3 1 0 1", "6 3 2 0 1", "Flagmap=2"),
+    for old, new, what in [("6 3 1 0 1", "6 3 3 0 1", "Flagmap=3"),
                            ("16 16 32 1 0.02", "16 16 32 2 0.02", "Flagbc=2")]:
         p = tmp_path / "ImpactZ.in"
         _write(p, HEADER.replace(old, new) + "0.4 4 10 0 1.0 /\n")
@@ -148,3 +148,32 @@ def test_deck_linear_elements(tmp_path):
     assert lat[1].values[:3] == [1.5, 0.0, 0.03] and lat[1].radius == 0.03
     assert lat[2].values[:3] == [5.4, -1.0, 0.04] and lat[2].radius == 0.04
     assert notes == []
+
+
+def test_deck_integrator_switch(tmp_path):
+    """Flagmap=2 decks, the type -25 switch, and the per-integrator refusals:
+    a quadrupole field file ID in (0, 1e-5] and ideal-RF Param(5) < 0 are
+    refused or accepted according to the integrator in effect."""
+    from impactz.deck import DeckError, MapSwitch, program_of, read_deck
+    p = tmp_path / "ImpactZ.in"
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + """0.4 4 10 0 1.0 /
+0.1 4 10 1 10.0 -20.0 0.15 /
+0.0 0 1 -25 /
+0.1 4 10 2 1.0 2.0 3.0 1.0 /
+""")
+    prog, _ = program_of(read_deck(str(p)))
+    assert [type(x).__name__ for _, x in prog] == ["Element", "Element", "MapSwitch",
+                                                  "Element"]
+    assert prog[2][1] == MapSwitch(3, 1)
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.1 4 10 1 10.0 1e-6 0.15 /\n")
+    with pytest.raises(DeckError, match="file ID"):
+        program_of(read_deck(str(p)))
+    _write(p, HEADER + "0.0 0 2 -25 /\n0.1 4 10 2 1.0 2.0 3.0 -1.0 /\n0.0 0 1 -25 /\n"
+           "0.1 4 10 2 1.0 2.0 3.0 -1.0 /\n")
+    with pytest.raises(DeckError, match="element 4: type 2 .*ideal RF"):
+        program_of(read_deck(str(p)))
+    for line, what in (("0.0 0 3 -25 /", "element 1: Flagmap=3"),
+                       ("0.0 2 2 -25 /", "segments")):
+        _write(p, HEADER + line + "\n")
+        with pytest.raises(DeckError, match=what):
+            program_of(read_deck(str(p)))

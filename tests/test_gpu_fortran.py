@@ -26,6 +26,7 @@ def _namelist(path, d, fused, out):
             f"  current={bm.current!r}, kinetic={bm.kinetic!r}, mass={bm.mass!r},",
             f"  charge={bm.charge!r}, freq={bm.freq!r}, phase0={bm.phase0!r},",
             f"  nelem={len(lat)}, fused={'.true.' if fused else '.false.'},",
+            f"  flagmap={int(d['flagmap']) if 'flagmap' in d else 1},",
             f"  particles='{path}/particle.in', output='{out}',"]
     for k, e in enumerate(lat, start=1):
         vals = ", ".join(f"val({j},{k})={v!r}" for j, v in enumerate(e.values, start=1))
@@ -36,7 +37,8 @@ def _namelist(path, d, fused, out):
 
 
 @pytest.mark.parametrize("name,fused", [("fodo_traj", False), ("fodo_traj", True),
-                                        ("cf_sol", False), ("cf_sol", True)])
+                                        ("cf_sol", False), ("cf_sol", True),
+                                        ("nl_cf_sol", False)])
 def test_fortran_host_matches_reference(tmp_path, name, fused):
     from impactz import hip
     from impactz.beam import write_particle_in
@@ -51,12 +53,13 @@ def test_fortran_host_matches_reference(tmp_path, name, fused):
                    timeout=120)
     rec = np.loadtxt(out, ndmin=2)
     nsteps = int(d["nsteps"])
-    assert rec.shape == (nsteps, 47)
+    assert rec.shape == (nsteps, 48)
     bm = fixture_beam(d)
     c = hip.physconst(bm.freq)
-    refpt = np.zeros(6) - bm.gamma0
     worst = 0.0
     for k in range(nsteps):
+        refpt = np.zeros(6) - rec[k, 47]
+        assert rec[k, 47] == pytest.approx(d["fort18"][k + 1, 2], rel=1e-13)
         row = diag_row(rec[k, 3:40], rec[k, 40:46], rec[k, 46], int(rec[k, 2]), refpt,
                        bm.mass, c.scxl, c.pi)
         ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],

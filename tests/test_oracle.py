@@ -391,6 +391,38 @@ def test_oracle_nonlinear_trajectory_matches_reference(oracle, name):
     assert col_err(run.particles(), d["final"]) < 1e-12
 
 
+@pytest.mark.parametrize("tag", ["sw", "nl"])
+def test_oracle_matches_executable_lorentz_runs(oracle, tag):
+    """The reference executable's own runs (tests/golden/exe_nonlinear.npz):
+    Flagmap=1 switched to 2 by a -25 element, and a Flagmap=2 FODO period.
+    Pins the integrator switch and the zedge bookkeeping of the run loop."""
+    from impactz.beam import Element
+    d = load_golden("exe_nonlinear")
+    grid = tuple(int(v) for v in d["grid"])
+    lat = [Element(float(r[0]), int(r[1]), int(r[2]), int(r[3]),
+                   *(float(v) for v in r[4:8])) for r in d[f"{tag}_lattice"]]
+    run = oracle.Run(d["pts_in"], fixture_beam(d), [e for e in lat if e.type >= 0], grid,
+                     flagmap=int(d[f"{tag}_flagmap"]))
+    k = 0
+    for e in lat:
+        if e.type == -25:
+            run.flagmap = e.mapstp
+            continue
+        for _ in range(e.nseg):
+            run.step(e)
+            k += 1
+            row = run.diag()
+            for f, lo in ((24, 5), (25, 11), (26, 17)):
+                ref = d[f"{tag}_fort{f}"][k, 1:]
+                got = row[lo:lo + 6]
+                scale = np.abs(ref).copy()
+                scale[0] = max(scale[0], abs(ref[1]))
+                scale[2] = max(scale[2], abs(ref[3]))
+                assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+            assert run.st.z == pytest.approx(d[f"{tag}_fort24"][k, 0], rel=1e-14)
+    assert k == d[f"{tag}_fort24"].shape[0] - 1
+
+
 def test_drift2_equals_exact_drift(oracle):
     """drift2(2 tau) and the exact drift1(tau) are the same map."""
     from impactz.beam import Beam, gaussian_bunch
