@@ -405,21 +405,132 @@ int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info) {
 
 // ------------------------------------------------------- element maps
 
-// The linear ODE the element maps share: the reference particle's phase rate
+int izh_rfdata_table(const double *rows4, int n, izh_rfdata **out) {
+  if (!rows4 || n < 1 || !out) return fail(IZH_ERR_ARG, "izh_rfdata_table: bad argument");
+  auto *d = new izh_rfdata();
+  d->fourier = 0;
+  d->z.resize(n); d->ez.resize(n); d->ezp.resize(n); d->ezpp.resize(n);
+  // read1_Data shifts the positions so the table starts at 0
+  // (src/DataStruct/Data.f90:195-199)
+  const double z1 = rows4[0];
+  for (int i = 0; i < n; i++) {
+    d->z[i] = rows4[4 * i] - z1;
+    d->ez[i] = rows4[4 * i + 1];
+    d->ezp[i] = rows4[4 * i + 2];
+    d->ezpp[i] = rows4[4 * i + 3];
+  }
+  *out = d;
+  return IZH_OK;
+}
+
+int izh_rfdata_fourier(const double *coef, int n, izh_rfdata **out) {
+  if (!coef || n < 4 || !out) return fail(IZH_ERR_ARG, "izh_rfdata_fourier: bad argument");
+  const int ncoef = (int)(coef[0] + 0.1);
+  // getaxfldE_SolRF reads Fcoef(5 .. 4 + numpar1) (src/Appl/SolRF.f90:385-417)
+  if (ncoef < 1 || 4 + ncoef > n + (n % 2 == 0 ? 1 : 0))
+    return fail(IZH_ERR_ARG, "izh_rfdata_fourier: %d values hold fewer than the %d "
+                "coefficients the first value announces", n, ncoef);
+  auto *d = new izh_rfdata();
+  d->fourier = 1;
+  d->coef.assign(coef, coef + n);
+  // read3_Data pads to an odd count (src/DataStruct/Data.f90:402-406)
+  if (n % 2 == 0) d->coef.push_back(0.0);
+  *out = d;
+  return IZH_OK;
+}
+
+int izh_rfdata_destroy(izh_rfdata *d) {
+  delete d;
+  return IZH_OK;
+}
+
+// The linear ODE the element maps share: the reference particle's phase
 // (y[4]) and energy (y[5]), the cosine- and sine-like columns of three 2x2
-// systems q' = p/Scxl, p' = -s q (y[6..17]) and, for the solenoid, the 2x2
+// systems q' = p/Scxl, p' = -s q (y[6..17]) and, for the solenoids, the 2x2
 // rotation of the Larmor frame (y[18..21]).  Spring constants:
 //   quadrupole  s = (G/Brho, -G/Brho, 0)*Scxl  (src/Appl/Quadrupole.f90:283-291)
 //   ConstFoc    s = (kx0^2, ky0^2, kz0^2)*Scxl (src/Appl/ConstFoc.f90:276-281)
 //   solenoid    s = ((B/2Brho)^2, (B/2Brho)^2, 0)*Scxl, rotation rate
 //               -B/(2 Brho)                    (src/Appl/Sol.f90:298-318)
+//   RF cavity   the energy gain -qmcc Ez cos(phi) and the RF focusing of
+//               intfunc1_DTL/_CCL/_SolRF (src/Appl/DTL.f90:286-347,
+//               CCL.f90:269-328, SolRF.f90:287-353), plus the DTL's
+//               quadrupoles and SolRF's solenoid field.
 struct LinearElement {
-  int type;      // 1 quadrupole (constant gradient), 2 constant focusing, 3 solenoid
+  int type;      // 1, 2, 3 or 101-105
   double p[3];   // 1: gradient; 2: kx0^2, ky0^2, kz0^2; 3: Bz0
-  int nvar() const { return type == 3 ? 22 : 18; }
+  // RF cavities: Param(1..4), the element length, the DTL's quadrupole
+  // lengths and gradients (Param(7..10)), SolRF's Bz0 (Param(12))
+  double zedge, escale, ww, theta0, length;
+  double q[4];
+  double bz0;
+  const izh_rfdata *rf;
+  int nvar() const { return (type == 3 || type == 105) ? 22 : 18; }
 };
 
-static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc,
+// getaxfldE_DTL/_CCDTL/_CCL/_SC (src/Appl/DTL.f90:350-393): linear
+// interpolation in the table, bisection as the reference does it (and
+// extrapolation outside it); getaxfldE_SolRF (src/Appl/SolRF.f90:356-466):
+// the Fourier series inside the element and the field's range, else 0.
+static void rf_axis_field(const izh_const *c, const LinearElement &e, double z, double &ez1,
+                          double &ezp1) {
+  const izh_rfdata &d = *e.rf;
+  if (!d.fourier) {
+    const double zz = z - e.zedge;
+    int klo = 1, khi = (int)d.z.size();
+    while (khi - klo > 1) {
+      const int k = (khi + klo) / 2;
+      if (d.z[k - 1] > zz) khi = k; else klo = k;
+    }
+    const double hstep = d.z[khi - 1] - d.z[klo - 1];
+    double slope = (d.ez[khi - 1] - d.ez[klo - 1]) / hstep;
+    ez1 = d.ez[klo - 1] + slope * (zz - d.z[klo - 1]);
+    slope = (d.ezp[khi - 1] - d.ezp[klo - 1]) / hstep;
+    ezp1 = d.ezp[klo - 1] + slope * (zz - d.z[klo - 1]);
+    ez1 = ez1 * e.escale;
+    ezp1 = ezp1 * e.escale;
+    return;
+  }
+  const double epstol = (double)1.0e-10f;
+  const double *F = d.coef.data() - 1;   // 1-based, as Fcoef
+  ez1 = 0.0;
+  ezp1 = 0.0;
+  if ((z >= e.zedge && z <= e.zedge + e.length) || std::fabs(z - e.zedge) <= epstol ||
+      std::fabs(z - e.zedge - e.length) <= epstol) {
+    const double zlc = z - e.zedge;
+    const int numpar1 = (int)(F[1] + 0.1);
+    const double zstart1 = F[2], zend1 = F[3], zlength1 = F[4];
+    if ((zlc >= zstart1 && zlc <= zend1) || std::fabs(zlc - zstart1) <= epstol ||
+        std::fabs(zlc - zend1) <= epstol) {
+      const double zmid = zlength1 / 2;
+      const double zz = z - e.zedge - zstart1 - zmid;
+      const double zlen = zlength1;
+      const int ntmp = 4;
+      ez1 = F[ntmp + 1] / 2;
+      for (int i = 2; i <= (numpar1 - 1) / 2 + 1; i++) {
+        const double arg = (i - 1) * 2 * c->pi * zz / zlen;
+        const double cs = std::cos(arg), sn = std::sin(arg);
+        ez1 = ez1 + F[2 * i - 2 + ntmp] * cs + F[2 * i - 1 + ntmp] * sn;
+        ezp1 = ezp1 + (i - 1) * 2 * c->pi / zlen *
+                          (-F[2 * i - 2 + ntmp] * sn + F[2 * i - 1 + ntmp] * cs);
+      }
+      ez1 = ez1 * e.escale;
+      ezp1 = ezp1 * e.escale;
+    }
+  }
+}
+
+// getBgradfld_DTL (src/Appl/DTL.f90:395-418): the drift-tube quadrupoles'
+// gradient at z.
+static double dtl_gradient(const LinearElement &e, double z) {
+  const double len1 = e.q[0], bgrad1 = e.q[1], len2 = e.length - e.q[2], bgrad2 = e.q[3];
+  const double zz = z - e.zedge;
+  if (zz < len1) return bgrad1;
+  if (zz < len2) return 0.0;
+  return bgrad2;
+}
+
+static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc, double t,
                        const double *y, double *f) {
   const double gamma0 = -y[5];
   const double beta0 = std::sqrt(gamma0 * gamma0 - 1.0) / gamma0;
@@ -435,12 +546,44 @@ static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc,
     s[2] = 0.0;
   } else if (e.type == 2) {
     for (int a = 0; a < 3; a++) s[a] = e.p[a] * c->scxl;
-  } else {
+  } else if (e.type == 3) {
     const double brho = gbet / c->clight / qmcc;
     const double k = 0.5 * e.p[0] / brho;
     s[0] = s[1] = (k * k) * c->scxl;
     s[2] = 0.0;
     rot = -e.p[0] / brho / 2;
+  } else {
+    // the RF cavities (intfunc1_DTL, src/Appl/DTL.f90:286-347, and alike)
+    double ez1, ezp1;
+    rf_axis_field(c, e, t, ez1, ezp1);
+    const double sinphi = std::sin(e.ww * y[4] + e.theta0);
+    const double cosphi = std::cos(e.ww * y[4] + e.theta0);
+    const double rfdsgn = ez1 * cosphi;
+    f[5] = -qmcc * rfdsgn;
+    const double b2 = beta0 * beta0, g2 = gamma0 * gamma0;
+    const double acc = qmcc * rfdsgn / b2 / g2;
+    const double s11tmp = 0.5 * (1.0 + 0.5 * g2) * (acc * acc) +
+                          qmcc * 0.5 / (b2 * beta0) / (g2 * gamma0) * ez1 * sinphi * e.ww /
+                              c->scxl;
+    double kx = 0.0, ky = 0.0;
+    if (e.type == 101) {
+      const double brho = gbet / c->clight / qmcc;
+      const double bgrad = dtl_gradient(e, t);
+      kx = bgrad / brho;
+      ky = -bgrad / brho;
+    } else if (e.type == 105) {
+      const double brho = gbet / c->clight / qmcc;
+      const double kb = 0.5 * e.bz0 / brho;
+      kx = ky = (kb * kb) / 2;
+      rot = -e.bz0 / brho / 2;
+    }
+    s[0] = (s11tmp + kx) * c->scxl;
+    s[1] = (s11tmp + ky) * c->scxl;
+    const double s55 = -1.5 * qmcc / b2 / gamma0 * ezp1 * cosphi +
+                       (b2 + 0.5) / (b2 * beta0) / gamma0 * qmcc * ez1 * sinphi * e.ww /
+                           c->scxl +
+                       1.5 * (1.0 - 0.5 * g2) * (acc * acc);
+    s[2] = c->scxl * s55;
   }
   for (int blk = 0; blk < 3; blk++) {
     for (int col = 0; col < 2; col++) {
@@ -449,7 +592,7 @@ static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc,
       f[o + 1] = -s[blk] * y[o];
     }
   }
-  if (e.type == 3) {
+  if (e.nvar() == 22) {
     f[18] = -rot * y[20];
     f[19] = -rot * y[21];
     f[20] = rot * y[18];
@@ -457,12 +600,30 @@ static void linear_rhs(const izh_const *c, const LinearElement &e, double qmcc,
   }
 }
 
-// maplinear_Quadrupole / _ConstFoc / _Sol with rk6i_* (src/Appl/Quadrupole.f90:
-// 120-252, ConstFoc.f90:116-243, Sol.f90:120-268): an 8-stage 6th-order
-// Runge-Kutta over mapstp steps, then the 6x6 map with the energy scaling
-// of each block (dlt = thl = 0).
-static void linear_map(const izh_const *c, const LinearElement &e, double tau, int mapstp,
-                       double refpt[6], double charge, double mass, double xm[36]) {
+// The entrance/exit terms of the RF maps (maplinear_DTL, src/Appl/DTL.f90:
+// 189-220): dlt and thl from the on-axis field and its slope at z.
+static void rf_edge(const izh_const *c, const LinearElement &e, double qmcc, double z,
+                    const double refpt[6], double u, double &dlt, double &thl) {
+  double ez, ez1;
+  rf_axis_field(c, e, z, ez, ez1);
+  const double g = -refpt[5];
+  const double beta = std::sqrt(g * g - 1.0) / g;
+  const double sinth = std::sin(e.ww * refpt[4] + e.theta0);
+  const double costh = std::cos(e.ww * refpt[4] + e.theta0);
+  const double uprim = qmcc * ez / beta * costh;
+  const double qpw = 0.5 * qmcc * c->scxl / (u * e.ww);
+  dlt = c->scxl * (0.5 * uprim / u - qpw * ez1 * sinth);
+  thl = 1.5 * c->scxl * (uprim / u);
+}
+
+// maplinear_Quadrupole / _ConstFoc / _Sol / the RF cavities with their rk6i_*
+// (src/Appl/Quadrupole.f90:120-252, ConstFoc.f90:116-243, Sol.f90:120-268,
+// DTL.f90:136-284, SolRF.f90:126-285): an 8-stage 6th-order Runge-Kutta over
+// mapstp steps of the ODE in z from t, then the 6x6 map with the energy
+// scaling of each block (dlt = thl = 0 outside the RF cavities).
+static void linear_map(const izh_const *c, const LinearElement &e, double t, double tau,
+                       int mapstp, double refpt[6], double charge, double mass,
+                       double xm[36]) {
   static const double kA[7][7] = {
       {1.0 / 9.0},
       {1.0 / 24.0, 3.0 / 24.0},
@@ -478,6 +639,7 @@ static void linear_map(const izh_const *c, const LinearElement &e, double tau, i
                                272.0 / 840.0, 27.0 / 840.0, 216.0 / 840.0,
                                41.0 / 840.0};
   constexpr int kMaxVar = 22;
+  const bool rf = is_rf_cavity(e.type);
   const int nv = e.nvar();
   const double qmcc = charge / mass;
   double y[kMaxVar] = {0};
@@ -489,15 +651,21 @@ static void linear_map(const izh_const *c, const LinearElement &e, double tau, i
   const double gi = -refpt[5];
   const double ui = gi * (std::sqrt(gi * gi - 1.0) / gi);
   const double squi = std::sqrt(ui), sqi3 = squi * squi * squi;
+  double dlti = 0.0, thli = 0.0;
+  if (rf) rf_edge(c, e, qmcc, t, refpt, ui, dlti, thli);
   double k[8][kMaxVar] = {{0}}, yt[kMaxVar] = {0};
+  // stage positions of rk6i_* (t + h/9, h/6, h/3, h/2, 2h/3, 5h/6, h)
   for (int step = 0; step < mapstp; step++) {
+    const double ts = t + step * h;
+    const double tt[8] = {ts, ts + h / 9.0, ts + h / 6.0, ts + h / 3.0, ts + 0.5 * h,
+                          ts + 2.0 * h / 3.0, ts + 5.0 * h / 6.0, ts + h};
     for (int st = 0; st < 8; st++) {
       for (int v = 0; v < nv; v++) {
         double acc = 0.0;
         for (int q = 0; q < st; q++) acc += kA[st - 1][q] * k[q][v];
         yt[v] = y[v] + acc;
       }
-      linear_rhs(c, e, qmcc, st == 0 ? y : yt, k[st]);
+      linear_rhs(c, e, qmcc, tt[st], st == 0 ? y : yt, k[st]);
       for (int v = 0; v < nv; v++) k[st][v] *= h;
     }
     for (int v = 0; v < nv; v++) {
@@ -511,63 +679,78 @@ static void linear_map(const izh_const *c, const LinearElement &e, double tau, i
   const double gf = -refpt[5];
   const double uf = gf * (std::sqrt(gf * gf - 1.0) / gf);
   const double squf = std::sqrt(uf), sqf3 = squf * squf * squf;
+  double dltf = 0.0, thlf = 0.0;
+  if (rf) rf_edge(c, e, qmcc, t + tau, refpt, uf, dltf, thlf);
   for (int q = 0; q < 36; q++) xm[q] = 0.0;
-  // one 2x2 block from ODE columns o..o+3, scaled by w (the solenoid's
-  // rotation entry; 1 otherwise), at rows r and columns cc
-  auto block = [&](int r, int cc, int o, double si, double sf, double w, bool scaled) {
-    if (scaled) {
-      xm[6 * r + cc] = w * (y[o] * si / sf);
-      xm[6 * (r + 1) + cc] = w * (y[o + 1] * si * sf);
-      xm[6 * r + cc + 1] = w * y[o + 2] / (si * sf);
-      xm[6 * (r + 1) + cc + 1] = w * y[o + 3] * sf / si;
-    } else {
-      xm[6 * r + cc] = y[o] * si / sf;
-      xm[6 * (r + 1) + cc] = y[o + 1] * si * sf;
-      xm[6 * r + cc + 1] = y[o + 2] / (si * sf);
-      xm[6 * (r + 1) + cc + 1] = y[o + 3] * sf / si;
-    }
+  // one 2x2 block from ODE columns o..o+3 with entrance/exit terms di, df,
+  // times w (the solenoids' rotation entry; 1 otherwise), at rows r and
+  // columns cc (src/Appl/DTL.f90:205-228)
+  auto block = [&](int r, int cc, int o, double si, double sf, double di, double df,
+                   double w) {
+    xm[6 * r + cc] = w * (y[o] * si / sf + y[o + 2] * si / sf * di);
+    xm[6 * (r + 1) + cc] =
+        w * ((y[o + 1] - y[o] * df) * si * sf + (y[o + 3] - y[o + 2] * df) * si * sf * di);
+    xm[6 * r + cc + 1] = w * y[o + 2] / (si * sf);
+    xm[6 * (r + 1) + cc + 1] = w * (y[o + 3] - y[o + 2] * df) * sf / si;
   };
-  if (e.type == 3) {
-    // src/Appl/Sol.f90:187-207: the x and y blocks times the rotation
-    block(0, 0, 6, squi, squf, y[18], true);
-    block(0, 2, 6, squi, squf, y[19], true);
-    block(2, 0, 10, squi, squf, y[20], true);
-    block(2, 2, 10, squi, squf, y[21], true);
+  if (e.nvar() == 22) {
+    // src/Appl/Sol.f90:187-207, SolRF.f90:207-226: the x and y blocks times
+    // the rotation
+    block(0, 0, 6, squi, squf, dlti, dltf, y[18]);
+    block(0, 2, 6, squi, squf, dlti, dltf, y[19]);
+    block(2, 0, 10, squi, squf, dlti, dltf, y[20]);
+    block(2, 2, 10, squi, squf, dlti, dltf, y[21]);
   } else {
-    block(0, 0, 6, squi, squf, 1.0, false);
-    block(2, 2, 10, squi, squf, 1.0, false);
+    block(0, 0, 6, squi, squf, dlti, dltf, 1.0);
+    block(2, 2, 10, squi, squf, dlti, dltf, 1.0);
   }
   // the longitudinal block scales with the cube (src/Appl/Quadrupole.f90:191-196)
-  block(4, 4, 14, sqi3, sqf3, 1.0, false);
+  block(4, 4, 14, sqi3, sqf3, thli, thlf, 1.0);
 }
 
-// The linear element of a deck line's values, or an error message.
-static const char *linear_element(int bitype, const double v[4], LinearElement &e) {
+// The linear element of an izh_element, or an error message.
+static const char *linear_element(const izh_const *c, const izh_element &el,
+                                  LinearElement &e) {
   e = LinearElement{};
-  e.type = bitype;
-  if (bitype == 1) {
+  e.type = el.bitype;
+  const double *v = el.v;
+  if (el.bitype == 1) {
     // a field profile from a file (file ID > 1e-5) is not on this path
     if (v[1] > 1.0e-5) return "quadrupole fringe field from a file";
     e.p[0] = v[0];
-  } else if (bitype == 2) {
+  } else if (el.bitype == 2) {
     for (int a = 0; a < 3; a++) e.p[a] = v[a];
-  } else if (bitype == 3) {
+  } else if (el.bitype == 3) {
     e.p[0] = v[0];
+  } else if (is_rf_cavity(el.bitype)) {
+    if (!el.rf) return "an RF cavity needs its field (izh_rfdata)";
+    if ((el.bitype == 105) != (el.rf->fourier != 0))
+      return el.bitype == 105 ? "SolRF takes Fourier coefficients (izh_rfdata_fourier)"
+                              : "this RF cavity takes a field table (izh_rfdata_table)";
+    if (!(v[1] > 0.0)) return "RF frequency must be positive";
+    e.rf = el.rf;
+    e.zedge = el.zedge;
+    e.length = el.length;
+    e.escale = v[0];
+    e.ww = v[1] / c->scfreq;
+    // Param(4)*asin(1.0)/90: a REAL(4) asin (src/Appl/DTL.f90:158)
+    e.theta0 = v[2] * (double)std::asin(1.0f) / 90;
+    for (int a = 0; a < 4; a++) e.q[a] = v[5 + a];   // DTL: Param(7..10)
+    e.bz0 = v[10];                                  // SolRF: Param(12)
   } else {
     return "no linear map for this element type";
   }
   return nullptr;
 }
 
-int izh_element_maplinear(const izh_const *c, double tau, int bitype, const double v[4],
-                          int mapstp, double refpt[6], double charge, double mass,
-                          double xm[36]) {
-  if (!c || !v || !refpt || !xm || mapstp < 1 || !(mass > 0))
+int izh_element_maplinear(const izh_const *c, double z, double tau, const izh_element *e,
+                          double refpt[6], double charge, double mass, double xm[36]) {
+  if (!c || !e || !refpt || !xm || e->mapstp < 1 || !(mass > 0))
     return fail(IZH_ERR_ARG, "izh_element_maplinear: bad argument");
-  LinearElement e;
-  if (const char *why = linear_element(bitype, v, e))
-    return fail(IZH_ERR_ARG, "izh_element_maplinear: bitype %d: %s", bitype, why);
-  linear_map(c, e, tau, mapstp, refpt, charge, mass, xm);
+  LinearElement le;
+  if (const char *why = linear_element(c, *e, le))
+    return fail(IZH_ERR_ARG, "izh_element_maplinear: bitype %d: %s", e->bitype, why);
+  linear_map(c, le, z, tau, e->mapstp, refpt, charge, mass, xm);
   return IZH_OK;
 }
 
@@ -576,20 +759,22 @@ int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
                        double xm[36]) {
   if (!c || !refpt || !xm || mapstp < 1 || !(mass > 0))
     return fail(IZH_ERR_ARG, "izh_quad_maplinear: bad argument");
-  const double v[4] = {gradient, 0.0, 0.0, 0.0};
-  return izh_element_maplinear(c, tau, 1, v, mapstp, refpt, charge, mass, xm);
+  izh_element e{};
+  e.bitype = 1;
+  e.mapstp = mapstp;
+  e.v[0] = gradient;
+  return izh_element_maplinear(c, 0.0, tau, &e, refpt, charge, mass, xm);
 }
 
 // ------------------------------------------------------------- map push
 
-int izh_map1(izh_bunch *b, double *z, double tau, int bitype, const double v[4],
-             int mapstp) {
-  if (!b || !z || !v) return fail(IZH_ERR_ARG, "izh_map1: bad argument");
+int izh_map1(izh_bunch *b, double *z, double tau, const izh_element *e) {
+  if (!b || !z || !e) return fail(IZH_ERR_ARG, "izh_map1: bad argument");
   if (b->frame != kFrameZ)
     return fail(IZH_ERR_STATE, "izh_map1: bunch is not in the z frame");
   const char *why = nullptr;
-  const int kind = map1_kind(bitype, v, &why);
-  if (kind < 0) return fail(IZH_ERR_ARG, "izh_map1: bitype %d: %s", bitype, why);
+  const int kind = map1_kind(e->bitype, e->v, &why);
+  if (kind < 0) return fail(IZH_ERR_ARG, "izh_map1: bitype %d: %s", e->bitype, why);
   if (kind == kMap1Drift) {
     // src/Appl/BeamBunch.f90:276-298
     const double gam0 = -b->refpt[5];
@@ -601,12 +786,11 @@ int izh_map1(izh_bunch *b, double *z, double tau, int bitype, const double v[4],
     const double r6 = b->refpt[5];
     const double gambet0 = std::sqrt(r6 * r6 - 1.0);
     const double beta0 = std::sqrt(1.0 - 1.0 / (r6 * r6));
-    IZH_CHECK(launch_quad_kmode(b, tau, v[0], b->charge / b->mass, gambet0, beta0));
+    IZH_CHECK(launch_quad_kmode(b, tau, e->v[0], b->charge / b->mass, gambet0, beta0));
     b->refpt[4] = b->refpt[4] + tau / beta0 / b->c.scxl;
   } else {
     double xm[36];
-    IZH_CHECK(izh_element_maplinear(&b->c, tau, bitype, v, mapstp, b->refpt, b->charge,
-                                    b->mass, xm));
+    IZH_CHECK(izh_element_maplinear(&b->c, *z, tau, e, b->refpt, b->charge, b->mass, xm));
     IZH_CHECK(launch_map_linear(b, xm));
   }
   *z = *z + tau;
@@ -618,8 +802,11 @@ int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
   if (bitype != 0 && bitype != 1)
     return fail(IZH_ERR_ARG, "izh_drift1: bitype %d not supported (0 drift, 1 quad; "
                 "izh_map1 takes the other elements)", bitype);
-  const double v[4] = {gradient, 0.0, 0.0, 0.0};
-  return izh_map1(b, z, tau, bitype, v, mapstp);
+  izh_element e{};
+  e.bitype = bitype;
+  e.mapstp = mapstp;
+  e.v[0] = gradient;
+  return izh_map1(b, z, tau, &e);
 }
 
 int izh_map_linear(izh_bunch *b, const double xm[36]) {
@@ -973,7 +1160,7 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   if (!b || !f || !dom || !z || !seg) return fail(IZH_ERR_ARG, "izh_step: bad argument");
   const double tau1 = seg->tau1, tau2 = 2.0 * tau1;
   // src/Contrl/AccSimulator.f90:1106-1110 first half push
-  IZH_CHECK(izh_map1(b, z, tau1, seg->bitype, seg->v, seg->mapstp));
+  IZH_CHECK(izh_map1(b, z, tau1, &seg->e));
   // an empty bunch has nothing to deposit or kick (the reference would still
   // build a domain from its +-1e20 initial range; nothing reads it)
   if (bcurr >= (double)1.0e-10f && b->n > 0) {
@@ -986,7 +1173,7 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     IZH_CHECK(izh_cvbkforth1st(b));                                         // :1399
     IZH_CHECK(izh_kick1(b, tau2, f, dom, 1));                               // :1425
   }
-  IZH_CHECK(izh_map1(b, z, tau1, seg->bitype, seg->v, seg->mapstp));       // :1429
+  IZH_CHECK(izh_map1(b, z, tau1, &seg->e));                                // :1429
   if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
   return IZH_OK;
 }
@@ -1005,11 +1192,11 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   for (int64_t s = 0; s < nseg; s++) {
     const izh_segment &sg = segs[s];
     const char *why = nullptr;
-    const int kind = map1_kind(sg.bitype, sg.v, &why);
+    const int kind = map1_kind(sg.e.bitype, sg.e.v, &why);
     if (kind < 0)
       return fail(IZH_ERR_ARG, "izh_run: segment %lld: bitype %d: %s", (long long)s,
-                  sg.bitype, why);
-    if (!(sg.tau1 >= 0) || (kind == kMap1Linear && sg.mapstp < 1))
+                  sg.e.bitype, why);
+    if (!(sg.tau1 >= 0) || (kind == kMap1Linear && sg.e.mapstp < 1))
       return fail(IZH_ERR_ARG, "izh_run: segment %lld: bad step or map steps", (long long)s);
     if (kind == kMap1QuadK && bcurr >= (double)1.0e-10f)
       return fail(IZH_ERR_ARG, "izh_run: segment %lld is a K-mode quadrupole (chromatic "
@@ -1019,9 +1206,13 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   // no space charge or nothing to carry it: the reference loop is maps only
   if (bcurr < (double)1.0e-10f || b->n == 0) {
     for (int64_t s = 0; s < nseg; s++) {
-      IZH_CHECK(izh_step(b, f, dom, z, &segs[s], bcurr,
-                         moments ? moments + s * kRunRecord : nullptr));
-      if (moments) moments[s * kRunRecord + kMoments] = (double)b->npt;
+      double *rec = moments ? moments + s * IZH_RUN_RECORD : nullptr;
+      IZH_CHECK(izh_step(b, f, dom, z, &segs[s], bcurr, rec));
+      if (rec) {
+        rec[kMoments] = (double)b->npt;
+        rec[kMoments + 1] = b->refpt[4];
+        rec[kMoments + 2] = b->refpt[5];
+      }
     }
     return IZH_OK;
   }

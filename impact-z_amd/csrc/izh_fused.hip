@@ -62,6 +62,7 @@ struct PassCfg {
   int nx, ny, nz;
   int tshift[3], tnx, tny;
   double gam, xl, rad;      // -refpt(6), Scxl, pipe radius of step s
+  double gam_k;             // -refpt(6) of step s-1 (its kick; RF cavities change it)
   double tau2_prev;         // 2*tau1 of step s-1
   double inv_rho_scale;     // undo the grid-charge scaling in the kick
   HalfPush pb, pa;          // second half of step s-1, first half of step s
@@ -380,9 +381,9 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
         // Computed on every lane (a lane without a particle holds zeros, which
         // give an in-range node), so no branch separates the slot's loads from
         // their first use; only valid lanes keep the result.
-        const double rcp = 1.0 / (-pt + c.gam);
+        const double rcp = 1.0 / (-pt + c.gam_k);
         const double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
-        const double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam * betai * ph * c.xl;
+        const double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam_k * betai * ph * c.xl;
         int ix = (int)((X0 - xmin) * hxi + 1.0);
         const double ab = ((xmin - X0) + ix * hx) * hxi;
         int jx = (int)((Y0 - ymin) * hyi + 1.0);
@@ -410,11 +411,11 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
               },
               o, sy, sz, (T)ab, (T)cd, (T)ef, gx, gy, gz);
         }
-        const double exn = (double)gx * c.gam, eyn = (double)gy * c.gam, ezn = (double)gz;
+        const double exn = (double)gx * c.gam_k, eyn = (double)gy * c.gam_k, ezn = (double)gz;
         if (valid) {
           x = X0 * xk;
           y = Y0 * xk;
-          ph = Z0 * xk / (-c.gam * betai);
+          ph = Z0 * xk / (-c.gam_k * betai);
           px = px + txy * exn;
           py = py + txy * eyn;
           pt = pt - ttc * ezn;
@@ -856,7 +857,7 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
                         double *z) {
   p = HalfPush{};
   const char *why = nullptr;
-  const int m1 = map1_kind(sg.bitype, sg.v, &why);
+  const int m1 = map1_kind(sg.e.bitype, sg.e.v, &why);
   if (m1 == kMap1Drift) {
     const double gam0 = -b->refpt[5];
     const double beta0 = std::sqrt(gam0 * gam0 - 1.0) / gam0;
@@ -867,8 +868,8 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
     b->refpt[4] = b->refpt[4] + sg.tau1 / (b->c.scxl * beta0);
   } else if (m1 == kMap1Linear) {
     double xm[36];
-    IZH_CHECK(izh_element_maplinear(&b->c, sg.tau1, sg.bitype, sg.v, sg.mapstp, b->refpt,
-                                    b->charge, b->mass, xm));
+    IZH_CHECK(izh_element_maplinear(&b->c, *z, sg.tau1, &sg.e, b->refpt, b->charge,
+                                    b->mass, xm));
     // the longitudinal plane is never coupled to the transverse ones
     for (int j = 0; j < 6; j++)
       for (int k = 0; k < 6; k++)
@@ -897,7 +898,7 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
       }
     }
   } else {
-    return fail(IZH_ERR_ARG, "izh_run: bitype %d: %s", sg.bitype,
+    return fail(IZH_ERR_ARG, "izh_run: bitype %d: %s", sg.e.bitype,
                 why ? why : "the K-mode quadrupole runs through izh_step");
   }
   *z = *z + sg.tau1;
@@ -996,6 +997,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   pc.tshift[0] = s0; pc.tshift[1] = s1; pc.tshift[2] = s2;
   pc.tnx = tnx; pc.tny = tny;
   pc.gam = gam;
+  pc.gam_k = gam;
   pc.xl = b->c.scxl;
   pc.inv_rho_scale = 1.0 / rho_scale;
   pc.tpos = b->tpos;
@@ -1004,11 +1006,19 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   HalfPush pb_prev{};
   int kb_prev = kPushNone;
   double tau2_prev = 0.0;
+  double gam_sc = gam;   // -refpt(6) during the space-charge stages of step s-1
+  std::vector<double> refpt_after(2 * (size_t)nseg);  // refpt(5:6) after each step
   for (int64_t s = 0; s <= nseg && rc == IZH_OK; s++) {
     const bool last = s == nseg;
     HalfPush pa{};
     int ka = kPushNone;
     if (!last) IZH_CHECK(segment_push(b, segs[s], pa, ka, z));  // first half of step s
+    // the kick of step s-1 and the z->t transform of step s each use the
+    // reference energy of their own step (after its first half push)
+    pc.gam_k = gam_sc;
+    pc.gam = -b->refpt[5];
+    gam_sc = pc.gam;
+    fc.gam = pc.gam;
     pc.in = b->r[0];
     pc.out = b->r_alt[0];
     pc.in_orig = b->orig;
@@ -1057,6 +1067,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     if (rc == IZH_OK) rc = launch_efield(f);
     // second half of step s: host bookkeeping now, applied by the next pass
     if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, kb_prev, z);
+    refpt_after[2 * s] = b->refpt[4];
+    refpt_after[2 * s + 1] = b->refpt[5];
     tau2_prev = 2.0 * segs[s].tau1;
   }
   f->stream = fstream;
@@ -1064,10 +1076,17 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   // ---- results back to the host
   IZH_HIP(hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(DevScal), hipMemcpyDeviceToHost, st));
   IZH_HIP(hipMemcpyAsync(f->h_dom, f->d_dom, sizeof(DevDom), hipMemcpyDeviceToHost, st));
+  std::vector<double> recs(moments ? (size_t)nseg * kRunRecord : 0);
   if (moments && nseg > 0)
-    IZH_HIP(hipMemcpyAsync(moments, b->d_moments, nseg * kRunRecord * sizeof(double),
+    IZH_HIP(hipMemcpyAsync(recs.data(), b->d_moments, nseg * kRunRecord * sizeof(double),
                            hipMemcpyDeviceToHost, st));
   IZH_HIP(hipStreamSynchronize(st));
+  for (int64_t s = 0; moments && s < nseg; s++) {
+    double *o = moments + s * IZH_RUN_RECORD;
+    std::memcpy(o, recs.data() + s * kRunRecord, kRunRecord * sizeof(double));
+    o[kRunRecord] = refpt_after[2 * s];
+    o[kRunRecord + 1] = refpt_after[2 * s + 1];
+  }
   const DevScal out = *b->h_scal;
   if (out.err != 0)
     return fail(IZH_ERR_STATE, "izh_run: %lld particles had no sort slot",

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "impactz_hip.h"
 
@@ -34,7 +35,8 @@ int fail(int code, const char *fmt, ...);
 constexpr int kBlock = 256;          // threads per block for particle passes
 constexpr int kMaxRedBlocks = 2048;  // fixed partial count => deterministic sums
 constexpr int kMoments = 44;         // 37 sums + 6 abs-max + r^2 max
-constexpr int kRunRecord = kMoments + 1;  // + Npt, per step of izh_run
+constexpr int kRunRecord = kMoments + 1;  // + Npt, per step of izh_run (device)
+static_assert(IZH_RUN_RECORD == kRunRecord + 2, "izh_run record: + refpt(5:6)");
 constexpr int kConvRed = 8;          // 3 min, 3 max, lost, alive
 constexpr int kPartStride = kMoments + kConvRed;
 constexpr int kChunk = 16384;        // particles per deposit chunk
@@ -199,7 +201,8 @@ int launch_quad_kmode(izh_bunch *b, double tau, double kq, double qmass, double 
 // 276-449): kMap1Drift, kMap1Linear (a host 6x6 map) or kMap1QuadK; -1 with
 // *why set for the branches not on this path.
 enum { kMap1Drift = 0, kMap1Linear = 1, kMap1QuadK = 2 };
-inline int map1_kind(int bitype, const double v[4], const char **why) {
+inline bool is_rf_cavity(int bitype) { return bitype >= 101 && bitype <= 105; }
+inline int map1_kind(int bitype, const double v[IZH_NVAL], const char **why) {
   *why = nullptr;
   if (bitype == 0) return kMap1Drift;
   if (bitype == 1) {
@@ -209,15 +212,28 @@ inline int map1_kind(int bitype, const double v[4], const char **why) {
     if (v[1] > 1.0e-5) { *why = "quadrupole fringe field from a file"; return -1; }
     return kMap1Linear;
   }
-  if (bitype == 2 || bitype == 3) {
+  if (bitype == 2 || bitype == 3 || is_rf_cavity(bitype)) {
     // Param(5) < 0 selects the ideal RF cavity model (:324-330)
     if (v[3] < 0.0) { *why = "Param(5) < 0 (ideal RF cavity branch)"; return -1; }
+    // SolRF's wakefield (Param(15) > 0) turns on kick1wake
+    // (src/Contrl/AccSimulator.f90:859-870)
+    if (bitype == 105 && v[13] > 0.0) { *why = "SolRF wakefield (Param(15) > 0)"; return -1; }
     return kMap1Linear;
   }
   *why = "element type not on this path (0 drift, 1 quadrupole, 2 constant "
-         "focusing, 3 solenoid)";
+         "focusing, 3 solenoid, 101-105 RF cavities)";
   return -1;
 }
+
+}  // namespace izh
+// The on-axis field of the Data module (src/DataStruct/Data.f90:41-61):
+// read1_Data's table (z from 0, Ez, Ez', Ez'') or read3_Data's coefficients.
+struct izh_rfdata {
+  int fourier;                 // 0: table (read1_Data), 1: coefficients (read3_Data)
+  std::vector<double> z, ez, ezp, ezpp;
+  std::vector<double> coef;    // Fcoef(1..Ndata)
+};
+namespace izh {
 
 // ---------------------------------------- nonlinear integrator (Flagmap=2)
 

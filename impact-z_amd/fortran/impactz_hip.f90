@@ -40,24 +40,29 @@ module impactz_hip
     real(c_double) :: msize(3)
   end type izh_compdom
 
-  ! one segment of a lattice: the element's half step, type, map steps,
-  ! values v1..v4 (its Param(2:5)) and pipe aperture
-  type, bind(C) :: izh_segment
-    real(c_double) :: tau1
-    integer(c_int) :: bitype
-    integer(c_int) :: mapstp
-    real(c_double) :: v(4)
-    real(c_double) :: xrad, yrad
-  end type izh_segment
+  ! IZH_NVAL: the deck line's values v1..v14 (Param(2:15))
+  integer, parameter :: izh_nval = 14
+  ! IZH_RUN_RECORD: values per step from izh_run
+  integer, parameter :: izh_run_record = 47
 
-  ! an element for the Lorentz integrator (Flagmap=2): type, values v1..v4,
-  ! length and entrance z
+  ! a lattice element: type, RK6 steps, values, length, entrance z (its
+  ! Param(1)) and an RF cavity's on-axis field (izh_rfdata, else c_null_ptr)
   type, bind(C) :: izh_element
     integer(c_int) :: bitype
-    real(c_double) :: v(4)
+    integer(c_int) :: mapstp
+    real(c_double) :: v(izh_nval)
     real(c_double) :: length
     real(c_double) :: zedge
+    type(c_ptr) :: rf
   end type izh_element
+
+  ! one segment of a lattice: the element's half step, the element and the
+  ! pipe aperture
+  type, bind(C) :: izh_segment
+    real(c_double) :: tau1
+    type(izh_element) :: e
+    real(c_double) :: xrad, yrad
+  end type izh_segment
 
   interface
     function izh_last_error() bind(C, name="izh_last_error") result(p)
@@ -130,27 +135,43 @@ module impactz_hip
       real(c_double) :: xm(36)
       integer(c_int) :: rc
     end function
-    function izh_element_maplinear(c, tau, bitype, v, mapstp, refpt, charge, mass, xm) &
+    function izh_element_maplinear(c, z, tau, e, refpt, charge, mass, xm) &
         bind(C, name="izh_element_maplinear") result(rc)
-      import :: c_int, c_double, izh_const
+      import :: c_int, c_double, izh_const, izh_element
       type(izh_const), intent(in) :: c
-      real(c_double), value :: tau
-      integer(c_int), value :: bitype
-      real(c_double), intent(in) :: v(4)
-      integer(c_int), value :: mapstp
+      real(c_double), value :: z, tau
+      type(izh_element), intent(in) :: e
       real(c_double) :: refpt(6)
       real(c_double), value :: charge, mass
       real(c_double) :: xm(36)
       integer(c_int) :: rc
     end function
-    function izh_map1(b, z, tau, bitype, v, mapstp) bind(C, name="izh_map1") result(rc)
-      import :: c_int, c_double, c_ptr
+    function izh_map1(b, z, tau, e) bind(C, name="izh_map1") result(rc)
+      import :: c_int, c_double, c_ptr, izh_element
       type(c_ptr), value :: b
       real(c_double) :: z
       real(c_double), value :: tau
-      integer(c_int), value :: bitype
-      real(c_double), intent(in) :: v(4)
-      integer(c_int), value :: mapstp
+      type(izh_element), intent(in) :: e
+      integer(c_int) :: rc
+    end function
+    ! read1_Data's table (rows z, Ez, Ez', Ez'') / read3_Data's coefficients
+    function izh_rfdata_table(rows4, n, d) bind(C, name="izh_rfdata_table") result(rc)
+      import :: c_int, c_double, c_ptr
+      real(c_double), intent(in) :: rows4(*)
+      integer(c_int), value :: n
+      type(c_ptr) :: d
+      integer(c_int) :: rc
+    end function
+    function izh_rfdata_fourier(coef, n, d) bind(C, name="izh_rfdata_fourier") result(rc)
+      import :: c_int, c_double, c_ptr
+      real(c_double), intent(in) :: coef(*)
+      integer(c_int), value :: n
+      type(c_ptr) :: d
+      integer(c_int) :: rc
+    end function
+    function izh_rfdata_destroy(d) bind(C, name="izh_rfdata_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: d
       integer(c_int) :: rc
     end function
     function izh_drift2(b, z, tau) bind(C, name="izh_drift2") result(rc)
@@ -353,16 +374,30 @@ contains
     stop 1
   end subroutine izh_check
 
+  ! The izh_element of a lattice element: type, values v1..v14 (Param(2:15)),
+  ! mapstp, length, entrance z and an RF cavity's field (c_null_ptr if none).
+  function element_gpu(bitype, v, mapstp, elen, zedge, rf) result(e)
+    integer, intent(in) :: bitype, mapstp
+    real(c_double), intent(in) :: v(izh_nval), elen, zedge
+    type(c_ptr), intent(in) :: rf
+    type(izh_element) :: e
+    e%bitype = int(bitype, c_int)
+    e%mapstp = int(mapstp, c_int)
+    e%v = v
+    e%length = elen
+    e%zedge = zedge
+    e%rf = rf
+  end function element_gpu
+
   ! drift1_BeamBunch(this,beamln,z,tau,bitype,nseg,nst,ihlf)
-  ! (src/Appl/BeamBunch.f90:250-258): the element's data enter as bitype,
-  ! its values v1..v4 (Param(2:5)) and mapstp.
-  subroutine drift1_gpu(b, z, tau, bitype, v, mapstp)
+  ! (src/Appl/BeamBunch.f90:250-258): the element's data enter as an
+  ! izh_element (element_gpu).
+  subroutine drift1_gpu(b, z, tau, e)
     type(c_ptr), intent(in) :: b
     real(c_double), intent(inout) :: z
-    real(c_double), intent(in) :: tau, v(4)
-    integer, intent(in) :: bitype, mapstp
-    call izh_check(izh_map1(b, z, tau, int(bitype, c_int), v, int(mapstp, c_int)), &
-                   'drift1')
+    real(c_double), intent(in) :: tau
+    type(izh_element), intent(in) :: e
+    call izh_check(izh_map1(b, z, tau, e), 'drift1')
   end subroutine drift1_gpu
 
   ! conv1st_BeamBunch(this,tau,nplc,nptot,ptrange,Flagbc,perd,xrad,yrad)
@@ -427,18 +462,14 @@ contains
   end subroutine drift2_gpu
 
   ! kick2_BeamBunch(this,beamelem,z,tau,...) (src/Appl/BeamBunch.f90:
-  ! 2187-2508): space-charge field plus the element's field, Lorentz push;
-  ! zedge is the z of the element's entrance
-  subroutine kick2_gpu(b, z, tau, bitype, v, elen, zedge, f, dom, flagbc)
+  ! 2187-2508): space-charge field plus the element's field (e, with its
+  ! entrance zedge), Lorentz push
+  subroutine kick2_gpu(b, z, tau, e, f, dom, flagbc)
     type(c_ptr), intent(in) :: b, f
-    real(c_double), intent(in) :: z, tau, v(4), elen, zedge
-    integer, intent(in) :: bitype, flagbc
+    real(c_double), intent(in) :: z, tau
+    type(izh_element), intent(in) :: e
+    integer, intent(in) :: flagbc
     type(izh_compdom), intent(in) :: dom
-    type(izh_element) :: e
-    e%bitype = int(bitype, c_int)
-    e%v = v
-    e%length = elen
-    e%zedge = zedge
     call izh_check(izh_kick2(b, z, tau, e, f, dom, int(flagbc, c_int)), 'kick2')
   end subroutine kick2_gpu
 

@@ -11,8 +11,11 @@
 ! With fused = .true. the Flagmap=1 segments go through izh_run in one call.
 !
 ! Input: particle.in (readin_Dist format, src/Appl/Distribution.f90:1677-1705)
-! and the namelist file named by the first command-line argument; val(1:4,i)
-! holds element i's deck values v1..v4 (its Param(2:5)).  Output:
+! and the namelist file named by the first command-line argument; val(1:14,i)
+! holds element i's deck values v1..v14 (its Param(2:15)).  An RF cavity
+! (bitype 101-105) reads rfdataN.in, N = its file ID v4, as read1_Data
+! (rows z Ez Ez' Ez'', bitypes 101-104) or read3_Data (one coefficient per
+! line, SolRF) do (src/DataStruct/Data.f90:139-215,348-409).  Output:
 ! one line per step with z, Npt, the 44 diagnostic1 reductions (37 sums,
 ! 6 absolute maxima, max x^2+y^2) and the reference particle's gamma
 ! (-refptcl(6), which the Lorentz integrator changes in a longitudinal field)
@@ -27,7 +30,7 @@ program izh_lattice
   real(8) :: current = 0.1d0, kinetic = 150.0d6, mass = 938.272088d6
   real(8) :: charge = 1.0d0, freq = 324.0d6, phase0 = 0.0d0
   integer :: nelem = 0
-  real(8) :: elen(maxelem) = 0, val(4, maxelem) = 0, radius(maxelem) = 1
+  real(8) :: elen(maxelem) = 0, val(izh_nval, maxelem) = 0, radius(maxelem) = 1
   integer :: nseg(maxelem) = 1, mapstp(maxelem) = 1, bitype(maxelem) = 0
   logical :: fused = .false.
   integer :: flagmap = 1
@@ -43,6 +46,8 @@ program izh_lattice
   type(izh_compdom) :: dom
   type(izh_bunch_info) :: info
   real(c_double) :: z, zedge, tau1, tau2, ptrange(6), sums(37), amax(6), r2max
+  type(c_ptr), allocatable :: rfh(:)
+  type(izh_element) :: e
   integer(c_int64_t) :: n, nplc, nptot, nstep, k
   integer :: i, j, u, istep
 
@@ -65,6 +70,11 @@ program izh_lattice
                                   phase0, int(precision, c_int), b), 'bunch_create')
   call izh_check(izh_field_create(int(nx, c_int), int(ny, c_int), int(nz, c_int), &
                                   int(precision, c_int), f), 'field_create')
+  allocate(rfh(nelem))
+  do i = 1, nelem
+    rfh(i) = c_null_ptr
+    if (bitype(i) >= 101 .and. bitype(i) <= 105) call read_rfdata(i)
+  end do
   dom%nx = nx; dom%ny = ny; dom%nz = nz
   dom%range = 0; dom%msize = 0
   z = 0.0d0
@@ -84,9 +94,10 @@ program izh_lattice
       tau1 = 0.5d0 * elen(i) / nseg(i)
       tau2 = 2.0d0 * tau1
       zedge = z
+      e = element_gpu(bitype(i), val(:, i), mapstp(i), elen(i), zedge, rfh(i))
       do j = 1, nseg(i)
         if (flagmap == 1) then
-          call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+          call drift1_gpu(b, z, tau1, e)
         else
           call drift2_gpu(b, z, tau2)
         end if
@@ -99,9 +110,9 @@ program izh_lattice
         end if
         if (flagmap == 1) then
           if (current >= 1.0e-10) call kick1_gpu(b, tau2, f, dom, 1)
-          call drift1_gpu(b, z, tau1, bitype(i), val(:, i), mapstp(i))
+          call drift1_gpu(b, z, tau1, e)
         else
-          call kick2_gpu(b, z, tau2, bitype(i), val(:, i), elen(i), zedge, f, dom, 1)
+          call kick2_gpu(b, z, tau2, e, f, dom, 1)
           call drift2_gpu(b, z, tau2)
         end if
         call diagnostic1_sums_gpu(b, sums, amax, r2max)
@@ -113,22 +124,22 @@ program izh_lattice
     end do
   else
     nstep = sum(nseg(1:nelem))
-    allocate(segs(nstep), recs(45, nstep))
+    allocate(segs(nstep), recs(izh_run_record, nstep))
     k = 0
+    zedge = 0.0d0
     do i = 1, nelem
+      tau1 = 0.5d0 * elen(i) / nseg(i)
+      e = element_gpu(bitype(i), val(:, i), mapstp(i), elen(i), zedge, rfh(i))
       do j = 1, nseg(i)
         k = k + 1
-        segs(k)%tau1 = 0.5d0 * elen(i) / nseg(i)
-        segs(k)%bitype = int(bitype(i), c_int)
-        segs(k)%v = val(:, i)
-        segs(k)%mapstp = int(mapstp(i), c_int)
+        segs(k)%tau1 = tau1
+        segs(k)%e = e
         segs(k)%xrad = radius(i)
         segs(k)%yrad = radius(i)
+        zedge = (zedge + tau1) + tau1
       end do
     end do
     call izh_check(izh_run(b, f, dom, z, segs, nstep, current, recs), 'run')
-    ! the linear maps leave refptcl(6) alone
-    call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
     ! z after each step: the lattice's cumulative length
     z = 0.0d0
     k = 0
@@ -137,7 +148,7 @@ program izh_lattice
         k = k + 1
         z = z + elen(i) / nseg(i)
         write(u, '(I8,1X,ES25.17,1X,I12,45(1X,ES25.17))') k, z, &
-            int(recs(45, k), c_int64_t), recs(1:44, k), -info%refpt(6)
+            int(recs(45, k), c_int64_t), recs(1:44, k), -recs(47, k)
       end do
     end do
   end if
@@ -145,4 +156,41 @@ program izh_lattice
 
   call izh_check(izh_bunch_destroy(b), 'bunch_destroy')
   call izh_check(izh_field_destroy(f), 'field_destroy')
+  do i = 1, nelem
+    if (c_associated(rfh(i))) call izh_check(izh_rfdata_destroy(rfh(i)), 'rfdata_destroy')
+  end do
+
+contains
+
+  ! rfdataN.in of element ie (N = its file ID v4) into an izh_rfdata
+  subroutine read_rfdata(ie)
+    integer, intent(in) :: ie
+    character(len=32) :: fname
+    real(c_double), allocatable :: buf(:)
+    real(c_double) :: row(4)
+    integer :: uu, ios, nrow, ncol, id, kr
+    id = int(val(4, ie) + 0.1d0)
+    write(fname, '(a,i0,a)') 'rfdata', id, '.in'
+    ncol = 4
+    if (bitype(ie) == 105) ncol = 1
+    open(newunit=uu, file=trim(fname), status='old')
+    nrow = 0
+    do
+      read(uu, *, iostat=ios) row(1:ncol)
+      if (ios /= 0) exit
+      nrow = nrow + 1
+    end do
+    rewind(uu)
+    allocate(buf(ncol * nrow))
+    do kr = 1, nrow
+      read(uu, *) buf(ncol * (kr - 1) + 1:ncol * kr)
+    end do
+    close(uu)
+    if (ncol == 4) then
+      call izh_check(izh_rfdata_table(buf, int(nrow, c_int), rfh(ie)), 'rfdata_table')
+    else
+      call izh_check(izh_rfdata_fourier(buf, int(nrow, c_int), rfh(ie)), 'rfdata_fourier')
+    end if
+    deallocate(buf)
+  end subroutine read_rfdata
 end program izh_lattice

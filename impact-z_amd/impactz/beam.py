@@ -21,6 +21,7 @@ reference through ``particle.in`` (Flagdist=19, src/Appl/Distribution.f90:1666).
 from __future__ import annotations
 
 import dataclasses
+import os
 import math
 
 import numpy as np
@@ -58,6 +59,9 @@ class Beam:
         return CLIGHT / (2 * (2 * math.asin(1.0)) * self.freq)
 
 
+NVAL = 14   # deck values v1..v14 of an element line
+
+
 @dataclasses.dataclass(frozen=True)
 class Element:
     """One lattice line ``L nseg mapstp type v1 v2 v3 v4 /`` (src/Contrl/Input.f90:307-314;
@@ -69,6 +73,15 @@ class Element:
     type 2: constant focusing, v1..v3 = kx0^2, ky0^2, kz0^2 [1/m^2],
             v4 = pipe radius.
     type 3: solenoid, v1 = Bz0 [T], v2 = file ID, v3 = pipe radius.
+    types 101-104 (DTL, CCDTL, CCL, SC): v1 = field scale, v2 = RF frequency
+            [Hz], v3 = phase [deg], v4 = file ID, v5 = pipe radius; the DTL's
+            v6..v9 are its quadrupoles' lengths and gradients.
+    type 105 (SolRF): v1..v5 as above, v6..v10 misalignments, v11 = Bz0 [T],
+            v12..v14 wakefield.
+
+    ``more`` holds v5..v14; ``field`` the RF cavity's rfdata file contents:
+    an (n, 4) table (types 101-104, read1_Data) or the coefficient vector
+    (type 105, read3_Data).
     """
 
     length: float
@@ -79,15 +92,21 @@ class Element:
     v2: float = 0.0
     v3: float = 0.0
     v4: float = 0.0
+    more: tuple = ()
+    field: object = dataclasses.field(default=None, compare=False, repr=False)
 
     @property
     def radius(self) -> float:
         # getradius_BeamLineElem (src/Appl/BeamLineElem.f90:500-552)
+        if 101 <= self.type <= 105:
+            return self.values[4]
         return {0: self.v1, 2: self.v4}.get(self.type, self.v3)
 
     @property
     def values(self) -> list[float]:
-        return [self.v1, self.v2, self.v3, self.v4]
+        """v1..v14 (Param(2:15)), zero-padded."""
+        out = [self.v1, self.v2, self.v3, self.v4, *(float(x) for x in self.more)]
+        return (out + [0.0] * NVAL)[:NVAL]
 
 
 def fodo(periods: int, gradient: float = 10.0, quad_len: float = 0.1,
@@ -218,7 +237,27 @@ def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
         if e.type == 0:
             lines.append(f"{e.length!r} {e.nseg} {e.mapstp} 0 {e.v1!r} /")
         else:
-            lines.append(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} {e.v1!r} "
-                         f"{e.v2!r} {e.v3!r} {e.v4!r} 0.0 0.0 0.0 0.0 /")
+            vals = e.values if e.more else e.values[:8]
+            lines.append(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} " +
+                         " ".join(repr(float(v)) for v in vals) + " /")
+        if e.field is not None:
+            write_rfdata(os.path.join(os.path.dirname(path) or ".",
+                                      rfdata_name(int(e.v4 + 0.1))), e.field)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def rfdata_name(file_id: int) -> str:
+    """rfdataN.in, N = the element's file ID (src/DataStruct/Data.f90:139-182)."""
+    if not 1 <= file_id <= 999:
+        raise ValueError(f"file ID {file_id} outside 1..999")
+    return f"rfdata{file_id}.in"
+
+
+def write_rfdata(path: str, field) -> None:
+    """An rfdata file: the rows z Ez Ez' Ez'' of a table (n x 4) or one
+    Fourier-series value per line."""
+    a = np.asarray(field, dtype=np.float64)
+    with open(path, "w") as f:
+        for row in (a if a.ndim == 2 else a[:, None]):
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")

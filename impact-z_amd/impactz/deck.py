@@ -11,7 +11,9 @@ elements this build runs: drifts (type 0), quadrupoles (type 1: constant
 gradient with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270, or given by
 K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
 (type 2) and solenoids (type 3), under the linear map integrator (Flagmap=1)
-or the nonlinear Lorentz integrator (Flagmap=2); a type -25 element switches
+or the nonlinear Lorentz integrator (Flagmap=2), and the RF cavities DTL,
+CCDTL, CCL, SC and SolRF (types 101-105) with their rfdataN.in fields under
+Flagmap=1; a type -25 element switches
 the integrator to its mapstp for the elements after it
 (src/Contrl/AccSimulator.f90:807-809).  Zero-length diagnostic elements that
 do not change the particles (types -3 to -6, -8) are skipped; -2 and -7
@@ -25,6 +27,7 @@ space-charge step, values separated by blanks.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import os
 from dataclasses import dataclass, field
 
@@ -32,7 +35,7 @@ import numpy as np
 
 from . import hip
 from . import particle_io as PIO
-from .beam import Beam, Element
+from .beam import Beam, Element, rfdata_name
 from .sim import Simulation
 
 # Zero-length elements that only write files in the reference
@@ -217,6 +220,19 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {v[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3])))
+        elif 101 <= e.type <= 105:
+            vv = (v + [0.0] * 14)[:14]
+            if flagmap != 1:
+                raise DeckError(f"element {k}: RF cavity type {e.type} under Flagmap=2 "
+                                "(field maps from files) is not on this path")
+            if vv[3] < 0.0:
+                raise DeckError(f"element {k}: type {e.type} with Param(5) = {vv[3]:g} < 0 "
+                                "(ideal RF cavity model) is not on this path")
+            if e.type == 105 and vv[13] > 0.0:
+                raise DeckError(f"element {k}: SolRF wakefield (Param(15) = {vv[13]:g}) "
+                                "is not on this path")
+            out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
+                                   more=tuple(vv[4:]))))
         elif e.type == -25:
             if e.nseg != 0:
                 raise DeckError(f"element {k}: a type -25 element with {e.nseg} segments "
@@ -231,7 +247,8 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
                              "skipped: it writes files only")
         else:
             raise DeckError(f"element {k}: type {e.type} is not on this path "
-                            "(drift 0, quadrupole 1, constant focusing 2 and solenoid 3 are)")
+                            "(drift 0, quadrupole 1, constant focusing 2, solenoid 3 and the "
+                            "RF cavities 101-105 are)")
     return out, notes
 
 
@@ -281,6 +298,44 @@ def write_fort(outdir: str, rows: np.ndarray) -> None:
                 f.write(" ".join(repr(float(x)) for x in vals) + "\n")
 
 
+def read_rfdata(path: str, per_record: int) -> np.ndarray:
+    """An rfdataN.in as read1_Data (per_record = 4: z, Ez, Ez', Ez'') or
+    read3_Data (per_record = 1) read it (src/DataStruct/Data.f90:183-192,
+    387-392): each list-directed READ takes per_record values, continuing onto
+    following lines if needed and dropping the rest of its last line."""
+    rows, cur = [], []
+    with open(path) as f:
+        for line in f:
+            toks = line.replace("d", "e").replace("D", "e").split()
+            for t in toks:
+                cur.append(float(t))
+                if len(cur) == per_record:
+                    rows.append(cur)
+                    cur = []
+                    break
+    if not rows:
+        raise DeckError(f"{path}: no field data")
+    a = np.array(rows, dtype=np.float64)
+    return a if per_record > 1 else a.ravel()
+
+
+def with_fields(prog, workdir: str):
+    """The program with each RF cavity's rfdataN.in (N = its file ID v4)
+    attached, read once per file and reader as run_AccSimulator does
+    (src/Contrl/AccSimulator.f90:818-849)."""
+    cache, out = {}, []
+    for index, p in prog:
+        if isinstance(p, Element) and 101 <= p.type <= 105:
+            fid = int(p.v4 + 0.1)
+            key = (fid, p.type == 105)
+            if key not in cache:
+                cache[key] = read_rfdata(os.path.join(workdir, rfdata_name(fid)),
+                                         1 if p.type == 105 else 4)
+            p = dataclasses.replace(p, field=cache[key])
+        out.append((index, p))
+    return out
+
+
 def initial_particles(deck: Deck, workdir: str) -> np.ndarray:
     """The initial particles: particle.in (Flagdist=19, readin_Dist) or the
     binary fort.N of readPar_Dist (Flagdist=199, N = distparam(2))."""
@@ -306,6 +361,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
     Returns the diagnostic rows."""
     deck = read_deck(os.path.join(workdir, "ImpactZ.in"))
     prog, notes = program_of(deck)
+    prog = with_fields(prog, workdir)
     for n in notes:
         print(f"[impactz.deck] {n}", flush=True)
     out = outdir or workdir

@@ -33,7 +33,8 @@ SYMBOLS = (
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
     "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
-    "izh_map1", "izh_drift2", "izh_kick2", "izh_step2",
+    "izh_map1", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
+    "izh_rfdata_fourier", "izh_rfdata_destroy",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -57,45 +58,53 @@ class BunchInfo(C.Structure):
                 ("freq", C.c_double), ("precision", C.c_int)]
 
 
-class Segment(C.Structure):
-    """izh_segment: an element's half step tau1, type, map steps, values
-    v1..v4 (the deck line's, i.e. Param(2:5)) and pipe aperture."""
-    _fields_ = [("tau1", C.c_double), ("bitype", C.c_int), ("mapstp", C.c_int),
-                ("v", C.c_double * 4), ("xrad", C.c_double), ("yrad", C.c_double)]
+NVAL = 14          # IZH_NVAL: deck values v1..v14 of an element
+RUN_RECORD = 47    # IZH_RUN_RECORD: 44 diagnostic values, Npt, refpt(5:6)
 
 
 class ElementC(C.Structure):
-    """izh_element: type, values v1..v4, length and entrance z of an element
-    for the Lorentz integrator."""
-    _fields_ = [("bitype", C.c_int), ("v", C.c_double * 4), ("length", C.c_double),
-                ("zedge", C.c_double)]
+    """izh_element: type, RK6 steps, values v1..v14 (the deck line's, i.e.
+    Param(2:15)), length, entrance z and the RF cavity's field."""
+    _fields_ = [("bitype", C.c_int), ("mapstp", C.c_int), ("v", C.c_double * NVAL),
+                ("length", C.c_double), ("zedge", C.c_double), ("rf", C.c_void_p)]
 
 
-def element(bitype: int, v, length: float, zedge: float) -> ElementC:
-    e = ElementC()
-    e.bitype = bitype
-    for k, x in enumerate(values4(v)):
-        e.v[k] = x
-    e.length, e.zedge = length, zedge
-    return e
+class Segment(C.Structure):
+    """izh_segment: an element's half step tau1, the element and the pipe
+    aperture."""
+    _fields_ = [("tau1", C.c_double), ("e", ElementC), ("xrad", C.c_double),
+                ("yrad", C.c_double)]
 
 
-def values4(v) -> np.ndarray:
-    """The four element values izh_map1 takes (a lone number: a gradient)."""
-    out = np.zeros(4)
+def values14(v) -> np.ndarray:
+    """The element values the library takes (a lone number: a gradient)."""
+    out = np.zeros(NVAL)
     if np.isscalar(v):
         out[0] = float(v)
     else:
-        vals = np.asarray(v, dtype=np.float64).ravel()[:4]
+        vals = np.asarray(v, dtype=np.float64).ravel()[:NVAL]
         out[: vals.size] = vals
     return out
 
 
-def segment(tau1: float, bitype: int, v, mapstp: int, xrad: float, yrad: float) -> Segment:
+def element(bitype: int, v, length: float = 0.0, zedge: float = 0.0, mapstp: int = 1,
+            rf: "RFData | None" = None) -> ElementC:
+    e = ElementC()
+    e.bitype, e.mapstp = bitype, mapstp
+    for k, x in enumerate(values14(v)):
+        e.v[k] = x
+    e.length, e.zedge = length, zedge
+    e.rf = rf.h if rf is not None else None
+    e._rf_keep = rf   # the handle lives as long as the element
+    return e
+
+
+def segment(tau1: float, bitype: int, v, mapstp: int, xrad: float, yrad: float,
+            length: float = 0.0, zedge: float = 0.0, rf: "RFData | None" = None) -> Segment:
     sg = Segment()
-    sg.tau1, sg.bitype, sg.mapstp = tau1, bitype, mapstp
-    for k, x in enumerate(values4(v)):
-        sg.v[k] = x
+    sg.tau1 = tau1
+    sg.e = element(bitype, v, length, zedge, mapstp, rf)
+    sg._rf_keep = rf
     sg.xrad, sg.yrad = xrad, yrad
     return sg
 
@@ -131,9 +140,13 @@ def lib() -> C.CDLL:
     L.izh_quad_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_double,
                                      C.c_int, _dp, C.c_double, C.c_double, _dp]
     L.izh_drift1.argtypes = [vp, _dp, C.c_double, C.c_int, C.c_double, C.c_int]
-    L.izh_element_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_int, _dp,
-                                        C.c_int, _dp, C.c_double, C.c_double, _dp]
-    L.izh_map1.argtypes = [vp, _dp, C.c_double, C.c_int, _dp, C.c_int]
+    L.izh_element_maplinear.argtypes = [C.POINTER(Const), C.c_double, C.c_double,
+                                        C.POINTER(ElementC), _dp, C.c_double, C.c_double,
+                                        _dp]
+    L.izh_map1.argtypes = [vp, _dp, C.c_double, C.POINTER(ElementC)]
+    L.izh_rfdata_table.argtypes = [_dp, C.c_int, C.POINTER(vp)]
+    L.izh_rfdata_fourier.argtypes = [_dp, C.c_int, C.POINTER(vp)]
+    L.izh_rfdata_destroy.argtypes = [vp]
     L.izh_drift2.argtypes = [vp, _dp, C.c_double]
     L.izh_kick2.argtypes = [vp, C.c_double, C.c_double, C.POINTER(ElementC), vp,
                             C.POINTER(CompDomC), C.c_int]
@@ -213,15 +226,54 @@ def quad_maplinear(c: Const, tau, gradient, mapstp, refpt, charge, mass):
     return xm.reshape(6, 6), r
 
 
-def element_maplinear(c: Const, tau, bitype, v, mapstp, refpt, charge, mass):
+def element_maplinear(c: Const, tau, bitype, v, mapstp, refpt, charge, mass, z=0.0,
+                      length=0.0, zedge=0.0, rf: "RFData | None" = None):
     """maplinear_BeamLineElem (src/Appl/BeamLineElem.f90:776-825) for bitypes
-    1-3; host only.  Returns (xm 6x6, refpt after)."""
+    1-3 and the RF cavities 101-105 from z; host only.  Returns (xm 6x6,
+    refpt after)."""
     r = np.ascontiguousarray(refpt, dtype=np.float64).copy()
     xm = np.zeros(36)
-    vv = values4(v)
-    check(lib().izh_element_maplinear(C.byref(c), tau, bitype, _p(vv), mapstp, _p(r),
-                                      charge, mass, _p(xm)))
+    e = element(bitype, v, length, zedge, mapstp, rf)
+    check(lib().izh_element_maplinear(C.byref(c), z, tau, C.byref(e), _p(r), charge, mass,
+                                      _p(xm)))
     return xm.reshape(6, 6), r
+
+
+class RFData:
+    """izh_rfdata: an RF cavity's on-axis field as the Data module holds it.
+    ``field`` is an rfdata file's contents: an (n, 4) table of z, Ez, Ez',
+    Ez'' (read1_Data, src/DataStruct/Data.f90:139-215) or, with
+    fourier=True, the coefficient vector (read3_Data, :348-409)."""
+
+    def __init__(self, field, fourier: bool):
+        a = np.ascontiguousarray(field, dtype=np.float64)
+        h = C.c_void_p()
+        if fourier:
+            a = a.ravel().copy()
+            check(lib().izh_rfdata_fourier(_p(a), a.size, C.byref(h)))
+        else:
+            a = a.reshape(-1, 4).copy()
+            check(lib().izh_rfdata_table(_p(a), a.shape[0], C.byref(h)))
+        self.h = h.value
+        self.fourier = fourier
+
+    @classmethod
+    def for_element(cls, e) -> "RFData | None":
+        """The field of a beam.Element (None when it has none)."""
+        if e.field is None:
+            return None
+        return cls(e.field, fourier=e.type == 105)
+
+    def close(self):
+        if self.h:
+            lib().izh_rfdata_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class CompDom:
@@ -364,12 +416,13 @@ class BeamBunch:
         check(lib().izh_drift1(self.h, C.byref(zz), tau, bitype, gradient, mapstp))
         return zz.value
 
-    def map1(self, z: float, tau: float, bitype: int, v, mapstp: int = 1) -> float:
+    def map1(self, z: float, tau: float, bitype: int, v, mapstp: int = 1,
+             length: float = 0.0, zedge: float = 0.0, rf: "RFData | None" = None) -> float:
         """drift1_BeamBunch for every element izh_map1 takes; v = the element's
-        values v1..v4.  Returns z."""
+        values v1..v14, zedge its entrance, rf an RF cavity's field.  Returns z."""
         zz = C.c_double(z)
-        vv = values4(v)
-        check(lib().izh_map1(self.h, C.byref(zz), tau, bitype, _p(vv), mapstp))
+        e = element(bitype, v, length, zedge, mapstp, rf)
+        check(lib().izh_map1(self.h, C.byref(zz), tau, C.byref(e)))
         return zz.value
 
     def drift2(self, z: float, tau: float) -> float:
@@ -435,14 +488,14 @@ class BeamBunch:
         """izh_run: consecutive segments as one device-resident pipeline.
 
         segments: iterable of Segment (see segment()).
-        Returns (z, records[nseg, 45] or None): per step the 44 diagnostic
-        values of izh_step, then Npt."""
+        Returns (z, records[nseg, 47] or None): per step the 44 diagnostic
+        values of izh_step, then Npt, refpt(5) and refpt(6)."""
         segs = list(segments)
         arr = (Segment * max(len(segs), 1))()
         for k, sg in enumerate(segs):
             arr[k] = sg if isinstance(sg, Segment) else segment(*sg)
         zz = C.c_double(z)
-        mom = np.zeros((len(segs), 45)) if want_moments else None
+        mom = np.zeros((len(segs), RUN_RECORD)) if want_moments else None
         check(lib().izh_run(self.h, field.h, C.byref(dom.c), C.byref(zz), arr,
                             len(segs), bcurr, _p(mom) if want_moments else None))
         return zz.value, mom

@@ -64,8 +64,9 @@ class Simulation:
                  grid: tuple[int, int, int], precision: int = hip.FP64, flagmap: int = 1):
         self.beam = beam
         self.flagmap = flagmap
-        self._left = 0      # segments left in the current element (Flagmap=2)
+        self._left = 0      # segments left in the current element
         self.zedge = 0.0    # z of the current element's entrance
+        self._rf = {}       # RF cavity fields on the device library's side
         self.lattice = lattice
         self.grid = tuple(grid)
         self.const = hip.physconst(beam.freq)
@@ -80,29 +81,46 @@ class Simulation:
             for _ in range(e.nseg):
                 yield e
 
+    def rfdata(self, e: Element):
+        """The device library's copy of an RF cavity's field (one per field
+        array), or None."""
+        if e.field is None:
+            return None
+        key = id(e.field)
+        if key not in self._rf:
+            self._rf[key] = (hip.RFData.for_element(e), e.field)
+        return self._rf[key][0]
+
+    def _enter(self, e: Element) -> float:
+        """The zedge of the next segment of e: z at the element's entrance
+        (src/Contrl/AccSimulator.f90:1025-1026)."""
+        if self._left == 0:
+            self._left, self.zedge = e.nseg, self.z
+        self._left -= 1
+        return self.zedge
+
     def step(self, e: Element, want_moments: bool = True):
+        zedge = self._enter(e)
         if self.flagmap == 2:
-            if self._left == 0:
-                # the element's entrance (src/Contrl/AccSimulator.f90:1025-1026)
-                self._left, self.zedge = e.nseg, self.z
-            self._left -= 1
             self.z, mom = self.bunch.step2(
                 self.field, self.dom, self.z, 0.5 * e.length / e.nseg,
-                hip.element(e.type, e.values, e.length, self.zedge), e.radius, e.radius,
+                hip.element(e.type, e.values, e.length, zedge), e.radius, e.radius,
                 self.beam.current, want_moments)
         else:
             self.z, mom = self.bunch.step(self.field, self.dom, self.z,
-                                          self.segment_args(e), self.beam.current,
+                                          self.segment_args(e, zedge), self.beam.current,
                                           want_moments)
         self.nstep += 1
         return mom
 
     def step_granular(self, e: Element):
         """The same segment through the individual procedure entry points."""
+        zedge = self._enter(e)
         tau1 = 0.5 * e.length / e.nseg
         tau2 = 2.0 * tau1
         b = self.bunch
-        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp)
+        rf = self.rfdata(e)
+        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp, e.length, zedge, rf)
         if self.beam.current >= float(np.float32(1.0e-10)):
             ptrange, _, _ = b.conv1st(tau2, e.radius, e.radius)
             self.dom.update(ptrange)
@@ -110,51 +128,68 @@ class Simulation:
             self.field.update3O(self.dom)
             b.cvbkforth1st()
             b.kick1(tau2, self.field, self.dom)
-        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp)
+        self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp, e.length, zedge, rf)
         self.nstep += 1
 
-    def segment_args(self, e: Element) -> hip.Segment:
-        """The izh_segment of one space-charge step of element e."""
+    def segment_args(self, e: Element, zedge: float = 0.0) -> hip.Segment:
+        """The izh_segment of one space-charge step of element e entered at
+        zedge."""
         tau1 = 0.5 * e.length / e.nseg
-        return hip.segment(tau1, e.type, e.values, e.mapstp, e.radius, e.radius)
+        return hip.segment(tau1, e.type, e.values, e.mapstp, e.radius, e.radius,
+                           e.length, zedge, self.rfdata(e))
+
+    def _segments_of(self, elements) -> list:
+        """izh_segments of consecutive steps, with the entrance bookkeeping
+        advanced as the steps will advance it (each step moves z by 2 tau1)."""
+        out, z = [], self.z
+        for e in elements:
+            if self._left == 0:
+                self._left, self.zedge = e.nseg, z
+            self._left -= 1
+            out.append(self.segment_args(e, self.zedge))
+            tau1 = 0.5 * e.length / e.nseg
+            z = (z + tau1) + tau1
+        return out
 
     def run_fused(self, nsteps: int, start: int = 0, want_moments: bool = True):
         """Steps start..start+nsteps-1 of the lattice through izh_run.
 
-        Returns the per-step records (nsteps x 45: moments, then Npt) or None."""
+        Returns the per-step records (nsteps x 47: moments, Npt, refpt(5:6))
+        or None."""
         if self.flagmap != 1:
             raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
                                "Flagmap=2 runs step by step")
         segs = list(self.segments())[start:start + nsteps]
-        self.z, mom = self.bunch.run(self.field, self.dom, self.z,
-                                     [self.segment_args(e) for e in segs],
+        self.z, mom = self.bunch.run(self.field, self.dom, self.z, self._segments_of(segs),
                                      self.beam.current, want_moments)
         self.nstep += len(segs)
         return mom
 
     def run_elements(self, elements, want_moments: bool = True):
         """One space-charge step per entry of ``elements`` through izh_run;
-        returns the per-step records (45 values: moments, then Npt)."""
+        returns the per-step records (47 values: moments, Npt, refpt(5:6))."""
         if self.flagmap != 1:
             raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
                                "Flagmap=2 runs step by step")
         elements = list(elements)
         self.z, mom = self.bunch.run(self.field, self.dom, self.z,
-                                     [self.segment_args(e) for e in elements],
-                                     self.beam.current, want_moments)
+                                     self._segments_of(elements), self.beam.current,
+                                     want_moments)
         self.nstep += len(elements)
         return mom
 
     def diag(self, mom=None, refpt=None):
         """diagnostic1_Output row of the current state, or of a moments record
-        (44 values, or 45 with Npt as izh_run gives them)."""
+        (44 values, or 47 with Npt and refpt(5:6) as izh_run gives them)."""
         if mom is None:
             s, m, r2 = self.bunch.diagnostic1()
         else:
             s, m, r2 = mom[:37], mom[37:43], mom[43]
         info = self.bunch.info()
         npt = int(mom[44]) if mom is not None and len(mom) > 44 else info.npt
-        rp = np.array(info.refpt[:]) if refpt is None else np.asarray(refpt)
+        rp = np.array(info.refpt[:]) if refpt is None else np.asarray(refpt, dtype=float)
+        if refpt is None and mom is not None and len(mom) > 46:
+            rp[4], rp[5] = mom[45], mom[46]
         return diag_row(s, m, r2, npt, rp, info.mass, self.const.scxl,
                         self.const.pi)
 

@@ -89,24 +89,64 @@ int izh_bunch_upload(izh_bunch *b, const double *pts9, int64_t n);
 int izh_bunch_get_info(const izh_bunch *b, izh_bunch_info *info);
 int izh_bunch_set_info(izh_bunch *b, const izh_bunch_info *info);
 
-/* Element values.  v[0..3] are the deck line's v1..v4, which become the
- * element's Param(2:5) (src/Contrl/AccSimulator.f90:262-320):
+/* Element values.  v[0..13] are the deck line's v1..v14, which become the
+ * element's Param(2:15) (src/Contrl/AccSimulator.f90:262-536); unused
+ * entries are 0:
  *   bitype 0 drift:              pipe radius
  *   bitype 1 quadrupole:         gradient T/m (K in 1/m^2 when v[1] < -1e-6),
  *                                file ID, pipe radius
  *   bitype 2 constant focusing:  kx0^2, ky0^2, kz0^2 (1/m^2), pipe radius
- *   bitype 3 solenoid:           Bz0 (T), file ID, pipe radius, x offset */
+ *   bitype 3 solenoid:           Bz0 (T), file ID, pipe radius, x offset
+ *   bitype 101-104 DTL, CCDTL, CCL, SC (src/Appl/DTL.f90:14-45 and alike):
+ *                                field scale, RF frequency (Hz), phase
+ *                                (degrees), file ID, pipe radius; for the DTL
+ *                                also quadrupole 1 length and gradient,
+ *                                quadrupole 2 length and gradient
+ *   bitype 105 SolRF (src/Appl/SolRF.f90:16-34): field scale, RF frequency,
+ *                                phase, file ID, pipe radius, 5 misalignment
+ *                                values, Bz0 (T), 3 wakefield values */
+#define IZH_NVAL 14
+
+/* The on-axis field of RF cavities, as the reference's Data module holds it
+ * (src/DataStruct/Data.f90:41-61) after reading an rfdataN.in file.  Host
+ * memory; the library copies the values. */
+typedef struct izh_rfdata izh_rfdata;
+/* read1_Data (src/DataStruct/Data.f90:139-215): n rows of z, Ez, Ez', Ez''
+ * (row-major, 4 doubles each); z is shifted so the first row is at 0.  The
+ * DTL, CCDTL, CCL and SC maps interpolate it linearly. */
+int izh_rfdata_table(const double *rows4, int n, izh_rfdata **out);
+/* read3_Data (src/DataStruct/Data.f90:348-409): the n values of the file
+ * (coefficient count, field start, end and period, then the Fourier
+ * coefficients), padded with one zero to an odd count.  SolRF's map sums
+ * the series. */
+int izh_rfdata_fourier(const double *coef, int n, izh_rfdata **out);
+int izh_rfdata_destroy(izh_rfdata *d);
+
+/* A lattice element as the push routines see it: its type, RK6 steps, the
+ * deck values, its length, the z of its entrance (the Param(1) "zedge"
+ * run_AccSimulator sets on entry, src/Contrl/AccSimulator.f90:1025-1026)
+ * and, for RF cavities, its on-axis field (else NULL). */
+typedef struct {
+  int bitype;
+  int mapstp;
+  double v[IZH_NVAL];
+  double length;
+  double zedge;
+  const izh_rfdata *rf;
+} izh_element;
 
 /* maplinear_BeamLineElem (src/Appl/BeamLineElem.f90:776-825) for a
  * constant-gradient quadrupole (bitype 1, file ID <= 1e-5), constant
- * focusing (2) and a solenoid (3): the 6x6 linear map of a step tau from
- * RK6 integration over mapstp steps (maplinear_Quadrupole, _ConstFoc, _Sol:
- * src/Appl/Quadrupole.f90:120-305, src/Appl/ConstFoc.f90:116-294,
- * src/Appl/Sol.f90:120-321).  Host only.  xm is row-major,
- * xm[6*(j-1)+(k-1)] = xm(j,k); refpt is updated. */
-int izh_element_maplinear(const izh_const *c, double tau, int bitype,
-                          const double v[4], int mapstp, double refpt[6],
-                          double charge, double mass, double xm[36]);
+ * focusing (2), a solenoid (3) and the RF cavities 101-105: the 6x6 linear
+ * map of a step tau from z, from RK6 integration over e->mapstp steps
+ * (maplinear_Quadrupole, _ConstFoc, _Sol, _DTL, _CCDTL, _CCL, _SC, _SolRF:
+ * src/Appl/Quadrupole.f90:120-305, ConstFoc.f90:116-294, Sol.f90:120-321,
+ * DTL.f90:136-418, CCDTL.f90:122-373, CCL.f90:122-375, SC.f90:122-373,
+ * SolRF.f90:126-479).  Host only.  xm is row-major,
+ * xm[6*(j-1)+(k-1)] = xm(j,k); refpt(5:6) are updated (an RF cavity
+ * changes the reference energy). */
+int izh_element_maplinear(const izh_const *c, double z, double tau, const izh_element *e,
+                          double refpt[6], double charge, double mass, double xm[36]);
 
 /* The quadrupole case of izh_element_maplinear, v = {gradient, 0, 0, 0}
  * (maplinear_Quadrupole, src/Appl/Quadrupole.f90:120-305). */
@@ -117,15 +157,15 @@ int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
 /* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465): the exact drift
  * (bitype 0); a quadrupole (1) through its linear map, or through the
  * chromatic K-mode map transfmapK_Quadrupole (src/Appl/Quadrupole.f90:
- * 509-607) when v[1] < -1e-6; constant focusing (2) and a solenoid (3)
- * through their linear maps.  *z += tau.  Refused with IZH_ERR_ARG: the
- * undulator map (bitype 1, v[1] < -10), quadrupole fields from a file
- * (v[1] > 1e-5), the ideal RF-cavity branch (bitype 2 or 3 with
- * Param(5) = v[3] < 0) and other element types. */
-int izh_map1(izh_bunch *b, double *z, double tau, int bitype, const double v[4],
-             int mapstp);
+ * 509-607) when v[1] < -1e-6; constant focusing (2), a solenoid (3) and the
+ * RF cavities 101-105 through their linear maps (izh_element_maplinear).
+ * *z += tau.  Refused with IZH_ERR_ARG: the undulator map (bitype 1,
+ * v[1] < -10), quadrupole fields from a file (v[1] > 1e-5), the ideal
+ * RF-cavity branch (bitype >= 2 with Param(5) = v[3] < 0), an RF cavity
+ * without its field, SolRF wakefields (v[13] > 0) and other element types. */
+int izh_map1(izh_bunch *b, double *z, double tau, const izh_element *e);
 
-/* izh_map1 for bitype 0 and 1 with v = {gradient, 0, 0, 0}. */
+/* izh_map1 for bitype 0 and 1 with v = {gradient, 0, ...}. */
 int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
                double gradient, int mapstp);
 /* The map-apply half of drift1_BeamBunch for any host-computed 6x6 map
@@ -174,11 +214,9 @@ int izh_field_upload_rho(izh_field *f, const double *rho);
 
 /* One segment of a lattice: an element's half step and values. */
 typedef struct {
-  double tau1;        /* half step, L/(2*nseg) */
-  int bitype;         /* 0 drift, 1 quadrupole, 2 constant focusing, 3 solenoid */
-  int mapstp;         /* RK6 steps of the element's linear map */
-  double v[4];        /* the element's values, as for izh_map1 */
-  double xrad, yrad;  /* pipe aperture of the loss test */
+  double tau1;            /* half step, L/(2*nseg) */
+  izh_element e;          /* the element */
+  double xrad, yrad;      /* pipe aperture of the loss test */
 } izh_segment;
 
 /* One segment of the reference loop: map1(tau1), conv1st, update_CompDom,
@@ -191,19 +229,10 @@ typedef struct {
 int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
              const izh_segment *seg, double bcurr, double *moments);
 
-/* ---- The nonlinear Lorentz integrator (Flagmap=2) ---- */
-
-/* An element for the Lorentz integrator: its type and values (as for
- * izh_map1), its length and the z of its entrance (the Param(1) "zedge"
- * run_AccSimulator sets, src/Contrl/AccSimulator.f90:1025-1026).  Fields:
+/* ---- The nonlinear Lorentz integrator (Flagmap=2) ----
+ * Element fields (the izh_element's bitype, values, length and zedge):
  * none for a drift, B = G (y, x, 0) for a quadrupole (file ID <= 0), the
  * constant-focusing and solenoid fields of getfld_ConstFoc / getfld_Sol. */
-typedef struct {
-  int bitype;
-  double v[4];
-  double length;
-  double zedge;
-} izh_element;
 
 /* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2;
  * *z and refpt(5) advance by tau/2. */
@@ -234,10 +263,12 @@ int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double ta
  * one pass over the particles, which are re-sorted into 8x8x8-cell tiles in
  * that same pass so the deposit accumulates in LDS; the domain update stays
  * on the device, so the host does not wait between steps.  Results equal
- * izh_step's to rounding.  moments (NULL or 45*nseg doubles) receives per
- * step the 44 diagnostic1_Output values izh_step gives, then Npt.  After the call the
+ * izh_step's to rounding.  moments (NULL or IZH_RUN_RECORD*nseg doubles)
+ * receives per step the 44 diagnostic1_Output values izh_step gives, then
+ * Npt, refpt(5) and refpt(6) (RF cavities change the reference energy).  After the call the
  * bunch is in the z frame; its particle order is by tile (izh_bunch_download
  * restores the reference's order). */
+#define IZH_RUN_RECORD 47
 int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             const izh_segment *segs, int64_t nseg, double bcurr,
             double *moments);

@@ -35,16 +35,80 @@ void or_physconst(double freq, or_const *c) {
 
 /* ---------------------------------------------------------------- maps */
 
-/* The linear ODE of the element maps: the reference particle's phase and
- * energy (y1..y6), cosine- and sine-like solutions of three 2x2 systems
- * (y7..y18) and, for the solenoid, its rotation pair (y19..y22).
+/* getaxfldE_DTL / _CCDTL / _CCL / _SC (src/Appl/DTL.f90:350-393): linear
+ * interpolation in read1_Data's table found by bisection on zdat(1..Ndata)
+ * (extrapolating outside it), times the field scale. */
+static void axfld_table(const or_element *e, double z, double *ez1, double *ezp1) {
+  const or_rfdata *d = e->rf;
+  double zz = z - e->zedge, hstep, slope;
+  int klo = 1, khi = d->n, k;
+  while (khi - klo > 1) {
+    k = (khi + klo) / 2;
+    if (d->zdat[k - 1] > zz) khi = k;
+    else klo = k;
+  }
+  hstep = d->zdat[khi - 1] - d->zdat[klo - 1];
+  slope = (d->edat[khi - 1] - d->edat[klo - 1]) / hstep;
+  *ez1 = d->edat[klo - 1] + slope * (zz - d->zdat[klo - 1]);
+  slope = (d->epdat[khi - 1] - d->epdat[klo - 1]) / hstep;
+  *ezp1 = d->epdat[klo - 1] + slope * (zz - d->zdat[klo - 1]);
+  *ez1 = *ez1 * e->v[0];
+  *ezp1 = *ezp1 * e->v[0];
+}
+
+/* getaxfldE_SolRF (src/Appl/SolRF.f90:356-466): the Fourier series of
+ * read3_Data's coefficients, inside the element and the field's own range
+ * (with a REAL(4) 1e-10 tolerance at the ends), else 0. */
+static void axfld_fourier(const or_const *c, const or_element *e, double z, double *ez1,
+                          double *ezp1) {
+  const double *F = e->rf->fcoef - 1; /* Fcoef(1..) */
+  const double epstol = (double)1.0e-10f;
+  double zedge = e->zedge, len = e->length;
+  *ez1 = 0.0;
+  *ezp1 = 0.0;
+  if ((z >= zedge && z <= zedge + len) || fabs(z - zedge) <= epstol ||
+      fabs(z - zedge - len) <= epstol) {
+    double zlc = z - zedge;
+    int numpar1 = (int)(F[1] + 0.1), i;
+    double zstart1 = F[2], zend1 = F[3], zlength1 = F[4];
+    if ((zlc >= zstart1 && zlc <= zend1) || fabs(zlc - zstart1) <= epstol ||
+        fabs(zlc - zend1) <= epstol) {
+      double zmid = zlength1 / 2, zz = z - zedge - zstart1 - zmid, zlen = zlength1;
+      *ez1 = F[5] / 2;
+      for (i = 2; i <= (numpar1 - 1) / 2 + 1; i++) {
+        double kk = (i - 1) * 2 * c->pi / zlen;
+        double arg = (i - 1) * 2 * c->pi * zz / zlen;
+        *ez1 = *ez1 + F[2 * i + 2] * cos(arg) + F[2 * i + 3] * sin(arg);
+        *ezp1 = *ezp1 + kk * (-F[2 * i + 2] * sin(arg) + F[2 * i + 3] * cos(arg));
+      }
+      *ez1 = *ez1 * e->v[0];
+      *ezp1 = *ezp1 * e->v[0];
+    }
+  }
+}
+
+static void axfld(const or_const *c, const or_element *e, double z, double *ez1,
+                  double *ezp1) {
+  if (e->rf->fourier) axfld_fourier(c, e, z, ez1, ezp1);
+  else axfld_table(e, z, ez1, ezp1);
+}
+
+/* Right-hand side of the linear map ODE at position t: the reference
+ * particle's phase and energy (y5, y6), the cosine- and sine-like
+ * solutions of the three planes (y7..y18) and, for the solenoids, the
+ * rotation pair (y19..y22).
  *   quadrupole: intfunc1_Quadrupole, constant gradient
- *               (src/Appl/Quadrupole.f90:254-305), p[0] = gradient;
- *   ConstFoc:   intfunc1_ConstFoc (src/Appl/ConstFoc.f90:245-293),
- *               p = kx0^2, ky0^2, kz0^2;
- *   solenoid:   intfunc1_Sol (src/Appl/Sol.f90:271-321), p[0] = Bz0. */
-static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
+ *               (src/Appl/Quadrupole.f90:254-305);
+ *   ConstFoc:   intfunc1_ConstFoc (src/Appl/ConstFoc.f90:245-293);
+ *   solenoid:   intfunc1_Sol (src/Appl/Sol.f90:271-321);
+ *   RF cavity:  intfunc1_DTL (src/Appl/DTL.f90:286-347, with the
+ *               drift-tube quadrupoles of getBgradfld_DTL, :395-418),
+ *               intfunc1_CCDTL / _CCL / _SC (CCL.f90:269-328) and
+ *               intfunc1_SolRF (SolRF.f90:287-353, Bz0 = Param(12)). */
+static void lin_rhs(const or_const *c, const or_element *e, double qmcc, double t,
                     const double *y, double *f) {
+  int type = e->bitype;
+  const double *pv = e->v;
   double gamma0 = -y[5];
   double beta0 = sqrt(gamma0 * gamma0 - 1.0) / gamma0;
   double gbet = sqrt((gamma0 - 1.0) * (gamma0 + 1.0));
@@ -61,7 +125,7 @@ static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
     s11 = pv[0] * c->scxl;
     s33 = pv[1] * c->scxl;
     s55 = pv[2] * c->scxl;
-  } else {
+  } else if (type == 3) {
     double k;
     brho = gbet / c->clight / qmcc;
     k = 0.5 * pv[0] / brho;
@@ -69,6 +133,42 @@ static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
     s33 = (k * k) * c->scxl;
     s55 = 0.0;
     c2alpha = -pv[0] / brho / 2;
+  } else {
+    double ez1, ezp1, ww, theta0, sinphi, cosphi, rfdsgn, s11tmp, r;
+    axfld(c, e, t, &ez1, &ezp1);
+    ww = pv[1] / c->scfreq;
+    theta0 = pv[2] * (double)asinf(1.0f) / 90;
+    sinphi = sin(ww * y[4] + theta0);
+    cosphi = cos(ww * y[4] + theta0);
+    rfdsgn = ez1 * cosphi;
+    f[5] = -qmcc * rfdsgn;
+    brho = gbet / c->clight / qmcc;
+    r = qmcc * rfdsgn / (beta0 * beta0) / (gamma0 * gamma0);
+    s11tmp = 0.5 * (1.0 + 0.5 * gamma0 * gamma0) * (r * r) +
+             qmcc * 0.5 / (beta0 * beta0 * beta0) / (gamma0 * gamma0 * gamma0) * ez1 *
+                 sinphi * ww / c->scxl;
+    if (type == 101) {
+      /* getBgradfld_DTL: Param(7..10) = v[5..8] */
+      double zz = t - e->zedge, bgrad;
+      if (zz < pv[5]) bgrad = pv[6];
+      else if (zz < e->length - pv[7]) bgrad = 0.0;
+      else bgrad = pv[8];
+      s11 = (s11tmp + bgrad / brho) * c->scxl;
+      s33 = (s11tmp - bgrad / brho) * c->scxl;
+    } else if (type == 105) {
+      double b0 = pv[10], kb = 0.5 * b0 / brho;
+      s11 = (s11tmp + (kb * kb) / 2) * c->scxl;
+      s33 = (s11tmp + (kb * kb) / 2) * c->scxl;
+      c2alpha = -b0 / brho / 2;
+    } else {
+      s11 = s11tmp * c->scxl;
+      s33 = s11tmp * c->scxl;
+    }
+    s55 = -1.5 * qmcc / (beta0 * beta0) / gamma0 * ezp1 * cosphi +
+          (beta0 * beta0 + 0.5) / (beta0 * beta0 * beta0) / gamma0 * qmcc * ez1 * sinphi *
+              ww / c->scxl +
+          1.5 * (1.0 - 0.5 * gamma0 * gamma0) * (r * r);
+    s55 = c->scxl * s55;
   }
   f[6] = y[7] / c->scxl;
   f[7] = -s11 * y[6];
@@ -82,7 +182,7 @@ static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
   f[15] = -s55 * y[14];
   f[16] = y[17] / c->scxl;
   f[17] = -s55 * y[16];
-  if (type == 3) {
+  if (type == 3 || type == 105) {
     f[18] = -c2alpha * y[20];
     f[19] = -c2alpha * y[21];
     f[20] = c2alpha * y[18];
@@ -90,21 +190,42 @@ static void lin_rhs(const or_const *c, int type, const double *pv, double qmcc,
   }
 }
 
-int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
-                   int mapstp, double refpt[6], double charge, double mass,
-                   double xm[36]) {
-  /* maplinear_Quadrupole / _ConstFoc / _Sol with rk6i_* (src/Appl/
-   * Quadrupole.f90:120-252, ConstFoc.f90:116-243, Sol.f90:120-268) */
+/* dlt and thl of maplinear_DTL at the entrance or exit (src/Appl/DTL.f90:
+ * 189-220): from the on-axis field and slope at z and the reference phase
+ * and energy in refpt. */
+static void rf_edge_terms(const or_const *c, const or_element *e, double qmcc, double z,
+                          const double refpt[6], double *dlt, double *thl) {
+  double ez, ez1, g = -refpt[5], beta = sqrt(g * g - 1.0) / g, u = g * beta;
+  double ww = e->v[1] / c->scfreq, theta0 = e->v[2] * (double)asinf(1.0f) / 90;
+  double sinth, costh, uprim, qpw;
+  axfld(c, e, z, &ez, &ez1);
+  sinth = sin(ww * refpt[4] + theta0);
+  costh = cos(ww * refpt[4] + theta0);
+  uprim = qmcc * ez / beta * costh;
+  qpw = 0.5 * qmcc * c->scxl / (u * ww);
+  *dlt = c->scxl * (0.5 * uprim / u - qpw * ez1 * sinth);
+  *thl = 1.5 * c->scxl * (uprim / u);
+}
+
+int or_element_map(const or_const *c, double t, double tau, const or_element *el,
+                   double refpt[6], double charge, double mass, double xm[36]) {
+  /* maplinear_Quadrupole / _ConstFoc / _Sol / _DTL / _SolRF with rk6i_*
+   * (src/Appl/Quadrupole.f90:120-252, ConstFoc.f90:116-243, Sol.f90:120-268,
+   * DTL.f90:136-284, SolRF.f90:126-285) */
   enum { NV = 22 };
   double y[NV], yt[NV], f[NV], a[NV], b[NV], cc[NV], d[NV], e[NV], g[NV], o[NV],
       p[NV];
-  double h, gi, betai, ui, squi, sqi3, gf, betaf, uf, squf, sqf3;
+  double h, gi, betai, ui, squi, sqi3, gf, betaf, uf, squf, sqf3, tt;
+  double dlti = 0.0, thli = 0.0, dltf = 0.0, thlf = 0.0;
   double qmcc = charge / mass;
-  int i, k, nv;
-  if (bitype < 1 || bitype > 3 || mapstp < 1) return -1;
+  int i, k, nv, rf, bitype = el->bitype, mapstp = el->mapstp;
+  const double *v = el->v;
+  rf = bitype >= 101 && bitype <= 105;
+  if (!((bitype >= 1 && bitype <= 3) || rf) || mapstp < 1) return -1;
   /* a quadrupole's field from a file (file ID > 1e-5) is not restated */
   if (bitype == 1 && v[1] > 1.0e-5) return -1;
-  nv = bitype == 3 ? 22 : 18;
+  if (rf && (!el->rf || (bitype == 105) != (el->rf->fourier != 0))) return -1;
+  nv = (bitype == 3 || bitype == 105) ? 22 : 18;
   memset(y, 0, sizeof y);
   memset(f, 0, sizeof f);
   y[4] = refpt[4];
@@ -123,44 +244,46 @@ int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
   ui = gi * betai;
   squi = sqrt(ui);
   sqi3 = squi * squi * squi;
+  if (rf) rf_edge_terms(c, el, qmcc, t, refpt, &dlti, &thli);
   for (i = 0; i < mapstp; i++) {
-    lin_rhs(c, bitype, v, qmcc, y, f);
+    tt = t + i * h;  /* rk6i_*: t = tint + i*h after each step */
+    lin_rhs(c, el, qmcc, tt, y, f);
     for (k = 0; k < nv; k++) { a[k] = h * f[k]; yt[k] = y[k] + a[k] / 9.0; }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + h / 9.0, yt, f);
     for (k = 0; k < nv; k++) {
       b[k] = h * f[k];
       yt[k] = y[k] + (a[k] + 3.0 * b[k]) / 24.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + h / 6.0, yt, f);
     for (k = 0; k < nv; k++) {
       cc[k] = h * f[k];
       yt[k] = y[k] + (a[k] - 3.0 * b[k] + 4.0 * cc[k]) / 6.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + h / 3.0, yt, f);
     for (k = 0; k < nv; k++) {
       d[k] = h * f[k];
       yt[k] = y[k] + (278.0 * a[k] - 945.0 * b[k] + 840.0 * cc[k] + 99.0 * d[k]) / 544.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + 0.5 * h, yt, f);
     for (k = 0; k < nv; k++) {
       e[k] = h * f[k];
       yt[k] = y[k] + (-106.0 * a[k] + 273.0 * b[k] - 104.0 * cc[k] - 107.0 * d[k] +
                       48.0 * e[k]) / 6.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + 2.0 * h / 3.0, yt, f);
     for (k = 0; k < nv; k++) {
       g[k] = h * f[k];
       yt[k] = y[k] + (110974.0 * a[k] - 236799.0 * b[k] + 68376.0 * cc[k] +
                       103803.0 * d[k] - 10240.0 * e[k] + 1926.0 * g[k]) / 45648.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + 5.0 * h / 6.0, yt, f);
     for (k = 0; k < nv; k++) {
       o[k] = h * f[k];
       yt[k] = y[k] + (-101195.0 * a[k] + 222534.0 * b[k] - 71988.0 * cc[k] -
                       26109.0 * d[k] - 2.0e4 * e[k] - 72.0 * g[k] + 22824.0 * o[k]) /
                          25994.0;
     }
-    lin_rhs(c, bitype, v, qmcc, yt, f);
+    lin_rhs(c, el, qmcc, tt + h, yt, f);
     for (k = 0; k < nv; k++) {
       p[k] = h * f[k];
       y[k] = y[k] + (41.0 * a[k] + 216.0 * cc[k] + 27.0 * d[k] + 272.0 * e[k] +
@@ -174,47 +297,48 @@ int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
   uf = gf * betaf;
   squf = sqrt(uf);
   sqf3 = squf * squf * squf;
-  /* xm row-major: xm[6*(j-1)+(k-1)] = xm(j,k); dlti = dltf = thli = thlf = 0 */
+  if (rf) rf_edge_terms(c, el, qmcc, t + tau, refpt, &dltf, &thlf);
+  /* xm row-major: xm[6*(j-1)+(k-1)] = xm(j,k); dlt = thl = 0 outside the
+   * RF cavities (src/Appl/DTL.f90:205-220, SolRF.f90:207-231) */
   for (k = 0; k < 36; k++) xm[k] = 0.0;
-  if (bitype == 3) {
-    /* src/Appl/Sol.f90:187-207: each transverse block times the rotation */
-    xm[0] = y[18] * (y[6] * squi / squf);
-    xm[6] = y[18] * (y[7] * squi * squf);
-    xm[1] = y[18] * y[8] / (squi * squf);
-    xm[7] = y[18] * y[9] * squf / squi;
-    xm[2] = y[19] * (y[6] * squi / squf);
-    xm[8] = y[19] * (y[7] * squi * squf);
-    xm[3] = y[19] * y[8] / (squi * squf);
-    xm[9] = y[19] * y[9] * squf / squi;
-    xm[12] = y[20] * (y[10] * squi / squf);
-    xm[18] = y[20] * (y[11] * squi * squf);
-    xm[13] = y[20] * y[12] / (squi * squf);
-    xm[19] = y[20] * y[13] * squf / squi;
-    xm[14] = y[21] * (y[10] * squi / squf);
-    xm[20] = y[21] * (y[11] * squi * squf);
-    xm[15] = y[21] * y[12] / (squi * squf);
-    xm[21] = y[21] * y[13] * squf / squi;
+  if (nv == 22) {
+    /* each transverse block times the rotation (src/Appl/Sol.f90:187-207) */
+    int blk;
+    for (blk = 0; blk < 4; blk++) {
+      int r = blk < 2 ? 0 : 2, cl = (blk % 2) * 2, oo = r == 0 ? 6 : 10;
+      double w = y[18 + blk];
+      xm[6 * r + cl] = w * (y[oo] * squi / squf + y[oo + 2] * squi / squf * dlti);
+      xm[6 * (r + 1) + cl] = w * ((y[oo + 1] - y[oo] * dltf) * squi * squf +
+                                  (y[oo + 3] - y[oo + 2] * dltf) * squi * squf * dlti);
+      xm[6 * r + cl + 1] = w * y[oo + 2] / (squi * squf);
+      xm[6 * (r + 1) + cl + 1] = w * (y[oo + 3] - y[oo + 2] * dltf) * squf / squi;
+    }
   } else {
-    xm[0] = y[6] * squi / squf;
-    xm[6] = y[7] * squi * squf;
+    xm[0] = y[6] * squi / squf + y[8] * squi / squf * dlti;
+    xm[6] = (y[7] - y[6] * dltf) * squi * squf + (y[9] - y[8] * dltf) * squi * squf * dlti;
     xm[1] = y[8] / (squi * squf);
-    xm[7] = y[9] * squf / squi;
-    xm[14] = y[10] * squi / squf;
-    xm[20] = y[11] * squi * squf;
+    xm[7] = (y[9] - y[8] * dltf) * squf / squi;
+    xm[14] = y[10] * squi / squf + y[12] * squi / squf * dlti;
+    xm[20] = (y[11] - y[10] * dltf) * squi * squf +
+             (y[13] - y[12] * dltf) * squi * squf * dlti;
     xm[15] = y[12] / (squi * squf);
-    xm[21] = y[13] * squf / squi;
+    xm[21] = (y[13] - y[12] * dltf) * squf / squi;
   }
-  xm[28] = y[14] * sqi3 / sqf3;
-  xm[34] = y[15] * sqi3 * sqf3;
+  xm[28] = y[14] * sqi3 / sqf3 + y[16] * sqi3 / sqf3 * thli;
+  xm[34] = (y[15] - y[14] * thlf) * sqi3 * sqf3 + (y[17] - y[16] * thlf) * sqi3 * sqf3 * thli;
   xm[29] = y[16] / (sqi3 * sqf3);
-  xm[35] = y[17] * sqf3 / sqi3;
+  xm[35] = (y[17] - y[16] * thlf) * sqf3 / sqi3;
   return 0;
 }
 
 void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
                  double refpt[6], double charge, double mass, double xm[36]) {
-  const double v[4] = {grad, 0.0, 0.0, 0.0};
-  or_element_map(c, tau, 1, v, mapstp, refpt, charge, mass, xm);
+  or_element e;
+  memset(&e, 0, sizeof e);
+  e.bitype = 1;
+  e.mapstp = mapstp;
+  e.v[0] = grad;
+  or_element_map(c, 0.0, tau, &e, refpt, charge, mass, xm);
 }
 
 /* transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607): the chromatic
@@ -286,9 +410,10 @@ static void quad_kmode(const or_const *c, double *pts, int64_t n, double refpt[6
 }
 
 int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
-            double *z, double tau, int bitype, const double v[4], int mapstp,
-            double charge, double mass) {
+            double *z, double tau, const or_element *e, double charge, double mass) {
   int64_t i;
+  int bitype = e->bitype;
+  const double *v = e->v;
   if (bitype == 0) {
     /* exact drift, src/Appl/BeamBunch.f90:276-298 */
     double gam0 = -refpt[5];
@@ -311,8 +436,8 @@ int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
     /* 6x6 map, src/Appl/BeamBunch.f90:308-320, 436-449 */
     double xm[36];
     if (bitype >= 2 && v[3] < 0.0) return -1;  /* ideal RF cavity model (:325-329) */
-    if (or_element_map(c, tau, bitype, v, mapstp, refpt, charge, mass, xm) != 0)
-      return -1;
+    if (bitype == 105 && v[13] > 0.0) return -1;  /* SolRF wakefield: kick1wake */
+    if (or_element_map(c, *z, tau, e, refpt, charge, mass, xm) != 0) return -1;
     for (i = 0; i < n; i++) {
       double t[6];
       int j, k;
@@ -752,7 +877,7 @@ void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], doubl
 /* getfld_BeamLineElem for the analytic fields (src/Appl/BeamLineElem.f90:
  * 827-870): E (V/m) and B (T) at pos = (x, y, z, t).  Returns -1 for the
  * elements not restated. */
-int or_elem_field(const or_const *c, int bitype, const double v[4], double length,
+int or_elem_field(const or_const *c, int bitype, const double *v, double length,
                   double zedge, const double pos[4], double f[6]) {
   int k;
   for (k = 0; k < 6; k++) f[k] = 0.0;
@@ -835,7 +960,7 @@ static void lorentz_push(double *p, double ex, double ey, double ez, double bx, 
 int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
-             int bitype, const double v[4], double length, double zedge, double z) {
+             int bitype, const double *v, double length, double zedge, double z) {
   /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508) with scatter2
    * (:2510-3071) or, without current, scatter20 (:3075-3531), for the
    * elements with analytic fields */
@@ -905,8 +1030,8 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
 }
 
 int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                 int nx, int ny, int nz, double tau1, int bitype, const double v[4],
-                 double length, double zedge, double pipe_radius, double *rho_out,
+                 int nx, int ny, int nz, double tau1, const or_element *e,
+                 double pipe_radius, double *rho_out,
                  double *phi_out) {
   /* one segment of the reference loop with Flagmap=2
    * (src/Contrl/AccSimulator.f90:1106-1113,1446-1458) */
@@ -929,7 +1054,7 @@ int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
   }
   if (rc == 0)
     rc = or_kick2(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, curr, tau2,
-                  st->mass, st->charge, bitype, v, length, zedge, st->z);
+                  st->mass, st->charge, e->bitype, e->v, e->length, e->zedge, st->z);
   if (rc == 0) or_drift2(c, pts, n, st->refpt, &st->z, tau2);
   if (!rho_out) free(rho);
   if (!phi_out) free(phi);
@@ -1023,15 +1148,15 @@ void or_diag_row(const or_const *c, const double t[37], const double gm[6],
 /* ------------------------------------------------------------ one step */
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                int nx, int ny, int nz, double tau1, int bitype, const double v[4],
-                int mapstp, double pipe_radius, double *rho_out,
+                int nx, int ny, int nz, double tau1, const or_element *e,
+                double pipe_radius, double *rho_out,
                 double *phi_out) {
   size_t g = (size_t)nx * ny * nz;
   double *rho = rho_out ? rho_out : (double *)malloc(g * sizeof(double));
   double *phi = phi_out ? phi_out : (double *)malloc(g * sizeof(double));
   double ptrange[6], range[6], msize[3];
   const double tau2 = 2.0 * tau1;
-  int rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, v, mapstp, st->charge,
+  int rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, e, st->charge,
                    st->mass);
   if (rc == 0 && st->bcurr >= (double)1.0e-10f) {
     n = or_conv1st(c, pts, n, st->refpt, pipe_radius, pipe_radius, ptrange,
@@ -1045,7 +1170,7 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                tau2, st->mass, st->charge);
   }
   if (rc == 0)
-    rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, bitype, v, mapstp, st->charge,
+    rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, e, st->charge,
                  st->mass);
   if (!rho_out) free(rho);
   if (!phi_out) free(phi);

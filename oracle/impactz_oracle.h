@@ -25,33 +25,57 @@ typedef struct {
 
 void or_physconst(double freq, or_const *c);
 
-/* Element values: v[0..3] = the deck line's v1..v4, i.e. the element's
- * Param(2:5) (src/Contrl/AccSimulator.f90:262-320).
+/* Element values: v[0..13] = the deck line's v1..v14, i.e. the element's
+ * Param(2:15) (src/Contrl/AccSimulator.f90:262-536).
  *   bitype 1 quadrupole: gradient T/m (K in 1/m^2 when v[1] < -1e-6),
  *            file ID, pipe radius;
  *   bitype 2 constant focusing: kx0^2, ky0^2, kz0^2, pipe radius;
- *   bitype 3 solenoid: Bz0, file ID, pipe radius, x misalignment. */
+ *   bitype 3 solenoid: Bz0, file ID, pipe radius, x misalignment;
+ *   bitype 101-104 DTL, CCDTL, CCL, SC: field scale, RF frequency, phase
+ *            (degrees), file ID, pipe radius, (DTL) quadrupole 1 length and
+ *            gradient, quadrupole 2 length and gradient;
+ *   bitype 105 SolRF: as 103, then 5 misalignment values, Bz0. */
+#define OR_NVAL 14
 
-/* The element's 6x6 linear map: RK6 over mapstp steps of maplinear_
- * Quadrupole (constant gradient), _ConstFoc or _Sol (src/Appl/Quadrupole.f90:
- * 120-305, ConstFoc.f90:116-294, Sol.f90:120-321).  refpt is updated as the
- * reference does.  Returns -1 for anything else. */
-int or_element_map(const or_const *c, double tau, int bitype, const double v[4],
-                   int mapstp, double refpt[6], double charge, double mass,
-                   double xm[36]);
+/* The Data module's on-axis field (src/DataStruct/Data.f90:41-61): the
+ * table of read1_Data (zdat already shifted to start at 0) or the Fcoef
+ * array of read3_Data (n values, already padded to an odd count). */
+typedef struct {
+  int fourier;
+  int n;
+  const double *zdat, *edat, *epdat;
+  const double *fcoef;
+} or_rfdata;
 
-/* The quadrupole case of or_element_map with v = {grad, 0, 0, 0}. */
+/* An element: type, RK6 steps, values, length, entrance z (its Param(1),
+ * set on entry, src/Contrl/AccSimulator.f90:1025-1026), RF field. */
+typedef struct {
+  int bitype, mapstp;
+  double v[OR_NVAL];
+  double length, zedge;
+  const or_rfdata *rf;
+} or_element;
+
+/* The element's 6x6 linear map for a step tau from t: RK6 over mapstp
+ * steps of maplinear_Quadrupole (constant gradient), _ConstFoc, _Sol, _DTL,
+ * _CCDTL, _CCL, _SC or _SolRF (src/Appl/Quadrupole.f90:120-305,
+ * ConstFoc.f90:116-294, Sol.f90:120-321, DTL.f90:136-418, CCDTL.f90:
+ * 122-373, CCL.f90:122-375, SC.f90:122-373, SolRF.f90:126-479).  refpt is
+ * updated as the reference does.  Returns -1 for anything else. */
+int or_element_map(const or_const *c, double t, double tau, const or_element *e,
+                   double refpt[6], double charge, double mass, double xm[36]);
+
+/* The quadrupole case of or_element_map with v = {grad, 0, ...}. */
 void or_quad_map(const or_const *c, double tau, double grad, int mapstp,
                  double refpt[6], double charge, double mass, double xm[36]);
 
 /* drift1_BeamBunch (src/Appl/BeamBunch.f90:250-465): bitype 0 exact drift,
  * bitype 1 quadrupole (the K-mode chromatic map transfmapK_Quadrupole,
- * src/Appl/Quadrupole.f90:509-607, when v[1] < -1e-6), bitypes 2 and 3
- * through their linear maps.  z is advanced by tau.  Returns -1 for the
- * branches not restated (undulator, ideal RF cavity, field files). */
+ * src/Appl/Quadrupole.f90:509-607, when v[1] < -1e-6), bitypes 2, 3 and
+ * 101-105 through their linear maps.  z is advanced by tau.  Returns -1 for
+ * the branches not restated (undulator, ideal RF cavity, field files). */
 int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
-            double *z, double tau, int bitype, const double v[4], int mapstp,
-            double charge, double mass);
+            double *z, double tau, const or_element *e, double charge, double mass);
 
 /* conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352,1657-1663).
  * Compacts lost particles in place; returns the new count; rescales *current. */
@@ -117,7 +141,7 @@ void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], doubl
  * constant focusing and solenoid (with its linear end ramps).  pos = (x, y
  * in m, z, t); f = (Ex, Ey, Ez, Bx, By, Bz).  zedge is the element's entrance
  * z.  Returns -1 for anything else. */
-int or_elem_field(const or_const *c, int bitype, const double v[4], double length,
+int or_elem_field(const or_const *c, int bitype, const double *v, double length,
                   double zedge, const double pos[4], double f[6]);
 
 /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508): the space-charge E of
@@ -127,7 +151,7 @@ int or_elem_field(const or_const *c, int bitype, const double v[4], double lengt
 int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
-             int bitype, const double v[4], double length, double zedge, double z);
+             int bitype, const double *v, double length, double zedge, double z);
 
 /* diagnostic1_Output sums (src/Contrl/Output.f90:119-245): 37 sums in the
  * reference's tmplc order, 6 abs-maxima, max of x^2+y^2. */
@@ -158,16 +182,14 @@ typedef struct {
 } or_bunch_state;
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                int nx, int ny, int nz, double tau1, int bitype, const double v[4],
-                int mapstp, double pipe_radius, double *rho_out,
-                double *phi_out);
+                int nx, int ny, int nz, double tau1, const or_element *e,
+                double pipe_radius, double *rho_out, double *phi_out);
 
 /* One Flagmap=2 segment: drift2(2 tau1) -> conv1st -> update_CompDom ->
  * deposit -> solve -> cvbkforth1st -> kick2(2 tau1) -> drift2(2 tau1). */
 int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
-                 int nx, int ny, int nz, double tau1, int bitype, const double v[4],
-                 double length, double zedge, double pipe_radius, double *rho_out,
-                 double *phi_out);
+                 int nx, int ny, int nz, double tau1, const or_element *e,
+                 double pipe_radius, double *rho_out, double *phi_out);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,57 @@ class OrConst(C.Structure):
                 ("scxl", C.c_double), ("scfreq", C.c_double)]
 
 
+NVAL = 14
+
+
+class OrRFData(C.Structure):
+    _fields_ = [("fourier", C.c_int), ("n", C.c_int), ("zdat", _dp), ("edat", _dp),
+                ("epdat", _dp), ("fcoef", _dp)]
+
+
+class OrElement(C.Structure):
+    _fields_ = [("bitype", C.c_int), ("mapstp", C.c_int), ("v", C.c_double * NVAL),
+                ("length", C.c_double), ("zedge", C.c_double),
+                ("rf", C.POINTER(OrRFData))]
+
+
+class RFData:
+    """The Data module's arrays for one rfdata file: read1_Data's table with
+    z shifted to start at 0 (src/DataStruct/Data.f90:183-199) or read3_Data's
+    coefficients padded to an odd count (:387-406)."""
+
+    def __init__(self, field, fourier: bool):
+        a = np.asarray(field, dtype=np.float64)
+        self.c = OrRFData()
+        self.c.fourier = int(fourier)
+        if fourier:
+            coef = a.ravel().copy()
+            if coef.size % 2 == 0:
+                coef = np.append(coef, 0.0)
+            self.keep = [coef]
+            self.c.n = coef.size
+            self.c.fcoef = _p(coef)
+        else:
+            t = a.reshape(-1, 4)
+            cols = [np.ascontiguousarray(t[:, k]) for k in range(4)]
+            cols[0] = cols[0] - t[0, 0]
+            self.keep = cols
+            self.c.n = t.shape[0]
+            self.c.zdat, self.c.edat, self.c.epdat = (_p(cols[k]) for k in range(3))
+
+
+def make_element(bitype, v, mapstp=1, length=0.0, zedge=0.0, rf: RFData | None = None):
+    """An or_element; v = the element's values (a lone number is a
+    quadrupole gradient)."""
+    e = OrElement()
+    e.bitype, e.mapstp, e.length, e.zedge = bitype, mapstp, length, zedge
+    for k, x in enumerate(element_values(v)):
+        e.v[k] = x
+    if rf is not None:
+        e.rf = C.pointer(rf.c)
+    return e
+
+
 class OrState(C.Structure):
     _fields_ = [("bcurr", C.c_double), ("current", C.c_double),
                 ("mass", C.c_double), ("charge", C.c_double),
@@ -56,12 +107,12 @@ def lib() -> C.CDLL:
         L.or_physconst.argtypes = [C.c_double, C.POINTER(OrConst)]
         L.or_quad_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_double,
                                   C.c_int, _dp, C.c_double, C.c_double, _dp]
-        L.or_element_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_int, _dp,
-                                     C.c_int, _dp, C.c_double, C.c_double, _dp]
+        L.or_element_map.argtypes = [C.POINTER(OrConst), C.c_double, C.c_double,
+                                     C.POINTER(OrElement), _dp, C.c_double, C.c_double,
+                                     _dp]
         L.or_element_map.restype = C.c_int
-        L.or_map1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp,
-                              C.c_double, C.c_int, _dp, C.c_int,
-                              C.c_double, C.c_double]
+        L.or_map1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double,
+                              C.POINTER(OrElement), C.c_double, C.c_double]
         L.or_map1.restype = C.c_int
         L.or_conv1st.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp,
                                  C.c_double, C.c_double, _dp, _i64p, _dp]
@@ -86,8 +137,7 @@ def lib() -> C.CDLL:
                                   C.c_int64, _dp, C.c_double, _dp]
         L.or_step.argtypes = [C.POINTER(OrConst), _dp, C.c_int64,
                               C.POINTER(OrState), C.c_int, C.c_int, C.c_int,
-                              C.c_double, C.c_int, _dp, C.c_int,
-                              C.c_double, _dp, _dp]
+                              C.c_double, C.POINTER(OrElement), C.c_double, _dp, _dp]
         L.or_step.restype = C.c_int64
         L.or_drift2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double]
         L.or_elem_field.argtypes = [C.POINTER(OrConst), C.c_int, _dp, C.c_double,
@@ -99,8 +149,8 @@ def lib() -> C.CDLL:
                                C.c_double, C.c_double]
         L.or_kick2.restype = C.c_int
         L.or_step2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.POINTER(OrState),
-                               C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
-                               C.c_double, C.c_double, C.c_double, _dp, _dp]
+                               C.c_int, C.c_int, C.c_int, C.c_double,
+                               C.POINTER(OrElement), C.c_double, _dp, _dp]
         L.or_step2.restype = C.c_int64
         _lib = L
     return _lib
@@ -125,33 +175,37 @@ def quad_map(c: OrConst, tau, grad, mapstp, refpt, charge, mass):
 
 
 def element_values(v) -> np.ndarray:
-    """v1..v4 of an element (a lone number is a quadrupole gradient)."""
+    """v1..v14 of an element, zero-padded (a lone number is a quadrupole
+    gradient)."""
+    out = np.zeros(NVAL)
     if np.isscalar(v):
-        return np.array([float(v), 0.0, 0.0, 0.0])
-    out = np.zeros(4)
-    vals = np.asarray(v, dtype=np.float64).ravel()[:4]
+        out[0] = float(v)
+        return out
+    vals = np.asarray(v, dtype=np.float64).ravel()[:NVAL]
     out[: vals.size] = vals
     return out
 
 
-def element_map(c, tau, bitype, v, mapstp, refpt, charge, mass):
+def element_map(c, tau, bitype, v, mapstp, refpt, charge, mass, t=0.0, length=0.0,
+                zedge=0.0, rf: RFData | None = None):
     refpt = np.ascontiguousarray(refpt, dtype=np.float64).copy()
     xm = np.zeros(36)
-    vv = element_values(v)
-    if lib().or_element_map(C.byref(c), tau, bitype, _p(vv), mapstp, _p(refpt),
-                            charge, mass, _p(xm)) != 0:
+    e = make_element(bitype, v, mapstp, length, zedge, rf)
+    if lib().or_element_map(C.byref(c), t, tau, C.byref(e), _p(refpt), charge, mass,
+                            _p(xm)) != 0:
         raise ValueError(f"no linear map for element type {bitype}")
     return xm.reshape(6, 6), refpt
 
 
-def map1(c, pts, refpt, z, tau, bitype, v, mapstp, charge, mass):
-    """drift1_BeamBunch in place on pts; v = the element's v1..v4 (or a
+def map1(c, pts, refpt, z, tau, bitype, v, mapstp, charge, mass, length=0.0, zedge=0.0,
+         rf: RFData | None = None):
+    """drift1_BeamBunch in place on pts; v = the element's values (or a
     quadrupole gradient).  Returns the new z."""
     zz = C.c_double(z)
-    vv = element_values(v)
+    e = make_element(bitype, v, mapstp, length, zedge, rf)
     if lib().or_map1(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz),
-                     tau, bitype, _p(vv), mapstp, charge, mass) != 0:
-        raise ValueError(f"element type {bitype} with values {vv} is not restated")
+                     tau, C.byref(e), charge, mass) != 0:
+        raise ValueError(f"element type {bitype} with values {list(e.v)} is not restated")
     return zz.value
 
 
@@ -271,6 +325,7 @@ class Run:
         self.flagmap = flagmap
         self.zedge = 0.0
         self._left = 0   # segments left in the current element
+        self._rf = {}
         self.c = physconst(beam.freq)
         self.pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
         self.n = self.pts.shape[0]
@@ -294,26 +349,29 @@ class Run:
             for _ in range(e.nseg):
                 yield e
 
+    def rfdata(self, e):
+        """The element's field arrays (cached per element)."""
+        if e.field is None:
+            return None
+        key = id(e.field)
+        if key not in self._rf:
+            self._rf[key] = (RFData(e.field, fourier=e.type == 105), e.field)
+        return self._rf[key][0]
+
     def step(self, e, want_fields=False):
         nx, ny, nz = self.grid
         rho = np.zeros((nz, ny, nx)) if want_fields else None
         phi = np.zeros((nz, ny, nx)) if want_fields else None
         tau1 = 0.5 * e.length / e.nseg
-        vv = element_values([e.v1, e.v2, e.v3, e.v4])
-        if self.flagmap == 1:
-            n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
-                              nx, ny, nz, tau1, e.type, _p(vv), e.mapstp, e.radius,
-                              _p(rho) if want_fields else None,
-                              _p(phi) if want_fields else None)
-        else:
-            if self._left == 0:
-                # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
-                self._left, self.zedge = e.nseg, self.st.z
-            self._left -= 1
-            n = lib().or_step2(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st),
-                               nx, ny, nz, tau1, e.type, _p(vv), e.length, self.zedge,
-                               e.radius, _p(rho) if want_fields else None,
-                               _p(phi) if want_fields else None)
+        if self._left == 0:
+            # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
+            self._left, self.zedge = e.nseg, self.st.z
+        self._left -= 1
+        el = make_element(e.type, e.values, e.mapstp, e.length, self.zedge, self.rfdata(e))
+        step = lib().or_step if self.flagmap == 1 else lib().or_step2
+        n = step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx, ny, nz, tau1,
+                 C.byref(el), e.radius, _p(rho) if want_fields else None,
+                 _p(phi) if want_fields else None)
         if n < 0:
             raise RuntimeError("or_step failed")
         self.n = n

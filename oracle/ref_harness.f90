@@ -5,7 +5,9 @@
 ! space-charge step sequence exactly in the order of the reference's
 ! segment loop (src/Contrl/AccSimulator.f90:1099-1488, Flagmap=1,
 ! Flagbc=1, Flagsubstep=0, Flagerr=0; drift, quadrupole, constant-focusing
-! and solenoid elements),
+! and solenoid elements and the RF cavities DTL, CCDTL, CCL, SC and SolRF,
+! whose on-axis field comes from rfdataN.in through the reference's own
+! read1_Data / read3_Data, as run_AccSimulator reads it, :818-849),
 ! writing the intermediate quantities the reference never writes itself:
 ! the particle array after every stage, the conv1st range, the updated
 ! domain, the charge density and the potential.
@@ -16,12 +18,15 @@
 !   curr kinetic_eV mass_eV charge freq phase0
 !   flagmap (1 linear maps, 2 nonlinear Lorentz integrator)
 !   nelem
-!   nelem lines:  length nseg mapstp type v1 v2 v3 v4
-!                 (the deck line's values, src/Contrl/AccSimulator.f90:
-!                  262-320: type 0: v1 = pipe radius; type 1: v1 = gradient
-!                  T/m (or K, 1/m^2, when v2 < 0), v2 = file ID, v3 = pipe
-!                  radius; type 2: v1..v3 = kx0^2, ky0^2, kz0^2, v4 = pipe
-!                  radius; type 3: v1 = Bz0, v2 = file ID, v3 = pipe radius)
+!   nelem lines:  length nseg mapstp type v1 ... v14
+!                 (the deck line's values = Param(2:15),
+!                  src/Contrl/AccSimulator.f90:262-536: type 0: v1 = pipe
+!                  radius; type 1: v1 = gradient T/m (or K, 1/m^2, when
+!                  v2 < 0), v2 = file ID, v3 = pipe radius; type 2: v1..v3 =
+!                  kx0^2, ky0^2, kz0^2, v4 = pipe radius; type 3: v1 = Bz0,
+!                  v2 = file ID, v3 = pipe radius; types 101-105: v1 = field
+!                  scale, v2 = RF frequency, v3 = phase, v4 = file ID,
+!                  v5 = pipe radius, v6.. as the element class lists them)
 !   nsteps ndump
 !   ndump step indices (1-based) to dump
 !   particle file name (stream: int32 N, then Pts1(9,N) doubles)
@@ -39,6 +44,7 @@
         use Outputclass
         use PhysConstclass
         use NumConstclass
+        use Dataclass
         implicit none
         include 'mpif.h'
         type (Pgrid2d) :: grid2d
@@ -49,6 +55,11 @@
         type (Quadrupole), target, dimension(:), allocatable :: quads
         type (ConstFoc), target, dimension(:), allocatable :: cfs
         type (Sol), target, dimension(:), allocatable :: sols
+        type (DTL), target, dimension(:), allocatable :: dtls
+        type (CCDTL), target, dimension(:), allocatable :: ccdtls
+        type (CCL), target, dimension(:), allocatable :: ccls
+        type (SC), target, dimension(:), allocatable :: scs
+        type (SolRF), target, dimension(:), allocatable :: slrfs
         type (BeamLineElem), dimension(:), allocatable :: Blnelem
         integer :: N, Nx, Ny, Nz, nelem, nsteps, ndump, ierr, i, j, k
         integer :: bnseg, bmpstp, bitype, ihlf, nstep, idr, iqr, icf, isl
@@ -56,7 +67,10 @@
         integer, dimension(:), allocatable :: etype, enseg, empstp, dumpat
         integer, dimension(1) :: nptlist0, idchg
         double precision, dimension(1) :: qmcclist0
-        double precision, dimension(:), allocatable :: elen, ev1, ev2, ev3, ev4
+        double precision, dimension(:), allocatable :: elen
+        double precision, dimension(:,:), allocatable :: ev
+        double precision, dimension(25) :: tmprf
+        integer :: ifile, tmpfile
         double precision :: curr, kin, mass, chg, freq, ph0, gambet
         double precision :: z, tau1, tau2, blength, piperad, piperad2
         double precision, dimension(21) :: distparam
@@ -79,10 +93,9 @@
         read(11,*) flagmap
         read(11,*) nelem
         allocate(elen(nelem), enseg(nelem), empstp(nelem), etype(nelem))
-        allocate(ev1(nelem), ev2(nelem), ev3(nelem), ev4(nelem))
+        allocate(ev(14, nelem))
         do i = 1, nelem
-          read(11,*) elen(i), enseg(i), empstp(i), etype(i), ev1(i), &
-                     ev2(i), ev3(i), ev4(i)
+          read(11,*) elen(i), enseg(i), empstp(i), etype(i), ev(1:14, i)
         enddo
         read(11,*) nsteps, ndump
         allocate(dumpat(max(ndump,1)))
@@ -95,6 +108,7 @@
         call construct_Timer(0.0d0)
         call construct_Pgrid2d(grid2d, MPI_COMM_WORLD, 1, 1)
         call construct_PhysConst(freq)
+        call init_Data()
         distparam = 0.0d0
         distparam(1) = 1.0d-3
         distparam(2) = 1.0d-3
@@ -126,6 +140,7 @@
         ! lattice: one target object per element, as init_AccSimulator does
         allocate(drifts(nelem), quads(nelem), Blnelem(nelem))
         allocate(cfs(nelem), sols(nelem))
+        allocate(dtls(nelem), ccdtls(nelem), ccls(nelem), scs(nelem), slrfs(nelem))
         idr = 0
         iqr = 0
         icf = 0
@@ -136,7 +151,7 @@
             call construct_DriftTube(drifts(idr), enseg(i), empstp(i), &
                  etype(i), elen(i))
             tmpdr = 0.0d0
-            tmpdr(2) = ev1(i)
+            tmpdr(2) = ev(1,i)
             call setparam_DriftTube(drifts(idr), tmpdr(1:2))
             Blnelem(i) = assign_BeamLineElem(drifts(idr))
           else if (etype(i) == 1) then
@@ -144,9 +159,7 @@
             call construct_Quadrupole(quads(iqr), enseg(i), empstp(i), &
                  etype(i), elen(i))
             tmpquad = 0.0d0
-            tmpquad(2) = ev1(i)
-            tmpquad(3) = ev2(i)
-            tmpquad(4) = ev3(i)
+            tmpquad(2:4) = ev(1:3,i)
             call setparam_Quadrupole(quads(iqr), tmpquad)
             Blnelem(i) = assign_BeamLineElem(quads(iqr))
           else if (etype(i) == 2) then
@@ -154,10 +167,7 @@
             call construct_ConstFoc(cfs(icf), enseg(i), empstp(i), &
                  etype(i), elen(i))
             tmpcf = 0.0d0
-            tmpcf(2) = ev1(i)
-            tmpcf(3) = ev2(i)
-            tmpcf(4) = ev3(i)
-            tmpcf(5) = ev4(i)
+            tmpcf(2:5) = ev(1:4,i)
             call setparam_ConstFoc(cfs(icf), tmpcf)
             Blnelem(i) = assign_BeamLineElem(cfs(icf))
           else if (etype(i) == 3) then
@@ -165,14 +175,36 @@
             call construct_Sol(sols(isl), enseg(i), empstp(i), &
                  etype(i), elen(i))
             tmpquad = 0.0d0
-            tmpquad(2) = ev1(i)
-            tmpquad(3) = ev2(i)
-            tmpquad(4) = ev3(i)
-            tmpquad(5) = ev4(i)
+            tmpquad(2:5) = ev(1:4,i)
             call setparam_Sol(sols(isl), tmpquad)
             Blnelem(i) = assign_BeamLineElem(sols(isl))
+          else if (etype(i) >= 101 .and. etype(i) <= 105) then
+            ! Param(2:15) = v1..v14 (src/Contrl/AccSimulator.f90:375-490)
+            tmprf = 0.0d0
+            tmprf(2:15) = ev(1:14,i)
+            if (etype(i) == 101) then
+              call construct_DTL(dtls(i), enseg(i), empstp(i), etype(i), elen(i))
+              call setparam_DTL(dtls(i), tmprf)
+              Blnelem(i) = assign_BeamLineElem(dtls(i))
+            else if (etype(i) == 102) then
+              call construct_CCDTL(ccdtls(i), enseg(i), empstp(i), etype(i), elen(i))
+              call setparam_CCDTL(ccdtls(i), tmprf)
+              Blnelem(i) = assign_BeamLineElem(ccdtls(i))
+            else if (etype(i) == 103) then
+              call construct_CCL(ccls(i), enseg(i), empstp(i), etype(i), elen(i))
+              call setparam_CCL(ccls(i), tmprf)
+              Blnelem(i) = assign_BeamLineElem(ccls(i))
+            else if (etype(i) == 104) then
+              call construct_SC(scs(i), enseg(i), empstp(i), etype(i), elen(i))
+              call setparam_SC(scs(i), tmprf)
+              Blnelem(i) = assign_BeamLineElem(scs(i))
+            else
+              call construct_SolRF(slrfs(i), enseg(i), empstp(i), etype(i), elen(i))
+              call setparam_SolRF(slrfs(i), tmprf)
+              Blnelem(i) = assign_BeamLineElem(slrfs(i))
+            endif
           else
-            print*, "harness supports element types 0 to 3 only"
+            print*, "harness supports element types 0 to 3 and 101 to 105 only"
             stop
           endif
         enddo
@@ -183,12 +215,25 @@
         allocate(chgdens(1,1,1))
         z = 0.0d0
         nstep = 0
+        tmpfile = 0
         call diagnostic1_Output(z, Bpts, nchrg, nptlist0)
 
         do i = 1, nelem
           call getparam_BeamLineElem(Blnelem(i), blength, bnseg, bmpstp, &
                                      bitype)
           call getradius_BeamLineElem(Blnelem(i), piperad, piperad2)
+          ! the on-axis field of an RF cavity (src/Contrl/AccSimulator.f90:818-849)
+          if (bitype > 100) then
+            ifile = int(ev(4,i) + 0.1)
+            if (ifile /= tmpfile) then
+              if (flagmap == 1 .and. bitype /= 105 .and. bitype /= 106) then
+                call read1_Data(ifile)
+              else
+                call read3_Data(ifile)
+              endif
+              tmpfile = ifile
+            endif
+          endif
           ! the element's entrance z (src/Contrl/AccSimulator.f90:1025-1026)
           call setparam_BeamLineElem(Blnelem(i), 1, z)
           tau1 = 0.0d0
@@ -209,8 +254,7 @@
               call wpts(u)
             endif
             if (flagmap == 1) then
-              call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
-                                  bnseg, j, ihlf)
+              call push1(i, j)
             else
               call map1_BeamBunch(Bpts, z, tau2)
             endif
@@ -253,8 +297,7 @@
             endif
             if (dodump) call wpts(u)
             if (flagmap == 1) then
-              call map1_BeamBunch(Bpts, Blnelem(i), z, tau1, bitype, &
-                                  bnseg, j, ihlf)
+              call push1(i, j)
             else
               call map1_BeamBunch(Bpts, z, tau2)
             endif
@@ -280,6 +323,39 @@
         close(21)
 
       contains
+        ! map1_BeamBunch with Flagmap=1 (drift1_BeamBunch).  For the DTL and
+        ! SolRF, drift1_BeamBunch's getparam_BeamLineElem(beamln,drange)
+        ! copies the element's 25 (DTL) or 15 (SolRF) Param values into its
+        ! 12-element drange (src/Appl/BeamBunch.f90:270,323), past the end of
+        ! the array; for those two types the harness makes the rest of
+        ! drift1's calls itself: the element's maplinear and the 6x6 product
+        ! of src/Appl/BeamBunch.f90:436-449, then z += tau.
+        subroutine push1(ie, js)
+          integer, intent(in) :: ie, js
+          double precision, dimension(6,6) :: xm
+          double precision, dimension(6) :: tmp
+          integer :: ip, jj, kk
+          if (bitype == 101 .or. bitype == 105) then
+            call maplinear_BeamLineElem(Blnelem(ie), z, tau1, xm, Bpts%refptcl, &
+                                        Bpts%Charge, Bpts%Mass)
+            do ip = 1, Bpts%Nptlocal
+              do jj = 1, 6
+                tmp(jj) = 0.0
+                do kk = 1, 6
+                  tmp(jj) = tmp(jj) + Bpts%Pts1(kk,ip)*xm(jj,kk)
+                enddo
+              enddo
+              do jj = 1, 6
+                Bpts%Pts1(jj,ip) = tmp(jj)
+              enddo
+            enddo
+            z = z + tau1
+            ihlf = ihlf + 1
+          else
+            call map1_BeamBunch(Bpts, Blnelem(ie), z, tau1, bitype, bnseg, js, ihlf)
+          endif
+        end subroutine push1
+
         subroutine wpts(uu)
           integer, intent(in) :: uu
           integer :: nn

@@ -27,10 +27,28 @@ def fixture_beam(d):
     return Beam(current=cur, kinetic=kin, mass=mass, charge=chg, freq=freq, phase0=ph)
 
 
-def fixture_lattice(d):
+def fixture_lattice(d, key="lattice"):
+    """The lattice of a fixture: rows of length, nseg, mapstp, type and the
+    values, with RF fields stored as field<i>."""
     from impactz.beam import Element
-    return [Element(float(r[0]), int(r[1]), int(r[2]), int(r[3]),
-                    *(float(v) for v in r[4:8])) for r in d["lattice"]]
+    out = []
+    for i, r in enumerate(d[key]):
+        v = [float(x) for x in r[4:]]
+        fk = f"field{i}"
+        out.append(Element(float(r[0]), int(r[1]), int(r[2]), int(r[3]), *v[:4],
+                           more=tuple(v[4:]), field=d[fk] if fk in d else None))
+    return out
+
+
+def element_entrance(d, step):
+    """z of the entrance of the element the 1-based step belongs to."""
+    z, k = 0.0, 0
+    for e in fixture_lattice(d):
+        if k + e.nseg >= step:
+            return z
+        k += e.nseg
+        z += e.length
+    raise IndexError(step)
 
 
 def segment(d, step):

@@ -31,6 +31,14 @@ from tests.golden import refdump as R  # noqa: E402
 STAGES = ("p0", "p_map1a", "p_conv1st", "p_cvbk", "p_kick", "p_map1b")
 
 
+def lattice_array(lattice):
+    """length, nseg, mapstp, type, then v1..v4 (or v1..v14 when an element
+    has more values), one row per element."""
+    wide = any(e.more for e in lattice)
+    return np.array([[e.length, e.nseg, e.mapstp, e.type,
+                      *(e.values if wide else e.values[:4])] for e in lattice])
+
+
 def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
             beam=B.Beam(), stages=STAGES, flagmap=1):
     pts = B.gaussian_bunch(n, seed=seed, beam=beam)
@@ -41,8 +49,7 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
         "grid": np.array(grid),
         "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
                           beam.freq, beam.phase0]),
-        "lattice": np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2,
-                              e.v3, e.v4] for e in lattice]),
+        "lattice": lattice_array(lattice),
         "nsteps": np.array(nsteps),
         "dump_steps": np.array(dump_steps),
         "final": ref["final"],
@@ -50,6 +57,9 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
         "final_current": np.array(ref["final_current"]),
         "final_z": np.array(ref["final_z"]),
     }
+    for i, e in enumerate(lattice):
+        if e.field is not None:
+            out[f"field{i}"] = np.asarray(e.field, dtype=np.float64)
     for k in (18, 24, 25, 26, 27, 29, 30):
         out[f"fort{k}"] = ref[f"fort{k}"]
     for s in dump_steps:
@@ -161,6 +171,42 @@ def io_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
     return out
 
 
+def rf_table(length, n=201, z0=0.05):
+    """A half-sine gap field sampled for read1_Data: rows z, Ez, Ez', Ez''
+    (unit amplitude; the element's Param(2) scales it).  z starts at z0 so
+    the reader's shift to 0 is exercised."""
+    s = np.linspace(0.0, length, n)
+    k = np.pi / length
+    return np.stack([s + z0, np.sin(k * s), k * np.cos(k * s), -k * k * np.sin(k * s)],
+                    axis=1)
+
+
+def rf_fourier(length):
+    """read3_Data coefficients: count, field start, end and period, then
+    a0, a1, b1, a2, b2, a3, b3 (an even count, so the reader pads it)."""
+    return np.array([7.0, 0.0, length, length, 0.4, -1.0, 0.05, 0.3, -0.02, -0.05, 0.01])
+
+
+def rf_lattice():
+    """Every RF cavity class on the linear-map path: DTL with its two
+    quadrupoles, CCDTL, CCL and SC share one field table (file 1), SolRF
+    with Bz0 = 0.4 T takes Fourier coefficients (file 2)."""
+    L = 0.2345
+    tab, four = rf_table(L), rf_fourier(0.2)
+    rf = (5.0e6, 324.0e6)
+    return [B.Element(0.1, 4, 10, 0, 1.0),
+            B.Element(L, 6, 20, 101, rf[0], rf[1], -30.0, 1.0,
+                      (1.0, 0.03, 12.0, 0.03, -12.0), field=tab),
+            B.Element(0.1, 2, 10, 0, 1.0),
+            B.Element(L, 4, 20, 102, rf[0], rf[1], 20.0, 1.0, (1.0,), field=tab),
+            B.Element(L, 4, 20, 103, 0.8 * rf[0], rf[1], -60.0, 1.0, (1.0,), field=tab),
+            B.Element(L, 4, 20, 104, rf[0], rf[1], 0.0, 1.0, (1.0,), field=tab),
+            B.Element(0.1, 2, 10, 0, 1.0),
+            B.Element(0.2, 6, 20, 105, 3.0e6, rf[1], 45.0, 2.0,
+                      (1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.4), field=four),
+            B.Element(0.1, 4, 10, 0, 1.0)]
+
+
 def switch_lattice():
     """Linear maps through a FODO half-period, a type -25 element switching
     to the Lorentz integrator (Flagmap = its mapstp, 2), then constant
@@ -234,6 +280,10 @@ def main():
     io_fixture()
     meta["io_ckpt"] = ("ImpactZexe runs: -2 phase dump and -7 checkpoint after 8 steps, "
                        "the run restarted from that checkpoint, and Flagdist=199 binary input")
+    fixture("rf", 1000, (16, 16, 16), rf_lattice(), 36, (6, 22, 30))
+    meta["rf"] = ("RF cavities on the linear-map path: DTL (with quadrupoles), CCDTL, CCL, "
+                  "SC on a field table (rfdata1.in), SolRF on Fourier coefficients "
+                  "(rfdata2.in); dumps at steps 6 (DTL), 22 (SC) and 30 (SolRF)")
     exe_nonlinear_fixture()
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "

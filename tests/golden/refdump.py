@@ -82,6 +82,7 @@ def read_fort(path: str) -> np.ndarray:
 def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
                 dump_steps=(), workdir: str | None = None, flagmap: int = 1) -> dict:
     """Run the reference on ``pts`` through ``lattice``; return dumps and diagnostics."""
+    from impactz import beam as B
     nx, ny, nz = grid
     n = pts.shape[0]
     tmp = workdir or tempfile.mkdtemp(prefix="impz_ref_")
@@ -95,8 +96,10 @@ def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
         f.write(f"{flagmap}\n")
         f.write(f"{len(lattice)}\n")
         for e in lattice:
-            f.write(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} {e.v1!r} "
-                    f"{e.v2!r} {e.v3!r} {e.v4!r}\n")
+            f.write(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} " +
+                    " ".join(repr(float(v)) for v in e.values) + "\n")
+            if e.field is not None:
+                B.write_rfdata(os.path.join(tmp, B.rfdata_name(int(e.v4 + 0.1))), e.field)
         f.write(f"{nsteps} {len(dump_steps)}\n")
         if dump_steps:
             f.write(" ".join(str(s) for s in dump_steps) + "\n")

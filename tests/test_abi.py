@@ -7,7 +7,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import ROOT, fixture_beam, load_golden, segment
+from conftest import ROOT, fixture_beam, fixture_lattice, load_golden, segment
 
 
 def header_symbols():
@@ -85,6 +85,48 @@ def test_element_maplinear_matches_oracle(oracle, bitype, v):
     assert r1[4] == pytest.approx(r2[4], rel=1e-15)
 
 
+@pytest.mark.parametrize("idx", [1, 3, 4, 5, 7])
+@pytest.mark.parametrize("z_in", [0.0, 0.031, 0.2])
+def test_rf_maplinear_matches_oracle(oracle, idx, z_in):
+    """RF cavity maps of the library (DTL, CCDTL, CCL, SC on the field table;
+    SolRF on the Fourier series) against the oracle, at the entrance, inside
+    the DTL's first quadrupole and past the field's end."""
+    from impactz import hip
+    from conftest import fixture_beam
+    d = load_golden("rf")
+    bm = fixture_beam(d)
+    e = fixture_lattice(d)[idx]
+    zedge = 1.0
+    refpt = np.zeros(6)
+    refpt[4], refpt[5] = 2.3, -bm.gamma0
+    tau = 0.5 * e.length / e.nseg
+    xm, r1 = hip.element_maplinear(hip.physconst(bm.freq), tau, e.type, e.values, e.mapstp,
+                                   refpt, bm.charge, bm.mass, zedge + z_in, e.length, zedge,
+                                   hip.RFData.for_element(e))
+    xo, r2 = oracle.element_map(oracle.physconst(bm.freq), tau, e.type, e.values, e.mapstp,
+                                refpt, bm.charge, bm.mass, zedge + z_in, e.length, zedge,
+                                oracle.RFData(e.field, fourier=e.type == 105))
+    assert np.allclose(xm, xo, rtol=1e-13, atol=1e-16)
+    assert np.array_equal(xm == 0.0, xo == 0.0)
+    assert np.allclose(r1, r2, rtol=1e-15)
+
+
+def test_rfdata_is_checked():
+    from impactz import hip
+    with pytest.raises(hip.IZHError, match="announces"):
+        hip.RFData(np.array([9.0, 0.0, 1.0, 1.0, 0.5, 0.2]), fourier=True)
+    tab = np.array([[0.0, 1.0, 0.0, 0.0], [1.0, 1.0, 0.0, 0.0]])
+    c = hip.physconst(324e6)
+    r = np.zeros(6)
+    r[5] = -1.16
+    v = [1e6, 324e6, 0.0, 1.0, 1.0]
+    with pytest.raises(hip.IZHError, match="needs its field"):
+        hip.element_maplinear(c, 0.01, 103, v, 10, r, 1.0, 938e6, length=1.0)
+    with pytest.raises(hip.IZHError, match="Fourier"):
+        hip.element_maplinear(c, 0.01, 105, v, 10, r, 1.0, 938e6, length=1.0,
+                              rf=hip.RFData(tab, fourier=False))
+
+
 def test_element_maplinear_refuses_other_elements():
     from impactz import hip
     c = hip.physconst(324e6)
@@ -94,6 +136,8 @@ def test_element_maplinear_refuses_other_elements():
         hip.element_maplinear(c, 0.01, 1, [1.0, 1.0, 0.0, 0.0], 10, r, 1.0, 938e6)
     with pytest.raises(hip.IZHError, match="no linear map"):
         hip.element_maplinear(c, 0.01, 4, [1.0, 0.0, 0.0, 0.0], 10, r, 1.0, 938e6)
+    with pytest.raises(hip.IZHError, match="no linear map"):
+        hip.element_maplinear(c, 0.01, 106, [1.0, 324e6, 0.0, 1.0], 10, r, 1.0, 938e6)
 
 
 @pytest.mark.parametrize("name,step", [("fodo_16cube", 1), ("fodo_16cube", 5),

@@ -82,7 +82,7 @@ def test_deck_written_for_the_reference_reads_back(tmp_path):
 
 
 @pytest.mark.parametrize("line, what", [
-    ("0.1 4 10 101 2503370.0 3.24e8 222 852.0 0.0065 /", "type 101"),
+    ("0.1 4 10 106 2503370.0 3.24e8 222 852.0 0.0065 /", "type 106"),
     ("0.1 4 10 1 10.0 2.0 0.15 /", "field file ID"),
     ("0.1 4 10 1 10.0 -20.0 0.15 /", "undulator"),
     ("0.1 4 10 2 30.0 20.0 5.0 -1.0 /", "ideal RF"),
@@ -144,7 +144,7 @@ def test_deck_linear_elements(tmp_path):
                "0.1 4 10 1 5.4 -1.0 0.04 /\n")
     lat, notes = lattice_of(read_deck(str(p)))
     assert [e.type for e in lat] == [2, 3, 1]
-    assert lat[0].values == [30.0, 20.0, 5.0, 0.02] and lat[0].radius == 0.02
+    assert lat[0].values[:4] == [30.0, 20.0, 5.0, 0.02] and lat[0].radius == 0.02
     assert lat[1].values[:3] == [1.5, 0.0, 0.03] and lat[1].radius == 0.03
     assert lat[2].values[:3] == [5.4, -1.0, 0.04] and lat[2].radius == 0.04
     assert notes == []
@@ -177,3 +177,38 @@ def test_deck_integrator_switch(tmp_path):
         _write(p, HEADER + line + "\n")
         with pytest.raises(DeckError, match=what):
             program_of(read_deck(str(p)))
+
+
+def test_deck_rf_cavities_and_field_files(tmp_path):
+    """RF cavity lines carry v1..v14 and the pipe radius v5; their rfdataN.in
+    is read the way read1_Data / read3_Data read it (one list-directed READ
+    per row: 4 values for a table, 1 for Fourier coefficients, continuing
+    onto the next line when short and dropping the rest of a long one)."""
+    from impactz.deck import (DeckError, program_of, read_deck, read_rfdata, with_fields)
+    p = tmp_path / "ImpactZ.in"
+    _write(p, HEADER + """0.2345 6 20 101 5.0d6 324.0e6 -30.0 1.0 0.02 0.03 12.0 0.03 -12.0 /
+0.2 6 20 105 3.0e6 324.0e6 45.0 2.0 0.015 0 0 0 0 0 0.4 /
+""")
+    _write(tmp_path / "rfdata1.in", "0.1 1.0 0.0 0.0 extra\n0.2 2.0\n0.5 0.0\n0.3 3.0 1.0 2.0\n")
+    _write(tmp_path / "rfdata2.in", "7.0 ignored\n0.0\n0.2\n0.2\n0.4\n-1.0\n0.05\n")
+    prog, _ = program_of(read_deck(str(p)))
+    dtl, slrf = prog[0][1], prog[1][1]
+    assert dtl.type == 101 and dtl.values[:9] == [5e6, 324e6, -30.0, 1.0, 0.02, 0.03, 12.0,
+                                                  0.03, -12.0]
+    assert dtl.radius == 0.02 and slrf.radius == 0.015 and slrf.values[10] == 0.4
+    tab = read_rfdata(str(tmp_path / "rfdata1.in"), 4)
+    assert tab.tolist() == [[0.1, 1.0, 0.0, 0.0], [0.2, 2.0, 0.5, 0.0], [0.3, 3.0, 1.0, 2.0]]
+    coef = read_rfdata(str(tmp_path / "rfdata2.in"), 1)
+    assert coef.tolist() == [7.0, 0.0, 0.2, 0.2, 0.4, -1.0, 0.05]
+    withf = with_fields(prog, str(tmp_path))
+    assert np.array_equal(withf[0][1].field, tab) and np.array_equal(withf[1][1].field, coef)
+    for line, what in (("0.2 4 20 103 1e6 324e6 0.0 -1.0 0.02 /", "ideal RF"),
+                       ("0.2 4 20 105 1e6 324e6 0.0 2.0 0.02 0 0 0 0 0 0.4 0.01 0.02 0.03 /",
+                        "wakefield"),
+                       ("0.2 4 20 106 1e6 324e6 0.0 2.0 0.02 /", "type 106")):
+        _write(p, HEADER + line + "\n")
+        with pytest.raises(DeckError, match=what):
+            program_of(read_deck(str(p)))
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.2 4 20 103 1e6 324e6 0 1 0.02 /\n")
+    with pytest.raises(DeckError, match="Flagmap=2"):
+        program_of(read_deck(str(p)))

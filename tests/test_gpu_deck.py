@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("name,fused", [("fodo_traj", False), ("fodo_traj", True),
                                         ("kquad", False), ("cf_sol", True),
-                                        ("nl_fodo", True), ("nl_cf_sol", False)])
+                                        ("nl_fodo", True), ("nl_cf_sol", False),
+                                        ("rf", True), ("rf", False)])
 def test_deck_run_matches_reference_fort_files(tmp_path, gpu, name, fused):
     from impactz.beam import write_deck, write_particle_in
     from impactz.deck import run_deck

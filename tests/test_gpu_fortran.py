@@ -38,14 +38,17 @@ def _namelist(path, d, fused, out):
 
 @pytest.mark.parametrize("name,fused", [("fodo_traj", False), ("fodo_traj", True),
                                         ("cf_sol", False), ("cf_sol", True),
-                                        ("nl_cf_sol", False)])
+                                        ("nl_cf_sol", False), ("rf", False), ("rf", True)])
 def test_fortran_host_matches_reference(tmp_path, name, fused):
     from impactz import hip
-    from impactz.beam import write_particle_in
+    from impactz.beam import rfdata_name, write_particle_in, write_rfdata
     from impactz.sim import diag_row
     assert os.access(EXE, os.X_OK), "Fortran host not built (impactz._build.build)"
     d = load_golden(name)
     write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
+    for e in fixture_lattice(d):
+        if e.field is not None:
+            write_rfdata(str(tmp_path / rfdata_name(int(e.v4 + 0.1))), e.field)
     out = str(tmp_path / "moments.txt")
     _namelist(str(tmp_path), d, fused, out)
     print(f"fortran host {name} fused={fused}", flush=True)

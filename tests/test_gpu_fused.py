@@ -40,13 +40,14 @@ def test_fused_matches_stepwise_100k_32cube(gpu):
 
 
 @pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_aniso", "fodo_lossy",
-                                  "cf_sol"])
+                                  "cf_sol", "rf"])
 def test_fused_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
     sim = Simulation(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
     nsteps = int(d["nsteps"])
+    print(f"fused run {name}", flush=True)
     rec = sim.run_fused(nsteps)
     worst = 0.0
     for k in range(nsteps):
@@ -60,6 +61,8 @@ def test_fused_matches_reference(gpu, name):
     record(test="fused_vs_reference", case=name, steps=nsteps, moments=worst,
            particles=pe, current=ce)
     assert worst < TOL and pe < TOL and ce < 1e-14
+    # the reference energy of every step (RF cavities change it)
+    assert np.allclose(-rec[:, 46], d["fort18"][1:nsteps + 1, 2], rtol=1e-13)
     # Npt of each step's moments follows the reference's losses
     if name == "fodo_lossy":
         assert rec[-1, 44] == d["final"].shape[0]

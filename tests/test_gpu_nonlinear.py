@@ -7,24 +7,14 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import col_err, diag_row_err, fixture_beam, fixture_lattice, load_golden, \
-    rel_err, segment
+from conftest import col_err, diag_row_err, element_entrance, fixture_beam, \
+    fixture_lattice, load_golden, rel_err, segment
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
 
 CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
          ("nl_cf_sol", 11)]
-
-
-def _zedge(d, step):
-    z, k = 0.0, 0
-    for e in fixture_lattice(d):
-        if k + e.nseg >= step:
-            return z
-        k += e.nseg
-        z += e.length
-    raise ValueError(step)
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -55,7 +45,7 @@ def test_gpu_nonlinear_stages_match_reference(gpu, name, step):
     field.update3O(dom)
     err["phi"] = rel_err(field.download(hip.PHI), d[s + "phi"])
     b.cvbkforth1st()
-    b.kick2(z, 2 * tau1, hip.element(e.type, e.values, e.length, _zedge(d, step)), field,
+    b.kick2(z, 2 * tau1, hip.element(e.type, e.values, e.length, element_entrance(d, step)), field,
             dom)
     err["kick2"] = col_err(b.getpts(), d[s + "p_kick"])
     z = b.drift2(z, 2 * tau1)

@@ -10,8 +10,8 @@ import os
 import numpy as np
 import pytest
 
-from conftest import (ROOT, col_err, diag_row_err, fixture_beam, fixture_lattice,
-                      load_golden, rel_err, segment)
+from conftest import (ROOT, col_err, diag_row_err, element_entrance, fixture_beam,
+                      fixture_lattice, load_golden, rel_err, segment)
 
 pytestmark = pytest.mark.gpu
 
@@ -32,7 +32,7 @@ def record(**kw):
 
 CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
          ("fodo_lossy", 1), ("fodo_lossy", 2), ("cf_sol", 2), ("cf_sol", 10),
-         ("kquad", 3)]
+         ("kquad", 3), ("rf", 6), ("rf", 22), ("rf", 30)]
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -53,8 +53,10 @@ def test_gpu_stages_match_reference(gpu, name, step):
     hip.check(hip.lib().izh_bunch_set_info(b.h, ctypes.byref(info)))
     field = hip.FieldQuant(*grid)
     dom = hip.CompDom(*grid)
+    rf, zedge = hip.RFData.for_element(e), element_entrance(d, step)
+    print(f"stages {name}:{step}", flush=True)
     err = {}
-    z = b.map1(float(d[s + "z0"]), tau1, e.type, e.values, e.mapstp)
+    z = b.map1(float(d[s + "z0"]), tau1, e.type, e.values, e.mapstp, e.length, zedge, rf)
     err["map1a"] = col_err(b.getpts(), d[s + "p_map1a"])
     pr, nplc, nptot = b.conv1st(2 * tau1, e.radius, e.radius)
     assert nplc == nptot == d[s + "p_conv1st"].shape[0]
@@ -72,7 +74,7 @@ def test_gpu_stages_match_reference(gpu, name, step):
     err["cvbk"] = col_err(b.getpts(), d[s + "p_cvbk"])
     b.kick1(2 * tau1, field, dom)
     err["kick"] = col_err(b.getpts(), d[s + "p_kick"])
-    z = b.map1(z, tau1, e.type, e.values, e.mapstp)
+    z = b.map1(z, tau1, e.type, e.values, e.mapstp, e.length, zedge, rf)
     err["map1b"] = col_err(b.getpts(), d[s + "p_map1b"])
     err["refpt"] = rel_err(b.refpt, d[s + "refpt"])
     assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
@@ -100,7 +102,7 @@ def test_gpu_solver_matches_direct_convolution(gpu, oracle):
 
 
 @pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy", "cf_sol",
-                                  "kquad"])
+                                  "kquad", "rf"])
 def test_gpu_trajectory_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)

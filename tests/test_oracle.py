@@ -13,11 +13,17 @@ import math
 import numpy as np
 import pytest
 
-from conftest import col_err, fixture_beam, fixture_lattice, load_golden, rel_err, segment
+from conftest import (col_err, element_entrance, fixture_beam, fixture_lattice, load_golden,
+                      rel_err, segment)
 
 CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
          ("fodo_lossy", 1), ("fodo_lossy", 2), ("cf_sol", 2), ("cf_sol", 10),
-         ("kquad", 3)]
+         ("kquad", 3), ("rf", 6), ("rf", 22), ("rf", 30)]
+
+
+def _rf(oracle, e):
+    """The oracle's copy of an RF cavity's field file."""
+    return None if e.field is None else oracle.RFData(e.field, fourier=e.type == 105)
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -29,11 +35,12 @@ def test_oracle_stages_match_reference(oracle, name, step):
     s = f"s{step}_"
     c = oracle.physconst(beam.freq)
     tau1 = 0.5 * e.length / e.nseg
+    rf, zedge = _rf(oracle, e), element_entrance(d, step)
     # drift1 (first half)
     pts = d[s + "p0"].copy()
     refpt = d[s + "refpt0"].copy()
     z = oracle.map1(c, pts, refpt, float(d[s + "z0"]), tau1, e.type, e.values,
-                    e.mapstp, beam.charge, beam.mass)
+                    e.mapstp, beam.charge, beam.mass, e.length, zedge, rf)
     assert col_err(pts, d[s + "p_map1a"]) < 1e-14
     # conv1st
     pts = d[s + "p_map1a"].copy()
@@ -66,14 +73,14 @@ def test_oracle_stages_match_reference(oracle, name, step):
     # drift1 (second half)
     pts = d[s + "p_kick"].copy()
     z = oracle.map1(c, pts, refpt, z, tau1, e.type, e.values, e.mapstp,
-                    beam.charge, beam.mass)
+                    beam.charge, beam.mass, e.length, zedge, rf)
     assert col_err(pts, d[s + "p_map1b"]) < 1e-14
     assert z == pytest.approx(float(d[s + "z"]), rel=1e-15)
     assert rel_err(refpt, d[s + "refpt"]) < 1e-15
 
 
 @pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_lossy", "cf_sol",
-                                  "kquad"])
+                                  "kquad", "rf"])
 def test_oracle_trajectory_matches_reference(oracle, name):
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
@@ -476,3 +483,50 @@ def test_kick2_lorentz_push_properties(oracle):
     want = -g * (pts[:, 0] * c.scxl) * tau / brho * b.gambet0
     assert np.allclose(dpx, want, rtol=2e-2, atol=1e-12)
     del qm
+
+
+def _const_table(e0=1.0):
+    return np.array([[0.0, e0, 0.0, 0.0], [1.0, e0, 0.0, 0.0]])
+
+
+@pytest.mark.parametrize("bitype", [101, 102, 103, 104])
+def test_rf_map_without_field_is_the_linear_drift(oracle, bitype):
+    """Field scale 0: the RF cavity's map is the field-free linear map (the
+    constant-focusing map with k = 0) and the energy stays."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    refpt = np.array([0, 0, 0, 0, 0.7, -b.gamma0])
+    rf = oracle.RFData(_const_table(), fourier=False)
+    v = [0.0, 324e6, 25.0, 1.0, 1.0]
+    xm, r1 = oracle.element_map(c, 0.05, bitype, v, 10, refpt, b.charge, b.mass, 0.3, 0.5,
+                                0.2, rf)
+    xd, r2 = oracle.element_map(c, 0.05, 2, [0.0, 0.0, 0.0, 1.0], 10, refpt, b.charge,
+                                b.mass)
+    assert np.allclose(xm, xd, rtol=1e-15, atol=0)
+    assert r1[5] == refpt[5] and r1[4] == pytest.approx(r2[4], rel=1e-15)
+
+
+@pytest.mark.parametrize("bitype", [101, 103])
+def test_rf_energy_gain_in_a_uniform_field(oracle, bitype):
+    """A uniform on-axis field E0 at a frequency low enough that the phase
+    stays theta0: gamma grows by q E0 cos(theta0) L / mc^2 (theta0 with the
+    reference's single-precision asin(1.0), a 1e-8 effect), and every 2x2
+    block of the map keeps determinant 1 (the Wronskian of q' = p/Scxl,
+    p' = -s q; the entrance and exit terms are shears)."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    e0, th, L = 3.0e6, 30.0, 0.4
+    refpt = np.array([0, 0, 0, 0, 0.7, -b.gamma0])
+    rf = oracle.RFData(_const_table(), fourier=False)
+    v = [e0, 1.0e-3, th, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0]
+    xm, r = oracle.element_map(c, L, bitype, v, 40, refpt, b.charge, b.mass, 0.0, L, 0.0, rf)
+    # theta0 = Param(4)*asin(1.0)/90 with the REAL(4) asin of the reference
+    half_pi = float(np.arcsin(np.float32(1.0)))
+    dgam = (b.charge / b.mass) * e0 * math.cos(th * half_pi / 90) * L
+    assert -r[5] - b.gamma0 == pytest.approx(dgam, rel=1e-12)
+    for blk in range(3):
+        m = xm[2 * blk:2 * blk + 2, 2 * blk:2 * blk + 2]
+        assert np.linalg.det(m) == pytest.approx(1.0, abs=1e-12)
+    assert np.all(xm[:2, 2:] == 0) and np.all(xm[2:4, :2] == 0)
