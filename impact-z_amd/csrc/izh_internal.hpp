@@ -215,9 +215,6 @@ inline int map1_kind(int bitype, const double v[IZH_NVAL], const char **why) {
   if (bitype == 2 || bitype == 3 || is_rf_cavity(bitype)) {
     // Param(5) < 0 selects the ideal RF cavity model (:324-330)
     if (v[3] < 0.0) { *why = "Param(5) < 0 (ideal RF cavity branch)"; return -1; }
-    // SolRF's wakefield (Param(15) > 0) turns on kick1wake
-    // (src/Contrl/AccSimulator.f90:859-870)
-    if (bitype == 105 && v[13] > 0.0) { *why = "SolRF wakefield (Param(15) > 0)"; return -1; }
     return kMap1Linear;
   }
   *why = "element type not on this path (0 drift, 1 quadrupole, 2 constant "

@@ -79,9 +79,12 @@ class Element:
     type 105 (SolRF): v1..v5 as above, v6..v10 misalignments, v11 = Bz0 [T],
             v12..v14 wakefield.
 
-    ``more`` holds v5..v14; ``field`` the RF cavity's rfdata file contents:
-    an (n, 4) table (types 101-104, read1_Data) or the coefficient vector
-    (type 105, read3_Data).
+    type -41 (wakefield switch): v1 = scale, v2 = file ID of the wake
+            functions, v3 > 0 turns them on (< 10: no transverse wake).
+
+    ``more`` holds v5..v14; ``field`` the element's rfdata file contents:
+    an (n, 4) table (types 101-104, read1_Data; type -41, read1wk_Data) or
+    the coefficient vector (type 105, read3_Data).
     """
 
     length: float
@@ -242,9 +245,16 @@ def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
                          " ".join(repr(float(v)) for v in vals) + " /")
         if e.field is not None:
             write_rfdata(os.path.join(os.path.dirname(path) or ".",
-                                      rfdata_name(int(e.v4 + 0.1))), e.field)
+                                      rfdata_name(field_file_id(e))), e.field)
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def field_file_id(e: Element) -> int:
+    """The rfdata file number of an element's field: Param(5) = v4 of an RF
+    cavity, Param(3) = v2 of a -41 wakefield element
+    (src/Contrl/AccSimulator.f90:821-823,962-965)."""
+    return int((e.v2 if e.type == -41 else e.v4) + 0.1)
 
 
 def rfdata_name(file_id: int) -> str:

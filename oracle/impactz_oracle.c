@@ -436,7 +436,6 @@ int or_map1(const or_const *c, double *pts, int64_t n, double refpt[6],
     /* 6x6 map, src/Appl/BeamBunch.f90:308-320, 436-449 */
     double xm[36];
     if (bitype >= 2 && v[3] < 0.0) return -1;  /* ideal RF cavity model (:325-329) */
-    if (bitype == 105 && v[13] > 0.0) return -1;  /* SolRF wakefield: kick1wake */
     if (or_element_map(c, *z, tau, e, refpt, charge, mass, xm) != 0) return -1;
     for (i = 0; i < n; i++) {
       double t[6];
@@ -854,6 +853,156 @@ void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
   free(e);
 }
 
+/* ------------------------------------------------------------- wakefields */
+
+/* The line charge density and centroids of the run loop
+ * (src/Contrl/AccSimulator.f90:1276-1321, one process): each particle's
+ * t-frame z binned linearly on the Nz mesh lines (unit weight), x and y
+ * weighted for the centroids; rho in C/m of the lab frame (the bin width
+ * divided by gamma). */
+static void wake_line(const double *pts, int64_t n, const double range[6], int nz,
+                      double current, int64_t np, double scfreq, double gam,
+                      double charge, double *hzwake, double *rho, double *xb, double *yb) {
+  double hz = (range[5] - range[4]) / (nz - 1);
+  double *dens = (double *)calloc(nz + 1, sizeof(double));
+  double *xw = (double *)calloc(nz + 1, sizeof(double));
+  double *yw = (double *)calloc(nz + 1, sizeof(double));
+  int64_t i;
+  int k;
+  for (i = 0; i < n; i++) {
+    int iz = (int)((P(5, i) - range[4]) / hz + 1);
+    double ab = ((range[4] - P(5, i)) + iz * hz) / hz;
+    if (iz < 1) iz = 1;
+    if (iz > nz - 1) iz = nz - 1;
+    xw[iz - 1] += P(1, i) * ab;
+    xw[iz] += P(1, i) * (1.0 - ab);
+    yw[iz - 1] += P(3, i) * ab;
+    yw[iz] += P(3, i) * (1.0 - ab);
+    dens[iz - 1] += ab;
+    dens[iz] += 1.0 - ab;
+  }
+  hz = hz / gam;
+  for (k = 0; k < nz; k++) {
+    rho[k] = dens[k] * current / scfreq / (double)np / hz * (charge / fabs(charge));
+    xb[k] = dens[k] != 0.0 ? xw[k] / dens[k] : 0.0;
+    yb[k] = dens[k] != 0.0 ? yw[k] / dens[k] : 0.0;
+  }
+  *hzwake = hz;
+  free(dens);
+  free(xw);
+  free(yw);
+}
+
+/* A wake function at distance zz behind its source: the analytic
+ * short-range wakes of wakefield_FieldQuant (K. Bane's fits with the
+ * reference's constants, src/Appl/Field.f90:621-642,692-706; plane 0
+ * longitudinal, 1 and 2 transverse), or the read-in table of
+ * wakefieldread_FieldQuant with its interpolation weights as written there
+ * (ab = iwk*hwk - zwk, not divided by hwk; :820-830). */
+static double wake_fn(const or_const *c, const or_wake *w, int plane, double zz) {
+  if (w->kind == 1) {
+    const double pilc = 2 * asin(1.0), z0 = 120 * pilc;
+    const double aa = w->aa, gg = w->gg, leng = w->leng;
+    if (plane == 0) {
+      double zz00 = (double)0.41f * pow(aa / leng, (double)0.8f) *
+                    pow(gg / leng, (double)1.6f) * aa;
+      return z0 * c->clight * exp(-sqrt(zz / zz00)) / (pilc * aa * aa);
+    } else {
+      double zz00 = 1.0 * (double)0.169f * aa * pow(aa / leng, (double)1.17f) *
+                    pow(gg / aa, (double)0.38f);
+      double coef1 = 4 * z0 * c->clight * zz00 / (pilc * aa * aa * aa * aa);
+      return coef1 * (1.0 - (1. + sqrt(zz / zz00)) * exp(-sqrt(zz / zz00)));
+    }
+  } else {
+    const double *col = plane == 0 ? w->edat : (plane == 1 ? w->epdat : w->eppdat);
+    double hwk = w->zdat[1] - w->zdat[0], zwk = zz - w->zdat[0];
+    int iwk = (int)(zwk / hwk + 1), iwk1 = iwk + 1;
+    double ab = iwk * hwk - zwk;
+    /* rows past the table read the Data module's zero-filled arrays */
+    double e1 = iwk <= w->n ? col[iwk - 1] : 0.0, e2 = iwk1 <= w->n ? col[iwk1 - 1] : 0.0;
+    return e1 * ab + e2 * (1.0 - ab);
+  }
+}
+
+int or_wakefield(const or_const *c, const double *pts, int64_t n, const double range[6],
+                 int nz, double current, int64_t np, double gam, double charge,
+                 const or_wake *w, double *ex, double *ey, double *ez) {
+  /* src/Contrl/AccSimulator.f90:1272-1338 with wakefield_FieldQuant /
+   * wakefieldread_FieldQuant (src/Appl/Field.f90:590-1001).  The cyclic
+   * convolution those routines take by FFT on 2*Nz points is, with their
+   * Green function (half weight at distance 0 for the longitudinal wake,
+   * none for the transverse ones) is, for bin k, the sum over bin k itself
+   * (longitudinal, half weight) and the bins j > k at distance (j-k)*hz,
+   * computed here directly. */
+  double *rho = (double *)malloc(nz * sizeof(double));
+  double *xb = (double *)malloc(nz * sizeof(double));
+  double *yb = (double *)malloc(nz * sizeof(double));
+  double hz;
+  int k, j, lon, tra;
+  if (w->kind != 1 && w->kind != 2) return -1;
+  wake_line(pts, n, range, nz, current, np, c->scfreq, gam, charge, &hz, rho, xb, yb);
+  lon = !(w->kind == 2 && w->flagbtw == 4);
+  tra = !(w->kind == 2 && (w->flagbtw == 2 || w->flagbtw == 3));
+  for (k = 0; k < nz; k++) {
+    double sz = 0.0, sx = 0.0, sy = 0.0;
+    if (lon) sz = 0.5 * wake_fn(c, w, 0, 0.0) * rho[k];
+    for (j = k + 1; j < nz; j++) {
+      double zz = (j - k) * hz;
+      if (lon) sz += wake_fn(c, w, 0, zz) * rho[j];
+      if (tra) {
+        sx += wake_fn(c, w, 1, zz) * rho[j] * xb[j];
+        sy += wake_fn(c, w, w->kind == 1 ? 1 : 2, zz) * rho[j] * yb[j];
+      }
+    }
+    ez[k] = w->scale * (-sz * hz);
+    ex[k] = w->scale * (sx * hz);
+    ey[k] = w->scale * (sy * hz);
+  }
+  free(rho);
+  free(xb);
+  free(yb);
+  return 0;
+}
+
+void or_kick1wake(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+                  const double *phi, const double range[6], const double msize[3],
+                  const double refpt[6], double curr, double tau, double mass,
+                  double chrg, const double *ex, const double *ey, const double *ez) {
+  /* kick1wake_BeamBunch (src/Appl/BeamBunch.f90:4694-5046): the wake of each
+   * z plane added to the grid field, divided by the constants scatter1
+   * multiplies it with, then scatter1 */
+  size_t g = (size_t)nx * ny * nz, pl = (size_t)nx * ny, q;
+  double *e, gam, bet, gambet, brho, vz0, perv0, xycon, tcon, tmpscale, fpei, twopi, xk,
+      bbyk;
+  int k;
+  if (curr < (double)1.0e-20f) return;
+  e = (double *)malloc(3 * g * sizeof(double));
+  or_efield(phi, nx, ny, nz, msize, e, e + g, e + 2 * g);
+  gam = -refpt[5];
+  twopi = 2.0 * c->pi;
+  tmpscale = curr / c->scfreq;
+  fpei = c->clight * c->clight * (double)1.0e-7f;
+  xk = 1 / c->scxl;
+  bet = sqrt(gam * gam - 1.0) / gam;
+  bbyk = bet / xk;
+  gambet = gam * bet;
+  brho = gambet / (c->clight * chrg) * mass;
+  vz0 = bet * c->clight;
+  perv0 = 2.0 * curr * fpei / (brho * vz0 * vz0 * gam * gam);
+  xycon = 0.5 * perv0 * gambet * bbyk * twopi / tmpscale;
+  tcon = bet * xycon * gam * gam;
+  for (k = 0; k < nz; k++)
+    for (q = 0; q < pl; q++) {
+      size_t o = (size_t)k * pl + q;
+      e[o] = e[o] + chrg * ex[k] / xycon / mass / bet / gam;
+      e[g + o] = e[g + o] + chrg * ey[k] / xycon / mass / bet / gam;
+      e[2 * g + o] = e[2 * g + o] + chrg * ez[k] / tcon / mass;
+    }
+  or_scatter1(c, pts, n, nx, ny, nz, e, e + g, e + 2 * g, range, msize, gam, curr, tau,
+              mass, chrg);
+  free(e);
+}
+
 /* ----------------------------------------- nonlinear integrator (Flagmap=2) */
 
 void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], double *z,
@@ -1149,7 +1298,7 @@ void or_diag_row(const or_const *c, const double t[37], const double gm[6],
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                 int nx, int ny, int nz, double tau1, const or_element *e,
-                double pipe_radius, double *rho_out,
+                double pipe_radius, const or_wake *wake, double *rho_out,
                 double *phi_out) {
   size_t g = (size_t)nx * ny * nz;
   double *rho = rho_out ? rho_out : (double *)malloc(g * sizeof(double));
@@ -1164,10 +1313,22 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
     or_update_compdom(ptrange, nx, ny, nz, range, msize);
     or_deposit(pts, n, nx, ny, nz, range, msize, rho);
     rc = or_solve_openbc(rho, nx, ny, nz, msize[0], msize[1], msize[2], phi);
-    or_cvbkforth1st(c, pts, n, st->refpt);
-    if (rc == 0)
-      or_kick1(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, st->current,
-               tau2, st->mass, st->charge);
+    if (rc == 0 && wake && wake->kind) {
+      /* src/Contrl/AccSimulator.f90:1272-1338, then kick1wake (:1417-1421) */
+      double *wk = (double *)malloc(3 * (size_t)nz * sizeof(double));
+      rc = or_wakefield(c, pts, n, range, nz, st->current, st->npt, -st->refpt[5],
+                        st->charge, wake, wk, wk + nz, wk + 2 * nz);
+      or_cvbkforth1st(c, pts, n, st->refpt);
+      if (rc == 0)
+        or_kick1wake(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, st->current,
+                     tau2, st->mass, st->charge, wk, wk + nz, wk + 2 * nz);
+      free(wk);
+    } else {
+      or_cvbkforth1st(c, pts, n, st->refpt);
+      if (rc == 0)
+        or_kick1(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, st->current,
+                 tau2, st->mass, st->charge);
+    }
   }
   if (rc == 0)
     rc = or_map1(c, pts, n, st->refpt, &st->z, tau1, e, st->charge,

@@ -168,6 +168,32 @@ void or_diag_row(const or_const *c, const double sums[37], const double amax[6],
                  double r2max, int64_t npt, const double refpt[6], double mass,
                  double out[41]);
 
+/* A wakefield (src/Contrl/AccSimulator.f90:859-875,960-980): kind 1 the
+ * analytic short-range wake of a SolRF/TWS element with iris radius aa, gap
+ * gg and period leng (its Param(13:15)); kind 2 the wake functions read by
+ * read1wk_Data (n rows: zdat from 0, Wz, Wx, Wy) with flagbtw (2, 3: no
+ * transverse wake; 4: no longitudinal); scale = scwk. */
+typedef struct {
+  int kind;
+  double aa, gg, leng;
+  int n;
+  const double *zdat, *edat, *epdat, *eppdat;
+  int flagbtw;
+  double scale;
+} or_wake;
+
+/* The wake fields ex, ey, ez (nz each) of the particles after conv1st. */
+int or_wakefield(const or_const *c, const double *pts, int64_t n, const double range[6],
+                 int nz, double current, int64_t np, double gam, double charge,
+                 const or_wake *w, double *ex, double *ey, double *ez);
+
+/* kick1wake_BeamBunch (src/Appl/BeamBunch.f90:4694-5046): kick1 with the
+ * wake fields added to the grid field plane by plane. */
+void or_kick1wake(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
+                  const double *phi, const double range[6], const double msize[3],
+                  const double refpt[6], double curr, double tau, double mass,
+                  double chrg, const double *ex, const double *ey, const double *ez);
+
 /* One segment of the reference loop (src/Contrl/AccSimulator.f90:1099-1488)
  * for an element or_map1 supports:
  * map1(tau1) -> conv1st -> update_CompDom -> deposit -> solve -> cvbkforth1st
@@ -183,7 +209,8 @@ typedef struct {
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                 int nx, int ny, int nz, double tau1, const or_element *e,
-                double pipe_radius, double *rho_out, double *phi_out);
+                double pipe_radius, const or_wake *wake, double *rho_out,
+                double *phi_out);
 
 /* One Flagmap=2 segment: drift2(2 tau1) -> conv1st -> update_CompDom ->
  * deposit -> solve -> cvbkforth1st -> kick2(2 tau1) -> drift2(2 tau1). */

@@ -66,6 +66,50 @@ class RFData:
             self.c.zdat, self.c.edat, self.c.epdat = (_p(cols[k]) for k in range(3))
 
 
+class OrWake(C.Structure):
+    _fields_ = [("kind", C.c_int), ("aa", C.c_double), ("gg", C.c_double),
+                ("leng", C.c_double), ("n", C.c_int), ("zdat", _dp), ("edat", _dp),
+                ("epdat", _dp), ("eppdat", _dp), ("flagbtw", C.c_int),
+                ("scale", C.c_double)]
+
+
+def make_wake(kind, aa=0.0, gg=0.0, leng=0.0, table=None, flagbtw=0, scale=1.0):
+    """An or_wake: kind 1 analytic (aa, gg, leng), kind 2 the rows of a
+    read1wk_Data file (z shifted to start at 0, src/DataStruct/Data.f90:
+    613-617)."""
+    w = OrWake()
+    w.kind, w.aa, w.gg, w.leng, w.flagbtw, w.scale = kind, aa, gg, leng, flagbtw, scale
+    if table is not None:
+        t = np.asarray(table, dtype=np.float64).reshape(-1, 4)
+        cols = [np.ascontiguousarray(t[:, k]) for k in range(4)]
+        cols[0] = cols[0] - t[0, 0]
+        w._keep = cols
+        w.n = t.shape[0]
+        w.zdat, w.edat, w.epdat, w.eppdat = (_p(x) for x in cols)
+    return w
+
+
+def wakefield(c, pts, rng, nz, current, npt, gam, charge, w: OrWake):
+    ex, ey, ez = np.zeros(nz), np.zeros(nz), np.zeros(nz)
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    if lib().or_wakefield(C.byref(c), _p(p), p.shape[0],
+                          _p(np.ascontiguousarray(rng, dtype=np.float64)), nz, current,
+                          npt, gam, charge, C.byref(w), _p(ex), _p(ey), _p(ez)) != 0:
+        raise ValueError("bad wake kind")
+    return np.stack([ex, ey, ez])
+
+
+def kick1wake(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, wake):
+    """kick1wake_BeamBunch in place on pts; wake = (exwake, eywake, ezwake)."""
+    nz, ny, nx = phi.shape
+    w = np.ascontiguousarray(wake, dtype=np.float64)
+    lib().or_kick1wake(C.byref(c), _p(pts), pts.shape[0], nx, ny, nz, _p(phi),
+                       _p(np.ascontiguousarray(rng, dtype=np.float64)),
+                       _p(np.ascontiguousarray(ms, dtype=np.float64)),
+                       _p(np.ascontiguousarray(refpt, dtype=np.float64)), curr, tau, mass,
+                       chrg, _p(w[0].copy()), _p(w[1].copy()), _p(w[2].copy()))
+
+
 def make_element(bitype, v, mapstp=1, length=0.0, zedge=0.0, rf: RFData | None = None):
     """An or_element; v = the element's values (a lone number is a
     quadrupole gradient)."""
@@ -137,7 +181,15 @@ def lib() -> C.CDLL:
                                   C.c_int64, _dp, C.c_double, _dp]
         L.or_step.argtypes = [C.POINTER(OrConst), _dp, C.c_int64,
                               C.POINTER(OrState), C.c_int, C.c_int, C.c_int,
-                              C.c_double, C.POINTER(OrElement), C.c_double, _dp, _dp]
+                              C.c_double, C.POINTER(OrElement), C.c_double,
+                              C.POINTER(OrWake), _dp, _dp]
+        L.or_wakefield.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, C.c_int,
+                                   C.c_double, C.c_int64, C.c_double, C.c_double,
+                                   C.POINTER(OrWake), _dp, _dp, _dp]
+        L.or_wakefield.restype = C.c_int
+        L.or_kick1wake.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.c_int, C.c_int,
+                                   C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_step.restype = C.c_int64
         L.or_drift2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double]
         L.or_elem_field.argtypes = [C.POINTER(OrConst), C.c_int, _dp, C.c_double,
@@ -326,6 +378,10 @@ class Run:
         self.zedge = 0.0
         self._left = 0   # segments left in the current element
         self._rf = {}
+        # the loop's wake switches: flagwake, flagwakeread, flagbtw, scwk,
+        # aawk/ggwk/lengwk and the read-in table (src/Contrl/AccSimulator.f90:656-720)
+        self.wk = {"on": 0, "read": 0, "btw": 0, "scale": 1.0, "aa": 0.0, "gg": 0.0,
+                   "leng": 0.0, "table": None}
         self.c = physconst(beam.freq)
         self.pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
         self.n = self.pts.shape[0]
@@ -345,9 +401,47 @@ class Run:
         self.st = st
 
     def segments(self):
+        """The lattice's steps; a -41 element switches the read-in wake as
+        the loop reaches it."""
         for e in self.lattice:
+            if e.type == -41:
+                self.wake_switch(e)
+                continue
             for _ in range(e.nseg):
                 yield e
+
+    def wake_switch(self, e):
+        """A -41 element (src/Contrl/AccSimulator.f90:960-980)."""
+        if self.wk["read"] == 0:
+            self.wk["on"] = 0   # :759, for every element
+        v = e.values
+        if v[1] > 0.0:
+            self.wk["table"] = e.field
+        if v[2] > 0.0:
+            self.wk.update(on=1, read=1, scale=v[0])
+            if v[2] < 10.0:
+                self.wk["btw"] = 2
+        else:
+            self.wk.update(on=0, read=0, scale=1.0)
+
+    def _wake_entry(self, e):
+        """The wake switches on entering an element (src/Contrl/
+        AccSimulator.f90:759,859-875)."""
+        if self.wk["read"] == 0:
+            self.wk["on"] = 0
+        if e.type in (105, 106):
+            v = e.values
+            self.wk["aa"], self.wk["gg"], self.wk["leng"] = v[11], v[12], v[13]
+            self.wk["on"] = 1 if v[13] > 0.0 else 0
+
+    def wake(self):
+        """The or_wake of the current step, or None."""
+        w = self.wk
+        if not w["on"]:
+            return None
+        if w["read"]:
+            return make_wake(2, table=w["table"], flagbtw=w["btw"], scale=w["scale"])
+        return make_wake(1, w["aa"], w["gg"], w["leng"], scale=w["scale"])
 
     def rfdata(self, e):
         """The element's field arrays (cached per element)."""
@@ -366,12 +460,18 @@ class Run:
         if self._left == 0:
             # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
             self._left, self.zedge = e.nseg, self.st.z
+            self._wake_entry(e)
         self._left -= 1
         el = make_element(e.type, e.values, e.mapstp, e.length, self.zedge, self.rfdata(e))
-        step = lib().or_step if self.flagmap == 1 else lib().or_step2
-        n = step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx, ny, nz, tau1,
-                 C.byref(el), e.radius, _p(rho) if want_fields else None,
-                 _p(phi) if want_fields else None)
+        out = (_p(rho) if want_fields else None, _p(phi) if want_fields else None)
+        if self.flagmap == 1:
+            w = self.wake()
+            n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx,
+                              ny, nz, tau1, C.byref(el), e.radius,
+                              C.byref(w) if w is not None else None, *out)
+        else:
+            n = lib().or_step2(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx,
+                               ny, nz, tau1, C.byref(el), e.radius, *out)
         if n < 0:
             raise RuntimeError("or_step failed")
         self.n = n

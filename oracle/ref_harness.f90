@@ -60,6 +60,13 @@
         type (CCL), target, dimension(:), allocatable :: ccls
         type (SC), target, dimension(:), allocatable :: scs
         type (SolRF), target, dimension(:), allocatable :: slrfs
+        type (BPM), target, dimension(:), allocatable :: bpms
+        ! wakefields (src/Contrl/AccSimulator.f90:627-660,859-875,960-980)
+        integer :: flagwake, flagwakeread, flagbtw, iizz, iizz1, kz, ipt
+        double precision :: scwk, aawk, ggwk, lengwk, hzwake, ab, tmpwk, rfile, tmp1
+        double precision, dimension(:), allocatable :: xwakez, ywakez, densz, &
+             exwake, eywake, ezwake
+        double precision, dimension(:,:), allocatable :: recvdensz
         type (BeamLineElem), dimension(:), allocatable :: Blnelem
         integer :: N, Nx, Ny, Nz, nelem, nsteps, ndump, ierr, i, j, k
         integer :: bnseg, bmpstp, bitype, ihlf, nstep, idr, iqr, icf, isl
@@ -141,6 +148,7 @@
         allocate(drifts(nelem), quads(nelem), Blnelem(nelem))
         allocate(cfs(nelem), sols(nelem))
         allocate(dtls(nelem), ccdtls(nelem), ccls(nelem), scs(nelem), slrfs(nelem))
+        allocate(bpms(nelem))
         idr = 0
         iqr = 0
         icf = 0
@@ -203,8 +211,16 @@
               call setparam_SolRF(slrfs(i), tmprf)
               Blnelem(i) = assign_BeamLineElem(slrfs(i))
             endif
+          else if (etype(i) == -41) then
+            ! v1 = wake scale, v2 = wake file ID, v3 > 0 turns the read-in
+            ! wake on (< 10: no transverse wake)
+            call construct_BPM(bpms(i), enseg(i), empstp(i), etype(i), elen(i))
+            tmprf = 0.0d0
+            tmprf(2:15) = ev(1:14,i)
+            call setparam_BPM(bpms(i), tmprf)
+            Blnelem(i) = assign_BeamLineElem(bpms(i))
           else
-            print*, "harness supports element types 0 to 3 and 101 to 105 only"
+            print*, "harness supports element types 0 to 3, 101 to 105 and -41 only"
             stop
           endif
         enddo
@@ -216,6 +232,15 @@
         z = 0.0d0
         nstep = 0
         tmpfile = 0
+        flagwake = 0
+        flagwakeread = 0
+        flagbtw = 0
+        scwk = 1.0d0
+        allocate(xwakez(Nz), ywakez(Nz), densz(Nz), exwake(Nz), eywake(Nz), &
+                 ezwake(Nz), recvdensz(Nz,2))
+        exwake = 0.0d0
+        eywake = 0.0d0
+        ezwake = 0.0d0
         call diagnostic1_Output(z, Bpts, nchrg, nptlist0)
 
         do i = 1, nelem
@@ -232,6 +257,35 @@
                 call read3_Data(ifile)
               endif
               tmpfile = ifile
+            endif
+          endif
+          ! the wake switches of the element loop
+          ! (src/Contrl/AccSimulator.f90:759,859-875,960-980)
+          if (flagwakeread == 0) flagwake = 0
+          if (bitype == 105 .or. bitype == 106) then
+            aawk = ev(12,i)
+            ggwk = ev(13,i)
+            lengwk = ev(14,i)
+            if (lengwk > 0.0d0) then
+              flagwake = 1
+            else
+              flagwake = 0
+            endif
+          endif
+          if (bitype == -41) then
+            tmpwk = ev(1,i)
+            rfile = ev(2,i)
+            if (rfile > 0.0d0) call read1wk_Data(int(rfile + 0.1))
+            tmp1 = ev(3,i)
+            if (tmp1 > 0.0d0) then
+              flagwake = 1
+              flagwakeread = 1
+              if (tmp1 < 10.0) flagbtw = 2
+              scwk = tmpwk
+            else
+              flagwake = 0
+              flagwakeread = 0
+              scwk = 1.0d0
             endif
           endif
           ! the element's entrance z (src/Contrl/AccSimulator.f90:1025-1026)
@@ -285,10 +339,22 @@
             call update3O_FieldQuant(Potential, chgdens, Ageom, grid2d, &
                  lcgrid(1), lcgrid(2), lcgrid(3), 1, 1, lcgrid(2), lcgrid(3))
             if (dodump) write(u) Potential%FieldQ
+            if (flagwake == 1) then
+              call wake(u, dodump)
+            else if (dodump) then
+              densz = 0.0d0
+              write(u) densz
+              write(u) densz
+              write(u) densz
+            endif
             call cvbkforth1st_BeamBunch(Bpts)
             call setnpt_BeamBunch(Bpts, Nplocal)
             if (dodump) call wpts(u)
-            if (flagmap == 1) then
+            if (flagmap == 1 .and. flagwake == 1) then
+              call kick1wake_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
+                   lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0, &
+                   exwake, eywake, ezwake, Nz, 1, 1)
+            else if (flagmap == 1) then
               call map2_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
                    lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0)
             else
@@ -323,6 +389,64 @@
         close(21)
 
       contains
+        ! The wake fields of the run loop (src/Contrl/AccSimulator.f90:
+        ! 1272-1338, one process): the particles' t-frame z binned linearly
+        ! on the Nz mesh lines (unit weight; x and y weighted for the
+        ! centroids), the line charge density, then the reference's
+        ! wakefield_FieldQuant (analytic) or wakefieldread_FieldQuant (read
+        ! in), scaled by scwk.  The dump record (when dumping; zeros without
+        ! a wake) is Nz values each of exwake, eywake, ezwake.
+        subroutine wake(uu, dd)
+          integer, intent(in) :: uu
+          logical, intent(in) :: dd
+          call getrange_CompDom(Ageom, range)
+          hzwake = (range(6)-range(5))/(Nz-1)
+          xwakez = 0.0d0
+          ywakez = 0.0d0
+          densz = 0.0d0
+          do ipt = 1, Nplocal
+            iizz = (Bpts%Pts1(5,ipt)-range(5))/hzwake + 1
+            iizz1 = iizz + 1
+            ab = ((range(5)-Bpts%Pts1(5,ipt))+iizz*hzwake)/hzwake
+            xwakez(iizz) = xwakez(iizz) + Bpts%Pts1(1,ipt)*ab
+            xwakez(iizz1) = xwakez(iizz1) + Bpts%Pts1(1,ipt)*(1.0-ab)
+            ywakez(iizz) = ywakez(iizz) + Bpts%Pts1(3,ipt)*ab
+            ywakez(iizz1) = ywakez(iizz1) + Bpts%Pts1(3,ipt)*(1.0-ab)
+            densz(iizz) = densz(iizz) + ab
+            densz(iizz1) = densz(iizz1) + 1.0 -ab
+          enddo
+          hzwake = hzwake/(-Bpts%refptcl(6))
+          do kz = 1, Nz
+            recvdensz(kz,1) = densz(kz)*Bpts%Current/Scfreq/Np/(hzwake)*&
+                              Bpts%Charge/abs(Bpts%Charge)
+          enddo
+          do kz = 1, Nz
+            if (densz(kz) .ne. 0.0) then
+              xwakez(kz) = xwakez(kz)/densz(kz)
+              ywakez(kz) = ywakez(kz)/densz(kz)
+            else
+              xwakez(kz) = 0.0
+              ywakez(kz) = 0.0
+            endif
+            recvdensz(kz,2) = ywakez(kz)
+          enddo
+          if (flagwakeread == 1) then
+            call wakefieldread_FieldQuant(Nz, xwakez, ywakez, recvdensz, exwake, &
+                 eywake, ezwake, hzwake, aawk, ggwk, lengwk, flagbtw)
+          else
+            call wakefield_FieldQuant(Nz, xwakez, ywakez, recvdensz, exwake, &
+                 eywake, ezwake, hzwake, aawk, ggwk, lengwk)
+          endif
+          exwake = scwk*exwake
+          eywake = scwk*eywake
+          ezwake = scwk*ezwake
+          if (dd) then
+            write(uu) exwake
+            write(uu) eywake
+            write(uu) ezwake
+          endif
+        end subroutine wake
+
         ! map1_BeamBunch with Flagmap=1 (drift1_BeamBunch).  For the DTL and
         ! SolRF, drift1_BeamBunch's getparam_BeamLineElem(beamln,drange)
         ! copies the element's 25 (DTL) or 15 (SolRF) Param values into its

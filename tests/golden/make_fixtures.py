@@ -66,7 +66,7 @@ def fixture(name, n, grid, lattice, nsteps, dump_steps, seed=1,
         d = ref["dumps"][s]
         for key in stages:
             out[f"s{s}_{key}"] = d[key]
-        for key in ("ptrange", "range", "msize", "rho", "phi", "refpt", "refpt0"):
+        for key in ("ptrange", "range", "msize", "rho", "phi", "refpt", "refpt0", "wake"):
             out[f"s{s}_{key}"] = d[key]
         out[f"s{s}_current"] = np.array(d["current"])
         out[f"s{s}_current0"] = np.array(d["current0"])
@@ -207,6 +207,39 @@ def rf_lattice():
             B.Element(0.1, 4, 10, 0, 1.0)]
 
 
+def wake_table(n=1001, length=0.1):
+    """Wake functions for read1wk_Data: rows z, W_z (V/C/m), W_x and W_y
+    (V/C/m^2)."""
+    s = np.linspace(0.0, length, n)
+    return np.stack([s + 0.01, 2.0e14 * np.exp(-s / 0.004) * np.cos(s / 0.01),
+                     3.0e18 * (1.0 - np.exp(-s / 0.002)),
+                     2.0e18 * (1.0 - np.exp(-s / 0.003))], axis=1)
+
+
+def wake_file_lattice():
+    """A FODO period with the read-in wake of a -41 element (file 3): on with
+    its transverse part (flag 20, scale 1.5), then switched to the
+    longitudinal part only (flag 5, scale 0.8) before the defocusing half."""
+    tab = wake_table()
+    return [B.Element(0.0, 0, 1, -41, 1.5, 3.0, 20.0, field=tab),
+            B.Element(0.1, 4, 10, 1, 10.0, 0.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0),
+            B.Element(0.0, 0, 1, -41, 0.8, 3.0, 5.0, field=tab),
+            B.Element(0.1, 4, 10, 1, -10.0, 0.0, 1.0),
+            B.Element(0.4, 4, 10, 0, 1.0)]
+
+
+def wake_solrf_lattice():
+    """SolRF with the analytic short-range wake of its Param(13:15) (iris
+    radius, gap, period) around drifts that switch it off again."""
+    four = rf_fourier(0.2)
+    return [B.Element(0.1, 4, 10, 0, 1.0),
+            B.Element(0.2, 8, 20, 105, 3.0e6, 324.0e6, 45.0, 2.0,
+                      (1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.4, 10.75e-3, 30.33e-3, 33.321e-3),
+                      field=four),
+            B.Element(0.1, 4, 10, 0, 1.0)]
+
+
 def switch_lattice():
     """Linear maps through a FODO half-period, a type -25 element switching
     to the Lorentz integrator (Flagmap = its mapstp, 2), then constant
@@ -284,6 +317,12 @@ def main():
     meta["rf"] = ("RF cavities on the linear-map path: DTL (with quadrupoles), CCDTL, CCL, "
                   "SC on a field table (rfdata1.in), SolRF on Fourier coefficients "
                   "(rfdata2.in); dumps at steps 6 (DTL), 22 (SC) and 30 (SolRF)")
+    fixture("wake_file", 1000, (16, 16, 32), wake_file_lattice(), 16, (2, 12))
+    meta["wake_file"] = ("read-in wake functions of a -41 element (rfdata3.in): transverse "
+                         "and longitudinal from step 1 (dump 2), longitudinal only after the "
+                         "second -41 (dump 12)")
+    fixture("wake_solrf", 1000, (16, 16, 32), wake_solrf_lattice(), 16, (6,))
+    meta["wake_solrf"] = "SolRF with its analytic short-range wake (Param(13:15)), dump 6"
     exe_nonlinear_fixture()
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "

@@ -57,6 +57,7 @@ def read_dump(path: str, grid) -> dict:
     d["msize"] = r.f64(3)
     d["rho"] = r.f64(nx * ny * nz).reshape(nz, ny, nx)
     d["phi"] = r.f64(nx * ny * nz).reshape(nz, ny, nx)
+    d["wake"] = np.stack([r.f64(nz), r.f64(nz), r.f64(nz)])   # exwake, eywake, ezwake
     d["p_cvbk"] = r.pts()
     d["p_kick"] = r.pts()
     d["p_map1b"] = r.pts()
@@ -99,7 +100,7 @@ def run_harness(pts: np.ndarray, beam, lattice, grid, nsteps: int,
             f.write(f"{e.length!r} {e.nseg} {e.mapstp} {e.type} " +
                     " ".join(repr(float(v)) for v in e.values) + "\n")
             if e.field is not None:
-                B.write_rfdata(os.path.join(tmp, B.rfdata_name(int(e.v4 + 0.1))), e.field)
+                B.write_rfdata(os.path.join(tmp, B.rfdata_name(B.field_file_id(e))), e.field)
         f.write(f"{nsteps} {len(dump_steps)}\n")
         if dump_steps:
             f.write(" ".join(str(s) for s in dump_steps) + "\n")

@@ -530,3 +530,107 @@ def test_rf_energy_gain_in_a_uniform_field(oracle, bitype):
         m = xm[2 * blk:2 * blk + 2, 2 * blk:2 * blk + 2]
         assert np.linalg.det(m) == pytest.approx(1.0, abs=1e-12)
     assert np.all(xm[:2, 2:] == 0) and np.all(xm[2:4, :2] == 0)
+
+
+# ------------------------------------------------------------- wakefields
+
+WAKE_CASES = [("wake_file", 2, 2, 1.5), ("wake_file", 12, 2, 0.8), ("wake_solrf", 6, 1, 1.0)]
+
+
+def _wake_of(oracle, d, step, kind, scale):
+    lat = fixture_lattice(d)
+    if kind == 2:
+        sw = [e for e in lat if e.type == -41]
+        btw = 2 if step > 8 else 0
+        return oracle.make_wake(2, table=sw[0].field, flagbtw=btw, scale=scale)
+    e = segment(d, step)
+    v = e.values
+    return oracle.make_wake(1, v[11], v[12], v[13], scale=scale)
+
+
+@pytest.mark.parametrize("name,step,kind,scale", WAKE_CASES)
+def test_oracle_wake_stages_match_reference(oracle, name, step, kind, scale):
+    """The wake fields from the particles after conv1st (with gamma after
+    the first half map, as the run loop has it) and kick1wake from the
+    reference's own inputs."""
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    s = f"s{step}_"
+    c = oracle.physconst(beam.freq)
+    tau1 = 0.5 * e.length / e.nseg
+    pts = d[s + "p0"].copy()
+    refpt = d[s + "refpt0"].copy()
+    oracle.map1(c, pts, refpt, float(d[s + "z0"]), tau1, e.type, e.values, e.mapstp,
+                beam.charge, beam.mass, e.length, element_entrance(d, step), _rf(oracle, e))
+    w = _wake_of(oracle, d, step, kind, scale)
+    got = oracle.wakefield(c, d[s + "p_conv1st"], d[s + "range"], grid[2],
+                           float(d[s + "current"]), d[s + "p_conv1st"].shape[0],
+                           -refpt[5], beam.charge, w)
+    ref = d[s + "wake"]
+    assert np.abs(ref).max() > 0
+    for k in range(3):
+        assert np.abs(got[k] - ref[k]).max() <= 1e-13 * max(np.abs(ref[k]).max(), 1e-300)
+    pk = d[s + "p_cvbk"].copy()
+    oracle.kick1wake(c, pk, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt,
+                     float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge, ref)
+    assert col_err(pk, d[s + "p_kick"]) < 1e-13
+    # without the wake the kick differs (the wake is not negligible here)
+    pn = d[s + "p_cvbk"].copy()
+    oracle.kick1(c, pn, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt,
+                 float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge)
+    assert col_err(pn, d[s + "p_kick"]) > 1e-9
+
+
+@pytest.mark.parametrize("name", ["wake_file", "wake_solrf"])
+def test_oracle_wake_trajectory_matches_reference(oracle, name):
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
+    nsteps = int(d["nsteps"])
+    for k, e in enumerate(run.segments(), 1):
+        if k > nsteps:
+            break
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+    assert col_err(run.particles(), d["final"]) < 1e-12
+
+
+def test_wake_convolution_closed_forms(oracle):
+    """A constant longitudinal wake W0 over a bunch of uniform line density:
+    E_z(k) = -W0 hz (rho/2 + rho * (number of bins ahead)); the transverse
+    wake of a bunch on axis is zero, and it is linear in the offset."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    nz = 16
+    tab = np.array([[0.0, 5.0e13, 1.0e18, 2.0e18], [1.0, 5.0e13, 1.0e18, 2.0e18]])
+    w = oracle.make_wake(2, table=tab)
+    # particles exactly on the mesh lines 1..14 (t frame z), one per line
+    rng = np.array([-1, 1, -1, 1, 0.0, 0.15])
+    hz = (rng[5] - rng[4]) / (nz - 1)
+    pts = np.zeros((14, 9))
+    pts[:, 4] = rng[4] + hz * np.arange(1, 15)
+    out = oracle.wakefield(c, pts, rng, nz, b.current, 14, 2.0, b.charge, w)
+    rho = b.current / b.freq / 14 / (hz / 2.0)
+    lines = np.zeros(nz)
+    lines[1:15] = rho
+    want = np.array([-(0.5 * lines[k] + lines[k + 1:].sum()) * 5.0e13 * hz / 2.0
+                     for k in range(nz)])
+    # particles on mesh lines may spill a rounding-level weight to a neighbour
+    assert np.abs(out[2] - want).max() <= 1e-13 * np.abs(want).max()
+    assert np.all(out[0] == 0) and np.all(out[1] == 0)
+    pts[:, 0] = 1e-3
+    out2 = oracle.wakefield(c, pts, rng, nz, b.current, 14, 2.0, b.charge, w)
+    pts[:, 0] = 2e-3
+    out3 = oracle.wakefield(c, pts, rng, nz, b.current, 14, 2.0, b.charge, w)
+    assert np.abs(out3[0] - 2 * out2[0]).max() <= 1e-13 * np.abs(out3[0]).max()
+    assert np.abs(out2[0]).max() > 0
