@@ -440,6 +440,10 @@ int izh_rfdata_fourier(const double *coef, int n, izh_rfdata **out) {
 }
 
 int izh_rfdata_destroy(izh_rfdata *d) {
+  if (d && d->d_table) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(d->d_table);
+  }
   delete d;
   return IZH_OK;
 }
@@ -912,7 +916,7 @@ int izh_field_destroy(izh_field *f) {
   if (f->stream) (void)hipStreamSynchronize(f->stream);
   void *bufs[] = {f->rho_fx, f->rho,  f->phi,    f->e,    f->spec_a,
                   f->spec_b, f->green, f->green_t, f->corner, f->tw_x,
-                  f->tw_y,   f->tw_z, f->d_dom,  f->chunks};
+                  f->tw_y,   f->tw_z, f->d_dom,  f->chunks, f->d_wake};
   for (void *p : bufs)
     if (p) (void)hipFree(p);
   if (f->h_dom) (void)hipHostFree(f->h_dom);
@@ -1153,11 +1157,119 @@ int izh_field_upload_rho(izh_field *f, const double *rho) {
   return IZH_OK;
 }
 
+// ---------------------------------------------------------- wakefields
+
+}  // extern "C"
+
+namespace izh {
+// The kernel inputs of a wake for the field's mesh; the table moves to the
+// device on first use.
+int wake_params_of(const izh_bunch *b, const izh_field *f, const izh_wake *w,
+                   WakeParams *p) {
+  if (!w || (w->kind != 1 && w->kind != 2))
+    return fail(IZH_ERR_ARG, "wake: kind must be 1 (analytic) or 2 (read-in table)");
+  if (f->nz > kWakeMaxNz)
+    return fail(IZH_ERR_ARG, "wake: Nz = %d above %d", f->nz, kWakeMaxNz);
+  *p = WakeParams{};
+  p->kind = w->kind;
+  p->nz = f->nz;
+  p->flagbtw = w->flagbtw;
+  p->scale = w->scale;
+  if (w->kind == 1) {
+    if (!(w->aa > 0) || !(w->gg > 0) || !(w->leng > 0))
+      return fail(IZH_ERR_ARG, "wake: iris radius, gap and period must be positive");
+    p->aa = w->aa;
+    p->gg = w->gg;
+    p->leng = w->leng;
+  } else {
+    if (!w->table || w->table->fourier || w->table->z.size() < 2)
+      return fail(IZH_ERR_ARG, "wake: a read-in wake needs a table of at least 2 rows "
+                  "(izh_rfdata_table)");
+    IZH_CHECK(wake_table_device(w->table, &p->tab));
+    p->ntab = (int)w->table->z.size();
+  }
+  p->gam = -b->refpt[5];
+  p->mass = b->mass;
+  p->charge = b->charge;
+  p->scfreq = b->c.scfreq;
+  p->clight = b->c.clight;
+  p->current = b->current;
+  p->npt = (double)b->npt;
+  sc_constants(b, &p->xycon, &p->tcon);
+  if (!(b->current > 0)) p->xycon = p->tcon = 0.0;
+  return IZH_OK;
+}
+
+// A SolRF/TWS element with a wake length switches the wake on; izh_step and
+// izh_run take it from the segment (src/Contrl/AccSimulator.f90:859-870).
+static bool needs_wake(const izh_element &e) {
+  return (e.bitype == 105 || e.bitype == 106) && e.v[13] > 0.0;
+}
+}  // namespace izh
+
+extern "C" {
+
+int izh_wakefield(izh_bunch *b, izh_field *f, const izh_compdom *dom, const izh_wake *w) {
+  if (!b || !f || !w) return fail(IZH_ERR_ARG, "izh_wakefield: bad argument");
+  if (b->frame != kFrameT1)
+    return fail(IZH_ERR_STATE, "izh_wakefield: call izh_conv1st first (t frame)");
+  IZH_CHECK(check_dom(f, dom, "izh_wakefield"));
+  WakeParams p;
+  IZH_CHECK(wake_params_of(b, f, w, &p));
+  IZH_CHECK(field_set_domain(f, dom));
+  IZH_CHECK(join_streams(b->stream, f->stream));
+  IZH_CHECK(launch_wake(b, f, p, nullptr, nullptr, nullptr, 0));
+  IZH_CHECK(join_streams(f->stream, b->stream));
+  return IZH_OK;
+}
+
+int izh_kick1wake(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
+                  int flagbc) {
+  if (!b || !f) return fail(IZH_ERR_ARG, "izh_kick1wake: bad argument");
+  if (flagbc != 1) return fail(IZH_ERR_ARG, "izh_kick1wake: only Flagbc=1 is supported");
+  if (b->frame != kFrameT0)
+    return fail(IZH_ERR_STATE, "izh_kick1wake: call izh_cvbkforth1st first");
+  if (b->current < (double)1.0e-20f) {   // src/Appl/BeamBunch.f90:4720
+    b->frame = kFrameZ;
+    return IZH_OK;
+  }
+  IZH_CHECK(check_dom(f, dom, "izh_kick1wake"));
+  if (!f->have_phi) return fail(IZH_ERR_STATE, "izh_kick1wake: no potential");
+  if (!f->have_wake) return fail(IZH_ERR_STATE, "izh_kick1wake: call izh_wakefield first");
+  if (b->precision != f->precision)
+    return fail(IZH_ERR_ARG, "izh_kick1wake: bunch and field precision differ");
+  IZH_CHECK(field_set_domain(f, dom));
+  IZH_CHECK(launch_efield(f));
+  IZH_CHECK(launch_wake_add(f));
+  IZH_CHECK(join_streams(f->stream, b->stream));
+  double xycon, tcon;
+  sc_constants(b, &xycon, &tcon);
+  IZH_CHECK(launch_gather_kick(b, f, tau, xycon, tcon));
+  IZH_CHECK(join_streams(b->stream, f->stream));
+  b->frame = kFrameZ;
+  return IZH_OK;
+}
+
+int izh_wake_download(izh_field *f, double *ex, double *ey, double *ez) {
+  if (!f || !ex || !ey || !ez) return fail(IZH_ERR_ARG, "izh_wake_download: bad argument");
+  if (!f->have_wake) return fail(IZH_ERR_STATE, "izh_wake_download: no wake computed");
+  IZH_HIP(hipStreamSynchronize(f->stream));
+  const size_t nz = (size_t)f->nz;
+  IZH_HIP(hipMemcpy(ex, f->d_wake, nz * sizeof(double), hipMemcpyDeviceToHost));
+  IZH_HIP(hipMemcpy(ey, f->d_wake + nz, nz * sizeof(double), hipMemcpyDeviceToHost));
+  IZH_HIP(hipMemcpy(ez, f->d_wake + 2 * nz, nz * sizeof(double), hipMemcpyDeviceToHost));
+  return IZH_OK;
+}
+
 // ---------------------------------------------------------------- step
 
 int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
              const izh_segment *seg, double bcurr, double *moments) {
   if (!b || !f || !dom || !z || !seg) return fail(IZH_ERR_ARG, "izh_step: bad argument");
+  if (needs_wake(seg->e) && !seg->wake)
+    return fail(IZH_ERR_ARG, "izh_step: bitype %d with a wake length (Param(15) > 0) "
+                "needs its izh_wake", seg->e.bitype);
+  const bool wake = seg->wake && seg->wake->kind != 0;
   const double tau1 = seg->tau1, tau2 = 2.0 * tau1;
   // src/Contrl/AccSimulator.f90:1106-1110 first half push
   IZH_CHECK(izh_map1(b, z, tau1, &seg->e));
@@ -1170,8 +1282,12 @@ int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     IZH_CHECK(izh_update_compdom(dom, ptrange, 1));                         // :1156
     IZH_CHECK(izh_charge(b, dom, f));                                       // :1191
     IZH_CHECK(izh_update3O(f, dom));                                        // :1263
+    if (wake) IZH_CHECK(izh_wakefield(b, f, dom, seg->wake));               // :1272
     IZH_CHECK(izh_cvbkforth1st(b));                                         // :1399
-    IZH_CHECK(izh_kick1(b, tau2, f, dom, 1));                               // :1425
+    if (wake)
+      IZH_CHECK(izh_kick1wake(b, tau2, f, dom, 1));                         // :1417
+    else
+      IZH_CHECK(izh_kick1(b, tau2, f, dom, 1));                             // :1425
   }
   IZH_CHECK(izh_map1(b, z, tau1, &seg->e));                                // :1429
   if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
@@ -1201,6 +1317,13 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     if (kind == kMap1QuadK && bcurr >= (double)1.0e-10f)
       return fail(IZH_ERR_ARG, "izh_run: segment %lld is a K-mode quadrupole (chromatic "
                   "map): run it through izh_step", (long long)s);
+    if (needs_wake(sg.e) && !sg.wake)
+      return fail(IZH_ERR_ARG, "izh_run: segment %lld: bitype %d with a wake length "
+                  "(Param(15) > 0) needs its izh_wake", (long long)s, sg.e.bitype);
+    if (sg.wake && sg.wake->kind != 0) {
+      WakeParams p;
+      IZH_CHECK(wake_params_of(b, f, sg.wake, &p));
+    }
   }
   if (nseg == 0) return IZH_OK;
   // no space charge or nothing to carry it: the reference loop is maps only

@@ -1065,6 +1065,16 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
                         b->nchunks_cap);
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
+    // the step's wakefield (src/Contrl/AccSimulator.f90:1272-1338) from the
+    // same t positions, into E before the next pass kicks
+    if (rc == IZH_OK && segs[s].wake && segs[s].wake->kind != 0) {
+      WakeParams wp;
+      rc = wake_params_of(b, f, segs[s].wake, &wp);
+      if (rc == IZH_OK)
+        rc = launch_wake(b, f, wp, b->d_scal, b->chunks + ((s + po) & 1) * b->nchunks_cap,
+                         &b->d_scal->nchunks[(s + po) & 1], b->nchunks_cap);
+      if (rc == IZH_OK) rc = launch_wake_add(f);
+    }
     // second half of step s: host bookkeeping now, applied by the next pass
     if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, kb_prev, z);
     refpt_after[2 * s] = b->refpt[4];

@@ -178,6 +178,8 @@ struct izh_field {
   double rho_scale = 1.0;       // grid value = physical charge * rho_scale
   int have_rho = 0, have_phi = 0;
   double fixed_scale = 0;       // 2^e used by the fixed-point deposit
+  double *d_wake = nullptr;     // exwake, eywake, ezwake, dEx, dEy, dEz (nz each), bins
+  int have_wake = 0;
 };
 
 namespace izh {
@@ -188,6 +190,27 @@ struct Push {
   double tau, gam0, ibeta0, scxl;
   double xm[36];
 };
+
+// The inputs of the wake kernels (izh_wake.hip): the wake (kind 1 analytic
+// aa, gg, leng; kind 2 a device table of ntab rows: z, Wz, Wx, Wy), the
+// scale scwk, the reference energy of the step and the bunch constants;
+// current, npt, xycon and tcon come from a DevScal when the launch gives one.
+struct WakeParams {
+  int kind, nz, flagbtw, ntab;
+  double aa, gg, leng, scale;
+  const double *tab;
+  double gam, mass, charge, scfreq, clight;
+  double current, npt, xycon, tcon;
+};
+constexpr int kWakeMaxNz = 1024;   // LDS of the one-workgroup convolution
+
+int launch_wake(izh_bunch *b, izh_field *f, const WakeParams &p, const DevScal *S,
+                const Chunk *chunks, const int64_t *d_nchunks, int64_t max_chunks);
+int launch_wake_add(izh_field *f);
+// WakeParams of a wake for the bunch's present reference energy (izh_api.hip).
+int wake_params_of(const izh_bunch *b, const izh_field *f, const izh_wake *w,
+                   WakeParams *p);
+int wake_table_device(const izh_rfdata *d, const double **out);
 
 // Called by every operation that changes the particles or their order: the
 // sort state a run of steps leaves for the next one no longer applies.
@@ -229,6 +252,7 @@ struct izh_rfdata {
   int fourier;                 // 0: table (read1_Data), 1: coefficients (read3_Data)
   std::vector<double> z, ez, ezp, ezpp;
   std::vector<double> coef;    // Fcoef(1..Ndata)
+  mutable double *d_table = nullptr;  // device copy of the four columns (wake tables)
 };
 namespace izh {
 

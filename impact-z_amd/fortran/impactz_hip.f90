@@ -56,12 +56,23 @@ module impactz_hip
     type(c_ptr) :: rf
   end type izh_element
 
-  ! one segment of a lattice: the element's half step, the element and the
-  ! pipe aperture
+  ! a wakefield (kind 1 analytic: aa, gg, leng; kind 2 a read-in table with
+  ! flagbtw; scale = scwk), src/Contrl/AccSimulator.f90:859-875,960-980
+  type, bind(C) :: izh_wake
+    integer(c_int) :: kind
+    real(c_double) :: aa, gg, leng
+    type(c_ptr) :: table
+    integer(c_int) :: flagbtw
+    real(c_double) :: scale
+  end type izh_wake
+
+  ! one segment of a lattice: the element's half step, the element, the
+  ! pipe aperture and the step's izh_wake (c_null_ptr: none)
   type, bind(C) :: izh_segment
     real(c_double) :: tau1
     type(izh_element) :: e
     real(c_double) :: xrad, yrad
+    type(c_ptr) :: wake
   end type izh_segment
 
   interface
@@ -172,6 +183,27 @@ module impactz_hip
     function izh_rfdata_destroy(d) bind(C, name="izh_rfdata_destroy") result(rc)
       import :: c_int, c_ptr
       type(c_ptr), value :: d
+      integer(c_int) :: rc
+    end function
+    function izh_wakefield(b, f, dom, w) bind(C, name="izh_wakefield") result(rc)
+      import :: c_int, c_ptr, izh_compdom, izh_wake
+      type(c_ptr), value :: b, f
+      type(izh_compdom), intent(in) :: dom
+      type(izh_wake), intent(in) :: w
+      integer(c_int) :: rc
+    end function
+    function izh_kick1wake(b, tau, f, dom, flagbc) bind(C, name="izh_kick1wake") result(rc)
+      import :: c_int, c_double, c_ptr, izh_compdom
+      type(c_ptr), value :: b, f
+      real(c_double), value :: tau
+      type(izh_compdom), intent(in) :: dom
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
+    function izh_wake_download(f, ex, ey, ez) bind(C, name="izh_wake_download") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: f
+      real(c_double) :: ex(*), ey(*), ez(*)
       integer(c_int) :: rc
     end function
     function izh_drift2(b, z, tau) bind(C, name="izh_drift2") result(rc)

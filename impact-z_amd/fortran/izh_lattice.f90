@@ -73,6 +73,10 @@ program izh_lattice
   allocate(rfh(nelem))
   do i = 1, nelem
     rfh(i) = c_null_ptr
+    if (bitype(i) == 105 .and. val(14, i) > 0.0d0) then
+      print *, 'izh_lattice: element ', i, ': SolRF wakefields are not run by this host'
+      stop 1
+    end if
     if (bitype(i) >= 101 .and. bitype(i) <= 105) call read_rfdata(i)
   end do
   dom%nx = nx; dom%ny = ny; dom%nz = nz
@@ -136,6 +140,7 @@ program izh_lattice
         segs(k)%e = e
         segs(k)%xrad = radius(i)
         segs(k)%yrad = radius(i)
+        segs(k)%wake = c_null_ptr
         zedge = (zedge + tau1) + tau1
       end do
     end do

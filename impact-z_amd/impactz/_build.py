@@ -18,7 +18,8 @@ LIB = os.path.join(PKG, "libimpactz_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["izh_api.hip", "izh_particles.hip", "izh_field.hip", "izh_fused.hip"]
+SOURCES = ["izh_api.hip", "izh_particles.hip", "izh_field.hip", "izh_fused.hip",
+           "izh_wake.hip"]
 
 # Packed FP32 instructions are disabled for all device code: they have given
 # intermittently wrong results on this hardware generation's toolchain.

@@ -174,6 +174,15 @@ class FileAction:
 
 
 @dataclass
+class WakeSwitch:
+    """A type -41 element: the read-in wakefield on or off
+    (src/Contrl/AccSimulator.f90:960-980); its wake functions come from
+    rfdataN.in, N = v2."""
+    index: int
+    element: Element
+
+
+@dataclass
 class MapSwitch:
     """A type -25 element: Flagmap = its mapstp from here on
     (src/Contrl/AccSimulator.f90:807-809)."""
@@ -228,11 +237,14 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             if vv[3] < 0.0:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {vv[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")
-            if e.type == 105 and vv[13] > 0.0:
-                raise DeckError(f"element {k}: SolRF wakefield (Param(15) = {vv[13]:g}) "
-                                "is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
                                    more=tuple(vv[4:]))))
+        elif e.type == -41:
+            if flagmap != 1:
+                raise DeckError(f"element {k}: wakefields under Flagmap=2 are not on this path")
+            vv = (v + [0.0] * 14)[:14]
+            out.append((k, WakeSwitch(k, Element(e.length, e.nseg, e.mapstp, -41, *vv[:4],
+                                                 more=tuple(vv[4:])))))
         elif e.type == -25:
             if e.nseg != 0:
                 raise DeckError(f"element {k}: a type -25 element with {e.nseg} segments "
@@ -263,6 +275,8 @@ def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
                          f"({DIAGNOSTIC_TYPES[p.type]}) writes fort.{p.unit}")
         elif isinstance(p, MapSwitch):
             notes.append(f"element {p.index}: type -25 switches to Flagmap={p.flagmap}")
+        elif isinstance(p, WakeSwitch):
+            notes.append(f"element {p.index}: type -41 switches the read-in wakefield")
         else:
             out.append(p)
     return out, notes
@@ -332,6 +346,13 @@ def with_fields(prog, workdir: str):
                 cache[key] = read_rfdata(os.path.join(workdir, rfdata_name(fid)),
                                          1 if p.type == 105 else 4)
             p = dataclasses.replace(p, field=cache[key])
+        elif isinstance(p, WakeSwitch) and p.element.v2 > 0.0:
+            # read1wk_Data (src/DataStruct/Data.f90:559-633)
+            fid = int(p.element.v2 + 0.1)
+            key = (fid, False)
+            if key not in cache:
+                cache[key] = read_rfdata(os.path.join(workdir, rfdata_name(fid)), 4)
+            p = WakeSwitch(p.index, dataclasses.replace(p.element, field=cache[key]))
         out.append((index, p))
     return out
 
@@ -414,6 +435,11 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
             if isinstance(p, MapSwitch):
                 flush()
                 sim.flagmap = p.flagmap
+                st["i"], st["j"] = index, 1
+                continue
+            if isinstance(p, WakeSwitch):
+                flush()
+                sim.wake_switch(p.element)
                 st["i"], st["j"] = index, 1
                 continue
             if isinstance(p, FileAction):

@@ -34,12 +34,13 @@ SYMBOLS = (
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
     "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
     "izh_map1", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
-    "izh_rfdata_fourier", "izh_rfdata_destroy",
+    "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
+    "izh_wake_download",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
 TIMERS = ("push", "conv1st", "deposit", "green", "fft", "efield", "kick",
-          "cvbk", "diag", "pass", "final")
+          "cvbk", "diag", "pass", "final", "wake")
 
 
 class IZHError(RuntimeError):
@@ -69,11 +70,29 @@ class ElementC(C.Structure):
                 ("length", C.c_double), ("zedge", C.c_double), ("rf", C.c_void_p)]
 
 
+class WakeC(C.Structure):
+    """izh_wake: kind 1 analytic (iris radius aa, gap gg, period leng), kind 2
+    a read-in table (an RFData table of z, Wz, Wx, Wy) with flagbtw; scale =
+    scwk."""
+    _fields_ = [("kind", C.c_int), ("aa", C.c_double), ("gg", C.c_double),
+                ("leng", C.c_double), ("table", C.c_void_p), ("flagbtw", C.c_int),
+                ("scale", C.c_double)]
+
+
+def wake(kind: int, aa: float = 0.0, gg: float = 0.0, leng: float = 0.0,
+         table: "RFData | None" = None, flagbtw: int = 0, scale: float = 1.0) -> WakeC:
+    w = WakeC()
+    w.kind, w.aa, w.gg, w.leng, w.flagbtw, w.scale = kind, aa, gg, leng, flagbtw, scale
+    w.table = table.h if table is not None else None
+    w._keep = table
+    return w
+
+
 class Segment(C.Structure):
-    """izh_segment: an element's half step tau1, the element and the pipe
-    aperture."""
+    """izh_segment: an element's half step tau1, the element, the pipe
+    aperture and the step's wakefield (None: none)."""
     _fields_ = [("tau1", C.c_double), ("e", ElementC), ("xrad", C.c_double),
-                ("yrad", C.c_double)]
+                ("yrad", C.c_double), ("wake", C.POINTER(WakeC))]
 
 
 def values14(v) -> np.ndarray:
@@ -100,12 +119,15 @@ def element(bitype: int, v, length: float = 0.0, zedge: float = 0.0, mapstp: int
 
 
 def segment(tau1: float, bitype: int, v, mapstp: int, xrad: float, yrad: float,
-            length: float = 0.0, zedge: float = 0.0, rf: "RFData | None" = None) -> Segment:
+            length: float = 0.0, zedge: float = 0.0, rf: "RFData | None" = None,
+            wake: WakeC | None = None) -> Segment:
     sg = Segment()
     sg.tau1 = tau1
     sg.e = element(bitype, v, length, zedge, mapstp, rf)
-    sg._rf_keep = rf
+    sg._rf_keep = (rf, wake)
     sg.xrad, sg.yrad = xrad, yrad
+    if wake is not None:
+        sg.wake = C.pointer(wake)
     return sg
 
 
@@ -147,6 +169,9 @@ def lib() -> C.CDLL:
     L.izh_rfdata_table.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_fourier.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_destroy.argtypes = [vp]
+    L.izh_wakefield.argtypes = [vp, vp, C.POINTER(CompDomC), C.POINTER(WakeC)]
+    L.izh_kick1wake.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
+    L.izh_wake_download.argtypes = [vp, _dp, _dp, _dp]
     L.izh_drift2.argtypes = [vp, _dp, C.c_double]
     L.izh_kick2.argtypes = [vp, C.c_double, C.c_double, C.POINTER(ElementC), vp,
                             C.POINTER(CompDomC), C.c_int]
@@ -331,6 +356,15 @@ class FieldQuant:
         check(lib().izh_field_download(self.h, which, _p(out)))
         return out
 
+    def wake_download(self) -> np.ndarray:
+        """exwake, eywake, ezwake of the last izh_wakefield, (3, nz)."""
+        nz = self.shape[0]
+        out = np.zeros((3, nz))
+        a, b_, c_ = (np.zeros(nz) for _ in range(3))
+        check(lib().izh_wake_download(self.h, _p(a), _p(b_), _p(c_)))
+        out[0], out[1], out[2] = a, b_, c_
+        return out
+
     def upload_rho(self, rho: np.ndarray) -> None:
         assert rho.shape == self.shape
         check(lib().izh_field_upload_rho(self.h, _p(np.ascontiguousarray(rho, dtype=np.float64))))
@@ -463,6 +497,14 @@ class BeamBunch:
     def cvbkforth1st(self) -> None:
         """cvbkforth1st_BeamBunch (src/Appl/BeamBunch.f90:1667-1701)."""
         check(lib().izh_cvbkforth1st(self.h))
+
+    def wakefield(self, field: FieldQuant, dom: CompDom, w: WakeC) -> None:
+        """The run loop's wake fields (src/Contrl/AccSimulator.f90:1272-1338)."""
+        check(lib().izh_wakefield(self.h, field.h, C.byref(dom.c), C.byref(w)))
+
+    def kick1wake(self, tau: float, field: FieldQuant, dom: CompDom, flagbc: int = 1) -> None:
+        """kick1wake_BeamBunch (src/Appl/BeamBunch.f90:4694-5046)."""
+        check(lib().izh_kick1wake(self.h, tau, field.h, C.byref(dom.c), flagbc))
 
     def kick1(self, tau: float, field: FieldQuant, dom: CompDom, flagbc: int = 1) -> None:
         """map2_BeamBunch = kick1_BeamBunch (src/Appl/BeamBunch.f90:1707-2038)."""

@@ -67,6 +67,8 @@ class Simulation:
         self._left = 0      # segments left in the current element
         self.zedge = 0.0    # z of the current element's entrance
         self._rf = {}       # RF cavity fields on the device library's side
+        self.wk = {"on": 0, "read": 0, "btw": 0, "scale": 1.0, "aa": 0.0, "gg": 0.0,
+                   "leng": 0.0, "table": None}
         self.lattice = lattice
         self.grid = tuple(grid)
         self.const = hip.physconst(beam.freq)
@@ -77,9 +79,51 @@ class Simulation:
         self.nstep = 0
 
     def segments(self):
+        """The lattice's steps; a -41 element switches the read-in wake as
+        the loop reaches it."""
         for e in self.lattice:
+            if e.type == -41:
+                self.wake_switch(e)
+                continue
             for _ in range(e.nseg):
                 yield e
+
+    # --- the run loop's wake switches (src/Contrl/AccSimulator.f90:656-720,
+    # 759, 859-875, 960-980): flagwake, flagwakeread, flagbtw, scwk, the last
+    # SolRF/TWS element's aawk, ggwk, lengwk and the read-in table
+    def wake_switch(self, e: Element) -> None:
+        """A -41 element: v1 = scale, v2 = file of the wake functions (its
+        table is e.field), v3 > 0 on (< 10: no transverse wake), else off."""
+        w = self.wk
+        if not w["read"]:
+            w["on"] = 0
+        v = e.values
+        if v[1] > 0.0:
+            w["table"] = hip.RFData(e.field, fourier=False)
+        if v[2] > 0.0:
+            w.update(on=1, read=1, scale=v[0])
+            if v[2] < 10.0:
+                w["btw"] = 2
+        else:
+            w.update(on=0, read=0, scale=1.0)
+
+    def _wake_entry(self, e: Element) -> None:
+        w = self.wk
+        if not w["read"]:
+            w["on"] = 0
+        if e.type in (105, 106):
+            v = e.values
+            w["aa"], w["gg"], w["leng"] = v[11], v[12], v[13]
+            w["on"] = 1 if v[13] > 0.0 else 0
+
+    def wake(self):
+        """The izh_wake of the current step, or None."""
+        w = self.wk
+        if not w["on"]:
+            return None
+        if w["read"]:
+            return hip.wake(2, table=w["table"], flagbtw=w["btw"], scale=w["scale"])
+        return hip.wake(1, w["aa"], w["gg"], w["leng"], scale=w["scale"])
 
     def rfdata(self, e: Element):
         """The device library's copy of an RF cavity's field (one per field
@@ -93,9 +137,11 @@ class Simulation:
 
     def _enter(self, e: Element) -> float:
         """The zedge of the next segment of e: z at the element's entrance
-        (src/Contrl/AccSimulator.f90:1025-1026)."""
+        (src/Contrl/AccSimulator.f90:1025-1026); entering an element also
+        sets the wake switches."""
         if self._left == 0:
             self._left, self.zedge = e.nseg, self.z
+            self._wake_entry(e)
         self._left -= 1
         return self.zedge
 
@@ -133,18 +179,40 @@ class Simulation:
 
     def segment_args(self, e: Element, zedge: float = 0.0) -> hip.Segment:
         """The izh_segment of one space-charge step of element e entered at
-        zedge."""
+        zedge, with the wake the switches give it."""
         tau1 = 0.5 * e.length / e.nseg
         return hip.segment(tau1, e.type, e.values, e.mapstp, e.radius, e.radius,
-                           e.length, zedge, self.rfdata(e))
+                           e.length, zedge, self.rfdata(e),
+                           self.wake() if self.flagmap == 1 else None)
+
+    def _window(self, start: int, nsteps: int) -> list:
+        """The lattice items of steps start..start+nsteps-1 in order: the
+        elements of the steps and, as ("switch", element), the -41 elements
+        whose next step lies in the window."""
+        out, k = [], 0
+        for e in self.lattice:
+            if e.type == -41:
+                if start <= k < start + nsteps:
+                    out.append(("switch", e))
+                continue
+            for _ in range(e.nseg):
+                if start <= k < start + nsteps:
+                    out.append(e)
+                k += 1
+        return out
 
     def _segments_of(self, elements) -> list:
         """izh_segments of consecutive steps, with the entrance bookkeeping
-        advanced as the steps will advance it (each step moves z by 2 tau1)."""
+        and wake switches advanced as the steps will advance them (each step
+        moves z by 2 tau1)."""
         out, z = [], self.z
         for e in elements:
+            if isinstance(e, tuple):
+                self.wake_switch(e[1])
+                continue
             if self._left == 0:
                 self._left, self.zedge = e.nseg, z
+                self._wake_entry(e)
             self._left -= 1
             out.append(self.segment_args(e, self.zedge))
             tau1 = 0.5 * e.length / e.nseg
@@ -159,8 +227,8 @@ class Simulation:
         if self.flagmap != 1:
             raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
                                "Flagmap=2 runs step by step")
-        segs = list(self.segments())[start:start + nsteps]
-        self.z, mom = self.bunch.run(self.field, self.dom, self.z, self._segments_of(segs),
+        segs = self._segments_of(self._window(start, nsteps))
+        self.z, mom = self.bunch.run(self.field, self.dom, self.z, segs,
                                      self.beam.current, want_moments)
         self.nstep += len(segs)
         return mom

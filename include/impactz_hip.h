@@ -212,20 +212,58 @@ int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
 int izh_field_download(izh_field *f, int which, double *host);
 int izh_field_upload_rho(izh_field *f, const double *rho);
 
+/* ---- 1D wakefields (kick1wake_BeamBunch) ---- */
+
+/* The wakefield of a space-charge step, as run_AccSimulator switches it
+ * (src/Contrl/AccSimulator.f90:759,859-875,960-980): kind 1 the analytic
+ * short-range wake of a SolRF/TWS element with iris radius aa, gap gg and
+ * period leng (its Param(13:15), wakefield_FieldQuant); kind 2 the wake
+ * functions of a -41 element's file (read1wk_Data rows z, Wz, Wx, Wy, given
+ * as an izh_rfdata_table; wakefieldread_FieldQuant) with flagbtw (2, 3: no
+ * transverse wake; 4: no longitudinal); scale = scwk. */
+typedef struct {
+  int kind;
+  double aa, gg, leng;
+  const izh_rfdata *table;
+  int flagbtw;
+  double scale;
+} izh_wake;
+
+/* The wake fields of the bunch after izh_conv1st (its t-frame z binned on
+ * the Nz mesh lines, src/Contrl/AccSimulator.f90:1272-1338) and
+ * wakefield_FieldQuant / wakefieldread_FieldQuant (src/Appl/Field.f90:
+ * 590-1001), kept in the field for izh_kick1wake.  Call it after
+ * izh_update3O and before izh_cvbkforth1st, with the domain of the step;
+ * Nz <= 1024. */
+int izh_wakefield(izh_bunch *b, izh_field *f, const izh_compdom *dom, const izh_wake *w);
+
+/* kick1wake_BeamBunch (src/Appl/BeamBunch.f90:4694-5046): izh_kick1 with the
+ * wake fields of the last izh_wakefield added to each z plane of the grid
+ * field. */
+int izh_kick1wake(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
+                  int flagbc);
+
+/* The wake fields exwake, eywake, ezwake (nz doubles each, V/m) of the last
+ * izh_wakefield, for parity checks (the reference never writes them). */
+int izh_wake_download(izh_field *f, double *ex, double *ey, double *ez);
+
 /* One segment of a lattice: an element's half step and values. */
 typedef struct {
   double tau1;            /* half step, L/(2*nseg) */
   izh_element e;          /* the element */
   double xrad, yrad;      /* pipe aperture of the loss test */
+  const izh_wake *wake;   /* the step's wakefield (NULL: none) */
 } izh_segment;
 
 /* One segment of the reference loop: map1(tau1), conv1st, update_CompDom,
- * charge, update3O, cvbkforth1st, kick1(2 tau1), map1(tau1)
+ * charge, update3O, [wakefield,] cvbkforth1st, kick1 or kick1wake (2 tau1),
+ * map1(tau1)
  * (src/Contrl/AccSimulator.f90:1099-1488 with Flagmap=1, Flagbc=1,
- * Flagsubstep=0), for every element izh_map1 takes.  moments (NULL or 44
- * doubles) receives the diagnostic1_Output sums of the state after the step:
- * sums[37], amax[6], r2max.  bcurr is the deck current (no space charge
- * below 1e-10 A). */
+ * Flagsubstep=0), for every element izh_map1 takes; seg->wake (NULL: none)
+ * is the step's wakefield, which a SolRF with Param(15) > 0 must have.
+ * moments (NULL or 44 doubles) receives the diagnostic1_Output sums of the
+ * state after the step: sums[37], amax[6], r2max.  bcurr is the deck
+ * current (no space charge below 1e-10 A). */
 int izh_step(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
              const izh_segment *seg, double bcurr, double *moments);
 
@@ -303,7 +341,8 @@ int izh_probe_copy(int64_t nbytes, double *ms);
 #define IZH_T_DIAG 8       /* diagnostic1 moment reductions */
 #define IZH_T_PASS 9       /* izh_run: fused kick/push/moments/z->t/sort pass */
 #define IZH_T_FINAL 10     /* izh_run: per-step reductions, domain, scans */
-#define IZH_T_COUNT 11
+#define IZH_T_WAKE 11      /* 1D wakefield: binning and convolution */
+#define IZH_T_COUNT 12
 int izh_timer_enable(int on);
 int izh_timer_read(double ms[IZH_T_COUNT], int64_t launches[IZH_T_COUNT],
                    int reset);

@@ -240,6 +240,32 @@ def wake_solrf_lattice():
             B.Element(0.1, 4, 10, 0, 1.0)]
 
 
+def exe_wake_fixture(n=1000, grid=(16, 16, 32), beam=B.Beam()):
+    """The reference executable (ImpactZexe built from source) on a deck with
+    the -41 wake elements of wake_file_lattice: fort.18/24-26 of its own run
+    loop, wake switches included."""
+    pts = B.gaussian_bunch(n, seed=5, beam=beam)
+    lat = wake_file_lattice()
+    tmp = tempfile.mkdtemp(prefix="impz_wk_")
+    B.write_particle_in(os.path.join(tmp, "particle.in"), pts)
+    B.write_deck(os.path.join(tmp, "ImpactZ.in"), n, grid, beam, lat)
+    _run_exe(tmp)
+    out = {"pts_in": pts, "grid": np.array(grid),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0]),
+           "lattice": lattice_array(lat)}
+    for i, e in enumerate(lat):
+        if e.field is not None:
+            out[f"field{i}"] = np.asarray(e.field, dtype=np.float64)
+    for k in (18, 24, 25, 26):
+        out[f"fort{k}"] = R.read_fort(os.path.join(tmp, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "exe_wake.npz")
+    np.savez_compressed(path, **out)
+    print(f"exe_wake: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
 def switch_lattice():
     """Linear maps through a FODO half-period, a type -25 element switching
     to the Lorentz integrator (Flagmap = its mapstp, 2), then constant
@@ -323,6 +349,9 @@ def main():
                          "second -41 (dump 12)")
     fixture("wake_solrf", 1000, (16, 16, 32), wake_solrf_lattice(), 16, (6,))
     meta["wake_solrf"] = "SolRF with its analytic short-range wake (Param(13:15)), dump 6"
+    exe_wake_fixture()
+    meta["exe_wake"] = ("ImpactZexe run of the wake_file lattice: -41 read-in wake on with "
+                        "its transverse part, then longitudinal only; fort.18/24-26")
     exe_nonlinear_fixture()
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "

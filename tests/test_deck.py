@@ -203,12 +203,31 @@ def test_deck_rf_cavities_and_field_files(tmp_path):
     withf = with_fields(prog, str(tmp_path))
     assert np.array_equal(withf[0][1].field, tab) and np.array_equal(withf[1][1].field, coef)
     for line, what in (("0.2 4 20 103 1e6 324e6 0.0 -1.0 0.02 /", "ideal RF"),
-                       ("0.2 4 20 105 1e6 324e6 0.0 2.0 0.02 0 0 0 0 0 0.4 0.01 0.02 0.03 /",
-                        "wakefield"),
                        ("0.2 4 20 106 1e6 324e6 0.0 2.0 0.02 /", "type 106")):
         _write(p, HEADER + line + "\n")
         with pytest.raises(DeckError, match=what):
             program_of(read_deck(str(p)))
     _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.2 4 20 103 1e6 324e6 0 1 0.02 /\n")
     with pytest.raises(DeckError, match="Flagmap=2"):
+        program_of(read_deck(str(p)))
+
+
+def test_deck_wake_switches(tmp_path):
+    """-41 lines become wake switches carrying their wake-function file
+    (rfdataN.in, N = v2); SolRF lines keep their wake values v12..v14."""
+    from impactz.deck import DeckError, WakeSwitch, program_of, read_deck, with_fields
+    p = tmp_path / "ImpactZ.in"
+    _write(p, HEADER + """0.0 0 1 -41 1.5 3.0 20.0 /
+0.4 4 10 0 1.0 /
+0.2 6 20 105 3.0e6 324.0e6 45.0 2.0 0.015 0 0 0 0 0 0.4 10.75d-3 30.33d-3 33.321d-3 /
+""")
+    _write(tmp_path / "rfdata3.in", "0.0 1e14 1e18 2e18\n0.001 2e14 3e18 4e18\n")
+    _write(tmp_path / "rfdata2.in", "3.0\n0.0\n0.2\n0.2\n0.4\n-1.0\n0.05\n")
+    prog = with_fields(program_of(read_deck(str(p)))[0], str(tmp_path))
+    sw = prog[0][1]
+    assert isinstance(sw, WakeSwitch) and sw.element.values[:3] == [1.5, 3.0, 20.0]
+    assert sw.element.field.shape == (2, 4) and sw.element.field[1, 2] == 3e18
+    assert prog[2][1].values[11:14] == [10.75e-3, 30.33e-3, 33.321e-3]
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.0 0 1 -41 1.5 3.0 20.0 /\n")
+    with pytest.raises(DeckError, match="wakefields under Flagmap=2"):
         program_of(read_deck(str(p)))

@@ -634,3 +634,24 @@ def test_wake_convolution_closed_forms(oracle):
     out3 = oracle.wakefield(c, pts, rng, nz, b.current, 14, 2.0, b.charge, w)
     assert np.abs(out3[0] - 2 * out2[0]).max() <= 1e-13 * np.abs(out3[0]).max()
     assert np.abs(out2[0]).max() > 0
+
+
+def test_oracle_matches_executable_wake_run(oracle):
+    """The reference executable's own run of the -41 wake lattice
+    (tests/golden/exe_wake.npz): the wake switches of its element loop and
+    the read-in wake end to end."""
+    d = load_golden("exe_wake")
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
+    k = 0
+    for k, e in enumerate(run.segments(), 1):
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+    assert k == d["fort24"].shape[0] - 1
