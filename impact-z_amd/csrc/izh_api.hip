@@ -424,12 +424,7 @@ int izh_rfdata_table(const double *rows4, int n, izh_rfdata **out) {
 }
 
 int izh_rfdata_fourier(const double *coef, int n, izh_rfdata **out) {
-  if (!coef || n < 4 || !out) return fail(IZH_ERR_ARG, "izh_rfdata_fourier: bad argument");
-  const int ncoef = (int)(coef[0] + 0.1);
-  // getaxfldE_SolRF reads Fcoef(5 .. 4 + numpar1) (src/Appl/SolRF.f90:385-417)
-  if (ncoef < 1 || 4 + ncoef > n + (n % 2 == 0 ? 1 : 0))
-    return fail(IZH_ERR_ARG, "izh_rfdata_fourier: %d values hold fewer than the %d "
-                "coefficients the first value announces", n, ncoef);
+  if (!coef || n < 1 || !out) return fail(IZH_ERR_ARG, "izh_rfdata_fourier: bad argument");
   auto *d = new izh_rfdata();
   d->fourier = 1;
   d->coef.assign(coef, coef + n);
@@ -731,6 +726,12 @@ static const char *linear_element(const izh_const *c, const izh_element &el,
     if ((el.bitype == 105) != (el.rf->fourier != 0))
       return el.bitype == 105 ? "SolRF takes Fourier coefficients (izh_rfdata_fourier)"
                               : "this RF cavity takes a field table (izh_rfdata_table)";
+    // getaxfldE_SolRF reads Fcoef(5 .. 4 + numpar1), numpar1 = Fcoef(1)
+    // (src/Appl/SolRF.f90:385-417)
+    if (el.bitype == 105 && (el.rf->coef.size() < 4 ||
+                             (int)(el.rf->coef[0] + 0.1) < 1 ||
+                             4 + (int)(el.rf->coef[0] + 0.1) > (int)el.rf->coef.size()))
+      return "SolRF coefficients hold fewer values than their first value announces";
     if (!(v[1] > 0.0)) return "RF frequency must be positive";
     e.rf = el.rf;
     e.zedge = el.zedge;
@@ -1008,6 +1009,37 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
 
 // ------------------------------------------------ Lorentz integrator
 
+}  // extern "C"
+
+namespace izh {
+// The on-axis field of getfld_DTL and its first three derivatives at z
+// (src/Appl/DTL.f90:811-833): read3_Data's series (Fcoef(1) = a0, then
+// cosine and sine pairs over the element length, about its middle) with
+// the reference's REAL(4) pi, times the field scale.
+void rf_fourier_axis(const izh_rfdata &d, double escale, double length, double zedge,
+                     double z, double ax[4]) {
+  const double pi = (double)(2 * std::asin(1.0f));
+  const double *F = d.coef.data() - 1;
+  const int ndata = (int)d.coef.size();
+  const double zz = z - (zedge + length / 2);
+  double ez1 = F[1] / 2, ezp1 = 0.0, ezpp1 = 0.0, ezppp = 0.0;
+  for (int i = 2; i <= (ndata - 1) / 2 + 1; i++) {
+    const double k = (i - 1) * 2 * pi / length, arg = (i - 1) * 2 * pi * zz / length;
+    const double cs = std::cos(arg), sn = std::sin(arg);
+    ez1 = ez1 + F[2 * i - 2] * cs + F[2 * i - 1] * sn;
+    ezp1 = ezp1 + k * (-F[2 * i - 2] * sn + F[2 * i - 1] * cs);
+    ezpp1 = ezpp1 + (k * k) * (-F[2 * i - 2] * cs - F[2 * i - 1] * sn);
+    ezppp = ezppp + (k * k * k) * (F[2 * i - 2] * sn - F[2 * i - 1] * cs);
+  }
+  ax[0] = ez1 * escale;
+  ax[1] = ezp1 * escale;
+  ax[2] = ezpp1 * escale;
+  ax[3] = ezppp * escale;
+}
+}  // namespace izh
+
+extern "C" {
+
 int izh_drift2(izh_bunch *b, double *z, double tau) {
   if (!b || !z) return fail(IZH_ERR_ARG, "izh_drift2: bad argument");
   if (b->frame != kFrameZ)
@@ -1042,9 +1074,18 @@ int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_fiel
   }
   Kick2 k{};
   k.e.type = e->bitype;
-  for (int a = 0; a < 4; a++) k.e.v[a] = e->v[a];
+  for (int a = 0; a < 9; a++) k.e.v[a] = e->v[a];
   k.e.length = e->length;
   k.e.zedge = e->zedge;
+  if (e->bitype >= 101 && e->bitype <= 104) {
+    if (!e->rf || !e->rf->fourier)
+      return fail(IZH_ERR_ARG, "izh_kick2: bitype %d needs its Fourier coefficients "
+                  "(izh_rfdata_fourier, as read3_Data reads them)", e->bitype);
+    rf_fourier_axis(*e->rf, e->v[0], e->length, e->zedge, z, k.e.ax);
+    k.e.ww = e->v[1] / b->c.scfreq;
+    k.e.xl = b->c.scxl / k.e.ww;
+    k.e.theta0 = e->v[2] * (double)std::asin(1.0f) / 90;
+  }
   k.gam = -b->refpt[5];
   k.clight = b->c.clight;
   k.xl = b->c.scxl;

@@ -264,8 +264,14 @@ namespace izh {
 // 297-322, Sol.f90:388-435).  pos = (x, y in m, z, t); f = E (V/m), B (T).
 struct ElemField {
   int type;
-  double v[4];
+  double v[9];          // Param(2:10)
   double length, zedge;
+  // RF cavities 101-104: the on-axis field and its first three derivatives
+  // at the kick's z (the same for every particle: scatter2 evaluates the
+  // field at pos(3) = z), the angular frequency in units of the reference
+  // one, Scxl/ww and the phase theta0 (src/Appl/DTL.f90:780-838)
+  double ax[4];
+  double ww, xl, theta0;
 };
 
 __host__ __device__ inline void elem_field(const ElemField &e, double scxl, double clight,
@@ -296,16 +302,39 @@ __host__ __device__ inline void elem_field(const ElemField &e, double scxl, doub
       f[4] = -bb / 2 * pos[1];
       f[5] = b0 + bb * (zz - zshift);
     }
+  } else if (e.type >= 101 && e.type <= 104) {
+    // getfld_DTL / _CCDTL / _CCL / _SC (src/Appl/DTL.f90:769-855): the DTL's
+    // drift-tube quadrupoles, then the paraxial expansion of the on-axis
+    // field with the REAL(4) speed of light of the reference
+    if (e.type == 101) {
+      const double zz = pos[2] - e.zedge;
+      const double bgrad = zz < e.v[5] ? e.v[6] : (zz < e.length - e.v[7] ? 0.0 : e.v[8]);
+      f[3] = bgrad * pos[1];
+      f[4] = bgrad * pos[0];
+    }
+    const double clite = (double)299792458.e0f;
+    const double ez1 = e.ax[0], ezp1 = e.ax[1], ezpp1 = e.ax[2], ezppp = e.ax[3];
+    const double tmpcos = cos(e.ww * pos[3] + e.theta0);
+    const double tmpsin = sin(e.ww * pos[3] + e.theta0);
+    const double f1 = -(ezpp1 + ez1 / e.xl / e.xl) / 4;
+    const double f1p = -(ezppp + ezp1 / e.xl / e.xl) / 4;
+    const double r2 = pos[0] * pos[0] + pos[1] * pos[1];
+    f[0] += -pos[0] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+    f[1] += -pos[1] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+    f[2] += (ez1 + f1 * r2) * tmpcos;
+    f[3] += pos[1] / (e.xl * clite) * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+    f[4] += -pos[0] / (e.xl * clite) * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
   }
 }
 
 // The element types with an analytic field on this path, or a message.
 inline const char *elem_field_unsupported(int bitype, const double v[4]) {
   if (bitype == 0 || bitype == 2 || bitype == 3) return nullptr;
+  if (bitype >= 101 && bitype <= 104) return nullptr;   // with Fourier coefficients
   if (bitype == 1)
     return v[1] > 0.0 ? "quadrupole field profile from a file (file ID > 0)" : nullptr;
   return "element type not on this path (0 drift, 1 quadrupole, 2 constant focusing, "
-         "3 solenoid)";
+         "3 solenoid, 101-104 RF cavities)";
 }
 
 // Implicit Lorentz push of scatter2/scatter20 (src/Appl/BeamBunch.f90:

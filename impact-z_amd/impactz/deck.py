@@ -13,7 +13,7 @@ K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
 (type 2) and solenoids (type 3), under the linear map integrator (Flagmap=1)
 or the nonlinear Lorentz integrator (Flagmap=2), and the RF cavities DTL,
 CCDTL, CCL, SC and SolRF (types 101-105) with their rfdataN.in fields under
-Flagmap=1; a type -25 element switches
+Flagmap=1 (101-104 also under Flagmap=2); a type -25 element switches
 the integrator to its mapstp for the elements after it
 (src/Contrl/AccSimulator.f90:807-809).  Zero-length diagnostic elements that
 do not change the particles (types -3 to -6, -8) are skipped; -2 and -7
@@ -231,10 +231,10 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3])))
         elif 101 <= e.type <= 105:
             vv = (v + [0.0] * 14)[:14]
-            if flagmap != 1:
-                raise DeckError(f"element {k}: RF cavity type {e.type} under Flagmap=2 "
-                                "(field maps from files) is not on this path")
-            if vv[3] < 0.0:
+            if flagmap != 1 and e.type == 105:
+                raise DeckError(f"element {k}: SolRF under Flagmap=2 (getfld_SolRF) is not "
+                                "on this path")
+            if flagmap == 1 and vv[3] < 0.0:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {vv[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
@@ -333,18 +333,23 @@ def read_rfdata(path: str, per_record: int) -> np.ndarray:
     return a if per_record > 1 else a.ravel()
 
 
-def with_fields(prog, workdir: str):
+def with_fields(prog, workdir: str, flagmap: int = 1):
     """The program with each RF cavity's rfdataN.in (N = its file ID v4)
     attached, read once per file and reader as run_AccSimulator does
-    (src/Contrl/AccSimulator.f90:818-849)."""
+    (src/Contrl/AccSimulator.f90:818-849): Fourier coefficients (read3_Data)
+    for SolRF and for every cavity under Flagmap=2, else the field table
+    (read1_Data).  ``flagmap`` is the deck's; -25 switches change it."""
     cache, out = {}, []
     for index, p in prog:
+        if isinstance(p, MapSwitch):
+            flagmap = p.flagmap
         if isinstance(p, Element) and 101 <= p.type <= 105:
             fid = int(p.v4 + 0.1)
-            key = (fid, p.type == 105)
+            fourier = p.type == 105 or flagmap != 1
+            key = (fid, fourier)
             if key not in cache:
                 cache[key] = read_rfdata(os.path.join(workdir, rfdata_name(fid)),
-                                         1 if p.type == 105 else 4)
+                                         1 if fourier else 4)
             p = dataclasses.replace(p, field=cache[key])
         elif isinstance(p, WakeSwitch) and p.element.v2 > 0.0:
             # read1wk_Data (src/DataStruct/Data.f90:559-633)
@@ -382,7 +387,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
     Returns the diagnostic rows."""
     deck = read_deck(os.path.join(workdir, "ImpactZ.in"))
     prog, notes = program_of(deck)
-    prog = with_fields(prog, workdir)
+    prog = with_fields(prog, workdir, deck.flagmap)
     for n in notes:
         print(f"[impactz.deck] {n}", flush=True)
     out = outdir or workdir

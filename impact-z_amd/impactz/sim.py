@@ -130,9 +130,12 @@ class Simulation:
         array), or None."""
         if e.field is None:
             return None
-        key = id(e.field)
+        # read3_Data's coefficients for SolRF and for every cavity under the
+        # Lorentz integrator (src/Contrl/AccSimulator.f90:826-847)
+        fourier = e.type == 105 or self.flagmap != 1
+        key = (id(e.field), fourier)
         if key not in self._rf:
-            self._rf[key] = (hip.RFData.for_element(e), e.field)
+            self._rf[key] = (hip.RFData(e.field, fourier=fourier), e.field)
         return self._rf[key][0]
 
     def _enter(self, e: Element) -> float:
@@ -150,8 +153,8 @@ class Simulation:
         if self.flagmap == 2:
             self.z, mom = self.bunch.step2(
                 self.field, self.dom, self.z, 0.5 * e.length / e.nseg,
-                hip.element(e.type, e.values, e.length, zedge), e.radius, e.radius,
-                self.beam.current, want_moments)
+                hip.element(e.type, e.values, e.length, zedge, rf=self.rfdata(e)), e.radius,
+                e.radius, self.beam.current, want_moments)
         else:
             self.z, mom = self.bunch.step(self.field, self.dom, self.z,
                                           self.segment_args(e, zedge), self.beam.current,

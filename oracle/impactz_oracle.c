@@ -1026,10 +1026,64 @@ void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], doubl
 /* getfld_BeamLineElem for the analytic fields (src/Appl/BeamLineElem.f90:
  * 827-870): E (V/m) and B (T) at pos = (x, y, z, t).  Returns -1 for the
  * elements not restated. */
+/* getfld_DTL / _CCDTL / _CCL / _SC (src/Appl/DTL.f90:769-855, CCL.f90,
+ * CCDTL.f90, SC.f90): the DTL's drift-tube quadrupoles, then the on-axis
+ * field of read3_Data's Fourier series (Fcoef(1) = a0, then cosine and
+ * sine pairs over the element length about its middle) and its paraxial
+ * expansion, with the REAL(4) constants of the reference (clite, pi,
+ * asin(1.0)). */
+static int rf_field(const or_const *c, int bitype, const double *v, double length,
+                    double zedge, const or_rfdata *rf, const double pos[4], double f[6]) {
+  const double clite = (double)299792458.e0f, pi = (double)(2 * asinf(1.0f));
+  const double escale = v[0], ww = v[1] / c->scfreq, xl = c->scxl / ww;
+  const double theta0 = v[2] * (double)asinf(1.0f) / 90;
+  const double zmid = zedge + length / 2;
+  const double *F = rf->fcoef - 1;
+  double zz, ez1, ezp1 = 0.0, ezpp1 = 0.0, ezppp = 0.0, tt, tmpcos, tmpsin, f1, f1p, r2;
+  int i;
+  if (!rf || !rf->fourier) return -1;
+  if (bitype == 101) {
+    double bgrad;
+    zz = pos[2] - zedge;
+    if (zz < v[5]) bgrad = v[6];
+    else if (zz < length - v[7]) bgrad = 0.0;
+    else bgrad = v[8];
+    f[3] = bgrad * pos[1];
+    f[4] = bgrad * pos[0];
+  }
+  zz = pos[2] - zmid;
+  ez1 = F[1] / 2;
+  for (i = 2; i <= (rf->n - 1) / 2 + 1; i++) {
+    double k = (i - 1) * 2 * pi / length, arg = (i - 1) * 2 * pi * zz / length;
+    double cs = cos(arg), sn = sin(arg);
+    ez1 = ez1 + F[2 * i - 2] * cs + F[2 * i - 1] * sn;
+    ezp1 = ezp1 + k * (-F[2 * i - 2] * sn + F[2 * i - 1] * cs);
+    ezpp1 = ezpp1 + (k * k) * (-F[2 * i - 2] * cs - F[2 * i - 1] * sn);
+    ezppp = ezppp + (k * k * k) * (F[2 * i - 2] * sn - F[2 * i - 1] * cs);
+  }
+  ez1 = ez1 * escale;
+  ezp1 = ezp1 * escale;
+  ezpp1 = ezpp1 * escale;
+  ezppp = ezppp * escale;
+  tt = pos[3];
+  tmpcos = cos(ww * tt + theta0);
+  tmpsin = sin(ww * tt + theta0);
+  f1 = -(ezpp1 + ez1 / xl / xl) / 4;
+  f1p = -(ezppp + ezp1 / xl / xl) / 4;
+  r2 = pos[0] * pos[0] + pos[1] * pos[1];
+  f[0] += -pos[0] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+  f[1] += -pos[1] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+  f[2] += (ez1 + f1 * r2) * tmpcos;
+  f[3] += pos[1] / (xl * clite) * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+  f[4] += -pos[0] / (xl * clite) * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+  return 0;
+}
+
 int or_elem_field(const or_const *c, int bitype, const double *v, double length,
-                  double zedge, const double pos[4], double f[6]) {
+                  double zedge, const or_rfdata *rf, const double pos[4], double f[6]) {
   int k;
   for (k = 0; k < 6; k++) f[k] = 0.0;
+  if (bitype >= 101 && bitype <= 104) return rf_field(c, bitype, v, length, zedge, rf, pos, f);
   if (bitype == 0) return 0;  /* getfld_DriftTube: no field */
   if (bitype == 1) {
     /* getfld_Quadrupole (src/Appl/Quadrupole.f90:369-402), file ID <= 0 */
@@ -1109,7 +1163,8 @@ static void lorentz_push(double *p, double ex, double ey, double ez, double bx, 
 int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
-             int bitype, const double *v, double length, double zedge, double z) {
+             int bitype, const double *v, double length, double zedge,
+             const or_rfdata *rf, double z) {
   /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508) with scatter2
    * (:2510-3071) or, without current, scatter20 (:3075-3531), for the
    * elements with analytic fields */
@@ -1122,7 +1177,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
   double pos[4] = {0.0, 0.0, z, tg}, fld[6];
   double ez0, tmpgamma0;
   int64_t i;
-  if (or_elem_field(c, bitype, v, length, zedge, pos, fld) != 0) return -1;
+  if (or_elem_field(c, bitype, v, length, zedge, rf, pos, fld) != 0) return -1;
   ez0 = fld[2];
   tmpgamma0 = gam + 0.5 * tau * qmcc * ez0;
   if (curr > 0.0) {
@@ -1155,7 +1210,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
       pos[1] = P(3, i) * xl;
       pos[2] = z;
       pos[3] = P(5, i) + tg;
-      or_elem_field(c, bitype, v, length, zedge, pos, fld);
+      or_elem_field(c, bitype, v, length, zedge, rf, pos, fld);
       lorentz_push(&P(1, i), tmpscale * exn + fld[0], tmpscale * eyn + fld[1],
                    tmpscale * ezn + fld[2], -beta0 / c->clight * tmpscale * eyn + fld[3],
                    beta0 / c->clight * tmpscale * exn + fld[4], fld[5], tau, c->clight,
@@ -1168,7 +1223,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
       pos[1] = P(3, i) * xl;
       pos[2] = z;
       pos[3] = P(5, i) + tg;
-      or_elem_field(c, bitype, v, length, zedge, pos, fld);
+      or_elem_field(c, bitype, v, length, zedge, rf, pos, fld);
       lorentz_push(&P(1, i), fld[0], fld[1], fld[2], fld[3], fld[4], fld[5], tau,
                    c->clight, tmpgamma0, gam, qmcc, ez0);
     }
@@ -1203,7 +1258,8 @@ int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
   }
   if (rc == 0)
     rc = or_kick2(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, curr, tau2,
-                  st->mass, st->charge, e->bitype, e->v, e->length, e->zedge, st->z);
+                  st->mass, st->charge, e->bitype, e->v, e->length, e->zedge, e->rf,
+                  st->z);
   if (rc == 0) or_drift2(c, pts, n, st->refpt, &st->z, tau2);
   if (!rho_out) free(rho);
   if (!phi_out) free(phi);

@@ -138,11 +138,12 @@ void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], doubl
 
 /* getfld_BeamLineElem (src/Appl/BeamLineElem.f90:827-870) for the elements
  * with analytic fields: drift (none), quadrupole (file ID <= 0: B = G (y, x, 0)),
- * constant focusing and solenoid (with its linear end ramps).  pos = (x, y
+ * constant focusing, solenoid (with its linear end ramps) and the RF
+ * cavities 101-104 on read3_Data's Fourier coefficients (rf).  pos = (x, y
  * in m, z, t); f = (Ex, Ey, Ez, Bx, By, Bz).  zedge is the element's entrance
  * z.  Returns -1 for anything else. */
 int or_elem_field(const or_const *c, int bitype, const double *v, double length,
-                  double zedge, const double pos[4], double f[6]);
+                  double zedge, const or_rfdata *rf, const double pos[4], double f[6]);
 
 /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508): the space-charge E of
  * phi (when curr > 0, scatter2) plus the element's field, applied by the
@@ -151,7 +152,8 @@ int or_elem_field(const or_const *c, int bitype, const double *v, double length,
 int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
-             int bitype, const double *v, double length, double zedge, double z);
+             int bitype, const double *v, double length, double zedge,
+             const or_rfdata *rf, double z);
 
 /* diagnostic1_Output sums (src/Contrl/Output.f90:119-245): 37 sums in the
  * reference's tmplc order, 6 abs-maxima, max of x^2+y^2. */

@@ -193,12 +193,12 @@ def lib() -> C.CDLL:
         L.or_step.restype = C.c_int64
         L.or_drift2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double]
         L.or_elem_field.argtypes = [C.POINTER(OrConst), C.c_int, _dp, C.c_double,
-                                    C.c_double, _dp, _dp]
+                                    C.c_double, C.POINTER(OrRFData), _dp, _dp]
         L.or_elem_field.restype = C.c_int
         L.or_kick2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.c_int, C.c_int,
                                C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_int, _dp, C.c_double,
-                               C.c_double, C.c_double]
+                               C.c_double, C.POINTER(OrRFData), C.c_double]
         L.or_kick2.restype = C.c_int
         L.or_step2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.POINTER(OrState),
                                C.c_int, C.c_int, C.c_int, C.c_double,
@@ -268,23 +268,26 @@ def drift2(c, pts, refpt, z, tau):
     return zz.value
 
 
-def elem_field(c, bitype, v, length, zedge, pos):
+def elem_field(c, bitype, v, length, zedge, pos, rf: RFData | None = None):
     f = np.zeros(6)
     vv = element_values(v)
     p = np.ascontiguousarray(pos, dtype=np.float64)
-    if lib().or_elem_field(C.byref(c), bitype, _p(vv), length, zedge, _p(p), _p(f)) != 0:
+    if lib().or_elem_field(C.byref(c), bitype, _p(vv), length, zedge,
+                           C.byref(rf.c) if rf is not None else None, _p(p), _p(f)) != 0:
         raise ValueError(f"no analytic field for element type {bitype}")
     return f
 
 
-def kick2(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, bitype, v, length, zedge, z):
+def kick2(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, bitype, v, length, zedge, z,
+          rf: RFData | None = None):
     """kick2_BeamBunch in place on pts (and refpt(6))."""
     nz, ny, nx = phi.shape
     vv = element_values(v)
     if lib().or_kick2(C.byref(c), _p(pts), pts.shape[0], nx, ny, nz, _p(phi),
                       _p(np.ascontiguousarray(rng, dtype=np.float64)),
                       _p(np.ascontiguousarray(ms, dtype=np.float64)), _p(refpt), curr, tau,
-                      mass, chrg, bitype, _p(vv), length, zedge, z) != 0:
+                      mass, chrg, bitype, _p(vv), length, zedge,
+                      C.byref(rf.c) if rf is not None else None, z) != 0:
         raise ValueError(f"no analytic field for element type {bitype}")
 
 
@@ -447,9 +450,12 @@ class Run:
         """The element's field arrays (cached per element)."""
         if e.field is None:
             return None
-        key = id(e.field)
+        # read3_Data's coefficients for SolRF, and for every cavity under the
+        # Lorentz integrator (src/Contrl/AccSimulator.f90:826-847)
+        fourier = e.type == 105 or self.flagmap != 1
+        key = (id(e.field), fourier)
         if key not in self._rf:
-            self._rf[key] = (RFData(e.field, fourier=e.type == 105), e.field)
+            self._rf[key] = (RFData(e.field, fourier=fourier), e.field)
         return self._rf[key][0]
 
     def step(self, e, want_fields=False):

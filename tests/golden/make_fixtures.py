@@ -266,6 +266,57 @@ def exe_wake_fixture(n=1000, grid=(16, 16, 32), beam=B.Beam()):
     return out
 
 
+def nl_rf_lattice():
+    """The DTL (with its quadrupoles) and a CCL under the Lorentz integrator:
+    their fields from read3_Data's coefficients (a0, then cosine and sine
+    pairs over the element length)."""
+    coef = np.array([0.2, 1.0, 0.0, -0.3, 0.05, 0.1, -0.02])
+    L = 0.1
+    return [B.Element(0.05, 4, 10, 0, 1.0),
+            B.Element(L, 12, 20, 101, 2.5e6, 324.0e6, 222.0, 7.0,
+                      (0.0065, 0.02, 60.0, 0.02, -60.0), field=coef),
+            B.Element(0.05, 4, 10, 0, 1.0),
+            B.Element(L, 8, 20, 103, 2.0e6, 324.0e6, 150.0, 7.0, (0.0065,), field=coef)]
+
+
+def example1_fixture():
+    """The reference's own examples/Example1 (H- through nine DTL cells,
+    two quadrupoles and a drift, Lorentz integrator, 200 particles) run by
+    ImpactZexe built from source.  Its inputs are kept as data: the
+    particles of its particle.in, the element values and the Fourier
+    coefficients of its rfdataN.in files."""
+    from impactz.deck import lattice_of, read_deck, read_particle_in, read_rfdata
+    ex = "/root/reference/examples/Example1"
+    deck = read_deck(os.path.join(ex, "ImpactZ.in"))
+    lat, _ = lattice_of(deck)
+    fields = {}
+    for i, e in enumerate(lat):
+        if 101 <= e.type <= 105:
+            fields[i] = read_rfdata(os.path.join(ex, B.rfdata_name(B.field_file_id(e))), 1)
+    lat = [B.Element(e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2, e.v3, e.v4, e.more,
+                     field=fields.get(i)) for i, e in enumerate(lat)]
+    pts = read_particle_in(os.path.join(ex, "particle.in"))
+    beam = deck.beam()
+    grid = deck.grid
+    tmp = tempfile.mkdtemp(prefix="impz_ex1_")
+    B.write_particle_in(os.path.join(tmp, "particle.in"), pts)
+    B.write_deck(os.path.join(tmp, "ImpactZ.in"), pts.shape[0], grid, beam, lat, flagmap=2)
+    _run_exe(tmp)
+    out = {"pts_in": pts, "grid": np.array(grid), "flagmap": np.array(2),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0]),
+           "lattice": lattice_array(lat)}
+    for i, f in fields.items():
+        out[f"field{i}"] = f
+    for k in (18, 24, 25, 26):
+        out[f"fort{k}"] = R.read_fort(os.path.join(tmp, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "example1.npz")
+    np.savez_compressed(path, **out)
+    print(f"example1: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
 def switch_lattice():
     """Linear maps through a FODO half-period, a type -25 element switching
     to the Lorentz integrator (Flagmap = its mapstp, 2), then constant
@@ -349,6 +400,12 @@ def main():
                          "second -41 (dump 12)")
     fixture("wake_solrf", 1000, (16, 16, 32), wake_solrf_lattice(), 16, (6,))
     meta["wake_solrf"] = "SolRF with its analytic short-range wake (Param(13:15)), dump 6"
+    fixture("nl_rf", 1000, (16, 16, 16), nl_rf_lattice(), 28, (6, 22), flagmap=2)
+    meta["nl_rf"] = ("Flagmap=2 through a DTL (with quadrupoles) and a CCL on Fourier "
+                     "coefficients (rfdata7.in); dumps at steps 6 (DTL) and 22 (CCL)")
+    example1_fixture()
+    meta["example1"] = ("examples/Example1 of the reference run by ImpactZexe: inputs kept as "
+                        "data (particles, element values, Fourier coefficients), fort.18/24-26")
     exe_wake_fixture()
     meta["exe_wake"] = ("ImpactZexe run of the wake_file lattice: -41 read-in wake on with "
                         "its transverse part, then longitudinal only; fort.18/24-26")

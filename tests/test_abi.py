@@ -113,13 +113,15 @@ def test_rf_maplinear_matches_oracle(oracle, idx, z_in):
 
 def test_rfdata_is_checked():
     from impactz import hip
-    with pytest.raises(hip.IZHError, match="announces"):
-        hip.RFData(np.array([9.0, 0.0, 1.0, 1.0, 0.5, 0.2]), fourier=True)
     tab = np.array([[0.0, 1.0, 0.0, 0.0], [1.0, 1.0, 0.0, 0.0]])
     c = hip.physconst(324e6)
     r = np.zeros(6)
     r[5] = -1.16
     v = [1e6, 324e6, 0.0, 1.0, 1.0]
+    with pytest.raises(hip.IZHError, match="announces"):
+        hip.element_maplinear(c, 0.01, 105, v, 10, r, 1.0, 938e6, length=1.0,
+                              rf=hip.RFData(np.array([9.0, 0.0, 1.0, 1.0, 0.5, 0.2]),
+                                            fourier=True))
     with pytest.raises(hip.IZHError, match="needs its field"):
         hip.element_maplinear(c, 0.01, 103, v, 10, r, 1.0, 938e6, length=1.0)
     with pytest.raises(hip.IZHError, match="Fourier"):

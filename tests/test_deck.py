@@ -207,9 +207,16 @@ def test_deck_rf_cavities_and_field_files(tmp_path):
         _write(p, HEADER + line + "\n")
         with pytest.raises(DeckError, match=what):
             program_of(read_deck(str(p)))
-    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.2 4 20 103 1e6 324e6 0 1 0.02 /\n")
-    with pytest.raises(DeckError, match="Flagmap=2"):
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") +
+           "0.2 4 20 105 1e6 324e6 0 1 0.02 /\n")
+    with pytest.raises(DeckError, match="SolRF under Flagmap=2"):
         program_of(read_deck(str(p)))
+    # under Flagmap=2 the DTL family reads Fourier coefficients (read3_Data)
+    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") +
+           "0.2 4 20 103 1e6 324e6 0 1 0.02 /\n")
+    _write(tmp_path / "rfdata1.in", "0.5\n1.0\n0.0\n")
+    prog = with_fields(program_of(read_deck(str(p)))[0], str(tmp_path), 2)
+    assert prog[0][1].field.tolist() == [0.5, 1.0, 0.0]
 
 
 def test_deck_wake_switches(tmp_path):

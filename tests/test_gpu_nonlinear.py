@@ -14,7 +14,7 @@ from test_gpu_parity import TOL, record
 pytestmark = pytest.mark.gpu
 
 CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
-         ("nl_cf_sol", 11)]
+         ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22)]
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -45,7 +45,9 @@ def test_gpu_nonlinear_stages_match_reference(gpu, name, step):
     field.update3O(dom)
     err["phi"] = rel_err(field.download(hip.PHI), d[s + "phi"])
     b.cvbkforth1st()
-    b.kick2(z, 2 * tau1, hip.element(e.type, e.values, e.length, element_entrance(d, step)), field,
+    rf = None if e.field is None else hip.RFData(e.field, fourier=True)
+    b.kick2(z, 2 * tau1,
+            hip.element(e.type, e.values, e.length, element_entrance(d, step), rf=rf), field,
             dom)
     err["kick2"] = col_err(b.getpts(), d[s + "p_kick"])
     z = b.drift2(z, 2 * tau1)
@@ -57,7 +59,7 @@ def test_gpu_nonlinear_stages_match_reference(gpu, name, step):
         assert v < TOL, (k, v)
 
 
-@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol"])
+@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf"])
 def test_gpu_nonlinear_trajectory_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)
@@ -89,3 +91,32 @@ def test_gpu_kick2_without_space_charge_and_refusals(gpu):
     for bt, v, what in ((1, [10.0, 1.0], "file"), (5, [2.0], "not on this path")):
         with pytest.raises(hip.IZHError, match=what):
             b.kick2(0.0, 0.01, hip.element(bt, v, 0.1, 0.0), None, None)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_gpu_reference_example1_deck(tmp_path, gpu, fused):
+    """The reference's examples/Example1 (H- through nine DTL cells on their
+    Fourier fields, Lorentz integrator, 1002 steps) as a deck through
+    impactz.deck, against the fort.24-26 its executable wrote
+    (tests/golden/example1.npz).  fused only fuses Flagmap=1 stretches, so
+    both settings run the steps one by one here."""
+    from impactz.beam import write_deck, write_particle_in
+    from impactz.deck import run_deck
+    d = load_golden("example1")
+    grid = tuple(int(v) for v in d["grid"])
+    write_deck(str(tmp_path / "ImpactZ.in"), d["pts_in"].shape[0], grid, fixture_beam(d),
+               fixture_lattice(d), flagmap=2)
+    write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
+    print(f"example1 deck fused={fused}", flush=True)
+    run_deck(str(tmp_path), fused=fused)
+    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (18, 24, 25, 26)}
+    assert f[24].shape == d["fort24"].shape
+    worst = 0.0
+    for k in range(1, f[24].shape[0]):
+        got = np.concatenate([[0.0] * 5, f[24][k, 1:], f[25][k, 1:], f[26][k, 1:]])
+        ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
+        worst = max(worst, diag_row_err(got, ref, ref_is_fort=True))
+    gam = np.abs(f[18][:, 2] / d["fort18"][:, 2] - 1).max()
+    record(test="example1_deck", fused=fused, steps=f[24].shape[0] - 1, moments=worst,
+           gamma=float(gam))
+    assert worst < TOL and gam < 1e-13

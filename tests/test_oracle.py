@@ -335,7 +335,7 @@ def test_frame_round_trip_with_zero_field(oracle):
 # ------------------------------------------ nonlinear integrator (Flagmap=2)
 
 NL_CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
-            ("nl_cf_sol", 11)]
+            ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22)]
 
 
 def _zedge(d, step):
@@ -366,9 +366,10 @@ def test_oracle_nonlinear_stages_match_reference(oracle, name, step):
     assert col_err(pts, d[s + "p_map1a"]) < 1e-14
     pts = d[s + "p_cvbk"].copy()
     refpt_k = refpt.copy()
+    rf = None if e.field is None else oracle.RFData(e.field, fourier=True)
     oracle.kick2(c, pts, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt_k,
                  float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge, e.type,
-                 e.values, e.length, _zedge(d, step), z)
+                 e.values, e.length, _zedge(d, step), z, rf)
     assert col_err(pts, d[s + "p_kick"]) < 1e-13
     pts = d[s + "p_kick"].copy()
     z = oracle.drift2(c, pts, refpt_k, z, 2 * tau1)
@@ -377,7 +378,7 @@ def test_oracle_nonlinear_stages_match_reference(oracle, name, step):
     assert rel_err(refpt_k, d[s + "refpt"]) < 1e-15
 
 
-@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol"])
+@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf"])
 def test_oracle_nonlinear_trajectory_matches_reference(oracle, name):
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
@@ -655,3 +656,26 @@ def test_oracle_matches_executable_wake_run(oracle):
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
     assert k == d["fort24"].shape[0] - 1
+
+
+def test_oracle_matches_reference_example1(oracle):
+    """The reference's examples/Example1 as its executable ran it
+    (tests/golden/example1.npz): H- through DTL cells on their Fourier fields
+    with the Lorentz integrator; the first 130 of its 1002 steps (the first
+    cell and the start of the second) to keep the CPU suite short."""
+    d = load_golden("example1")
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid, flagmap=2)
+    for k, e in enumerate(run.segments(), 1):
+        if k > 130:
+            break
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-10 * scale), (k, f, got, ref)
+        assert run.st.refpt[5] == pytest.approx(-d["fort18"][k, 2], rel=1e-13)
