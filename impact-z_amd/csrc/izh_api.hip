@@ -1036,6 +1036,54 @@ void rf_fourier_axis(const izh_rfdata &d, double escale, double length, double z
   ax[2] = ezpp1 * escale;
   ax[3] = ezppp * escale;
 }
+// One Fourier section of getfld_SolRF (src/Appl/SolRF.f90:513-547,
+// 577-611): count, start, end and period at F[o+1..o+4] (1-based),
+// coefficients from F[o+5]; the series and its three derivatives at zz.
+static void solrf_series(const double *F, int o, double zz, double pi, double d[4]) {
+  const int np = (int)(F[o + 1] + 0.1), nt = o + 4;
+  const double zlen = F[o + 4];
+  d[0] = F[nt + 1] / 2;
+  d[1] = d[2] = d[3] = 0.0;
+  for (int i = 2; i <= (np - 1) / 2 + 1; i++) {
+    const double arg = (i - 1) * 2 * pi * zz / zlen, k = (i - 1) * 2 * pi / zlen;
+    const double cs = std::cos(arg), sn = std::sin(arg);
+    d[0] = d[0] + F[2 * i - 2 + nt] * cs + F[2 * i - 1 + nt] * sn;
+    d[1] = d[1] + k * (-F[2 * i - 2 + nt] * sn + F[2 * i - 1 + nt] * cs);
+    d[2] = d[2] + (k * k) * (-F[2 * i - 2 + nt] * cs - F[2 * i - 1 + nt] * sn);
+    d[3] = d[3] + (k * k * k) * (F[2 * i - 2 + nt] * sn - F[2 * i - 1 + nt] * cs);
+  }
+}
+
+// getfld_SolRF's sections at the kick's z (every particle's pos(3)): which
+// cover it, the RF section's field times the scale, the solenoid section's
+// unscaled; ww (the real angular frequency) and ww/clite with the
+// reference's REAL(4) pi and clite.  A message if the file is too short.
+const char *solrf_axis(const izh_rfdata &d, double escale, double length, double zedge,
+                       double z, ElemField &e) {
+  const double pi = (double)(2 * std::asin(1.0f)), clite = (double)299792458.e0f;
+  const double *F = d.coef.data() - 1;
+  const int n = (int)d.coef.size();
+  const int np1 = n >= 1 ? (int)(F[1] + 0.1) : 0;
+  if (n < 4 || np1 < 1 || 8 + np1 > n || 8 + np1 + (int)(F[5 + np1] + 0.1) > n)
+    return "coefficients hold fewer values than their two section headers announce";
+  e.win = 0;
+  e.ww = e.v[1] * 2 * pi;
+  e.xl = e.ww / clite;
+  if (!(z > zedge && z < zedge + length)) return nullptr;
+  const double zlc = z - zedge;
+  double dd[4];
+  if (zlc >= F[2] && zlc <= F[3]) {
+    solrf_series(F, 0, z - zedge - F[2] - F[4] / 2, pi, dd);
+    for (int a = 0; a < 4; a++) e.ax[a] = dd[a] * escale;
+    e.win |= 1;
+  }
+  const int o = 4 + np1;
+  if (zlc >= F[o + 2] && zlc <= F[o + 3]) {
+    solrf_series(F, o, z - zedge - F[o + 2] - F[o + 4] / 2, pi, e.bz);
+    e.win |= 2;
+  }
+  return nullptr;
+}
 }  // namespace izh
 
 extern "C" {
@@ -1077,14 +1125,21 @@ int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_fiel
   for (int a = 0; a < 9; a++) k.e.v[a] = e->v[a];
   k.e.length = e->length;
   k.e.zedge = e->zedge;
-  if (e->bitype >= 101 && e->bitype <= 104) {
+  if (e->bitype >= 101 && e->bitype <= 105) {
     if (!e->rf || !e->rf->fourier)
       return fail(IZH_ERR_ARG, "izh_kick2: bitype %d needs its Fourier coefficients "
                   "(izh_rfdata_fourier, as read3_Data reads them)", e->bitype);
-    rf_fourier_axis(*e->rf, e->v[0], e->length, e->zedge, z, k.e.ax);
-    k.e.ww = e->v[1] / b->c.scfreq;
-    k.e.xl = b->c.scxl / k.e.ww;
     k.e.theta0 = e->v[2] * (double)std::asin(1.0f) / 90;
+    if (e->bitype == 105) {
+      if (const char *why = solrf_axis(*e->rf, e->v[0], e->length, e->zedge, z, k.e))
+        return fail(IZH_ERR_ARG, "izh_kick2: SolRF: %s", why);
+      k.e.bscale = e->v[10];
+      k.e.tdiv = 2 * (double)(2 * std::asin(1.0f)) * b->c.scfreq;
+    } else {
+      rf_fourier_axis(*e->rf, e->v[0], e->length, e->zedge, z, k.e.ax);
+      k.e.ww = e->v[1] / b->c.scfreq;
+      k.e.xl = b->c.scxl / k.e.ww;
+    }
   }
   k.gam = -b->refpt[5];
   k.clight = b->c.clight;

@@ -272,6 +272,13 @@ struct ElemField {
   // one, Scxl/ww and the phase theta0 (src/Appl/DTL.f90:780-838)
   double ax[4];
   double ww, xl, theta0;
+  // SolRF (src/Appl/SolRF.f90:482-620): which sections cover the kick's z
+  // (bit 0 the RF one, bit 1 the solenoid one; 0 outside the element), the
+  // solenoid section's Bz and derivatives (unscaled), its scale Param(12)
+  // and 2*pi*Scfreq of the time conversion; xl holds ww/clite there
+  int win;
+  double bz[4];
+  double bscale, tdiv;
 };
 
 __host__ __device__ inline void elem_field(const ElemField &e, double scxl, double clight,
@@ -302,6 +309,28 @@ __host__ __device__ inline void elem_field(const ElemField &e, double scxl, doub
       f[4] = -bb / 2 * pos[1];
       f[5] = b0 + bb * (zz - zshift);
     }
+  } else if (e.type == 105) {
+    const double clite = (double)299792458.e0f;
+    const double r2 = pos[0] * pos[0] + pos[1] * pos[1];
+    if (e.win & 1) {
+      const double ez1 = e.ax[0], ezp1 = e.ax[1], ezpp1 = e.ax[2], ezppp = e.ax[3];
+      const double xlrep = e.xl;
+      const double tt = pos[3] / e.tdiv;
+      const double tmpcos = cos(e.ww * tt + e.theta0), tmpsin = sin(e.ww * tt + e.theta0);
+      const double f1 = -(ezpp1 + ez1 * xlrep * xlrep) / 4;
+      const double f1p = -(ezppp + ezp1 * xlrep * xlrep) / 4;
+      f[0] += -pos[0] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+      f[1] += -pos[1] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+      f[2] += (ez1 + f1 * r2) * tmpcos;
+      f[3] += pos[1] * xlrep / clite * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+      f[4] += -pos[0] * xlrep / clite * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+    }
+    if (e.win & 2) {
+      const double bs = e.bscale;
+      f[3] = f[3] - bs * e.bz[1] / 2 * pos[0] + bs * e.bz[3] * pos[0] * r2 / 16;
+      f[4] = f[4] - bs * e.bz[1] / 2 * pos[1] + bs * e.bz[3] * pos[1] * r2 / 16;
+      f[5] = f[5] + bs * e.bz[0] - bs * e.bz[2] * r2 / 4;
+    }
   } else if (e.type >= 101 && e.type <= 104) {
     // getfld_DTL / _CCDTL / _CCL / _SC (src/Appl/DTL.f90:769-855): the DTL's
     // drift-tube quadrupoles, then the paraxial expansion of the on-axis
@@ -330,11 +359,11 @@ __host__ __device__ inline void elem_field(const ElemField &e, double scxl, doub
 // The element types with an analytic field on this path, or a message.
 inline const char *elem_field_unsupported(int bitype, const double v[4]) {
   if (bitype == 0 || bitype == 2 || bitype == 3) return nullptr;
-  if (bitype >= 101 && bitype <= 104) return nullptr;   // with Fourier coefficients
+  if (bitype >= 101 && bitype <= 105) return nullptr;   // with Fourier coefficients
   if (bitype == 1)
     return v[1] > 0.0 ? "quadrupole field profile from a file (file ID > 0)" : nullptr;
   return "element type not on this path (0 drift, 1 quadrupole, 2 constant focusing, "
-         "3 solenoid, 101-104 RF cavities)";
+         "3 solenoid, 101-105 RF cavities)";
 }
 
 // Implicit Lorentz push of scatter2/scatter20 (src/Appl/BeamBunch.f90:

@@ -13,7 +13,7 @@ K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
 (type 2) and solenoids (type 3), under the linear map integrator (Flagmap=1)
 or the nonlinear Lorentz integrator (Flagmap=2), and the RF cavities DTL,
 CCDTL, CCL, SC and SolRF (types 101-105) with their rfdataN.in fields under
-Flagmap=1 (101-104 also under Flagmap=2); a type -25 element switches
+Flagmap=1 and Flagmap=2; a type -25 element switches
 the integrator to its mapstp for the elements after it
 (src/Contrl/AccSimulator.f90:807-809).  Zero-length diagnostic elements that
 do not change the particles (types -3 to -6, -8) are skipped; -2 and -7
@@ -231,9 +231,6 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, v[0], v[1], v[2], v[3])))
         elif 101 <= e.type <= 105:
             vv = (v + [0.0] * 14)[:14]
-            if flagmap != 1 and e.type == 105:
-                raise DeckError(f"element {k}: SolRF under Flagmap=2 (getfld_SolRF) is not "
-                                "on this path")
             if flagmap == 1 and vv[3] < 0.0:
                 raise DeckError(f"element {k}: type {e.type} with Param(5) = {vv[3]:g} < 0 "
                                 "(ideal RF cavity model) is not on this path")

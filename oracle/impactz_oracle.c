@@ -1079,11 +1079,81 @@ static int rf_field(const or_const *c, int bitype, const double *v, double lengt
   return 0;
 }
 
+/* One Fourier section of getfld_SolRF (src/Appl/SolRF.f90:513-547,577-611):
+ * the series whose count, start, end and period sit at Fcoef(o+1..o+4),
+ * coefficients from Fcoef(o+5), at zz from the section's middle; returns
+ * the field and its first three derivatives (unscaled). */
+static void solrf_series(const double *F, int o, double zz, double pi, double d[4]) {
+  int np = (int)(F[o + 1] + 0.1), i, nt = o + 4;
+  double zlen = F[o + 4];
+  d[0] = F[nt + 1] / 2;
+  d[1] = d[2] = d[3] = 0.0;
+  for (i = 2; i <= (np - 1) / 2 + 1; i++) {
+    double arg = (i - 1) * 2 * pi * zz / zlen, k = (i - 1) * 2 * pi / zlen;
+    double cs = cos(arg), sn = sin(arg);
+    d[0] = d[0] + F[2 * i - 2 + nt] * cs + F[2 * i - 1 + nt] * sn;
+    d[1] = d[1] + k * (-F[2 * i - 2 + nt] * sn + F[2 * i - 1 + nt] * cs);
+    d[2] = d[2] + (k * k) * (-F[2 * i - 2 + nt] * cs - F[2 * i - 1 + nt] * sn);
+    d[3] = d[3] + (k * k * k) * (F[2 * i - 2 + nt] * sn - F[2 * i - 1 + nt] * cs);
+  }
+}
+
+/* getfld_SolRF (src/Appl/SolRF.f90:482-620): inside the element, the RF
+ * field of the first Fourier section (its paraxial expansion at the real
+ * angular frequency) and the solenoid field of the second, scaled by
+ * Param(12), each within its own z range. */
+static int solrf_field(const or_const *c, const double *v, double length, double zedge,
+                       const or_rfdata *rf, const double pos[4], double f[6]) {
+  const double clite = (double)299792458.e0f, pi = (double)(2 * asinf(1.0f));
+  const double *F;
+  int np1;
+  double zlc, d[4];
+  if (!rf || !rf->fourier) return -1;
+  F = rf->fcoef - 1;
+  if (!(pos[2] > zedge && pos[2] < zedge + length)) return 0;
+  zlc = pos[2] - zedge;
+  np1 = (int)(F[1] + 0.1);
+  if (zlc >= F[2] && zlc <= F[3]) {
+    double ww = v[1] * 2 * pi, xlrep = ww / clite;
+    double theta0 = v[2] * (double)asinf(1.0f) / 90;
+    double ez1, ezp1, ezpp1, ezppp, tt, tmpcos, tmpsin, f1, f1p, r2;
+    solrf_series(F, 0, pos[2] - zedge - F[2] - F[4] / 2, pi, d);
+    ez1 = d[0] * v[0];
+    ezp1 = d[1] * v[0];
+    ezpp1 = d[2] * v[0];
+    ezppp = d[3] * v[0];
+    tt = pos[3] / (2 * pi * c->scfreq);
+    tmpcos = cos(ww * tt + theta0);
+    tmpsin = sin(ww * tt + theta0);
+    f1 = -(ezpp1 + ez1 * xlrep * xlrep) / 4;
+    f1p = -(ezppp + ezp1 * xlrep * xlrep) / 4;
+    r2 = pos[0] * pos[0] + pos[1] * pos[1];
+    f[0] += -pos[0] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+    f[1] += -pos[1] * (ezp1 / 2 + f1p * r2 / 4) * tmpcos;
+    f[2] += (ez1 + f1 * r2) * tmpcos;
+    f[3] += pos[1] * xlrep / clite * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+    f[4] += -pos[0] * xlrep / clite * (ez1 / 2 + f1 * r2 / 4) * tmpsin;
+  }
+  {
+    int o = 4 + np1;
+    double bscale = v[10], r2;
+    if (zlc >= F[o + 2] && zlc <= F[o + 3]) {
+      solrf_series(F, o, pos[2] - zedge - F[o + 2] - F[o + 4] / 2, pi, d);
+      r2 = pos[0] * pos[0] + pos[1] * pos[1];
+      f[3] = f[3] - bscale * d[1] / 2 * pos[0] + bscale * d[3] * pos[0] * r2 / 16;
+      f[4] = f[4] - bscale * d[1] / 2 * pos[1] + bscale * d[3] * pos[1] * r2 / 16;
+      f[5] = f[5] + bscale * d[0] - bscale * d[2] * r2 / 4;
+    }
+  }
+  return 0;
+}
+
 int or_elem_field(const or_const *c, int bitype, const double *v, double length,
                   double zedge, const or_rfdata *rf, const double pos[4], double f[6]) {
   int k;
   for (k = 0; k < 6; k++) f[k] = 0.0;
   if (bitype >= 101 && bitype <= 104) return rf_field(c, bitype, v, length, zedge, rf, pos, f);
+  if (bitype == 105) return solrf_field(c, v, length, zedge, rf, pos, f);
   if (bitype == 0) return 0;  /* getfld_DriftTube: no field */
   if (bitype == 1) {
     /* getfld_Quadrupole (src/Appl/Quadrupole.f90:369-402), file ID <= 0 */

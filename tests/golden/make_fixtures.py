@@ -279,6 +279,18 @@ def nl_rf_lattice():
             B.Element(L, 8, 20, 103, 2.0e6, 324.0e6, 150.0, 7.0, (0.0065,), field=coef)]
 
 
+def nl_solrf_lattice():
+    """SolRF under the Lorentz integrator: its coefficient file holds the RF
+    section (count 7, over the whole element) and the solenoid section
+    (count 5, over its first 0.15 m), as getfld_SolRF reads them."""
+    coef = np.array([7.0, 0.0, 0.2, 0.2, 0.4, -1.0, 0.05, 0.3, -0.02, -0.05, 0.01,
+                     5.0, 0.0, 0.15, 0.15, 1.2, 0.6, 0.0, -0.1, 0.02])
+    return [B.Element(0.05, 4, 10, 0, 1.0),
+            B.Element(0.2, 12, 20, 105, 3.0e6, 324.0e6, 45.0, 4.0,
+                      (1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.3), field=coef),
+            B.Element(0.05, 4, 10, 0, 1.0)]
+
+
 def example1_fixture():
     """The reference's own examples/Example1 (H- through nine DTL cells,
     two quadrupoles and a drift, Lorentz integrator, 200 particles) run by
@@ -403,6 +415,9 @@ def main():
     fixture("nl_rf", 1000, (16, 16, 16), nl_rf_lattice(), 28, (6, 22), flagmap=2)
     meta["nl_rf"] = ("Flagmap=2 through a DTL (with quadrupoles) and a CCL on Fourier "
                      "coefficients (rfdata7.in); dumps at steps 6 (DTL) and 22 (CCL)")
+    fixture("nl_solrf", 1000, (16, 16, 16), nl_solrf_lattice(), 20, (7,), flagmap=2)
+    meta["nl_solrf"] = ("Flagmap=2 through SolRF: RF and solenoid sections of its coefficient "
+                        "file (rfdata4.in); dump at step 7")
     example1_fixture()
     meta["example1"] = ("examples/Example1 of the reference run by ImpactZexe: inputs kept as "
                         "data (particles, element values, Fourier coefficients), fort.18/24-26")

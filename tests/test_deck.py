@@ -207,10 +207,6 @@ def test_deck_rf_cavities_and_field_files(tmp_path):
         _write(p, HEADER + line + "\n")
         with pytest.raises(DeckError, match=what):
             program_of(read_deck(str(p)))
-    _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") +
-           "0.2 4 20 105 1e6 324e6 0 1 0.02 /\n")
-    with pytest.raises(DeckError, match="SolRF under Flagmap=2"):
-        program_of(read_deck(str(p)))
     # under Flagmap=2 the DTL family reads Fourier coefficients (read3_Data)
     _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") +
            "0.2 4 20 103 1e6 324e6 0 1 0.02 /\n")

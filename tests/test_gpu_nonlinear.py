@@ -14,7 +14,7 @@ from test_gpu_parity import TOL, record
 pytestmark = pytest.mark.gpu
 
 CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
-         ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22)]
+         ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22), ("nl_solrf", 7)]
 
 
 @pytest.mark.parametrize("name,step", CASES)
@@ -59,7 +59,7 @@ def test_gpu_nonlinear_stages_match_reference(gpu, name, step):
         assert v < TOL, (k, v)
 
 
-@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf"])
+@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf", "nl_solrf"])
 def test_gpu_nonlinear_trajectory_matches_reference(gpu, name):
     from impactz.sim import Simulation
     d = load_golden(name)

@@ -335,7 +335,7 @@ def test_frame_round_trip_with_zero_field(oracle):
 # ------------------------------------------ nonlinear integrator (Flagmap=2)
 
 NL_CASES = [("nl_fodo", 1), ("nl_fodo", 5), ("nl_cf_sol", 2), ("nl_cf_sol", 10),
-            ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22)]
+            ("nl_cf_sol", 11), ("nl_rf", 6), ("nl_rf", 22), ("nl_solrf", 7)]
 
 
 def _zedge(d, step):
@@ -378,7 +378,7 @@ def test_oracle_nonlinear_stages_match_reference(oracle, name, step):
     assert rel_err(refpt_k, d[s + "refpt"]) < 1e-15
 
 
-@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf"])
+@pytest.mark.parametrize("name", ["nl_fodo", "nl_cf_sol", "nl_rf", "nl_solrf"])
 def test_oracle_nonlinear_trajectory_matches_reference(oracle, name):
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
