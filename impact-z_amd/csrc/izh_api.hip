@@ -1101,8 +1101,8 @@ int izh_drift2(izh_bunch *b, double *z, double tau) {
   return IZH_OK;
 }
 
-int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
-              const izh_compdom *dom, int flagbc) {
+static int kick2_impl(izh_bunch *b, double z, double tau, const izh_element *e,
+                      izh_field *f, const izh_compdom *dom, int flagbc, bool wake) {
   if (!b || !e) return fail(IZH_ERR_ARG, "izh_kick2: bad argument");
   if (flagbc != 1) return fail(IZH_ERR_ARG, "izh_kick2: only Flagbc=1 is supported");
   if (const char *why = elem_field_unsupported(e->bitype, e->v))
@@ -1156,6 +1156,12 @@ int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_fiel
   k.ez0 = f0[2];
   k.tmpgamma0 = k.gam + 0.5 * tau * k.qmcc * k.ez0;
   if (sc) {
+    if (wake) {
+      if (!f->have_wake)
+        return fail(IZH_ERR_STATE, "izh_kick2wake: call izh_wakefield first");
+      k.wake = f->d_wake;
+      k.nzw = f->nz;
+    }
     IZH_CHECK(field_set_domain(f, dom));
     IZH_CHECK(launch_efield(f));
     IZH_CHECK(join_streams(f->stream, b->stream));
@@ -1172,12 +1178,26 @@ int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_fiel
   return IZH_OK;
 }
 
+int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
+              const izh_compdom *dom, int flagbc) {
+  return kick2_impl(b, z, tau, e, f, dom, flagbc, false);
+}
+
+int izh_kick2wake(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
+                  const izh_compdom *dom, int flagbc) {
+  return kick2_impl(b, z, tau, e, f, dom, flagbc, true);
+}
+
 int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double tau1,
               const izh_element *e, double xrad, double yrad, double bcurr,
-              double *moments) {
+              const izh_wake *wake, double *moments) {
   if (!b || !f || !dom || !z || !e) return fail(IZH_ERR_ARG, "izh_step2: bad argument");
   if (const char *why = elem_field_unsupported(e->bitype, e->v))
     return fail(IZH_ERR_ARG, "izh_step2: bitype %d: %s", e->bitype, why);
+  if ((e->bitype == 105 || e->bitype == 106) && e->v[13] > 0.0 && !wake)
+    return fail(IZH_ERR_ARG, "izh_step2: bitype %d with a wake length (Param(15) > 0) "
+                "needs its izh_wake", e->bitype);
+  const bool wk = wake && wake->kind != 0;
   const double tau2 = 2.0 * tau1;
   IZH_CHECK(izh_drift2(b, z, tau2));                                        // :1112
   if (bcurr >= (double)1.0e-10f && b->n > 0) {
@@ -1187,9 +1207,13 @@ int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double ta
     IZH_CHECK(izh_update_compdom(dom, ptrange, 1));
     IZH_CHECK(izh_charge(b, dom, f));
     IZH_CHECK(izh_update3O(f, dom));
+    if (wk) IZH_CHECK(izh_wakefield(b, f, dom, wake));                      // :1272
     IZH_CHECK(izh_cvbkforth1st(b));
   }
-  IZH_CHECK(izh_kick2(b, *z, tau2, e, f, dom, 1));                          // :1452
+  if (wk && bcurr >= (double)1.0e-10f && b->n > 0)
+    IZH_CHECK(izh_kick2wake(b, *z, tau2, e, f, dom, 1));                    // :1448
+  else
+    IZH_CHECK(izh_kick2(b, *z, tau2, e, f, dom, 1));                        // :1452
   IZH_CHECK(izh_drift2(b, z, tau2));                                        // :1456
   if (moments) IZH_CHECK(izh_diagnostic1(b, moments, moments + 37, moments + 43));
   return IZH_OK;

@@ -405,6 +405,10 @@ __device__ inline void lorentz_push(double &px, double &py, double &pt, double q
 struct Kick2 {
   ElemField e;
   double gam, beta0_c, xk, xl, clight, tg, z, tau, tmpscale, tmpgamma0, qmcc, ez0;
+  // kick2wake (src/Appl/BeamBunch.f90:5048-5410): exwake, eywake, ezwake of
+  // the field (nz each, V/m) interpolated in z and added to E, or null
+  const double *wake;
+  int nzw;
 };
 
 int launch_drift2(izh_bunch *b, double tau, double gamma0, double beta0);

@@ -368,8 +368,21 @@ k_kick2(double *__restrict__ X, double *__restrict__ PX, double *__restrict__ Y,
     const double pos[4] = {x * k.xl, y * k.xl, k.z, ph + k.tg};
     double f[6];
     elem_field(k.e, k.xl, k.clight, pos, f);
-    const double ex = k.tmpscale * exn + f[0], ey = k.tmpscale * eyn + f[1];
-    const double ez = k.tmpscale * ezn + f[2];
+    double ex = k.tmpscale * exn + f[0], ey = k.tmpscale * eyn + f[1];
+    double ez = k.tmpscale * ezn + f[2];
+    if (SC && k.wake) {
+      // scatter2wake: the wake of the particle's z, linear in its CIC weight
+      // (src/Appl/BeamBunch.f90:5634-5636,5886-5888)
+      const double zmin = dom->range[4], hz = dom->msize[2], hzi = dom->hinv[2];
+      const double phz = PH[i];
+      int kx = (int)((phz - zmin) * hzi + 1.0);
+      const double ef = ((zmin - phz) + kx * hz) * hzi;
+      kx = min(max(kx, 1), nz - 1);
+      const double *w = k.wake;
+      ex = ex + (w[kx - 1] * ef + w[kx] * (1.0 - ef));
+      ey = ey + (w[k.nzw + kx - 1] * ef + w[k.nzw + kx] * (1.0 - ef));
+      ez = ez + (w[2 * k.nzw + kx - 1] * ef + w[2 * k.nzw + kx] * (1.0 - ef));
+    }
     const double bx = -k.beta0_c * k.tmpscale * eyn + f[3];
     const double by = k.beta0_c * k.tmpscale * exn + f[4];
     lorentz_push(px, py, pt, qm0[orig[i]], ex, ey, ez, bx, by, f[5], k.tau, k.clight,

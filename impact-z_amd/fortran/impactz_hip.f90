@@ -200,6 +200,16 @@ module impactz_hip
       integer(c_int), value :: flagbc
       integer(c_int) :: rc
     end function
+    function izh_kick2wake(b, z, tau, e, f, dom, flagbc) bind(C, name="izh_kick2wake") &
+        result(rc)
+      import :: c_int, c_double, c_ptr, izh_element, izh_compdom
+      type(c_ptr), value :: b, f
+      real(c_double), value :: z, tau
+      type(izh_element), intent(in) :: e
+      type(izh_compdom), intent(in) :: dom
+      integer(c_int), value :: flagbc
+      integer(c_int) :: rc
+    end function
     function izh_wake_download(f, ex, ey, ez) bind(C, name="izh_wake_download") result(rc)
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: f
@@ -222,7 +232,7 @@ module impactz_hip
       integer(c_int), value :: flagbc
       integer(c_int) :: rc
     end function
-    function izh_step2(b, f, dom, z, tau1, e, xrad, yrad, bcurr, moments) &
+    function izh_step2(b, f, dom, z, tau1, e, xrad, yrad, bcurr, wake, moments) &
         bind(C, name="izh_step2") result(rc)
       import :: c_int, c_double, c_ptr, izh_element, izh_compdom
       type(c_ptr), value :: b, f
@@ -231,6 +241,7 @@ module impactz_hip
       real(c_double), value :: tau1
       type(izh_element), intent(in) :: e
       real(c_double), value :: xrad, yrad, bcurr
+      type(c_ptr), value :: wake
       real(c_double) :: moments(44)
       integer(c_int) :: rc
     end function

@@ -237,8 +237,6 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
                                    more=tuple(vv[4:]))))
         elif e.type == -41:
-            if flagmap != 1:
-                raise DeckError(f"element {k}: wakefields under Flagmap=2 are not on this path")
             vv = (v + [0.0] * 14)[:14]
             out.append((k, WakeSwitch(k, Element(e.length, e.nseg, e.mapstp, -41, *vv[:4],
                                                  more=tuple(vv[4:])))))

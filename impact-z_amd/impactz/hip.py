@@ -35,7 +35,7 @@ SYMBOLS = (
     "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
     "izh_map1", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
     "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
-    "izh_wake_download",
+    "izh_wake_download", "izh_kick2wake",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -176,7 +176,10 @@ def lib() -> C.CDLL:
     L.izh_kick2.argtypes = [vp, C.c_double, C.c_double, C.POINTER(ElementC), vp,
                             C.POINTER(CompDomC), C.c_int]
     L.izh_step2.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.c_double,
-                            C.POINTER(ElementC), C.c_double, C.c_double, C.c_double, _dp]
+                            C.POINTER(ElementC), C.c_double, C.c_double, C.c_double,
+                            C.POINTER(WakeC), _dp]
+    L.izh_kick2wake.argtypes = [vp, C.c_double, C.c_double, C.POINTER(ElementC), vp,
+                                C.POINTER(CompDomC), C.c_int]
     L.izh_map_linear.argtypes = [vp, _dp]
     L.izh_conv1st.argtypes = [vp, C.c_double, _dp, _i64p, _i64p, C.c_double,
                               C.c_double, C.c_int]
@@ -473,14 +476,22 @@ class BeamBunch:
 
     def step2(self, field: FieldQuant, dom: CompDom, z: float, tau1: float,
               elem: ElementC, xrad: float, yrad: float, bcurr: float,
-              want_moments: bool = False):
-        """One Flagmap=2 segment of run_AccSimulator."""
+              want_moments: bool = False, wake: WakeC | None = None):
+        """One Flagmap=2 segment of run_AccSimulator (with kick2wake when a
+        wake is given)."""
         zz = C.c_double(z)
         mom = np.zeros(44) if want_moments else None
         check(lib().izh_step2(self.h, field.h, C.byref(dom.c), C.byref(zz), tau1,
                               C.byref(elem), xrad, yrad, bcurr,
+                              C.byref(wake) if wake is not None else None,
                               _p(mom) if want_moments else None))
         return zz.value, mom
+
+    def kick2wake(self, z: float, tau: float, elem: ElementC, field: FieldQuant,
+                  dom: CompDom, flagbc: int = 1) -> None:
+        """kick2wake_BeamBunch (src/Appl/BeamBunch.f90:5048-5410)."""
+        check(lib().izh_kick2wake(self.h, z, tau, C.byref(elem), field.h, C.byref(dom.c),
+                                  flagbc))
 
     def conv1st(self, tau: float, xrad: float, yrad: float, flagbc: int = 1):
         """conv1st_BeamBunch (src/Appl/BeamBunch.f90:1252-1665); returns (ptrange, nplc, nptot)."""

@@ -154,7 +154,7 @@ class Simulation:
             self.z, mom = self.bunch.step2(
                 self.field, self.dom, self.z, 0.5 * e.length / e.nseg,
                 hip.element(e.type, e.values, e.length, zedge, rf=self.rfdata(e)), e.radius,
-                e.radius, self.beam.current, want_moments)
+                e.radius, self.beam.current, want_moments, self.wake())
         else:
             self.z, mom = self.bunch.step(self.field, self.dom, self.z,
                                           self.segment_args(e, zedge), self.beam.current,

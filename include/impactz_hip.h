@@ -285,13 +285,21 @@ int izh_drift2(izh_bunch *b, double *z, double tau);
 int izh_kick2(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
               const izh_compdom *dom, int flagbc);
 
+/* kick2wake_BeamBunch (src/Appl/BeamBunch.f90:5048-5410) with
+ * scatter2wake (:5412-5938): izh_kick2 with the wake fields of the last
+ * izh_wakefield, interpolated linearly in each particle's z and added to
+ * its E (not to B).  Without current it is izh_kick2 (scatter20). */
+int izh_kick2wake(izh_bunch *b, double z, double tau, const izh_element *e, izh_field *f,
+                  const izh_compdom *dom, int flagbc);
+
 /* One segment of the reference loop with Flagmap=2: drift2(2 tau1),
  * conv1st, update_CompDom, charge, update3O, cvbkforth1st, kick2(2 tau1),
- * drift2(2 tau1) (src/Contrl/AccSimulator.f90:1099-1488).  Arguments and
- * moments as for izh_step. */
+ * drift2(2 tau1) (src/Contrl/AccSimulator.f90:1099-1488); with a wake
+ * (NULL: none) the wake fields after update3O and kick2wake instead of
+ * kick2 (:1446-1454).  Arguments and moments as for izh_step. */
 int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double tau1,
               const izh_element *e, double xrad, double yrad, double bcurr,
-              double *moments);
+              const izh_wake *wake, double *moments);
 
 /* nseg consecutive izh_step calls (the element/segment loop of
  * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504; every element of

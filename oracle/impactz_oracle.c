@@ -1234,7 +1234,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
              int bitype, const double *v, double length, double zedge,
-             const or_rfdata *rf, double z) {
+             const or_rfdata *rf, double z, const double *wake) {
   /* kick2_BeamBunch (src/Appl/BeamBunch.f90:2187-2508) with scatter2
    * (:2510-3071) or, without current, scatter20 (:3075-3531), for the
    * elements with analytic fields */
@@ -1244,7 +1244,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
   double gam = -refpt[5];
   const double beta0 = sqrt(gam * gam - 1.0) / gam;
   const double qmcc = chrg / mass;
-  double pos[4] = {0.0, 0.0, z, tg}, fld[6];
+  double pos[4] = {0.0, 0.0, z, tg}, fld[6], wk[3] = {0.0, 0.0, 0.0};
   double ez0, tmpgamma0;
   int64_t i;
   if (or_elem_field(c, bitype, v, length, zedge, rf, pos, fld) != 0) return -1;
@@ -1281,8 +1281,16 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
       pos[2] = z;
       pos[3] = P(5, i) + tg;
       or_elem_field(c, bitype, v, length, zedge, rf, pos, fld);
-      lorentz_push(&P(1, i), tmpscale * exn + fld[0], tmpscale * eyn + fld[1],
-                   tmpscale * ezn + fld[2], -beta0 / c->clight * tmpscale * eyn + fld[3],
+      if (wake) {
+        /* kick2wake with scatter2wake (src/Appl/BeamBunch.f90:5048-5938): the
+         * wake of the particle's z, linear in its CIC weight, added to E */
+        int k1 = kx < 1 ? 1 : (kx > nz - 1 ? nz - 1 : kx);
+        wk[0] = wake[k1 - 1] * ef + wake[k1] * (1.0 - ef);
+        wk[1] = wake[nz + k1 - 1] * ef + wake[nz + k1] * (1.0 - ef);
+        wk[2] = wake[2 * nz + k1 - 1] * ef + wake[2 * nz + k1] * (1.0 - ef);
+      }
+      lorentz_push(&P(1, i), tmpscale * exn + fld[0] + wk[0], tmpscale * eyn + fld[1] + wk[1],
+                   tmpscale * ezn + fld[2] + wk[2], -beta0 / c->clight * tmpscale * eyn + fld[3],
                    beta0 / c->clight * tmpscale * exn + fld[4], fld[5], tau, c->clight,
                    tmpgamma0, gam, qmcc, ez0);
     }
@@ -1305,7 +1313,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
 
 int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                  int nx, int ny, int nz, double tau1, const or_element *e,
-                 double pipe_radius, double *rho_out,
+                 double pipe_radius, const or_wake *wake, double *rho_out,
                  double *phi_out) {
   /* one segment of the reference loop with Flagmap=2
    * (src/Contrl/AccSimulator.f90:1106-1113,1446-1458) */
@@ -1315,7 +1323,7 @@ int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
   double ptrange[6], range[6] = {0, 0, 0, 0, 0, 0}, msize[3] = {1, 1, 1};
   const double tau2 = 2.0 * tau1;
   int rc = 0;
-  double curr = 0.0;
+  double curr = 0.0, *wk = NULL;
   or_drift2(c, pts, n, st->refpt, &st->z, tau2);
   if (st->bcurr >= (double)1.0e-10f) {
     n = or_conv1st(c, pts, n, st->refpt, pipe_radius, pipe_radius, ptrange,
@@ -1323,13 +1331,20 @@ int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
     or_update_compdom(ptrange, nx, ny, nz, range, msize);
     or_deposit(pts, n, nx, ny, nz, range, msize, rho);
     rc = or_solve_openbc(rho, nx, ny, nz, msize[0], msize[1], msize[2], phi);
+    if (rc == 0 && wake && wake->kind) {
+      /* src/Contrl/AccSimulator.f90:1272-1338, then kick2wake (:1446-1450) */
+      wk = (double *)malloc(3 * (size_t)nz * sizeof(double));
+      rc = or_wakefield(c, pts, n, range, nz, st->current, st->npt, -st->refpt[5],
+                        st->charge, wake, wk, wk + nz, wk + 2 * nz);
+    }
     or_cvbkforth1st(c, pts, n, st->refpt);
     curr = st->current;
   }
   if (rc == 0)
     rc = or_kick2(c, pts, n, nx, ny, nz, phi, range, msize, st->refpt, curr, tau2,
                   st->mass, st->charge, e->bitype, e->v, e->length, e->zedge, e->rf,
-                  st->z);
+                  st->z, curr > 0.0 ? wk : NULL);
+  free(wk);
   if (rc == 0) or_drift2(c, pts, n, st->refpt, &st->z, tau2);
   if (!rho_out) free(rho);
   if (!phi_out) free(phi);

@@ -153,7 +153,7 @@ int or_kick2(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
              const double *phi, const double range[6], const double msize[3],
              double refpt[6], double curr, double tau, double mass, double chrg,
              int bitype, const double *v, double length, double zedge,
-             const or_rfdata *rf, double z);
+             const or_rfdata *rf, double z, const double *wake);
 
 /* diagnostic1_Output sums (src/Contrl/Output.f90:119-245): 37 sums in the
  * reference's tmplc order, 6 abs-maxima, max of x^2+y^2. */
@@ -218,7 +218,8 @@ int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
  * deposit -> solve -> cvbkforth1st -> kick2(2 tau1) -> drift2(2 tau1). */
 int64_t or_step2(const or_const *c, double *pts, int64_t n, or_bunch_state *st,
                  int nx, int ny, int nz, double tau1, const or_element *e,
-                 double pipe_radius, double *rho_out, double *phi_out);
+                 double pipe_radius, const or_wake *wake, double *rho_out,
+                 double *phi_out);
 
 #ifdef __cplusplus
 }

@@ -198,11 +198,11 @@ def lib() -> C.CDLL:
         L.or_kick2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.c_int, C.c_int,
                                C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                C.c_double, C.c_double, C.c_int, _dp, C.c_double,
-                               C.c_double, C.POINTER(OrRFData), C.c_double]
+                               C.c_double, C.POINTER(OrRFData), C.c_double, _dp]
         L.or_kick2.restype = C.c_int
         L.or_step2.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, C.POINTER(OrState),
                                C.c_int, C.c_int, C.c_int, C.c_double,
-                               C.POINTER(OrElement), C.c_double, _dp, _dp]
+                               C.POINTER(OrElement), C.c_double, C.POINTER(OrWake), _dp, _dp]
         L.or_step2.restype = C.c_int64
         _lib = L
     return _lib
@@ -279,15 +279,18 @@ def elem_field(c, bitype, v, length, zedge, pos, rf: RFData | None = None):
 
 
 def kick2(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, bitype, v, length, zedge, z,
-          rf: RFData | None = None):
-    """kick2_BeamBunch in place on pts (and refpt(6))."""
+          rf: RFData | None = None, wake=None):
+    """kick2_BeamBunch in place on pts (and refpt(6)); with wake =
+    (exwake, eywake, ezwake) kick2wake_BeamBunch."""
+    wk = None if wake is None else np.ascontiguousarray(wake, dtype=np.float64).ravel()
     nz, ny, nx = phi.shape
     vv = element_values(v)
     if lib().or_kick2(C.byref(c), _p(pts), pts.shape[0], nx, ny, nz, _p(phi),
                       _p(np.ascontiguousarray(rng, dtype=np.float64)),
                       _p(np.ascontiguousarray(ms, dtype=np.float64)), _p(refpt), curr, tau,
                       mass, chrg, bitype, _p(vv), length, zedge,
-                      C.byref(rf.c) if rf is not None else None, z) != 0:
+                      C.byref(rf.c) if rf is not None else None, z,
+                      _p(wk) if wk is not None else None) != 0:
         raise ValueError(f"no analytic field for element type {bitype}")
 
 
@@ -476,8 +479,10 @@ class Run:
                               ny, nz, tau1, C.byref(el), e.radius,
                               C.byref(w) if w is not None else None, *out)
         else:
+            w = self.wake()
             n = lib().or_step2(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx,
-                               ny, nz, tau1, C.byref(el), e.radius, *out)
+                               ny, nz, tau1, C.byref(el), e.radius,
+                               C.byref(w) if w is not None else None, *out)
         if n < 0:
             raise RuntimeError("or_step failed")
         self.n = n

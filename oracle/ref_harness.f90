@@ -357,6 +357,10 @@
             else if (flagmap == 1) then
               call map2_BeamBunch(Bpts, tau2, lcgrid(1), lcgrid(2), &
                    lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 1.0d0)
+            else if (flagwake == 1) then
+              call kick2wake_BeamBunch(Bpts, Blnelem(i), z, tau2, lcgrid(1), &
+                   lcgrid(2), lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 0, &
+                   exwake, eywake, ezwake, Nz, 1, 1)
             else
               call map2_BeamBunch(Bpts, Blnelem(i), z, tau2, lcgrid(1), &
                    lcgrid(2), lcgrid(3), Potential%FieldQ, Ageom, grid2d, 1, 0)

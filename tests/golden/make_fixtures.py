@@ -291,6 +291,17 @@ def nl_solrf_lattice():
             B.Element(0.05, 4, 10, 0, 1.0)]
 
 
+def nl_wake_solrf_lattice():
+    """SolRF with its analytic short-range wake under the Lorentz integrator
+    (kick2wake)."""
+    lat = nl_solrf_lattice()
+    e = lat[1]
+    lat[1] = B.Element(e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2, e.v3, e.v4,
+                       (1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.3, 10.75e-3, 30.33e-3, 33.321e-3),
+                       field=e.field)
+    return lat
+
+
 def example1_fixture():
     """The reference's own examples/Example1 (H- through nine DTL cells,
     two quadrupoles and a drift, Lorentz integrator, 200 particles) run by
@@ -418,6 +429,12 @@ def main():
     fixture("nl_solrf", 1000, (16, 16, 16), nl_solrf_lattice(), 20, (7,), flagmap=2)
     meta["nl_solrf"] = ("Flagmap=2 through SolRF: RF and solenoid sections of its coefficient "
                         "file (rfdata4.in); dump at step 7")
+    fixture("nl_wake_file", 1000, (16, 16, 32), wake_file_lattice(), 16, (6,), flagmap=2)
+    meta["nl_wake_file"] = ("Flagmap=2 with the read-in wake of -41 elements (kick2wake), "
+                            "dump at step 6")
+    fixture("nl_wake_solrf", 1000, (16, 16, 32), nl_wake_solrf_lattice(), 20, (7,),
+            flagmap=2)
+    meta["nl_wake_solrf"] = "Flagmap=2 through SolRF with its analytic wake, dump at step 7"
     example1_fixture()
     meta["example1"] = ("examples/Example1 of the reference run by ImpactZexe: inputs kept as "
                         "data (particles, element values, Fourier coefficients), fort.18/24-26")

@@ -231,6 +231,6 @@ def test_deck_wake_switches(tmp_path):
     assert isinstance(sw, WakeSwitch) and sw.element.values[:3] == [1.5, 3.0, 20.0]
     assert sw.element.field.shape == (2, 4) and sw.element.field[1, 2] == 3e18
     assert prog[2][1].values[11:14] == [10.75e-3, 30.33e-3, 33.321e-3]
+    # under Flagmap=2 the wake is applied by kick2wake
     _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.0 0 1 -41 1.5 3.0 20.0 /\n")
-    with pytest.raises(DeckError, match="wakefields under Flagmap=2"):
-        program_of(read_deck(str(p)))
+    assert isinstance(program_of(read_deck(str(p)))[0][0][1], WakeSwitch)

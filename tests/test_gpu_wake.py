@@ -128,3 +128,71 @@ def test_gpu_solrf_wake_needs_its_wake(gpu):
         b.step(f, dom, 0.0, seg, bm.current)
     with pytest.raises(hip.IZHError, match="needs its izh_wake"):
         b.run(f, dom, 0.0, [seg], bm.current)
+
+
+@pytest.mark.parametrize("name,step,kind", [("nl_wake_file", 6, 2), ("nl_wake_solrf", 7, 1)])
+def test_gpu_kick2wake_stages_match_reference(gpu, name, step, kind):
+    """Flagmap=2 with a wake: izh_wakefield after drift2 and izh_kick2wake."""
+    import ctypes
+    hip = gpu
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    s = f"s{step}_"
+    tau1 = 0.5 * e.length / e.nseg
+    lat = fixture_lattice(d)
+    if kind == 2:
+        table = hip.RFData([x for x in lat if x.type == -41][0].field, fourier=False)
+        w = hip.wake(2, table=table, scale=1.5)
+    else:
+        v = e.values
+        w = hip.wake(1, v[11], v[12], v[13])
+    b = hip.BeamBunch.from_beam(d[s + "p0"], beam, hip.FP64)
+    info = b.info()
+    for k in range(6):
+        info.refpt[k] = float(d[s + "refpt0"][k])
+    info.current = float(d[s + "current0"])
+    hip.check(hip.lib().izh_bunch_set_info(b.h, ctypes.byref(info)))
+    field = hip.FieldQuant(*grid)
+    dom = hip.CompDom(*grid)
+    rf = None if e.field is None else hip.RFData(e.field, fourier=True)
+    elem = hip.element(e.type, e.values, e.length, element_entrance(d, step), rf=rf)
+    print(f"kick2wake stages {name}:{step}", flush=True)
+    err = {}
+    z = b.drift2(float(d[s + "z0"]), 2 * tau1)
+    b.conv1st(2 * tau1, e.radius, e.radius)
+    dom.update(d[s + "ptrange"])
+    b.charge(dom, field)
+    field.update3O(dom)
+    b.wakefield(field, dom, w)
+    got, ref = field.wake_download(), d[s + "wake"]
+    for k, nm in enumerate(("exwake", "eywake", "ezwake")):
+        err[nm] = float(np.abs(got[k] - ref[k]).max() / max(np.abs(ref[k]).max(), 1e-300))
+    b.cvbkforth1st()
+    b.kick2wake(z, 2 * tau1, elem, field, dom)
+    err["kick2wake"] = col_err(b.getpts(), d[s + "p_kick"])
+    b.drift2(z, 2 * tau1)
+    err["drift2b"] = col_err(b.getpts(), d[s + "p_map1b"])
+    record(test="kick2wake_stages", case=f"{name}:{step}", **err)
+    for k, v in err.items():
+        assert v < TOL, (k, v)
+
+
+@pytest.mark.parametrize("name", ["nl_wake_file", "nl_wake_solrf"])
+def test_gpu_kick2wake_runs_match_reference(gpu, name):
+    from impactz.sim import Simulation
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    sim = Simulation(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid, flagmap=2)
+    nsteps = int(d["nsteps"])
+    print(f"kick2wake run {name}", flush=True)
+    rows = [sim.diag(sim.step(e)) for _, e in zip(range(nsteps), sim.segments())]
+    worst = 0.0
+    for k in range(nsteps):
+        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
+                              d["fort26"][k + 1, 1:]])
+        worst = max(worst, diag_row_err(rows[k], ref, ref_is_fort=True))
+    pe = col_err(sim.bunch.getpts(), d["final"])
+    record(test="kick2wake_run", case=name, moments=worst, particles=pe)
+    assert worst < TOL and pe < TOL

@@ -679,3 +679,53 @@ def test_oracle_matches_reference_example1(oracle):
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-10 * scale), (k, f, got, ref)
         assert run.st.refpt[5] == pytest.approx(-d["fort18"][k, 2], rel=1e-13)
+
+
+@pytest.mark.parametrize("name,step,kind", [("nl_wake_file", 6, 2), ("nl_wake_solrf", 7, 1)])
+def test_oracle_kick2wake_matches_reference(oracle, name, step, kind):
+    """Flagmap=2 with a wake: the wake fields after drift2 and kick2wake from
+    the reference's own inputs (tests/golden/nl_wake_*.npz)."""
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    beam = fixture_beam(d)
+    e = segment(d, step)
+    s = f"s{step}_"
+    c = oracle.physconst(beam.freq)
+    tau1 = 0.5 * e.length / e.nseg
+    pts = d[s + "p0"].copy()
+    refpt = d[s + "refpt0"].copy()
+    z = oracle.drift2(c, pts, refpt, float(d[s + "z0"]), 2 * tau1)
+    w = _wake_of(oracle, d, step, kind, 1.5 if kind == 2 else 1.0)
+    got = oracle.wakefield(c, d[s + "p_conv1st"], d[s + "range"], grid[2],
+                           float(d[s + "current"]), d[s + "p_conv1st"].shape[0],
+                           -refpt[5], beam.charge, w)
+    ref = d[s + "wake"]
+    for k in range(3):
+        assert np.abs(got[k] - ref[k]).max() <= 1e-13 * max(np.abs(ref[k]).max(), 1e-300)
+    pk = d[s + "p_cvbk"].copy()
+    rf = None if e.field is None else oracle.RFData(e.field, fourier=True)
+    oracle.kick2(c, pk, d[s + "phi"], d[s + "range"], d[s + "msize"], refpt,
+                 float(d[s + "current"]), 2 * tau1, beam.mass, beam.charge, e.type,
+                 e.values, e.length, _zedge(d, step), z, rf, ref)
+    assert col_err(pk, d[s + "p_kick"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["nl_wake_file", "nl_wake_solrf"])
+def test_oracle_kick2wake_trajectory_matches_reference(oracle, name):
+    d = load_golden(name)
+    grid = tuple(int(v) for v in d["grid"])
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid, flagmap=2)
+    nsteps = int(d["nsteps"])
+    for k, e in enumerate(run.segments(), 1):
+        if k > nsteps:
+            break
+        run.step(e)
+        row = run.diag()
+        for f, lo in ((24, 5), (25, 11), (26, 17)):
+            ref = d[f"fort{f}"][k, 1:]
+            got = row[lo:lo + 6]
+            scale = np.abs(ref).copy()
+            scale[0] = max(scale[0], abs(ref[1]))
+            scale[2] = max(scale[2], abs(ref[3]))
+            assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+    assert col_err(run.particles(), d["final"]) < 1e-12
