@@ -814,6 +814,128 @@ int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
   return izh_map1(b, z, tau, &e);
 }
 
+// ------------------------------------------------------------------ bends
+
+static int dipole_check(izh_bunch *b, const izh_element *e, const char *who) {
+  if (!b || !e) return fail(IZH_ERR_ARG, "%s: bad argument", who);
+  if (e->bitype != 4) return fail(IZH_ERR_ARG, "%s: bitype %d is not a bend", who, e->bitype);
+  if (!(e->length > 0.0) || e->v[0] == 0.0)
+    return fail(IZH_ERR_ARG, "%s: a bend needs a length > 0 and an angle != 0", who);
+  if (b->frame != kFrameZ) return fail(IZH_ERR_STATE, "%s: bunch is not in the z frame", who);
+  return IZH_OK;
+}
+
+// The pole-face constants of run_AccSimulator (src/Contrl/AccSimulator.f90:
+// 1058-1088) for the entrance (exit = false) or the exit, and the energy as
+// Fpol_Dipole/Bpol_Dipole form it (src/Appl/Dipole.f90:478-485).
+static DipoleMap dipole_face(const izh_bunch *b, const izh_element *e, bool exit) {
+  DipoleMap m{};
+  m.mode = exit ? kDipoleExit : kDipoleEntrance;
+  m.scxl = b->c.scxl;
+  m.gam0 = -b->refpt[5];
+  m.beta0 = std::sqrt(1.0 - 1.0 / (m.gam0 * m.gam0));
+  m.gambet = m.beta0 * m.gam0;
+  m.qm0 = b->charge / b->mass;
+  const double hgap = 2.0 * e->v[3];
+  const double ang = exit ? e->v[5] : e->v[4];
+  m.h0 = e->v[0] / e->length;
+  m.h1 = exit ? e->v[7] : e->v[6];
+  m.k1 = e->v[1];
+  m.psi = m.h0 * hgap * e->v[8] * (1.0 + std::sin(ang) * std::sin(ang)) / std::cos(ang);
+  m.tanphi = std::tan(ang);
+  m.tanphib = std::tan(ang - m.psi);
+  m.tanphi2 = m.tanphi * m.tanphi;
+  m.secphi2 = 1.0 + m.tanphi2;
+  m.secphi3 = m.secphi2 * (1.0 / std::cos(ang));
+  m.secphib2 = 1.0 + m.tanphib * m.tanphib;
+  return m;
+}
+
+int izh_fpol_dipole(izh_bunch *b, const izh_element *e) {
+  IZH_CHECK(dipole_check(b, e, "izh_fpol_dipole"));
+  const double gam0 = -b->refpt[5];
+  const double beta0 = std::sqrt(1.0 - 1.0 / gam0 / gam0);
+  b->refpt[4] = b->refpt[4] + e->length / (b->c.scxl * beta0);
+  return launch_dipole(b, dipole_face(b, e, false));
+}
+
+int izh_bpol_dipole(izh_bunch *b, const izh_element *e) {
+  IZH_CHECK(dipole_check(b, e, "izh_bpol_dipole"));
+  return launch_dipole(b, dipole_face(b, e, true));
+}
+
+int izh_sector_dipole(izh_bunch *b, double *z, double tau, const izh_element *e) {
+  IZH_CHECK(dipole_check(b, e, "izh_sector_dipole"));
+  if (!z) return fail(IZH_ERR_ARG, "izh_sector_dipole: bad argument");
+  // Sector_Dipole (src/Appl/Dipole.f90:712-754), beta0 from the run loop
+  // (src/Contrl/AccSimulator.f90:1060-1061)
+  const double g = -b->refpt[5];
+  const double beta0 = std::sqrt(1.0 - 1.0 / g / g);
+  DipoleMap m{};
+  m.mode = kDipoleSector;
+  m.scxl = b->c.scxl;
+  m.beta0 = beta0;
+  m.gambet = beta0 / std::sqrt(1.0 - beta0 * beta0);
+  m.gam0 = std::sqrt(1.0 / (1.0 - beta0 * beta0));
+  m.qm0 = b->charge / b->mass;
+  const double h0 = e->v[0] / e->length;
+  const double rho = 1.0 / h0, th = tau * h0;
+  const double s = std::sin(th), c = std::cos(th), sh2 = std::sin(th / 2.0);
+  m.len = tau;
+  m.R11 = c;
+  m.R12 = rho * s;
+  m.R16 = rho * (1.0 - c);
+  m.R21 = -s / rho;
+  m.R22 = c;
+  m.R26 = s;
+  m.R51 = -m.R26;
+  m.R52 = -m.R16;
+  m.R56 = rho * (s - beta0 * beta0 * th);
+  m.T111 = -s * s / (2.0 * rho);
+  m.T112 = s * c;
+  m.T116 = s * s;
+  m.T122 = rho * c * sh2 * sh2;
+  m.T126 = -rho * s * (-1.0 + c);
+  m.T166 = -rho * s * s / 2.0;
+  m.T144 = rho * (-1.0 + c) / 2.0;
+  m.T216 = s / rho;
+  m.T222 = -s / 2.0;
+  m.T266 = -s;
+  m.T244 = -s / 2.0;
+  m.T314 = s;
+  m.T324 = 2.0 * rho * sh2 * sh2;
+  m.T346 = rho * (th - s);
+  m.T522 = -0.5 * rho * s;
+  m.T544 = m.T522;   // the reference keeps T544 = T522
+  m.T526 = rho * (c - 1.0);
+  IZH_CHECK(launch_dipole(b, m));
+  *z = *z + tau;
+  return IZH_OK;
+}
+
+int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd) {
+  IZH_CHECK(dipole_check(b, e, "izh_isr_chicane"));
+  if (!rd && b->n > 0) return fail(IZH_ERR_ARG, "izh_isr_chicane: no deviates");
+  // b0 = |B rho angle / L| (src/Contrl/AccSimulator.f90:1436-1439) and the
+  // constants of ISRchicane (:1553-1570)
+  const double clight = b->c.clight;
+  const double gamma0 = -b->refpt[5];
+  const double beta = std::sqrt((gamma0 + 1.0) * (gamma0 - 1.0)) / gamma0;
+  const double brho = gamma0 * beta / clight * b->mass;
+  const double b0 = std::fabs(brho * e->v[0] / e->length);
+  const double re = 2.81794028946e-15, emass = 9.109382154e-31, hbar = 1.054571628e-34;
+  const double qmass = 1.0 / 0.511e6;
+  const double gambet0 = std::sqrt(gamma0 * gamma0 - 1.0);
+  const double c1 = 2.0 * re * gambet0 * gambet0 / 3.0;
+  const double c2 = 55.0 * re * hbar * gambet0 * gambet0 * gambet0 /
+                    (24.0 * std::sqrt(3.0) * (emass * clight));
+  const double bb = b0 * qmass * clight / gambet0;
+  const double p1 = c1 * bb * bb * tau;
+  const double sb = std::sqrt(bb);
+  const double p2 = std::sqrt(c2) * (sb * sb * sb) * std::sqrt(tau);
+  return launch_isr(b, gamma0, p1, p2, rd);
+}
+
 int izh_map_linear(izh_bunch *b, const double xm[36]) {
   if (!b || !xm) return fail(IZH_ERR_ARG, "izh_map_linear: bad argument");
   if (b->frame != kFrameZ)

@@ -191,6 +191,24 @@ struct Push {
   double xm[36];
 };
 
+// A bend map applied in Transport coordinates (x, x', y, y', z, delta) by
+// k_dipole: mode 0 the second-order sector map Sector_Dipole
+// (src/Appl/Dipole.f90:693-788), 1 the entrance pole face Fpol_Dipole
+// (:466-526), 2 the exit pole face Bpol_Dipole (:528-591).  gam0, beta0 and
+// gambet are the reference energy as each routine forms it; qm0 is the beam's
+// Charge/Mass.
+enum { kDipoleSector = 0, kDipoleEntrance = 1, kDipoleExit = 2 };
+struct DipoleMap {
+  int mode;
+  double scxl, gam0, beta0, gambet, qm0;
+  // pole faces
+  double h0, h1, k1, psi, tanphi, tanphi2, secphi2, secphi3, tanphib, secphib2;
+  // sector: first- and second-order transport coefficients
+  double len, R11, R12, R16, R21, R22, R26, R51, R52, R56;
+  double T111, T112, T116, T122, T126, T166, T144, T216, T222, T266, T244, T314,
+      T324, T346, T522, T544, T526;
+};
+
 // The inputs of the wake kernels (izh_wake.hip): the wake (kind 1 analytic
 // aa, gg, leng; kind 2 a device table of ntab rows: z, Wz, Wx, Wy), the
 // scale scwk, the reference energy of the step and the bunch constants;
@@ -220,6 +238,8 @@ inline void touch_particles(izh_bunch *b) { b->keys_valid = 0; }
 int launch_drift(izh_bunch *b, double tau, double gam0, double beta0);
 int launch_quad_kmode(izh_bunch *b, double tau, double kq, double qmass, double gambet0,
                       double beta0);
+int launch_dipole(izh_bunch *b, const DipoleMap &m);
+int launch_isr(izh_bunch *b, double gam0, double p1, double p2, const double *rd);
 // The model drift1_BeamBunch applies to an element (src/Appl/BeamBunch.f90:
 // 276-449): kMap1Drift, kMap1Linear (a host 6x6 map) or kMap1QuadK; -1 with
 // *why set for the branches not on this path.

@@ -110,6 +110,94 @@ k_quad_kmode(double *__restrict__ X, double *__restrict__ PX, double *__restrict
   }
 }
 
+// ------------------------------------------------------------------ bends
+// Sector_Dipole, Fpol_Dipole and Bpol_Dipole (src/Appl/Dipole.f90:466-788):
+// each particle goes to Transport coordinates x, x' = px/(gamma beta_z),
+// y, y', z = -phi beta0 Scxl, delta = -pt/(beta0 gambet) - dq with dq the
+// particle's relative charge-to-mass offset, through the second-order map
+// and back with gamma beta_z = |gamma beta| / sqrt(1 + x'^2 + y'^2).
+// In place, one particle per lane.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_dipole(double *__restrict__ X, double *__restrict__ PX, double *__restrict__ Y,
+         double *__restrict__ PY, double *__restrict__ PH, double *__restrict__ PT,
+         const uint32_t *__restrict__ orig, const double *__restrict__ qm0, int64_t n,
+         DipoleMap m) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double px = PX[i], py = PY[i], pt = PT[i];
+    const double dq = (qm0[orig[i]] - m.qm0) / m.qm0;
+    const double gam = m.gam0 - pt;
+    const double gbtot = sqrt(gam * gam - 1.0);
+    const double gbz = MODE == kDipoleSector ? sqrt(gbtot * gbtot - px * px - py * py)
+                                             : sqrt(gam * gam - 1.0 - px * px - py * py);
+    const double x = X[i] * m.scxl, xp = px / gbz, y = Y[i] * m.scxl, yp = py / gbz;
+    const double z = -PH[i] * m.beta0 * m.scxl;
+    const double eta = -pt / m.beta0 / m.gambet - dq;
+    double o1, o2, o3, o4, o5;
+    if (MODE == kDipoleSector) {
+      o1 = m.R11 * x + m.R12 * xp + m.R16 * eta + m.T111 * x * x + m.T112 * x * xp +
+           m.T116 * x * eta + m.T122 * xp * xp + m.T126 * xp * eta + m.T166 * eta * eta +
+           m.T144 * yp * yp;
+      o2 = m.R21 * x + m.R22 * xp + m.R26 * eta + m.T216 * x * eta + m.T222 * xp * xp +
+           m.T266 * eta * eta + m.T244 * yp * yp;
+      o3 = y + m.len * yp + m.T314 * x * yp + m.T324 * xp * yp + m.T346 * yp * eta;
+      o4 = yp;
+      o5 = m.R51 * x + m.R52 * xp + z + m.R56 * eta + m.T522 * xp * xp + m.T544 * yp * yp +
+           m.T526 * xp * eta;
+    } else {
+      const double h0 = m.h0, t = m.tanphi, t2 = m.tanphi2, s2 = m.secphi2;
+      const double a = 0.5 * h0 * m.h1 * m.secphi3 + m.k1 * t;
+      const double yeta = h0 * t - h0 * m.psi * m.secphib2;
+      if (MODE == kDipoleEntrance) {
+        o1 = x - 0.5 * h0 * t2 * x * x + 0.5 * h0 * s2 * y * y;
+        o2 = h0 * t * x + xp + h0 * t2 * x * xp + a * x * x -
+             (a - h0 * h0 * t * t2 - 0.5 * h0 * h0 * t) * y * y - h0 * t2 * y * yp -
+             h0 * t * x * eta;
+        o3 = y + h0 * t2 * x * y;
+        o4 = -h0 * m.tanphib * y + yp - h0 * t2 * x * yp - h0 * s2 * xp * y -
+             (h0 * m.h1 * m.secphi3 + 2.0 * m.k1 * t) * x * y + yeta * y * eta;
+      } else {
+        o1 = x + 0.5 * h0 * t2 * x * x - 0.5 * h0 * s2 * y * y;
+        o2 = h0 * t * x + xp - h0 * t2 * x * xp + (a - h0 * h0 * t2 * t / 2.0) * x * x -
+             (a + h0 * h0 * t * t2 / 2.0) * y * y + h0 * t2 * y * yp - h0 * t * x * eta;
+        o3 = y - h0 * t2 * x * y;
+        o4 = -h0 * m.tanphib * y + yp + h0 * t2 * x * yp + h0 * s2 * xp * y -
+             (h0 * m.h1 * m.secphi3 + 2.0 * m.k1 * t - h0 * h0 * s2 * t) * x * y +
+             yeta * y * eta;
+      }
+      o5 = z;
+    }
+    const double gbz2 = gbtot / sqrt(o2 * o2 + o4 * o4 + 1.0);
+    X[i] = o1 / m.scxl;
+    PX[i] = o2 * gbz2;
+    Y[i] = o3 / m.scxl;
+    PY[i] = o4 * gbz2;
+    PH[i] = -o5 / m.scxl / m.beta0;
+    PT[i] = -m.beta0 * m.gambet * (eta + dq);
+  }
+}
+
+// ISRchicane (src/Contrl/AccSimulator.f90:1542-1584): the mean loss p1 and
+// the fluctuation p2 * rd of incoherent synchrotron radiation remove
+// (gamma beta)^2 (p1 + p2 rd) from each particle's momentum along its
+// direction.  rd holds one deviate per particle, by original index.
+__global__ void __launch_bounds__(kBlock)
+k_isr(double *__restrict__ PX, double *__restrict__ PY, double *__restrict__ PT,
+      const uint32_t *__restrict__ orig, const double *__restrict__ rd, int64_t n,
+      double gam0, double p1, double p2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double gam = gam0 - PT[i];
+    const double gambet = sqrt(gam * gam - 1.0);
+    const double ptot = gambet * gambet * (p1 + p2 * rd[orig[i]]);
+    PX[i] = PX[i] - ptot * PX[i] / gambet;
+    PY[i] = PY[i] - ptot * PY[i] / gambet;
+    const double gambet2 = gambet - ptot;
+    PT[i] = gam0 - sqrt(1.0 + gambet2 * gambet2);
+  }
+}
+
 // -------------------------------------------------------------- conv1st
 // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352), in place.
 // Partials per block: 3 minima, 3 maxima, lost count.
@@ -474,6 +562,34 @@ int launch_quad_kmode(izh_bunch *b, double tau, double kq, double qmass, double 
   hipLaunchKernelGGL(k_quad_kmode, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream,
                      b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->orig, b->qm0,
                      b->n, tau, kq, qmass, -b->refpt[5], gambet0, beta0, b->c.scxl);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+int launch_dipole(izh_bunch *b, const DipoleMap &m) {
+  touch_particles(b);
+  if (b->n == 0) return IZH_OK;
+  ScopedTimer tm_(IZH_T_PUSH, b->stream);
+  const dim3 g(grid_for(b->n)), t(kBlock);
+  if (m.mode == kDipoleSector)
+    hipLaunchKernelGGL(k_dipole<kDipoleSector>, g, t, 0, b->stream, b->r[0], b->r[1],
+                       b->r[2], b->r[3], b->r[4], b->r[5], b->orig, b->qm0, b->n, m);
+  else if (m.mode == kDipoleEntrance)
+    hipLaunchKernelGGL(k_dipole<kDipoleEntrance>, g, t, 0, b->stream, b->r[0], b->r[1],
+                       b->r[2], b->r[3], b->r[4], b->r[5], b->orig, b->qm0, b->n, m);
+  else
+    hipLaunchKernelGGL(k_dipole<kDipoleExit>, g, t, 0, b->stream, b->r[0], b->r[1],
+                       b->r[2], b->r[3], b->r[4], b->r[5], b->orig, b->qm0, b->n, m);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+int launch_isr(izh_bunch *b, double gam0, double p1, double p2, const double *rd) {
+  touch_particles(b);
+  if (b->n == 0) return IZH_OK;
+  ScopedTimer tm_(IZH_T_PUSH, b->stream);
+  hipLaunchKernelGGL(k_isr, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream, b->r[1],
+                     b->r[3], b->r[5], b->orig, rd, b->n, gam0, p1, p2);
   IZH_HIP(hipGetLastError());
   return IZH_OK;
 }

@@ -165,6 +165,37 @@ module impactz_hip
       type(izh_element), intent(in) :: e
       integer(c_int) :: rc
     end function
+    ! bends: Fpol_Dipole (with the reference phase advance over the bend),
+    ! Sector_Dipole, Bpol_Dipole and ISRchicane (src/Appl/Dipole.f90:466-788,
+    ! src/Contrl/AccSimulator.f90:1542-1584)
+    function izh_fpol_dipole(b, e) bind(C, name="izh_fpol_dipole") result(rc)
+      import :: c_int, c_ptr, izh_element
+      type(c_ptr), value :: b
+      type(izh_element), intent(in) :: e
+      integer(c_int) :: rc
+    end function
+    function izh_sector_dipole(b, z, tau, e) bind(C, name="izh_sector_dipole") result(rc)
+      import :: c_int, c_double, c_ptr, izh_element
+      type(c_ptr), value :: b
+      real(c_double) :: z
+      real(c_double), value :: tau
+      type(izh_element), intent(in) :: e
+      integer(c_int) :: rc
+    end function
+    function izh_bpol_dipole(b, e) bind(C, name="izh_bpol_dipole") result(rc)
+      import :: c_int, c_ptr, izh_element
+      type(c_ptr), value :: b
+      type(izh_element), intent(in) :: e
+      integer(c_int) :: rc
+    end function
+    function izh_isr_chicane(b, tau, e, rd) bind(C, name="izh_isr_chicane") result(rc)
+      import :: c_int, c_double, c_ptr, izh_element
+      type(c_ptr), value :: b
+      real(c_double), value :: tau
+      type(izh_element), intent(in) :: e
+      type(c_ptr), value :: rd
+      integer(c_int) :: rc
+    end function
     ! read1_Data's table (rows z, Ez, Ez', Ez'') / read3_Data's coefficients
     function izh_rfdata_table(rows4, n, d) bind(C, name="izh_rfdata_table") result(rc)
       import :: c_int, c_double, c_ptr

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import argparse
 import dataclasses
+import itertools
 import os
 from dataclasses import dataclass, field
 
@@ -236,6 +237,19 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
                                 "(ideal RF cavity model) is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
                                    more=tuple(vv[4:]))))
+        elif e.type == 4:
+            # bends at zero current: with current the run loop adds ISRchicane,
+            # whose deviates come from the compiler's random generator, and CSR
+            # (src/Contrl/AccSimulator.f90:1132-1139, 1430-1448)
+            if deck.bcurr > 1.0e-15:
+                raise DeckError(f"element {k}: a bend (type 4) with current "
+                                f"{deck.bcurr:g} A (ISR, space charge, CSR) is not on "
+                                "this path")
+            vv = (v + [0.0] * 14)[:14]
+            if not (e.length > 0.0 and vv[0] != 0.0):
+                raise DeckError(f"element {k}: a bend needs a length > 0 and an angle != 0")
+            out.append((k, Element(e.length, e.nseg, e.mapstp, 4, *vv[:4],
+                                   more=tuple(vv[4:]))))
         elif e.type == -41:
             vv = (v + [0.0] * 14)[:14]
             out.append((k, WakeSwitch(k, Element(e.length, e.nseg, e.mapstp, -41, *vv[:4],
@@ -255,7 +269,7 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
         else:
             raise DeckError(f"element {k}: type {e.type} is not on this path "
                             "(drift 0, quadrupole 1, constant focusing 2, solenoid 3 and the "
-                            "RF cavities 101-105 are)")
+                            "RF cavities 101-105 are, and bends (4) at zero current)")
     return out, notes
 
 
@@ -415,16 +429,19 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
         def flush():
             if not pending:
                 return
-            if fused and sim.flagmap == 1:
-                z = sim.z
-                recs = sim.run_elements([e for e, _ in pending])
-                for (e, tau1), rec in zip(pending, recs):
-                    z = (z + tau1) + tau1  # drift1_BeamBunch advances z by tau1 twice
-                    rows.append(np.concatenate([[z], sim.diag(rec)]))
-            else:
-                for e, _ in pending:
-                    mom = sim.step(e)
-                    rows.append(np.concatenate([[sim.z], sim.diag(mom)]))
+            # stretches of linear-map steps fuse; bends run step by step
+            for bend, grp in itertools.groupby(pending, key=lambda p: p[0].type == 4):
+                grp = list(grp)
+                if fused and sim.flagmap == 1 and not bend:
+                    z = sim.z
+                    recs = sim.run_elements([e for e, _ in grp])
+                    for (e, tau1), rec in zip(grp, recs):
+                        z = (z + tau1) + tau1  # drift1_BeamBunch advances z by tau1 twice
+                        rows.append(np.concatenate([[z], sim.diag(rec)]))
+                else:
+                    for e, _ in grp:
+                        mom = sim.step(e)
+                        rows.append(np.concatenate([[sim.z], sim.diag(mom)]))
             pending.clear()
 
         for index, p in prog:
@@ -457,6 +474,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
                     budget -= 1
             st["i"], st["j"] = index, (p.nseg + 1 if p.nseg >= 1 else 1)
         flush()
+        sim.finish_element()
         rows = np.array(rows)
     finally:
         sim.close()
@@ -465,6 +483,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
 
 
 def _file_action(p: FileAction, sim: Simulation, outdir: str, st: dict, deck: Deck):
+    sim.finish_element()
     pts = sim.bunch.getpts()
     refpt = sim.bunch.refpt
     if p.type == -2:

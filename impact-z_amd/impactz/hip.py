@@ -33,7 +33,8 @@ SYMBOLS = (
     "izh_cvbkforth1st", "izh_kick1", "izh_diagnostic1", "izh_field_download",
     "izh_field_upload_rho", "izh_step", "izh_run", "izh_timer_enable",
     "izh_timer_read", "izh_run_stats", "izh_probe_copy", "izh_element_maplinear",
-    "izh_map1", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
+    "izh_map1", "izh_fpol_dipole", "izh_sector_dipole", "izh_bpol_dipole",
+    "izh_isr_chicane", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
     "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
     "izh_wake_download", "izh_kick2wake",
 )
@@ -166,6 +167,10 @@ def lib() -> C.CDLL:
                                         C.POINTER(ElementC), _dp, C.c_double, C.c_double,
                                         _dp]
     L.izh_map1.argtypes = [vp, _dp, C.c_double, C.POINTER(ElementC)]
+    L.izh_fpol_dipole.argtypes = [vp, C.POINTER(ElementC)]
+    L.izh_bpol_dipole.argtypes = [vp, C.POINTER(ElementC)]
+    L.izh_sector_dipole.argtypes = [vp, _dp, C.c_double, C.POINTER(ElementC)]
+    L.izh_isr_chicane.argtypes = [vp, C.c_double, C.POINTER(ElementC), vp]
     L.izh_rfdata_table.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_fourier.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_destroy.argtypes = [vp]
@@ -461,6 +466,26 @@ class BeamBunch:
         e = element(bitype, v, length, zedge, mapstp, rf)
         check(lib().izh_map1(self.h, C.byref(zz), tau, C.byref(e)))
         return zz.value
+
+    def fpol_dipole(self, length: float, v) -> None:
+        """A bend's entrance: refptcl(5) += L/(Scxl beta0) and Fpol_Dipole."""
+        check(lib().izh_fpol_dipole(self.h, C.byref(element(4, v, length))))
+
+    def bpol_dipole(self, length: float, v) -> None:
+        """A bend's exit face Bpol_Dipole."""
+        check(lib().izh_bpol_dipole(self.h, C.byref(element(4, v, length))))
+
+    def sector_dipole(self, z: float, tau: float, length: float, v) -> float:
+        """Sector_Dipole over tau; returns z + tau."""
+        zz = C.c_double(z)
+        check(lib().izh_sector_dipole(self.h, C.byref(zz), tau, C.byref(element(4, v, length))))
+        return zz.value
+
+    def isr_chicane(self, tau: float, length: float, v, rd_ptr: int) -> None:
+        """ISRchicane over tau with the device deviates at rd_ptr (one double
+        per particle, by row at upload)."""
+        check(lib().izh_isr_chicane(self.h, tau, C.byref(element(4, v, length)),
+                                    C.c_void_p(rd_ptr)))
 
     def drift2(self, z: float, tau: float) -> float:
         """drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2; returns z."""

@@ -148,7 +148,43 @@ class Simulation:
         self._left -= 1
         return self.zedge
 
+    def finish_element(self) -> None:
+        """The exit face of a bend whose last step has run; the run loop
+        applies it after that step's diagnostics, at the element's end
+        (src/Contrl/AccSimulator.f90:1491-1494)."""
+        e, self._exit = getattr(self, "_exit", None), None
+        if e is not None:
+            self.bunch.bpol_dipole(e.length, e.values)
+
+    def bend_step(self, e: Element, want_moments: bool = True):
+        """A bend segment (src/Contrl/AccSimulator.f90:1032-1091, 1114-1120,
+        1430-1448) at zero current under either integrator: the entrance face
+        on the element's first step, then two Sector_Dipole half steps.  With
+        current the reference adds ISRchicane with its own random deviates and
+        CSR, which this path does not run."""
+        if self.beam.current > 1.0e-15:
+            raise hip.IZHError("bends with current (ISR, space charge, CSR) are not on "
+                               "this path")
+        entering = self._left == 0
+        self._enter(e)
+        b = self.bunch
+        if entering:
+            b.fpol_dipole(e.length, e.values)
+        tau1 = 0.5 * e.length / e.nseg
+        self.z = b.sector_dipole(self.z, tau1, e.length, e.values)
+        self.z = b.sector_dipole(self.z, tau1, e.length, e.values)
+        if self._left == 0:
+            self._exit = e
+        self.nstep += 1
+        if not want_moments:
+            return None
+        s, m, r2 = b.diagnostic1()
+        return np.concatenate([s, m, [r2]])
+
     def step(self, e: Element, want_moments: bool = True):
+        self.finish_element()
+        if e.type == 4:
+            return self.bend_step(e, want_moments)
         zedge = self._enter(e)
         if self.flagmap == 2:
             self.z, mom = self.bunch.step2(
@@ -230,6 +266,7 @@ class Simulation:
         if self.flagmap != 1:
             raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
                                "Flagmap=2 runs step by step")
+        self.finish_element()
         segs = self._segments_of(self._window(start, nsteps))
         self.z, mom = self.bunch.run(self.field, self.dom, self.z, segs,
                                      self.beam.current, want_moments)
@@ -242,6 +279,7 @@ class Simulation:
         if self.flagmap != 1:
             raise hip.IZHError("izh_run fuses the linear-map steps (Flagmap=1); "
                                "Flagmap=2 runs step by step")
+        self.finish_element()
         elements = list(elements)
         self.z, mom = self.bunch.run(self.field, self.dom, self.z,
                                      self._segments_of(elements), self.beam.current,

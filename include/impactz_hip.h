@@ -162,8 +162,28 @@ int izh_quad_maplinear(const izh_const *c, double tau, double gradient,
  * *z += tau.  Refused with IZH_ERR_ARG: the undulator map (bitype 1,
  * v[1] < -10), quadrupole fields from a file (v[1] > 1e-5), the ideal
  * RF-cavity branch (bitype >= 2 with Param(5) = v[3] < 0), an RF cavity
- * without its field, SolRF wakefields (v[13] > 0) and other element types. */
+ * without its field and other element types (bends: izh_sector_dipole). */
 int izh_map1(izh_bunch *b, double *z, double tau, const izh_element *e);
+
+/* Bends (bitype 4).  Their values, as run_AccSimulator reads them
+ * (src/Contrl/AccSimulator.f90:1058-1088): v[0] bend angle [rad], v[1] k1,
+ * v[2] CSR switch, v[3] half gap [m], v[4] and v[5] the entrance and exit
+ * pole-face angles [rad], v[6] and v[7] the pole-face curvatures 1/r1 and
+ * 1/r2 [1/m], v[8] the fringe-field integral of both faces.  The reference
+ * energy is the bunch's, which no bend map changes.
+ * izh_fpol_dipole: the phase advance of the reference particle over the
+ * whole bend, refpt[4] += L/(Scxl beta0) (src/Contrl/AccSimulator.f90:1062),
+ * then the entrance face Fpol_Dipole (src/Appl/Dipole.f90:466-526). */
+int izh_fpol_dipole(izh_bunch *b, const izh_element *e);
+/* Sector_Dipole over tau (src/Appl/Dipole.f90:693-788); *z += tau. */
+int izh_sector_dipole(izh_bunch *b, double *z, double tau, const izh_element *e);
+/* The exit face Bpol_Dipole (src/Appl/Dipole.f90:528-591). */
+int izh_bpol_dipole(izh_bunch *b, const izh_element *e);
+/* ISRchicane (src/Contrl/AccSimulator.f90:1542-1584), the synchrotron
+ * radiation of a bend with current, over tau.  rd: device array of one
+ * standard-normal deviate per particle, indexed by the particle's row at
+ * izh_bunch_create (the reference draws them with normVec1). */
+int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd);
 
 /* izh_map1 for bitype 0 and 1 with v = {gradient, 0, ...}. */
 int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,

@@ -1005,6 +1005,158 @@ void or_kick1wake(const or_const *c, double *pts, int64_t n, int nx, int ny, int
 
 /* ----------------------------------------- nonlinear integrator (Flagmap=2) */
 
+/* ---------------------------------------------------------------- bends */
+
+/* Transport coordinates of particle i (src/Appl/Dipole.f90:489-497,
+ * 758-764): t[0..5] = x, x', y, y', z, delta with the particle's own
+ * gamma beta_z = bz. */
+static void to_transport(const or_const *c, const double *pts, int64_t i, double bz,
+                         double beta0, double gambet, double qm0, double t[6]) {
+  t[0] = P(1, i) * c->scxl;
+  t[1] = P(2, i) / bz;
+  t[2] = P(3, i) * c->scxl;
+  t[3] = P(4, i) / bz;
+  t[4] = -P(5, i) * beta0 * c->scxl;
+  t[5] = -P(6, i) / beta0 / gambet - (P(7, i) - qm0) / qm0;
+}
+
+/* and back (src/Appl/Dipole.f90:516-523, 779-786); gb = |gamma beta| */
+static void from_transport(const or_const *c, double *pts, int64_t i, double gb,
+                           double beta0, double gambet, double qm0, const double t[6]) {
+  double bz = gb / sqrt(t[1] * t[1] + t[3] * t[3] + 1.0);
+  P(1, i) = t[0] / c->scxl;
+  P(2, i) = t[1] * bz;
+  P(3, i) = t[2] / c->scxl;
+  P(4, i) = t[3] * bz;
+  P(5, i) = -t[4] / c->scxl / beta0;
+  P(6, i) = -beta0 * gambet * (t[5] + (P(7, i) - qm0) / qm0);
+}
+
+void or_dipole_face(const or_const *c, double *pts, int64_t n, double refpt[6],
+                    const or_element *e, int exit, double charge, double mass) {
+  /* the run loop's constants (src/Contrl/AccSimulator.f90:1058-1088) */
+  const double *v = e->v;
+  double gam0 = -refpt[5];
+  double beta0 = sqrt(1.0 - 1.0 / gam0 / gam0);
+  double gambet = beta0 * gam0;
+  double qm0 = charge / mass;
+  double hgap = 2 * v[3];
+  double h0 = v[0] / e->length, k1 = v[1];
+  double ang = exit ? v[5] : v[4];
+  double h1 = exit ? v[7] : v[6];
+  double psi = h0 * hgap * v[8] * (1.0 + sin(ang) * sin(ang)) / cos(ang);
+  double tanphi = tan(ang), tanphib = tan(ang - psi);
+  double tanphi2 = tanphi * tanphi, secphi2 = 1.0 + tanphi2;
+  double secphi3 = secphi2 * (1.0 / cos(ang)), secphib2 = 1.0 + tanphib * tanphib;
+  int64_t i;
+  if (!exit) refpt[4] = refpt[4] + e->length / (c->scxl * beta0);
+  for (i = 0; i < n; i++) {
+    double gamn = gam0 - P(6, i);
+    double bz = sqrt(gamn * gamn - 1.0 - P(2, i) * P(2, i) - P(4, i) * P(4, i));
+    double t[6], o[6];
+    double x, xp, y, yp, d;
+    to_transport(c, pts, i, bz, beta0, gambet, qm0, t);
+    x = t[0]; xp = t[1]; y = t[2]; yp = t[3]; d = t[5];
+    if (!exit) {  /* Fpol_Dipole (src/Appl/Dipole.f90:499-512) */
+      o[0] = x - 0.5 * h0 * tanphi2 * x * x + 0.5 * h0 * secphi2 * y * y;
+      o[1] = h0 * tanphi * x + xp + h0 * tanphi2 * x * xp +
+             (0.5 * h0 * h1 * secphi3 + k1 * tanphi) * x * x -
+             (0.5 * h0 * h1 * secphi3 + k1 * tanphi - h0 * h0 * tanphi * tanphi2 -
+              0.5 * h0 * h0 * tanphi) * y * y -
+             h0 * tanphi2 * y * yp - h0 * tanphi * x * d;
+      o[2] = y + h0 * tanphi2 * x * y;
+      o[3] = -h0 * tanphib * y + yp - h0 * tanphi2 * x * yp - h0 * secphi2 * xp * y -
+             (h0 * h1 * secphi3 + 2 * k1 * tanphi) * x * y +
+             (h0 * tanphi - h0 * psi * secphib2) * y * d;
+    } else {      /* Bpol_Dipole (src/Appl/Dipole.f90:562-576) */
+      o[0] = x + 0.5 * h0 * tanphi2 * x * x - 0.5 * h0 * secphi2 * y * y;
+      o[1] = h0 * tanphi * x + xp - h0 * tanphi2 * x * xp +
+             (0.5 * h0 * h1 * secphi3 + k1 * tanphi - h0 * h0 * tanphi2 * tanphi / 2) * x * x -
+             (0.5 * h0 * h1 * secphi3 + k1 * tanphi + h0 * h0 * tanphi * tanphi2 / 2) * y * y +
+             h0 * tanphi2 * y * yp - h0 * tanphi * x * d;
+      o[2] = y - h0 * tanphi2 * x * y;
+      o[3] = -h0 * tanphib * y + yp + h0 * tanphi2 * x * yp + h0 * secphi2 * xp * y -
+             (h0 * h1 * secphi3 + 2 * k1 * tanphi - h0 * h0 * secphi2 * tanphi) * x * y +
+             (h0 * tanphi - h0 * psi * secphib2) * y * d;
+    }
+    o[4] = t[4];
+    o[5] = t[5];
+    from_transport(c, pts, i, sqrt(gamn * gamn - 1.0), beta0, gambet, qm0, o);
+  }
+}
+
+void or_sector_dipole(const or_const *c, double *pts, int64_t n, const double refpt[6],
+                      double *z, double tau, const or_element *e, double charge,
+                      double mass) {
+  double g = -refpt[5];
+  double beta0 = sqrt(1.0 - 1.0 / g / g);   /* src/Contrl/AccSimulator.f90:1060-1061 */
+  double qm0 = charge / mass;
+  double h0 = e->v[0] / e->length;
+  double rho = 1.0 / h0, th = tau * h0;
+  double s = sin(th), co = cos(th), sh = sin(th / 2.0);
+  /* first- and second-order transport elements (src/Appl/Dipole.f90:712-743) */
+  double R11 = co, R12 = rho * s, R16 = rho * (1.0 - co), R21 = -s / rho, R22 = co;
+  double R26 = s, R51 = -R26, R52 = -R16, R56 = rho * (s - beta0 * beta0 * th);
+  double T111 = -s * s / (2.0 * rho), T112 = s * co, T116 = s * s;
+  double T122 = rho * co * sh * sh, T126 = -rho * s * (-1.0 + co);
+  double T166 = -rho * s * s / 2.0, T144 = rho * (-1.0 + co) / 2.0;
+  double T216 = s / rho, T222 = -s / 2.0, T266 = -s, T244 = -s / 2.0;
+  double T314 = s, T324 = 2.0 * rho * sh * sh, T346 = rho * (th - s);
+  double T522 = -0.5 * rho * s, T544 = T522, T526 = rho * (co - 1.0);
+  double gambet = beta0 / sqrt(1.0 - beta0 * beta0);
+  double gam0 = sqrt(1.0 / (1.0 - beta0 * beta0));
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    double gami = -P(6, i) + gam0;
+    double gbi = sqrt(gami * gami - 1.0);
+    double bz = sqrt(gbi * gbi - P(2, i) * P(2, i) - P(4, i) * P(4, i));
+    double t[6], o[6];
+    double x0, xp0, y0, yp0, z0, eta;
+    to_transport(c, pts, i, bz, beta0, gambet, qm0, t);
+    x0 = t[0]; xp0 = t[1]; y0 = t[2]; yp0 = t[3]; z0 = t[4]; eta = t[5];
+    o[0] = R11 * x0 + R12 * xp0 + R16 * eta + T111 * x0 * x0 + T112 * x0 * xp0 +
+           T116 * x0 * eta + T122 * xp0 * xp0 + T126 * xp0 * eta + T166 * eta * eta +
+           T144 * yp0 * yp0;
+    o[1] = R21 * x0 + R22 * xp0 + R26 * eta + T216 * x0 * eta + T222 * xp0 * xp0 +
+           T266 * eta * eta + T244 * yp0 * yp0;
+    o[2] = y0 + tau * yp0 + T314 * x0 * yp0 + T324 * xp0 * yp0 + T346 * yp0 * eta;
+    o[3] = yp0;
+    o[4] = R51 * x0 + R52 * xp0 + z0 + R56 * eta + T522 * xp0 * xp0 + T544 * yp0 * yp0 +
+           T526 * xp0 * eta;
+    o[5] = eta;
+    from_transport(c, pts, i, gbi, beta0, gambet, qm0, o);
+  }
+  *z = *z + tau;
+}
+
+void or_isr_chicane(const or_const *c, double *pts, int64_t n, const double refpt[6],
+                    double tau, const or_element *e, double mass, const double *rd) {
+  /* b0 of the run loop (src/Contrl/AccSimulator.f90:1436-1439) */
+  double gamma0 = -refpt[5];
+  double beta = sqrt((gamma0 + 1.0) * (gamma0 - 1.0)) / gamma0;
+  double brho = gamma0 * beta / c->clight * mass;
+  double b0 = fabs(brho * e->v[0] / e->length);
+  /* ISRchicane (src/Contrl/AccSimulator.f90:1553-1582) */
+  double re = 2.81794028946e-15, emass = 9.109382154e-31, hbar = 1.054571628e-34;
+  double qmass = 1.0 / 0.511e6;
+  double gambet0 = sqrt(gamma0 * gamma0 - 1.0);
+  double c1 = 2 * re * gambet0 * gambet0 / 3;
+  double c2 = 55.0 * re * hbar * pow(gambet0, 3) / (24.0 * sqrt(3.0) * (emass * c->clight));
+  double bb = b0 * qmass * c->clight / gambet0;
+  double p1 = c1 * bb * bb * tau;
+  double p2 = sqrt(c2) * pow(sqrt(bb), 3) * sqrt(tau);
+  int64_t i;
+  for (i = 0; i < n; i++) {
+    double gam = gamma0 - P(6, i);
+    double gb = sqrt(gam * gam - 1.0);
+    double ptot = gb * gb * (p1 + p2 * rd[i]);
+    double gb2 = gb - ptot;
+    P(2, i) = P(2, i) - ptot * P(2, i) / gb;
+    P(4, i) = P(4, i) - ptot * P(4, i) / gb;
+    P(6, i) = gamma0 - sqrt(1.0 + gb2 * gb2);
+  }
+}
+
 void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], double *z,
                double tau) {
   /* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509) */

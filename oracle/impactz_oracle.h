@@ -133,6 +133,21 @@ void or_kick1(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,
 
 /* drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2;
  * z and refpt(5) advance. */
+/* Bends (bitype 4; v[0] angle, v[1] k1, v[3] half gap, v[4]/v[5] pole-face
+ * angles, v[6]/v[7] pole-face curvatures, v[8] fringe integral):
+ * or_dipole_face: exit = 0 the run loop's refpt[4] += L/(Scxl beta0)
+ * (src/Contrl/AccSimulator.f90:1058-1091) and Fpol_Dipole
+ * (src/Appl/Dipole.f90:466-526); exit = 1 Bpol_Dipole (:528-591).
+ * or_sector_dipole: Sector_Dipole (:693-788) over tau, *z += tau.
+ * or_isr_chicane: ISRchicane (src/Contrl/AccSimulator.f90:1542-1584) with
+ * the caller's deviates rd[i] for particle i. */
+void or_dipole_face(const or_const *c, double *pts, int64_t n, double refpt[6],
+                    const or_element *e, int exit, double charge, double mass);
+void or_sector_dipole(const or_const *c, double *pts, int64_t n, const double refpt[6],
+                      double *z, double tau, const or_element *e, double charge,
+                      double mass);
+void or_isr_chicane(const or_const *c, double *pts, int64_t n, const double refpt[6],
+                    double tau, const or_element *e, double mass, const double *rd);
 void or_drift2(const or_const *c, double *pts, int64_t n, double refpt[6], double *z,
                double tau);
 

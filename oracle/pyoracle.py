@@ -158,6 +158,13 @@ def lib() -> C.CDLL:
         L.or_map1.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp, C.c_double,
                               C.POINTER(OrElement), C.c_double, C.c_double]
         L.or_map1.restype = C.c_int
+        L.or_dipole_face.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp,
+                                     C.POINTER(OrElement), C.c_int, C.c_double, C.c_double]
+        L.or_sector_dipole.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, _dp,
+                                       C.c_double, C.POINTER(OrElement), C.c_double,
+                                       C.c_double]
+        L.or_isr_chicane.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp, C.c_double,
+                                     C.POINTER(OrElement), C.c_double, _dp]
         L.or_conv1st.argtypes = [C.POINTER(OrConst), _dp, C.c_int64, _dp,
                                  C.c_double, C.c_double, _dp, _i64p, _dp]
         L.or_conv1st.restype = C.c_int64
@@ -259,6 +266,32 @@ def map1(c, pts, refpt, z, tau, bitype, v, mapstp, charge, mass, length=0.0, zed
                      tau, C.byref(e), charge, mass) != 0:
         raise ValueError(f"element type {bitype} with values {list(e.v)} is not restated")
     return zz.value
+
+
+def dipole_face(c, pts, refpt, length, v, exit, charge, mass):
+    """Fpol_Dipole (exit=False, with the bend's refpt(5) advance) or
+    Bpol_Dipole (exit=True) in place."""
+    e = make_element(4, v, 1, length)
+    lib().or_dipole_face(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(e),
+                         int(bool(exit)), charge, mass)
+
+
+def sector_dipole(c, pts, refpt, z, tau, length, v, charge, mass):
+    """Sector_Dipole over tau in place; returns the new z."""
+    zz = C.c_double(z)
+    e = make_element(4, v, 1, length)
+    lib().or_sector_dipole(C.byref(c), _p(pts), pts.shape[0], _p(refpt), C.byref(zz),
+                           tau, C.byref(e), charge, mass)
+    return zz.value
+
+
+def isr_chicane(c, pts, refpt, tau, length, v, mass, rd):
+    """ISRchicane over tau in place with deviates rd (one per row of pts)."""
+    e = make_element(4, v, 1, length)
+    rd = np.ascontiguousarray(rd, dtype=np.float64)
+    assert rd.shape == (pts.shape[0],)
+    lib().or_isr_chicane(C.byref(c), _p(pts), pts.shape[0], _p(refpt), tau, C.byref(e),
+                         mass, _p(rd))
 
 
 def drift2(c, pts, refpt, z, tau):
@@ -465,12 +498,15 @@ class Run:
         nx, ny, nz = self.grid
         rho = np.zeros((nz, ny, nx)) if want_fields else None
         phi = np.zeros((nz, ny, nx)) if want_fields else None
+        self.finish_element()
         tau1 = 0.5 * e.length / e.nseg
         if self._left == 0:
             # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
             self._left, self.zedge = e.nseg, self.st.z
             self._wake_entry(e)
         self._left -= 1
+        if e.type == 4:
+            return self._bend_step(e, tau1, entering=self._left == e.nseg - 1)
         el = make_element(e.type, e.values, e.mapstp, e.length, self.zedge, self.rfdata(e))
         out = (_p(rho) if want_fields else None, _p(phi) if want_fields else None)
         if self.flagmap == 1:
@@ -488,10 +524,38 @@ class Run:
         self.n = n
         return rho, phi
 
+    def _bend_step(self, e, tau1, entering):
+        """A bend segment at zero current (src/Contrl/AccSimulator.f90:1032-1091,
+        1114-1120, 1434-1448, 1491-1494): the entrance face first, two half
+        sector maps, the exit face last; both integrators take this path."""
+        if self.st.bcurr >= float(np.float32(1.0e-10)):
+            raise ValueError("bends with space charge (ISR, CSR) are not restated")
+        pts = self.pts[: self.n]
+        refpt = np.ctypeslib.as_array(self.st.refpt)
+        if entering:
+            dipole_face(self.c, pts, refpt, e.length, e.values, False, self.st.charge,
+                        self.st.mass)
+        z = sector_dipole(self.c, pts, refpt, self.st.z, tau1, e.length, e.values,
+                          self.st.charge, self.st.mass)
+        self.st.z = sector_dipole(self.c, pts, refpt, z, tau1, e.length, e.values,
+                                  self.st.charge, self.st.mass)
+        if self._left == 0:
+            self._exit = e
+        return None, None
+
+    def finish_element(self):
+        """The exit face of a bend whose last step has run: the run loop
+        applies it after that step's diagnostics, at the element's end."""
+        e, self._exit = getattr(self, "_exit", None), None
+        if e is not None:
+            dipole_face(self.c, self.pts[: self.n], np.ctypeslib.as_array(self.st.refpt),
+                        e.length, e.values, True, self.st.charge, self.st.mass)
+
     def particles(self):
+        self.finish_element()
         return self.pts[: self.n]
 
     def diag(self):
-        s, m, r2 = diag_sums(self.particles())
+        s, m, r2 = diag_sums(self.pts[: self.n])
         refpt = np.array(self.st.refpt[:])
         return diag_row(self.c, s, m, r2, self.st.npt, refpt, self.st.mass)

@@ -380,6 +380,50 @@ def exe_nonlinear_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
     return out
 
 
+def bend_lattice():
+    """Two bends at zero current with pole-face angles, curvatures, fringe
+    integrals and k1 (which enters the pole faces), between drifts and a
+    quadrupole, then a -2 phase dump to fort.60."""
+    return [B.Element(0.3, 2, 1, 0, 1.0),
+            B.Element(0.8, 4, 1, 4, 0.25, 0.5, 0.0, 0.02,
+                      more=(0.1, 0.15, 0.5, -0.3, 0.4)),
+            B.Element(0.4, 2, 1, 0, 1.0),
+            B.Element(0.1, 2, 10, 1, 5.0, 0.0, 1.0),
+            B.Element(0.5, 3, 1, 4, -0.15, 0.0, 0.0, 0.015,
+                      more=(-0.05, 0.0, 0.0, 0.0, 0.5)),
+            B.Element(0.2, 2, 1, 0, 1.0),
+            B.Element(0.0, 0, 60, -2, 1.0)]
+
+
+def exe_bend_fixture(n=1000, grid=(16, 16, 16)):
+    """Bends (type 4) through the reference executable (ImpactZexe built from
+    source) at zero current, with the linear-map and the Lorentz integrator:
+    the moments every step and the final particles of the -2 dump."""
+    beam = B.Beam(current=0.0)
+    pts = B.gaussian_bunch(n, seed=11, beam=beam)
+    lat = bend_lattice()
+    tmp = tempfile.mkdtemp(prefix="impz_bend_")
+    out = {"pts_in": pts, "grid": np.array(grid),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0]),
+           "lattice": np.array([[e.length, e.nseg, e.mapstp, e.type, *e.values]
+                                for e in lat])}
+    for flagmap in (1, 2):
+        d = os.path.join(tmp, f"m{flagmap}")
+        os.makedirs(d)
+        B.write_particle_in(os.path.join(d, "particle.in"), pts)
+        B.write_deck(os.path.join(d, "ImpactZ.in"), n, grid, beam, lat, flagmap=flagmap)
+        _run_exe(d)
+        out[f"m{flagmap}_phase60"] = np.loadtxt(os.path.join(d, "fort.60"), ndmin=2)
+        for k in (18, 24, 25, 26):
+            out[f"m{flagmap}_fort{k}"] = R.read_fort(os.path.join(d, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "exe_bend.npz")
+    np.savez_compressed(path, **out)
+    print(f"exe_bend: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
 def main():
     if not R.harness_available():
         raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
@@ -442,6 +486,9 @@ def main():
     meta["exe_wake"] = ("ImpactZexe run of the wake_file lattice: -41 read-in wake on with "
                         "its transverse part, then longitudinal only; fort.18/24-26")
     exe_nonlinear_fixture()
+    exe_bend_fixture()
+    meta["exe_bend"] = ("ImpactZexe runs of two bends at zero current (pole faces, "
+                        "fringe, k1) under Flagmap=1 and 2: fort.18/24-26 and the -2 dump")
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "
                              "a solenoid, and a Flagmap=2 FODO period")

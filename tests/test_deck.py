@@ -234,3 +234,25 @@ def test_deck_wake_switches(tmp_path):
     # under Flagmap=2 the wake is applied by kick2wake
     _write(p, HEADER.replace("6 3 1 0 1", "6 3 2 0 1") + "0.0 0 1 -41 1.5 3.0 20.0 /\n")
     assert isinstance(program_of(read_deck(str(p)))[0][0][1], WakeSwitch)
+
+
+def test_deck_bends_at_zero_current_only(tmp_path):
+    """Bends are taken at zero current; with current the reference adds
+    ISRchicane (random deviates) and CSR, so the deck is refused by name."""
+    from impactz import beam as B
+    from impactz.deck import DeckError, program_of, read_deck
+    bend = B.Element(0.8, 4, 1, 4, 0.25, 0.5, 0.0, 0.02, more=(0.1, 0.15, 0.5, -0.3, 0.4))
+    pts = B.gaussian_bunch(10, seed=1)
+    for cur, ok in ((0.0, True), (0.1, False)):
+        path = str(tmp_path / f"deck{cur}.in")
+        B.write_deck(path, 10, (8, 8, 8), B.Beam(current=cur), [B.Element(0.3, 2, 1, 0, 1.0),
+                                                                 bend])
+        deck = read_deck(path)
+        if ok:
+            prog, _ = program_of(deck)
+            e = prog[1][1]
+            assert e.type == 4 and e.values[:9] == pytest.approx(
+                [0.25, 0.5, 0.0, 0.02, 0.1, 0.15, 0.5, -0.3, 0.4])
+        else:
+            with pytest.raises(DeckError, match="bend"):
+                program_of(deck)

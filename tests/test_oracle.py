@@ -729,3 +729,67 @@ def test_oracle_kick2wake_trajectory_matches_reference(oracle, name):
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
     assert col_err(run.particles(), d["final"]) < 1e-12
+
+
+@pytest.mark.parametrize("flagmap", [1, 2])
+def test_oracle_matches_executable_bends(oracle, flagmap):
+    """Bends at zero current through the reference executable
+    (tests/golden/exe_bend.npz, both integrators): the moments of every step
+    and the particles of the final -2 dump.  Pins Fpol_Dipole, Sector_Dipole
+    and Bpol_Dipole with pole-face angles, curvatures, fringe integrals and
+    k1, the refptcl(5) advance at the entrance and the z bookkeeping."""
+    d = load_golden("exe_bend")
+    grid = tuple(int(v) for v in d["grid"])
+    lat = fixture_lattice(d)
+    run = oracle.Run(d["pts_in"], fixture_beam(d), [e for e in lat if e.type >= 0], grid,
+                     flagmap=flagmap)
+    k = 0
+    for e in lat:
+        if e.type < 0:
+            continue
+        for _ in range(e.nseg):
+            run.step(e)
+            k += 1
+            row = run.diag()
+            for f, lo in ((24, 5), (25, 11), (26, 17)):
+                ref = d[f"m{flagmap}_fort{f}"][k, 1:]
+                got = row[lo:lo + 6]
+                scale = np.abs(ref).copy()
+                scale[0] = max(scale[0], abs(ref[1]))
+                scale[2] = max(scale[2], abs(ref[3]))
+                assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+            assert run.st.z == pytest.approx(d[f"m{flagmap}_fort24"][k, 0], rel=1e-14)
+    assert k == d[f"m{flagmap}_fort24"].shape[0] - 1
+    ref = d[f"m{flagmap}_phase60"]
+    got = run.particles()
+    scale = np.abs(ref).max(axis=0) + 1e-300
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale), np.abs(got - ref).max(axis=0)
+
+
+def test_oracle_isr_closed_forms(oracle):
+    """ISRchicane against the textbook synchrotron-radiation results for an
+    electron: with rd = 0 the loss over tau is the Larmor energy
+    (2/3) r_e gamma^4 tau / rho^2 (in units of m c^2, ultra-relativistic),
+    and the spread rd adds has the variance of quantum excitation,
+    (55 / (24 sqrt 3)) r_e (hbar / m c) gamma^7 tau / rho^3, in (gamma beta)^2."""
+    re, hbar, me, c = 2.81794028946e-15, 1.054571628e-34, 9.109382154e-31, 299792458.0
+    mass, gam0, angle, length, tau = 0.511e6, 2000.0, 0.1, 2.0, 0.05
+    rho = length / angle
+    cst = oracle.physconst(1.3e9)
+    v = [angle] + [0.0] * 13
+    refpt = np.array([0, 0, 0, 0, 0, -gam0])
+    pts = np.zeros((2, 9))
+    pts[:, 6] = -1.0 / mass
+    pts[:, 7] = 1.0
+    oracle.isr_chicane(cst, pts, refpt, tau, length, v, mass, np.array([0.0, 1.0]))
+    # in gamma beta; beta^2 = 1 - 2.5e-7 sets the tolerance of the
+    # ultra-relativistic forms
+    gb0 = np.sqrt(gam0 ** 2 - 1.0)
+    gbm = np.sqrt((gam0 - pts[0, 5]) ** 2 - 1.0)
+    gb1 = np.sqrt((gam0 - pts[1, 5]) ** 2 - 1.0)
+    larmor = 2.0 / 3.0 * re * gam0 ** 4 * tau / rho ** 2
+    assert gb0 - gbm == pytest.approx(larmor, rel=2e-6)
+    var = 55.0 / (24.0 * np.sqrt(3.0)) * re * hbar / (me * c) * gam0 ** 7 * tau / rho ** 3
+    assert (gbm - gb1) ** 2 == pytest.approx(var, rel=4e-6)
+    # transverse momenta shrink with the momentum: zero stays zero
+    assert np.all(pts[:, [1, 3]] == 0.0)
