@@ -151,6 +151,9 @@ static void bunch_free_all(izh_bunch *b) {
   dfree(b->cursor);
   dfree(b->chunks);
   dfree(b->d_moments);
+  dfree(b->d_chg);
+  if (b->h_chg) (void)hipHostFree(b->h_chg);
+  b->h_chg = nullptr;
   b->ntiles_cap = b->nchunks_cap = b->moments_cap = 0;
   b->cap = 0;
 }
@@ -176,10 +179,15 @@ static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
   b->wabs_sum = 0.0;
   b->w_uniform = 1;
   b->w_value = n > 0 ? pts9[7] : 0.0;
+  b->qm_uniform = 1;
+  b->qm_value = n > 0 ? pts9[6] : 0.0;
   double *dst[9] = {b->r[0], b->r[1], b->r[2], b->r[3], b->r[4], b->r[5],
                     b->qm0, b->w0, b->id0};
   for (int r = 0; r < 9; r++) {
     for (int64_t i = 0; i < n; i++) row[i] = pts9[9 * i + r];
+    if (r == 6)
+      for (int64_t i = 0; i < n; i++)
+        if (row[i] != b->qm_value) b->qm_uniform = 0;
     if (r == 7)
       for (int64_t i = 0; i < n; i++) {
         b->wabs_sum += std::fabs(row[i]);
@@ -1354,6 +1362,27 @@ int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
   if (amax) std::memcpy(amax, m + 37, 6 * sizeof(double));
   if (r2max) *r2max = m[43];
   return IZH_OK;
+}
+
+int izh_chgupdate(izh_bunch *b, int nchge, const double *qmcclist, int64_t *counts) {
+  if (!b || !qmcclist || !counts || nchge < 1 || nchge > kMaxChargeStates)
+    return fail(IZH_ERR_ARG, "izh_chgupdate: bad argument (nchge %d, at most %d)", nchge,
+                kMaxChargeStates);
+  ChgList cl{};
+  cl.n = nchge;
+  cl.eps = (double)1.0e-4f;  // eps = 1.0e-4, a REAL(4) literal (BeamBunch.f90:700)
+  for (int j = 0; j < nchge; j++) {
+    if (qmcclist[j] == 0.0)
+      return fail(IZH_ERR_ARG, "izh_chgupdate: charge state %d has q/mc^2 = 0", j + 1);
+    cl.qmcc[j] = qmcclist[j];
+  }
+  if (b->qm_uniform || b->n == 0) {
+    // every live particle has the same q/mc^2: the counts are n or 0
+    for (int j = 0; j < nchge; j++)
+      counts[j] = std::fabs((b->qm_value - cl.qmcc[j]) / cl.qmcc[j]) < cl.eps ? b->n : 0;
+    return IZH_OK;
+  }
+  return launch_chg_count(b, cl, counts);
 }
 
 int izh_field_download(izh_field *f, int which, double *host) {

@@ -132,6 +132,9 @@ struct izh_bunch {
   double *qm0 = nullptr, *w0 = nullptr, *id0 = nullptr;
   int w_uniform = 0;
   double w_value = 0;
+  int qm_uniform = 0;    // every particle has q/mc^2 == qm_value
+  double qm_value = 0;
+  unsigned long long *d_chg = nullptr, *h_chg = nullptr;  // charge-state counts
   // reduction scratch
   double *d_part = nullptr;     // kMaxRedBlocks * kPartStride
   double *d_out = nullptr;      // final reductions (kPartStride + 8)
@@ -435,6 +438,15 @@ int launch_drift2(izh_bunch *b, double tau, double gamma0, double beta0);
 int launch_kick2(izh_bunch *b, izh_field *f, const Kick2 &k, bool space_charge);
 
 int launch_map_linear(izh_bunch *b, const double xm[36]);
+// Charge states of chgupdate_BeamBunch: their q/mc^2 and the relative
+// tolerance a particle's value must be within.
+constexpr int kMaxChargeStates = 100;   // the reference's list length
+struct ChgList {
+  int n;
+  double eps;
+  double qmcc[kMaxChargeStates];
+};
+int launch_chg_count(izh_bunch *b, const ChgList &cl, int64_t *out);
 int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
                    int64_t *nlost);
 int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep);

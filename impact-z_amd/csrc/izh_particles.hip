@@ -50,6 +50,10 @@ int launch_reduce(hipStream_t s, const double *part, int nparts, int nq,
 
 namespace {
 
+__global__ void k_zero_u64_small(unsigned long long *__restrict__ p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 // ------------------------------------------------------------------- push
 __global__ void __launch_bounds__(kBlock)
 k_push(double *__restrict__ x, double *__restrict__ px, double *__restrict__ y,
@@ -725,6 +729,48 @@ int launch_kick2(izh_bunch *b, izh_field *f, const Kick2 &k, bool space_charge) 
                        f->ny, f->nz, k);
   }
   IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+namespace {
+// chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): live particles per
+// charge state, a particle counting for state j when its q/mc^2 lies within
+// relative eps of qmcc[j].  Counts are integers, so the atomics' order does
+// not matter.
+__global__ void __launch_bounds__(kBlock)
+k_chg_count(const uint32_t *__restrict__ orig, const double *__restrict__ qm0, int64_t n,
+            ChgList cl, unsigned long long *__restrict__ counts) {
+  __shared__ unsigned int lc[kMaxChargeStates];
+  for (int j = threadIdx.x; j < cl.n; j += blockDim.x) lc[j] = 0u;
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double q = qm0[orig[i]];
+    for (int j = 0; j < cl.n; j++)
+      if (fabs((q - cl.qmcc[j]) / cl.qmcc[j]) < cl.eps) atomicAdd(&lc[j], 1u);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < cl.n; j += blockDim.x)
+    if (lc[j]) atomicAdd(&counts[j], (unsigned long long)lc[j]);
+}
+}  // namespace
+
+int launch_chg_count(izh_bunch *b, const ChgList &cl, int64_t *out) {
+  if (!b->d_chg) {
+    IZH_HIP(hipMalloc((void **)&b->d_chg, kMaxChargeStates * sizeof(unsigned long long)));
+    IZH_HIP(hipHostMalloc((void **)&b->h_chg, kMaxChargeStates * sizeof(unsigned long long),
+                          hipHostMallocDefault));
+  }
+  hipLaunchKernelGGL(k_zero_u64_small, dim3(1), dim3(kMaxChargeStates), 0, b->stream,
+                     b->d_chg, cl.n);
+  if (b->n > 0)
+    hipLaunchKernelGGL(k_chg_count, dim3(grid_for(b->n)), dim3(kBlock), 0, b->stream,
+                       b->orig, b->qm0, b->n, cl, b->d_chg);
+  IZH_HIP(hipGetLastError());
+  IZH_HIP(hipMemcpyAsync(b->h_chg, b->d_chg, cl.n * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, b->stream));
+  IZH_HIP(hipStreamSynchronize(b->stream));
+  for (int j = 0; j < cl.n; j++) out[j] = (int64_t)b->h_chg[j];
   return IZH_OK;
 }
 

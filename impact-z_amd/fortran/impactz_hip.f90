@@ -358,6 +358,15 @@ module impactz_hip
       real(c_double) :: sums(37), amax(6), r2max
       integer(c_int) :: rc
     end function
+    function izh_chgupdate(b, nchge, qmcclist, counts) &
+        bind(C, name="izh_chgupdate") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nchge
+      real(c_double) :: qmcclist(*)
+      integer(c_int64_t) :: counts(*)
+      integer(c_int) :: rc
+    end function
     function izh_field_download(f, which, host) &
         bind(C, name="izh_field_download") result(rc)
       import :: c_int, c_double, c_ptr
@@ -553,5 +562,17 @@ contains
     real(c_double), intent(out) :: sums(37), amax(6), r2max
     call izh_check(izh_diagnostic1(b, sums, amax, r2max), 'diagnostic1')
   end subroutine diagnostic1_sums_gpu
+
+  ! chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): nptlist(j) = live
+  ! particles of charge state j
+  subroutine chgupdate_gpu(b, nchge, qmcclist, nptlist)
+    type(c_ptr), intent(in) :: b
+    integer, intent(in) :: nchge
+    real(c_double), intent(in) :: qmcclist(nchge)
+    integer, intent(out) :: nptlist(nchge)
+    integer(c_int64_t) :: counts(nchge)
+    call izh_check(izh_chgupdate(b, int(nchge, c_int), qmcclist, counts), 'chgupdate')
+    nptlist = int(counts)
+  end subroutine chgupdate_gpu
 
 end module impactz_hip

@@ -17,9 +17,9 @@
 ! (rows z Ez Ez' Ez'', bitypes 101-104) or read3_Data (one coefficient per
 ! line, SolRF) do (src/DataStruct/Data.f90:139-215,348-409).  Output:
 ! one line per step with z, Npt, the 44 diagnostic1 reductions (37 sums,
-! 6 absolute maxima, max x^2+y^2) and the reference particle's gamma
+! 6 absolute maxima, max x^2+y^2), the reference particle's gamma
 ! (-refptcl(6), which the Lorentz integrator changes in a longitudinal field)
-! in the file named by the namelist.
+! and its phase refptcl(5), in the file named by the namelist.
 !----------------------------------------------------------------------------
 program izh_lattice
   use iso_c_binding
@@ -122,8 +122,8 @@ program izh_lattice
         call diagnostic1_sums_gpu(b, sums, amax, r2max)
         call izh_check(izh_bunch_get_info(b, info), 'bunch_get_info')
         istep = istep + 1
-        write(u, '(I8,1X,ES25.17,1X,I12,45(1X,ES25.17))') istep, z, info%npt, &
-            sums, amax, r2max, -info%refpt(6)
+        write(u, '(I8,1X,ES25.17,1X,I12,46(1X,ES25.17))') istep, z, info%npt, &
+            sums, amax, r2max, -info%refpt(6), info%refpt(5)
       end do
     end do
   else
@@ -152,8 +152,8 @@ program izh_lattice
       do j = 1, nseg(i)
         k = k + 1
         z = z + elen(i) / nseg(i)
-        write(u, '(I8,1X,ES25.17,1X,I12,45(1X,ES25.17))') k, z, &
-            int(recs(45, k), c_int64_t), recs(1:44, k), -recs(47, k)
+        write(u, '(I8,1X,ES25.17,1X,I12,46(1X,ES25.17))') k, z, &
+            int(recs(45, k), c_int64_t), recs(1:44, k), -recs(47, k), recs(46, k)
       end do
     end do
   end if

@@ -211,25 +211,30 @@ def write_particle_in(path: str, pts: np.ndarray) -> None:
 
 def write_deck(path: str, n: int, grid: tuple[int, int, int], beam: Beam,
                lattice: list[Element], flagdiag: int = 1, rstartflg: int = 0,
-               flagdist: int = 19, distparam=None, flagmap: int = 1) -> None:
+               flagdist: int = 19, distparam=None, flagmap: int = 1,
+               charge_states=None) -> None:
     """An ``ImpactZ.in`` for Flagbc=1 (src/Contrl/Input.f90:62-314) and the
     integrator ``flagmap`` (1 linear map, 2 nonlinear Lorentz): particles
     from particle.in (Flagdist=19) or a binary fort.N (199, the file number in
-    distparam[1]); rstartflg=1 restarts from a -7 checkpoint."""
+    distparam[1]); rstartflg=1 restarts from a -7 checkpoint.
+    charge_states: (particles, current, q/mc^2) of each charge state, the
+    deck's nchrg, nptlist, currlist and qmcclist (default: one state, the
+    beam's)."""
     nx, ny, nz = grid
     dp = list(distparam) if distparam is not None else [
         0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
         0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
         0.1, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0]
-    qm = beam.charge / beam.mass
+    cs = list(charge_states) if charge_states else [(n, beam.current,
+                                                     beam.charge / beam.mass)]
     lines = [
         "1 1",
         f"6 {n} {flagmap} 0 {flagdiag}",
         f"{nx} {ny} {nz} 1 1.0 1.0 1.0",
-        f"{flagdist} {rstartflg} 0 1",
-        f"{n}",
-        f"{beam.current!r}",
-        f"{qm!r}",
+        f"{flagdist} {rstartflg} 0 {len(cs)}",
+        " ".join(str(int(c[0])) for c in cs),
+        " ".join(repr(float(c[1])) for c in cs),
+        " ".join(repr(float(c[2])) for c in cs),
         " ".join(repr(float(v)) for v in dp[0:7]),
         " ".join(repr(float(v)) for v in dp[7:14]),
         " ".join(repr(float(v)) for v in dp[14:21]),

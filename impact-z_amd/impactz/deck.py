@@ -206,8 +206,8 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
         raise DeckError(f"Flagbc={deck.flagbc}: only open boundaries (Flagbc=1)")
     if deck.flagsbstp != 0:
         raise DeckError("Flagsbstp != 0 (sub-cycling) is not on this path")
-    if deck.nchrg != 1:
-        raise DeckError(f"{deck.nchrg} charge states: one is supported")
+    if not 1 <= deck.nchrg <= 100:
+        raise DeckError(f"{deck.nchrg} charge states: the reference holds 1 to 100")
     out, notes = [], []
     for k, e in enumerate(deck.elements, start=1):
         v = e.values + [0.0] * 4
@@ -305,20 +305,31 @@ def read_particle_in(path: str) -> np.ndarray:
     return vals.reshape(n, 9)
 
 
-FORT_FILES = (18, 24, 25, 26, 27, 29, 30)
+FORT_FILES = (18, 24, 25, 26, 27, 28, 29, 30, 32)
 
 
 def write_fort(outdir: str, rows: np.ndarray) -> None:
-    """fort.18/24-27/29/30 of diagnostic1_Output from diagnostic rows
-    (z, then the 41 values of impactz.sim.diag_row)."""
+    """The files of diagnostic1_Output (src/Contrl/Output.f90:330-352) from
+    diagnostic rows: z, the 41 values of impactz.sim.diag_row, then Nptlocal,
+    Npt and the particle count of each charge state.  fort.18/24-27/29/30
+    hold the reals; fort.28 z, the smallest and largest local count and Npt
+    (one process: both local counts are Nptlocal); fort.32 z and the counts
+    per charge state."""
     rows = np.asarray(rows, dtype=np.float64)
     cols = {18: slice(1, 6), 24: slice(6, 12), 25: slice(12, 18), 26: slice(18, 24),
             27: slice(24, 30), 29: slice(30, 36), 30: slice(36, 42)}
     for unit in FORT_FILES:
         with open(os.path.join(outdir, f"fort.{unit}"), "w") as f:
             for r in rows:
-                vals = [r[0], *r[cols[unit]]]
-                f.write(" ".join(repr(float(x)) for x in vals) + "\n")
+                if unit == 28:
+                    n = [int(round(v)) for v in r[42:44]]
+                    f.write(f"{float(r[0])!r} {n[0]:10d}{n[0]:10d}{n[1]:10d}\n")
+                elif unit == 32:
+                    f.write(" ".join([repr(float(r[0]))] +
+                                     [str(int(round(v))) for v in r[44:]]) + "\n")
+                else:
+                    vals = [r[0], *r[cols[unit]]]
+                    f.write(" ".join(repr(float(x)) for x in vals) + "\n")
 
 
 def read_rfdata(path: str, per_record: int) -> np.ndarray:
@@ -414,14 +425,18 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
         pts = initial_particles(deck, workdir)
     elements = [p for _, p in prog if isinstance(p, Element)]
     sim = Simulation(pts, deck.beam(), elements, deck.grid, precision=precision,
-                     flagmap=deck.flagmap)
+                     flagmap=deck.flagmap, qmcclist=deck.qmcclist, nptlist=deck.nptlist)
+
+    def counts():
+        info = sim.bunch.info()
+        return [info.nptlocal, info.npt, *sim.nptlist]
     try:
         if ck is not None:
             # inpoint_Output: the particles, refptcl and z of the checkpoint;
             # Current stays the deck's and Npt is the particle count
             sim.bunch.set_refpt(ck.refpt)
             sim.z = ck.z
-        rows = [np.concatenate([[sim.z], sim.diag()])]
+        rows = [np.concatenate([[sim.z], sim.diag(), counts()])]
         sc = deck.bcurr >= float(np.float32(1.0e-10))
         budget = nsteps
         pending = []   # (element, tau1) of segments not yet run
@@ -432,16 +447,22 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
             # stretches of linear-map steps fuse; bends run step by step
             for bend, grp in itertools.groupby(pending, key=lambda p: p[0].type == 4):
                 grp = list(grp)
-                if fused and sim.flagmap == 1 and not bend:
+                # a charge-state count of a bunch of several q/mc^2 scans the
+                # particles after each step, which the fused run does not stop for
+                if fused and sim.flagmap == 1 and not bend and not sim.chg_needs_scan:
                     z = sim.z
                     recs = sim.run_elements([e for e, _ in grp])
                     for (e, tau1), rec in zip(grp, recs):
                         z = (z + tau1) + tau1  # drift1_BeamBunch advances z by tau1 twice
-                        rows.append(np.concatenate([[z], sim.diag(rec)]))
+                        npt = int(rec[44])
+                        chg = sim.update_chg(npt)
+                        rows.append(np.concatenate([[z], sim.diag(rec), [npt, npt], chg]))
                 else:
                     for e, _ in grp:
                         mom = sim.step(e)
-                        rows.append(np.concatenate([[sim.z], sim.diag(mom)]))
+                        if e.type != 4:
+                            sim.update_chg()
+                        rows.append(np.concatenate([[sim.z], sim.diag(mom), counts()]))
             pending.clear()
 
         for index, p in prog:

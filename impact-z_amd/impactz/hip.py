@@ -36,7 +36,7 @@ SYMBOLS = (
     "izh_map1", "izh_fpol_dipole", "izh_sector_dipole", "izh_bpol_dipole",
     "izh_isr_chicane", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
     "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
-    "izh_wake_download", "izh_kick2wake",
+    "izh_wake_download", "izh_kick2wake", "izh_chgupdate",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -197,6 +197,7 @@ def lib() -> C.CDLL:
     L.izh_cvbkforth1st.argtypes = [vp]
     L.izh_kick1.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
     L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
+    L.izh_chgupdate.argtypes = [vp, C.c_int, _dp, _i64p]
     L.izh_field_download.argtypes = [vp, C.c_int, _dp]
     L.izh_field_upload_rho.argtypes = [vp, _dp]
     L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
@@ -551,6 +552,14 @@ class BeamBunch:
         s, m, r2 = np.zeros(37), np.zeros(6), C.c_double()
         check(lib().izh_diagnostic1(self.h, _p(s), _p(m), C.byref(r2)))
         return s, m, r2.value
+
+    def chgupdate(self, qmcclist) -> np.ndarray:
+        """chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): live particles
+        of each charge state in qmcclist (q/mc^2 within relative 1e-4)."""
+        q = np.ascontiguousarray(qmcclist, dtype=np.float64)
+        out = (C.c_int64 * len(q))()
+        check(lib().izh_chgupdate(self.h, len(q), _p(q), out))
+        return np.array(out[:], dtype=np.int64)
 
     def step(self, field: FieldQuant, dom: CompDom, z: float, seg: Segment,
              bcurr: float, want_moments: bool = False):

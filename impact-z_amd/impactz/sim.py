@@ -61,8 +61,19 @@ class Simulation:
     """A device-resident bunch + field marched through a lattice."""
 
     def __init__(self, pts: np.ndarray, beam: Beam, lattice: list[Element],
-                 grid: tuple[int, int, int], precision: int = hip.FP64, flagmap: int = 1):
+                 grid: tuple[int, int, int], precision: int = hip.FP64, flagmap: int = 1,
+                 qmcclist=None, nptlist=None):
         self.beam = beam
+        # charge states of chgupdate_BeamBunch: their q/mc^2 and particle
+        # counts, which start as the deck's nptlist (src/Contrl/
+        # AccSimulator.f90:155-161) and are recounted after every conv1st
+        self.qmcclist = np.asarray(qmcclist if qmcclist is not None
+                                   else [beam.charge / beam.mass], dtype=np.float64)
+        self.nptlist = np.asarray(nptlist if nptlist is not None else [pts.shape[0]],
+                                  dtype=np.int64)
+        qm = np.asarray(pts)[:, 6] if len(pts) else np.zeros(0)
+        # one q/mc^2 for every particle: the counts follow Npt without a scan
+        self._qm_uniform = None if qm.size == 0 or np.any(qm != qm[0]) else float(qm[0])
         self.flagmap = flagmap
         self._left = 0      # segments left in the current element
         self.zedge = 0.0    # z of the current element's entrance
@@ -286,6 +297,30 @@ class Simulation:
                                      want_moments)
         self.nstep += len(elements)
         return mom
+
+    def chg_counts(self, npt: int | None = None) -> np.ndarray:
+        """chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728) after a
+        conv1st that left npt particles: with one q/mc^2 for all particles
+        each state holds all of them or none; otherwise the device counts."""
+        if self._qm_uniform is not None or (npt == 0):
+            n = self.bunch.info().npt if npt is None else npt
+            q = self._qm_uniform if self._qm_uniform is not None else 0.0
+            eps = float(np.float32(1.0e-4))
+            return np.array([n if abs((q - c) / c) < eps else 0 for c in self.qmcclist],
+                            dtype=np.int64)
+        return self.bunch.chgupdate(self.qmcclist)
+
+    @property
+    def chg_needs_scan(self) -> bool:
+        """Charge-state counts need a pass over the particles (q/mc^2 varies)."""
+        return self._qm_uniform is None
+
+    def update_chg(self, npt: int | None = None) -> np.ndarray:
+        """The counts after a space-charge step (conv1st runs only with
+        current, src/Contrl/AccSimulator.f90:1124-1130)."""
+        if self.beam.current >= float(np.float32(1.0e-10)):
+            self.nptlist = self.chg_counts(npt)
+        return self.nptlist
 
     def diag(self, mom=None, refpt=None):
         """diagnostic1_Output row of the current state, or of a moments record

@@ -228,6 +228,13 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
 int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
                     double *r2max);
 
+/* chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): counts[j] = live
+ * particles of charge state j, a particle counting for state j when
+ * |(Pts1(7) - qmcclist[j]) / qmcclist[j]| < 1.0e-4 (the reference's REAL(4)
+ * literal); 1 <= nchge <= 100.  The reference keeps these as nptlist for
+ * diagnostic1_Output's fort.32. */
+int izh_chgupdate(izh_bunch *b, int nchge, const double *qmcclist, int64_t *counts);
+
 /* Parity access to the field (host arrays of nx*ny*nz doubles). */
 int izh_field_download(izh_field *f, int which, double *host);
 int izh_field_upload_rho(izh_field *f, const double *rho);

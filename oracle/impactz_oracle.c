@@ -1587,6 +1587,20 @@ void or_diag_row(const or_const *c, const double t[37], const double gm[6],
   memcpy(out, v, sizeof v);
 }
 
+/* chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): particles per charge
+ * state, a particle counting for state j when |(Pts1(7)-qmcc(j))/qmcc(j)| is
+ * below the REAL(4) literal 1.0e-4 (:700). */
+void or_chgupdate(const double *pts, int64_t n, int nchge, const double *qmcc,
+                  int64_t *counts) {
+  const double eps = (double)1.0e-4f;
+  for (int j = 0; j < nchge; j++) counts[j] = 0;
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < nchge; j++) {
+      const double dd = fabs((pts[9 * i + 6] - qmcc[j]) / qmcc[j]);
+      if (dd < eps) counts[j]++;
+    }
+}
+
 /* ------------------------------------------------------------ one step */
 
 int64_t or_step(const or_const *c, double *pts, int64_t n, or_bunch_state *st,

@@ -184,6 +184,9 @@ void or_diag_sums(const double *pts, int64_t n, double sums[37], double amax[6],
 void or_diag_row(const or_const *c, const double sums[37], const double amax[6],
                  double r2max, int64_t npt, const double refpt[6], double mass,
                  double out[41]);
+/* chgupdate_BeamBunch: particles of each charge state (q/mc^2 within 1e-4). */
+void or_chgupdate(const double *pts, int64_t n, int nchge, const double *qmcc,
+                  int64_t *counts);
 
 /* A wakefield (src/Contrl/AccSimulator.f90:859-875,960-980): kind 1 the
  * analytic short-range wake of a SolRF/TWS element with iris radius aa, gap

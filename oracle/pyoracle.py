@@ -184,6 +184,8 @@ def lib() -> C.CDLL:
                                C.c_int, C.c_int, _dp, _dp, _dp, _dp,
                                C.c_double, C.c_double, C.c_double, C.c_double]
         L.or_diag_sums.argtypes = [_dp, C.c_int64, _dp, _dp, _dp]
+        L.or_chgupdate.argtypes = [_dp, C.c_int64, C.c_int, _dp,
+                                   C.POINTER(C.c_int64)]
         L.or_diag_row.argtypes = [C.POINTER(OrConst), _dp, _dp, C.c_double,
                                   C.c_int64, _dp, C.c_double, _dp]
         L.or_step.argtypes = [C.POINTER(OrConst), _dp, C.c_int64,
@@ -399,6 +401,15 @@ def diag_sums(pts):
     s, m, r2 = np.zeros(37), np.zeros(6), C.c_double(0)
     lib().or_diag_sums(_p(pts), pts.shape[0], _p(s), _p(m), C.byref(r2))
     return s, m, r2.value
+
+
+def chgupdate(pts, qmcclist):
+    """chgupdate_BeamBunch: particles of each charge state."""
+    q = np.ascontiguousarray(qmcclist, dtype=np.float64)
+    out = (C.c_int64 * len(q))()
+    p = np.ascontiguousarray(pts, dtype=np.float64)
+    lib().or_chgupdate(_p(p), p.shape[0], len(q), _p(q), out)
+    return np.array(out[:], dtype=np.int64)
 
 
 def diag_row(c, sums, amax, r2max, npt, refpt, mass):

@@ -97,6 +97,84 @@ def diag_row_err(got, ref, ref_is_fort=False):
     return worst
 
 
+# Where each fort file's columns (after z) sit in a 41-value diagnostic row:
+# fort.18 refpt(5), gamma, energy, beta, r_max; fort.24/25/26 centroid, rms,
+# momentum centroid, momentum rms, alpha, emittance; fort.27 the six maxima;
+# fort.29 the cube roots of the third central moments; fort.30 the fourth
+# roots of the fourth (src/Contrl/Output.f90:330-341).
+FORT_COLS = ((18, 0, 5), (24, 5, 6), (25, 11, 6), (26, 17, 6), (27, 23, 6), (29, 29, 6),
+             (30, 35, 6))
+
+
+def fort_row(d, k, prefix=""):
+    """The 41-value diagnostic row k (0 = before the first step) assembled
+    from a fixture's fort.18/24-27/29/30 arrays."""
+    out = np.empty(41)
+    for unit, lo, n in FORT_COLS:
+        out[lo:lo + n] = d[f"{prefix}fort{unit}"][k, 1:1 + n]
+    return out
+
+
+def has_full_fort(d, prefix=""):
+    return all(f"{prefix}fort{u}" in d for u, _, _ in FORT_COLS)
+
+
+def full_row_err(got, ref):
+    """Worst error over all 41 values of a diagnostic row.
+
+    fort.24-26 as diag_row_err; every other value (fort.18's reference phase,
+    gamma, energy, beta and r_max, the fort.27 maxima, the fort.29 cube roots
+    and the fort.30 fourth roots) is normalised by its own magnitude, so a
+    wrong exponent, sum index or scale factor shows at its full size.
+    """
+    g = np.asarray(got, dtype=np.float64)[:41]
+    r = np.asarray(ref, dtype=np.float64)[:41]
+    worst = diag_row_err(g, r)
+    idx = np.r_[0:5, 23:41]
+    scale = np.abs(r[idx])
+    scale[scale == 0] = 1.0
+    return max(worst, float(np.max(np.abs(g[idx] - r[idx]) / scale)))
+
+
+def fixture_row_err(got, d, k, prefix=""):
+    """Error of a 41-value row against row k of a fixture: every column when
+    the fixture holds all the diagnostic files, else fort.18 and 24-26."""
+    got = np.asarray(got, dtype=np.float64)
+    if has_full_fort(d, prefix):
+        return full_row_err(got, fort_row(d, k, prefix))
+    ref = np.concatenate([d[f"{prefix}fort{u}"][k, 1:] for u in (24, 25, 26)])
+    r18 = d[f"{prefix}fort18"][k, 1:6]
+    s18 = np.abs(r18)
+    s18[s18 == 0] = 1.0
+    return max(diag_row_err(got, ref, ref_is_fort=True),
+               float(np.max(np.abs(got[:5] - r18) / s18)))
+
+
+def deck_files_err(outdir, d, prefix=""):
+    """Worst error of the diagnostic files a deck run wrote into outdir
+    against a fixture's, every row and column: the z column absolutely, the
+    41 values as fixture_row_err; fort.28 and fort.32 (where the fixture has
+    them) must match exactly."""
+    units = [u for u, _, _ in FORT_COLS if f"{prefix}fort{u}" in d]
+    f = {u: np.loadtxt(os.path.join(outdir, f"fort.{u}"), ndmin=2) for u in units}
+    nrows = d[f"{prefix}fort24"].shape[0]
+    for u in units:
+        assert f[u].shape == d[f"{prefix}fort{u}"].shape, (u, f[u].shape)
+        assert np.max(np.abs(f[u][:, 0] - d[f"{prefix}fort{u}"][:, 0])) < 1e-12, u
+    worst = 0.0
+    for k in range(nrows):
+        got = np.zeros(41)
+        for u, lo, n in FORT_COLS:
+            if u in f:
+                got[lo:lo + n] = f[u][k, 1:1 + n]
+        worst = max(worst, fixture_row_err(got, d, k, prefix))
+    for u in (28, 32):
+        if f"{prefix}fort{u}" in d:
+            mine = np.loadtxt(os.path.join(outdir, f"fort.{u}"), ndmin=2)
+            assert np.array_equal(mine[:, 1:], d[f"{prefix}fort{u}"][:, 1:]), u
+    return worst
+
+
 @pytest.fixture(scope="session")
 def oracle():
     import pyoracle

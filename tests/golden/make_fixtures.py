@@ -29,6 +29,8 @@ from impactz import beam as B  # noqa: E402
 from tests.golden import refdump as R  # noqa: E402
 
 STAGES = ("p0", "p_map1a", "p_conv1st", "p_cvbk", "p_kick", "p_map1b")
+# every file diagnostic1_Output writes (src/Contrl/Output.f90:330-352)
+EXE_UNITS = (18, 24, 25, 26, 27, 28, 29, 30, 32)
 
 
 def lattice_array(lattice):
@@ -143,14 +145,14 @@ def io_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
     with open(os.path.join(a, "fort.50"), "rb") as f:
         out["ckpt50"] = np.frombuffer(f.read(), dtype=np.uint8)
     out["phase60"] = np.loadtxt(os.path.join(a, "fort.60"), ndmin=2)
-    for k in (18, 24, 25, 26):
+    for k in EXE_UNITS:
         out[f"a_fort{k}"] = R.read_fort(os.path.join(a, f"fort.{k}"))
     b = os.path.join(tmp, "b")
     os.makedirs(b)
     shutil.copy(os.path.join(a, "fort.50"), os.path.join(b, "fort.50"))
     B.write_deck(os.path.join(b, "ImpactZ.in"), n, grid, beam, lat, rstartflg=1)
     _run_exe(b)
-    for k in (18, 24, 25, 26):
+    for k in EXE_UNITS:
         out[f"b_fort{k}"] = R.read_fort(os.path.join(b, f"fort.{k}"))
     c = os.path.join(tmp, "c")
     os.makedirs(c)
@@ -162,7 +164,7 @@ def io_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
                  distparam=dp)
     _run_exe(c)
     out["c_distparam"] = np.array(dp)
-    for k in (18, 24, 25, 26):
+    for k in EXE_UNITS:
         out[f"c_fort{k}"] = R.read_fort(os.path.join(c, f"fort.{k}"))
     shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, "io_ckpt.npz")
@@ -257,7 +259,7 @@ def exe_wake_fixture(n=1000, grid=(16, 16, 32), beam=B.Beam()):
     for i, e in enumerate(lat):
         if e.field is not None:
             out[f"field{i}"] = np.asarray(e.field, dtype=np.float64)
-    for k in (18, 24, 25, 26):
+    for k in EXE_UNITS:
         out[f"fort{k}"] = R.read_fort(os.path.join(tmp, f"fort.{k}"))
     shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, "exe_wake.npz")
@@ -331,7 +333,7 @@ def example1_fixture():
            "lattice": lattice_array(lat)}
     for i, f in fields.items():
         out[f"field{i}"] = f
-    for k in (18, 24, 25, 26):
+    for k in EXE_UNITS:
         out[f"fort{k}"] = R.read_fort(os.path.join(tmp, f"fort.{k}"))
     shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, "example1.npz")
@@ -371,7 +373,7 @@ def exe_nonlinear_fixture(n=1000, grid=(16, 16, 16), beam=B.Beam()):
         out[f"{tag}_flagmap"] = np.array(flagmap)
         out[f"{tag}_lattice"] = np.array([[e.length, e.nseg, e.mapstp, e.type, e.v1, e.v2,
                                            e.v3, e.v4] for e in lat])
-        for k in (18, 24, 25, 26):
+        for k in EXE_UNITS:
             out[f"{tag}_fort{k}"] = R.read_fort(os.path.join(d, f"fort.{k}"))
     shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, "exe_nonlinear.npz")
@@ -415,7 +417,7 @@ def exe_bend_fixture(n=1000, grid=(16, 16, 16)):
         B.write_deck(os.path.join(d, "ImpactZ.in"), n, grid, beam, lat, flagmap=flagmap)
         _run_exe(d)
         out[f"m{flagmap}_phase60"] = np.loadtxt(os.path.join(d, "fort.60"), ndmin=2)
-        for k in (18, 24, 25, 26):
+        for k in EXE_UNITS:
             out[f"m{flagmap}_fort{k}"] = R.read_fort(os.path.join(d, f"fort.{k}"))
     shutil.rmtree(tmp, ignore_errors=True)
     path = os.path.join(HERE, "exe_bend.npz")
@@ -424,7 +426,50 @@ def exe_bend_fixture(n=1000, grid=(16, 16, 16)):
     return out
 
 
-def main():
+def exe_chg_fixture(n=2000, grid=(16, 16, 16)):
+    """Two charge states through the reference executable (ImpactZexe built
+    from source): half the particles with the beam's q/mc^2 and charge weight
+    for 0.06 A, half with half its q/mc^2 and weights for 0.04 A, through two
+    FODO periods in a 2.2 mm pipe, so conv1st loses particles of both states
+    and chgupdate_BeamBunch recounts them.  Keeps every diagnostic file,
+    fort.28 (particle counts) and fort.32 (counts per charge state) too."""
+    beam = B.Beam()
+    pts = B.gaussian_bunch(n, seed=13, beam=beam)
+    na = n // 2
+    qa = beam.charge / beam.mass
+    cs = [(na, 0.06, qa), (n - na, 0.04, 0.5 * qa)]
+    pts[:na, 6], pts[na:, 6] = cs[0][2], cs[1][2]
+    pts[:na, 7] = cs[0][1] / beam.freq / na
+    pts[na:, 7] = cs[1][1] / beam.freq / (n - na)
+    lat = B.fodo(2, radius=0.0022)
+    tmp = tempfile.mkdtemp(prefix="impz_chg_")
+    B.write_particle_in(os.path.join(tmp, "particle.in"), pts)
+    B.write_deck(os.path.join(tmp, "ImpactZ.in"), n, grid, beam, lat, charge_states=cs)
+    _run_exe(tmp)
+    out = {"pts_in": pts, "grid": np.array(grid),
+           "beam": np.array([beam.current, beam.kinetic, beam.mass, beam.charge,
+                             beam.freq, beam.phase0]),
+           "lattice": lattice_array(lat), "charge_states": np.array(cs)}
+    for k in EXE_UNITS:
+        out[f"fort{k}"] = R.read_fort(os.path.join(tmp, f"fort.{k}"))
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "exe_chg.npz")
+    np.savez_compressed(path, **out)
+    print(f"exe_chg: {os.path.getsize(path) / 1024:.0f} KiB")
+    return out
+
+
+def main(only=None):
+    """Write every fixture, or only the named ones (by file name without
+    .npz; the executable runs io_ckpt, example1, exe_wake, exe_nonlinear,
+    exe_bend and exe_chg by their function)."""
+    if only:
+        table = {"io_ckpt": io_fixture, "example1": example1_fixture,
+                 "exe_wake": exe_wake_fixture, "exe_nonlinear": exe_nonlinear_fixture,
+                 "exe_bend": exe_bend_fixture, "exe_chg": exe_chg_fixture}
+        for name in only:
+            table[name]()
+        return
     if not R.harness_available():
         raise SystemExit("oracle/_ref/ref_harness missing: make -f oracle/Makefile.ref")
     meta = {}
@@ -489,6 +534,10 @@ def main():
     exe_bend_fixture()
     meta["exe_bend"] = ("ImpactZexe runs of two bends at zero current (pole faces, "
                         "fringe, k1) under Flagmap=1 and 2: fort.18/24-26 and the -2 dump")
+    exe_chg_fixture()
+    meta["exe_chg"] = ("ImpactZexe run of two charge states (q/mc^2 and half of it, "
+                       "0.06 A and 0.04 A) through two FODO periods in a 2.2 mm pipe: "
+                       "every diagnostic file, fort.28 and fort.32 included")
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "
                              "a solenoid, and a Flagmap=2 FODO period")
@@ -497,4 +546,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

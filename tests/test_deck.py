@@ -123,7 +123,7 @@ def test_particle_in_round_trip_and_fortran_exponents(tmp_path):
 
 def test_fort_writer_columns(tmp_path):
     from impactz.deck import write_fort
-    rows = np.arange(2 * 42, dtype=np.float64).reshape(2, 42)
+    rows = np.arange(2 * 46, dtype=np.float64).reshape(2, 46)
     write_fort(str(tmp_path), rows)
     f24 = np.loadtxt(tmp_path / "fort.24")
     assert f24.shape == (2, 7)
@@ -132,6 +132,31 @@ def test_fort_writer_columns(tmp_path):
     assert np.array_equal(f18[:, 1:], rows[:, 1:6])
     f30 = np.loadtxt(tmp_path / "fort.30")
     assert np.array_equal(f30[:, 1:], rows[:, 36:42])
+    # fort.28: z, then Nptlocal twice (smallest and largest over one process)
+    # and Npt in I10 fields (src/Contrl/Output.f90:101,348)
+    lines = (tmp_path / "fort.28").read_text().splitlines()
+    z, ints = lines[1].split(" ", 1)
+    assert float(z) == rows[1, 0] and len(ints) == 30
+    assert [int(ints[10 * i:10 * i + 10]) for i in range(3)] == [88, 88, 89]
+    # fort.32: z and the count of each charge state
+    f32 = np.loadtxt(tmp_path / "fort.32")
+    assert np.array_equal(f32[:, 1:], rows[:, 44:46]) and np.array_equal(f32[:, 0], rows[:, 0])
+
+
+def test_deck_reads_charge_states(tmp_path):
+    """nchrg, nptlist, currlist and qmcclist as the deck writer writes them
+    and Input.f90:106-136 reads them."""
+    from impactz.beam import Beam, fodo, write_deck
+    from impactz.deck import program_of, read_deck
+    p = tmp_path / "ImpactZ.in"
+    q = 1.0 / 938.272088e6
+    write_deck(str(p), 300, (8, 8, 8), Beam(), fodo(1),
+               charge_states=[(100, 0.03, q), (200, 0.07, 0.5 * q)])
+    deck = read_deck(str(p))
+    assert deck.nchrg == 2 and deck.nptlist == [100, 200]
+    assert deck.currlist == [0.03, 0.07] and deck.qmcclist == [q, 0.5 * q]
+    prog, _ = program_of(deck)
+    assert len(prog) == 4
 
 
 def test_deck_linear_elements(tmp_path):

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import fixture_beam, fixture_lattice, load_golden
+from conftest import deck_files_err, fixture_beam, fixture_lattice, load_golden
 from test_gpu_parity import record
 
 pytestmark = pytest.mark.gpu
@@ -80,6 +80,8 @@ def test_bend_deck_matches_executable(tmp_path, gpu, flagmap):
         assert got.shape == ref.shape
         scale = np.abs(ref).max(axis=0) + 1e-300
         worst = max(worst, float((np.abs(got - ref) / scale).max()))
+    # every column of every diagnostic file, fort.28 and fort.32 exactly
+    worst = max(worst, deck_files_err(str(tmp_path), d, f"m{flagmap}_"))
     err = _col_err(P.read_phase(str(tmp_path / "fort.60")), d[f"m{flagmap}_phase60"])
     record(test="bend_deck", case=f"exe_bend flagmap={flagmap}", moments=worst,
            particles=err)

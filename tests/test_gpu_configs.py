@@ -1,0 +1,186 @@
+"""The benchmark configurations on the GPU.
+
+C2: 2M particles on a 64^3 grid (128^3 doubled), FP64, compared with the C
+oracle on the same seeded input for the first eight steps of the FODO
+lattice (the focusing quadrupole and the drift after it): the grid charge,
+the potential, every particle and every diagnostic column within 1e-10
+relative; the mixed form against the same oracle with its stated
+tolerances.
+
+C3: 50M particles (Gaussian core plus waterbag halo) on 128x128x256
+(256x256x512 doubled), checked by properties that hold at any size: the
+fused run equals the step-by-step run, the fixed-point grid charge is
+bitwise the same in two runs, a run split in two equals one run, and the
+mixed form stays within 1e-4 of FP64 in the rms moments after 400 steps.
+"""
+import numpy as np
+import pytest
+
+from conftest import col_err, full_row_err, rel_err
+from test_gpu_parity import TOL, record
+
+pytestmark = pytest.mark.gpu
+
+C2_N, C2_GRID, C2_STEPS = 2_000_000, (64, 64, 64), 8
+C3_N, C3_GRID = 50_000_000, (128, 128, 256)
+RMS_COLS = [6, 8, 12, 14, 18, 20]   # x, px, y, py, z, pz rms in a diagnostic row
+
+
+@pytest.fixture(scope="module")
+def c2_input():
+    from impactz import beam as B
+    return B.Beam(), B.gaussian_bunch(C2_N, seed=1), B.fodo(1)
+
+
+def test_c2_fp64_matches_oracle_stage_by_stage(gpu, oracle, c2_input):
+    from impactz.sim import Simulation
+    bm, pts, lat = c2_input
+    sim = Simulation(pts, bm, lat, C2_GRID)
+    run = oracle.Run(pts, bm, lat, C2_GRID)
+    worst = {"rho": 0.0, "phi": 0.0, "particles": 0.0, "moments": 0.0}
+    try:
+        for k, e in zip(range(C2_STEPS), sim.segments()):
+            print(f"C2 step {k + 1} type {e.type}", flush=True)
+            mom = sim.step(e)
+            rho_o, phi_o = run.step(e, want_fields=True)
+            worst["rho"] = max(worst["rho"], rel_err(sim.field.download(gpu.RHO), rho_o))
+            worst["phi"] = max(worst["phi"], rel_err(sim.field.download(gpu.PHI), phi_o))
+            worst["particles"] = max(worst["particles"],
+                                     col_err(sim.bunch.getpts(), run.particles()))
+            worst["moments"] = max(worst["moments"], full_row_err(sim.diag(mom), run.diag()))
+        record(test="c2_fp64_vs_oracle", particles=C2_N, grid=C2_GRID, steps=C2_STEPS,
+               **worst)
+        for k, v in worst.items():
+            assert v < TOL, (k, v)
+    finally:
+        sim.close()
+
+
+def test_c2_fused_run_matches_oracle(gpu, oracle, c2_input):
+    """izh_run at the C2 size against the oracle's trajectory."""
+    from impactz.sim import Simulation
+    bm, pts, lat = c2_input
+    sim = Simulation(pts, bm, lat, C2_GRID)
+    run = oracle.Run(pts, bm, lat, C2_GRID)
+    try:
+        rec = sim.run_fused(C2_STEPS)
+        worst = 0.0
+        for k, e in zip(range(C2_STEPS), run.segments()):
+            run.step(e)
+            worst = max(worst, full_row_err(sim.diag(rec[k]), run.diag()))
+        pe = col_err(sim.bunch.getpts(), run.particles())
+        record(test="c2_fused_vs_oracle", steps=C2_STEPS, moments=worst, particles=pe)
+        assert worst < TOL and pe < TOL
+    finally:
+        sim.close()
+
+
+def test_c2_mixed_against_oracle(gpu, oracle, c2_input):
+    """The mixed form (FP32 deposit, convolution, field and gather; FP64
+    state, push and moments) against the FP64 oracle: potential L2 error of
+    the first step (same input, so the same domain) below 1e-5, rms moments
+    within 1e-6 over the eight steps."""
+    from impactz.sim import Simulation
+    bm, pts, lat = c2_input
+    sim = Simulation(pts, bm, lat, C2_GRID, precision=gpu.MIXED)
+    run = oracle.Run(pts, bm, lat, C2_GRID)
+    try:
+        phi_l2, rms = 0.0, 0.0
+        for k, e in zip(range(C2_STEPS), sim.segments()):
+            mom = sim.step(e)
+            _, phi_o = run.step(e, want_fields=True)
+            if k == 0:
+                phi = sim.field.download(gpu.PHI)
+                phi_l2 = float(np.linalg.norm(phi - phi_o) / np.linalg.norm(phi_o))
+            a, b = sim.diag(mom), run.diag()
+            rms = max(rms, float(np.max(np.abs(a[RMS_COLS] - b[RMS_COLS]) /
+                                        np.abs(b[RMS_COLS]))))
+        record(test="c2_mixed_vs_oracle", steps=C2_STEPS, phi_l2=phi_l2, rms=rms)
+        assert phi_l2 < 1e-5 and rms < 1e-6, (phi_l2, rms)
+    finally:
+        sim.close()
+
+
+@pytest.fixture(scope="module")
+def c3_input():
+    from impactz import beam as B
+    print("sampling 50M particles", flush=True)
+    return B.Beam(), B.core_halo_bunch(C3_N, seed=1), B.fodo(1)
+
+
+def test_c3_fused_equals_steps_and_is_reproducible(gpu, c3_input):
+    """At 50M particles on 128x128x256: two fused steps equal two izh_step
+    calls (moments, particles and domain within 1e-10), and a second fused
+    run from the same input gives the bitwise same grid charge."""
+    from impactz.sim import Simulation
+    bm, pts, lat = c3_input
+    a = Simulation(pts, bm, lat, C3_GRID)
+    rows_a = [a.diag(a.step(e)) for _, e in zip(range(2), a.segments())]
+    pa = a.bunch.getpts()
+    rng_a = np.array(a.dom.range)
+    a.close()
+    b = Simulation(pts, bm, lat, C3_GRID)
+    rec = b.run_fused(2)
+    rho1 = b.field.download(gpu.RHO)
+    worst = max(full_row_err(b.diag(rec[k]), rows_a[k]) for k in range(2))
+    pe = col_err(b.bunch.getpts(), pa)
+    de = rel_err(b.dom.range, rng_a)
+    b.close()
+    c = Simulation(pts, bm, lat, C3_GRID)
+    c.run_fused(2)
+    rho2 = c.field.download(gpu.RHO)
+    c.close()
+    same = bool(np.array_equal(rho1, rho2))
+    record(test="c3_fused_vs_step", particles=C3_N, grid=C3_GRID, steps=2, moments=worst,
+           particles_err=pe, domain=de, rho_bitwise=same)
+    assert worst < TOL and pe < TOL and de < TOL
+    assert same
+
+
+def test_c3_split_run_equals_one_run(gpu, c3_input):
+    """Two izh_run calls of 4 steps equal one of 8 (the second starts from
+    the sort state the first left)."""
+    from impactz.sim import Simulation
+    bm, pts, lat = c3_input
+    a = Simulation(pts, bm, lat, C3_GRID)
+    ra = a.run_fused(8)
+    pa = a.bunch.getpts()
+    a.close()
+    b = Simulation(pts, bm, lat, C3_GRID)
+    rb = np.concatenate([b.run_fused(4), b.run_fused(4, start=4)])
+    worst = max(full_row_err(b.diag(rb[k]), b.diag(ra[k])) for k in range(8))
+    pe = col_err(b.bunch.getpts(), pa)
+    b.close()
+    record(test="c3_split_run", steps=8, moments=worst, particles=pe)
+    assert worst < 1e-12 and pe < 1e-12
+
+
+def test_c3_mixed_within_1e4_of_fp64_after_400_steps(gpu, c3_input):
+    """The north star's accuracy bar for the mixed form at full size: rms
+    moments within 1e-4 relative of the FP64 form over 400 steps (25 FODO
+    periods), and the field L2 error at the last step reported."""
+    from impactz import beam as B
+    from impactz.sim import Simulation
+    bm, pts, _ = c3_input
+    lat = B.fodo(25)
+    f64 = Simulation(pts, bm, lat, C3_GRID)
+    mx = Simulation(pts, bm, lat, C3_GRID, precision=gpu.MIXED)
+    try:
+        worst = 0.0
+        for start in range(0, 400, 100):
+            print(f"C3 mixed vs fp64 steps {start}..{start + 99}", flush=True)
+            r64 = f64.run_fused(100, start=start)
+            r32 = mx.run_fused(100, start=start)
+            for k in range(100):
+                d64, d32 = f64.diag(r64[k]), mx.diag(r32[k])
+                worst = max(worst, float(np.max(np.abs(d32[RMS_COLS] - d64[RMS_COLS]) /
+                                                np.abs(d64[RMS_COLS]))))
+        e64 = np.stack([f64.field.download(w) for w in (gpu.EX, gpu.EY, gpu.EZ)])
+        e32 = np.stack([mx.field.download(w) for w in (gpu.EX, gpu.EY, gpu.EZ)])
+        fl2 = float(np.linalg.norm(e32 - e64) / np.linalg.norm(e64))
+        record(test="c3_mixed_vs_fp64", steps=400, rms=worst, field_l2=fl2)
+        assert worst < 1e-4, worst
+        assert fl2 < 1e-3, fl2
+    finally:
+        f64.close()
+        mx.close()

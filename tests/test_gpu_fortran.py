@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import diag_row_err, fixture_beam, fixture_lattice, load_golden
+from conftest import fixture_beam, fixture_lattice, fixture_row_err, load_golden
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
@@ -56,18 +56,17 @@ def test_fortran_host_matches_reference(tmp_path, name, fused):
                    timeout=120)
     rec = np.loadtxt(out, ndmin=2)
     nsteps = int(d["nsteps"])
-    assert rec.shape == (nsteps, 48)
+    assert rec.shape == (nsteps, 49)
     bm = fixture_beam(d)
     c = hip.physconst(bm.freq)
     worst = 0.0
     for k in range(nsteps):
-        refpt = np.zeros(6) - rec[k, 47]
+        refpt = np.zeros(6)
+        refpt[5], refpt[4] = -rec[k, 47], rec[k, 48]
         assert rec[k, 47] == pytest.approx(d["fort18"][k + 1, 2], rel=1e-13)
         row = diag_row(rec[k, 3:40], rec[k, 40:46], rec[k, 46], int(rec[k, 2]), refpt,
                        bm.mass, c.scxl, c.pi)
-        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
-                              d["fort26"][k + 1, 1:]])
-        worst = max(worst, diag_row_err(row, ref, ref_is_fort=True))
+        worst = max(worst, fixture_row_err(row, d, k + 1))
     assert abs(rec[-1, 1] - d["fort24"][nsteps, 0]) < 1e-12
     record(test="fortran_host", case=name, fused=fused, steps=nsteps, moments=worst)
     assert worst < TOL

@@ -3,8 +3,8 @@ the reference.  Same bar as the step-by-step path: 1e-10 relative (FP64)."""
 import numpy as np
 import pytest
 
-from conftest import (col_err, diag_row_err, fixture_beam, fixture_lattice,
-                      load_golden, rel_err)
+from conftest import (col_err, fixture_beam, fixture_lattice, fixture_row_err,
+                      full_row_err, load_golden, rel_err)
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
@@ -29,7 +29,7 @@ def test_fused_matches_stepwise_100k_32cube(gpu):
     rec = b.run_fused(16)
     worst = 0.0
     for k in range(16):
-        worst = max(worst, diag_row_err(b.diag(rec[k], refpt=np.zeros(6) - bm.gamma0), rows_a[k]))
+        worst = max(worst, full_row_err(b.diag(rec[k]), rows_a[k]))
     pe = col_err(b.bunch.getpts(), a.bunch.getpts())
     de = rel_err(b.dom.range, a.dom.range)
     record(test="fused_vs_step", case="100k_32cube", steps=16, moments=worst,
@@ -51,9 +51,7 @@ def test_fused_matches_reference(gpu, name):
     rec = sim.run_fused(nsteps)
     worst = 0.0
     for k in range(nsteps):
-        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
-                              d["fort26"][k + 1, 1:]])
-        worst = max(worst, diag_row_err(sim.diag(rec[k]), ref, ref_is_fort=True))
+        worst = max(worst, fixture_row_err(sim.diag(rec[k]), d, k + 1))
     fin = sim.bunch.getpts()
     assert fin.shape == d["final"].shape
     pe = col_err(fin, d["final"])
@@ -80,7 +78,7 @@ def test_fused_split_runs_and_mixing_with_steps(gpu):
     rb1 = b.run_fused(8)
     rb2 = b.run_fused(8, start=8)
     rb = np.concatenate([rb1, rb2])
-    worst = max(diag_row_err(b.diag(rb[k]), a.diag(ra[k])) for k in range(16))
+    worst = max(full_row_err(b.diag(rb[k]), a.diag(ra[k])) for k in range(16))
     assert worst < 1e-12
     assert col_err(b.bunch.getpts(), a.bunch.getpts()) < 1e-12
     c, _ = _sims(gpu, pts, bm, lat, (16, 16, 32))

@@ -5,22 +5,29 @@ compiled reference wrote for the same decks (tests/golden/io_ckpt.npz)."""
 import numpy as np
 import pytest
 
-from conftest import col_err, diag_row_err, fixture_beam, fixture_lattice, load_golden, rel_err
+from conftest import (col_err, fixture_beam, fixture_lattice, fixture_row_err, load_golden,
+                      rel_err)
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
 
 
 def _rows_err(tmp_path, d, prefix, nrows):
-    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (18, 24, 25, 26)}
+    """Every column of the diagnostic files the deck wrote against the
+    executable's; fort.28 and fort.32 exactly."""
+    from conftest import FORT_COLS
+    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u, _, _ in FORT_COLS}
     assert f[24].shape == (nrows, 7)
     worst = 0.0
-    for k in range(1, nrows):
-        got = np.concatenate([[0.0] * 5, f[24][k, 1:], f[25][k, 1:], f[26][k, 1:]])
-        ref = np.concatenate([d[f"{prefix}_fort24"][k, 1:], d[f"{prefix}_fort25"][k, 1:],
-                              d[f"{prefix}_fort26"][k, 1:]])
-        worst = max(worst, diag_row_err(got, ref, ref_is_fort=True))
+    for k in range(nrows):
+        got = np.zeros(41)
+        for u, lo, n in FORT_COLS:
+            got[lo:lo + n] = f[u][k, 1:1 + n]
+        worst = max(worst, fixture_row_err(got, d, k, f"{prefix}_"))
     assert np.max(np.abs(f[24][:, 0] - d[f"{prefix}_fort24"][:nrows, 0])) < 1e-12
+    for u in (28, 32):
+        mine = np.loadtxt(tmp_path / f"fort.{u}", ndmin=2)
+        assert np.array_equal(mine[:, 1:], d[f"{prefix}_fort{u}"][:nrows, 1:]), u
     return worst
 
 

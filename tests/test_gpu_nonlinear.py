@@ -7,7 +7,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import col_err, diag_row_err, element_entrance, fixture_beam, \
+from conftest import col_err, deck_files_err, element_entrance, fixture_beam, fixture_row_err, \
     fixture_lattice, load_golden, rel_err, segment
 from test_gpu_parity import TOL, record
 
@@ -68,9 +68,8 @@ def test_gpu_nonlinear_trajectory_matches_reference(gpu, name):
     print(f"nonlinear trajectory {name}", flush=True)
     rows = sim.run(nsteps=int(d["nsteps"]))
     worst = 0.0
-    for k in range(1, rows.shape[0]):
-        ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
-        worst = max(worst, diag_row_err(rows[k, 1:], ref, ref_is_fort=True))
+    for k in range(rows.shape[0]):
+        worst = max(worst, fixture_row_err(rows[k, 1:], d, k))
     pe = col_err(sim.bunch.getpts(), d["final"])
     record(test="nonlinear_trajectory", case=name, moments=worst, particles=pe)
     assert worst < TOL and pe < TOL
@@ -109,13 +108,9 @@ def test_gpu_reference_example1_deck(tmp_path, gpu, fused):
     write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
     print(f"example1 deck fused={fused}", flush=True)
     run_deck(str(tmp_path), fused=fused)
-    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (18, 24, 25, 26)}
+    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (18, 24)}
     assert f[24].shape == d["fort24"].shape
-    worst = 0.0
-    for k in range(1, f[24].shape[0]):
-        got = np.concatenate([[0.0] * 5, f[24][k, 1:], f[25][k, 1:], f[26][k, 1:]])
-        ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
-        worst = max(worst, diag_row_err(got, ref, ref_is_fort=True))
+    worst = deck_files_err(str(tmp_path), d)
     gam = np.abs(f[18][:, 2] / d["fort18"][:, 2] - 1).max()
     record(test="example1_deck", fused=fused, steps=f[24].shape[0] - 1, moments=worst,
            gamma=float(gam))

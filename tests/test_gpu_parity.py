@@ -10,8 +10,8 @@ import os
 import numpy as np
 import pytest
 
-from conftest import (ROOT, col_err, diag_row_err, element_entrance, fixture_beam,
-                      fixture_lattice, load_golden, rel_err, segment)
+from conftest import (ROOT, col_err, element_entrance, fixture_beam, fixture_lattice,
+                      fixture_row_err, full_row_err, load_golden, rel_err, segment)
 
 pytestmark = pytest.mark.gpu
 
@@ -110,9 +110,8 @@ def test_gpu_trajectory_matches_reference(gpu, name):
     sim = Simulation(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
     rows = sim.run(nsteps=int(d["nsteps"]))
     worst = 0.0
-    for k in range(1, rows.shape[0]):
-        ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
-        worst = max(worst, diag_row_err(rows[k, 1:], ref, ref_is_fort=True))
+    for k in range(rows.shape[0]):
+        worst = max(worst, fixture_row_err(rows[k, 1:], d, k))
     fin = sim.bunch.getpts()
     assert fin.shape == d["final"].shape
     pe = col_err(fin, d["final"])
@@ -139,7 +138,7 @@ def test_gpu_matches_oracle_100k_32cube(gpu, oracle):
         rho_o, phi_o = run.step(e, want_fields=True)
         worst["rho"] = max(worst["rho"], rel_err(sim.field.download(gpu.RHO), rho_o))
         worst["phi"] = max(worst["phi"], rel_err(sim.field.download(gpu.PHI), phi_o))
-        worst["moments"] = max(worst["moments"], diag_row_err(sim.diag(mom), run.diag()))
+        worst["moments"] = max(worst["moments"], full_row_err(sim.diag(mom), run.diag()))
     worst["pts"] = col_err(sim.bunch.getpts(), run.particles())
     record(test="oracle_100k_32cube", **worst)
     for k, v in worst.items():

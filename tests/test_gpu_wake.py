@@ -7,7 +7,7 @@ executable's own -41 run, tests/golden/exe_wake.npz)."""
 import numpy as np
 import pytest
 
-from conftest import (col_err, diag_row_err, element_entrance, fixture_beam,
+from conftest import (col_err, deck_files_err, fixture_row_err, element_entrance, fixture_beam,
                       fixture_lattice, load_golden, segment)
 from test_gpu_parity import TOL, record
 
@@ -82,9 +82,7 @@ def test_gpu_wake_runs_match_reference(gpu, name, fused):
         rows = [sim.diag(sim.step(e)) for _, e in zip(range(nsteps), sim.segments())]
     worst = 0.0
     for k in range(nsteps):
-        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
-                              d["fort26"][k + 1, 1:]])
-        worst = max(worst, diag_row_err(rows[k], ref, ref_is_fort=True))
+        worst = max(worst, fixture_row_err(rows[k], d, k + 1))
     record(test="wake_run", case=name, fused=fused, moments=worst)
     assert worst < TOL
     if "final" in d:
@@ -103,13 +101,7 @@ def test_gpu_wake_deck_matches_reference_executable(tmp_path, gpu, fused):
     write_particle_in(str(tmp_path / "particle.in"), d["pts_in"])
     print(f"wake deck fused={fused}", flush=True)
     run_deck(str(tmp_path), fused=fused)
-    f = {u: np.loadtxt(tmp_path / f"fort.{u}", ndmin=2) for u in (24, 25, 26)}
-    assert f[24].shape == d["fort24"].shape
-    worst = 0.0
-    for k in range(1, f[24].shape[0]):
-        got = np.concatenate([[0.0] * 5, f[24][k, 1:], f[25][k, 1:], f[26][k, 1:]])
-        ref = np.concatenate([d["fort24"][k, 1:], d["fort25"][k, 1:], d["fort26"][k, 1:]])
-        worst = max(worst, diag_row_err(got, ref, ref_is_fort=True))
+    worst = deck_files_err(str(tmp_path), d)
     record(test="wake_deck", case="exe_wake", fused=fused, moments=worst)
     assert worst < TOL
 
@@ -190,9 +182,7 @@ def test_gpu_kick2wake_runs_match_reference(gpu, name):
     rows = [sim.diag(sim.step(e)) for _, e in zip(range(nsteps), sim.segments())]
     worst = 0.0
     for k in range(nsteps):
-        ref = np.concatenate([d["fort24"][k + 1, 1:], d["fort25"][k + 1, 1:],
-                              d["fort26"][k + 1, 1:]])
-        worst = max(worst, diag_row_err(rows[k], ref, ref_is_fort=True))
+        worst = max(worst, fixture_row_err(rows[k], d, k + 1))
     pe = col_err(sim.bunch.getpts(), d["final"])
     record(test="kick2wake_run", case=name, moments=worst, particles=pe)
     assert worst < TOL and pe < TOL

@@ -13,8 +13,8 @@ import math
 import numpy as np
 import pytest
 
-from conftest import (col_err, element_entrance, fixture_beam, fixture_lattice, load_golden,
-                      rel_err, segment)
+from conftest import (col_err, element_entrance, fixture_beam, fixture_lattice, fort_row,
+                      full_row_err, load_golden, rel_err, segment)
 
 CASES = [("fodo_16cube", 1), ("fodo_16cube", 5), ("fodo_aniso", 2),
          ("fodo_lossy", 1), ("fodo_lossy", 2), ("cf_sol", 2), ("cf_sol", 10),
@@ -100,6 +100,8 @@ def test_oracle_trajectory_matches_reference(oracle, name):
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
         assert row[0] == pytest.approx(d["fort18"][k, 1], rel=1e-14)
+        # every column of fort.18, 27, 29 and 30 as well, each on its own scale
+        assert full_row_err(row, fort_row(d, k)) < 1e-11, k
     fin = d["final"]
     assert run.n == fin.shape[0]
     assert col_err(run.particles(), fin) < 1e-12
@@ -427,6 +429,7 @@ def test_oracle_matches_executable_lorentz_runs(oracle, tag):
                 scale[0] = max(scale[0], abs(ref[1]))
                 scale[2] = max(scale[2], abs(ref[3]))
                 assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+            assert full_row_err(row, fort_row(d, k, f"{tag}_")) < 1e-11, k
             assert run.st.z == pytest.approx(d[f"{tag}_fort24"][k, 0], rel=1e-14)
     assert k == d[f"{tag}_fort24"].shape[0] - 1
 
@@ -655,6 +658,7 @@ def test_oracle_matches_executable_wake_run(oracle):
             scale[0] = max(scale[0], abs(ref[1]))
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+        assert full_row_err(row, fort_row(d, k)) < 1e-11, k
     assert k == d["fort24"].shape[0] - 1
 
 
@@ -679,6 +683,7 @@ def test_oracle_matches_reference_example1(oracle):
             scale[2] = max(scale[2], abs(ref[3]))
             assert np.all(np.abs(got - ref) <= 1e-10 * scale), (k, f, got, ref)
         assert run.st.refpt[5] == pytest.approx(-d["fort18"][k, 2], rel=1e-13)
+        assert full_row_err(row, fort_row(d, k)) < 1e-10, k
 
 
 @pytest.mark.parametrize("name,step,kind", [("nl_wake_file", 6, 2), ("nl_wake_solrf", 7, 1)])
@@ -758,6 +763,7 @@ def test_oracle_matches_executable_bends(oracle, flagmap):
                 scale[0] = max(scale[0], abs(ref[1]))
                 scale[2] = max(scale[2], abs(ref[3]))
                 assert np.all(np.abs(got - ref) <= 1e-11 * scale), (k, f, got, ref)
+            assert full_row_err(row, fort_row(d, k, f"m{flagmap}_")) < 1e-11, k
             assert run.st.z == pytest.approx(d[f"m{flagmap}_fort24"][k, 0], rel=1e-14)
     assert k == d[f"m{flagmap}_fort24"].shape[0] - 1
     ref = d[f"m{flagmap}_phase60"]
@@ -793,3 +799,42 @@ def test_oracle_isr_closed_forms(oracle):
     assert (gbm - gb1) ** 2 == pytest.approx(var, rel=4e-6)
     # transverse momenta shrink with the momentum: zero stays zero
     assert np.all(pts[:, [1, 3]] == 0.0)
+
+
+def test_oracle_matches_executable_charge_states(oracle):
+    """Two charge states through the reference executable
+    (tests/golden/exe_chg.npz): every column of every diagnostic file, the
+    particle counts of fort.28 and the per-state counts of fort.32, which
+    chgupdate_BeamBunch recounts after each conv1st as particles of both
+    states are lost."""
+    d = load_golden("exe_chg")
+    grid = tuple(int(v) for v in d["grid"])
+    qmcc = d["charge_states"][:, 2]
+    run = oracle.Run(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
+    assert full_row_err(run.diag(), fort_row(d, 0)) < 1e-12
+    # before the first step fort.32 holds the deck's nptlist
+    assert np.array_equal(d["fort32"][0, 1:], d["charge_states"][:, 0])
+    k = 0
+    for k, e in enumerate(run.segments(), 1):
+        run.step(e)
+        assert full_row_err(run.diag(), fort_row(d, k)) < 1e-11, k
+        assert d["fort28"][k, 1] == d["fort28"][k, 2] == run.n
+        assert d["fort28"][k, 3] == run.st.npt
+        counts = oracle.chgupdate(run.particles(), qmcc)
+        assert np.array_equal(counts, d["fort32"][k, 1:]), (k, counts)
+    assert k == d["fort24"].shape[0] - 1
+    assert d["fort32"][-1, 1:].sum() < d["fort32"][0, 1:].sum()   # losses of both states
+    assert np.all(d["fort32"][-1, 1:] < d["fort32"][0, 1:])
+
+
+def test_chgupdate_closed_form(oracle):
+    """A particle counts for a state when its q/mc^2 lies within the REAL(4)
+    1e-4 of that state, for every state it is that close to."""
+    q = 1.0 / 938.272088e6
+    eps = float(np.float32(1.0e-4))
+    pts = np.zeros((6, 9))
+    pts[:, 6] = [q, q * (1 + 0.99 * eps), q * (1 - 1.01 * eps), 2 * q, -q, q]
+    assert list(oracle.chgupdate(pts, [q, 2 * q, -q])) == [3, 1, 1]
+    assert list(oracle.chgupdate(pts, [q * (1 + 0.5 * eps)])) == [3]
+    assert list(oracle.chgupdate(pts, [q * (1 - 0.5 * eps)])) == [3]
+    assert list(oracle.chgupdate(pts, [q * (1 + 1.5 * eps)])) == [1]
