@@ -921,9 +921,15 @@ int izh_sector_dipole(izh_bunch *b, double *z, double tau, const izh_element *e)
   return IZH_OK;
 }
 
-int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd) {
+int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd,
+                    int64_t nrd) {
   IZH_CHECK(dipole_check(b, e, "izh_isr_chicane"));
   if (!rd && b->n > 0) return fail(IZH_ERR_ARG, "izh_isr_chicane: no deviates");
+  // the kernel reads rd[orig], and orig runs over the particles of the upload
+  if (b->n > 0 && nrd < b->n0)
+    return fail(IZH_ERR_ARG, "izh_isr_chicane: %lld deviates for a bunch uploaded with "
+                "%lld particles (one per uploaded particle, indexed by its upload row)",
+                (long long)nrd, (long long)b->n0);
   // b0 = |B rho angle / L| (src/Contrl/AccSimulator.f90:1436-1439) and the
   // constants of ISRchicane (:1553-1570)
   const double clight = b->c.clight;
@@ -1585,9 +1591,6 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
                   sg.e.bitype, why);
     if (!(sg.tau1 >= 0) || (kind == kMap1Linear && sg.e.mapstp < 1))
       return fail(IZH_ERR_ARG, "izh_run: segment %lld: bad step or map steps", (long long)s);
-    if (kind == kMap1QuadK && bcurr >= (double)1.0e-10f)
-      return fail(IZH_ERR_ARG, "izh_run: segment %lld is a K-mode quadrupole (chromatic "
-                  "map): run it through izh_step", (long long)s);
     if (needs_wake(sg.e) && !sg.wake)
       return fail(IZH_ERR_ARG, "izh_run: segment %lld: bitype %d with a wake length "
                   "(Param(15) > 0) needs its izh_wake", (long long)s, sg.e.bitype);
@@ -1611,6 +1614,22 @@ int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     return IZH_OK;
   }
   return run_steps(b, f, dom, z, segs, nseg, moments);
+}
+
+int izh_bunch_set_tuning(izh_bunch *b, const izh_tuning *t) {
+  if (!b || !t) return fail(IZH_ERR_ARG, "izh_bunch_set_tuning: bad argument");
+  if (t->resort_period < 0 || t->resort_period > 1 << 20)
+    return fail(IZH_ERR_ARG, "izh_bunch_set_tuning: resort_period %d outside 0..2^20",
+                t->resort_period);
+  b->resort_period = t->resort_period;
+  return IZH_OK;
+}
+
+int izh_bunch_get_tuning(const izh_bunch *b, izh_tuning *t) {
+  if (!b || !t) return fail(IZH_ERR_ARG, "izh_bunch_get_tuning: bad argument");
+  *t = izh_tuning{};
+  t->resort_period = b->resort_period > 0 ? b->resort_period : kDefaultResortPeriod;
+  return IZH_OK;
 }
 
 int izh_run_stats(const izh_bunch *b, int64_t stats[4]) {

@@ -36,31 +36,48 @@ struct HalfPush {
   double tau_xl, gam0, phase;  // exact drift: tau/Scxl, gamma0, tau/(Scxl beta0)
   // kPushMap: the (x,px), (y,py), (phi,pt) 2x2 blocks, row major (m[0..11]);
   // kPushMapC: the 4x4 transverse block, row major (m[0..15]), then the
-  // (phi,pt) block (m[16..19])
+  // (phi,pt) block (m[16..19]);
+  // kPushQuadK: m[0] tau, m[1] K, m[2] the beam's q/mc^2, m[3] gamma0,
+  // m[4] gamma0 beta0, m[5] beta0, m[6] Scxl
   double m[20];
 };
 
-// kPushMapC is a map whose transverse planes are coupled (the solenoid).
-enum PushKind : int { kPushNone = 0, kPushDrift = 1, kPushMap = 2, kPushMapC = 3 };
+// kPushMapC is a map whose transverse planes are coupled (the solenoid);
+// kPushQuadK the chromatic map of a quadrupole given by K, per particle.
+enum PushKind : int {
+  kPushNone = 0, kPushDrift = 1, kPushMap = 2, kPushMapC = 3, kPushQuadK = 4
+};
+
+// What a pass does with the slots: kPassScatter writes each particle to its
+// counting-sort slot by the tile key the previous pass computed (and moves
+// its original index); without it the pass writes every particle back to
+// the slot it read (coalesced, no atomics, the original index stays).
+// kPassKeys computes the tile key of each particle's new position and its
+// histogram, for a scatter in the next pass.
+enum PassMode : int { kPassInPlace = 0, kPassScatter = 1, kPassKeys = 2 };
 
 struct PassCfg {
   const double *in;         // six rows of cap doubles, row k at in + k*cap
-  double *out;
-  const uint32_t *in_orig;
-  uint32_t *out_orig;
-  const uint32_t *in_key;
-  uint32_t *out_key;
+  double *out;              // (== in for a pass without kPassScatter)
+  const uint32_t *in_orig;  // original index of the input slots
+  uint32_t *out_orig;       // kPassScatter: original index of the output slots
+  const uint32_t *in_key;   // kPassScatter: tile key of the input slots
+  uint32_t *out_key;        // kPassKeys: tile key of the output slots
   void *tpos;               // t-frame positions (T), 3 blocks of cap
   int64_t cap;
   const int64_t *d_nslots;  // input slots (device)
-  uint32_t *cursor;         // scatter cursor per tile of the input keys
-  uint32_t *hist_new;       // histogram of the output keys
+  uint32_t *cursor;         // kPassScatter: scatter cursor per tile of the input keys
+  uint32_t *hist_new;       // kPassKeys: histogram of the output keys
   const DevDom *dom;        // domain of step s-1: gather and key geometry
   const void *e;            // E4<T> of step s-1
   const DevScal *scal;      // current and kick constants of step s-1
   DevScal *err;             // the same record, for the slot-overflow flag
   int nx, ny, nz;
   int tshift[3], tnx, tny;
+  int kb, ka;               // push kinds of pb and pa (the same for every lane)
+  int need_orig;            // a K-mode push needs each particle's own q/mc^2
+  const double *qm0;        // q/mc^2 by original index
+  double qm_value;          // the q/mc^2 of every particle (without need_orig)
   double gam, xl, rad;      // -refpt(6), Scxl, pipe radius of step s
   double gam_k;             // -refpt(6) of step s-1 (its kick; RF cavities change it)
   double tau2_prev;         // 2*tau1 of step s-1
@@ -102,6 +119,56 @@ __device__ inline void half_push(const HalfPush &p, double &x, double &px,
     const double ph1 = fma(pt, m[17], ph * m[16]), pt1 = fma(pt, m[19], ph * m[18]);
     x = t[0]; px = t[1]; y = t[2]; py = t[3]; ph = ph1; pt = pt1;
   }
+}
+
+// transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607) for one particle of
+// q/mc^2 qm: its own strength kstr = qm/qm0 * gambet0/|gamma beta| * K gives a
+// cos/sin (focusing) and cosh/sinh (defocusing) pair of 2x2 maps; the phase
+// slips by (1/beta_z - 1/beta0) tau/Scxl.  The arithmetic of k_quad_kmode.
+__device__ inline void half_push_kmode(const HalfPush &p, double qm, double &x, double &px,
+                                       double &y, double &py, double &ph, double pt) {
+  const double tau = p.m[0], kq = p.m[1], qmass = p.m[2], gam0 = p.m[3];
+  const double gambet0 = p.m[4], beta0 = p.m[5], scxl = p.m[6];
+  const double gam = gam0 - pt;
+  double gambetz = sqrt(gam * gam - 1.0 - px * px - py * py);
+  const double betaz = gambetz / gam;
+  gambetz = sqrt(gam * gam - 1.0);
+  const double kstr = qm / qmass * gambet0 / gambetz * kq;
+  const double rtkstr = sqrt(fabs(kstr));
+  const double rtkstrzz = rtkstr * tau;
+  double xm11, xm12, xm21, xm22, xm33, xm34, xm43, xm44;
+  if (kstr != 0.0) {
+    const double co = cos(rtkstrzz), si = sin(rtkstrzz);
+    const double t = exp(rtkstrzz);
+    const double ch = (t * t + 1.0) / (t + t), sh = (t * t - 1.0) / (t + t);
+    if (kstr > 0.0) {
+      xm11 = co; xm12 = si / rtkstr; xm21 = -rtkstr * si; xm22 = co;
+      xm33 = ch; xm34 = sh / rtkstr; xm43 = sh * rtkstr; xm44 = ch;
+    } else {
+      xm11 = ch; xm12 = sh / rtkstr; xm21 = rtkstr * sh; xm22 = ch;
+      xm33 = co; xm34 = si / rtkstr; xm43 = -si * rtkstr; xm44 = co;
+    }
+  } else {
+    xm11 = 1.0; xm12 = tau; xm21 = 0.0; xm22 = 1.0;
+    xm33 = 1.0; xm34 = tau; xm43 = 0.0; xm44 = 1.0;
+  }
+  const double x1 = xm11 * x + xm12 * px / gambetz / scxl;
+  const double px1 = gambetz * scxl * xm21 * x + xm22 * px;
+  const double y1 = xm33 * y + xm34 * py / gambetz / scxl;
+  const double py1 = gambetz * scxl * xm43 * y + xm44 * py;
+  ph = ph + (1.0 / betaz - 1.0 / beta0) * tau / scxl;
+  x = x1; px = px1; y = y1; py = py1;
+}
+
+// The push of a kind chosen at launch (the same for every lane, so the
+// branches are scalar).
+__device__ inline void half_push_any(int kind, const HalfPush &p, double qm, double &x,
+                                     double &px, double &y, double &py, double &ph,
+                                     double &pt) {
+  if (kind == kPushDrift) half_push<kPushDrift>(p, x, px, y, py, ph, pt);
+  else if (kind == kPushMap) half_push<kPushMap>(p, x, px, y, py, ph, pt);
+  else if (kind == kPushMapC) half_push<kPushMapC>(p, x, px, y, py, ph, pt);
+  else if (kind == kPushQuadK) half_push_kmode(p, qm, x, px, y, py, ph, pt);
 }
 
 // Counting-sort slot of each participating lane, in two halves so the
@@ -244,14 +311,17 @@ __device__ inline void gather_batched(Load ld, int64_t o, int64_t sy, int64_t sz
   gz = szz;
 }
 
-// PB: push of the second half of step s-1 (kPushNone on the first pass, which
-// then has no kick and no moments); PA: first half of step s (kPushNone on
-// the last pass, which then has no z->t transform).  The slot loop is
-// unrolled by two over two register sets, so the loads of the next slot
-// group land in the set not being worked on.
-template <typename T, int PB, int PA>
-__global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
-  constexpr bool kKick = PB != kPushNone, kConv = PA != kPushNone;
+// One pass finishes step s-1 and starts step s.  pb (kind kb) is the second
+// half push of step s-1 (kPushNone on the first pass of a run, which then has
+// no kick and no moments); pa (kind ka) the first half push of step s
+// (kPushNone on the last pass, which then has no z->t transform).  MODE is
+// a set of PassMode bits.  The slot loop is unrolled by two over two
+// register sets, so the loads of the next slot group land in the set not
+// being worked on.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
+  constexpr bool kScatter = (MODE & kPassScatter) != 0, kKeys = (MODE & kPassKeys) != 0;
+  const bool has_kick = c.kb != kPushNone, has_conv = c.ka != kPushNone;
   const int64_t nslots = *c.d_nslots;
   int64_t lo, hi;
   block_range(nslots, lo, hi);
@@ -260,7 +330,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   const double xmin = c.dom->range[0], ymin = c.dom->range[2], zmin = c.dom->range[4];
   const double hx = c.dom->msize[0], hy = c.dom->msize[1], hz = c.dom->msize[2];
   const double hxi = c.dom->hinv[0], hyi = c.dom->hinv[1], hzi = c.dom->hinv[2];
-  const bool kick = kKick && c.scal->current >= (double)1.0e-20f;
+  const bool kick = has_kick && c.scal->current >= (double)1.0e-20f;
   const double txy = c.tau2_prev * c.scal->xycon * c.inv_rho_scale;
   const double ttc = c.tau2_prev * c.scal->tcon * c.inv_rho_scale;
   const int sy = c.nx, sz = c.nx * c.ny;
@@ -274,17 +344,17 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   pmom_zero(mom);
   double mn[3] = {1e300, 1e300, 1e300}, mxv[3] = {-1e300, -1e300, -1e300};
   uint32_t nlost = 0, nalive = 0, nerr = 0;
-  // The input slots are grouped by the tiles of the keys the previous pass
-  // scattered by; in_chunks lists those slot ranges in slot order.  The
-  // gather box follows the tile of each group's first slot: the block walks
-  // the list forward from the chunk holding its first slot.
+  // The input slots are grouped by the tiles of the keys of the last
+  // scatter; in_chunks lists those slot ranges in slot order.  The gather
+  // box follows the tile of each group's first slot: the block walks the
+  // list forward from the chunk holding its first slot.
   // The list is not written during the pass, so it is read through the
   // constant address space: scalar loads, which no vector-memory wait counts.
   using ConstChunk = __attribute__((address_space(4))) const Chunk;
   const ConstChunk *chs = (const ConstChunk *)c.in_chunks;
   int64_t nch = 0, ci = 0, nxt_start = INT64_MAX;
   uint32_t cur_tile = kDeadKey;
-  if constexpr (kKick) {
+  if (has_kick) {
     nch = *(__attribute__((address_space(4))) const int64_t *)c.in_nchunks;
     int64_t a = 0, b = nch;  // last chunk starting at or before lo
     while (b - a > 1) {
@@ -306,42 +376,50 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     const int64_t j = base + threadIdx.x;
     const int64_t jj = j < hi ? j : hi - 1;
     uint32_t k0 = kDeadKey;
-    if constexpr (kKick) {
-      if (base < hi) {
-        // the next chunk's start is held, so only a crossing loads
-        while (nxt_start <= base) {
-          ci++;
-          cur_tile = chs[ci].tile;
-          nxt_start = ci + 1 < nch ? (int64_t)chs[ci + 1].start : INT64_MAX;
-        }
-        k0 = cur_tile;
+    if (has_kick && base < hi) {
+      // the next chunk's start is held, so only a crossing loads
+      while (nxt_start <= base) {
+        ci++;
+        cur_tile = chs[ci].tile;
+        nxt_start = ci + 1 < nch ? (int64_t)chs[ci + 1].start : INT64_MAX;
       }
+      k0 = cur_tile;
     }
-    const uint32_t key = c.in_key[jj];
-    const uint32_t og = c.in_orig[jj];
+    uint32_t key = 0u, og = 0u;
+    if constexpr (kScatter) {
+      key = c.in_key[jj];
+      og = c.in_orig[jj];
+    } else {
+      if (c.need_orig) og = c.in_orig[jj];
+    }
     double r[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) r[k] = c.in[k * cap + jj];
     const bool in = j < hi;
     sl.k0 = k0;
-    sl.key = in ? key : kDeadKey;
+    // a scatter pass takes the particles the previous pass keyed; a pass in
+    // place every particle not marked lost (x = NaN)
+    if constexpr (kScatter) sl.key = in ? key : kDeadKey;
+    else sl.key = in && r[0] == r[0] ? 0u : kDeadKey;
     sl.orig = in ? og : 0u;
 #pragma unroll
     for (int k = 0; k < 6; k++) sl.r[k] = in ? r[k] : 0.0;
   };
 
-  auto step = [&](SlotIn &sl, uint32_t k0, auto &&prefetch) {
+  auto step = [&](SlotIn &sl, uint32_t k0, int64_t base, auto &&prefetch) {
     const uint32_t kin = sl.key;
     double &x = sl.r[0], &px = sl.r[1], &y = sl.r[2], &py = sl.r[3], &ph = sl.r[4],
            &pt = sl.r[5];
     const bool valid = kin != kDeadKey;
     nalive += valid ? 1u : 0u;
-    // counting-sort scatter by the key of the previous step; the next slot
+    // counting-sort scatter by the key of the previous pass; the next slot
     // group's loads are issued after the atomic, so waiting for its result
     // does not wait for them
-    const RankTicket tk = wave_rank_issue(c.cursor, kin, valid);
+    RankTicket tk{0u, 0, 0u};
+    if constexpr (kScatter) tk = wave_rank_issue(c.cursor, kin, valid);
     prefetch();
-    if constexpr (kKick) {
+    const double qm = c.need_orig ? c.qm0[sl.orig] : c.qm_value;
+    if (has_kick) {
       // restage the gather box when the slot group starts another tile
       if (kick && k0 != kDeadKey && k0 != box_key) {
         const uint32_t txy = (uint32_t)c.tnx * (uint32_t)c.tny;
@@ -421,16 +499,16 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
           pt = pt - ttc * ezn;
         }
       }
-      half_push<PB>(c.pb, x, px, y, py, ph, pt);
+      half_push_any(c.kb, c.pb, qm, x, px, y, py, ph, pt);
       // diagnostic1 sums of the state after step s-1 (all lanes take part)
       pmom_add(mom, valid ? x : 0.0, valid ? px : 0.0, valid ? y : 0.0,
                valid ? py : 0.0, valid ? ph : 0.0, valid ? pt : 0.0);
     }
     uint32_t knew = valid ? 0u : kDeadKey;
     T tx = (T)NAN, ty = (T)NAN, tz = (T)NAN;
-    if constexpr (kConv) {
+    if (has_conv) {
       if (valid) {
-        half_push<PA>(c.pa, x, px, y, py, ph, pt);
+        half_push_any(c.ka, c.pa, qm, x, px, y, py, ph, pt);
         // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
         ph = wrap_phase(ph);
         const double rcp = 1.0 / (-pt + c.gam);
@@ -447,42 +525,46 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
         mxv[2] = fmax(mxv[2], zt);
         const double radtest = sqrt(xt * xt + yt * yt);
         if (radtest >= c.rad || tmpbet <= 0.0) {
+          // lost: marked by x = NaN for the passes that keep it in its slot
           nlost++;
           knew = kDeadKey;
+          x = (double)NAN;
         } else {
-          const int ci = min(max((int)((xt - xmin) * hxi), 0), c.nx - 1);
-          const int cj = min(max((int)((yt - ymin) * hyi), 0), c.ny - 1);
-          const int ck = min(max((int)((zt - zmin) * hzi), 0), c.nz - 1);
-          knew = (uint32_t)((ci >> c.tshift[0]) +
-                            c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
+          if constexpr (kKeys) {
+            const int ci = min(max((int)((xt - xmin) * hxi), 0), c.nx - 1);
+            const int cj = min(max((int)((yt - ymin) * hyi), 0), c.ny - 1);
+            const int ck = min(max((int)((zt - zmin) * hzi), 0), c.nz - 1);
+            knew = (uint32_t)((ci >> c.tshift[0]) +
+                              c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
+          }
           tx = (T)xt;
           ty = (T)yt;
           tz = (T)zt;
         }
       }
-      // histogram of the new keys for the next pass
-      wave_count_add(c.hist_new, knew, knew != kDeadKey);
-    } else {
+    } else if (kKeys && valid) {
       // last pass: the tile of the 0th-order t-frame position the next run
       // starts from (k_prologue_keys' position, on the current domain), so
       // the next run needs no prologue
-      if (valid) {
-        const double rcp = 1.0 / (-pt + c.gam);
-        const double betai = sqrt(fabs(1.0 - rcp * rcp * (1.0 + px * px + py * py)));
-        const double X0 = x * c.xl, Y0 = y * c.xl;
-        const double Z0 = -c.gam * betai * wrap_phase(ph) * c.xl;
-        const int ci = min(max((int)((X0 - xmin) * hxi), 0), c.nx - 1);
-        const int cj = min(max((int)((Y0 - ymin) * hyi), 0), c.ny - 1);
-        const int ck = min(max((int)((Z0 - zmin) * hzi), 0), c.nz - 1);
-        knew = (uint32_t)((ci >> c.tshift[0]) +
-                          c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
-      }
-      wave_count_add(c.hist_new, knew, knew != kDeadKey);
+      const double rcp = 1.0 / (-pt + c.gam);
+      const double betai = sqrt(fabs(1.0 - rcp * rcp * (1.0 + px * px + py * py)));
+      const double X0 = x * c.xl, Y0 = y * c.xl;
+      const double Z0 = -c.gam * betai * wrap_phase(ph) * c.xl;
+      const int ci = min(max((int)((X0 - xmin) * hxi), 0), c.nx - 1);
+      const int cj = min(max((int)((Y0 - ymin) * hyi), 0), c.ny - 1);
+      const int ck = min(max((int)((Z0 - zmin) * hzi), 0), c.nz - 1);
+      knew = (uint32_t)((ci >> c.tshift[0]) +
+                        c.tnx * ((cj >> c.tshift[1]) + c.tny * (ck >> c.tshift[2])));
     }
-    const uint32_t slot = wave_rank_slot(tk);
+    // histogram of the new keys for the next pass's scatter
+    if constexpr (kKeys) wave_count_add(c.hist_new, knew, knew != kDeadKey);
+    uint32_t slot;
+    if constexpr (kScatter) slot = wave_rank_slot(tk);
+    else slot = (uint32_t)(base + threadIdx.x);
     // Every lane stores: a lane without a particle (or with a slot past the
     // buffer, which must not happen and is counted) writes into the sink, so
-    // no branch skips the stores and later waits can be counted exactly.
+    // no branch skips the stores and later waits can be counted exactly.  A
+    // pass in place leaves the slots of particles lost earlier as they are.
     const bool ok = valid && slot < (uint32_t)cap;
     nerr += (valid && !ok) ? 1u : 0u;
     const int lane = threadIdx.x & 63;
@@ -495,9 +577,9 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
     (ok ? c.out[3 * cap + slot] : sd[192]) = py;
     (ok ? c.out[4 * cap + slot] : sd[256]) = ph;
     (ok ? c.out[5 * cap + slot] : sd[320]) = pt;
-    (ok ? c.out_orig[slot] : su[0]) = sl.orig;
-    (ok ? c.out_key[slot] : su[64]) = knew;
-    if constexpr (kConv) {
+    if constexpr (kScatter) (ok ? c.out_orig[slot] : su[0]) = sl.orig;
+    if constexpr (kKeys) (ok ? c.out_key[slot] : su[64]) = knew;
+    if (has_conv) {
       (ok ? tpx[slot] : st[0]) = tx;
       (ok ? tpx[cap + slot] : st[64]) = ty;
       (ok ? tpx[2 * cap + slot] : st[128]) = tz;
@@ -510,13 +592,13 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   SlotIn sa, sb;
   if (lo < hi) {
     load(lo, sa);
-    step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0),
+    step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), lo,
          [&] { load(lo + kBlock, sb); });
     for (int64_t base = lo + kBlock; base < hi; base += 2 * kBlock) {
-      step(sb, (uint32_t)__builtin_amdgcn_readfirstlane((int)sb.k0),
+      step(sb, (uint32_t)__builtin_amdgcn_readfirstlane((int)sb.k0), base,
            [&] { load(base + kBlock, sa); });
       if (base + kBlock >= hi) break;
-      step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0),
+      step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), base + kBlock,
            [&] { load(base + 2 * kBlock, sb); });
     }
   }
@@ -525,7 +607,7 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   __shared__ double sh[kBlock / 64][kMoments];
   // partials are quantity-major: value q of block b at part[q * kMaxRedBlocks + b]
   double *out = c.part + blockIdx.x;
-  if constexpr (kKick) {
+  if (has_kick) {
     pmom_block_store(mom, sh, out, kMaxRedBlocks);
   } else if (threadIdx.x < kMoments) {
     out[(int64_t)threadIdx.x * kMaxRedBlocks] = 0.0;
@@ -548,26 +630,16 @@ __global__ void __launch_bounds__(kBlock) k_pass(PassCfg c) {
   }
 }
 
-template <typename T, int PB>
-void launch_pass_pa(int pa, int nb, hipStream_t st, const PassCfg &pc) {
-  const dim3 g(nb), b(kBlock);
-  switch (pa) {
-    case kPushNone:
-      if constexpr (PB != kPushNone) hipLaunchKernelGGL((k_pass<T, PB, kPushNone>), g, b, 0, st, pc);
-      break;
-    case kPushDrift: hipLaunchKernelGGL((k_pass<T, PB, kPushDrift>), g, b, 0, st, pc); break;
-    case kPushMap: hipLaunchKernelGGL((k_pass<T, PB, kPushMap>), g, b, 0, st, pc); break;
-    default: hipLaunchKernelGGL((k_pass<T, PB, kPushMapC>), g, b, 0, st, pc); break;
-  }
-}
-
 template <typename T>
-void launch_pass(int pb, int pa, int nb, hipStream_t st, const PassCfg &pc) {
-  switch (pb) {
-    case kPushNone: launch_pass_pa<T, kPushNone>(pa, nb, st, pc); break;
-    case kPushDrift: launch_pass_pa<T, kPushDrift>(pa, nb, st, pc); break;
-    case kPushMap: launch_pass_pa<T, kPushMap>(pa, nb, st, pc); break;
-    default: launch_pass_pa<T, kPushMapC>(pa, nb, st, pc); break;
+void launch_pass(int mode, int nb, hipStream_t st, const PassCfg &pc) {
+  const dim3 g(nb), b(kBlock);
+  switch (mode) {
+    case kPassInPlace: hipLaunchKernelGGL((k_pass<T, kPassInPlace>), g, b, 0, st, pc); break;
+    case kPassKeys: hipLaunchKernelGGL((k_pass<T, kPassKeys>), g, b, 0, st, pc); break;
+    case kPassScatter: hipLaunchKernelGGL((k_pass<T, kPassScatter>), g, b, 0, st, pc); break;
+    default:
+      hipLaunchKernelGGL((k_pass<T, kPassScatter | kPassKeys>), g, b, 0, st, pc);
+      break;
   }
 }
 
@@ -666,6 +738,8 @@ struct FinCfg {
   Chunk *chunks;            // -> deposit chunks of that grouping
   int64_t *nchunks;         // -> their number
   uint32_t *hist_zero;      // histogram the next pass fills
+  int scan;                 // the pass computed keys: scan their histogram
+  int inplace;              // the pass kept every particle in its slot
   int kg;                   // grouping whose keys were just written (dom->kgrid)
   int ntiles;
   int nx, ny, nz;
@@ -710,7 +784,7 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
     DevScal &S = *f.scal;
     // the keys the pass just wrote were computed in the domain as it stands
     // before this step's update; k_prologue_keys used the unpadded box
-    {
+    if (f.scan) {
       DevDom &D = *f.dom;
       const int nn[3] = {f.nx, f.ny, f.nz};
       for (int a = 0; a < 3; a++) {
@@ -727,7 +801,8 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
       const double *cv = red + kMoments;
       const int64_t alive_in = (int64_t)cv[7];
       const int64_t nnew = alive_in - (int64_t)cv[6];
-      S.nslots = alive_in;
+      // a scatter writes one slot per particle it took; in place the slots stay
+      if (!f.inplace) S.nslots = alive_in;
       S.nalive = nnew;
       // conv1st: Current = Current*nptot/Npt (src/Appl/BeamBunch.f90:1657-1663)
       if (S.npt > 0) S.current = S.current * (double)nnew / S.npt;
@@ -772,6 +847,7 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
     }
   }
   __syncthreads();
+  if (!f.scan) return;
   // ---- exclusive scan of the tile histogram and the chunk list (after the
   // last pass: of the keys the next run starts from)
   const int per = (f.ntiles + blockDim.x - 1) / blockDim.x;
@@ -897,9 +973,20 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
         p.m[4 * blk + 3] = xm[6 * (r + 1) + r + 1];
       }
     }
+  } else if (m1 == kMap1QuadK) {
+    // transfmapK_Quadrupole (src/Appl/Quadrupole.f90:509-607), per particle
+    const double r6 = b->refpt[5];
+    kind = kPushQuadK;
+    p.m[0] = sg.tau1;
+    p.m[1] = sg.e.v[0];
+    p.m[2] = b->charge / b->mass;
+    p.m[3] = -r6;
+    p.m[4] = std::sqrt(r6 * r6 - 1.0);
+    p.m[5] = std::sqrt(1.0 - 1.0 / (r6 * r6));
+    p.m[6] = b->c.scxl;
+    b->refpt[4] = b->refpt[4] + sg.tau1 / p.m[5] / b->c.scxl;
   } else {
-    return fail(IZH_ERR_ARG, "izh_run: bitype %d: %s", sg.e.bitype,
-                why ? why : "the K-mode quadrupole runs through izh_step");
+    return fail(IZH_ERR_ARG, "izh_run: bitype %d: %s", sg.e.bitype, why);
   }
   *z = *z + sg.tau1;
   return IZH_OK;
@@ -927,13 +1014,19 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
   const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
   const int nb = (grid_for(b->n) + 7) / 8 * 8;  // multiple of 8 for block_range
+  const int period = b->resort_period > 0 ? b->resort_period : kDefaultResortPeriod;
   int rc = IZH_OK;
-  // The previous run on this bunch and field left sort keys, their scan and
-  // the first deposit chunks (in chunk slot b->keys_po): start from them.
+  // Two chunk lists (with their key grids and counts) describe groupings of
+  // the slots: g_cur the order the particles are in, g_next the order the
+  // keys of the last scan put them in at the next scatter.  The previous run
+  // on this bunch and field left its last scan (keys, cursors, chunks) in
+  // slot b->keys_po: start from it.
   const bool warm = b->keys_valid && b->keys_field == f && b->keys_grid[0] == nx &&
                     b->keys_grid[1] == ny && b->keys_grid[2] == nz;
-  const int po = warm ? b->keys_po : 0;  // chunk slot of deposit(s) = (s + po) & 1
+  int g_next = warm ? b->keys_po : 0;
+  int g_cur = 1 - g_next;
   b->keys_valid = 0;
+  uint32_t *hist = b->hist[0];
   // ---- device scalars
   DevScal hs{};
   if (warm) {
@@ -946,9 +1039,6 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   hs.nslots = b->n;
   *b->h_scal = hs;
   IZH_HIP(hipMemcpyAsync(b->d_scal, b->h_scal, sizeof hs, hipMemcpyHostToDevice, st));
-  // ---- prologue: initial keys, their histogram, cursors and chunks
-  hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st,
-                     b->hist[warm ? 1 : 0], (int64_t)ntiles);
   FinCfg fc{};
   fc.part = b->d_part;
   fc.nparts = nb;
@@ -959,36 +1049,46 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
   fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
   fc.cursor = b->cursor;
+  fc.hist = hist;
+  fc.hist_zero = hist;
+  // ---- prologue: initial keys, their histogram, cursors and chunks
   if (!warm) {
-  hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
-                     b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
-                     b->c.scxl, b->d_part);
-  IZH_CHECK(launch_reduce(st, b->d_part, nb, 6, 0x7ull, 0x38ull, b->d_out));
-  hipLaunchKernelGGL(k_prologue_keys, dim3(nb), dim3(kBlock), 0, st, b->r[0],
-                     b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
-                     b->c.scxl, b->d_out, nx, ny, nz, s0, s1, s2, tnx, tny,
-                     b->key, b->hist[0]);
-  hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
-                     nx, ny, nz);
-  IZH_HIP(hipGetLastError());
-  fc.mode = 0;
-  fc.kg = 0;
-  fc.prologue_box = b->d_out;
-  fc.hist = b->hist[0];
-  fc.chunks = b->chunks;
-  fc.nchunks = &b->d_scal->nchunks[0];
-  fc.hist_zero = b->hist[1];
-  {
-    ScopedTimer tm_(IZH_T_FINAL, st);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st, hist,
+                       (int64_t)ntiles);
+    hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                       b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                       b->c.scxl, b->d_part);
+    IZH_CHECK(launch_reduce(st, b->d_part, nb, 6, 0x7ull, 0x38ull, b->d_out));
+    hipLaunchKernelGGL(k_prologue_keys, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                       b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                       b->c.scxl, b->d_out, nx, ny, nz, s0, s1, s2, tnx, tny,
+                       b->key, hist);
+    hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
+                       nx, ny, nz);
+    IZH_HIP(hipGetLastError());
+    fc.mode = 0;
+    fc.scan = 1;
+    fc.inplace = 0;
+    fc.kg = g_next;
+    fc.prologue_box = b->d_out;
+    fc.chunks = b->chunks + g_next * b->nchunks_cap;
+    fc.nchunks = &b->d_scal->nchunks[g_next];
+    {
+      ScopedTimer tm_(IZH_T_FINAL, st);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+    }
+    IZH_HIP(hipGetLastError());
   }
-  }  // !warm
-  IZH_HIP(hipGetLastError());
-  // ---- the steps: pass s finishes step s-1 and starts step s
+  // ---- the steps: pass s finishes step s-1 and starts step s.  Passes
+  // 0 and nseg and every period-th pass between scatter the particles into
+  // tile order; the others write them back in place.  A pass before a
+  // scatter computes the keys it scatters by.
+  auto scatters = [&](int64_t s) { return s == 0 || s == nseg || s % period == 0; };
   PassCfg pc{};
   pc.cap = b->cap;
   pc.d_nslots = &b->d_scal->nslots;
   pc.cursor = b->cursor;
+  pc.hist_new = hist;
   pc.dom = f->d_dom;
   pc.e = f->e;
   pc.scal = b->d_scal;
@@ -1003,6 +1103,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   pc.tpos = b->tpos;
   pc.part = b->d_part;
   pc.sink = b->sink;
+  pc.qm0 = b->qm0;
+  pc.qm_value = b->qm_value;
   HalfPush pb_prev{};
   int kb_prev = kPushNone;
   double tau2_prev = 0.0;
@@ -1010,6 +1112,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   std::vector<double> refpt_after(2 * (size_t)nseg);  // refpt(5:6) after each step
   for (int64_t s = 0; s <= nseg && rc == IZH_OK; s++) {
     const bool last = s == nseg;
+    const bool scatter = scatters(s);
+    const bool keys = last || scatters(s + 1);
     HalfPush pa{};
     int ka = kPushNone;
     if (!last) IZH_CHECK(segment_push(b, segs[s], pa, ka, z));  // first half of step s
@@ -1020,48 +1124,56 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     gam_sc = pc.gam;
     fc.gam = pc.gam;
     pc.in = b->r[0];
-    pc.out = b->r_alt[0];
+    pc.out = scatter ? b->r_alt[0] : b->r[0];
     pc.in_orig = b->orig;
     pc.out_orig = b->orig_alt;
     pc.in_key = b->key;
-    pc.out_key = b->key_alt;
-    pc.hist_new = b->hist[(s + 1) & 1];
-    // the input slots: the output of pass s-1, grouped by the keys of pass
-    // s-2 (or of the prologue), which deposit s-1 used too
-    pc.in_chunks = b->chunks + ((s + 1 + po) & 1) * b->nchunks_cap;
-    pc.in_nchunks = &b->d_scal->nchunks[(s + 1 + po) & 1];
-    pc.kgrid = &f->d_dom->kgrid[(s + 1 + po) & 1];
+    pc.out_key = scatter ? b->key_alt : b->key;
+    // the input slots are grouped as g_cur says (the last scatter's keys)
+    pc.in_chunks = b->chunks + g_cur * b->nchunks_cap;
+    pc.in_nchunks = &b->d_scal->nchunks[g_cur];
+    pc.kgrid = &f->d_dom->kgrid[g_cur];
+    pc.kb = kb_prev;
+    pc.ka = ka;
     pc.pb = pb_prev;
     pc.pa = pa;
+    pc.need_orig = (kb_prev == kPushQuadK || ka == kPushQuadK) && !b->qm_uniform;
     pc.tau2_prev = tau2_prev;
     pc.rad = last ? 0.0 : 0.5 * (segs[s].xrad + segs[s].yrad);
+    const int mode = (scatter ? kPassScatter : 0) | (keys ? kPassKeys : 0);
     {
       ScopedTimer tm_(IZH_T_PASS, st);
       if (f->precision == IZH_FP64)
-        launch_pass<double>(kb_prev, ka, nb, st, pc);
+        launch_pass<double>(mode, nb, st, pc);
       else
-        launch_pass<float>(kb_prev, ka, nb, st, pc);
+        launch_pass<float>(mode, nb, st, pc);
     }
     IZH_HIP(hipGetLastError());
-    for (int r = 0; r < 6; r++) std::swap(b->r[r], b->r_alt[r]);
-    std::swap(b->orig, b->orig_alt);
-    std::swap(b->key, b->key_alt);
+    if (scatter) {
+      for (int r = 0; r < 6; r++) std::swap(b->r[r], b->r_alt[r]);
+      std::swap(b->orig, b->orig_alt);
+      std::swap(b->key, b->key_alt);
+      g_cur = g_next;   // the slots now follow the scanned keys
+    }
     fc.mode = last ? 2 : 1;
-    fc.kg = (int)((s + 1 + po) & 1);
+    fc.scan = keys ? 1 : 0;
+    fc.inplace = scatter ? 0 : 1;
     fc.moments_out = s > 0 ? b->d_moments + (s - 1) * kRunRecord : nullptr;
-    fc.hist = b->hist[(s + 1) & 1];
-    fc.chunks = b->chunks + ((s + 1 + po) & 1) * b->nchunks_cap;
-    fc.nchunks = &b->d_scal->nchunks[(s + 1 + po) & 1];
-    fc.hist_zero = b->hist[s & 1];
+    if (keys) {
+      g_next = 1 - g_cur;
+      fc.kg = g_next;
+      fc.chunks = b->chunks + g_next * b->nchunks_cap;
+      fc.nchunks = &b->d_scal->nchunks[g_next];
+    }
     {
       ScopedTimer tm_(IZH_T_FINAL, st);
       hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
     }
     IZH_HIP(hipGetLastError());
     if (last) break;
-    // charge, update3O and the E field of step s on the grouping just written
-    rc = launch_deposit(b, f, b->chunks + ((s + po) & 1) * b->nchunks_cap,
-                        &f->d_dom->kgrid[(s + po) & 1], &b->d_scal->nchunks[(s + po) & 1],
+    // charge, update3O and the E field of step s on the grouping the slots are in
+    const Chunk *dch = b->chunks + g_cur * b->nchunks_cap;
+    rc = launch_deposit(b, f, dch, &f->d_dom->kgrid[g_cur], &b->d_scal->nchunks[g_cur],
                         b->nchunks_cap);
     if (rc == IZH_OK) rc = launch_solve(f);
     if (rc == IZH_OK) rc = launch_efield(f);
@@ -1071,8 +1183,8 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
       WakeParams wp;
       rc = wake_params_of(b, f, segs[s].wake, &wp);
       if (rc == IZH_OK)
-        rc = launch_wake(b, f, wp, b->d_scal, b->chunks + ((s + po) & 1) * b->nchunks_cap,
-                         &b->d_scal->nchunks[(s + po) & 1], b->nchunks_cap);
+        rc = launch_wake(b, f, wp, b->d_scal, dch, &b->d_scal->nchunks[g_cur],
+                         b->nchunks_cap);
       if (rc == IZH_OK) rc = launch_wake_add(f);
     }
     // second half of step s: host bookkeeping now, applied by the next pass
@@ -1104,7 +1216,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   b->n = out.nslots;
   b->npt = (int64_t)out.npt;
   b->current = out.current;
-  b->run_stats[0] = out.nchunks[(nseg + 1 + po) & 1];
+  b->run_stats[0] = out.nchunks[g_next];
   b->run_stats[1] = out.overflow;
   b->run_stats[2] = out.nalive;
   b->run_stats[3] = ntiles;
@@ -1112,7 +1224,7 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
   for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
   f->have_phi = 1;
   b->keys_valid = 1;
-  b->keys_po = (int)((nseg + 1 + po) & 1);
+  b->keys_po = g_next;
   b->keys_field = f;
   b->keys_grid[0] = nx; b->keys_grid[1] = ny; b->keys_grid[2] = nz;
   return IZH_OK;

@@ -44,6 +44,9 @@ constexpr int kWindowCells = 4096;   // LDS deposit window (int64 cells)
 constexpr int kTileShift = 3;        // sort tiles of 8x8x8 cells
 constexpr uint32_t kDeadKey = 0xffffffffu;
 constexpr int kSinkBytes = 64 * 8 * 16;  // one 8-byte word per lane, 16 rows
+// Passes of a fused run between two counting-sort scatters when the bunch's
+// tuning leaves it open (izh_tuning.resort_period = 0).
+constexpr int kDefaultResortPeriod = 1;
 
 // Frame of the particle coordinates held by a bunch; the reference's call
 // order moves the bunch through these (src/Contrl/AccSimulator.f90:1110-1429).
@@ -149,6 +152,7 @@ struct izh_bunch {
   double *d_moments = nullptr;              // per-step moments of a run
   int64_t moments_cap = 0;
   int64_t run_stats[4] = {0, 0, 0, 0};      // izh_run_stats
+  int resort_period = 0;                    // izh_tuning (0: the default)
   // Sort keys, their scan (cursor) and first deposit chunks left by the last
   // izh_run for the next one; any other change to the particles clears this.
   int keys_valid = 0;

@@ -28,12 +28,16 @@ __global__ void k_zero_f64(double *__restrict__ p, int64_t n) {
 }
 
 // Bins of one pass: rows 0..Nz of unit weight, x-weighted and y-weighted
-// sums (one extra row catches a particle on the last mesh line).
+// sums.  A particle on the last mesh line (iz = Nz, ab = 1) adds its zero
+// upper weight to the extra row Nz, where the reference's loop writes it one
+// past densz(Nz) (src/Contrl/AccSimulator.f90:1284-1294); the domain's pad
+// keeps every particle of a Flagbc=1 step below that line, so the clamp only
+// guards memory.
 __device__ inline void wake_bin(double x, double y, double z, double zmin, double hz,
                                 int nz, double *lds) {
   int iz = (int)((z - zmin) / hz + 1);
   const double ab = ((zmin - z) + iz * hz) / hz;
-  iz = min(max(iz, 1), nz - 1);
+  iz = min(max(iz, 1), nz);
   const int s = nz + 1;
   atomicAdd(&lds[iz - 1], ab);
   atomicAdd(&lds[iz], 1.0 - ab);

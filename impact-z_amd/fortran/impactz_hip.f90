@@ -34,6 +34,12 @@ module impactz_hip
   end type izh_bunch_info
 
   ! CompDom for one process (src/Appl/CompDom.f90:19-32)
+  ! izh_tuning: how izh_run works on a bunch (no reference counterpart)
+  type, bind(C) :: izh_tuning
+    integer(c_int) :: resort_period = 0
+    integer(c_int) :: reserved(7) = 0
+  end type izh_tuning
+
   type, bind(C) :: izh_compdom
     integer(c_int) :: nx, ny, nz
     real(c_double) :: range(6)
@@ -188,12 +194,13 @@ module impactz_hip
       type(izh_element), intent(in) :: e
       integer(c_int) :: rc
     end function
-    function izh_isr_chicane(b, tau, e, rd) bind(C, name="izh_isr_chicane") result(rc)
-      import :: c_int, c_double, c_ptr, izh_element
+    function izh_isr_chicane(b, tau, e, rd, nrd) bind(C, name="izh_isr_chicane") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr, izh_element
       type(c_ptr), value :: b
       real(c_double), value :: tau
       type(izh_element), intent(in) :: e
       type(c_ptr), value :: rd
+      integer(c_int64_t), value :: nrd
       integer(c_int) :: rc
     end function
     ! read1_Data's table (rows z, Ez, Ez', Ez'') / read3_Data's coefficients
@@ -356,6 +363,18 @@ module impactz_hip
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: b
       real(c_double) :: sums(37), amax(6), r2max
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_set_tuning(b, t) bind(C, name="izh_bunch_set_tuning") result(rc)
+      import :: c_int, c_ptr, izh_tuning
+      type(c_ptr), value :: b
+      type(izh_tuning) :: t
+      integer(c_int) :: rc
+    end function
+    function izh_bunch_get_tuning(b, t) bind(C, name="izh_bunch_get_tuning") result(rc)
+      import :: c_int, c_ptr, izh_tuning
+      type(c_ptr), value :: b
+      type(izh_tuning) :: t
       integer(c_int) :: rc
     end function
     function izh_chgupdate(b, nchge, qmcclist, counts) &

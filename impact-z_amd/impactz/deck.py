@@ -209,8 +209,15 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
     if not 1 <= deck.nchrg <= 100:
         raise DeckError(f"{deck.nchrg} charge states: the reference holds 1 to 100")
     out, notes = [], []
+    # whether the run loop has read Fourier coefficients (read3_Data) into
+    # the Data module's Fcoef by the time it reaches an element: SolRF under
+    # either integrator, every cavity under Flagmap=2
+    # (src/Contrl/AccSimulator.f90:818-849)
+    fcoef_read = False
     for k, e in enumerate(deck.elements, start=1):
         v = e.values + [0.0] * 4
+        if e.type > 100 and (e.type in (105, 106) or flagmap != 1):
+            fcoef_read = True
         if e.type == 0:
             out.append((k, Element(e.length, e.nseg, e.mapstp, 0, v[0])))
         elif e.type == 1:
@@ -248,6 +255,14 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             vv = (v + [0.0] * 14)[:14]
             if not (e.length > 0.0 and vv[0] != 0.0):
                 raise DeckError(f"element {k}: a bend needs a length > 0 and an angle != 0")
+            # under Flagmap=2 the loop also kicks a bend with getfld_Dipole,
+            # whose pole-face geometry is the Data module's Fcoef
+            # (src/Appl/Dipole.f90:361-414): zero, and the kick the identity,
+            # until a cavity's Fourier coefficients have been read into it
+            if flagmap == 2 and fcoef_read:
+                raise DeckError(f"element {k}: a bend under Flagmap=2 after an RF cavity's "
+                                "Fourier coefficients were read (getfld_Dipole then reads "
+                                "them as its pole-face geometry) is not on this path")
             out.append((k, Element(e.length, e.nseg, e.mapstp, 4, *vv[:4],
                                    more=tuple(vv[4:]))))
         elif e.type == -41:

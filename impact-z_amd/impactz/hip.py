@@ -36,7 +36,8 @@ SYMBOLS = (
     "izh_map1", "izh_fpol_dipole", "izh_sector_dipole", "izh_bpol_dipole",
     "izh_isr_chicane", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
     "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
-    "izh_wake_download", "izh_kick2wake", "izh_chgupdate",
+    "izh_wake_download", "izh_kick2wake", "izh_chgupdate", "izh_bunch_set_tuning",
+    "izh_bunch_get_tuning",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -132,6 +133,11 @@ def segment(tau1: float, bitype: int, v, mapstp: int, xrad: float, yrad: float,
     return sg
 
 
+class TuningC(C.Structure):
+    """izh_tuning: how izh_run works on a bunch (results do not depend on it)."""
+    _fields_ = [("resort_period", C.c_int), ("reserved", C.c_int * 7)]
+
+
 class CompDomC(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("range", C.c_double * 6), ("msize", C.c_double * 3)]
@@ -170,7 +176,7 @@ def lib() -> C.CDLL:
     L.izh_fpol_dipole.argtypes = [vp, C.POINTER(ElementC)]
     L.izh_bpol_dipole.argtypes = [vp, C.POINTER(ElementC)]
     L.izh_sector_dipole.argtypes = [vp, _dp, C.c_double, C.POINTER(ElementC)]
-    L.izh_isr_chicane.argtypes = [vp, C.c_double, C.POINTER(ElementC), vp]
+    L.izh_isr_chicane.argtypes = [vp, C.c_double, C.POINTER(ElementC), vp, C.c_int64]
     L.izh_rfdata_table.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_fourier.argtypes = [_dp, C.c_int, C.POINTER(vp)]
     L.izh_rfdata_destroy.argtypes = [vp]
@@ -198,6 +204,8 @@ def lib() -> C.CDLL:
     L.izh_kick1.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
     L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
     L.izh_chgupdate.argtypes = [vp, C.c_int, _dp, _i64p]
+    L.izh_bunch_set_tuning.argtypes = [vp, C.POINTER(TuningC)]
+    L.izh_bunch_get_tuning.argtypes = [vp, C.POINTER(TuningC)]
     L.izh_field_download.argtypes = [vp, C.c_int, _dp]
     L.izh_field_upload_rho.argtypes = [vp, _dp]
     L.izh_step.argtypes = [vp, vp, C.POINTER(CompDomC), _dp, C.POINTER(Segment),
@@ -482,11 +490,11 @@ class BeamBunch:
         check(lib().izh_sector_dipole(self.h, C.byref(zz), tau, C.byref(element(4, v, length))))
         return zz.value
 
-    def isr_chicane(self, tau: float, length: float, v, rd_ptr: int) -> None:
-        """ISRchicane over tau with the device deviates at rd_ptr (one double
-        per particle, by row at upload)."""
+    def isr_chicane(self, tau: float, length: float, v, rd_ptr: int, nrd: int) -> None:
+        """ISRchicane over tau with the nrd device deviates at rd_ptr (one
+        double per particle of the upload, by its row there)."""
         check(lib().izh_isr_chicane(self.h, tau, C.byref(element(4, v, length)),
-                                    C.c_void_p(rd_ptr)))
+                                    C.c_void_p(rd_ptr), nrd))
 
     def drift2(self, z: float, tau: float) -> float:
         """drift2_BeamBunch (src/Appl/BeamBunch.f90:467-509): a drift of tau/2; returns z."""
@@ -552,6 +560,19 @@ class BeamBunch:
         s, m, r2 = np.zeros(37), np.zeros(6), C.c_double()
         check(lib().izh_diagnostic1(self.h, _p(s), _p(m), C.byref(r2)))
         return s, m, r2.value
+
+    @property
+    def resort_period(self) -> int:
+        """Passes of izh_run between counting-sort scatters (izh_tuning)."""
+        t = TuningC()
+        check(lib().izh_bunch_get_tuning(self.h, C.byref(t)))
+        return t.resort_period
+
+    @resort_period.setter
+    def resort_period(self, period: int) -> None:
+        t = TuningC()
+        t.resort_period = int(period)
+        check(lib().izh_bunch_set_tuning(self.h, C.byref(t)))
 
     def chgupdate(self, qmcclist) -> np.ndarray:
         """chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): live particles

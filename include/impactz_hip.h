@@ -182,8 +182,11 @@ int izh_bpol_dipole(izh_bunch *b, const izh_element *e);
 /* ISRchicane (src/Contrl/AccSimulator.f90:1542-1584), the synchrotron
  * radiation of a bend with current, over tau.  rd: device array of one
  * standard-normal deviate per particle, indexed by the particle's row at
- * izh_bunch_create (the reference draws them with normVec1). */
-int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd);
+ * izh_bunch_create or the last izh_bunch_upload (the reference draws them
+ * with normVec1); nrd: its length, at least the particles of that upload
+ * (losses since do not shorten it), else IZH_ERR_ARG. */
+int izh_isr_chicane(izh_bunch *b, double tau, const izh_element *e, const double *rd,
+                    int64_t nrd);
 
 /* izh_map1 for bitype 0 and 1 with v = {gradient, 0, ...}. */
 int izh_drift1(izh_bunch *b, double *z, double tau, int bitype,
@@ -330,11 +333,12 @@ int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double ta
 
 /* nseg consecutive izh_step calls (the element/segment loop of
  * run_AccSimulator, src/Contrl/AccSimulator.f90:742-1504; every element of
- * izh_map1 except the K-mode quadrupole, which runs through izh_step) run as one
+ * izh_map1, the K-mode quadrupole's chromatic map included) run as one
  * device-resident pipeline: each step's kick, second half push and moment
  * sums are fused with the next step's first half push and z->t transform in
- * one pass over the particles, which are re-sorted into 8x8x8-cell tiles in
- * that same pass so the deposit accumulates in LDS; the domain update stays
+ * one pass over the particles, which are re-sorted into 8x8x8-cell tiles
+ * (every izh_tuning.resort_period-th pass, in that same pass) so the deposit
+ * accumulates in LDS; the domain update stays
  * on the device, so the host does not wait between steps.  Results equal
  * izh_step's to rounding.  moments (NULL or IZH_RUN_RECORD*nseg doubles)
  * receives per step the 44 diagnostic1_Output values izh_step gives, then
@@ -345,6 +349,20 @@ int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double ta
 int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             const izh_segment *segs, int64_t nseg, double bcurr,
             double *moments);
+
+/* Tuning of izh_run on a bunch (no reference counterpart; the results do
+ * not depend on it beyond the order of the moment sums).  resort_period: a
+ * run re-sorts the particles into grid tiles (a counting-sort scatter) at
+ * its first and last pass and every resort_period-th pass between; the
+ * passes in between write each particle back to its own slot, so the
+ * deposit and the gather work on the grouping of the last sort (1: sort
+ * every pass; 0: the library's default). */
+typedef struct {
+  int resort_period;
+  int reserved[7];
+} izh_tuning;
+int izh_bunch_set_tuning(izh_bunch *b, const izh_tuning *t);
+int izh_bunch_get_tuning(const izh_bunch *b, izh_tuning *t);
 
 /* Counters of the last izh_run on this bunch (no reference counterpart;
  * for tuning): stats[0] deposit chunks of the grouping the run ends with

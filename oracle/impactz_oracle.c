@@ -872,8 +872,11 @@ static void wake_line(const double *pts, int64_t n, const double range[6], int n
   for (i = 0; i < n; i++) {
     int iz = (int)((P(5, i) - range[4]) / hz + 1);
     double ab = ((range[4] - P(5, i)) + iz * hz) / hz;
+    /* a particle on the last mesh line (iz = Nz, ab = 1) puts its zero
+     * upper weight in the extra row nz, as the reference's loop writes it
+     * one past densz(Nz) */
     if (iz < 1) iz = 1;
-    if (iz > nz - 1) iz = nz - 1;
+    if (iz > nz) iz = nz;
     xw[iz - 1] += P(1, i) * ab;
     xw[iz] += P(1, i) * (1.0 - ab);
     yw[iz - 1] += P(3, i) * ab;

@@ -281,3 +281,25 @@ def test_deck_bends_at_zero_current_only(tmp_path):
         else:
             with pytest.raises(DeckError, match="bend"):
                 program_of(deck)
+
+
+def test_deck_refuses_lorentz_bend_after_fourier_coefficients(tmp_path):
+    """A Flagmap=2 bend runs as its two Sector_Dipole halves because
+    getfld_Dipole's geometry (the Data module's Fcoef) is zero; once a
+    cavity's Fourier coefficients have been read, it is not, and the deck
+    is refused.  The same bend before the cavity, or under Flagmap=1, runs."""
+    from impactz.deck import DeckError, program_of, read_deck
+    p = tmp_path / "ImpactZ.in"
+    hdr = HEADER.replace("0.1 1.5d8 938.272088d6 1.0 324.0e6 0.0",
+                         "0.0 1.5d8 938.272088d6 1.0 324.0e6 0.0")
+    bend = "0.5 4 1 4 0.1 0.0 0.0 0.02 /\n"
+    cav = "0.2 4 20 101 1.0e6 324.0e6 0.0 1.0 1.0 /\n"
+    for flagmap, lines, refused in ((2, cav + bend, True), (2, bend + cav, False),
+                                   (1, cav + bend, False)):
+        _write(p, hdr.replace("6 3 1 0 1", f"6 3 {flagmap} 0 1") + lines)
+        deck = read_deck(str(p))
+        if refused:
+            with pytest.raises(DeckError, match="Fourier coefficients"):
+                program_of(deck)
+        else:
+            program_of(deck)

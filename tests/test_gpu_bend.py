@@ -45,7 +45,10 @@ def test_bend_entry_points_match_oracle(gpu, oracle):
         for _ in range(3):
             zo = oracle.sector_dipole(cst, ref, refpt, zo, tau, L, BEND, beam.charge,
                                       beam.mass)
-        b.isr_chicane(tau, L, BEND, rd.value)
+        # fewer deviates than uploaded particles: refused before any launch
+        with pytest.raises(hip.IZHError, match="deviates"):
+            b.isr_chicane(tau, L, BEND, rd.value, pts.shape[0] - 1)
+        b.isr_chicane(tau, L, BEND, rd.value, pts.shape[0])
         oracle.isr_chicane(cst, ref, refpt, tau, L, BEND, beam.mass, rd_host)
         b.bpol_dipole(L, BEND)
         oracle.dipole_face(cst, ref, refpt, L, BEND, True, beam.charge, beam.mass)

@@ -39,15 +39,23 @@ def test_fused_matches_stepwise_100k_32cube(gpu):
     assert b.bunch.nptlocal == a.bunch.nptlocal == 100_000
 
 
-@pytest.mark.parametrize("name", ["fodo_traj", "fodo_16cube", "fodo_aniso", "fodo_lossy",
-                                  "cf_sol", "rf"])
-def test_fused_matches_reference(gpu, name):
+@pytest.mark.parametrize("name,period", [("fodo_traj", 1), ("fodo_traj", 4),
+                                         ("fodo_16cube", 1), ("fodo_aniso", 1),
+                                         ("fodo_lossy", 1), ("fodo_lossy", 3),
+                                         ("cf_sol", 1), ("cf_sol", 5), ("rf", 1), ("rf", 4),
+                                         ("kquad", 1), ("kquad", 4)])
+def test_fused_matches_reference(gpu, name, period):
+    """izh_run against the reference, re-sorting every pass or every
+    period-th pass (the passes between keep the particles in their slots,
+    lost ones marked in place)."""
     from impactz.sim import Simulation
     d = load_golden(name)
     grid = tuple(int(v) for v in d["grid"])
     sim = Simulation(d["pts_in"], fixture_beam(d), fixture_lattice(d), grid)
+    sim.bunch.resort_period = period
+    assert sim.bunch.resort_period == period
     nsteps = int(d["nsteps"])
-    print(f"fused run {name}", flush=True)
+    print(f"fused run {name} resort period {period}", flush=True)
     rec = sim.run_fused(nsteps)
     worst = 0.0
     for k in range(nsteps):
@@ -56,8 +64,8 @@ def test_fused_matches_reference(gpu, name):
     assert fin.shape == d["final"].shape
     pe = col_err(fin, d["final"])
     ce = abs(sim.bunch.current / float(d["final_current"]) - 1)
-    record(test="fused_vs_reference", case=name, steps=nsteps, moments=worst,
-           particles=pe, current=ce)
+    record(test="fused_vs_reference", case=name, period=period, steps=nsteps,
+           moments=worst, particles=pe, current=ce)
     assert worst < TOL and pe < TOL and ce < 1e-14
     # the reference energy of every step (RF cavities change it)
     assert np.allclose(-rec[:, 46], d["fort18"][1:nsteps + 1, 2], rtol=1e-13)
@@ -112,17 +120,28 @@ def test_fused_mixed_precision_error(gpu):
     assert worst < 1e-4 and fl2 < 1e-3
 
 
-def test_fused_refuses_kmode_quadrupole(gpu):
-    """The chromatic K-mode map is not a fused push kind: izh_run says so and
-    leaves the bunch as it was; izh_step runs it."""
+def test_fused_kmode_quadrupole_with_varying_charge(gpu):
+    """The K-mode quadrupole's chromatic map as a push kind of izh_run, on a
+    bunch whose particles carry three q/mc^2 (each particle's own strength):
+    equal to izh_step, with every re-sort period."""
     from impactz import beam as B
     from impactz.sim import Simulation
     bm = B.Beam()
-    pts = B.gaussian_bunch(4000, seed=2)
-    lat = [B.Element(0.1, 4, 10, 1, 5.4, -1.0, 1.0)]
-    sim = Simulation(pts, bm, lat, (16, 16, 16))
-    with pytest.raises(gpu.IZHError, match="K-mode"):
-        sim.run_fused(4)
-    assert col_err(sim.bunch.getpts(), pts) == 0.0
-    rows = sim.run(nsteps=4)
-    assert rows.shape[0] == 5
+    pts = B.gaussian_bunch(20_000, seed=2)
+    q = bm.charge / bm.mass
+    pts[:, 6] = q * np.array([1.0, 0.5, 0.25])[np.arange(pts.shape[0]) % 3]
+    lat = [B.Element(0.1, 4, 10, 1, 5.4, -1.0, 1.0), B.Element(0.4, 4, 10, 0, 1.0),
+           B.Element(0.1, 4, 10, 1, -5.4, -1.0, 1.0), B.Element(0.4, 4, 10, 0, 1.0)]
+    a = Simulation(pts, bm, lat, (16, 16, 16))
+    rows_a = [a.diag(a.step(e)) for e in a.segments()]
+    pa = a.bunch.getpts()
+    for period in (1, 3):
+        b = Simulation(pts, bm, lat, (16, 16, 16))
+        b.bunch.resort_period = period
+        rec = b.run_fused(16)
+        worst = max(full_row_err(b.diag(rec[k]), rows_a[k]) for k in range(16))
+        pe = col_err(b.bunch.getpts(), pa)
+        record(test="fused_kmode_vs_step", period=period, moments=worst, particles=pe)
+        assert worst < TOL and pe < TOL
+        b.close()
+    a.close()

@@ -608,6 +608,31 @@ def test_oracle_wake_trajectory_matches_reference(oracle, name):
     assert col_err(run.particles(), d["final"]) < 1e-12
 
 
+def test_wake_particle_on_the_last_mesh_line(oracle):
+    """A particle at z = range(6), on the last mesh line: the reference's loop
+    gives it weight 1 on line Nz (and its zero upper weight one row past the
+    array), so the line density of particles on lines 1..Nz-1 (0-based) is
+    uniform up to and including the last line."""
+    from impactz.beam import Beam
+    b = Beam()
+    c = oracle.physconst(b.freq)
+    nz = 16
+    tab = np.array([[0.0, 5.0e13, 1.0e18, 2.0e18], [1.0, 5.0e13, 1.0e18, 2.0e18]])
+    w = oracle.make_wake(2, table=tab)
+    rng = np.array([-1, 1, -1, 1, 0.0, 0.15])
+    hz = (rng[5] - rng[4]) / (nz - 1)
+    pts = np.zeros((15, 9))
+    pts[:, 4] = rng[4] + hz * np.arange(1, 16)
+    pts[-1, 4] = rng[5]
+    out = oracle.wakefield(c, pts, rng, nz, b.current, 15, 2.0, b.charge, w)
+    rho = b.current / b.freq / 15 / (hz / 2.0)
+    lines = np.zeros(nz)
+    lines[1:16] = rho
+    want = np.array([-(0.5 * lines[k] + lines[k + 1:].sum()) * 5.0e13 * hz / 2.0
+                     for k in range(nz)])
+    assert np.abs(out[2] - want).max() <= 1e-13 * np.abs(want).max()
+
+
 def test_wake_convolution_closed_forms(oracle):
     """A constant longitudinal wake W0 over a bunch of uniform line density:
     E_z(k) = -W0 hz (rho/2 + rho * (number of bins ahead)); the transverse
