@@ -30,8 +30,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "impact-z_amd"))
 
+from impactz import autotune  # noqa: E402
 from impactz import beam as B  # noqa: E402
 from impactz import hip  # noqa: E402
+from impactz import scan  # noqa: E402
 from impactz.sim import Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
@@ -57,7 +59,8 @@ def algorithmic_bytes(kernel: str, n: int, g: int, form: str) -> float:
     raise ValueError(kernel)
 
 
-def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps):
+def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps,
+             resort_period=0):
     """Warm up, then time `steps` space-charge steps run through izh_run
     (each step: both half pushes, z->t, domain, deposit, Green function,
     FFT solve, E, gather/kick and the 37 moment sums, all on the device).
@@ -66,6 +69,8 @@ def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps)
     form = "fp64" if precision == hip.FP64 else "mixed"
     n = pts.shape[0]
     sim = Simulation(pts, beam, lattice, grid, precision=precision)
+    sim.bunch.resort_period = resort_period
+    resort_period = sim.bunch.resort_period
     recs = []
     log(f"{form}: warmup {warmup} steps")
     recs.append(sim.run_fused(warmup, start=0))
@@ -94,6 +99,7 @@ def run_form(pts, beam, lattice, grid, precision, steps, warmup, accuracy_steps)
     sim.close()
     return {
         "form": form,
+        "resort_period": resort_period,
         "ms_per_step": 1e3 * dt / steps,
         "value": n * steps / dt,
         "ms_per_step_with_timers": 1e3 * dt_timed / steps,
@@ -294,6 +300,54 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, form):
     return "\n".join(out)
 
 
+def scans(args) -> dict:
+    """The batched launch (64 bunches of 1e5-4e6 particles on grids 32^3 to
+    128^3, seed 7, 160 steps each, mixed form) and the served lattice scan
+    (Poisson arrivals at 20/s, seed 11, 1e5-1e7 particles, 32^3 to
+    128x128x256, 50-400 steps), both as izh_jobs with graph replays and the
+    autotuner's re-sort periods, on bunches taken from one resident pool."""
+    out = {}
+    pool_n = 20_000_000
+    log(f"scans: sampling a pool of {pool_n} particles")
+    pool = scan.Pool(pool_n, seed=1)
+    try:
+        if not args.no_batch:
+            reqs = scan.c4_requests(seed=7)
+            log(f"batched: {len(reqs)} bunches, {sum(r.n for r in reqs)} particles")
+            scan.run_batch(scan.c4_requests(seed=7, nbunch=8, steps=32), pool,
+                           autotune.tuner_for_requests)   # warm the kernels
+            b = scan.run_batch(reqs, pool, autotune.tuner_for_requests)
+            out["batched"] = {
+                "workload": "64 bunches, particles log-uniform in [1e5, 4e6], grids from "
+                            "{32^3, 64^3, 64x64x128, 128^3}, gradients U[5,15] T/m, seed 7, "
+                            "160 steps each, mixed form",
+                "value": b["value"], "unit": "particle-steps/s", "seconds": b["seconds"],
+                "particle_steps": b["particle_steps"], "graph_launches": b["graph_launches"]}
+            log(f"batched: {b['value']:.3e} particle-steps/s in {b['seconds']:.2f} s")
+        if args.serve_seconds > 0:
+            out["serving"] = {}
+            for form, prec in (("fp64", hip.FP64), ("mixed", hip.MIXED)):
+                reqs = scan.c5_requests(seed=11, duration=args.serve_seconds, precision=prec)
+                log(f"serving {form}: {len(reqs)} requests over {args.serve_seconds} s")
+                r = scan.serve(reqs, pool, int(args.serve_budget_gb * 1e9),
+                               autotune.tuner_for_requests)
+                offered = sum(q.n * q.steps for q in reqs) / args.serve_seconds
+                out["serving"][form] = {
+                    "requests": r["requests"], "p50_s": r["p50_s"], "p99_s": r["p99_s"],
+                    "mean_s": r["mean_s"], "value": r["value"], "unit": "particle-steps/s",
+                    "offered": offered, "seconds": r["seconds"], "peak_jobs": r["peak_jobs"],
+                    "peak_gb": r["peak_bytes"] / 1e9}
+                log(f"serving {form}: p50 {r['p50_s']:.3f} s, p99 {r['p99_s']:.3f} s, "
+                    f"{r['value']:.3e} particle-steps/s (offered {offered:.3e})")
+            out["serving"]["workload"] = (
+                f"Poisson arrivals at 20/s for {args.serve_seconds:g} s, seed 11; particles "
+                "log-uniform in [1e5, 1e7], grids 32^3..128x128x256, 50-400 steps, gradients "
+                "U[5,15] T/m; admission by HBM footprint, continuous batching")
+    finally:
+        pool.close()
+    return out
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-profiles":
         if len(sys.argv) != 6:
@@ -308,6 +362,14 @@ def main():
     ap.add_argument("--grid", default="128,128,256")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--resort-period", type=int, default=-1,
+                    help="passes between counting-sort scatters (-1: the autotuner's "
+                         "table, 0: the library default)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched launch (64 bunches, seed 7)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="arrival window of the served lattice scan per form (0: skip)")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--accuracy-steps", type=int, default=400,
                     help="steps both forms run (untimed past the timed ones) for the "
                          "mixed-vs-FP64 error report")
@@ -328,8 +390,11 @@ def main():
     res = {}
     for form in args.forms.split(","):
         prec = hip.FP64 if form == "fp64" else hip.MIXED
+        period = args.resort_period
+        if period < 0:
+            period = autotune.lookup(n, grid, prec)
         res[form] = run_form(pts, beam, lattice, grid, prec, args.steps, args.warmup,
-                             args.accuracy_steps)
+                             args.accuracy_steps, period)
         log(f"{form}: {res[form]['ms_per_step']:.3f} ms/step, "
             f"{res[form]['value']:.3e} particle-steps/s, kernels {res[form]['kernels_ms']}")
     del pts
@@ -350,7 +415,8 @@ def main():
         "data": "synthetic (seeded 6D Gaussian core + waterbag halo)",
         "config": {"workload": "large single bunch: 50M particles, 128x128x256 grid, "
                                "FODO lattice, one space-charge step per timed step",
-                   "particles": n, "grid": list(grid), "form": main_form},
+                   "particles": n, "grid": list(grid), "form": main_form,
+                   "resort_period": r["resort_period"]},
         "roofline": roofline(r, n, g),
         "kernels_ms": r["kernels_ms"],
         "deposit": r["deposit"],
@@ -363,6 +429,8 @@ def main():
             "roofline": roofline(m, n, g), "kernels_ms": m["kernels_ms"],
             "error_vs_fp64": mixed_error(r, m),
         }
+    if not args.no_batch or args.serve_seconds > 0:
+        out.update(scans(args))
     # one plain copy kernel of 1 GiB: the calibration of the byte counters
     # when this run is profiled with rocprofv3 --pmc
     out["copy_probe_ms_1GiB"] = hip.probe_copy(1 << 30)

@@ -58,6 +58,9 @@ bool timing_on() { return timers().on; }
 void timer_begin(int slot, hipStream_t s) {
   TimerState &t = timers();
   if (!t.on || slot < 0 || slot >= IZH_T_COUNT || t.is_open[slot]) return;
+  // launches captured into a graph run later, not between these events
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
   PendingTimer p{slot, nullptr, nullptr};
   if (hipEventCreate(&p.a) != hipSuccess) return;
   if (hipEventCreate(&p.b) != hipSuccess) {
@@ -316,6 +319,75 @@ int izh_bunch_create(const double *pts9, int64_t n, double current,
     izh_bunch_destroy(b);
     return rc;
   }
+  b->frame = kFrameZ;
+  *out = b;
+  return IZH_OK;
+}
+
+int izh_bunch_create_from(const izh_bunch *src, int64_t offset, int64_t n, double current,
+                          int precision, izh_bunch **out) {
+  if (!src || !out || offset < 0 || n < 1 || offset + n > src->n ||
+      (precision != IZH_FP64 && precision != IZH_MIXED))
+    return fail(IZH_ERR_ARG, "izh_bunch_create_from: bad argument (slots %lld..%lld of %lld)",
+                (long long)offset, (long long)(offset + n), (long long)src->n);
+  if (src->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_bunch_create_from: source is not in the z frame");
+  *out = nullptr;
+  izh_bunch *b = new izh_bunch();
+  IZH_HIP(hipGetDevice(&b->device));
+  b->precision = precision;
+  b->current = current;
+  b->mass = src->mass;
+  b->charge = src->charge;
+  b->freq = src->freq;
+  b->npt = n;
+  for (int k = 0; k < 6; k++) b->refpt[k] = src->refpt[k];
+  b->c = src->c;
+  int rc = IZH_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(IZH_ERR_HIP, "izh_bunch_create_from: stream creation failed");
+  }
+  if ((rc = bunch_alloc_live(b, n)) != IZH_OK ||
+      (rc = dmalloc(&b->d_part, (int64_t)kMaxRedBlocks * kPartStride)) != IZH_OK ||
+      (rc = dmalloc(&b->d_out, (int64_t)kPartStride + 8)) != IZH_OK ||
+      (rc = dmalloc(&b->qm0, n)) != IZH_OK || (rc = dmalloc(&b->w0, n)) != IZH_OK ||
+      (rc = dmalloc(&b->id0, n)) != IZH_OK) {
+    izh_bunch_destroy(b);
+    return rc;
+  }
+  if (hipHostMalloc((void **)&b->h_out, (kPartStride + 8) * sizeof(double),
+                    hipHostMallocDefault) != hipSuccess) {
+    izh_bunch_destroy(b);
+    return fail(IZH_ERR_NOMEM, "izh_bunch_create_from: pinned allocation failed");
+  }
+  // Pts1(8) = I / (f N) sign(q) for the new particle count (distribution
+  // readers, e.g. src/Appl/Distribution.f90:1660-1661)
+  const double w = current / src->freq / (double)n * (src->charge < 0 ? -1.0 : 1.0);
+  b->w_uniform = 1;
+  b->w_value = w;
+  b->wabs_sum = std::fabs(w) * (double)n;
+  b->qm_uniform = src->qm_uniform;
+  b->qm_value = src->qm_value;
+  // the source's stream must have finished with its rows before the copy
+  hipEvent_t ev;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+    izh_bunch_destroy(b);
+    return fail(IZH_ERR_HIP, "izh_bunch_create_from: event creation failed");
+  }
+  const bool joined = hipEventRecord(ev, src->stream) == hipSuccess &&
+                      hipStreamWaitEvent(b->stream, ev, 0) == hipSuccess;
+  (void)hipEventDestroy(ev);
+  if (!joined) {
+    izh_bunch_destroy(b);
+    return fail(IZH_ERR_HIP, "izh_bunch_create_from: stream join failed");
+  }
+  if ((rc = launch_clone(src, b, offset, n, w)) != IZH_OK) {
+    izh_bunch_destroy(b);
+    return rc;
+  }
+  b->n = n;
+  b->n0 = n;
   b->frame = kFrameZ;
   *out = b;
   return IZH_OK;

@@ -39,6 +39,8 @@ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftPoints = 2048;  // points per block in LDS
+constexpr int kFftPointsZ = 4096; // the fused z pass: 8 lines of 512 at 128x128x256
+constexpr int kFftMinLine = 8;    // shortest line (a mesh of 4 nodes doubled)
 
 enum Load { kLoadReal = 0, kLoadCplx = 1, kLoadEven = 2, kLoadHerm = 3 };
 enum Store { kStoreCplx = 0, kStoreReal = 1 };
@@ -70,8 +72,8 @@ __device__ inline int bitrev(int v, int bits) {
 // twiddles, in the same order, as the stage-by-stage transform.  tw: L/2
 // twiddles exp(-2 pi i k / L) in LDS.
 template <typename T, bool DIT, int K>
-__device__ inline void fft_run(cx<T> *buf, int L, int lines, const cx<T> *tw, bool inverse,
-                               int done) {
+__device__ inline void fft_run(cx<T> *buf, int L, int LP, int lines, const cx<T> *tw,
+                               bool inverse, int done) {
   constexpr int span = 1 << K;
   // DIF runs halves L/2 .. 1 downwards, DIT 1 .. L/2 upwards
   const int s = DIT ? (1 << done) : (L >> (done + K));  // group stride
@@ -81,7 +83,7 @@ __device__ inline void fft_run(cx<T> *buf, int L, int lines, const cx<T> *tw, bo
   for (int g = threadIdx.x; g < ngroups; g += kFftThreads) {
     const int lb = g / per_line, q = g - lb * per_line;
     const int blk = q / s, r = q - blk * s;
-    const int base = lb * L + blk * bsize + r;
+    const int base = lb * LP + blk * bsize + r;
     cx<T> v[span];
 #pragma unroll
     for (int j = 0; j < span; j++) v[j] = buf[base + j * s];
@@ -113,17 +115,17 @@ __device__ inline void fft_run(cx<T> *buf, int L, int lines, const cx<T> *tw, bo
 }
 
 template <typename T, bool DIT>
-__device__ void fft_stages(cx<T> *buf, int L, int log2L, int lines, const cx<T> *tw,
+__device__ void fft_stages(cx<T> *buf, int L, int LP, int log2L, int lines, const cx<T> *tw,
                            bool inverse) {
   int done = 0;
   while (done < log2L) {
     const int k = min(3, log2L - done);
     if (k == 3)
-      fft_run<T, DIT, 3>(buf, L, lines, tw, inverse, done);
+      fft_run<T, DIT, 3>(buf, L, LP, lines, tw, inverse, done);
     else if (k == 2)
-      fft_run<T, DIT, 2>(buf, L, lines, tw, inverse, done);
+      fft_run<T, DIT, 2>(buf, L, LP, lines, tw, inverse, done);
     else
-      fft_run<T, DIT, 1>(buf, L, lines, tw, inverse, done);
+      fft_run<T, DIT, 1>(buf, L, LP, lines, tw, inverse, done);
     done += k;
   }
 }
@@ -148,14 +150,18 @@ __device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &l
 // (kLoadReal) is separated from Z(k) and conj Z(L-k) when stored; two
 // Hermitian spectra (kLoadHerm) are combined as X + iY, whose inverse has
 // the two real outputs as its parts.
-template <typename T, int LOAD, int STORE, bool CONV, bool PAIR>
+// PTS points of B = PTS/L lines per block; each line is held in LDS at a
+// pitch of L+1 points, so lanes that walk across lines (a line stride
+// other than 1, or the fused z pass's multiply) do not all hit one bank.
+template <typename T, int LOAD, int STORE, bool CONV, bool PAIR, int PTS>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
-  __shared__ cx<T> buf[kFftPoints];
+  __shared__ cx<T> buf[PTS + PTS / kFftMinLine];
+  const int LP = p.L + 1;
   extern __shared__ __align__(16) unsigned char fft_dyn[];  // L/2 twiddles
   cx<T> *tw = reinterpret_cast<cx<T> *>(fft_dyn);
   // twiddles are written to LDS together with the line data below
-  constexpr int kTwPer = kFftPoints / 2 / kFftThreads;  // L/2 <= kFftPoints/2
+  constexpr int kTwPer = PTS / 2 / kFftThreads;  // L/2 <= PTS/2
   cx<T> twv[kTwPer];
 #pragma unroll
   for (int it = 0; it < kTwPer; it++) {
@@ -172,7 +178,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   const int line0 = blockIdx.x * p.B;
   // ---- load (zero padding / even extension / Hermitian completion): all of
   // a thread's loads are issued before the first LDS write
-  constexpr int kPer = kFftPoints / kFftThreads;  // B * L == kFftPoints
+  constexpr int kPer = PTS / kFftThreads;  // B * L == PTS
   cx<T> ld[kPer];
   int slot[kPer];
 #pragma unroll
@@ -181,7 +187,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     int lb, m;
     map_thread(t, p.se, p, lb, m);
     const int line = line0 + lb;
-    slot[it] = lb * p.L + m;
+    slot[it] = lb * LP + m;
     T re = (T)0, im = (T)0;
     if (PAIR) {
       const int l0 = 2 * line, l1 = l0 + 1;
@@ -240,37 +246,40 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   }
   __syncthreads();
   // fused z pass: fetch the line's G^ values now, so their latency hides
-  // behind the forward transform.  The forward spectrum is bit-reversed:
-  // position q holds frequency bitrev(q).
+  // behind the forward transform.  Lanes walk across the block's lines
+  // first (neighbouring x indices, neighbouring addresses of G^), then
+  // along the frequencies; the forward spectrum is bit-reversed, so natural
+  // frequency kz sits at position bitrev(kz) of its line.
   T gv[CONV ? kPer : 1];
   if (CONV) {
     const T *G = (const T *)p.green;
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const int lb = t >> p.log2L, q = t & (p.L - 1);
+      const int kz = t / p.B, lb = t - kz * p.B;
       const int line = line0 + lb;
       T g = (T)0;
       if (line < nlines) {
         const int i = line % nf, j2 = line / nf;
         const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
-        const int kz = bitrev(q, p.log2L);
         const int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
         g = G[i + p.gkx * jf + p.gky * kf];
       }
       gv[it] = g;
     }
   }
-  fft_stages<T, false>(buf, p.L, p.log2L, p.B, tw, inverse != 0);
+  fft_stages<T, false>(buf, p.L, LP, p.log2L, p.B, tw, inverse != 0);
   if (CONV) {
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const cx<T> v = buf[t];
-      buf[t] = {v.re * gv[it], v.im * gv[it]};
+      const int kz = t / p.B, lb = t - kz * p.B;
+      const int q = lb * LP + bitrev(kz, p.log2L);
+      const cx<T> v = buf[q];
+      buf[q] = {v.re * gv[it], v.im * gv[it]};
     }
     __syncthreads();
-    fft_stages<T, true>(buf, p.L, p.log2L, p.B, tw, true);
+    fft_stages<T, true>(buf, p.L, LP, p.log2L, p.B, tw, true);
   }
   // ---- store the kept part of each line
   const int totout = p.B * p.L;
@@ -284,13 +293,13 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       const bool has1 = l1 < nlines;
       const int64_t o0 = (l0 % nf) * p.df + (l0 / nf) * p.ds + m * p.de;
       const int64_t o1 = has1 ? (l1 % nf) * p.df + (l1 / nf) * p.ds + m * p.de : o0;
-      const cx<T> z = buf[lb * p.L + bitrev(m, p.log2L)];
+      const cx<T> z = buf[lb * LP + bitrev(m, p.log2L)];
       if (STORE == kStoreReal) {
         ((T *)p.dst)[o0] = (T)(z.re * p.out_scale);
         if (has1) ((T *)p.dst)[o1] = (T)(z.im * p.out_scale);
       } else {
         // X(m) = (Z(m) + conj Z(L-m)) / 2,  Y(m) = (Z(m) - conj Z(L-m)) / 2i
-        const cx<T> w = buf[lb * p.L + bitrev((p.L - m) & (p.L - 1), p.log2L)];
+        const cx<T> w = buf[lb * LP + bitrev((p.L - m) & (p.L - 1), p.log2L)];
         const T h = (T)(0.5 * p.out_scale);
         ((cx<T> *)p.dst)[o0] = {(z.re + w.re) * h, (z.im - w.im) * h};
         if (has1) ((cx<T> *)p.dst)[o1] = {(z.im + w.im) * h, (w.re - z.re) * h};
@@ -301,7 +310,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     if (line >= nlines) continue;
     int f = line % nf, s = line / nf;
     int64_t o = f * p.df + s * p.ds + m * p.de;
-    cx<T> v = CONV ? buf[lb * p.L + m] : buf[lb * p.L + bitrev(m, p.log2L)];
+    cx<T> v = CONV ? buf[lb * LP + m] : buf[lb * LP + bitrev(m, p.log2L)];
     if (STORE == kStoreCplx) {
       ((cx<T> *)p.dst)[o] = {(T)(v.re * p.out_scale), (T)(v.im * p.out_scale)};
     } else {
@@ -659,13 +668,18 @@ int ilog2(int v) {
 
 template <typename T, int LOAD, int STORE, bool CONV, bool PAIR = false>
 int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
+  // the fused z pass takes twice the lines per block: its rows of the
+  // spectrum and of G^ are read across the lines
+  constexpr int pts = CONV ? kFftPointsZ : kFftPoints;
   p.log2L = ilog2(p.L);
-  p.B = kFftPoints / p.L;
+  if (p.L < kFftMinLine || p.L > pts)
+    return fail(IZH_ERR_ARG, "FFT line of %d points outside %d..%d", p.L, kFftMinLine, pts);
+  p.B = pts / p.L;
   const int64_t nlines = p.nf * p.ns;
   const int64_t nfft = PAIR ? (nlines + 1) / 2 : nlines;
   const int64_t nblk = (nfft + p.B - 1) / p.B;
   const size_t tw_bytes = (size_t)(p.L / 2) * sizeof(cx<T>);
-  hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR>), dim3((unsigned)nblk),
+  hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts>), dim3((unsigned)nblk),
                      dim3(kFftThreads), tw_bytes, f->stream, p, (const cx<T> *)tw,
                      inverse ? 1 : 0);
   IZH_HIP(hipGetLastError());

@@ -732,7 +732,8 @@ struct FinCfg {
   int mode;                 // 0 prologue, 1 step, 2 final
   DevScal *scal;
   DevDom *dom;
-  double *moments_out;      // kMoments doubles or null
+  double *moments;          // the run's records (kRunRecord doubles per step)
+  int want_moments;         // write the record of the step just finished
   const uint32_t *hist;     // histogram of the keys just written
   uint32_t *cursor;         // -> exclusive offsets of hist
   Chunk *chunks;            // -> deposit chunks of that grouping
@@ -793,9 +794,11 @@ __global__ void __launch_bounds__(1024) k_finalize(FinCfg f) {
         D.kgrid[f.kg].h[a] = f.mode == 0 ? (pb[3 + a] - pb[a]) / nn[a] : D.msize[a];
       }
     }
-    if (f.mode != 0 && f.moments_out) {
-      for (int k = 0; k < kMoments; k++) f.moments_out[k] = red[k];
-      f.moments_out[kMoments] = red[kMoments + 7];  // Npt of those moments
+    if (f.mode != 0 && f.want_moments) {
+      double *o = f.moments + S.rec * kRunRecord;
+      for (int k = 0; k < kMoments; k++) o[k] = red[k];
+      o[kMoments] = red[kMoments + 7];  // Npt of those moments
+      S.rec++;
     }
     if (f.mode == 1) {
       const double *cv = red + kMoments;
@@ -992,131 +995,166 @@ static int segment_push(izh_bunch *b, const izh_segment &sg, HalfPush &p, int &k
   return IZH_OK;
 }
 
-int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
-              const izh_segment *segs, int64_t nseg, double *moments) {
-  const int nx = f->nx, ny = f->ny, nz = f->nz;
-  const int s0 = tile_shift(nx), s1 = tile_shift(ny), s2 = tile_shift(nz);
-  const int tnx = nx >> s0, tny = ny >> s1, tnz = nz >> s2;
-  const int ntiles = tnx * tny * tnz;
-  IZH_CHECK(bunch_alloc_fused(b, ntiles, nseg));
-  hipStream_t st = b->stream;
-  // everything runs on the bunch's stream, after work queued on the field's
-  {
-    hipEvent_t ev;
-    IZH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    IZH_HIP(hipEventRecord(ev, f->stream));
-    IZH_HIP(hipStreamWaitEvent(st, ev, 0));
-    IZH_HIP(hipEventDestroy(ev));
-  }
-  const hipStream_t fstream = f->stream;
-  f->stream = st;
-  const double gam = -b->refpt[5];
-  const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
-  const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
-  const int nb = (grid_for(b->n) + 7) / 8 * 8;  // multiple of 8 for block_range
-  const int period = b->resort_period > 0 ? b->resort_period : kDefaultResortPeriod;
-  int rc = IZH_OK;
-  // Two chunk lists (with their key grids and counts) describe groupings of
-  // the slots: g_cur the order the particles are in, g_next the order the
-  // keys of the last scan put them in at the next scatter.  The previous run
-  // on this bunch and field left its last scan (keys, cursors, chunks) in
-  // slot b->keys_po: start from it.
-  const bool warm = b->keys_valid && b->keys_field == f && b->keys_grid[0] == nx &&
-                    b->keys_grid[1] == ny && b->keys_grid[2] == nz;
-  int g_next = warm ? b->keys_po : 0;
-  int g_cur = 1 - g_next;
-  b->keys_valid = 0;
-  uint32_t *hist = b->hist[0];
-  // ---- device scalars
-  DevScal hs{};
-  if (warm) {
-    hs.nchunks[0] = b->h_scal->nchunks[0];
-    hs.nchunks[1] = b->h_scal->nchunks[1];
-  }
-  hs.current = b->current;
-  hs.npt = (double)b->npt;
-  hs.nalive = b->n;
-  hs.nslots = b->n;
-  *b->h_scal = hs;
-  IZH_HIP(hipMemcpyAsync(b->d_scal, b->h_scal, sizeof hs, hipMemcpyHostToDevice, st));
-  FinCfg fc{};
-  fc.part = b->d_part;
-  fc.nparts = nb;
-  fc.scal = b->d_scal;
-  fc.dom = f->d_dom;
-  fc.ntiles = ntiles;
-  fc.nx = nx; fc.ny = ny; fc.nz = nz;
-  fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
-  fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
-  fc.cursor = b->cursor;
-  fc.hist = hist;
-  fc.hist_zero = hist;
-  // ---- prologue: initial keys, their histogram, cursors and chunks
-  if (!warm) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st, hist,
-                       (int64_t)ntiles);
-    hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
-                       b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
-                       b->c.scxl, b->d_part);
-    IZH_CHECK(launch_reduce(st, b->d_part, nb, 6, 0x7ull, 0x38ull, b->d_out));
-    hipLaunchKernelGGL(k_prologue_keys, dim3(nb), dim3(kBlock), 0, st, b->r[0],
-                       b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
-                       b->c.scxl, b->d_out, nx, ny, nz, s0, s1, s2, tnx, tny,
-                       b->key, hist);
-    hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
-                       nx, ny, nz);
-    IZH_HIP(hipGetLastError());
-    fc.mode = 0;
-    fc.scan = 1;
-    fc.inplace = 0;
-    fc.kg = g_next;
-    fc.prologue_box = b->d_out;
-    fc.chunks = b->chunks + g_next * b->nchunks_cap;
-    fc.nchunks = &b->d_scal->nchunks[g_next];
-    {
-      ScopedTimer tm_(IZH_T_FINAL, st);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
-    }
-    IZH_HIP(hipGetLastError());
-  }
-  // ---- the steps: pass s finishes step s-1 and starts step s.  Passes
-  // 0 and nseg and every period-th pass between scatter the particles into
-  // tile order; the others write them back in place.  A pass before a
-  // scatter computes the keys it scatters by.
-  auto scatters = [&](int64_t s) { return s == 0 || s == nseg || s % period == 0; };
+// A run of steps on one bunch and field, pass by pass: begin() sets up the
+// device scalars and the initial sort keys, pass() makes one pass (and, but
+// for the last, the charge, field solve and E of the step it starts), end()
+// copies the results back.  izh_run drives it through all its passes; a
+// lattice-scan job replays captured graphs of its passes (dry passes keep the
+// host bookkeeping in step with the replays).
+struct Runner {
+  izh_bunch *b = nullptr;
+  izh_field *f = nullptr;
+  hipStream_t st = nullptr, fstream = nullptr;
+  int nx = 0, ny = 0, nz = 0, s0 = 0, s1 = 0, s2 = 0, tnx = 0, tny = 0, ntiles = 0;
+  int nb = 0, period = 1;
+  int g_cur = 1, g_next = 0;
+  int64_t total = 0;   // steps of the run
+  int64_t s = 0;       // index of the next pass
   PassCfg pc{};
-  pc.cap = b->cap;
-  pc.d_nslots = &b->d_scal->nslots;
-  pc.cursor = b->cursor;
-  pc.hist_new = hist;
-  pc.dom = f->d_dom;
-  pc.e = f->e;
-  pc.scal = b->d_scal;
-  pc.err = b->d_scal;
-  pc.nx = nx; pc.ny = ny; pc.nz = nz;
-  pc.tshift[0] = s0; pc.tshift[1] = s1; pc.tshift[2] = s2;
-  pc.tnx = tnx; pc.tny = tny;
-  pc.gam = gam;
-  pc.gam_k = gam;
-  pc.xl = b->c.scxl;
-  pc.inv_rho_scale = 1.0 / rho_scale;
-  pc.tpos = b->tpos;
-  pc.part = b->d_part;
-  pc.sink = b->sink;
-  pc.qm0 = b->qm0;
-  pc.qm_value = b->qm_value;
+  FinCfg fc{};
   HalfPush pb_prev{};
   int kb_prev = kPushNone;
   double tau2_prev = 0.0;
-  double gam_sc = gam;   // -refpt(6) during the space-charge stages of step s-1
-  std::vector<double> refpt_after(2 * (size_t)nseg);  // refpt(5:6) after each step
-  for (int64_t s = 0; s <= nseg && rc == IZH_OK; s++) {
-    const bool last = s == nseg;
+  double gam_sc = 0.0;             // -refpt(6) during the space-charge stages of step s-1
+  std::vector<double> refpt_after; // refpt(5:6) after each step
+  bool began = false;
+
+  bool scatters(int64_t k) const { return k == 0 || k == total || k % period == 0; }
+
+  int begin(izh_bunch *b_, izh_field *f_, int64_t nsteps) {
+    b = b_;
+    f = f_;
+    total = nsteps;
+    nx = f->nx; ny = f->ny; nz = f->nz;
+    s0 = tile_shift(nx); s1 = tile_shift(ny); s2 = tile_shift(nz);
+    tnx = nx >> s0; tny = ny >> s1;
+    const int tnz = nz >> s2;
+    ntiles = tnx * tny * tnz;
+    IZH_CHECK(bunch_alloc_fused(b, ntiles, total));
+    st = b->stream;
+    // everything runs on the bunch's stream, after work queued on the field's
+    {
+      hipEvent_t ev;
+      IZH_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+      IZH_HIP(hipEventRecord(ev, f->stream));
+      IZH_HIP(hipStreamWaitEvent(st, ev, 0));
+      IZH_HIP(hipEventDestroy(ev));
+    }
+    fstream = f->stream;
+    f->stream = st;
+    began = true;
+    const double gam = -b->refpt[5];
+    gam_sc = gam;
+    const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
+    const double rho_scale = f->precision == IZH_FP64 ? 1.0 : 1.0 / wsum;
+    nb = (grid_for(b->n) + 7) / 8 * 8;  // multiple of 8 for block_range
+    period = b->resort_period > 0 ? b->resort_period : kDefaultResortPeriod;
+    refpt_after.assign(2 * (size_t)total, 0.0);
+    // Two chunk lists (with their key grids and counts) describe groupings
+    // of the slots: g_cur the order the particles are in, g_next the order
+    // the keys of the last scan put them in at the next scatter.  The
+    // previous run on this bunch and field left its last scan (keys,
+    // cursors, chunks) in slot b->keys_po: start from it.
+    const bool warm = b->keys_valid && b->keys_field == f && b->keys_grid[0] == nx &&
+                      b->keys_grid[1] == ny && b->keys_grid[2] == nz;
+    g_next = warm ? b->keys_po : 0;
+    g_cur = 1 - g_next;
+    b->keys_valid = 0;
+    uint32_t *hist = b->hist[0];
+    // ---- device scalars
+    DevScal hs{};
+    if (warm) {
+      hs.nchunks[0] = b->h_scal->nchunks[0];
+      hs.nchunks[1] = b->h_scal->nchunks[1];
+    }
+    hs.current = b->current;
+    hs.npt = (double)b->npt;
+    hs.nalive = b->n;
+    hs.nslots = b->n;
+    *b->h_scal = hs;
+    IZH_HIP(hipMemcpyAsync(b->d_scal, b->h_scal, sizeof hs, hipMemcpyHostToDevice, st));
+    fc = FinCfg{};
+    fc.part = b->d_part;
+    fc.nparts = nb;
+    fc.scal = b->d_scal;
+    fc.dom = f->d_dom;
+    fc.moments = b->d_moments;
+    fc.ntiles = ntiles;
+    fc.nx = nx; fc.ny = ny; fc.nz = nz;
+    fc.gam = gam; fc.scfreq = b->c.scfreq; fc.clight = b->c.clight; fc.pi = b->c.pi;
+    fc.scxl = b->c.scxl; fc.mass = b->mass; fc.charge = b->charge;
+    fc.cursor = b->cursor;
+    fc.hist = hist;
+    fc.hist_zero = hist;
+    // ---- prologue: initial keys, their histogram, cursors and chunks
+    if (!warm) {
+      hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(ntiles)), dim3(kBlock), 0, st, hist,
+                         (int64_t)ntiles);
+      hipLaunchKernelGGL(k_prologue_range, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                         b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                         b->c.scxl, b->d_part);
+      IZH_CHECK(launch_reduce(st, b->d_part, nb, 6, 0x7ull, 0x38ull, b->d_out));
+      hipLaunchKernelGGL(k_prologue_keys, dim3(nb), dim3(kBlock), 0, st, b->r[0],
+                         b->r[1], b->r[2], b->r[3], b->r[4], b->r[5], b->n, gam,
+                         b->c.scxl, b->d_out, nx, ny, nz, s0, s1, s2, tnx, tny,
+                         b->key, hist);
+      hipLaunchKernelGGL(k_prologue_dom, dim3(1), dim3(64), 0, st, b->d_out, f->d_dom,
+                         nx, ny, nz);
+      IZH_HIP(hipGetLastError());
+      fc.mode = 0;
+      fc.scan = 1;
+      fc.inplace = 0;
+      fc.kg = g_next;
+      fc.prologue_box = b->d_out;
+      fc.chunks = b->chunks + g_next * b->nchunks_cap;
+      fc.nchunks = &b->d_scal->nchunks[g_next];
+      {
+        ScopedTimer tm_(IZH_T_FINAL, st);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+      }
+      IZH_HIP(hipGetLastError());
+    }
+    pc = PassCfg{};
+    pc.cap = b->cap;
+    pc.d_nslots = &b->d_scal->nslots;
+    pc.cursor = b->cursor;
+    pc.hist_new = hist;
+    pc.dom = f->d_dom;
+    pc.e = f->e;
+    pc.scal = b->d_scal;
+    pc.err = b->d_scal;
+    pc.nx = nx; pc.ny = ny; pc.nz = nz;
+    pc.tshift[0] = s0; pc.tshift[1] = s1; pc.tshift[2] = s2;
+    pc.tnx = tnx; pc.tny = tny;
+    pc.gam = gam;
+    pc.gam_k = gam;
+    pc.xl = b->c.scxl;
+    pc.inv_rho_scale = 1.0 / rho_scale;
+    pc.tpos = b->tpos;
+    pc.part = b->d_part;
+    pc.sink = b->sink;
+    pc.qm0 = b->qm0;
+    pc.qm_value = b->qm_value;
+    s = 0;
+    kb_prev = kPushNone;
+    pb_prev = HalfPush{};
+    tau2_prev = 0.0;
+    return IZH_OK;
+  }
+
+  // Pass s: it finishes step s-1 and starts step s (seg = the segment of
+  // step s, null for the last pass).  Passes 0 and total and every
+  // period-th pass between scatter the particles into tile order; the others
+  // write them back in place.  A pass before a scatter computes the keys it
+  // scatters by.  dry: the host bookkeeping only (a replayed graph launched
+  // the kernels).
+  int pass(const izh_segment *seg, double *z, bool dry = false) {
+    const bool last = seg == nullptr;
     const bool scatter = scatters(s);
     const bool keys = last || scatters(s + 1);
     HalfPush pa{};
     int ka = kPushNone;
-    if (!last) IZH_CHECK(segment_push(b, segs[s], pa, ka, z));  // first half of step s
+    if (!last) IZH_CHECK(segment_push(b, *seg, pa, ka, z));  // first half of step s
     // the kick of step s-1 and the z->t transform of step s each use the
     // reference energy of their own step (after its first half push)
     pc.gam_k = gam_sc;
@@ -1139,16 +1177,16 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     pc.pa = pa;
     pc.need_orig = (kb_prev == kPushQuadK || ka == kPushQuadK) && !b->qm_uniform;
     pc.tau2_prev = tau2_prev;
-    pc.rad = last ? 0.0 : 0.5 * (segs[s].xrad + segs[s].yrad);
+    pc.rad = last ? 0.0 : 0.5 * (seg->xrad + seg->yrad);
     const int mode = (scatter ? kPassScatter : 0) | (keys ? kPassKeys : 0);
-    {
+    if (!dry) {
       ScopedTimer tm_(IZH_T_PASS, st);
       if (f->precision == IZH_FP64)
         launch_pass<double>(mode, nb, st, pc);
       else
         launch_pass<float>(mode, nb, st, pc);
+      IZH_HIP(hipGetLastError());
     }
-    IZH_HIP(hipGetLastError());
     if (scatter) {
       for (int r = 0; r < 6; r++) std::swap(b->r[r], b->r_alt[r]);
       std::swap(b->orig, b->orig_alt);
@@ -1158,76 +1196,323 @@ int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
     fc.mode = last ? 2 : 1;
     fc.scan = keys ? 1 : 0;
     fc.inplace = scatter ? 0 : 1;
-    fc.moments_out = s > 0 ? b->d_moments + (s - 1) * kRunRecord : nullptr;
+    fc.want_moments = s > 0;
     if (keys) {
       g_next = 1 - g_cur;
       fc.kg = g_next;
       fc.chunks = b->chunks + g_next * b->nchunks_cap;
       fc.nchunks = &b->d_scal->nchunks[g_next];
     }
-    {
+    if (!dry) {
       ScopedTimer tm_(IZH_T_FINAL, st);
       hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, fc);
+      IZH_HIP(hipGetLastError());
     }
-    IZH_HIP(hipGetLastError());
-    if (last) break;
-    // charge, update3O and the E field of step s on the grouping the slots are in
-    const Chunk *dch = b->chunks + g_cur * b->nchunks_cap;
-    rc = launch_deposit(b, f, dch, &f->d_dom->kgrid[g_cur], &b->d_scal->nchunks[g_cur],
-                        b->nchunks_cap);
-    if (rc == IZH_OK) rc = launch_solve(f);
-    if (rc == IZH_OK) rc = launch_efield(f);
-    // the step's wakefield (src/Contrl/AccSimulator.f90:1272-1338) from the
-    // same t positions, into E before the next pass kicks
-    if (rc == IZH_OK && segs[s].wake && segs[s].wake->kind != 0) {
-      WakeParams wp;
-      rc = wake_params_of(b, f, segs[s].wake, &wp);
-      if (rc == IZH_OK)
-        rc = launch_wake(b, f, wp, b->d_scal, dch, &b->d_scal->nchunks[g_cur],
-                         b->nchunks_cap);
-      if (rc == IZH_OK) rc = launch_wake_add(f);
+    s++;
+    if (last) return IZH_OK;
+    const int64_t step = s - 1;
+    if (!dry) {
+      // charge, update3O and the E field of step s on the grouping the slots are in
+      const Chunk *dch = b->chunks + g_cur * b->nchunks_cap;
+      IZH_CHECK(launch_deposit(b, f, dch, &f->d_dom->kgrid[g_cur],
+                               &b->d_scal->nchunks[g_cur], b->nchunks_cap));
+      IZH_CHECK(launch_solve(f));
+      IZH_CHECK(launch_efield(f));
+      // the step's wakefield (src/Contrl/AccSimulator.f90:1272-1338) from the
+      // same t positions, into E before the next pass kicks
+      if (seg->wake && seg->wake->kind != 0) {
+        WakeParams wp;
+        IZH_CHECK(wake_params_of(b, f, seg->wake, &wp));
+        IZH_CHECK(launch_wake(b, f, wp, b->d_scal, dch, &b->d_scal->nchunks[g_cur],
+                              b->nchunks_cap));
+        IZH_CHECK(launch_wake_add(f));
+      }
     }
-    // second half of step s: host bookkeeping now, applied by the next pass
-    if (rc == IZH_OK) rc = segment_push(b, segs[s], pb_prev, kb_prev, z);
-    refpt_after[2 * s] = b->refpt[4];
-    refpt_after[2 * s + 1] = b->refpt[5];
-    tau2_prev = 2.0 * segs[s].tau1;
+    // second half of the step: host bookkeeping now, applied by the next pass
+    IZH_CHECK(segment_push(b, *seg, pb_prev, kb_prev, z));
+    refpt_after[2 * step] = b->refpt[4];
+    refpt_after[2 * step + 1] = b->refpt[5];
+    tau2_prev = 2.0 * seg->tau1;
+    return IZH_OK;
   }
-  f->stream = fstream;
-  IZH_CHECK(rc);
-  // ---- results back to the host
-  IZH_HIP(hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(DevScal), hipMemcpyDeviceToHost, st));
-  IZH_HIP(hipMemcpyAsync(f->h_dom, f->d_dom, sizeof(DevDom), hipMemcpyDeviceToHost, st));
-  std::vector<double> recs(moments ? (size_t)nseg * kRunRecord : 0);
-  if (moments && nseg > 0)
-    IZH_HIP(hipMemcpyAsync(recs.data(), b->d_moments, nseg * kRunRecord * sizeof(double),
-                           hipMemcpyDeviceToHost, st));
-  IZH_HIP(hipStreamSynchronize(st));
-  for (int64_t s = 0; moments && s < nseg; s++) {
-    double *o = moments + s * IZH_RUN_RECORD;
-    std::memcpy(o, recs.data() + s * kRunRecord, kRunRecord * sizeof(double));
-    o[kRunRecord] = refpt_after[2 * s];
-    o[kRunRecord + 1] = refpt_after[2 * s + 1];
+
+  // Give the field its own stream back (after an error too).
+  void release() {
+    if (began) f->stream = fstream;
+    began = false;
   }
-  const DevScal out = *b->h_scal;
-  if (out.err != 0)
-    return fail(IZH_ERR_STATE, "izh_run: %lld particles had no sort slot",
-                (long long)out.err);
-  b->n = out.nslots;
-  b->npt = (int64_t)out.npt;
-  b->current = out.current;
-  b->run_stats[0] = out.nchunks[g_next];
-  b->run_stats[1] = out.overflow;
-  b->run_stats[2] = out.nalive;
-  b->run_stats[3] = ntiles;
-  for (int k = 0; k < 6; k++) dom->range[k] = f->h_dom->range[k];
-  for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
-  f->have_phi = 1;
-  b->keys_valid = 1;
-  b->keys_po = g_next;
-  b->keys_field = f;
-  b->keys_grid[0] = nx; b->keys_grid[1] = ny; b->keys_grid[2] = nz;
-  return IZH_OK;
+
+  // After the last pass: wait for the stream and take the results.
+  int end(izh_compdom *dom, double *moments) {
+    release();
+    IZH_HIP(hipMemcpyAsync(b->h_scal, b->d_scal, sizeof(DevScal), hipMemcpyDeviceToHost,
+                           st));
+    IZH_HIP(hipMemcpyAsync(f->h_dom, f->d_dom, sizeof(DevDom), hipMemcpyDeviceToHost, st));
+    std::vector<double> recs(moments ? (size_t)total * kRunRecord : 0);
+    if (moments && total > 0)
+      IZH_HIP(hipMemcpyAsync(recs.data(), b->d_moments, total * kRunRecord * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    IZH_HIP(hipStreamSynchronize(st));
+    for (int64_t k = 0; moments && k < total; k++) {
+      double *o = moments + k * IZH_RUN_RECORD;
+      std::memcpy(o, recs.data() + k * kRunRecord, kRunRecord * sizeof(double));
+      o[kRunRecord] = refpt_after[2 * k];
+      o[kRunRecord + 1] = refpt_after[2 * k + 1];
+    }
+    const DevScal out = *b->h_scal;
+    if (out.err != 0)
+      return fail(IZH_ERR_STATE, "izh_run: %lld particles had no sort slot",
+                  (long long)out.err);
+    if (out.rec != total)
+      return fail(IZH_ERR_STATE, "izh_run: %lld step records for %lld steps",
+                  (long long)out.rec, (long long)total);
+    b->n = out.nslots;
+    b->npt = (int64_t)out.npt;
+    b->current = out.current;
+    b->run_stats[0] = out.nchunks[g_next];
+    b->run_stats[1] = out.overflow;
+    b->run_stats[2] = out.nalive;
+    b->run_stats[3] = ntiles;
+    for (int k = 0; k < 6; k++) dom->range[k] = f->h_dom->range[k];
+    for (int k = 0; k < 3; k++) dom->msize[k] = f->h_dom->msize[k];
+    f->have_phi = 1;
+    b->keys_valid = 1;
+    b->keys_po = g_next;
+    b->keys_field = f;
+    b->keys_grid[0] = nx; b->keys_grid[1] = ny; b->keys_grid[2] = nz;
+    return IZH_OK;
+  }
+};
+
+int run_steps(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
+              const izh_segment *segs, int64_t nseg, double *moments) {
+  Runner r;
+  int rc = r.begin(b, f, nseg);
+  for (int64_t k = 0; rc == IZH_OK && k <= nseg; k++)
+    rc = r.pass(k < nseg ? &segs[k] : nullptr, z);
+  if (rc != IZH_OK) {
+    r.release();
+    return rc;
+  }
+  return r.end(dom, moments);
+}
+
+// Steps of a lattice-scan job per captured graph: a whole number of lattice
+// periods holding an even number of re-sorts, so the buffers and chunk
+// lists a replay starts with are those the capture started with.
+static int64_t job_block(int64_t nperiod, int resort) {
+  int64_t b = nperiod;
+  while (b % (2 * (int64_t)resort) != 0) b += nperiod;
+  return b;
 }
 
 }  // namespace izh
+
+// A lattice-scan job (no reference counterpart: the reference runs one scan
+// point per process, utilities/Engscan.py:28-47): one bunch with its field
+// and domain, stepped nsteps times through a periodic lattice.  Its passes
+// are enqueued on the bunch's stream without waiting for the host; blocks of
+// whole lattice periods go through one captured graph, replayed.
+struct izh_job {
+  izh_bunch *b = nullptr;
+  izh_field *f = nullptr;
+  izh_compdom *dom = nullptr;
+  std::vector<izh_segment> period;
+  int64_t nsteps = 0;
+  double z = 0.0;
+  izh::Runner r;
+  bool use_graph = false;
+  int64_t block = 0;             // steps per graph replay
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t replays = 0;
+  hipEvent_t done_ev = nullptr;     // recorded after the work of each advance
+  bool enqueued_all = false, finished = false;
+};
+
+namespace izh {
+namespace {
+
+struct HostState {
+  double *r0, *alt0;
+  uint32_t *orig, *key;
+  int g_cur, g_next;
+  bool operator==(const HostState &o) const {
+    return r0 == o.r0 && alt0 == o.alt0 && orig == o.orig && key == o.key &&
+           g_cur == o.g_cur && g_next == o.g_next;
+  }
+};
+
+HostState host_state(const izh_job *j) {
+  return {j->b->r[0], j->b->r_alt[0], j->b->orig, j->b->key, j->r.g_cur, j->r.g_next};
+}
+
+const izh_segment *job_seg(const izh_job *j, int64_t step) {
+  return &j->period[(size_t)(step % (int64_t)j->period.size())];
+}
+
+// Passes s .. s+cnt-1 (steady passes: each finishes step s-1 and starts s).
+int job_passes(izh_job *j, int64_t cnt, bool dry) {
+  for (int64_t k = 0; k < cnt; k++) IZH_CHECK(j->r.pass(job_seg(j, j->r.s), &j->z, dry));
+  return IZH_OK;
+}
+
+// One block of steady passes through the job's graph: captured on the first
+// block (and launched), replayed after it with the host bookkeeping run dry.
+int job_graph_block(izh_job *j) {
+  hipStream_t st = j->b->stream;
+  if (!j->exec) {
+    const HostState before = host_state(j);
+    IZH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const int rc = job_passes(j, j->block, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    IZH_CHECK(rc);
+    if (ec != hipSuccess)
+      return fail(IZH_ERR_HIP, "izh_job_advance: graph capture: %s", hipGetErrorString(ec));
+    j->graph = g;
+    IZH_HIP(hipGraphInstantiate(&j->exec, g, nullptr, nullptr, 0));
+    IZH_HIP(hipGraphLaunch(j->exec, st));
+    j->replays = 1;
+    // a replay must start from the buffers this capture started from
+    if (!(host_state(j) == before)) j->use_graph = false;
+    return IZH_OK;
+  }
+  IZH_CHECK(job_passes(j, j->block, true));
+  IZH_HIP(hipGraphLaunch(j->exec, st));
+  j->replays++;
+  return IZH_OK;
+}
+
+}  // namespace
+}  // namespace izh
+
+using namespace izh;
+
+extern "C" {
+
+int izh_job_create(izh_bunch *b, izh_field *f, izh_compdom *dom, double z,
+                   const izh_segment *period, int64_t nperiod, int64_t nsteps, double bcurr,
+                   int use_graph, izh_job **out) {
+  if (!b || !f || !dom || !period || !out || nperiod < 1 || nsteps < 1)
+    return fail(IZH_ERR_ARG, "izh_job_create: bad argument");
+  *out = nullptr;
+  if (b->frame != kFrameZ)
+    return fail(IZH_ERR_STATE, "izh_job_create: bunch is not in the z frame");
+  if (dom->nx != f->nx || dom->ny != f->ny || dom->nz != f->nz)
+    return fail(IZH_ERR_ARG, "izh_job_create: domain mesh differs from the field's");
+  if (b->precision != f->precision)
+    return fail(IZH_ERR_ARG, "izh_job_create: bunch and field precision differ");
+  if (bcurr < (double)1.0e-10f || b->n == 0)
+    return fail(IZH_ERR_ARG, "izh_job_create: a job runs the space-charge step (current "
+                ">= 1e-10 A and particles; izh_step runs the maps alone)");
+  for (int64_t k = 0; k < nperiod; k++) {
+    const izh_segment &sg = period[k];
+    const char *why = nullptr;
+    const int kind = map1_kind(sg.e.bitype, sg.e.v, &why);
+    if (kind < 0 || is_rf_cavity(sg.e.bitype))
+      return fail(IZH_ERR_ARG, "izh_job_create: segment %lld: bitype %d: %s", (long long)k,
+                  sg.e.bitype, kind < 0 ? why : "an RF cavity changes the reference energy, "
+                  "so its steps do not repeat (izh_run runs it)");
+    if (!(sg.tau1 >= 0) || (kind == kMap1Linear && sg.e.mapstp < 1))
+      return fail(IZH_ERR_ARG, "izh_job_create: segment %lld: bad step or map steps",
+                  (long long)k);
+    if (sg.wake && sg.wake->kind != 0)
+      return fail(IZH_ERR_ARG, "izh_job_create: segment %lld has a wake (izh_run runs it)",
+                  (long long)k);
+  }
+  izh_job *j = new izh_job;
+  j->b = b;
+  j->f = f;
+  j->dom = dom;
+  j->period.assign(period, period + nperiod);
+  for (auto &sg : j->period) sg.wake = nullptr;
+  j->nsteps = nsteps;
+  j->z = z;
+  j->use_graph = use_graph != 0;
+  int rc = j->r.begin(b, f, nsteps);
+  j->block = job_block(nperiod, j->r.period);
+  if (rc == IZH_OK) rc = j->r.pass(job_seg(j, 0), &j->z);   // pass 0 starts step 0
+  if (rc == IZH_OK && (hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming) !=
+                           hipSuccess || hipEventRecord(j->done_ev, b->stream) != hipSuccess))
+    rc = fail(IZH_ERR_HIP, "izh_job_create: event");
+  if (rc != IZH_OK) {
+    j->r.release();
+    delete j;
+    return rc;
+  }
+  *out = j;
+  return IZH_OK;
+}
+
+int izh_job_advance(izh_job *j, int64_t max_steps, int64_t *left) {
+  if (!j || max_steps < 0) return fail(IZH_ERR_ARG, "izh_job_advance: bad argument");
+  int64_t budget = max_steps;
+  while (budget > 0 && !j->enqueued_all) {
+    const int64_t s = j->r.s;               // next pass; it finishes step s-1
+    if (s == j->nsteps) {                   // the last pass
+      IZH_CHECK(j->r.pass(nullptr, &j->z));
+      j->enqueued_all = true;
+      budget--;
+      break;
+    }
+    // a graph block covers steady passes s .. s+block-1 that start at a
+    // block boundary and leave a steady pass after them
+    if (j->use_graph && (s - 1) % j->block == 0 && s + j->block <= j->nsteps - 1 &&
+        budget >= j->block) {
+      IZH_CHECK(job_graph_block(j));
+      budget -= j->block;
+      continue;
+    }
+    IZH_CHECK(job_passes(j, 1, false));
+    budget--;
+  }
+  if (budget < max_steps) IZH_HIP(hipEventRecord(j->done_ev, j->b->stream));
+  if (left) *left = j->nsteps + 1 - j->r.s;
+  return IZH_OK;
+}
+
+int izh_job_query(izh_job *j, int *state) {
+  if (!j || !state) return fail(IZH_ERR_ARG, "izh_job_query: bad argument");
+  const hipError_t e = hipEventQuery(j->done_ev);
+  if (e != hipSuccess && e != hipErrorNotReady)
+    return fail(IZH_ERR_HIP, "izh_job_query: %s", hipGetErrorString(e));
+  const bool ran = e == hipSuccess;
+  *state = !ran ? 0 : (j->enqueued_all ? 2 : 1);
+  return IZH_OK;
+}
+
+int izh_job_finish(izh_job *j, double *z, double *moments) {
+  if (!j) return fail(IZH_ERR_ARG, "izh_job_finish: bad argument");
+  if (!j->enqueued_all) IZH_CHECK(izh_job_advance(j, INT64_MAX, nullptr));
+  if (j->finished) return fail(IZH_ERR_STATE, "izh_job_finish: already finished");
+  j->finished = true;
+  IZH_CHECK(j->r.end(j->dom, moments));
+  if (z) *z = j->z;
+  return IZH_OK;
+}
+
+int izh_job_info(const izh_job *j, int64_t info[4]) {
+  if (!j || !info) return fail(IZH_ERR_ARG, "izh_job_info: bad argument");
+  info[0] = j->r.s;                 // passes enqueued
+  info[1] = j->block;
+  info[2] = j->use_graph ? 1 : 0;
+  info[3] = j->replays;
+  return IZH_OK;
+}
+
+int izh_job_destroy(izh_job *j) {
+  if (!j) return IZH_OK;
+  if (!j->finished) {
+    // stop the run where it is: wait for what was enqueued
+    (void)hipStreamSynchronize(j->b->stream);
+    j->r.release();
+  }
+  if (j->exec) (void)hipGraphExecDestroy(j->exec);
+  if (j->graph) (void)hipGraphDestroy(j->graph);
+  if (j->done_ev) (void)hipEventDestroy(j->done_ev);
+  delete j;
+  return IZH_OK;
+}
+
+}  // extern "C"

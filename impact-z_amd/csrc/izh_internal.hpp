@@ -91,6 +91,8 @@ struct DevScal {
   int64_t nchunks[2]; // deposit chunks of the two groupings in flight
   int64_t overflow;   // deposited particles whose nodes left their LDS window
   int64_t err;        // scatter slots out of range (must stay 0)
+  int64_t rec;        // next record of the run's per-step moments (the finalize
+                      // kernels count it, so a replayed step graph writes in turn)
 };
 
 // A run of at most kChunk consecutive slots of one sort tile.
@@ -456,6 +458,7 @@ int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
 int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep);
 int launch_cvbk(izh_bunch *b);
 int launch_moments(izh_bunch *b, double out[kMoments]);
+int launch_clone(const izh_bunch *src, izh_bunch *dst, int64_t offset, int64_t n, double w);
 int field_set_domain(izh_field *f, const izh_compdom *dom);
 int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGrid *kgrid,
                    const int64_t *d_nchunks, int64_t max_chunks);

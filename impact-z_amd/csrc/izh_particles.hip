@@ -774,6 +774,39 @@ int launch_chg_count(izh_bunch *b, const ChgList &cl, int64_t *out) {
   return IZH_OK;
 }
 
+// ------------------------------------------------------------------ clone
+namespace {
+// Slots offset..offset+n-1 of a bunch into slots 0..n-1 of another: the six
+// rows, and the immutable rows through the source's original index; the new
+// bunch's original index is the identity and its charge weights are w.
+__global__ void __launch_bounds__(kBlock)
+k_clone(const double *__restrict__ src, int64_t src_cap, const uint32_t *__restrict__ sorig,
+        const double *__restrict__ sqm, const double *__restrict__ sid, double *__restrict__ dst,
+        int64_t dst_cap, uint32_t *__restrict__ dorig, double *__restrict__ dqm,
+        double *__restrict__ dw, double *__restrict__ did, int64_t offset, int64_t n, double w) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = offset + i;
+#pragma unroll
+    for (int k = 0; k < 6; k++) dst[k * dst_cap + i] = src[k * src_cap + j];
+    const uint32_t o = sorig[j];
+    dorig[i] = (uint32_t)i;
+    dqm[i] = sqm[o];
+    did[i] = sid[o];
+    dw[i] = w;
+  }
+}
+}  // namespace
+
+int launch_clone(const izh_bunch *src, izh_bunch *dst, int64_t offset, int64_t n, double w) {
+  if (n == 0) return IZH_OK;
+  hipLaunchKernelGGL(k_clone, dim3(grid_for(n)), dim3(kBlock), 0, dst->stream, src->r[0],
+                     src->cap, src->orig, src->qm0, src->id0, dst->r[0], dst->cap, dst->orig,
+                     dst->qm0, dst->w0, dst->id0, offset, n, w);
+  IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
 int launch_moments(izh_bunch *b, double out[kMoments]) {
   const int nb = grid_for(b->n);
   {

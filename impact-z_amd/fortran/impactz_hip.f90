@@ -377,6 +377,61 @@ module impactz_hip
       type(izh_tuning) :: t
       integer(c_int) :: rc
     end function
+    ! a bunch from a resident pool bunch (no reference counterpart)
+    function izh_bunch_create_from(src, offset, n, current, precision, b) &
+        bind(C, name="izh_bunch_create_from") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr
+      type(c_ptr), value :: src
+      integer(c_int64_t), value :: offset, n
+      real(c_double), value :: current
+      integer(c_int), value :: precision
+      type(c_ptr) :: b
+      integer(c_int) :: rc
+    end function
+    ! lattice-scan jobs (no reference counterpart)
+    function izh_job_create(b, f, dom, z, period, nperiod, nsteps, bcurr, use_graph, j) &
+        bind(C, name="izh_job_create") result(rc)
+      import :: c_int, c_int64_t, c_double, c_ptr, izh_compdom, izh_segment
+      type(c_ptr), value :: b, f
+      type(izh_compdom) :: dom
+      real(c_double), value :: z, bcurr
+      type(izh_segment), intent(in) :: period(*)
+      integer(c_int64_t), value :: nperiod, nsteps
+      integer(c_int), value :: use_graph
+      type(c_ptr) :: j
+      integer(c_int) :: rc
+    end function
+    function izh_job_advance(j, max_steps, left) bind(C, name="izh_job_advance") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: j
+      integer(c_int64_t), value :: max_steps
+      integer(c_int64_t) :: left
+      integer(c_int) :: rc
+    end function
+    function izh_job_query(j, state) bind(C, name="izh_job_query") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: j
+      integer(c_int) :: state
+      integer(c_int) :: rc
+    end function
+    function izh_job_finish(j, z, moments) bind(C, name="izh_job_finish") result(rc)
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: j
+      real(c_double) :: z
+      real(c_double) :: moments(*)
+      integer(c_int) :: rc
+    end function
+    function izh_job_info(j, info) bind(C, name="izh_job_info") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: j
+      integer(c_int64_t) :: info(4)
+      integer(c_int) :: rc
+    end function
+    function izh_job_destroy(j) bind(C, name="izh_job_destroy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: j
+      integer(c_int) :: rc
+    end function
     function izh_chgupdate(b, nchge, qmcclist, counts) &
         bind(C, name="izh_chgupdate") result(rc)
       import :: c_int, c_int64_t, c_double, c_ptr

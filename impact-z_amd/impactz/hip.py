@@ -37,7 +37,9 @@ SYMBOLS = (
     "izh_isr_chicane", "izh_drift2", "izh_kick2", "izh_step2", "izh_rfdata_table",
     "izh_rfdata_fourier", "izh_rfdata_destroy", "izh_wakefield", "izh_kick1wake",
     "izh_wake_download", "izh_kick2wake", "izh_chgupdate", "izh_bunch_set_tuning",
-    "izh_bunch_get_tuning",
+    "izh_bunch_get_tuning", "izh_job_create", "izh_job_advance",
+    "izh_job_query", "izh_job_finish", "izh_job_info", "izh_job_destroy",
+    "izh_bunch_create_from",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -204,6 +206,16 @@ def lib() -> C.CDLL:
     L.izh_kick1.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
     L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
     L.izh_chgupdate.argtypes = [vp, C.c_int, _dp, _i64p]
+    L.izh_job_create.argtypes = [vp, vp, C.POINTER(CompDomC), C.c_double,
+                                 C.POINTER(Segment), C.c_int64, C.c_int64, C.c_double,
+                                 C.c_int, C.POINTER(vp)]
+    L.izh_job_advance.argtypes = [vp, C.c_int64, _i64p]
+    L.izh_job_query.argtypes = [vp, C.POINTER(C.c_int)]
+    L.izh_job_finish.argtypes = [vp, _dp, _dp]
+    L.izh_job_info.argtypes = [vp, _i64p]
+    L.izh_job_destroy.argtypes = [vp]
+    L.izh_bunch_create_from.argtypes = [vp, C.c_int64, C.c_int64, C.c_double, C.c_int,
+                                        C.POINTER(vp)]
     L.izh_bunch_set_tuning.argtypes = [vp, C.POINTER(TuningC)]
     L.izh_bunch_get_tuning.argtypes = [vp, C.POINTER(TuningC)]
     L.izh_field_download.argtypes = [vp, C.c_int, _dp]
@@ -403,6 +415,18 @@ class BeamBunch:
     def from_beam(cls, pts, beam, precision: int = FP64):
         return cls(pts, beam.current, beam.kinetic, beam.mass, beam.charge,
                    beam.freq, beam.phase0, precision)
+
+    @classmethod
+    def from_pool(cls, pool: "BeamBunch", offset: int, n: int, current: float,
+                  precision: int = FP64) -> "BeamBunch":
+        """izh_bunch_create_from: the n particles of a resident pool bunch
+        from slot offset on, copied on the device, charge weights for n
+        particles at the given current."""
+        self = cls.__new__(cls)
+        self.h = C.c_void_p()
+        check(lib().izh_bunch_create_from(pool.h, offset, n, current, precision,
+                                          C.byref(self.h)))
+        return self
 
     def close(self):
         if self.h:
@@ -607,3 +631,58 @@ class BeamBunch:
         check(lib().izh_run(self.h, field.h, C.byref(dom.c), C.byref(zz), arr,
                             len(segs), bcurr, _p(mom) if want_moments else None))
         return zz.value, mom
+
+
+class Job:
+    """A lattice-scan job (izh_job): a bunch, its field and domain stepped
+    nsteps times through a periodic lattice (segments of one period),
+    enqueued without waiting, with captured graphs of whole periods."""
+
+    def __init__(self, bunch: "BeamBunch", field: FieldQuant, dom: CompDom, z: float,
+                 period, nsteps: int, bcurr: float, use_graph: bool = True):
+        segs = list(period)
+        self._segs = (Segment * len(segs))(*segs)
+        self.bunch, self.field, self.dom = bunch, field, dom
+        self.nsteps = nsteps
+        self.h = C.c_void_p()
+        check(lib().izh_job_create(bunch.h, field.h, C.byref(dom.c), z, self._segs,
+                                   len(segs), nsteps, bcurr, 1 if use_graph else 0,
+                                   C.byref(self.h)))
+
+    def advance(self, max_steps: int) -> int:
+        """Enqueue up to max_steps more passes; returns the passes left."""
+        left = C.c_int64()
+        check(lib().izh_job_advance(self.h, max_steps, C.byref(left)))
+        return left.value
+
+    def state(self) -> int:
+        """0 busy on the GPU, 1 idle with passes left to enqueue, 2 done."""
+        d = C.c_int()
+        check(lib().izh_job_query(self.h, C.byref(d)))
+        return d.value
+
+    def done(self) -> bool:
+        return self.state() == 2
+
+    def finish(self):
+        """Wait; returns (z, records nsteps x RUN_RECORD)."""
+        zz = C.c_double()
+        rec = np.zeros((self.nsteps, RUN_RECORD))
+        check(lib().izh_job_finish(self.h, C.byref(zz), _p(rec)))
+        return zz.value, rec
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 4)()
+        check(lib().izh_job_info(self.h, a))
+        return {"passes": a[0], "block": a[1], "graph": bool(a[2]), "launches": a[3]}
+
+    def close(self):
+        if self.h:
+            lib().izh_job_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -81,6 +81,15 @@ int izh_bunch_create(const double *pts9, int64_t n, double current,
                      double kinetic, double mass, double charge, double freq,
                      double phase0, int precision, izh_bunch **out);
 int izh_bunch_destroy(izh_bunch *b);
+/* A new bunch of the n particles that src holds in slots offset ..
+ * offset+n-1, copied on the device (no reference counterpart: a server
+ * keeps one resident particle pool and gives each request a bunch from it
+ * without a host upload).  The beam (mass, charge, frequency, refptcl) is
+ * src's; the current is the given one, and each particle's charge weight
+ * Pts1(8) = current / (freq n) sign(charge), as the distribution readers
+ * set it for n particles.  src must be in the z frame. */
+int izh_bunch_create_from(const izh_bunch *src, int64_t offset, int64_t n, double current,
+                          int precision, izh_bunch **out);
 /* getpts_BeamBunch (src/Appl/BeamBunch.f90:3661-3668): copy the local
  * particles back in the reference's order.  cap = rows of pts9. */
 int izh_bunch_download(izh_bunch *b, double *pts9, int64_t cap, int64_t *n);
@@ -349,6 +358,42 @@ int izh_step2(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z, double ta
 int izh_run(izh_bunch *b, izh_field *f, izh_compdom *dom, double *z,
             const izh_segment *segs, int64_t nseg, double bcurr,
             double *moments);
+
+/* ---- Lattice-scan jobs (no reference counterpart: the reference runs one
+ * scan point per process, all state through files, utilities/Engscan.py:
+ * 28-47) ----
+ * A job steps one bunch (with its own field and domain) nsteps times
+ * through a periodic lattice (period: nperiod segments, repeated) as izh_run
+ * does, with space charge (bcurr >= 1e-10 A).  Its passes are enqueued on
+ * the bunch's stream without the host waiting; with use_graph, blocks of
+ * whole lattice periods are captured once into a HIP graph and replayed.
+ * Many jobs run side by side on one GPU (each bunch has its own stream):
+ * the host enqueues some steps of each in turn, polls izh_job_query and
+ * starts new jobs while others are on the GPU.  The results equal izh_run's
+ * on the same lattice to rounding.  Elements: drifts, quadrupoles (gradient
+ * or K), constant focusing and solenoids; no RF cavities (they change the
+ * reference energy, so the steps do not repeat) and no wakes. */
+typedef struct izh_job izh_job;
+int izh_job_create(izh_bunch *b, izh_field *f, izh_compdom *dom, double z,
+                   const izh_segment *period, int64_t nperiod, int64_t nsteps, double bcurr,
+                   int use_graph, izh_job **out);
+/* Enqueue up to max_steps more passes (nsteps + 1 in all, the first made by
+ * izh_job_create); *left (may be NULL) = passes still to enqueue. */
+int izh_job_advance(izh_job *j, int64_t max_steps, int64_t *left);
+/* *state: 0 while work enqueued so far is still on the GPU, 1 when it has
+ * all run and passes are left to enqueue, 2 when every pass has run.  It
+ * does not wait. */
+int izh_job_query(izh_job *j, int *state);
+/* Enqueue what is left, wait, and return z after the last step and the
+ * per-step records (NULL or nsteps x IZH_RUN_RECORD, as izh_run gives them);
+ * the bunch and domain then hold the state after the last step. */
+int izh_job_finish(izh_job *j, double *z, double *moments);
+/* info[0] passes enqueued, [1] steps per graph replay, [2] graph in use,
+ * [3] graph launches. */
+int izh_job_info(const izh_job *j, int64_t info[4]);
+/* Free the job (an unfinished job's bunch is left mid-run: upload it anew
+ * before other use). */
+int izh_job_destroy(izh_job *j);
 
 /* Tuning of izh_run on a bunch (no reference counterpart; the results do
  * not depend on it beyond the order of the moment sums).  resort_period: a
