@@ -73,7 +73,7 @@ struct PassCfg {
   const DevScal *scal;      // current and kick constants of step s-1
   DevScal *err;             // the same record, for the slot-overflow flag
   int nx, ny, nz;
-  int tshift[3], tnx, tny;
+  int tshift[3], tnx, tny, ntiles;
   int kb, ka;               // push kinds of pb and pa (the same for every lane)
   int need_orig;            // a K-mode push needs each particle's own q/mc^2
   const double *qm0;        // q/mc^2 by original index
@@ -399,8 +399,15 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     sl.k0 = k0;
     // a scatter pass takes the particles the previous pass keyed; a pass in
     // place every particle not marked lost (x = NaN)
-    if constexpr (kScatter) sl.key = in ? key : kDeadKey;
-    else sl.key = in && r[0] == r[0] ? 0u : kDeadKey;
+    // (a key past the tile count, which must not happen, is counted as an
+    // error and the particle dropped, so no cursor outside the scan is touched)
+    if constexpr (kScatter) {
+      const bool bad = in && key != kDeadKey && key >= (uint32_t)c.ntiles;
+      nerr += bad ? 1u : 0u;
+      sl.key = in && !bad ? key : kDeadKey;
+    } else {
+      sl.key = in && r[0] == r[0] ? 0u : kDeadKey;
+    }
     sl.orig = in ? og : 0u;
 #pragma unroll
     for (int k = 0; k < 6; k++) sl.r[k] = in ? r[k] : 0.0;
@@ -578,7 +585,13 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     (ok ? c.out[4 * cap + slot] : sd[256]) = ph;
     (ok ? c.out[5 * cap + slot] : sd[320]) = pt;
     if constexpr (kScatter) (ok ? c.out_orig[slot] : su[0]) = sl.orig;
-    if constexpr (kKeys) (ok ? c.out_key[slot] : su[64]) = knew;
+    // In place, the key of EVERY slot of the block's range is rewritten: the
+    // slot of a particle lost before this pass takes kDeadKey, so the next
+    // scatter, which trusts the keys, drops it.
+    if constexpr (kKeys) {
+      const bool kok = kScatter ? ok : base + (int64_t)threadIdx.x < hi;
+      (kok ? c.out_key[slot] : su[64]) = knew;
+    }
     if (has_conv) {
       (ok ? tpx[slot] : st[0]) = tx;
       (ok ? tpx[cap + slot] : st[64]) = ty;
@@ -1126,6 +1139,7 @@ struct Runner {
     pc.nx = nx; pc.ny = ny; pc.nz = nz;
     pc.tshift[0] = s0; pc.tshift[1] = s1; pc.tshift[2] = s2;
     pc.tnx = tnx; pc.tny = tny;
+    pc.ntiles = ntiles;
     pc.gam = gam;
     pc.gam_k = gam;
     pc.xl = b->c.scxl;
@@ -1261,7 +1275,7 @@ struct Runner {
     }
     const DevScal out = *b->h_scal;
     if (out.err != 0)
-      return fail(IZH_ERR_STATE, "izh_run: %lld particles had no sort slot",
+      return fail(IZH_ERR_STATE, "izh_run: %lld particles had a bad sort key or no sort slot",
                   (long long)out.err);
     if (out.rec != total)
       return fail(IZH_ERR_STATE, "izh_run: %lld step records for %lld steps",

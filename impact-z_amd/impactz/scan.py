@@ -125,6 +125,8 @@ class Live:
                   for e in lat for _ in range(e.nseg)]
         self.job = hip.Job(self.bunch, self.field, self.dom, 0.0, period, req.steps,
                            beam.current, use_graph)
+        # a graph replay covers `block` steps: advance by at least that much
+        self.block = self.job.info()["block"]
         self.started = time.perf_counter()
         self.result = None
 
@@ -154,7 +156,7 @@ def run_batch(reqs: list[Request], pool: Pool, tuner=None, use_graph: bool = Tru
     pending = list(live)
     while pending:
         for lv in list(pending):
-            if lv.job.state() != 0 and lv.job.advance(blk) == 0:
+            if lv.job.state() != 0 and lv.job.advance(max(blk, lv.block)) == 0:
                 pending.remove(lv)
     recs = [lv.finish() for lv in live]
     dt = time.perf_counter() - t0
@@ -217,7 +219,7 @@ def serve(reqs: list[Request], pool: Pool, budget_bytes: int, tuner=None,
         for ri, lv in list(live):
             st = lv.job.state()
             if st == 1:
-                lv.job.advance(blk)
+                lv.job.advance(max(blk, lv.block))
                 progressed = True
             elif st == 2:
                 rec = lv.finish()

@@ -48,8 +48,7 @@ def test_c2_fp64_matches_oracle_stage_by_stage(gpu, oracle, c2_input):
             worst["particles"] = max(worst["particles"],
                                      col_err(sim.bunch.getpts(), run.particles()))
             worst["moments"] = max(worst["moments"], full_row_err(sim.diag(mom), run.diag()))
-        record(test="c2_fp64_vs_oracle", particles=C2_N, grid=C2_GRID, steps=C2_STEPS,
-               **worst)
+        record(test="c2_fp64_vs_oracle", n=C2_N, grid=C2_GRID, steps=C2_STEPS, **worst)
         for k, v in worst.items():
             assert v < TOL, (k, v)
     finally:
@@ -152,7 +151,12 @@ def test_c3_split_run_equals_one_run(gpu, c3_input):
     pe = col_err(b.bunch.getpts(), pa)
     b.close()
     record(test="c3_split_run", steps=8, moments=worst, particles=pe)
-    assert worst < 1e-12 and pe < 1e-12
+    # the particles are bitwise the same (fixed-point charge, exact ranges);
+    # the moment sums are not, since the atomics of the tile sort order the
+    # particles inside a tile differently from run to run: at 50M particles
+    # the cube roots of the near-zero third moments show that at ~1e-11
+    assert pe == 0.0
+    assert worst < TOL
 
 
 def test_c3_mixed_within_1e4_of_fp64_after_400_steps(gpu, c3_input):

@@ -53,13 +53,16 @@ def test_job_equals_fused_run(gpu, use_graph, period):
     rows_a, pts_a, z_a = _solo(gpu, 30_000, (16, 16, 32), 10.0, 5, nsteps, period=period)
     sim, job = _job(gpu, 30_000, (16, 16, 32), 10.0, 5, nsteps, use_graph, period=period)
     try:
-        while job.advance(7) > 0:
+        # budgets of a block and a bit: graph replays at block boundaries,
+        # single passes between them
+        step = job.info()["block"] + 7 if use_graph else 7
+        while job.advance(step) > 0:
             pass
         z, rec = job.finish()
         info = job.info()
         worst = max(full_row_err(sim.diag(rec[k]), rows_a[k]) for k in range(nsteps))
         pe = col_err(sim.bunch.getpts(), pts_a)
-        record(test="job_vs_run", graph=use_graph, period=period, steps=nsteps,
+        record(test="job_vs_run", use_graph=use_graph, period=period, steps=nsteps,
                moments=worst, particles=pe, **info)
         assert worst < TOL and pe < TOL
         assert z == pytest.approx(z_a, abs=1e-12)
@@ -91,13 +94,16 @@ def test_jobs_interleaved_equal_each_alone(gpu):
         pending = list(range(len(live)))
         while pending:
             for i in list(pending):
-                if live[i][1].advance(5) == 0:
+                # 21 passes: a graph block of 16 where one starts, else single passes
+                if live[i][1].advance(21) == 0:
                     pending.remove(i)
         for i, ((sim, job), (rows_a, pts_a, _)) in enumerate(zip(live, solos)):
             _, rec = job.finish()
             worst = max(full_row_err(sim.diag(rec[k]), rows_a[k]) for k in range(len(rows_a)))
             pe = col_err(sim.bunch.getpts(), pts_a)
-            record(test="jobs_interleaved", job=i, moments=worst, particles=pe)
+            record(test="jobs_interleaved", job=i, moments=worst, particles=pe,
+                   launches=job.info()["launches"])
+            assert job.info()["launches"] >= 1
             # the mixed form is deterministic too: the same job alone or in the mix
             assert worst < TOL and pe < TOL, (i, worst, pe)
     finally:
