@@ -271,6 +271,7 @@ int izh_init(int device) {
     return fail(IZH_ERR_ARG, "device %d not present (%d devices)", device, count);
   IZH_HIP(hipSetDevice(device));
   IZH_HIP(hipFree(nullptr));
+  IZH_CHECK(scratch_setup(device));
   return IZH_OK;
 }
 
@@ -1440,6 +1441,17 @@ int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
   if (amax) std::memcpy(amax, m + 37, 6 * sizeof(double));
   if (r2max) *r2max = m[43];
   return IZH_OK;
+}
+
+int izh_diagnostic1_rows(const double *rows, int64_t ld, int64_t n, double *out,
+                         void *stream) {
+  if (n < 0 || (n > 0 && !rows) || !out || ld < n)
+    return fail(IZH_ERR_ARG, "izh_diagnostic1_rows: bad argument (rows %p, ld %lld, n %lld, "
+                "out %p)", (const void *)rows, (long long)ld, (long long)n, (void *)out);
+  if (ld > IZH_DIAG_ROWS_MAX)
+    return fail(IZH_ERR_ARG, "izh_diagnostic1_rows: row length %lld beyond the maximum %lld",
+                (long long)ld, (long long)IZH_DIAG_ROWS_MAX);
+  return launch_moments_rows(rows, ld, n, out, (hipStream_t)stream);
 }
 
 int izh_chgupdate(izh_bunch *b, int nchge, const double *qmcclist, int64_t *counts) {

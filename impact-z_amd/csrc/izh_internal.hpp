@@ -458,6 +458,12 @@ int launch_conv1st(izh_bunch *b, double xrad, double yrad, double ptrange[6],
 int launch_compact(izh_bunch *b, double xrad, double yrad, int64_t nkeep);
 int launch_cvbk(izh_bunch *b);
 int launch_moments(izh_bunch *b, double out[kMoments]);
+// izh_diagnostic1_rows: the moments of caller-held rows into out (device),
+// partials in the library's static per-device scratch; allocates nothing.
+int launch_moments_rows(const double *rows, int64_t ld, int64_t n, double *out,
+                        hipStream_t st);
+// Looks up the scratch array's address on `device` (current) once.
+int scratch_setup(int device);
 int launch_clone(const izh_bunch *src, izh_bunch *dst, int64_t offset, int64_t n, double w);
 int field_set_domain(izh_field *f, const izh_compdom *dom);
 int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGrid *kgrid,

@@ -7,6 +7,7 @@
 // depends only on N, so results are bitwise reproducible run to run.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "izh_device.hpp"
 
@@ -804,6 +805,55 @@ int launch_clone(const izh_bunch *src, izh_bunch *dst, int64_t offset, int64_t n
                      src->cap, src->orig, src->qm0, src->id0, dst->r[0], dst->cap, dst->orig,
                      dst->qm0, dst->w0, dst->id0, offset, n, w);
   IZH_HIP(hipGetLastError());
+  return IZH_OK;
+}
+
+// ------------------------------------------------ allocation-free moments
+// The scratch of izh_diagnostic1_rows: the per-block partials of the moment
+// sums, kMaxRedBlocks x kMoments doubles (704 KiB).  Each GPU holds its own
+// copy for the life of the process.  Host code never hands `scratch` itself
+// to a kernel (on the host it names a stand-in, not device memory): its
+// address on each device comes from hipGetSymbolAddress, kept per device.
+static __device__ __align__(256) double scratch[kMaxRedBlocks * kMoments];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+int scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    return fail(IZH_ERR_ARG, "scratch: device %d beyond the %d the library tracks", device,
+                kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return IZH_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(IZH_ERR_HIP, "scratch: hipGetSymbolAddress on device %d: %s", device,
+                hipGetErrorString(e));
+  g_scratch_ptr[device] = (double *)p;
+  return IZH_OK;
+}
+
+int launch_moments_rows(const double *rows, int64_t ld, int64_t n, double *out,
+                        hipStream_t st) {
+  int dev = 0;
+  IZH_HIP(hipGetDevice(&dev));
+  IZH_CHECK(scratch_setup(dev));   // set up by izh_init; here for a device it skipped
+  double *part = g_scratch_ptr[dev];
+  if (n == 0) {
+    hipLaunchKernelGGL(k_zero_u64_small, dim3(1), dim3(64), 0, st,
+                       (unsigned long long *)out, kMoments);
+    IZH_HIP(hipGetLastError());
+    return IZH_OK;
+  }
+  // the kernels of launch_moments, on the scratch partials
+  const int nb = grid_for(n);
+  hipLaunchKernelGGL(k_moments, dim3(nb), dim3(kBlock), 0, st, rows, rows + ld,
+                     rows + 2 * ld, rows + 3 * ld, rows + 4 * ld, rows + 5 * ld, n, part);
+  IZH_CHECK(launch_reduce(st, part, nb, kMoments, 0ull, kMomentsMaxMask, out));
   return IZH_OK;
 }
 

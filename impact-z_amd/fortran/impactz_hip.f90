@@ -365,6 +365,15 @@ module impactz_hip
       real(c_double) :: sums(37), amax(6), r2max
       integer(c_int) :: rc
     end function
+    ! diagnostic1 sums of device rows the caller holds, allocating nothing
+    ! (no reference counterpart; for callers that cannot allocate)
+    function izh_diagnostic1_rows(rows, ld, n, out, stream) &
+        bind(C, name="izh_diagnostic1_rows") result(rc)
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: rows, out, stream
+      integer(c_int64_t), value :: ld, n
+      integer(c_int) :: rc
+    end function
     function izh_bunch_set_tuning(b, t) bind(C, name="izh_bunch_set_tuning") result(rc)
       import :: c_int, c_ptr, izh_tuning
       type(c_ptr), value :: b

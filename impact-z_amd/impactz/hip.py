@@ -39,7 +39,7 @@ SYMBOLS = (
     "izh_wake_download", "izh_kick2wake", "izh_chgupdate", "izh_bunch_set_tuning",
     "izh_bunch_get_tuning", "izh_job_create", "izh_job_advance",
     "izh_job_query", "izh_job_finish", "izh_job_info", "izh_job_destroy",
-    "izh_bunch_create_from",
+    "izh_bunch_create_from", "izh_diagnostic1_rows",
 )
 
 # timer slots (IZH_T_*), named after the reference's Timer accumulators
@@ -205,6 +205,7 @@ def lib() -> C.CDLL:
     L.izh_cvbkforth1st.argtypes = [vp]
     L.izh_kick1.argtypes = [vp, C.c_double, vp, C.POINTER(CompDomC), C.c_int]
     L.izh_diagnostic1.argtypes = [vp, _dp, _dp, _dp]
+    L.izh_diagnostic1_rows.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
     L.izh_chgupdate.argtypes = [vp, C.c_int, _dp, _i64p]
     L.izh_job_create.argtypes = [vp, vp, C.POINTER(CompDomC), C.c_double,
                                  C.POINTER(Segment), C.c_int64, C.c_int64, C.c_double,

@@ -240,6 +240,23 @@ int izh_kick1(izh_bunch *b, double tau, izh_field *f, const izh_compdom *dom,
 int izh_diagnostic1(izh_bunch *b, double sums[37], double amax[6],
                     double *r2max);
 
+/* izh_diagnostic1 on particle rows the caller holds, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  rows: device memory, six FP64 rows (x, px, y, py,
+ * phi, pt in the Pts1 units) of n particles, row k at rows + k*ld; out:
+ * device memory for 44 doubles, the izh_diagnostic1 results in one array
+ * (37 sums, 6 abs-maxima, max x^2+y^2), bitwise the same as izh_diagnostic1
+ * on a bunch holding those rows in that order.  stream: a hipStream_t (NULL:
+ * the null stream); the call only enqueues, the caller synchronises.  The
+ * per-block partials live in one static device array per GPU
+ * (2048 x 44 doubles, 704 KiB, held until the process ends), so calls that
+ * overlap on one GPU race and must be ordered by the caller.  0 <= n <= ld
+ * <= IZH_DIAG_ROWS_MAX. */
+#define IZH_DIAG_ROWS_MAX (1LL << 40)
+int izh_diagnostic1_rows(const double *rows, int64_t ld, int64_t n, double *out,
+                         void *stream);
+
 /* chgupdate_BeamBunch (src/Appl/BeamBunch.f90:688-728): counts[j] = live
  * particles of charge state j, a particle counting for state j when
  * |(Pts1(7) - qmcclist[j]) / qmcclist[j]| < 1.0e-4 (the reference's REAL(4)
