@@ -171,6 +171,22 @@ __device__ inline void half_push_any(int kind, const HalfPush &p, double qm, dou
   else if (kind == kPushQuadK) half_push_kmode(p, qm, x, px, y, py, ph, pt);
 }
 
+// The push of kind K fixed at compile time (K >= 0), or of the launch's
+// kind read at run time (K = kPushAnyKind).  The pass is instantiated for
+// the drift/map/none combinations of the FODO-type lattices, so their
+// launches carry neither the other kinds' code nor their coefficients as
+// live uniform values; every other combination takes the run-time form.
+constexpr int kPushAnyKind = -1;
+template <int K>
+__device__ inline void half_push_k(int kind, const HalfPush &p, double qm, double &x,
+                                   double &px, double &y, double &py, double &ph,
+                                   double &pt) {
+  if constexpr (K == kPushAnyKind) half_push_any(kind, p, qm, x, px, y, py, ph, pt);
+  else if constexpr (K == kPushDrift || K == kPushMap || K == kPushMapC)
+    half_push<K>(p, x, px, y, py, ph, pt);
+  else if constexpr (K == kPushQuadK) half_push_kmode(p, qm, x, px, y, py, ph, pt);
+}
+
 // Counting-sort slot of each participating lane, in two halves so the
 // atomics' latency overlaps other work.  wave_rank_issue groups the lanes by
 // key with ballots (no memory traffic), then makes ONE atomic instruction in
@@ -270,9 +286,13 @@ __device__ inline void box_load(const GatherLds<float> &g, int64_t q, float &x,
 }
 
 // One slot of the pass's input, read an iteration before it is processed.
+// The loaded values are kept as loaded (a lane past the range holds its
+// block's last slot): nothing between the loads and the next group's first
+// use reads them, so the waits for them fall where that group starts.
 struct SlotIn {
   uint32_t key, orig;
   uint32_t k0;       // sort tile of the group's first slot (block uniform)
+  bool in;           // the lane has a slot of the block's range
   double r[6];
 };
 
@@ -318,10 +338,11 @@ __device__ inline void gather_batched(Load ld, int64_t o, int64_t sy, int64_t sz
 // a set of PassMode bits.  The slot loop is unrolled by two over two
 // register sets, so the loads of the next slot group land in the set not
 // being worked on.
-template <typename T, int MODE>
+template <typename T, int MODE, int KB, int KA>
 __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
   constexpr bool kScatter = (MODE & kPassScatter) != 0, kKeys = (MODE & kPassKeys) != 0;
-  const bool has_kick = c.kb != kPushNone, has_conv = c.ka != kPushNone;
+  const bool has_kick = KB == kPushAnyKind ? c.kb != kPushNone : KB != kPushNone;
+  const bool has_conv = KA == kPushAnyKind ? c.ka != kPushNone : KA != kPushNone;
   const int64_t nslots = *c.d_nslots;
   int64_t lo, hi;
   block_range(nslots, lo, hi);
@@ -392,32 +413,30 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     } else {
       if (c.need_orig) og = c.in_orig[jj];
     }
-    double r[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) r[k] = c.in[k * cap + jj];
-    const bool in = j < hi;
+    for (int k = 0; k < 6; k++) sl.r[k] = c.in[k * cap + jj];
+    sl.in = j < hi;
     sl.k0 = k0;
-    // a scatter pass takes the particles the previous pass keyed; a pass in
-    // place every particle not marked lost (x = NaN)
-    // (a key past the tile count, which must not happen, is counted as an
-    // error and the particle dropped, so no cursor outside the scan is touched)
-    if constexpr (kScatter) {
-      const bool bad = in && key != kDeadKey && key >= (uint32_t)c.ntiles;
-      nerr += bad ? 1u : 0u;
-      sl.key = in && !bad ? key : kDeadKey;
-    } else {
-      sl.key = in && r[0] == r[0] ? 0u : kDeadKey;
-    }
-    sl.orig = in ? og : 0u;
-#pragma unroll
-    for (int k = 0; k < 6; k++) sl.r[k] = in ? r[k] : 0.0;
+    sl.key = key;
+    sl.orig = og;
   };
 
   auto step = [&](SlotIn &sl, uint32_t k0, int64_t base, auto &&prefetch) {
-    const uint32_t kin = sl.key;
     double &x = sl.r[0], &px = sl.r[1], &y = sl.r[2], &py = sl.r[3], &ph = sl.r[4],
            &pt = sl.r[5];
-    const bool valid = kin != kDeadKey;
+    // a scatter pass takes the particles the previous pass keyed; a pass in
+    // place every particle not marked lost (x = NaN).  A key past the tile
+    // count, which must not happen, is counted as an error and the particle
+    // dropped, so no cursor outside the scan is touched.
+    bool valid;
+    if constexpr (kScatter) {
+      const bool bad = sl.in && sl.key != kDeadKey && sl.key >= (uint32_t)c.ntiles;
+      nerr += bad ? 1u : 0u;
+      valid = sl.in && !bad && sl.key != kDeadKey;
+    } else {
+      valid = sl.in && x == x;
+    }
+    const uint32_t kin = valid ? sl.key : kDeadKey;
     nalive += valid ? 1u : 0u;
     // counting-sort scatter by the key of the previous pass; the next slot
     // group's loads are issued after the atomic, so waiting for its result
@@ -463,9 +482,9 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
       }
       if (kick) {
         // cvbkforth1st + scatter1 (src/Appl/BeamBunch.f90:1667-1701,2041-2181).
-        // Computed on every lane (a lane without a particle holds zeros, which
-        // give an in-range node), so no branch separates the slot's loads from
-        // their first use; only valid lanes keep the result.
+        // Computed on every lane, so no branch separates the slot's loads from
+        // their first use; only valid lanes gather (the node indices are
+        // clamped to the grid) and keep the result.
         const double rcp = 1.0 / (-pt + c.gam_k);
         const double betai = sqrt(1.0 - rcp * rcp * (1.0 + px * px + py * py));
         const double X0 = x * c.xl, Y0 = y * c.xl, Z0 = -c.gam_k * betai * ph * c.xl;
@@ -506,7 +525,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
           pt = pt - ttc * ezn;
         }
       }
-      half_push_any(c.kb, c.pb, qm, x, px, y, py, ph, pt);
+      half_push_k<KB>(c.kb, c.pb, qm, x, px, y, py, ph, pt);
       // diagnostic1 sums of the state after step s-1 (all lanes take part)
       pmom_add(mom, valid ? x : 0.0, valid ? px : 0.0, valid ? y : 0.0,
                valid ? py : 0.0, valid ? ph : 0.0, valid ? pt : 0.0);
@@ -515,7 +534,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     T tx = (T)NAN, ty = (T)NAN, tz = (T)NAN;
     if (has_conv) {
       if (valid) {
-        half_push_any(c.ka, c.pa, qm, x, px, y, py, ph, pt);
+        half_push_k<KA>(c.ka, c.pa, qm, x, px, y, py, ph, pt);
         // conv1st_BeamBunch, Flagbc=1 (src/Appl/BeamBunch.f90:1285-1352)
         ph = wrap_phase(ph);
         const double rcp = 1.0 / (-pt + c.gam);
@@ -643,16 +662,45 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
   }
 }
 
-template <typename T>
-void launch_pass(int mode, int nb, hipStream_t st, const PassCfg &pc) {
+template <typename T, int KB, int KA>
+void launch_pass_kinds(int mode, int nb, hipStream_t st, const PassCfg &pc) {
   const dim3 g(nb), b(kBlock);
   switch (mode) {
-    case kPassInPlace: hipLaunchKernelGGL((k_pass<T, kPassInPlace>), g, b, 0, st, pc); break;
-    case kPassKeys: hipLaunchKernelGGL((k_pass<T, kPassKeys>), g, b, 0, st, pc); break;
-    case kPassScatter: hipLaunchKernelGGL((k_pass<T, kPassScatter>), g, b, 0, st, pc); break;
-    default:
-      hipLaunchKernelGGL((k_pass<T, kPassScatter | kPassKeys>), g, b, 0, st, pc);
+    case kPassInPlace:
+      hipLaunchKernelGGL((k_pass<T, kPassInPlace, KB, KA>), g, b, 0, st, pc);
       break;
+    case kPassKeys: hipLaunchKernelGGL((k_pass<T, kPassKeys, KB, KA>), g, b, 0, st, pc); break;
+    case kPassScatter:
+      hipLaunchKernelGGL((k_pass<T, kPassScatter, KB, KA>), g, b, 0, st, pc);
+      break;
+    default:
+      hipLaunchKernelGGL((k_pass<T, kPassScatter | kPassKeys, KB, KA>), g, b, 0, st, pc);
+      break;
+  }
+}
+
+// Drift, map and none as compile-time kinds (3 x 3); the run-time form
+// for the rest (the coupled map, the K-mode quadrupole).
+template <typename T, int KB>
+void launch_pass_b(int mode, int nb, hipStream_t st, const PassCfg &pc) {
+  switch (pc.ka) {
+    case kPushNone: launch_pass_kinds<T, KB, kPushNone>(mode, nb, st, pc); break;
+    case kPushDrift: launch_pass_kinds<T, KB, kPushDrift>(mode, nb, st, pc); break;
+    default: launch_pass_kinds<T, KB, kPushMap>(mode, nb, st, pc); break;
+  }
+}
+
+template <typename T>
+void launch_pass(int mode, int nb, hipStream_t st, const PassCfg &pc) {
+  const auto simple = [](int k) { return k == kPushNone || k == kPushDrift || k == kPushMap; };
+  if (!simple(pc.kb) || !simple(pc.ka) || pc.need_orig) {
+    launch_pass_kinds<T, kPushAnyKind, kPushAnyKind>(mode, nb, st, pc);
+    return;
+  }
+  switch (pc.kb) {
+    case kPushNone: launch_pass_b<T, kPushNone>(mode, nb, st, pc); break;
+    case kPushDrift: launch_pass_b<T, kPushDrift>(mode, nb, st, pc); break;
+    default: launch_pass_b<T, kPushMap>(mode, nb, st, pc); break;
   }
 }
 
