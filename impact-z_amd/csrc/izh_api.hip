@@ -1527,8 +1527,8 @@ namespace izh {
 // device on first use.
 int wake_params_of(const izh_bunch *b, const izh_field *f, const izh_wake *w,
                    WakeParams *p) {
-  if (!w || (w->kind != 1 && w->kind != 2))
-    return fail(IZH_ERR_ARG, "wake: kind must be 1 (analytic) or 2 (read-in table)");
+  if (!w || w->kind < 1 || w->kind > 3)
+    return fail(IZH_ERR_ARG, "wake: kind must be 1 (analytic), 2 (read-in table) or 3 (CSR)");
   if (f->nz > kWakeMaxNz)
     return fail(IZH_ERR_ARG, "wake: Nz = %d above %d", f->nz, kWakeMaxNz);
   *p = WakeParams{};
@@ -1542,6 +1542,14 @@ int wake_params_of(const izh_bunch *b, const izh_field *f, const izh_wake *w,
     p->aa = w->aa;
     p->gg = w->gg;
     p->leng = w->leng;
+  } else if (w->kind == 3) {
+    if (!(w->r0 > 0) || !std::isfinite(w->r0) || !(w->blength > 0) ||
+        !std::isfinite(w->zoff))
+      return fail(IZH_ERR_ARG, "wake: a CSR wake needs a positive finite bend radius and "
+                  "length and a finite offset");
+    p->r0 = w->r0;
+    p->zoff = w->zoff;
+    p->blength = w->blength;
   } else {
     if (!w->table || w->table->fourier || w->table->z.size() < 2)
       return fail(IZH_ERR_ARG, "wake: a read-in wake needs a table of at least 2 rows "

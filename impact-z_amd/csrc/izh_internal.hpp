@@ -225,6 +225,7 @@ struct DipoleMap {
 struct WakeParams {
   int kind, nz, flagbtw, ntab;
   double aa, gg, leng, scale;
+  double r0, zoff, blength;   // kind 3: the CSR wake of a bend
   const double *tab;
   double gam, mass, charge, scfreq, clight;
   double current, npt, xycon, tcon;

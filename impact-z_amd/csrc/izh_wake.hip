@@ -1,7 +1,8 @@
 // izh_wake.hip -- 1D wakefields of the space-charge step: the line charge
 // density and centroids along z (src/Contrl/AccSimulator.f90:1272-1338),
 // the short-range wake convolutions of wakefield_FieldQuant and
-// wakefieldread_FieldQuant (src/Appl/Field.f90:590-1001), and the wake's
+// wakefieldread_FieldQuant (src/Appl/Field.f90:590-1001), the CSR wake of
+// csrwakeTrIGF_FieldQuant (:1007-1251, one workgroup per mesh line), and the wake's
 // share of the grid field kick1wake_BeamBunch adds plane by plane before
 // scatter1 (src/Appl/BeamBunch.f90:4694-5046).
 //
@@ -171,6 +172,203 @@ k_wake_field(const double *__restrict__ bins, WakeParams p, const DevDom *__rest
   }
 }
 
+// ------------------------------------------------------------- CSR wake
+// The integrated CSR kernels of the four regimes of Saldin et al. that
+// csrwakeTrIGF_FieldQuant sums (src/Appl/Field.f90:1253-1301): entrance
+// transient (A), steady state (B), exit transients with the source inside
+// (C) or past (D) the bend, where D's retarded angle is the root of a
+// quartic found by Newton-Raphson with a bisection safeguard (root and
+// funcd, :1305-1384, with their tolerance and iteration limit).
+__device__ inline double csr_a(double phih, double ssh, double xk2) {
+  const double p3 = phih * phih * phih, p4 = phih * phih * phih * phih;
+  const double bb = 2 * phih + p3 / 3 - 2 * ssh;
+  const double cc = phih * phih + p4 / 12 - 2 * ssh * phih;
+  const double yh = (-bb + sqrt(bb * bb - 4 * cc)) / 2;
+  const double phpy = phih + yh;
+  return xk2 * (-(2 * phpy + p3) / (phpy * phpy + p4 / 4) + 1.0 / ssh);
+}
+
+__device__ inline double csr_b(double ssh, double xk2) {
+  const double aa = sqrt(64.0 + 144.0 * (ssh * ssh));
+  const double uuh = pow(aa + 12 * ssh, 1.0 / 3.0) - pow(aa - 12 * ssh, 1.0 / 3.0);
+  const double u2 = uuh * uuh;
+  return xk2 * (-4 * uuh * (u2 + 8) / ((u2 + 4.0) * (u2 + 12.0)));
+}
+
+__device__ inline double csr_c(double phimh, double xxbarh, double ssh, double xk2) {
+  const double s = phimh + xxbarh;
+  const double bb = 2 * s - 2 * ssh + phimh * phimh * phimh / 3 + phimh * phimh * xxbarh;
+  const double cc = s * s + phimh * phimh * (phimh * phimh + 4 * phimh * xxbarh) / 12 -
+                    2 * ssh * s;
+  const double yh = (-bb + sqrt(bb * bb - 4 * cc)) / 2;
+  const double pa = s + yh, pb = phimh * xxbarh + phimh * phimh / 2.0;
+  return xk2 * (-2.0 * (pa + phimh * pb) / (pa * pa + pb * pb) + 1.0 / ssh);
+}
+
+__device__ inline void csr_funcd(double psi, double xxh, double ds, double &f, double &df) {
+  f = psi * psi * psi * psi / 12 + xxh * psi * psi * psi / 3 + psi * psi +
+      (2 * xxh - 2 * ds) * psi - 2 * ds * xxh + xxh * xxh;
+  df = psi * psi * psi / 3 + xxh * psi * psi + 2 * psi + 2 * xxh - 2 * ds;
+}
+
+__device__ inline double csr_root(double x1, double x2, double xacc, int maxit, double zeta,
+                                  double xxh) {
+  double fl, fh, df, f;
+  csr_funcd(x1, xxh, zeta, fl, df);
+  csr_funcd(x2, xxh, zeta, fh, df);
+  if (fabs(fl) < xacc) return x1;
+  if (fabs(fh) < xacc) return x2;
+  double xl = fl < 0.0 ? x1 : x2, xh = fl < 0.0 ? x2 : x1;
+  double rts = 0.5 * (x1 + x2), dxold = fabs(x2 - x1), dx = dxold;
+  csr_funcd(rts, xxh, zeta, f, df);
+  for (int j = 1; j <= maxit; j++) {
+    if (((rts - xh) * df - f) * ((rts - xl) * df - f) > 0.0 ||
+        fabs(2.0 * f) > fabs(dxold * df)) {
+      dxold = dx;
+      dx = 0.5 * (xh - xl);
+      rts = xl + dx;
+      if (xl == rts) return rts;
+    } else {
+      dxold = dx;
+      dx = f / df;
+      const double temp = rts;
+      rts = rts - dx;
+      if (temp == rts) return rts;
+    }
+    if (fabs(dx) < xacc) return rts;
+    csr_funcd(rts, xxh, zeta, f, df);
+    if (f < 0.0) xl = rts; else xh = rts;
+  }
+  return rts;
+}
+
+constexpr double kCsrTol = 2.0e-9;   // the reference's epstol
+constexpr int kCsrMaxIt = 100;       // and Nmax
+
+__device__ inline double csr_d(double xxbarh, double psimax, double ssh, double xk2) {
+  const double psi = csr_root(-kCsrTol, psimax * (1.0 + kCsrTol), kCsrTol, kCsrMaxIt, ssh,
+                              xxbarh);
+  const double pa = psi + xxbarh, pb = psi * (xxbarh + psi / 2.0);
+  return xk2 * (-2.0 * (pa + psi * pb) / (pa * pa + pb * pb) + 1.0 / ssh);
+}
+
+// One workgroup per mesh line i (1-based i = blockIdx.x + 1): the CSR wake
+// of csrwakeTrIGF_FieldQuant at line i, with the interior sums over the
+// lines behind it spread over the threads (each term the line density
+// times the kernel integrated over that line's cell) and the end-cell
+// corrections taken by thread 0; then the kick increment of kick1wake
+// (no transverse wake).  The line density is the wake's (the bins times
+// Current/Scfreq/Npt/(hz/gamma)); the first line sits range(5)/gamma + zoff
+// from the bend entrance (src/Contrl/AccSimulator.f90:1344-1396).
+__global__ void __launch_bounds__(kBlock)
+k_csr_lines(const double *__restrict__ bins, WakeParams p, const DevDom *__restrict__ dom,
+            const DevScal *__restrict__ S, double *__restrict__ out) {
+  const int nz = p.nz, i = (int)blockIdx.x + 1, tid = threadIdx.x;
+  const double current = S ? S->current : p.current, npt = S ? S->npt : p.npt;
+  const double xycon = S ? S->xycon : p.xycon, tcon = S ? S->tcon : p.tcon;
+  const double gam = p.gam, r0 = p.r0, blength = p.blength;
+  const double hx = (dom->range[5] - dom->range[4]) / (nz - 1) / gam;
+  const double ptmin = dom->range[4] / gam + p.zoff;
+  const double rscale = current / p.scfreq / npt / hx * (p.charge / fabs(p.charge));
+  const double pilc = 2 * asin(1.0), phim = blength / r0;
+  const double xconst = 1. / (4 * pilc * 8.854187817e-12);
+  const double xk2 = gam / r0, g3 = gam * gam * gam;
+  const double xx = ptmin + (i - 1) * hx;
+  auto rho = [&](int j) { return bins[j - 1] * rscale; };
+  auto ssh = [&](double xp) { return (xx - xp) * g3 / r0; };
+  auto edge = [&](int j, double half) { return ptmin + (j - 1) * hx + half; };
+  double part = 0.0, ends = 0.0;
+  if (i >= 2 && xx > 0) {
+    if (xx <= blength) {
+      const double xr = xx / r0, xxl = xr * xr * xr * r0 / 24;
+      const double xslp = (xx / r0) * r0 / 2 / gam / gam + xxl;
+      const int islp1 = (int)((xx - xslp - ptmin) / hx) + 1;
+      const double phih = xx / r0 * gam;
+      for (int j = 2 + tid; j <= islp1 - 1; j += blockDim.x)
+        part += rho(j) * (csr_a(phih, ssh(edge(j, -hx / 2)), xk2) -
+                          csr_a(phih, ssh(edge(j, hx / 2)), xk2));
+      const int jstart = islp1 > 1 ? islp1 : 1;
+      for (int j = jstart + 2 + tid; j <= i - 1; j += blockDim.x)
+        part += rho(j) * (csr_b(ssh(edge(j, -hx / 2)), xk2) - csr_b(ssh(edge(j, hx / 2)), xk2));
+      if (tid == 0) {
+        if (islp1 >= 2) {
+          ends += rho(islp1) * (csr_a(phih, ssh(edge(islp1, -hx / 2)), xk2) -
+                                csr_a(phih, ssh(xx - xslp), xk2));
+          ends += rho(1) * (csr_a(phih, ssh(ptmin), xk2) - csr_a(phih, ssh(ptmin + hx / 2), xk2));
+        } else if (islp1 == 1) {
+          ends += rho(1) * (csr_a(phih, ssh(ptmin), xk2) - csr_a(phih, ssh(xx - xslp), xk2));
+        }
+        if (jstart <= i - 2) {
+          ends += rho(i) * (csr_b(ssh(edge(i, -hx / 2)), xk2) - csr_b(ssh(edge(i, 0.0)), xk2));
+          if (islp1 < 1) {
+            ends += rho(1) * (csr_b(ssh(ptmin), xk2) - csr_b(ssh(ptmin + hx / 2), xk2));
+          } else {
+            const int j = islp1 + 1;
+            ends += rho(j) * (csr_b(ssh(xx - xslp), xk2) - csr_b(ssh(edge(j, hx / 2)), xk2));
+          }
+        } else if (jstart == i - 1) {
+          ends += rho(i) * (csr_b(ssh(xx - xslp), xk2) - csr_b(ssh(edge(i, 0.0)), xk2));
+        }
+      }
+    } else {
+      const double xxbar = xx - blength;
+      const double xslp = (r0 * phim + xxbar) / 2 / gam / gam +
+                          r0 * phim * phim * phim / 24 * (r0 * phim + 4 * xxbar) /
+                              (r0 * phim + xxbar);
+      const int islp1 = (int)((xx - xslp - ptmin) / hx) + 1;
+      const double xslpn = xxbar / 2.0 / gam / gam;
+      const int islpn1 = (int)((xx - xslpn - ptmin) / hx) + 1;
+      const double xxbarh = xxbar * gam / r0, phimh = phim * gam, psimax = phim * gam;
+      for (int j = 2 + tid; j <= islp1 - 1; j += blockDim.x)
+        part += rho(j) * (csr_c(phimh, xxbarh, ssh(edge(j, -hx / 2)), xk2) -
+                          csr_c(phimh, xxbarh, ssh(edge(j, hx / 2)), xk2));
+      const int jstart = islp1 > 0 ? islp1 : 0;
+      for (int j = jstart + 2 + tid; j <= islpn1 - 1; j += blockDim.x)
+        part += rho(j) * (csr_d(xxbarh, psimax, ssh(edge(j, -hx / 2)), xk2) -
+                          csr_d(xxbarh, psimax, ssh(edge(j, hx / 2)), xk2));
+      if (tid == 0) {
+        if (islp1 >= 2) {
+          ends += rho(islp1) * (csr_c(phimh, xxbarh, ssh(edge(islp1, -hx / 2)), xk2) -
+                                csr_c(phimh, xxbarh, ssh(xx - xslp), xk2));
+          ends += rho(1) * (csr_c(phimh, xxbarh, ssh(ptmin), xk2) -
+                            csr_c(phimh, xxbarh, ssh(ptmin + hx / 2), xk2));
+        } else if (islp1 == 1) {
+          ends += rho(1) * (csr_c(phimh, xxbarh, ssh(ptmin), xk2) -
+                            csr_c(phimh, xxbarh, ssh(xx - xslp), xk2));
+        }
+        if (islpn1 >= jstart + 2) {
+          ends += rho(islpn1) * (csr_d(xxbarh, psimax, ssh(edge(islpn1, -hx / 2)), xk2) -
+                                 csr_d(xxbarh, psimax, xslpn * g3 / r0, xk2));
+          ends += rho(jstart + 1) * (csr_d(xxbarh, psimax, xslp * g3 / r0, xk2) -
+                                     csr_d(xxbarh, psimax, ssh(ptmin + jstart * hx + hx / 2),
+                                           xk2));
+        } else if (islpn1 == jstart + 1) {
+          ends += rho(jstart + 1) * (csr_d(xxbarh, psimax, xslp * g3 / r0, xk2) -
+                                     csr_d(xxbarh, psimax, xslpn * g3 / r0, xk2));
+        }
+      }
+    }
+  }
+  // fixed-order block sum of the interior terms, then the end terms
+  __shared__ double red[kBlock / 64];
+  const double w = wave_sum(part);
+  if ((tid & 63) == 0) red[tid >> 6] = w;
+  __syncthreads();
+  if (tid == 0) {
+    double e = 0.0;
+    for (int k = 0; k < kBlock / 64; k++) e += red[k];
+    const double ez = i >= 2 ? (e + ends) * xconst : 0.0;
+    const int k = i - 1;
+    const bool kick = xycon != 0.0 && tcon != 0.0;
+    out[k] = 0.0;
+    out[nz + k] = 0.0;
+    out[2 * nz + k] = ez;
+    out[3 * nz + k] = 0.0;
+    out[4 * nz + k] = 0.0;
+    out[5 * nz + k] = kick ? p.charge * ez / tcon / p.mass : 0.0;
+  }
+}
+
 // E of every node of plane k += the plane's wake increment (times the grid
 // scaling of the mixed form).
 template <typename T>
@@ -244,9 +442,14 @@ int launch_wake(izh_bunch *b, izh_field *f, const WakeParams &p0, const DevScal 
     hipLaunchKernelGGL(k_wake_bins_rows, dim3(grid_for(b->n)), dim3(kBlock), lds_bins,
                        f->stream, b->r[0], b->r[2], b->r[4], b->n, f->d_dom, nz, bins);
   }
-  const size_t lds_field = (3 * (size_t)nz + 3 * (size_t)(nz + 1)) * sizeof(double);
-  hipLaunchKernelGGL(k_wake_field, dim3(1), dim3(kBlock), lds_field, f->stream, bins, p,
-                     f->d_dom, S, f->d_wake);
+  if (p.kind == 3) {
+    hipLaunchKernelGGL(k_csr_lines, dim3(nz), dim3(kBlock), 0, f->stream, bins, p, f->d_dom,
+                       S, f->d_wake);
+  } else {
+    const size_t lds_field = (3 * (size_t)nz + 3 * (size_t)(nz + 1)) * sizeof(double);
+    hipLaunchKernelGGL(k_wake_field, dim3(1), dim3(kBlock), lds_field, f->stream, bins, p,
+                       f->d_dom, S, f->d_wake);
+  }
   IZH_HIP(hipGetLastError());
   f->have_wake = 1;
   return IZH_OK;

@@ -70,6 +70,7 @@ module impactz_hip
     type(c_ptr) :: table
     integer(c_int) :: flagbtw
     real(c_double) :: scale
+    real(c_double) :: r0, zoff, blength
   end type izh_wake
 
   ! one segment of a lattice: the element's half step, the element, the

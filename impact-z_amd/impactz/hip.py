@@ -77,16 +77,20 @@ class ElementC(C.Structure):
 class WakeC(C.Structure):
     """izh_wake: kind 1 analytic (iris radius aa, gap gg, period leng), kind 2
     a read-in table (an RFData table of z, Wz, Wx, Wy) with flagbtw; scale =
-    scwk."""
+    scwk; kind 3 the CSR wake of a bend (radius r0, length blength, offset
+    zoff of the first mesh line from the bend entrance)."""
     _fields_ = [("kind", C.c_int), ("aa", C.c_double), ("gg", C.c_double),
                 ("leng", C.c_double), ("table", C.c_void_p), ("flagbtw", C.c_int),
-                ("scale", C.c_double)]
+                ("scale", C.c_double), ("r0", C.c_double), ("zoff", C.c_double),
+                ("blength", C.c_double)]
 
 
 def wake(kind: int, aa: float = 0.0, gg: float = 0.0, leng: float = 0.0,
-         table: "RFData | None" = None, flagbtw: int = 0, scale: float = 1.0) -> WakeC:
+         table: "RFData | None" = None, flagbtw: int = 0, scale: float = 1.0,
+         r0: float = 0.0, zoff: float = 0.0, blength: float = 0.0) -> WakeC:
     w = WakeC()
     w.kind, w.aa, w.gg, w.leng, w.flagbtw, w.scale = kind, aa, gg, leng, flagbtw, scale
+    w.r0, w.zoff, w.blength = r0, zoff, blength
     w.table = table.h if table is not None else None
     w._keep = table
     return w

@@ -276,13 +276,21 @@ int izh_field_upload_rho(izh_field *f, const double *rho);
  * period leng (its Param(13:15), wakefield_FieldQuant); kind 2 the wake
  * functions of a -41 element's file (read1wk_Data rows z, Wz, Wx, Wy, given
  * as an izh_rfdata_table; wakefieldread_FieldQuant) with flagbtw (2, 3: no
- * transverse wake; 4: no longitudinal); scale = scwk. */
+ * transverse wake; 4: no longitudinal); scale = scwk.  Kind 3 the coherent
+ * synchrotron radiation wake of a bend (csrwakeTrIGF_FieldQuant,
+ * src/Appl/Field.f90:1007-1251, as the loop calls it at
+ * src/Contrl/AccSimulator.f90:1344-1396): bend radius r0 (m), bend length
+ * blength (m), and zoff, the distance (m) the loop adds to range(5)/gamma
+ * for the first mesh line's distance from the bend entrance (z - zbleng in
+ * the bend, z - zbleng + sentr after it); its transverse wakes are zero and
+ * scale is not applied. */
 typedef struct {
   int kind;
   double aa, gg, leng;
   const izh_rfdata *table;
   int flagbtw;
   double scale;
+  double r0, zoff, blength;
 } izh_wake;
 
 /* The wake fields of the bunch after izh_conv1st (its t-frame z binned on

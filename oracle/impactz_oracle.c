@@ -942,8 +942,18 @@ int or_wakefield(const or_const *c, const double *pts, int64_t n, const double r
   double *yb = (double *)malloc(nz * sizeof(double));
   double hz;
   int k, j, lon, tra;
-  if (w->kind != 1 && w->kind != 2) return -1;
+  if (w->kind < 1 || w->kind > 3) return -1;
   wake_line(pts, n, range, nz, current, np, c->scfreq, gam, charge, &hz, rho, xb, yb);
+  if (w->kind == 3) {
+    /* the CSR branch (src/Contrl/AccSimulator.f90:1344-1396): zwkmin =
+     * range(5)/gamma + zoff, hzwake/gamma, exwake = eywake = 0 */
+    or_csr_wake(nz, w->r0, range[4] / gam + w->zoff, hz, w->blength, rho, gam, ez);
+    for (k = 0; k < nz; k++) ex[k] = ey[k] = 0.0;
+    free(rho);
+    free(xb);
+    free(yb);
+    return 0;
+  }
   lon = !(w->kind == 2 && w->flagbtw == 4);
   tra = !(w->kind == 2 && (w->flagbtw == 2 || w->flagbtw == 3));
   for (k = 0; k < nz; k++) {
@@ -965,6 +975,227 @@ int or_wakefield(const or_const *c, const double *pts, int64_t n, const double r
   free(xb);
   free(yb);
   return 0;
+}
+
+/* ------------------------------------------------------------- CSR wake */
+
+/* The integrated CSR kernels of the four regimes of Saldin et al.
+ * (src/Appl/Field.f90:1253-1301): IcsrCaseA (entrance transient),
+ * IcsrCaseB (steady state), IcsrCaseC (exit transient, source inside the
+ * bend), IcsrCaseD (exit transient, source after the bend: the retarded
+ * angle psi is the root of funcd, found by or_csr_root).  Integer powers
+ * are products, as the compiled reference evaluates them. */
+static double or_csr_a(double phih, double ssh, double xk2) {
+  const double bb = 2 * phih + phih * phih * phih / 3 - 2 * ssh;
+  const double cc = phih * phih + phih * phih * phih * phih / 12 - 2 * ssh * phih;
+  const double yh = (-bb + sqrt(bb * bb - 4 * cc)) / 2;
+  const double phpy = phih + yh;
+  return xk2 * (-(2 * phpy + phih * phih * phih) /
+                (phpy * phpy + phih * phih * phih * phih / 4) + 1.0 / ssh);
+}
+
+static double or_csr_b(double ssh, double xk2) {
+  const double aa = sqrt(64.0 + 144.0 * (ssh * ssh));
+  const double uuh = pow(aa + 12 * ssh, 1.0 / 3.0) - pow(aa - 12 * ssh, 1.0 / 3.0);
+  return xk2 * (-4 * uuh * (uuh * uuh + 8) / ((uuh * uuh + 4.0) * (uuh * uuh + 12.0)));
+}
+
+static double or_csr_c(double phimh, double xxbarh, double ssh, double xk2) {
+  const double bb = 2 * (phimh + xxbarh) - 2 * ssh + phimh * phimh * phimh / 3 +
+                    phimh * phimh * xxbarh;
+  const double cc = (phimh + xxbarh) * (phimh + xxbarh) +
+                    phimh * phimh * (phimh * phimh + 4 * phimh * xxbarh) / 12 -
+                    2 * ssh * (phimh + xxbarh);
+  const double yh = (-bb + sqrt(bb * bb - 4 * cc)) / 2;
+  const double phpxya = phimh + xxbarh + yh;
+  const double phpxyb = phimh * xxbarh + phimh * phimh / 2.0;
+  return xk2 * (-2.0 * (phpxya + phimh * phpxyb) / (phpxya * phpxya + phpxyb * phpxyb) +
+                1.0 / ssh);
+}
+
+/* funcd (src/Appl/Field.f90:1373-1384): f(psi) and f'(psi). */
+static void or_csr_funcd(double psi, double xxh, double deltas, double *f, double *df) {
+  *f = psi * psi * psi * psi / 12 + xxh * psi * psi * psi / 3 + psi * psi +
+       (2 * xxh - 2 * deltas) * psi - 2 * deltas * xxh + xxh * xxh;
+  *df = psi * psi * psi / 3 + xxh * psi * psi + 2 * psi + 2 * xxh - 2 * deltas;
+}
+
+/* root (src/Appl/Field.f90:1305-1371): Newton-Raphson with bisection
+ * safeguard in [x1, x2].  The reference PAUSEs when the root is not
+ * bracketed or the iterations run out; here the iteration goes on as it
+ * would after the pause. */
+static double or_csr_root(double x1, double x2, double xacc, int maxit, double zeta,
+                          double xxh) {
+  double fl, fh, df, f, xl, xh;
+  or_csr_funcd(x1, xxh, zeta, &fl, &df);
+  or_csr_funcd(x2, xxh, zeta, &fh, &df);
+  if (fabs(fl) < xacc) return x1;
+  if (fabs(fh) < xacc) return x2;
+  if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
+  double rts = 0.5 * (x1 + x2);
+  double dxold = fabs(x2 - x1), dx = dxold;
+  or_csr_funcd(rts, xxh, zeta, &f, &df);
+  for (int j = 1; j <= maxit; j++) {
+    if (((rts - xh) * df - f) * ((rts - xl) * df - f) > 0.0 || fabs(2.0 * f) > fabs(dxold * df)) {
+      dxold = dx;
+      dx = 0.5 * (xh - xl);
+      rts = xl + dx;
+      if (xl == rts) return rts;
+    } else {
+      dxold = dx;
+      dx = f / df;
+      const double temp = rts;
+      rts = rts - dx;
+      if (temp == rts) return rts;
+    }
+    if (fabs(dx) < xacc) return rts;
+    or_csr_funcd(rts, xxh, zeta, &f, &df);
+    if (f < 0.0) xl = rts; else xh = rts;
+  }
+  return rts;
+}
+
+static double or_csr_d(double xxbarh, double psimax, double ssh, double xk2, double epstol,
+                       int nmax) {
+  const double psi = or_csr_root(-epstol, psimax * (1.0 + epstol), epstol, nmax, ssh, xxbarh);
+  const double psipxa = psi + xxbarh;
+  const double psipxb = psi * (xxbarh + psi / 2.0);
+  return xk2 * (-2.0 * (psipxa + psi * psipxb) / (psipxa * psipxa + psipxb * psipxb) +
+                1.0 / ssh);
+}
+
+void or_csr_wake(int nx, double r0, double ptmin, double hx, double blength,
+                 const double *rhonew, double gam, double *ezwake) {
+  /* csrwakeTrIGF_FieldQuant (src/Appl/Field.f90:1007-1251): the 1D CSR
+   * wake on the Nx mesh lines, each a sum over the lines behind it of the
+   * line density times the kernel integrated over the line's cell (the IGF
+   * differences), regime by regime: inside the bend (xx <= blength) the
+   * entrance transient A up to the retarded slip point and the steady state
+   * B after it; past the bend the exit transients C and D. */
+  const double epstol = 2.0e-9;
+  const int nmax = 100;
+  const double pilc = 2 * asin(1.0);
+  const double phim = blength / r0;
+  const double xconst = 1. / (4 * pilc * 8.854187817e-12);
+  const double xk2 = gam / r0;
+  const double g3 = gam * gam * gam;
+#define RHO(j) rhonew[(j) - 1]
+#define SSH(xp) ((xx - (xp)) * g3 / r0)
+  ezwake[0] = 0.0;
+  for (int i = 2; i <= nx; i++) {
+    const double xx = ptmin + (i - 1) * hx;
+    const double xr = xx / r0;
+    const double xxl = xr * xr * xr * r0 / 24;
+    const int ni = i;
+    double e = 0.0;
+    if (xx <= 0) {
+    } else if (xx <= blength) {
+      const double xslp = (xx / r0) * r0 / 2 / gam / gam + xxl;
+      const int islp0 = (int)((xx - xslp - ptmin) / hx);
+      const int islp1 = islp0 + 1;
+      const double phih = xx / r0 * gam;
+      /* case A */
+      for (int j = 2; j <= islp1 - 1; j++) {
+        const double t2 = or_csr_a(phih, SSH(ptmin + (j - 1) * hx + hx / 2), xk2);
+        const double t1 = or_csr_a(phih, SSH(ptmin + (j - 1) * hx - hx / 2), xk2);
+        e = e + RHO(j) * (t1 - t2);
+      }
+      if (islp1 >= 2) {
+        double t2 = or_csr_a(phih, SSH(xx - xslp), xk2);
+        double t1 = or_csr_a(phih, SSH(ptmin + (islp1 - 1) * hx - hx / 2), xk2);
+        e = e + RHO(islp1) * (t1 - t2);
+        t2 = or_csr_a(phih, SSH(ptmin + hx / 2), xk2);
+        t1 = or_csr_a(phih, SSH(ptmin), xk2);
+        e = e + RHO(1) * (t1 - t2);
+      } else if (islp1 == 1) {
+        const double t2 = or_csr_a(phih, SSH(xx - xslp), xk2);
+        const double t1 = or_csr_a(phih, SSH(ptmin + (islp1 - 1) * hx), xk2);
+        e = e + RHO(islp1) * (t1 - t2);
+      }
+      /* case B */
+      const int jstart = islp1 > 1 ? islp1 : 1;
+      for (int j = jstart + 2; j <= ni - 1; j++) {
+        const double t2 = or_csr_b(SSH(ptmin + (j - 1) * hx + hx / 2), xk2);
+        const double t1 = or_csr_b(SSH(ptmin + (j - 1) * hx - hx / 2), xk2);
+        e = e + RHO(j) * (t1 - t2);
+      }
+      if (jstart <= ni - 2) {
+        double t2 = or_csr_b(SSH(ptmin + (ni - 1) * hx), xk2);
+        double t1 = or_csr_b(SSH(ptmin + (ni - 1) * hx - hx / 2), xk2);
+        e = e + RHO(ni) * (t1 - t2);
+        if (islp1 < 1) {
+          t2 = or_csr_b(SSH(ptmin + hx / 2), xk2);
+          t1 = or_csr_b(SSH(ptmin), xk2);
+          e = e + RHO(1) * (t1 - t2);
+        } else {
+          const int j = islp1 + 1;
+          t2 = or_csr_b(SSH(ptmin + (j - 1) * hx + hx / 2), xk2);
+          t1 = or_csr_b(SSH(xx - xslp), xk2);
+          e = e + RHO(j) * (t1 - t2);
+        }
+      } else if (jstart == ni - 1) {
+        const double t2 = or_csr_b(SSH(ptmin + (ni - 1) * hx), xk2);
+        const double t1 = or_csr_b(SSH(xx - xslp), xk2);
+        e = e + RHO(ni) * (t1 - t2);
+      }
+    } else {
+      const double xxbar = xx - blength;
+      const double xslp = (r0 * phim + xxbar) / 2 / gam / gam +
+                          r0 * phim * phim * phim / 24 * (r0 * phim + 4 * xxbar) /
+                              (r0 * phim + xxbar);
+      const int islp0 = (int)((xx - xslp - ptmin) / hx);
+      const int islp1 = islp0 + 1;
+      const double xslpn = xxbar / 2.0 / gam / gam;
+      const int islpn = (int)((xx - xslpn - ptmin) / hx);
+      const int islpn1 = islpn + 1;
+      /* case C */
+      const double xxbarh = xxbar * gam / r0;
+      const double phimh = phim * gam;
+      for (int j = 2; j <= islp1 - 1; j++) {
+        const double t2 = or_csr_c(phimh, xxbarh, SSH(ptmin + (j - 1) * hx + hx / 2), xk2);
+        const double t1 = or_csr_c(phimh, xxbarh, SSH(ptmin + (j - 1) * hx - hx / 2), xk2);
+        e = e + RHO(j) * (t1 - t2);
+      }
+      if (islp1 >= 2) {
+        double t2 = or_csr_c(phimh, xxbarh, SSH(xx - xslp), xk2);
+        double t1 = or_csr_c(phimh, xxbarh, SSH(ptmin + (islp1 - 1) * hx - hx / 2), xk2);
+        e = e + RHO(islp1) * (t1 - t2);
+        t2 = or_csr_c(phimh, xxbarh, SSH(ptmin + hx / 2), xk2);
+        t1 = or_csr_c(phimh, xxbarh, SSH(ptmin), xk2);
+        e = e + RHO(1) * (t1 - t2);
+      } else if (islp1 == 1) {
+        const double t2 = or_csr_c(phimh, xxbarh, SSH(xx - xslp), xk2);
+        const double t1 = or_csr_c(phimh, xxbarh, SSH(ptmin), xk2);
+        e = e + RHO(islp1) * (t1 - t2);
+      }
+      /* case D */
+      const double psimax = phim * gam;
+      const int jstart = islp1 > 0 ? islp1 : 0;
+      for (int j = jstart + 2; j <= islpn1 - 1; j++) {
+        const double t2 = or_csr_d(xxbarh, psimax, SSH(ptmin + (j - 1) * hx + hx / 2), xk2,
+                                   epstol, nmax);
+        const double t1 = or_csr_d(xxbarh, psimax, SSH(ptmin + (j - 1) * hx - hx / 2), xk2,
+                                   epstol, nmax);
+        e = e + RHO(j) * (t1 - t2);
+      }
+      if (islpn1 >= jstart + 2) {
+        double t2 = or_csr_d(xxbarh, psimax, xslpn * g3 / r0, xk2, epstol, nmax);
+        double t1 = or_csr_d(xxbarh, psimax, SSH(ptmin + (islpn1 - 1) * hx - hx / 2), xk2,
+                             epstol, nmax);
+        e = e + RHO(islpn1) * (t1 - t2);
+        t2 = or_csr_d(xxbarh, psimax, SSH(ptmin + jstart * hx + hx / 2), xk2, epstol, nmax);
+        t1 = or_csr_d(xxbarh, psimax, xslp * g3 / r0, xk2, epstol, nmax);
+        e = e + RHO(jstart + 1) * (t1 - t2);
+      } else if (islpn1 == jstart + 1) {
+        const double t2 = or_csr_d(xxbarh, psimax, xslpn * g3 / r0, xk2, epstol, nmax);
+        const double t1 = or_csr_d(xxbarh, psimax, xslp * g3 / r0, xk2, epstol, nmax);
+        e = e + RHO(jstart + 1) * (t1 - t2);
+      }
+    }
+    ezwake[i - 1] = e * xconst;
+  }
+#undef RHO
+#undef SSH
 }
 
 void or_kick1wake(const or_const *c, double *pts, int64_t n, int nx, int ny, int nz,

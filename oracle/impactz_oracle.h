@@ -192,7 +192,11 @@ void or_chgupdate(const double *pts, int64_t n, int nchge, const double *qmcc,
  * analytic short-range wake of a SolRF/TWS element with iris radius aa, gap
  * gg and period leng (its Param(13:15)); kind 2 the wake functions read by
  * read1wk_Data (n rows: zdat from 0, Wz, Wx, Wy) with flagbtw (2, 3: no
- * transverse wake; 4: no longitudinal); scale = scwk. */
+ * transverse wake; 4: no longitudinal); scale = scwk.  Kind 3 the CSR wake
+ * of the run loop (src/Contrl/AccSimulator.f90:1344-1396): bend radius r0,
+ * bend length blength, and zoff, the distance from the bend entrance the
+ * loop adds to range(5)/gamma (z - zbleng inside the bend, z - zbleng +
+ * sentr after it); its transverse wakes are zero and scale is not used. */
 typedef struct {
   int kind;
   double aa, gg, leng;
@@ -200,12 +204,20 @@ typedef struct {
   const double *zdat, *edat, *epdat, *eppdat;
   int flagbtw;
   double scale;
+  double r0, zoff, blength;
 } or_wake;
 
 /* The wake fields ex, ey, ez (nz each) of the particles after conv1st. */
 int or_wakefield(const or_const *c, const double *pts, int64_t n, const double range[6],
                  int nz, double current, int64_t np, double gam, double charge,
                  const or_wake *w, double *ex, double *ey, double *ez);
+
+/* csrwakeTrIGF_FieldQuant (src/Appl/Field.f90:1007-1251): the 1D CSR wake
+ * ezwake (V/m) on nx mesh lines from the line density rhonew, for a bend of
+ * radius r0 and length blength (m); ptmin is the first line's distance from
+ * the bend entrance and hx the line spacing, both in the lab frame (m). */
+void or_csr_wake(int nx, double r0, double ptmin, double hx, double blength,
+                 const double *rhonew, double gam, double *ezwake);
 
 /* kick1wake_BeamBunch (src/Appl/BeamBunch.f90:4694-5046): kick1 with the
  * wake fields added to the grid field plane by plane. */

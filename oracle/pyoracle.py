@@ -70,15 +70,18 @@ class OrWake(C.Structure):
     _fields_ = [("kind", C.c_int), ("aa", C.c_double), ("gg", C.c_double),
                 ("leng", C.c_double), ("n", C.c_int), ("zdat", _dp), ("edat", _dp),
                 ("epdat", _dp), ("eppdat", _dp), ("flagbtw", C.c_int),
-                ("scale", C.c_double)]
+                ("scale", C.c_double), ("r0", C.c_double), ("zoff", C.c_double),
+                ("blength", C.c_double)]
 
 
-def make_wake(kind, aa=0.0, gg=0.0, leng=0.0, table=None, flagbtw=0, scale=1.0):
+def make_wake(kind, aa=0.0, gg=0.0, leng=0.0, table=None, flagbtw=0, scale=1.0,
+              r0=0.0, zoff=0.0, blength=0.0):
     """An or_wake: kind 1 analytic (aa, gg, leng), kind 2 the rows of a
     read1wk_Data file (z shifted to start at 0, src/DataStruct/Data.f90:
-    613-617)."""
+    613-617), kind 3 the CSR wake of a bend (r0, zoff, blength)."""
     w = OrWake()
     w.kind, w.aa, w.gg, w.leng, w.flagbtw, w.scale = kind, aa, gg, leng, flagbtw, scale
+    w.r0, w.zoff, w.blength = r0, zoff, blength
     if table is not None:
         t = np.asarray(table, dtype=np.float64).reshape(-1, 4)
         cols = [np.ascontiguousarray(t[:, k]) for k in range(4)]
@@ -97,6 +100,14 @@ def wakefield(c, pts, rng, nz, current, npt, gam, charge, w: OrWake):
                           npt, gam, charge, C.byref(w), _p(ex), _p(ey), _p(ez)) != 0:
         raise ValueError("bad wake kind")
     return np.stack([ex, ey, ez])
+
+
+def csr_wake(r0, ptmin, hx, blength, rho, gam):
+    """csrwakeTrIGF_FieldQuant: the CSR ezwake (V/m) of line density rho."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    ez = np.zeros(rho.shape[0])
+    lib().or_csr_wake(rho.shape[0], r0, ptmin, hx, blength, _p(rho), gam, _p(ez))
+    return ez
 
 
 def kick1wake(c, pts, phi, rng, ms, refpt, curr, tau, mass, chrg, wake):
@@ -213,6 +224,8 @@ def lib() -> C.CDLL:
                                C.c_int, C.c_int, C.c_int, C.c_double,
                                C.POINTER(OrElement), C.c_double, C.POINTER(OrWake), _dp, _dp]
         L.or_step2.restype = C.c_int64
+        L.or_csr_wake.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  _dp, C.c_double, _dp]
         _lib = L
     return _lib
 

@@ -459,14 +459,61 @@ def exe_chg_fixture(n=2000, grid=(16, 16, 16)):
     return out
 
 
+CSR_CASES = [
+    # r0 (m), bend length (m), gamma, first line's distance from the entrance
+    # (m), line spacing (m), lines: every regime of csrwakeTrIGF_FieldQuant
+    (10.0, 0.5, 100.0, 0.2, 1e-6, 64),       # inside the bend: A and B
+    (10.0, 0.5, 100.0, 0.02, 1e-6, 64),      # near the entrance, mostly A
+    (10.0, 0.5, 100.0, 0.49997, 1e-6, 64),   # across the exit
+    (10.0, 0.5, 100.0, 0.6, 1e-6, 64),       # past the bend: C and D
+    (10.0, 0.5, 100.0, 1.5, 1e-6, 64),       # far past: D
+    (2.0, 0.3, 40.0, 0.1, 5e-6, 48),
+    (2.0, 0.3, 40.0, 0.35, 5e-6, 48),
+    (5.0, 0.2, 1.16, 0.05, 2e-4, 32),        # low gamma
+    (10.0, 0.5, 100.0, -2e-5, 1e-6, 64),     # first lines before the entrance
+]
+
+
+CSR_META = ("the reference's csrwakeTrIGF_FieldQuant (oracle/_ref/ref_csr) on 9 line "
+            "densities: inside the bend (entrance transient, steady state), across the "
+            "exit, past it (exit transients), low gamma, lines before the entrance")
+
+
+def csr_fixture(seed=5):
+    """The reference's own csrwakeTrIGF_FieldQuant (run by oracle/_ref/ref_csr)
+    on noisy Gaussian line densities for the cases of CSR_CASES."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_csr")
+    rng = np.random.default_rng(seed)
+    tmp = tempfile.mkdtemp(prefix="impz_csr_")
+    rows, rhos = [], []
+    with open(os.path.join(tmp, "csr.in"), "w") as f:
+        f.write(f"{len(CSR_CASES)}\n")
+        for r0, blen, gam, ptmin, hx, nx in CSR_CASES:
+            z = np.arange(nx)
+            rho = np.exp(-0.5 * ((z - nx / 2) / (nx / 6)) ** 2) * 1e3 * \
+                (1 + 0.1 * rng.standard_normal(nx))
+            f.write(f"{nx} {r0!r} {ptmin!r} {hx!r} {blen!r} {gam!r}\n")
+            f.write(" ".join(repr(float(v)) for v in rho) + "\n")
+            rows.append([nx, r0, ptmin, hx, blen, gam])
+            rhos.append(rho)
+    subprocess.run([exe, "csr.in", "csr.out"], cwd=tmp, check=True, stdin=subprocess.DEVNULL,
+                   timeout=120)
+    ez = np.fromfile(os.path.join(tmp, "csr.out"), dtype=np.float64)
+    shutil.rmtree(tmp, ignore_errors=True)
+    path = os.path.join(HERE, "csr.npz")
+    np.savez_compressed(path, cases=np.array(rows), rho=np.concatenate(rhos), ez=ez)
+    print(f"csr: {os.path.getsize(path) / 1024:.0f} KiB")
+
+
 def main(only=None):
     """Write every fixture, or only the named ones (by file name without
     .npz; the executable runs io_ckpt, example1, exe_wake, exe_nonlinear,
-    exe_bend and exe_chg by their function)."""
+    exe_bend, exe_chg and csr by their function)."""
     if only:
         table = {"io_ckpt": io_fixture, "example1": example1_fixture,
                  "exe_wake": exe_wake_fixture, "exe_nonlinear": exe_nonlinear_fixture,
-                 "exe_bend": exe_bend_fixture, "exe_chg": exe_chg_fixture}
+                 "exe_bend": exe_bend_fixture, "exe_chg": exe_chg_fixture,
+                 "csr": csr_fixture}
         for name in only:
             table[name]()
         return
@@ -538,6 +585,8 @@ def main(only=None):
     meta["exe_chg"] = ("ImpactZexe run of two charge states (q/mc^2 and half of it, "
                        "0.06 A and 0.04 A) through two FODO periods in a 2.2 mm pipe: "
                        "every diagnostic file, fort.28 and fort.32 included")
+    csr_fixture()
+    meta["csr"] = CSR_META
     meta["exe_nonlinear"] = ("ImpactZexe runs with the Lorentz integrator: Flagmap=1 "
                              "switched to 2 by a -25 element before constant focusing and "
                              "a solenoid, and a Flagmap=2 FODO period")

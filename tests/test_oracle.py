@@ -863,3 +863,51 @@ def test_chgupdate_closed_form(oracle):
     assert list(oracle.chgupdate(pts, [q * (1 + 0.5 * eps)])) == [3]
     assert list(oracle.chgupdate(pts, [q * (1 - 0.5 * eps)])) == [3]
     assert list(oracle.chgupdate(pts, [q * (1 + 1.5 * eps)])) == [1]
+
+
+def test_oracle_csr_wake_matches_reference(oracle):
+    """csrwakeTrIGF_FieldQuant restated (or_csr_wake) against the reference's
+    own routine on line densities in every regime it distinguishes: the
+    entrance transient and steady state inside the bend, lines across the
+    exit, the exit transients past it (with the retarded-angle root finder),
+    a low gamma, and lines before the entrance (tests/golden/csr.npz,
+    written by oracle/_ref/ref_csr)."""
+    d = load_golden("csr")
+    k = 0
+    for nx, r0, ptmin, hx, blen, gam in d["cases"]:
+        nx = int(nx)
+        rho, ref = d["rho"][k:k + nx], d["ez"][k:k + nx]
+        k += nx
+        got = oracle.csr_wake(r0, ptmin, hx, blen, rho, gam)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref)), (r0, ptmin, gam)
+        assert got[0] == 0.0
+
+
+def test_oracle_csr_wake_kind_in_the_loop(oracle):
+    """or_wakefield kind 3 bins the particles as the wake does and hands the
+    line density to the CSR routine with zwkmin = range(5)/gamma + zoff and
+    hzwake/gamma (src/Contrl/AccSimulator.f90:1344-1396): the same as binning
+    here and calling the routine directly; no transverse wake."""
+    rng = np.random.default_rng(3)
+    n, nz, gam = 5000, 32, 50.0
+    pts = np.zeros((n, 9))
+    pts[:, 0] = rng.normal(0, 1e-3, n)
+    pts[:, 2] = rng.normal(0, 1e-3, n)
+    pts[:, 4] = rng.normal(0, 2e-4, n)
+    pts[:, 6] = 1.0 / 0.511e6
+    pts[:, 7] = 1e-9 / n
+    zmin, zmax = pts[:, 4].min() * 1.01, pts[:, 4].max() * 1.01
+    rg = np.array([-4e-3, 4e-3, -4e-3, 4e-3, zmin, zmax])
+    c = oracle.physconst(1.3e9)
+    w = oracle.make_wake(3, r0=8.0, zoff=0.3, blength=0.4)
+    out = oracle.wakefield(c, pts, rg, nz, 1e-3, n, gam, -1.0, w)
+    hz = (zmax - zmin) / (nz - 1)
+    dens = np.zeros(nz + 1)
+    iz = ((pts[:, 4] - zmin) / hz + 1).astype(int)
+    ab = ((zmin - pts[:, 4]) + iz * hz) / hz
+    np.add.at(dens, iz - 1, ab)
+    np.add.at(dens, iz, 1.0 - ab)
+    rho = dens[:nz] * 1e-3 / c.scfreq / n / (hz / gam) * -1.0
+    ref = oracle.csr_wake(8.0, zmin / gam + 0.3, hz / gam, 0.4, rho, gam)
+    assert np.all(out[0] == 0) and np.all(out[1] == 0)
+    assert np.max(np.abs(out[2] - ref)) <= 1e-12 * np.max(np.abs(ref))
