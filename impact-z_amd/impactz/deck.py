@@ -245,13 +245,17 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
             out.append((k, Element(e.length, e.nseg, e.mapstp, e.type, *vv[:4],
                                    more=tuple(vv[4:]))))
         elif e.type == 4:
-            # bends at zero current: with current the run loop adds ISRchicane,
-            # whose deviates come from the compiler's random generator, and CSR
-            # (src/Contrl/AccSimulator.f90:1132-1139, 1430-1448)
+            # with current the run loop adds space charge, the CSR wake (v3 >
+            # 200 in the bend, v3 > 500 also in every element after it) and
+            # ISRchicane, whose deviates come from the compiler's random
+            # generator: here from a seeded one (src/Contrl/AccSimulator.f90:
+            # 1032-1056, 1131-1138, 1344-1396, 1430-1448); Flagmap=1 only
+            if deck.bcurr > 1.0e-15 and flagmap != 1:
+                raise DeckError(f"element {k}: a bend (type 4) with current under "
+                                "Flagmap=2 is not on this path")
             if deck.bcurr > 1.0e-15:
-                raise DeckError(f"element {k}: a bend (type 4) with current "
-                                f"{deck.bcurr:g} A (ISR, space charge, CSR) is not on "
-                                "this path")
+                notes.append(f"element {k}: bend with current: ISRchicane deviates from a "
+                             "seeded generator, not the reference executable's stream")
             vv = (v + [0.0] * 14)[:14]
             if not (e.length > 0.0 and vv[0] != 0.0):
                 raise DeckError(f"element {k}: a bend needs a length > 0 and an angle != 0")
@@ -284,7 +288,7 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
         else:
             raise DeckError(f"element {k}: type {e.type} is not on this path "
                             "(drift 0, quadrupole 1, constant focusing 2, solenoid 3 and the "
-                            "RF cavities 101-105 are, and bends (4) at zero current)")
+                            "RF cavities 101-105 are, and bends (4))")
     return out, notes
 
 
@@ -411,11 +415,13 @@ def initial_particles(deck: Deck, workdir: str) -> np.ndarray:
 
 
 def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
-             outdir: str | None = None, nsteps: int | None = None) -> np.ndarray:
+             outdir: str | None = None, nsteps: int | None = None,
+             isr_seed: int = 0) -> np.ndarray:
     """Run ``workdir/ImpactZ.in`` and write the diagnostic files into
     ``outdir`` (default: workdir).  fused=True runs each stretch of
     linear-map (Flagmap=1) segments between file-writing elements through
-    izh_run; Flagmap=2 segments run step by step.
+    izh_run; Flagmap=2 segments run step by step.  Bends with current draw
+    ISRchicane's deviates from numpy's generator seeded with isr_seed.
     Rstartflg=1 restarts from the -7 checkpoint fort.<mapstp of the last -7
     element> at the element after the one that wrote it, as
     run_AccSimulator does (src/Contrl/AccSimulator.f90:558-561,674-742).
@@ -441,6 +447,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
     elements = [p for _, p in prog if isinstance(p, Element)]
     sim = Simulation(pts, deck.beam(), elements, deck.grid, precision=precision,
                      flagmap=deck.flagmap, qmcclist=deck.qmcclist, nptlist=deck.nptlist)
+    sim.isr_rng = np.random.default_rng(isr_seed)
 
     def counts():
         info = sim.bunch.info()
@@ -475,8 +482,7 @@ def run_deck(workdir: str, precision: int = hip.FP64, fused: bool = False,
                 else:
                     for e, _ in grp:
                         mom = sim.step(e)
-                        if e.type != 4:
-                            sim.update_chg()
+                        sim.update_chg()
                         rows.append(np.concatenate([[sim.z], sim.diag(mom), counts()]))
             pending.clear()
 

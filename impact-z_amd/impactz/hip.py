@@ -298,6 +298,43 @@ def element_maplinear(c: Const, tau, bitype, v, mapstp, refpt, charge, mass, z=0
     return xm.reshape(6, 6), r
 
 
+class DeviceDoubles:
+    """A device array of doubles the host fills, for entry points that read
+    device memory the caller provides (the deviates of izh_isr_chicane)."""
+
+    _rt = None
+
+    def __init__(self, n: int):
+        if DeviceDoubles._rt is None:
+            rt = C.CDLL("libamdhip64.so")
+            rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            rt.hipFree.argtypes = [C.c_void_p]
+            rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            DeviceDoubles._rt = rt
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        if DeviceDoubles._rt.hipMalloc(C.byref(self.ptr), max(self.n, 1) * 8) != 0:
+            raise IZHError(f"hipMalloc of {self.n} doubles failed")
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.n,):
+            raise IZHError(f"DeviceDoubles.upload: {a.shape} for {self.n} values")
+        if DeviceDoubles._rt.hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) != 0:
+            raise IZHError("hipMemcpy to the device failed")
+
+    def close(self) -> None:
+        if self.ptr:
+            DeviceDoubles._rt.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class RFData:
     """izh_rfdata: an RF cavity's on-axis field as the Data module holds it.
     ``field`` is an rfdata file's contents: an (n, 4) table of z, Ez, Ez',

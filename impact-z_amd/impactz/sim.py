@@ -15,6 +15,7 @@ from . import hip
 from .beam import Beam, Element
 
 CLIGHT = 299792458.0
+SC_CURRENT = float(np.float32(1.0e-10))   # the loop's REAL(4) space-charge threshold
 
 
 def diag_row(sums, amax, r2max, npt, refpt, mass, scxl, pi):
@@ -80,6 +81,17 @@ class Simulation:
         self._rf = {}       # RF cavity fields on the device library's side
         self.wk = {"on": 0, "read": 0, "btw": 0, "scale": 1.0, "aa": 0.0, "gg": 0.0,
                    "leng": 0.0, "table": None}
+        # the loop's CSR bookkeeping (src/Contrl/AccSimulator.f90:722-740,
+        # 1032-1056, 1131-1138, 1374-1385, 1498-1499): zbleng, sentr,
+        # flagbendcsr, bendlencsr and the last bend's radius r0
+        self.csr = {"zbleng": 0.0, "sentr": 0.0, "flagbend": 0, "bendlen": 0.0, "r0": 0.0}
+        # ISRchicane's deviates in bends with current: the reference draws
+        # them from the compiler's random_number (normVec1), which no other
+        # program reproduces; here a generator the caller seeds gives one
+        # standard normal per particle (by upload row) per bend step
+        self.isr_rng = None
+        self._isr_buf = None
+        self.n0 = int(np.asarray(pts).shape[0])
         self.lattice = lattice
         self.grid = tuple(grid)
         self.const = hip.physconst(beam.freq)
@@ -127,6 +139,42 @@ class Simulation:
             w["aa"], w["gg"], w["leng"] = v[11], v[12], v[13]
             w["on"] = 1 if v[13] > 0.0 else 0
 
+    def _csr_entry(self, e: Element) -> None:
+        """Entering a bend (src/Contrl/AccSimulator.f90:1032-1056,1066-1091):
+        the transient-drift flag (current and v3 > 500), the bend's length
+        for the elements after it, sentr reset, and r0 = |1/hd0|."""
+        if e.type != 4:
+            return
+        v = e.values
+        if self.beam.current > 0.0 and v[2] > 500:
+            self.csr.update(flagbend=1, bendlen=e.length, sentr=0.0)
+        else:
+            self.csr["flagbend"] = 0
+        hd0 = v[0] / e.length
+        self.csr["r0"] = abs(1.0 / hd0)
+
+    def _csr_exit(self, e: Element) -> None:
+        """After an element's last step (:1498-1499)."""
+        self.csr["zbleng"] += e.length
+        self.csr["sentr"] += e.length
+
+    def csr_wake(self, e: Element, z: float):
+        """The CSR izh_wake of a step of e whose first half push left z: in a
+        bend with v3 > 200 its own steady-state and transient wake, after a
+        bend with v3 > 500 the transient wake in every element (flagcsr,
+        zwkmin and bendlen of :1131-1138,1374-1385); else None."""
+        k = self.csr
+        if self.beam.current < SC_CURRENT:
+            return None
+        if e.type == 4:
+            if not e.values[2] > 200:
+                return None
+            return hip.wake(3, r0=k["r0"], zoff=z - k["zbleng"], blength=e.length)
+        if not k["flagbend"]:
+            return None
+        return hip.wake(3, r0=k["r0"], zoff=z - k["zbleng"] + k["sentr"],
+                        blength=k["bendlen"])
+
     def wake(self):
         """The izh_wake of the current step, or None."""
         w = self.wk
@@ -156,6 +204,7 @@ class Simulation:
         if self._left == 0:
             self._left, self.zedge = e.nseg, self.z
             self._wake_entry(e)
+            self._csr_entry(e)
         self._left -= 1
         return self.zedge
 
@@ -168,24 +217,50 @@ class Simulation:
             self.bunch.bpol_dipole(e.length, e.values)
 
     def bend_step(self, e: Element, want_moments: bool = True):
-        """A bend segment (src/Contrl/AccSimulator.f90:1032-1091, 1114-1120,
-        1430-1448) at zero current under either integrator: the entrance face
-        on the element's first step, then two Sector_Dipole half steps.  With
-        current the reference adds ISRchicane with its own random deviates and
-        CSR, which this path does not run."""
-        if self.beam.current > 1.0e-15:
-            raise hip.IZHError("bends with current (ISR, space charge, CSR) are not on "
-                               "this path")
+        """A bend segment (src/Contrl/AccSimulator.f90:1032-1091, 1114-1138,
+        1141-1396, 1417-1448): the entrance face on the element's first step,
+        a Sector_Dipole half step, with current the space-charge block (pipe
+        radius 100 m, the CSR wake when v3 > 200, kick1wake) and ISRchicane,
+        then the second half step.  Zero current runs under either
+        integrator, current under Flagmap=1; ISR takes its deviates from
+        isr_rng (set a seeded numpy Generator)."""
+        cur = self.beam.current
+        if cur > 1.0e-15 and self.flagmap != 1:
+            raise hip.IZHError("a bend with current runs under Flagmap=1 only")
+        if cur > 1.0e-15 and self.isr_rng is None:
+            raise hip.IZHError("a bend with current needs ISR deviates: set "
+                               "Simulation.isr_rng to a seeded numpy Generator")
         entering = self._left == 0
         self._enter(e)
         b = self.bunch
         if entering:
             b.fpol_dipole(e.length, e.values)
         tau1 = 0.5 * e.length / e.nseg
+        tau2 = 2.0 * tau1
         self.z = b.sector_dipole(self.z, tau1, e.length, e.values)
+        if cur >= SC_CURRENT:
+            ptrange, _, _ = b.conv1st(tau2, 100.0, 100.0)
+            self.dom.update(ptrange)
+            b.charge(self.dom, self.field)
+            self.field.update3O(self.dom)
+            w = self.csr_wake(e, self.z)
+            w = w if w is not None else self.wake()
+            if w is not None:
+                b.wakefield(self.field, self.dom, w)
+            b.cvbkforth1st()
+            if w is not None:
+                b.kick1wake(tau2, self.field, self.dom)
+            else:
+                b.kick1(tau2, self.field, self.dom)
+        if cur > 1.0e-15:
+            if self._isr_buf is None:
+                self._isr_buf = hip.DeviceDoubles(self.n0)
+            self._isr_buf.upload(self.isr_rng.standard_normal(self.n0))
+            b.isr_chicane(tau2, e.length, e.values, self._isr_buf.ptr.value, self.n0)
         self.z = b.sector_dipole(self.z, tau1, e.length, e.values)
         if self._left == 0:
             self._exit = e
+            self._csr_exit(e)
         self.nstep += 1
         if not want_moments:
             return None
@@ -204,8 +279,10 @@ class Simulation:
                 e.radius, self.beam.current, want_moments, self.wake())
         else:
             self.z, mom = self.bunch.step(self.field, self.dom, self.z,
-                                          self.segment_args(e, zedge), self.beam.current,
-                                          want_moments)
+                                          self.segment_args(e, zedge, self.z),
+                                          self.beam.current, want_moments)
+        if self._left == 0:
+            self._csr_exit(e)
         self.nstep += 1
         return mom
 
@@ -225,15 +302,22 @@ class Simulation:
             b.cvbkforth1st()
             b.kick1(tau2, self.field, self.dom)
         self.z = b.map1(self.z, tau1, e.type, e.values, e.mapstp, e.length, zedge, rf)
+        if self._left == 0:
+            self._csr_exit(e)
         self.nstep += 1
 
-    def segment_args(self, e: Element, zedge: float = 0.0) -> hip.Segment:
+    def segment_args(self, e: Element, zedge: float = 0.0, z: float | None = None) -> hip.Segment:
         """The izh_segment of one space-charge step of element e entered at
-        zedge, with the wake the switches give it."""
+        zedge and starting at z, with the wake the switches give it (the
+        CSR wake after a bend takes precedence over the 1D wake, as the loop
+        computes it last, :1344-1396)."""
         tau1 = 0.5 * e.length / e.nseg
+        w = None
+        if self.flagmap == 1:
+            w = self.csr_wake(e, (self.z if z is None else z) + tau1)
+            w = w if w is not None else self.wake()
         return hip.segment(tau1, e.type, e.values, e.mapstp, e.radius, e.radius,
-                           e.length, zedge, self.rfdata(e),
-                           self.wake() if self.flagmap == 1 else None)
+                           e.length, zedge, self.rfdata(e), w)
 
     def _window(self, start: int, nsteps: int) -> list:
         """The lattice items of steps start..start+nsteps-1 in order: the
@@ -263,8 +347,11 @@ class Simulation:
             if self._left == 0:
                 self._left, self.zedge = e.nseg, z
                 self._wake_entry(e)
+                self._csr_entry(e)
             self._left -= 1
-            out.append(self.segment_args(e, self.zedge))
+            out.append(self.segment_args(e, self.zedge, z))
+            if self._left == 0:
+                self._csr_exit(e)
             tau1 = 0.5 * e.length / e.nseg
             z = (z + tau1) + tau1
         return out
@@ -349,5 +436,8 @@ class Simulation:
         return np.array(rows)
 
     def close(self):
+        if self._isr_buf is not None:
+            self._isr_buf.close()
+            self._isr_buf = None
         self.bunch.close()
         self.field.close()

@@ -433,6 +433,9 @@ def diag_row(c, sums, amax, r2max, npt, refpt, mass):
     return out
 
 
+_SC_CURRENT = float(np.float32(1.0e-10))   # the loop's REAL(4) space-charge threshold
+
+
 class Run:
     """The reference segment loop over a lattice, on the C oracle."""
 
@@ -445,9 +448,17 @@ class Run:
         # aawk/ggwk/lengwk and the read-in table (src/Contrl/AccSimulator.f90:656-720)
         self.wk = {"on": 0, "read": 0, "btw": 0, "scale": 1.0, "aa": 0.0, "gg": 0.0,
                    "leng": 0.0, "table": None}
+        # the loop's CSR bookkeeping (src/Contrl/AccSimulator.f90:722-740,
+        # 1032-1056, 1131-1138, 1374-1385, 1498-1499): zbleng, sentr,
+        # flagbendcsr, bendlencsr and the last bend's radius r0
+        self.csr = {"zbleng": 0.0, "sentr": 0.0, "flagbend": 0, "bendlen": 0.0, "r0": 0.0}
+        # ISRchicane's standard-normal deviates (the reference draws them from
+        # the compiler's random_number): a generator the caller seeds, one
+        # deviate per particle id per bend step
+        self.isr_rng = None
         self.c = physconst(beam.freq)
         self.pts = np.ascontiguousarray(pts, dtype=np.float64).copy()
-        self.n = self.pts.shape[0]
+        self.n = self.n0 = self.pts.shape[0]
         self.grid = tuple(grid)
         self.lattice = lattice
         st = OrState()
@@ -497,6 +508,38 @@ class Run:
             self.wk["aa"], self.wk["gg"], self.wk["leng"] = v[11], v[12], v[13]
             self.wk["on"] = 1 if v[13] > 0.0 else 0
 
+    def _csr_entry(self, e):
+        """Entering a bend (src/Contrl/AccSimulator.f90:1032-1056,1066-1091):
+        the transient-drift flag, the bend's length for the elements after
+        it, sentr reset, and r0 = |1/hd0|."""
+        if e.type != 4:
+            return
+        v = e.values
+        if self.st.bcurr > 0.0 and v[2] > 500:
+            self.csr.update(flagbend=1, bendlen=e.length, sentr=0.0)
+        else:
+            self.csr["flagbend"] = 0
+        hd0 = v[0] / e.length
+        self.csr["r0"] = abs(1.0 / hd0)
+
+    def _csr_exit(self, e):
+        """After an element's last step (:1498-1499)."""
+        self.csr["zbleng"] += e.length
+        self.csr["sentr"] += e.length
+
+    def csr_wake(self, e, z):
+        """The CSR or_wake of a step of e whose first half push left z (the
+        loop's flagcsr, zwkmin and bendlen, :1131-1138,1374-1385), or None."""
+        k = self.csr
+        if e.type == 4:
+            if not e.values[2] > 200:
+                return None
+            return make_wake(3, r0=k["r0"], zoff=z - k["zbleng"], blength=e.length)
+        if not k["flagbend"]:
+            return None
+        return make_wake(3, r0=k["r0"], zoff=z - k["zbleng"] + k["sentr"],
+                         blength=k["bendlen"])
+
     def wake(self):
         """The or_wake of the current step, or None."""
         w = self.wk
@@ -528,13 +571,20 @@ class Run:
             # a new element: its entrance z (src/Contrl/AccSimulator.f90:1025)
             self._left, self.zedge = e.nseg, self.st.z
             self._wake_entry(e)
+            self._csr_entry(e)
         self._left -= 1
         if e.type == 4:
-            return self._bend_step(e, tau1, entering=self._left == e.nseg - 1)
+            out = self._bend_step(e, tau1, entering=self._left == e.nseg - 1, rho=rho,
+                                  phi=phi)
+            if self._left == 0:
+                self._csr_exit(e)
+            return out
         el = make_element(e.type, e.values, e.mapstp, e.length, self.zedge, self.rfdata(e))
         out = (_p(rho) if want_fields else None, _p(phi) if want_fields else None)
         if self.flagmap == 1:
-            w = self.wake()
+            # map1 moves z by tau1 before the field solve the wake follows
+            w = self.csr_wake(e, self.st.z + tau1) if self.st.bcurr >= _SC_CURRENT else None
+            w = w if w is not None else self.wake()
             n = lib().or_step(C.byref(self.c), _p(self.pts), self.n, C.byref(self.st), nx,
                               ny, nz, tau1, C.byref(el), e.radius,
                               C.byref(w) if w is not None else None, *out)
@@ -546,26 +596,61 @@ class Run:
         if n < 0:
             raise RuntimeError("or_step failed")
         self.n = n
+        if self._left == 0:
+            self._csr_exit(e)
         return rho, phi
 
-    def _bend_step(self, e, tau1, entering):
-        """A bend segment at zero current (src/Contrl/AccSimulator.f90:1032-1091,
-        1114-1120, 1434-1448, 1491-1494): the entrance face first, two half
-        sector maps, the exit face last; both integrators take this path."""
-        if self.st.bcurr >= float(np.float32(1.0e-10)):
-            raise ValueError("bends with space charge (ISR, CSR) are not restated")
+    def _bend_step(self, e, tau1, entering, rho=None, phi=None):
+        """A bend segment (src/Contrl/AccSimulator.f90:1032-1091, 1114-1138,
+        1141-1396, 1417-1448, 1491-1494): the entrance face first, a half
+        sector map, with current the space-charge block (pipe radius 100 m,
+        the CSR wake when v3 > 200) and kick, ISRchicane with the caller's
+        deviates, the second half sector map, the exit face last.  Zero
+        current runs under both integrators; current under Flagmap=1."""
+        st = self.st
         pts = self.pts[: self.n]
-        refpt = np.ctypeslib.as_array(self.st.refpt)
+        refpt = np.ctypeslib.as_array(st.refpt)
         if entering:
-            dipole_face(self.c, pts, refpt, e.length, e.values, False, self.st.charge,
-                        self.st.mass)
-        z = sector_dipole(self.c, pts, refpt, self.st.z, tau1, e.length, e.values,
-                          self.st.charge, self.st.mass)
-        self.st.z = sector_dipole(self.c, pts, refpt, z, tau1, e.length, e.values,
-                                  self.st.charge, self.st.mass)
+            dipole_face(self.c, pts, refpt, e.length, e.values, False, st.charge, st.mass)
+        z = sector_dipole(self.c, pts, refpt, st.z, tau1, e.length, e.values, st.charge,
+                          st.mass)
+        if st.bcurr >= _SC_CURRENT:
+            if self.flagmap != 1:
+                raise ValueError("bends with current are restated under Flagmap=1")
+            n, pr, st.npt, st.current = conv1st(self.c, pts, refpt, 100.0, 100.0, st.npt,
+                                                st.current)
+            self.n = n
+            pts = self.pts[:n]
+            rng, ms = update_compdom(pr, self.grid)
+            r = deposit(pts, self.grid, rng, ms)
+            ph = solve_openbc(r, ms)
+            if rho is not None:
+                rho[...] = r
+                phi[...] = ph
+            w = self.csr_wake(e, z)
+            w = w if w is not None else self.wake()
+            wk = None
+            if w is not None:
+                wk = wakefield(self.c, pts, rng, self.grid[2], st.current, st.npt, -refpt[5],
+                               st.charge, w)
+            cvbkforth1st(self.c, pts, refpt)
+            if wk is not None:
+                kick1wake(self.c, pts, ph, rng, ms, refpt, st.current, 2 * tau1, st.mass,
+                          st.charge, wk)
+            else:
+                kick1(self.c, pts, ph, rng, ms, refpt, st.current, 2 * tau1, st.mass,
+                      st.charge)
+        if st.bcurr > 1.0e-15:
+            if self.isr_rng is None:
+                raise ValueError("a bend with current needs ISR deviates (set isr_rng)")
+            rd = self.isr_rng.standard_normal(self.n0)
+            isr_chicane(self.c, pts, refpt, 2 * tau1, e.length, e.values, st.mass,
+                        rd[pts[:, 8].astype(np.int64) - 1])
+        st.z = sector_dipole(self.c, pts, refpt, z, tau1, e.length, e.values, st.charge,
+                             st.mass)
         if self._left == 0:
             self._exit = e
-        return None, None
+        return rho, phi
 
     def finish_element(self):
         """The exit face of a bend whose last step has run: the run loop

@@ -87,7 +87,7 @@ def test_deck_written_for_the_reference_reads_back(tmp_path):
     ("0.1 4 10 1 10.0 -20.0 0.15 /", "undulator"),
     ("0.1 4 10 2 30.0 20.0 5.0 -1.0 /", "ideal RF"),
     ("0.0 0 10 -1 1.0 /", "type -1"),
-    ("0.1 4 10 4 0.02 0.0 150 0.03 /", "type 4"),
+    ("0.0 0 2 -25 /\n0.1 4 10 4 0.02 0.0 150 0.03 /", "with current under Flagmap=2"),
 ])
 def test_deck_refuses_elements_off_the_path(tmp_path, line, what):
     from impactz.deck import DeckError, lattice_of, read_deck
@@ -261,26 +261,22 @@ def test_deck_wake_switches(tmp_path):
     assert isinstance(program_of(read_deck(str(p)))[0][0][1], WakeSwitch)
 
 
-def test_deck_bends_at_zero_current_only(tmp_path):
-    """Bends are taken at zero current; with current the reference adds
-    ISRchicane (random deviates) and CSR, so the deck is refused by name."""
+def test_deck_bends_with_and_without_current(tmp_path):
+    """Bends are taken at zero current and, under Flagmap=1, with current
+    (space charge, CSR, ISRchicane with seeded deviates, which the deck's
+    notes say); the bend's values reach the element in the deck's order."""
     from impactz import beam as B
-    from impactz.deck import DeckError, program_of, read_deck
+    from impactz.deck import program_of, read_deck
     bend = B.Element(0.8, 4, 1, 4, 0.25, 0.5, 0.0, 0.02, more=(0.1, 0.15, 0.5, -0.3, 0.4))
-    pts = B.gaussian_bunch(10, seed=1)
-    for cur, ok in ((0.0, True), (0.1, False)):
+    for cur in (0.0, 0.1):
         path = str(tmp_path / f"deck{cur}.in")
         B.write_deck(path, 10, (8, 8, 8), B.Beam(current=cur), [B.Element(0.3, 2, 1, 0, 1.0),
                                                                  bend])
-        deck = read_deck(path)
-        if ok:
-            prog, _ = program_of(deck)
-            e = prog[1][1]
-            assert e.type == 4 and e.values[:9] == pytest.approx(
-                [0.25, 0.5, 0.0, 0.02, 0.1, 0.15, 0.5, -0.3, 0.4])
-        else:
-            with pytest.raises(DeckError, match="bend"):
-                program_of(deck)
+        prog, notes = program_of(read_deck(path))
+        e = prog[1][1]
+        assert e.type == 4 and e.values[:9] == pytest.approx(
+            [0.25, 0.5, 0.0, 0.02, 0.1, 0.15, 0.5, -0.3, 0.4])
+        assert any("seeded generator" in n for n in notes) == (cur > 0)
 
 
 def test_deck_refuses_lorentz_bend_after_fourier_coefficients(tmp_path):

@@ -7,6 +7,7 @@ granular and fused, against the oracle's steps."""
 import numpy as np
 import pytest
 
+from conftest import col_err, full_row_err
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
@@ -62,3 +63,53 @@ def test_gpu_csr_wake_matches_oracle(gpu, oracle, zoff):
     assert err < 1e-12
     field.close()
     b.close()
+
+
+def _csr_lattice():
+    from impactz.beam import Element
+    # drift, a 0.4 m bend of 0.15 rad with v3 = 600 (CSR in the bend and the
+    # transient CSR in every element after it), half gap 2 cm, drift, quad
+    return [Element(0.1, 2, 1, 0, 1.0),
+            Element(0.4, 4, 1, 4, 0.15, 0.0, 600.0, 0.02, more=(0.0, 0.0, 0.0, 0.0, 0.0)),
+            Element(0.3, 3, 1, 0, 1.0),
+            Element(0.1, 2, 1, 1, 5.0, 0.0, 1.0)]
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_gpu_bend_with_current_and_csr_matches_oracle(gpu, oracle, fused):
+    """A bend with current in the loop (src/Contrl/AccSimulator.f90:1032-1138,
+    1344-1448): space charge with the 100 m bend pipe, the CSR wake inside
+    the bend, ISRchicane (the same seeded deviates on both sides), then the
+    transient CSR wake in the drift and quadrupole after it (flagbendcsr,
+    zwkmin = range(5)/gamma + z - zbleng + sentr).  fused: the steps after
+    the bend run through izh_run with a CSR wake per segment."""
+    from impactz.sim import Simulation
+    beam, pts = _bunch(20_000, 11)
+    grid = (16, 16, 32)
+    lat = _csr_lattice()
+    sim = Simulation(pts, beam, lat, grid)
+    run = oracle.Run(pts, beam, lat, grid)
+    sim.isr_rng = np.random.default_rng(9)
+    run.isr_rng = np.random.default_rng(9)
+    steps = list(sim.segments())
+    n_before = 2 + 4    # the drift and the bend run step by step
+    worst = 0.0
+    try:
+        rows = []
+        for k, e in enumerate(steps[:n_before] if fused else steps):
+            print(f"csr lattice step {k + 1} type {e.type}", flush=True)
+            rows.append(sim.diag(sim.step(e)))
+        if fused:
+            recs = sim.run_fused(len(steps) - n_before, start=n_before)
+            rows += [sim.diag(r) for r in recs]
+        for k, e in enumerate(steps):
+            run.step(e)
+            worst = max(worst, full_row_err(rows[k], run.diag()))
+        pe = col_err(sim.bunch.getpts(), run.particles())
+        record(test="csr_lattice", fused=fused, moments=worst, particles=pe)
+        assert worst < TOL and pe < TOL, (worst, pe)
+        # the CSR wakes ran: the bunch's energy spread changed beyond the
+        # zero-current map's (checked in the oracle's own loop)
+        assert run.csr["flagbend"] == 1 and run.csr["bendlen"] == 0.4
+    finally:
+        sim.close()

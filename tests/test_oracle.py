@@ -911,3 +911,44 @@ def test_oracle_csr_wake_kind_in_the_loop(oracle):
     ref = oracle.csr_wake(8.0, zmin / gam + 0.3, hz / gam, 0.4, rho, gam)
     assert np.all(out[0] == 0) and np.all(out[1] == 0)
     assert np.max(np.abs(out[2] - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_oracle_csr_switches_follow_the_loop(oracle):
+    """The loop's CSR bookkeeping on drift, bend (v3 = 600), drift, quad with
+    current: inside the bend the wake's offset is z - zbleng (the bend's
+    entrance) and its length the bend's; after it, in every element, the
+    offset is measured from the bend's entrance (z - zbleng + sentr) with the
+    bend's length; a bend with v3 <= 500 switches the transient off, one with
+    v3 <= 200 has no CSR of its own, and no current means no CSR."""
+    from impactz.beam import Beam, Element
+    beam = Beam(current=1.0, kinetic=100e6, mass=0.51099895e6, charge=-1.0, freq=1.3e9)
+    lat = [Element(0.1, 2, 1, 0, 1.0),
+           Element(0.4, 4, 1, 4, 0.15, 0.0, 600.0, 0.02),
+           Element(0.3, 3, 1, 0, 1.0),
+           Element(0.2, 2, 1, 4, 0.1, 0.0, 150.0, 0.02),
+           Element(0.1, 1, 1, 0, 1.0)]
+    run = oracle.Run(np.zeros((1, 9)), beam, lat, (8, 8, 8))
+    z0 = 0.0
+    seen = []
+    for e in lat:
+        run._left, run.zedge = e.nseg, z0
+        run._csr_entry(e)
+        tau1 = 0.5 * e.length / e.nseg
+        for j in range(e.nseg):
+            zmid = z0 + (2 * j + 1) * tau1
+            w = run.csr_wake(e, zmid)
+            seen.append(None if w is None else (w.kind, w.r0, w.zoff, w.blength, zmid))
+        run._csr_exit(e)
+        z0 += e.length
+    r0 = abs(1.0 / (0.15 / 0.4))
+    # drift before the bend: no CSR
+    assert seen[:2] == [None, None]
+    for k in range(2, 6):          # in the bend: offset from its entrance at 0.1
+        kind, rr, zoff, blen, zmid = seen[k]
+        assert (kind, rr, blen) == (3, r0, 0.4) and zoff == pytest.approx(zmid - 0.1)
+    for k in range(6, 9):          # the drift after it: still from the entrance
+        kind, rr, zoff, blen, zmid = seen[k]
+        assert (kind, rr, blen) == (3, r0, 0.4) and zoff == pytest.approx(zmid - 0.1)
+    # a weak bend (v3 = 150): no CSR in it, and its entry switched the
+    # transient off for the drift after it
+    assert seen[9:] == [None, None, None]
