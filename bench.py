@@ -168,57 +168,71 @@ def roofline(res, n, g):
             "avg_ms": km[kern]}
 
 
-def _ref_step_time(exe, tmp, tag, n, grid, steps):
-    """Per-step wall time of the reference: (t(steps[1]) - t(steps[0])) / diff,
-    so deck parsing and the particle.in read cancel (fort.* writes stay in)."""
-    pts = B.core_halo_bunch(n, seed=1)
-    pin = os.path.join(tmp, f"{tag}_particle.in")
-    B.write_particle_in(pin, pts)
+def _ref_step_time(exe, tmp, tag, pts, grid, steps):
+    """Per-step wall time of the reference: (t(steps[1]) - t(steps[0])) /
+    difference, so start-up and the particle read cancel (the per-step
+    fort.* writes stay in).  Particles go in as the binary fort.40 of
+    readPar_Dist (Flagdist=199, no coordinate offsets)."""
+    from impactz import particle_io as P
+    n = pts.shape[0]
+    dp = [0.001, 40.0, 0.0, 1.0, 1.0, 0.0, 0.0, 0.001, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0,
+          0.1, 0.001, 0.0, 1.0, 1.0, 0.0, 0.0]
     times = []
     for s in steps:
         d = os.path.join(tmp, f"{tag}_s{s}")
         os.makedirs(d)
-        os.symlink(pin, os.path.join(d, "particle.in"))
+        P.write_binary_particles(os.path.join(d, "fort.40"), pts)
         B.write_deck(os.path.join(d, "ImpactZ.in"), n, grid, B.Beam(),
-                     [B.Element(0.4, s, 1, 0, 1.0)])
+                     [B.Element(0.4, s, 1, 0, 1.0)], flagdist=199, distparam=dp)
         t0 = time.perf_counter()
         subprocess.run(f"ulimit -s unlimited; exec {exe}", shell=True, cwd=d,
                        check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         times.append(time.perf_counter() - t0)
         log(f"cpu reference {tag}: n={n} grid={grid} {s} steps {times[-1]:.2f} s")
+        shutil.rmtree(d, ignore_errors=True)
     return (times[1] - times[0]) / (steps[1] - steps[0])
 
 
-def cpu_baseline(grid, n_full: int, n1=20_000, n2=1_000_000, small=(32, 32, 32)):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(grid, n_full: int, n1=20_000, n2=2_000_000, steps=(1, 4)):
     """The reference executable (built from source by oracle/Makefile.ref) on
     one host core, on a bounded sample of the workload, scaled to it.  The
-    per-step time is t(n) = a + n*b: a, the grid work (Green function, FFTs),
-    depends on the grid; b, the work per particle, does not.  Each time is
-    the difference of a long and a short run, so start-up and particle.in
-    parsing cancel:
-      b from n1 and n2 particles on a small grid (where the particle work
-        dominates): b = (t_small(n2) - t_small(n1)) / (n2 - n1);
-      a from n1 particles on the workload's grid: a = t(n1) - n1*b;
-      reported rate = n_full / (a + n_full*b)."""
+    per-step time on the workload's grid is t(n) = a + n*b: a, the grid work
+    (Green function, FFTs), b the work per particle (its deposit and gather
+    go to this grid's arrays).  Both come from the grid itself: t(n1) and
+    t(n2) for n1 and n2 particles of the workload's distribution, each the
+    difference of a 4-step and a 1-step run (start-up and the particle read
+    cancel); b = (t(n2) - t(n1)) / (n2 - n1), a = t(n1) - n1*b, and the
+    reported rate = n_full / (a + n_full*b)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ImpactZexe")
     if not os.access(exe, os.X_OK):
         return None
     tmp = tempfile.mkdtemp(prefix="impz_cpu_")
     try:
-        s_lo = _ref_step_time(exe, tmp, "small_n1", n1, small, (1, 5))
-        s_hi = _ref_step_time(exe, tmp, "small_n2", n2, small, (1, 5))
-        b = max((s_hi - s_lo) / (n2 - n1), 0.0)
-        t1 = _ref_step_time(exe, tmp, "full_n1", n1, grid, (1, 2))
+        pts = B.core_halo_bunch(n2, seed=1)
+        t1 = _ref_step_time(exe, tmp, "n1", np.ascontiguousarray(pts[:n1]), grid, steps)
+        t2 = _ref_step_time(exe, tmp, "n2", pts, grid, steps)
+        b = max((t2 - t1) / (n2 - n1), 0.0)
         a = max(t1 - n1 * b, 0.0)
         per_step = a + n_full * b
         return {"value": n_full / per_step, "unit": "particle-steps/s", "cores": 1,
-                "kind": "reference",
-                "sample": (f"ImpactZexe (serial build, amdflang -O3, 1 core): particle work "
-                           f"b={b * 1e6:.3f} us/particle-step from {n1} and {n2} particles "
-                           f"on a {small[0]}x{small[1]}x{small[2]} grid (1 and 5 steps), grid "
-                           f"work a={a:.3f} s/step from {n1} particles on the "
-                           f"{grid[0]}x{grid[1]}x{grid[2]} grid (1 and 2 steps); rate = "
-                           f"{n_full}/(a + {n_full}*b) (fort.* writes included)")}
+                "kind": "reference", "cpu_model": _cpu_model(),
+                "sample": (f"ImpactZexe (serial build, amdflang -O3, 1 core of {_cpu_model()}) "
+                           f"on the {grid[0]}x{grid[1]}x{grid[2]} grid: {n1} and {n2} particles "
+                           f"of the workload's distribution, {steps[0]} and {steps[1]} steps "
+                           f"each; grid work a={a:.3f} s/step, particle work "
+                           f"b={b * 1e6:.3f} us/particle-step; rate = {n_full}/(a + {n_full}*b) "
+                           f"(fort.* writes included)")}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
