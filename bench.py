@@ -426,11 +426,6 @@ def main():
     grid = tuple(int(v) for v in args.grid.split(","))
     n = args.particles
     g = grid[0] * grid[1] * grid[2]
-    # The scans keep one stream per job in flight; HIP maps a process's
-    # streams onto this many hardware queues (read once, when the runtime
-    # starts).  16 instead of 4 cut the served scan's p50 latency by a
-    # quarter on this workload; the single-bunch step uses one stream.
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     hip.init(0)
     beam = B.Beam()
     total = max(args.steps + args.warmup, args.accuracy_steps)

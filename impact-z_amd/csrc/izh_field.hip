@@ -522,14 +522,19 @@ k_deposit_chunks(const P *__restrict__ X, const P *__restrict__ Y,
 // at most kChunk slots grouped under one sort tile (the tile of the
 // particles' previous position, in the grid *kgrid that key was computed
 // in).  The block accumulates in an LDS window of the tile's nodes, mapped
-// into this step's grid and widened by kTileMargin nodes on every side (plus
-// the upper CIC node), so no pass over the positions is needed to bound
-// them; particles that moved further add to the grid directly and are
-// counted in *spill.  Same fixed-point arithmetic and node order as
-// k_deposit_chunks.
-constexpr int kTileMargin = 3;
-constexpr int kTileWinEdge = (1 << kTileShift) + 1 + 2 * kTileMargin;
-constexpr int kTileWinCells = kTileWinEdge * kTileWinEdge * kTileWinEdge;
+// into this step's grid and widened on every side (plus the upper CIC node)
+// by kTileMarginT nodes across the beam and kTileMarginL along it, so no
+// pass over the positions is needed to bound them; particles that moved
+// further add to the grid directly and are counted in *spill.  The margins
+// follow how far particles move in cells between the key and the deposit
+// (one or two steps): transversely several cells in a drift step, along the
+// bunch (the t-frame phase) about one.  Same fixed-point arithmetic and
+// node order as k_deposit_chunks.
+constexpr int kTileMarginT = 6;
+constexpr int kTileMarginL = 2;
+constexpr int kTileWinEdgeT = (1 << kTileShift) + 1 + 2 * kTileMarginT;
+constexpr int kTileWinEdgeL = (1 << kTileShift) + 1 + 2 * kTileMarginL;
+constexpr int kTileWinCells = kTileWinEdgeT * kTileWinEdgeT * kTileWinEdgeL;
 
 template <typename P>
 __global__ void __launch_bounds__(kBlock)
@@ -546,16 +551,16 @@ k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
   const int tz = (int)(ch.tile / ((uint32_t)tnx * (uint32_t)tny));
   const int rem = (int)(ch.tile % ((uint32_t)tnx * (uint32_t)tny));
   const int ty = rem / tnx, tx = rem % tnx;
-  const int wx = min((1 << s0) + 1 + 2 * kTileMargin, nx);
-  const int wy = min((1 << s1) + 1 + 2 * kTileMargin, ny);
-  const int wz = min((1 << s2) + 1 + 2 * kTileMargin, nz);
+  const int wx = min((1 << s0) + 1 + 2 * kTileMarginT, nx);
+  const int wy = min((1 << s1) + 1 + 2 * kTileMarginT, ny);
+  const int wz = min((1 << s2) + 1 + 2 * kTileMarginL, nz);
   const KeyGrid kg = *kgrid;
   const int bx = min(max(tile_first_node(tx << s0, kg.lo[0], kg.h[0], dom->range[0],
-                                         dom->hinv[0]) - kTileMargin, 0), nx - wx);
+                                         dom->hinv[0]) - kTileMarginT, 0), nx - wx);
   const int by = min(max(tile_first_node(ty << s1, kg.lo[1], kg.h[1], dom->range[2],
-                                         dom->hinv[1]) - kTileMargin, 0), ny - wy);
+                                         dom->hinv[1]) - kTileMarginT, 0), ny - wy);
   const int bz = min(max(tile_first_node(tz << s2, kg.lo[2], kg.h[2], dom->range[4],
-                                         dom->hinv[2]) - kTileMargin, 0), nz - wz);
+                                         dom->hinv[2]) - kTileMarginL, 0), nz - wz);
   const int wcells = wx * wy * wz;
   for (int c = threadIdx.x; c < wcells; c += kBlock) win[c] = 0ull;
   __syncthreads();

@@ -249,14 +249,16 @@ __device__ inline void block_range(int64_t n, int64_t &lo, int64_t &hi) {
   hi = lo + per < n ? lo + per : n;
 }
 
-// E of a kGBox^3-node neighbourhood of the block's current sort tile, staged
-// in LDS so the 8-node gather of the dense core reads LDS instead of the
-// vector L1 (whose tag rate bounds a gather over 8 scattered nodes).
-// Particles whose stencil leaves the box read global memory; both paths sum
-// the same values in the same order.
-constexpr int kGBox = 12;
-constexpr int kGBoxCells = kGBox * kGBox * kGBox;
-constexpr int kGBoxLow = 2;  // nodes below the tile's first node
+// E of a neighbourhood of the block's current sort tile (kGBoxT nodes
+// across the beam, kGBoxL along it), staged in LDS so the 8-node gather of
+// the dense core reads LDS instead of the vector L1 (whose tag rate bounds a
+// gather over 8 scattered nodes).  Particles whose stencil leaves the box
+// read global memory; both paths sum the same values in the same order.
+constexpr int kGBoxT = 12;
+constexpr int kGBoxL = 12;
+constexpr int kGBoxCells = kGBoxT * kGBoxT * kGBoxL;
+constexpr int kGBoxLowT = 2;  // nodes below the tile's first node, x and y
+constexpr int kGBoxLowL = 2;  // and z
 
 template <typename T> struct GatherLds;
 template <> struct GatherLds<double> {
@@ -358,7 +360,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
   const E4<T> *E = (const E4<T> *)c.e;
   T *tpx = (T *)c.tpos;
   __shared__ GatherLds<T> gbox;
-  const int bdx = min(kGBox, c.nx), bdy = min(kGBox, c.ny), bdz = min(kGBox, c.nz);
+  const int bdx = min(kGBoxT, c.nx), bdy = min(kGBoxT, c.ny), bdz = min(kGBoxL, c.nz);
   uint32_t box_key = kDeadKey;   // tile the staged box was built for
   int ox = 0, oy = 0, oz = 0;    // its first node
   PairMom mom;
@@ -454,11 +456,11 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
         // the tile's first node in the grid of this E field
         const KeyGrid &kg = *c.kgrid;
         ox = min(max(tile_first_node(tx << c.tshift[0], kg.lo[0], kg.h[0], xmin, hxi) -
-                     kGBoxLow, 0), c.nx - bdx);
+                     kGBoxLowT, 0), c.nx - bdx);
         oy = min(max(tile_first_node(ty << c.tshift[1], kg.lo[1], kg.h[1], ymin, hyi) -
-                     kGBoxLow, 0), c.ny - bdy);
+                     kGBoxLowT, 0), c.ny - bdy);
         oz = min(max(tile_first_node(tz << c.tshift[2], kg.lo[2], kg.h[2], zmin, hzi) -
-                     kGBoxLow, 0), c.nz - bdz);
+                     kGBoxLowL, 0), c.nz - bdz);
         __syncthreads();  // the previous box is no longer read
         const int bxy = bdx * bdy, nb = bxy * bdz;
         for (int q0 = threadIdx.x; q0 < nb; q0 += 2 * kBlock) {
