@@ -12,11 +12,11 @@
 // Only the first octant of the result is ever stored.  G^ of the even
 // extended integrated Green function is real and even, so it is built and
 // stored on the (nx+1)(ny+1)(nz+1) half-grid and folded on use.
-// Each block transforms B lines of length L (B*L = 2048 points) in LDS with
-// in-place radix-2 stages: decimation in frequency for forward transforms
-// (natural in, bit-reversed out) and decimation in time for the inverse of
-// the fused z pass (bit-reversed in, natural out), so no reorder pass is
-// needed between them.
+// Each block transforms B lines of length L (B*L = 2048 points, 4096 in the
+// fused z pass) in LDS with in-place radix-2 stages: decimation in
+// frequency for forward transforms (natural in, bit-reversed out) and
+// decimation in time for the inverse of the fused z pass (bit-reversed in,
+// natural out), so no reorder pass is needed between them.
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -49,9 +49,11 @@ struct Lines {
   const void *src;
   void *dst;
   int64_t nf, ns;          // lines = nf * ns, f fastest
+  int64_t nfv;             // lines with f >= nfv are padding: not read or written
   int64_t sf, ss, se;      // source strides (elements)
   int64_t df, ds, de;      // destination strides
-  int L, log2L, B, Lin, Lout;
+  int L, log2L, B, log2B, Lin, Lout;
+  uint64_t nf_mul;         // ceil(2^40 / nf): line / nf by a multiply
   double out_scale;
   // fused z pass: multiply by the half-grid G^ folded in the line's y index
   const void *green;
@@ -71,31 +73,39 @@ __device__ inline int bitrev(int v, int bits) {
 // three stages.  Every element sees the same butterflies with the same
 // twiddles, in the same order, as the stage-by-stage transform.  tw: L/2
 // twiddles exp(-2 pi i k / L) in LDS.
+// Position of point m in its line's LDS row.  Without it the radix-8 groups
+// of the later runs put the lanes of a wavefront on a few banks (group
+// stride 1: 16-byte points 128 bytes apart); XOR-ing the low 4 bits with
+// m >> 3 spreads every access pattern of the kernel over all banks.  mask
+// is 15 for lines of 64 points or more, else 0 (short lines do no better).
+__device__ inline int lds_pos(int m, int mask) { return m ^ ((m >> 3) & mask); }
+
 template <typename T, bool DIT, int K>
-__device__ inline void fft_run(cx<T> *buf, int L, int LP, int lines, const cx<T> *tw,
-                               bool inverse, int done) {
+__device__ inline void fft_run(cx<T> *buf, int log2L, int LP, int lines, const cx<T> *tw,
+                               bool inverse, int done, int mask) {
   constexpr int span = 1 << K;
-  // DIF runs halves L/2 .. 1 downwards, DIT 1 .. L/2 upwards
-  const int s = DIT ? (1 << done) : (L >> (done + K));  // group stride
-  const int bsize = s * span;                          // block of a group
-  const int per_line = L >> K;
-  const int ngroups = lines * per_line;
+  // DIF runs halves L/2 .. 1 downwards, DIT 1 .. L/2 upwards; every size
+  // here is a power of two, so the group index splits with shifts and masks
+  const int ls = DIT ? done : log2L - done - K;  // log2 of the group stride
+  const int s = 1 << ls;
+  const int lpl = log2L - K;                     // log2 of the groups per line
+  const int ngroups = lines << lpl;
   for (int g = threadIdx.x; g < ngroups; g += kFftThreads) {
-    const int lb = g / per_line, q = g - lb * per_line;
-    const int blk = q / s, r = q - blk * s;
-    const int base = lb * LP + blk * bsize + r;
+    const int lb = g >> lpl, q = g & ((1 << lpl) - 1);
+    const int blk = q >> ls, r = q & (s - 1);
+    const int row = lb * LP, m0 = (blk << (ls + K)) + r;
     cx<T> v[span];
 #pragma unroll
-    for (int j = 0; j < span; j++) v[j] = buf[base + j * s];
+    for (int j = 0; j < span; j++) v[j] = buf[row + lds_pos(m0 + j * s, mask)];
 #pragma unroll
     for (int st = 0; st < K; st++) {
       const int e = DIT ? st : K - 1 - st;  // pair distance 2^e in the group
-      const int tstride = L / (2 * (s << e));
+      const int lt = log2L - 1 - ls - e;    // log2 of the twiddle stride
 #pragma unroll
       for (int j = 0; j < span; j++) {
         if ((j >> e) & 1) continue;
-        const int k = r + (j & ((1 << e) - 1)) * s;
-        cx<T> w = tw[k * tstride];
+        const int k = r + ((j & ((1 << e) - 1)) << ls);
+        cx<T> w = tw[k << lt];
         if (inverse) w.im = -w.im;
         const cx<T> u = v[j], x = v[j + (1 << e)];
         if (DIT) {
@@ -109,25 +119,32 @@ __device__ inline void fft_run(cx<T> *buf, int L, int LP, int lines, const cx<T>
       }
     }
 #pragma unroll
-    for (int j = 0; j < span; j++) buf[base + j * s] = v[j];
+    for (int j = 0; j < span; j++) buf[row + lds_pos(m0 + j * s, mask)] = v[j];
   }
   __syncthreads();
 }
 
 template <typename T, bool DIT>
-__device__ void fft_stages(cx<T> *buf, int L, int LP, int log2L, int lines, const cx<T> *tw,
-                           bool inverse) {
+__device__ void fft_stages(cx<T> *buf, int LP, int log2L, int lines, const cx<T> *tw,
+                           bool inverse, int mask) {
   int done = 0;
   while (done < log2L) {
     const int k = min(3, log2L - done);
     if (k == 3)
-      fft_run<T, DIT, 3>(buf, L, LP, lines, tw, inverse, done);
+      fft_run<T, DIT, 3>(buf, log2L, LP, lines, tw, inverse, done, mask);
     else if (k == 2)
-      fft_run<T, DIT, 2>(buf, L, LP, lines, tw, inverse, done);
+      fft_run<T, DIT, 2>(buf, log2L, LP, lines, tw, inverse, done, mask);
     else
-      fft_run<T, DIT, 1>(buf, L, LP, lines, tw, inverse, done);
+      fft_run<T, DIT, 1>(buf, log2L, LP, lines, tw, inverse, done, mask);
     done += k;
   }
+}
+
+// line = f + nf * s for f < nf: s by a multiply with nf_mul = ceil(2^40 / nf),
+// exact for line < 2^24 and nf < 2^16 (run_lines checks both).
+__device__ inline void line_fs(int line, const Lines &p, int &f, int &s) {
+  s = (int)(((uint64_t)(uint32_t)line * p.nf_mul) >> 40);
+  f = line - s * (int)p.nf;
 }
 
 // Thread t of a block -> (line in block, element) with coalesced addresses:
@@ -139,8 +156,8 @@ __device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &l
     lb = t >> p.log2L;
     m = t & (p.L - 1);
   } else {
-    m = t / p.B;
-    lb = t - m * p.B;
+    m = t >> p.log2B;
+    lb = t & (p.B - 1);
   }
 }
 
@@ -158,6 +175,7 @@ __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   __shared__ cx<T> buf[PTS + PTS / kFftMinLine];
   const int LP = p.L + 1;
+  const int mask = p.L >= 64 ? 15 : 0;
   extern __shared__ __align__(16) unsigned char fft_dyn[];  // L/2 twiddles
   cx<T> *tw = reinterpret_cast<cx<T> *>(fft_dyn);
   // twiddles are written to LDS together with the line data below
@@ -187,16 +205,19 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     int lb, m;
     map_thread(t, p.se, p, lb, m);
     const int line = line0 + lb;
-    slot[it] = lb * LP + m;
+    slot[it] = lb * LP + lds_pos(m, mask);
     T re = (T)0, im = (T)0;
     if (PAIR) {
       const int l0 = 2 * line, l1 = l0 + 1;
       const T *src = (const T *)p.src;
       const cx<T> *csrc = (const cx<T> *)p.src;
       if (l0 < nlines) {
-        const int64_t b0 = (l0 % nf) * p.sf + (l0 / nf) * p.ss;
+        int f0, s0, f1, s1;
+        line_fs(l0, p, f0, s0);
+        line_fs(l1, p, f1, s1);
+        const int64_t b0 = f0 * p.sf + s0 * p.ss;
         const bool has1 = l1 < nlines;
-        const int64_t b1 = has1 ? (l1 % nf) * p.sf + (l1 / nf) * p.ss : b0;
+        const int64_t b1 = has1 ? f1 * p.sf + s1 * p.ss : b0;
         if (LOAD == kLoadEven) {
           const int mm = m <= (p.L >> 1) ? m : p.L - m;
           re = src[b0 + mm * p.se];
@@ -217,9 +238,12 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
         }
       }
     } else if (line < nlines) {
-      const int f = line % nf, s = line / nf;
+      int f, s;
+      line_fs(line, p, f, s);
       const int64_t base = f * p.sf + s * p.ss;
-      if (LOAD == kLoadReal) {
+      if (f >= p.nfv) {
+        // a padding line: zeros
+      } else if (LOAD == kLoadReal) {
         if (m < p.Lin) re = ((const T *)p.src)[base + m * p.se];
       } else if (LOAD == kLoadCplx) {
         if (m < p.Lin) {
@@ -256,11 +280,12 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t / p.B, lb = t - kz * p.B;
+      const int kz = t >> p.log2B, lb = t & (p.B - 1);
       const int line = line0 + lb;
+      int i, j2;
+      line_fs(line, p, i, j2);
       T g = (T)0;
-      if (line < nlines) {
-        const int i = line % nf, j2 = line / nf;
+      if (line < nlines && i < p.nfv) {
         const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
         const int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
         g = G[i + p.gkx * jf + p.gky * kf];
@@ -268,18 +293,18 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       gv[it] = g;
     }
   }
-  fft_stages<T, false>(buf, p.L, LP, p.log2L, p.B, tw, inverse != 0);
+  fft_stages<T, false>(buf, LP, p.log2L, p.B, tw, inverse != 0, mask);
   if (CONV) {
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t / p.B, lb = t - kz * p.B;
-      const int q = lb * LP + bitrev(kz, p.log2L);
+      const int kz = t >> p.log2B, lb = t & (p.B - 1);
+      const int q = lb * LP + lds_pos(bitrev(kz, p.log2L), mask);
       const cx<T> v = buf[q];
       buf[q] = {v.re * gv[it], v.im * gv[it]};
     }
     __syncthreads();
-    fft_stages<T, true>(buf, p.L, LP, p.log2L, p.B, tw, true);
+    fft_stages<T, true>(buf, LP, p.log2L, p.B, tw, true, mask);
   }
   // ---- store the kept part of each line
   const int totout = p.B * p.L;
@@ -291,15 +316,18 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       const int l0 = 2 * (line0 + lb), l1 = l0 + 1;
       if (l0 >= nlines) continue;
       const bool has1 = l1 < nlines;
-      const int64_t o0 = (l0 % nf) * p.df + (l0 / nf) * p.ds + m * p.de;
-      const int64_t o1 = has1 ? (l1 % nf) * p.df + (l1 / nf) * p.ds + m * p.de : o0;
-      const cx<T> z = buf[lb * LP + bitrev(m, p.log2L)];
+      int f0, s0, f1, s1;
+      line_fs(l0, p, f0, s0);
+      line_fs(l1, p, f1, s1);
+      const int64_t o0 = f0 * p.df + s0 * p.ds + m * p.de;
+      const int64_t o1 = has1 ? f1 * p.df + s1 * p.ds + m * p.de : o0;
+      const cx<T> z = buf[lb * LP + lds_pos(bitrev(m, p.log2L), mask)];
       if (STORE == kStoreReal) {
         ((T *)p.dst)[o0] = (T)(z.re * p.out_scale);
         if (has1) ((T *)p.dst)[o1] = (T)(z.im * p.out_scale);
       } else {
         // X(m) = (Z(m) + conj Z(L-m)) / 2,  Y(m) = (Z(m) - conj Z(L-m)) / 2i
-        const cx<T> w = buf[lb * LP + bitrev((p.L - m) & (p.L - 1), p.log2L)];
+        const cx<T> w = buf[lb * LP + lds_pos(bitrev((p.L - m) & (p.L - 1), p.log2L), mask)];
         const T h = (T)(0.5 * p.out_scale);
         ((cx<T> *)p.dst)[o0] = {(z.re + w.re) * h, (z.im - w.im) * h};
         if (has1) ((cx<T> *)p.dst)[o1] = {(z.im + w.im) * h, (w.re - z.re) * h};
@@ -308,9 +336,11 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     }
     int line = line0 + lb;
     if (line >= nlines) continue;
-    int f = line % nf, s = line / nf;
+    int f, s;
+    line_fs(line, p, f, s);
+    if (f >= p.nfv) continue;
     int64_t o = f * p.df + s * p.ds + m * p.de;
-    cx<T> v = CONV ? buf[lb * LP + m] : buf[lb * LP + bitrev(m, p.log2L)];
+    cx<T> v = buf[lb * LP + lds_pos(CONV ? m : bitrev(m, p.log2L), mask)];
     if (STORE == kStoreCplx) {
       ((cx<T> *)p.dst)[o] = {(T)(v.re * p.out_scale), (T)(v.im * p.out_scale)};
     } else {
@@ -677,9 +707,17 @@ int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
   // spectrum and of G^ are read across the lines
   constexpr int pts = CONV ? kFftPointsZ : kFftPoints;
   p.log2L = ilog2(p.L);
-  if (p.L < kFftMinLine || p.L > pts)
-    return fail(IZH_ERR_ARG, "FFT line of %d points outside %d..%d", p.L, kFftMinLine, pts);
+  if (p.nfv == 0) p.nfv = p.nf;
+  if (p.L < kFftMinLine || p.L > pts || (1 << p.log2L) != p.L)
+    return fail(IZH_ERR_ARG, "FFT line of %d points: not a power of two in %d..%d", p.L,
+                kFftMinLine, pts);
   p.B = pts / p.L;
+  p.log2B = ilog2(p.B);
+  // line / nf is a multiply by ceil(2^40 / nf), exact below these bounds
+  if (p.nf < 1 || p.nf >= (1 << 16) || 2 * p.nf * p.ns >= (1 << 24))
+    return fail(IZH_ERR_ARG, "FFT of %lld x %lld lines outside the kernel's index range",
+                (long long)p.nf, (long long)p.ns);
+  p.nf_mul = ((1ull << 40) + (uint64_t)p.nf - 1) / (uint64_t)p.nf;
   const int64_t nlines = p.nf * p.ns;
   const int64_t nfft = PAIR ? (nlines + 1) / 2 : nlines;
   const int64_t nblk = (nfft + p.B - 1) / p.B;
@@ -730,41 +768,45 @@ int solve_t(izh_field *f) {
     IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_z, false)));
   }
   ScopedTimer tm_(IZH_T_FFT, f->stream);
-  // ---- forward x: rho (nx,ny,nz) real -> A (kx, ny, nz)
+  // The half spectra hold x rows of ld = f->ldx >= kx values.  The y and z
+  // passes run over all ld x lines, so a block's run of neighbouring lines
+  // starts on a 128-byte boundary; lines kx..ld-1 are padding.
+  const int64_t ld = f->ldx;
+  // ---- forward x: rho (nx,ny,nz) real -> A (kx of ld, ny, nz)
   Lines p{};
   p.src = f->rho; p.dst = f->spec_a;
   p.nf = ny; p.ns = nz;
   p.sf = nx; p.ss = (int64_t)nx * ny; p.se = 1;
-  p.df = kx; p.ds = (int64_t)kx * ny; p.de = 1;
+  p.df = ld; p.ds = ld * ny; p.de = 1;
   p.L = mx; p.Lin = nx; p.Lout = kx; p.out_scale = 1.0;
   IZH_CHECK((run_lines<T, kLoadReal, kStoreCplx, false, true>(f, p, f->tw_x, false)));
-  // ---- forward y: A -> B (kx, my, nz)
+  // ---- forward y: A -> B (kx of ld, my, nz)
   p.src = f->spec_a; p.dst = f->spec_b;
-  p.nf = kx; p.ns = nz;
-  p.sf = 1; p.ss = (int64_t)kx * ny; p.se = kx;
-  p.df = 1; p.ds = (int64_t)kx * my; p.de = kx;
+  p.nf = ld; p.nfv = kx; p.ns = nz;
+  p.sf = 1; p.ss = ld * ny; p.se = ld;
+  p.df = 1; p.ds = ld * my; p.de = ld;
   p.L = my; p.Lin = ny; p.Lout = my;
   IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(f, p, f->tw_y, false)));
   // ---- z: FFT, * G^, IFFT, keep k < nz; in place in B
   p.src = f->spec_b; p.dst = f->spec_b;
-  p.nf = kx; p.ns = my;
-  p.sf = 1; p.ss = kx; p.se = (int64_t)kx * my;
-  p.df = 1; p.ds = kx; p.de = (int64_t)kx * my;
+  p.nf = ld; p.nfv = kx; p.ns = my;
+  p.sf = 1; p.ss = ld; p.se = ld * my;
+  p.df = 1; p.ds = ld; p.de = ld * my;
   p.L = mz; p.Lin = nz; p.Lout = nz;
   p.green = f->green; p.gkx = kx; p.gky = (int64_t)kx * (ny + 1); p.my = my;
   IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, true>(f, p, f->tw_z, false)));
   // ---- inverse y: B -> A, keep j < ny
   p.green = nullptr;
   p.src = f->spec_b; p.dst = f->spec_a;
-  p.nf = kx; p.ns = nz;
-  p.sf = 1; p.ss = (int64_t)kx * my; p.se = kx;
-  p.df = 1; p.ds = (int64_t)kx * ny; p.de = kx;
+  p.nf = ld; p.nfv = kx; p.ns = nz;
+  p.sf = 1; p.ss = ld * my; p.se = ld;
+  p.df = 1; p.ds = ld * ny; p.de = ld;
   p.L = my; p.Lin = my; p.Lout = ny;
   IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(f, p, f->tw_y, true)));
   // ---- inverse x (C2R): A -> phi, keep i < nx
   p.src = f->spec_a; p.dst = f->phi;
-  p.nf = ny; p.ns = nz;
-  p.sf = kx; p.ss = (int64_t)kx * ny; p.se = 1;
+  p.nf = ny; p.nfv = ny; p.ns = nz;
+  p.sf = ld; p.ss = ld * ny; p.se = 1;
   p.df = nx; p.ds = (int64_t)nx * ny; p.de = 1;
   p.L = mx; p.Lin = mx; p.Lout = nx;
   IZH_CHECK((run_lines<T, kLoadHerm, kStoreReal, false, true>(f, p, f->tw_x, true)));
@@ -810,12 +852,16 @@ int field_alloc(izh_field *f) {
   const size_t es = f->precision == IZH_FP64 ? sizeof(double) : sizeof(float);
   const int64_t g = f->g, kx = f->nx + 1;
   const int64_t ghalf = kx * (f->ny + 1) * (f->nz + 1);
+  // rows of whole 128-byte lines: the y and z passes read and write runs of
+  // neighbouring x lines, which then start on a line boundary
+  const int64_t row = 128 / (2 * (int64_t)es);
+  f->ldx = (kx + row - 1) / row * row;
   IZH_HIP(hipMalloc((void **)&f->rho_fx, g * sizeof(unsigned long long)));
   IZH_HIP(hipMalloc(&f->rho, g * es));
   IZH_HIP(hipMalloc(&f->phi, g * es));
   IZH_HIP(hipMalloc(&f->e, 4 * g * es));
-  IZH_HIP(hipMalloc(&f->spec_a, kx * f->ny * f->nz * 2 * es));
-  IZH_HIP(hipMalloc(&f->spec_b, kx * 2 * f->ny * f->nz * 2 * es));
+  IZH_HIP(hipMalloc(&f->spec_a, f->ldx * f->ny * f->nz * 2 * es));
+  IZH_HIP(hipMalloc(&f->spec_b, f->ldx * 2 * f->ny * f->nz * 2 * es));
   IZH_HIP(hipMalloc(&f->green, ghalf * es));
   IZH_HIP(hipMalloc(&f->green_t, ghalf * es));
   IZH_HIP(hipMalloc((void **)&f->corner,

@@ -174,8 +174,10 @@ struct izh_field {
   void *rho = nullptr;          // T[g]   charge density (T = double or float)
   void *phi = nullptr;          // T[g]   potential
   void *e = nullptr;            // T4[g]  (Ex, Ey, Ez, 0) interleaved
-  void *spec_a = nullptr;       // complex T[(nx+1)*ny*nz]
-  void *spec_b = nullptr;       // complex T[(nx+1)*2ny*nz]
+  int64_t ldx = 0;              // x pitch of the half spectra: nx+1 rounded up
+                                // to whole 128-byte rows
+  void *spec_a = nullptr;       // complex T[ldx*ny*nz]
+  void *spec_b = nullptr;       // complex T[ldx*2ny*nz]
   void *green = nullptr;        // T[(nx+1)*(ny+1)*(nz+1)] transformed Green fn
   void *green_t = nullptr;      // T[(nx+1)*(ny+1)*(nz+1)] work
   double *corner = nullptr;     // double[(nx+2)*(ny+2)*(nz+2)] IGF antiderivative

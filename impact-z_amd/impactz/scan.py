@@ -86,10 +86,12 @@ def footprint(n: int, grid, precision: int) -> int:
     pe = 8 if precision == hip.FP64 else 4
     g = nx * ny * nz
     kx = nx + 1
+    row = 128 // (2 * pe)           # complex values per 128-byte line
+    ldx = (kx + row - 1) // row * row  # x pitch of the half spectra
     tiles = max(1, (nx >> 3) * (ny >> 3) * (nz >> 3))
     bunch = n * (2 * 48 + 4 * 4 + 3 * pe + 3 * 8) + 2048 * 52 * 8 \
         + 2 * (n // 16384 + tiles + 1) * 12 + 3 * tiles * 4
-    field = g * (8 + 2 * pe + 4 * pe) + kx * ny * nz * 2 * pe + kx * 2 * ny * nz * 2 * pe \
+    field = g * (8 + 2 * pe + 4 * pe) + ldx * ny * nz * 2 * pe + ldx * 2 * ny * nz * 2 * pe \
         + 2 * kx * (ny + 1) * (nz + 1) * pe + (nx + 2) * (ny + 2) * (nz + 2) * 8
     return int(bunch + field)
 
