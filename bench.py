@@ -248,6 +248,40 @@ def cpu_baseline(grid, n_full: int, n1=20_000, n2=2_000_000, steps=(1, 4)):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def cpu_port(grid, n_full: int, n1=20_000, n2=5_000_000):
+    """The build's own C restatement of the step (oracle/impactz_oracle.c,
+    through oracle/pyoracle.py), timed beside the reference on the same
+    sample: one step of n1 and one of n2 particles of the workload's
+    distribution on the workload's grid, t(n) = a + n*b as for the
+    reference.  It is the correctness oracle, written for clarity, not
+    speed."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    try:
+        from oracle import pyoracle as O
+    except ImportError:
+        return None
+    pts = B.core_halo_bunch(n2, seed=1)
+    times = {}
+    for n in (n1, n2):
+        run = O.Run(np.ascontiguousarray(pts[:n]), B.Beam(), B.fodo(1), grid)
+        e = next(run.segments())
+        t0 = time.perf_counter()
+        run.step(e)
+        times[n] = time.perf_counter() - t0
+        log(f"cpu port: n={n} grid={grid} 1 step {times[n]:.2f} s")
+        del run
+    b = max((times[n2] - times[n1]) / (n2 - n1), 0.0)
+    a = max(times[n1] - n1 * b, 0.0)
+    return {"value": n_full / (a + n_full * b), "unit": "particle-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"the build's C restatement of the step (oracle/impactz_oracle.c, "
+                       f"gcc -O2, 1 core of {_cpu_model()}) on the {grid[0]}x{grid[1]}x{grid[2]} "
+                       f"grid: one step of {n1} and of {n2} particles; grid work a={a:.3f} "
+                       f"s/step, particle work b={b * 1e6:.3f} us/particle-step; "
+                       f"rate = {n_full}/(a + {n_full}*b)")}
+
+
 def _csv_rows(d, suffix):
     import csv
     import glob
@@ -481,6 +515,11 @@ def main():
     out["copy_probe_ms_1GiB"] = hip.probe_copy(1 << 30)
     if not args.no_cpu:
         cb = cpu_baseline(grid, n)
+        port = cpu_port(grid, n)
+        if cb is None:
+            cb = port
+        elif port is not None:
+            cb["port"] = port   # the build's own restatement, beside the reference
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
