@@ -222,6 +222,74 @@ __device__ inline uint32_t wave_rank_slot(const RankTicket &t) {
   return (uint32_t)__shfl((int)t.ret, t.leader, 64) + t.rank;
 }
 
+// Block-staged counting-sort scatter.  The kBlock slots a block takes per
+// group are grouped by key in an LDS table, so the cursor atomic is made
+// once per key per block instead of once per key per wavefront, and the
+// particles are staged in LDS in the order of their new slots: each
+// wavefront then stores consecutive staged particles, whose slots form runs
+// merged across the block's wavefronts, instead of the few short runs of its
+// own particles (short runs cost the vector memory path a write request per
+// partial line).  The table is double buffered by the slot loop's two
+// register sets, so filling one group's table needs no barrier against the
+// clearing of the previous group's.
+constexpr bool kStagedScatter = true;
+template <typename T, bool ON>
+struct ScatterStage {};
+template <typename T>
+struct ScatterStage<T, true> {
+  uint32_t key[2][kBlock];     // the group's distinct keys, hashed (kDeadKey: empty)
+  uint32_t cnt[2][kBlock];     // particles of each key
+  uint32_t base[kBlock];       // cursor value the block's run of a key starts at
+  uint32_t start[kBlock + 1];  // staged position of a key's run; [kBlock]: the total
+  double row[6][kBlock];
+  T tp[3][kBlock];
+  uint32_t orig[kBlock], nkey[kBlock], slot[kBlock];
+};
+
+struct StageTicket {
+  int h;         // table entry of the lane's key
+  uint32_t pos;  // position of the lane's particle in the block's run of that key
+};
+
+// Call with every lane of the wave active.  The lanes are grouped by key
+// with ballots; each group's first lane finds (or makes) its key's entry
+// with LDS compare-and-swap and adds the group's size to its count, whose
+// old value places the group within the block's run of that key.  The table
+// has kBlock entries, so a group's at most kBlock distinct keys always fit.
+template <typename S>
+__device__ inline StageTicket stage_insert(S &sg, int par, uint32_t key, bool take) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long pending = __ballot(take);
+  int leader = lane;
+  uint32_t rank = 0, count = 0;
+  while (pending) {
+    const int ld = __ffsll((long long)pending) - 1;
+    const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, ld);
+    const unsigned long long same = __ballot(take && key == lk) & pending;
+    if ((same >> lane) & 1ull) {
+      leader = ld;
+      rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    }
+    if (lane == ld) count = (uint32_t)__popcll(same);
+    pending &= ~same;
+  }
+  int h = 0;
+  uint32_t off = 0;
+  if (take && leader == lane) {
+    h = (int)(key & (kBlock - 1));
+    for (;;) {
+      const uint32_t old = atomicCAS(&sg.key[par][h], kDeadKey, key);
+      if (old == kDeadKey || old == key) break;
+      h = (h + 1) & (kBlock - 1);
+    }
+    off = atomicAdd(&sg.cnt[par][h], count);
+  }
+  StageTicket t;
+  t.h = __shfl(h, leader, 64);
+  t.pos = (uint32_t)__shfl((int)off, leader, 64) + rank;
+  return t;
+}
+
 // Histogram update: the lanes are grouped by key with ballots, then one
 // atomic instruction adds each group's size at its first lane.
 __device__ inline void wave_count_add(uint32_t *counters, uint32_t key, bool take) {
@@ -360,6 +428,15 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
   const E4<T> *E = (const E4<T> *)c.e;
   T *tpx = (T *)c.tpos;
   __shared__ GatherLds<T> gbox;
+  constexpr bool kStage = kScatter && kStagedScatter;
+  __shared__ ScatterStage<T, kStage> sg;
+  if constexpr (kStage) {
+    for (int par = 0; par < 2; par++) {
+      sg.key[par][threadIdx.x] = kDeadKey;
+      sg.cnt[par][threadIdx.x] = 0u;
+    }
+    __syncthreads();
+  }
   const int bdx = min(kGBoxT, c.nx), bdy = min(kGBoxT, c.ny), bdz = min(kGBoxL, c.nz);
   uint32_t box_key = kDeadKey;   // tile the staged box was built for
   int ox = 0, oy = 0, oz = 0;    // its first node
@@ -423,7 +500,7 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     sl.orig = og;
   };
 
-  auto step = [&](SlotIn &sl, uint32_t k0, int64_t base, auto &&prefetch) {
+  auto step = [&](SlotIn &sl, uint32_t k0, int64_t base, int par, auto &&prefetch) {
     double &x = sl.r[0], &px = sl.r[1], &y = sl.r[2], &py = sl.r[3], &ph = sl.r[4],
            &pt = sl.r[5];
     // a scatter pass takes the particles the previous pass keyed; a pass in
@@ -444,8 +521,44 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     // group's loads are issued after the atomic, so waiting for its result
     // does not wait for them
     RankTicket tk{0u, 0, 0u};
-    if constexpr (kScatter) tk = wave_rank_issue(c.cursor, kin, valid);
+    StageTicket stk{0, 0u};
+    if constexpr (kStage) stk = stage_insert(sg, par, kin, valid);
+    else if constexpr (kScatter) tk = wave_rank_issue(c.cursor, kin, valid);
     prefetch();
+    // staged scatter: once every wavefront's keys are in the table, thread t
+    // takes the cursor run of entry t (the result is first used after the
+    // group's arithmetic) and wavefront 0 places the entries' runs in the
+    // staged order (an exclusive scan of their counts in table order)
+    uint32_t run_base = 0u;
+    bool run_owner = false;
+    if constexpr (kStage) {
+      __syncthreads();
+      const int t = threadIdx.x;
+      const uint32_t ek = sg.key[par][t];
+      run_owner = ek != kDeadKey;
+      if (run_owner) run_base = atomicAdd(&c.cursor[ek], sg.cnt[par][t]);
+      if (t < 64) {
+        uint32_t a[4], loc = 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          a[u] = sg.cnt[par][4 * t + u];
+          loc += a[u];
+        }
+        uint32_t inc = loc;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const uint32_t o = (uint32_t)__shfl_up((int)inc, d, 64);
+          if (t >= d) inc += o;
+        }
+        uint32_t e = inc - loc;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          sg.start[4 * t + u] = e;
+          e += a[u];
+        }
+        if (t == 63) sg.start[kBlock] = inc;
+      }
+    }
     const double qm = c.need_orig ? c.qm0[sl.orig] : c.qm_value;
     if (has_kick) {
       // restage the gather box when the slot group starts another tile
@@ -586,6 +699,52 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
     }
     // histogram of the new keys for the next pass's scatter
     if constexpr (kKeys) wave_count_add(c.hist_new, knew, knew != kDeadKey);
+    if constexpr (kStage) {
+      // every key's cursor run is known; stage each particle at its place
+      if (run_owner) sg.base[threadIdx.x] = run_base;
+      __syncthreads();
+      if (valid) {
+        const uint32_t sp = sg.start[stk.h] + stk.pos;
+        sg.row[0][sp] = x;
+        sg.row[1][sp] = px;
+        sg.row[2][sp] = y;
+        sg.row[3][sp] = py;
+        sg.row[4][sp] = ph;
+        sg.row[5][sp] = pt;
+        if (has_conv) {
+          sg.tp[0][sp] = tx;
+          sg.tp[1][sp] = ty;
+          sg.tp[2][sp] = tz;
+        }
+        sg.orig[sp] = sl.orig;
+        sg.nkey[sp] = knew;
+        sg.slot[sp] = sg.base[stk.h] + stk.pos;
+      }
+      __syncthreads();
+      // thread t stores the t-th staged particle; the group's table is
+      // cleared for the group after next
+      const int t = threadIdx.x;
+      const bool have = (uint32_t)t < sg.start[kBlock];
+      const uint32_t q = have ? sg.slot[t] : 0u;
+      const bool ok = have && q < (uint32_t)cap;
+      nerr += (have && !ok) ? 1u : 0u;
+      const int lane = threadIdx.x & 63;
+      double *sd = (double *)c.sink + lane;
+      uint32_t *su = (uint32_t *)((double *)c.sink + 8 * 64) + lane;
+      T *stt = (T *)((double *)c.sink + 10 * 64) + lane;
+#pragma unroll
+      for (int k = 0; k < 6; k++) (ok ? c.out[k * cap + q] : sd[64 * k]) = sg.row[k][t];
+      (ok ? c.out_orig[q] : su[0]) = sg.orig[t];
+      if constexpr (kKeys) (ok ? c.out_key[q] : su[64]) = sg.nkey[t];
+      if (has_conv) {
+        (ok ? tpx[q] : stt[0]) = sg.tp[0][t];
+        (ok ? tpx[cap + q] : stt[64]) = sg.tp[1][t];
+        (ok ? tpx[2 * cap + q] : stt[128]) = sg.tp[2][t];
+      }
+      sg.key[par][t] = kDeadKey;
+      sg.cnt[par][t] = 0u;
+      return;
+    }
     uint32_t slot;
     if constexpr (kScatter) slot = wave_rank_slot(tk);
     else slot = (uint32_t)(base + threadIdx.x);
@@ -626,13 +785,13 @@ __global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
   SlotIn sa, sb;
   if (lo < hi) {
     load(lo, sa);
-    step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), lo,
+    step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), lo, 0,
          [&] { load(lo + kBlock, sb); });
     for (int64_t base = lo + kBlock; base < hi; base += 2 * kBlock) {
-      step(sb, (uint32_t)__builtin_amdgcn_readfirstlane((int)sb.k0), base,
+      step(sb, (uint32_t)__builtin_amdgcn_readfirstlane((int)sb.k0), base, 1,
            [&] { load(base + kBlock, sa); });
       if (base + kBlock >= hi) break;
-      step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), base + kBlock,
+      step(sa, (uint32_t)__builtin_amdgcn_readfirstlane((int)sa.k0), base + kBlock, 0,
            [&] { load(base + 2 * kBlock, sb); });
     }
   }
