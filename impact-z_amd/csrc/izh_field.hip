@@ -702,7 +702,7 @@ int ilog2(int v) {
 }
 
 template <typename T, int LOAD, int STORE, bool CONV, bool PAIR = false>
-int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
+int run_lines(hipStream_t stream, Lines p, const void *tw, bool inverse) {
   // the fused z pass takes twice the lines per block: its rows of the
   // spectrum and of G^ are read across the lines
   constexpr int pts = CONV ? kFftPointsZ : kFftPoints;
@@ -723,25 +723,26 @@ int run_lines(izh_field *f, Lines p, const void *tw, bool inverse) {
   const int64_t nblk = (nfft + p.B - 1) / p.B;
   const size_t tw_bytes = (size_t)(p.L / 2) * sizeof(cx<T>);
   hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts>), dim3((unsigned)nblk),
-                     dim3(kFftThreads), tw_bytes, f->stream, p, (const cx<T> *)tw,
+                     dim3(kFftThreads), tw_bytes, stream, p, (const cx<T> *)tw,
                      inverse ? 1 : 0);
   IZH_HIP(hipGetLastError());
   return IZH_OK;
 }
 
+// G^ of the field's current domain: IGF corners, 8-corner difference and
+// three even real transforms, on stream gs.
 template <typename T>
-int solve_t(izh_field *f) {
+int green_t(izh_field *f, hipStream_t gs) {
   const int nx = f->nx, ny = f->ny, nz = f->nz;
   const int mx = 2 * nx, my = 2 * ny, mz = 2 * nz, kx = nx + 1;
-  // ---- G^: IGF corners, 8-corner difference, 3 even real transforms
   {
-    ScopedTimer tm_(IZH_T_GREEN, f->stream);
+    ScopedTimer tm_(IZH_T_GREEN, gs);
     int nc = (nx + 2) * (ny + 2) * (nz + 2);
     hipLaunchKernelGGL(k_igf_corner, dim3(grid_for(nc)), dim3(kBlock), 0,
-                       f->stream, f->corner, nx + 2, ny + 2, nz + 2, f->d_dom);
+                       gs, f->corner, nx + 2, ny + 2, nz + 2, f->d_dom);
     int ng = (nx + 1) * (ny + 1) * (nz + 1);
     hipLaunchKernelGGL(k_igf_diff<T>, dim3(grid_for(ng)), dim3(kBlock), 0,
-                       f->stream, f->corner, (T *)f->green_t, nx, ny, nz);
+                       gs, f->corner, (T *)f->green_t, nx, ny, nz);
     IZH_HIP(hipGetLastError());
     Lines p{};
     // x: lines (j<=ny, k<=nz), contiguous; green_t -> green
@@ -750,14 +751,14 @@ int solve_t(izh_field *f) {
     p.sf = kx; p.ss = (int64_t)kx * (ny + 1); p.se = 1;
     p.df = kx; p.ds = (int64_t)kx * (ny + 1); p.de = 1;
     p.L = mx; p.Lin = p.L; p.Lout = nx + 1; p.out_scale = 1.0;
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_x, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(gs, p, f->tw_x, false)));
     // y: lines (i<=nx, k<=nz); green -> green_t
     p.src = f->green; p.dst = f->green_t;
     p.nf = kx; p.ns = nz + 1;
     p.sf = 1; p.ss = (int64_t)kx * (ny + 1); p.se = kx;
     p.df = 1; p.ds = (int64_t)kx * (ny + 1); p.de = kx;
     p.L = my; p.Lin = p.L; p.Lout = ny + 1;
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_y, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(gs, p, f->tw_y, false)));
     // z: lines (i<=nx, j<=ny); green_t -> green, scaled by 1/(mx my mz)
     p.src = f->green_t; p.dst = f->green;
     p.nf = kx; p.ns = ny + 1;
@@ -765,9 +766,19 @@ int solve_t(izh_field *f) {
     p.df = 1; p.ds = kx; p.de = (int64_t)kx * (ny + 1);
     p.L = mz; p.Lin = p.L; p.Lout = nz + 1;
     p.out_scale = 1.0 / ((double)mx * my * mz);
-    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(f, p, f->tw_z, false)));
+    IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(gs, p, f->tw_z, false)));
   }
-  ScopedTimer tm_(IZH_T_FFT, f->stream);
+  return IZH_OK;
+}
+
+// The convolution with the G^ in place: forward x, forward y, the fused z
+// pass, inverse y, inverse x, on the field's stream.
+template <typename T>
+int convolve_t(izh_field *f) {
+  const int nx = f->nx, ny = f->ny, nz = f->nz;
+  const int mx = 2 * nx, my = 2 * ny, mz = 2 * nz, kx = nx + 1;
+  const hipStream_t fs = f->stream;
+  ScopedTimer tm_(IZH_T_FFT, fs);
   // The half spectra hold x rows of ld = f->ldx >= kx values.  The y and z
   // passes run over all ld x lines, so a block's run of neighbouring lines
   // starts on a 128-byte boundary; lines kx..ld-1 are padding.
@@ -779,14 +790,14 @@ int solve_t(izh_field *f) {
   p.sf = nx; p.ss = (int64_t)nx * ny; p.se = 1;
   p.df = ld; p.ds = ld * ny; p.de = 1;
   p.L = mx; p.Lin = nx; p.Lout = kx; p.out_scale = 1.0;
-  IZH_CHECK((run_lines<T, kLoadReal, kStoreCplx, false, true>(f, p, f->tw_x, false)));
+  IZH_CHECK((run_lines<T, kLoadReal, kStoreCplx, false, true>(fs, p, f->tw_x, false)));
   // ---- forward y: A -> B (kx of ld, my, nz)
   p.src = f->spec_a; p.dst = f->spec_b;
   p.nf = ld; p.nfv = kx; p.ns = nz;
   p.sf = 1; p.ss = ld * ny; p.se = ld;
   p.df = 1; p.ds = ld * my; p.de = ld;
   p.L = my; p.Lin = ny; p.Lout = my;
-  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(f, p, f->tw_y, false)));
+  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(fs, p, f->tw_y, false)));
   // ---- z: FFT, * G^, IFFT, keep k < nz; in place in B
   p.src = f->spec_b; p.dst = f->spec_b;
   p.nf = ld; p.nfv = kx; p.ns = my;
@@ -794,7 +805,7 @@ int solve_t(izh_field *f) {
   p.df = 1; p.ds = ld; p.de = ld * my;
   p.L = mz; p.Lin = nz; p.Lout = nz;
   p.green = f->green; p.gkx = kx; p.gky = (int64_t)kx * (ny + 1); p.my = my;
-  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, true>(f, p, f->tw_z, false)));
+  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, true>(fs, p, f->tw_z, false)));
   // ---- inverse y: B -> A, keep j < ny
   p.green = nullptr;
   p.src = f->spec_b; p.dst = f->spec_a;
@@ -802,14 +813,14 @@ int solve_t(izh_field *f) {
   p.sf = 1; p.ss = ld * my; p.se = ld;
   p.df = 1; p.ds = ld * ny; p.de = ld;
   p.L = my; p.Lin = my; p.Lout = ny;
-  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(f, p, f->tw_y, true)));
+  IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, false>(fs, p, f->tw_y, true)));
   // ---- inverse x (C2R): A -> phi, keep i < nx
   p.src = f->spec_a; p.dst = f->phi;
   p.nf = ny; p.nfv = ny; p.ns = nz;
   p.sf = ld; p.ss = ld * ny; p.se = 1;
   p.df = nx; p.ds = (int64_t)nx * ny; p.de = 1;
   p.L = mx; p.Lin = mx; p.Lout = nx;
-  IZH_CHECK((run_lines<T, kLoadHerm, kStoreReal, false, true>(f, p, f->tw_x, true)));
+  IZH_CHECK((run_lines<T, kLoadHerm, kStoreReal, false, true>(fs, p, f->tw_x, true)));
   return IZH_OK;
 }
 
@@ -950,10 +961,19 @@ int launch_deposit_granular(izh_bunch *b, izh_field *f) {
   return launch_deposit(b, f, nullptr, nullptr, nullptr, nch);
 }
 
-int launch_solve(izh_field *f) {
-  int rc = f->precision == IZH_FP64 ? solve_t<double>(f) : solve_t<float>(f);
+int launch_green(izh_field *f, hipStream_t gs) {
+  return f->precision == IZH_FP64 ? green_t<double>(f, gs) : green_t<float>(f, gs);
+}
+
+int launch_convolve(izh_field *f) {
+  int rc = f->precision == IZH_FP64 ? convolve_t<double>(f) : convolve_t<float>(f);
   if (rc == IZH_OK) f->have_phi = 1;
   return rc;
+}
+
+int launch_solve(izh_field *f) {
+  const int rc = launch_green(f, f->stream);
+  return rc == IZH_OK ? launch_convolve(f) : rc;
 }
 
 int launch_efield(izh_field *f) {

@@ -1227,6 +1227,10 @@ struct Runner {
   izh_bunch *b = nullptr;
   izh_field *f = nullptr;
   hipStream_t st = nullptr, fstream = nullptr;
+  // G^ of a step needs only the step's domain: it runs on the field's own
+  // stream (idle during the run) beside the deposit, forked after the pass
+  // and joined before the convolution
+  hipEvent_t ev_dom = nullptr, ev_green = nullptr;
   int nx = 0, ny = 0, nz = 0, s0 = 0, s1 = 0, s2 = 0, tnx = 0, tny = 0, ntiles = 0;
   int nb = 0, period = 1;
   int g_cur = 1, g_next = 0;
@@ -1265,6 +1269,8 @@ struct Runner {
     fstream = f->stream;
     f->stream = st;
     began = true;
+    IZH_HIP(hipEventCreateWithFlags(&ev_dom, hipEventDisableTiming));
+    IZH_HIP(hipEventCreateWithFlags(&ev_green, hipEventDisableTiming));
     const double gam = -b->refpt[5];
     gam_sc = gam;
     const double wsum = b->wabs_sum > 0 ? b->wabs_sum : 1.0;
@@ -1437,9 +1443,14 @@ struct Runner {
     if (!dry) {
       // charge, update3O and the E field of step s on the grouping the slots are in
       const Chunk *dch = b->chunks + g_cur * b->nchunks_cap;
+      IZH_HIP(hipEventRecord(ev_dom, st));
+      IZH_HIP(hipStreamWaitEvent(fstream, ev_dom, 0));
+      IZH_CHECK(launch_green(f, fstream));
+      IZH_HIP(hipEventRecord(ev_green, fstream));
       IZH_CHECK(launch_deposit(b, f, dch, &f->d_dom->kgrid[g_cur],
                                &b->d_scal->nchunks[g_cur], b->nchunks_cap));
-      IZH_CHECK(launch_solve(f));
+      IZH_HIP(hipStreamWaitEvent(st, ev_green, 0));
+      IZH_CHECK(launch_convolve(f));
       IZH_CHECK(launch_efield(f));
       // the step's wakefield (src/Contrl/AccSimulator.f90:1272-1338) from the
       // same t positions, into E before the next pass kicks
@@ -1463,6 +1474,9 @@ struct Runner {
   void release() {
     if (began) f->stream = fstream;
     began = false;
+    if (ev_dom) (void)hipEventDestroy(ev_dom);
+    if (ev_green) (void)hipEventDestroy(ev_green);
+    ev_dom = ev_green = nullptr;
   }
 
   // After the last pass: wait for the stream and take the results.

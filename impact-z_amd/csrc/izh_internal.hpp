@@ -473,6 +473,10 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGri
                    const int64_t *d_nchunks, int64_t max_chunks);
 int launch_deposit_granular(izh_bunch *b, izh_field *f);
 int launch_solve(izh_field *f);
+// The two halves of launch_solve: G^ of the current domain on stream gs,
+// then the convolution on the field's stream (after G^ is complete).
+int launch_green(izh_field *f, hipStream_t gs);
+int launch_convolve(izh_field *f);
 int launch_efield(izh_field *f);
 int launch_gather_kick(izh_bunch *b, izh_field *f, double tau, double xycon,
                        double tcon);
