@@ -408,8 +408,18 @@ __device__ inline void gather_batched(Load ld, int64_t o, int64_t sy, int64_t sz
 // a set of PassMode bits.  The slot loop is unrolled by two over two
 // register sets, so the loads of the next slot group land in the set not
 // being worked on.
+// Two blocks (wavefronts) per SIMD; three for an in-place pass with an FP32
+// grid and compile-time push kinds, which then spills at most 4 VGPRs and
+// gains from the extra wavefront (measured: 3%).  FP64 would spill 70 VGPRs
+// to scratch at three (32% slower); a scatter pass's LDS (70 KB) allows two.
+template <typename T, int MODE, int KB>
+constexpr int pass_blocks_per_cu(int ka) {
+  return ((MODE & kPassScatter) == 0 && sizeof(T) == 4 && KB != kPushAnyKind &&
+          ka != kPushAnyKind) ? 3 : 2;
+}
+
 template <typename T, int MODE, int KB, int KA>
-__global__ void __launch_bounds__(kBlock, 2) k_pass(PassCfg c) {
+__global__ void __launch_bounds__(kBlock, (pass_blocks_per_cu<T, MODE, KB>(KA))) k_pass(PassCfg c) {
   constexpr bool kScatter = (MODE & kPassScatter) != 0, kKeys = (MODE & kPassKeys) != 0;
   const bool has_kick = KB == kPushAnyKind ? c.kb != kPushNone : KB != kPushNone;
   const bool has_conv = KA == kPushAnyKind ? c.ka != kPushNone : KA != kPushNone;
