@@ -125,7 +125,7 @@ __device__ inline void fft_run(cx<T> *buf, int log2L, int LP, int lines, const c
 }
 
 template <typename T, bool DIT>
-__device__ void fft_stages(cx<T> *buf, int LP, int log2L, int lines, const cx<T> *tw,
+__device__ __attribute__((always_inline)) inline void fft_stages(cx<T> *buf, int LP, int log2L, int lines, const cx<T> *tw,
                            bool inverse, int mask) {
   int done = 0;
   while (done < log2L) {
@@ -150,14 +150,14 @@ __device__ inline void line_fs(int line, const Lines &p, int &f, int &s) {
 // Thread t of a block -> (line in block, element) with coalesced addresses:
 // unit element stride -> consecutive threads walk one line; otherwise they
 // walk across neighbouring lines (unit line stride).
-__device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &lb,
+__device__ inline void map_thread(int t, int64_t estride, int log2L, int log2B, int &lb,
                                   int &m) {
   if (estride == 1) {
-    lb = t >> p.log2L;
-    m = t & (p.L - 1);
+    lb = t >> log2L;
+    m = t & ((1 << log2L) - 1);
   } else {
-    m = t >> p.log2B;
-    lb = t & (p.B - 1);
+    m = t >> log2B;
+    lb = t & ((1 << log2B) - 1);
   }
 }
 
@@ -170,12 +170,20 @@ __device__ inline void map_thread(int t, int64_t estride, const Lines &p, int &l
 // PTS points of B = PTS/L lines per block; each line is held in LDS at a
 // pitch of L+1 points, so lanes that walk across lines (a line stride
 // other than 1, or the fused z pass's multiply) do not all hit one bank.
-template <typename T, int LOAD, int STORE, bool CONV, bool PAIR, int PTS>
+// LL: log2 of the line length when it is fixed at compile time (the common
+// lengths, so every index split and LDS position folds), 0 to take it from p.
+template <typename T, int LOAD, int STORE, bool CONV, bool PAIR, int PTS, int LL>
 __global__ void __launch_bounds__(kFftThreads)
 k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   __shared__ cx<T> buf[PTS + PTS / kFftMinLine];
-  const int LP = p.L + 1;
-  const int mask = p.L >= 64 ? 15 : 0;
+  static_assert(PTS == 2048 || PTS == 4096, "points per block");
+  constexpr int kLog2Pts = PTS == 4096 ? 12 : 11;
+  const int log2L = LL ? LL : p.log2L;
+  const int L = 1 << log2L;
+  const int log2B = kLog2Pts - log2L;
+  const int B = 1 << log2B;
+  const int LP = L + 1;
+  const int mask = L >= 64 ? 15 : 0;
   extern __shared__ __align__(16) unsigned char fft_dyn[];  // L/2 twiddles
   cx<T> *tw = reinterpret_cast<cx<T> *>(fft_dyn);
   // twiddles are written to LDS together with the line data below
@@ -185,7 +193,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   for (int it = 0; it < kTwPer; it++) {
     const int t = threadIdx.x + it * kFftThreads;
     T re = (T)0, im = (T)0;
-    if (t < (p.L >> 1)) {
+    if (t < (L >> 1)) {
       const cx<T> v = twg[t];
       re = v.re; im = v.im;
     }
@@ -193,7 +201,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   }
   const int nf = (int)p.nf;
   const int nlines = nf * (int)p.ns;
-  const int line0 = blockIdx.x * p.B;
+  const int line0 = blockIdx.x * B;
   // ---- load (zero padding / even extension / Hermitian completion): all of
   // a thread's loads are issued before the first LDS write
   constexpr int kPer = PTS / kFftThreads;  // B * L == PTS
@@ -203,7 +211,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   for (int it = 0; it < kPer; it++) {
     const int t = threadIdx.x + it * kFftThreads;
     int lb, m;
-    map_thread(t, p.se, p, lb, m);
+    map_thread(t, p.se, log2L, log2B, lb, m);
     const int line = line0 + lb;
     slot[it] = lb * LP + lds_pos(m, mask);
     T re = (T)0, im = (T)0;
@@ -219,7 +227,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
         const bool has1 = l1 < nlines;
         const int64_t b1 = has1 ? f1 * p.sf + s1 * p.ss : b0;
         if (LOAD == kLoadEven) {
-          const int mm = m <= (p.L >> 1) ? m : p.L - m;
+          const int mm = m <= (L >> 1) ? m : L - m;
           re = src[b0 + mm * p.se];
           im = has1 ? src[b1 + mm * p.se] : (T)0;
         } else if (LOAD == kLoadReal) {
@@ -228,8 +236,8 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
             im = has1 ? src[b1 + m * p.se] : (T)0;
           }
         } else {  // kLoadHerm: Z = X + iY of the completed spectra
-          const bool lo = m <= (p.L >> 1);
-          const int mm = lo ? m : p.L - m;
+          const bool lo = m <= (L >> 1);
+          const int mm = lo ? m : L - m;
           cx<T> x = csrc[b0 + mm * p.se];
           cx<T> y = has1 ? csrc[b1 + mm * p.se] : cx<T>{(T)0, (T)0};
           if (!lo) { x.im = -x.im; y.im = -y.im; }
@@ -251,11 +259,11 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
           re = v.re; im = v.im;
         }
       } else if (LOAD == kLoadEven) {
-        const int mm = m <= (p.L >> 1) ? m : p.L - m;
+        const int mm = m <= (L >> 1) ? m : L - m;
         re = ((const T *)p.src)[base + mm * p.se];
       } else {  // kLoadHerm: first L/2+1 values stored
-        const bool lo = m <= (p.L >> 1);
-        const cx<T> v = ((const cx<T> *)p.src)[base + (lo ? m : p.L - m) * p.se];
+        const bool lo = m <= (L >> 1);
+        const cx<T> v = ((const cx<T> *)p.src)[base + (lo ? m : L - m) * p.se];
         re = v.re; im = lo ? v.im : -v.im;
       }
     }
@@ -266,7 +274,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
 #pragma unroll
   for (int it = 0; it < kTwPer; it++) {
     const int t = threadIdx.x + it * kFftThreads;
-    if (t < (p.L >> 1)) tw[t] = twv[it];
+    if (t < (L >> 1)) tw[t] = twv[it];
   }
   __syncthreads();
   // fused z pass: fetch the line's G^ values now, so their latency hides
@@ -280,37 +288,37 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t >> p.log2B, lb = t & (p.B - 1);
+      const int kz = t >> log2B, lb = t & (B - 1);
       const int line = line0 + lb;
       int i, j2;
       line_fs(line, p, i, j2);
       T g = (T)0;
       if (line < nlines && i < p.nfv) {
         const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
-        const int kf = kz <= (p.L >> 1) ? kz : p.L - kz;
+        const int kf = kz <= (L >> 1) ? kz : L - kz;
         g = G[i + p.gkx * jf + p.gky * kf];
       }
       gv[it] = g;
     }
   }
-  fft_stages<T, false>(buf, LP, p.log2L, p.B, tw, inverse != 0, mask);
+  fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask);
   if (CONV) {
 #pragma unroll
     for (int it = 0; it < kPer; it++) {
       const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t >> p.log2B, lb = t & (p.B - 1);
-      const int q = lb * LP + lds_pos(bitrev(kz, p.log2L), mask);
+      const int kz = t >> log2B, lb = t & (B - 1);
+      const int q = lb * LP + lds_pos(bitrev(kz, log2L), mask);
       const cx<T> v = buf[q];
       buf[q] = {v.re * gv[it], v.im * gv[it]};
     }
     __syncthreads();
-    fft_stages<T, true>(buf, LP, p.log2L, p.B, tw, true, mask);
+    fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask);
   }
   // ---- store the kept part of each line
-  const int totout = p.B * p.L;
+  const int totout = B * L;
   for (int t = threadIdx.x; t < totout; t += kFftThreads) {
     int lb, m;
-    map_thread(t, p.de, p, lb, m);
+    map_thread(t, p.de, log2L, log2B, lb, m);
     if (m >= p.Lout) continue;
     if (PAIR) {
       const int l0 = 2 * (line0 + lb), l1 = l0 + 1;
@@ -321,13 +329,13 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       line_fs(l1, p, f1, s1);
       const int64_t o0 = f0 * p.df + s0 * p.ds + m * p.de;
       const int64_t o1 = has1 ? f1 * p.df + s1 * p.ds + m * p.de : o0;
-      const cx<T> z = buf[lb * LP + lds_pos(bitrev(m, p.log2L), mask)];
+      const cx<T> z = buf[lb * LP + lds_pos(bitrev(m, log2L), mask)];
       if (STORE == kStoreReal) {
         ((T *)p.dst)[o0] = (T)(z.re * p.out_scale);
         if (has1) ((T *)p.dst)[o1] = (T)(z.im * p.out_scale);
       } else {
         // X(m) = (Z(m) + conj Z(L-m)) / 2,  Y(m) = (Z(m) - conj Z(L-m)) / 2i
-        const cx<T> w = buf[lb * LP + lds_pos(bitrev((p.L - m) & (p.L - 1), p.log2L), mask)];
+        const cx<T> w = buf[lb * LP + lds_pos(bitrev((L - m) & (L - 1), log2L), mask)];
         const T h = (T)(0.5 * p.out_scale);
         ((cx<T> *)p.dst)[o0] = {(z.re + w.re) * h, (z.im - w.im) * h};
         if (has1) ((cx<T> *)p.dst)[o1] = {(z.im + w.im) * h, (w.re - z.re) * h};
@@ -340,7 +348,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     line_fs(line, p, f, s);
     if (f >= p.nfv) continue;
     int64_t o = f * p.df + s * p.ds + m * p.de;
-    cx<T> v = buf[lb * LP + lds_pos(CONV ? m : bitrev(m, p.log2L), mask)];
+    cx<T> v = buf[lb * LP + lds_pos(CONV ? m : bitrev(m, log2L), mask)];
     if (STORE == kStoreCplx) {
       ((cx<T> *)p.dst)[o] = {(T)(v.re * p.out_scale), (T)(v.im * p.out_scale)};
     } else {
@@ -722,9 +730,28 @@ int run_lines(hipStream_t stream, Lines p, const void *tw, bool inverse) {
   const int64_t nfft = PAIR ? (nlines + 1) / 2 : nlines;
   const int64_t nblk = (nfft + p.B - 1) / p.B;
   const size_t tw_bytes = (size_t)(p.L / 2) * sizeof(cx<T>);
-  hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts>), dim3((unsigned)nblk),
-                     dim3(kFftThreads), tw_bytes, stream, p, (const cx<T> *)tw,
-                     inverse ? 1 : 0);
+  const dim3 g((unsigned)nblk), b(kFftThreads);
+  const cx<T> *twc = (const cx<T> *)tw;
+  const int inv = inverse ? 1 : 0;
+  // the line lengths of the 64^3 .. 128x128x256 grids as compile-time sizes
+  switch (p.log2L) {
+    case 7:
+      hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts, 7>), g, b, tw_bytes,
+                         stream, p, twc, inv);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts, 8>), g, b, tw_bytes,
+                         stream, p, twc, inv);
+      break;
+    case 9:
+      hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts, 9>), g, b, tw_bytes,
+                         stream, p, twc, inv);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fft_lines<T, LOAD, STORE, CONV, PAIR, pts, 0>), g, b, tw_bytes,
+                         stream, p, twc, inv);
+      break;
+  }
   IZH_HIP(hipGetLastError());
   return IZH_OK;
 }
