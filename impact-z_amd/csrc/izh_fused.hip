@@ -1237,9 +1237,13 @@ struct Runner {
   izh_bunch *b = nullptr;
   izh_field *f = nullptr;
   hipStream_t st = nullptr, fstream = nullptr;
-  // G^ of a step needs only the step's domain: it runs on the field's own
-  // stream (idle during the run) beside the deposit, forked after the pass
-  // and joined before the convolution
+  // G^ of a step needs only the step's domain: with fork_green it runs on
+  // the field's own stream (idle during the run) beside the deposit, forked
+  // after the pass and joined before the convolution.  Lattice-scan jobs
+  // keep to one stream: with many jobs in flight the device is already
+  // full, and a second stream per job only adds contention for the
+  // hardware queues (served scan p50 13.5 -> 17.9 s with it).
+  bool fork_green = true;
   hipEvent_t ev_dom = nullptr, ev_green = nullptr;
   int nx = 0, ny = 0, nz = 0, s0 = 0, s1 = 0, s2 = 0, tnx = 0, tny = 0, ntiles = 0;
   int nb = 0, period = 1;
@@ -1453,13 +1457,17 @@ struct Runner {
     if (!dry) {
       // charge, update3O and the E field of step s on the grouping the slots are in
       const Chunk *dch = b->chunks + g_cur * b->nchunks_cap;
-      IZH_HIP(hipEventRecord(ev_dom, st));
-      IZH_HIP(hipStreamWaitEvent(fstream, ev_dom, 0));
-      IZH_CHECK(launch_green(f, fstream));
-      IZH_HIP(hipEventRecord(ev_green, fstream));
+      if (fork_green) {
+        IZH_HIP(hipEventRecord(ev_dom, st));
+        IZH_HIP(hipStreamWaitEvent(fstream, ev_dom, 0));
+        IZH_CHECK(launch_green(f, fstream));
+        IZH_HIP(hipEventRecord(ev_green, fstream));
+      } else {
+        IZH_CHECK(launch_green(f, st));
+      }
       IZH_CHECK(launch_deposit(b, f, dch, &f->d_dom->kgrid[g_cur],
                                &b->d_scal->nchunks[g_cur], b->nchunks_cap));
-      IZH_HIP(hipStreamWaitEvent(st, ev_green, 0));
+      if (fork_green) IZH_HIP(hipStreamWaitEvent(st, ev_green, 0));
       IZH_CHECK(launch_convolve(f));
       IZH_CHECK(launch_efield(f));
       // the step's wakefield (src/Contrl/AccSimulator.f90:1272-1338) from the
@@ -1677,6 +1685,7 @@ int izh_job_create(izh_bunch *b, izh_field *f, izh_compdom *dom, double z,
   j->nsteps = nsteps;
   j->z = z;
   j->use_graph = use_graph != 0;
+  j->r.fork_green = false;   // one stream per job (see Runner)
   int rc = j->r.begin(b, f, nsteps);
   j->block = job_block(nperiod, j->r.period);
   if (rc == IZH_OK) rc = j->r.pass(job_seg(j, 0), &j->z);   // pass 0 starts step 0
