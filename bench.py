@@ -435,7 +435,9 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed steps, run as one izh_run call (N steps make N+1 "
+                         "passes: the first only pushes, the last only kicks)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=50_000_000)
     ap.add_argument("--grid", default="128,128,256")
