@@ -180,6 +180,7 @@ static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
   IZH_CHECK(dmalloc(&b->id0, n));
   std::vector<double> row((size_t)(n > 0 ? n : 1));
   b->wabs_sum = 0.0;
+  b->wabs_max = 0.0;
   b->w_uniform = 1;
   b->w_value = n > 0 ? pts9[7] : 0.0;
   b->qm_uniform = 1;
@@ -194,6 +195,7 @@ static int upload_pts(izh_bunch *b, const double *pts9, int64_t n) {
     if (r == 7)
       for (int64_t i = 0; i < n; i++) {
         b->wabs_sum += std::fabs(row[i]);
+        b->wabs_max = std::max(b->wabs_max, std::fabs(row[i]));
         if (row[i] != b->w_value) b->w_uniform = 0;
       }
     if (n > 0)
@@ -368,6 +370,7 @@ int izh_bunch_create_from(const izh_bunch *src, int64_t offset, int64_t n, doubl
   b->w_uniform = 1;
   b->w_value = w;
   b->wabs_sum = std::fabs(w) * (double)n;
+  b->wabs_max = std::fabs(w);
   b->qm_uniform = src->qm_uniform;
   b->qm_value = src->qm_value;
   // the source's stream must have finished with its rows before the copy

@@ -434,19 +434,36 @@ __device__ inline Cic<P> cic_of(P x, P y, P z, P xmin, P ymin, P zmin, P hx, P h
   return c;
 }
 
+// x rounded to the nearest integer (ties to even), as __double2ll_rn.
+// FAST: valid for |x| < 2^50 (the launch checks that every contribution is
+// below it): adding 1.5 * 2^52 leaves round(x) in the low mantissa bits, one
+// add and one integer subtract instead of the split conversion.
+template <bool FAST>
+__device__ inline long long fx_round(double x) {
+  if constexpr (FAST) {
+    const double t = __dadd_rn(x, 6755399441055744.0);
+    return (long long)(__double_as_longlong(t) - 0x4338000000000000LL);
+  } else {
+    return __double2ll_rn(x);
+  }
+}
+
 // Fixed-point contributions of one particle, in the reference's node order.
-template <typename P>
+template <typename P, bool FAST = false>
 __device__ inline void cic_contrib(const Cic<P> &c, double wq, long long v[8]) {
+  // each product is rounded to FP64 before it is rounded to an integer: no
+  // multiply may fuse with the short form's add
+#pragma clang fp contract(off)
   const P one = (P)1;
   const P a = c.ab, b = c.cd, e = c.ef;
-  v[0] = __double2ll_rn((double)(a * b * e) * wq);
-  v[1] = __double2ll_rn((double)(a * (one - b) * e) * wq);
-  v[2] = __double2ll_rn((double)(a * (one - b) * (one - e)) * wq);
-  v[3] = __double2ll_rn((double)(a * b * (one - e)) * wq);
-  v[4] = __double2ll_rn((double)((one - a) * b * (one - e)) * wq);
-  v[5] = __double2ll_rn((double)((one - a) * (one - b) * (one - e)) * wq);
-  v[6] = __double2ll_rn((double)((one - a) * (one - b) * e) * wq);
-  v[7] = __double2ll_rn((double)((one - a) * b * e) * wq);
+  v[0] = fx_round<FAST>((double)(a * b * e) * wq);
+  v[1] = fx_round<FAST>((double)(a * (one - b) * e) * wq);
+  v[2] = fx_round<FAST>((double)(a * (one - b) * (one - e)) * wq);
+  v[3] = fx_round<FAST>((double)(a * b * (one - e)) * wq);
+  v[4] = fx_round<FAST>((double)((one - a) * b * (one - e)) * wq);
+  v[5] = fx_round<FAST>((double)((one - a) * (one - b) * (one - e)) * wq);
+  v[6] = fx_round<FAST>((double)((one - a) * (one - b) * e) * wq);
+  v[7] = fx_round<FAST>((double)((one - a) * b * e) * wq);
 }
 
 // Node offsets (di, dj, dk) in the order of cic_contrib.
@@ -574,7 +591,7 @@ constexpr int kTileWinEdgeT = (1 << kTileShift) + 1 + 2 * kTileMarginT;
 constexpr int kTileWinEdgeL = (1 << kTileShift) + 1 + 2 * kTileMarginL;
 constexpr int kTileWinCells = kTileWinEdgeT * kTileWinEdgeT * kTileWinEdgeL;
 
-template <typename P>
+template <typename P, bool FAST>
 __global__ void __launch_bounds__(kBlock)
 k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
                 const P *__restrict__ Z, const uint32_t *__restrict__ orig,
@@ -631,7 +648,7 @@ k_deposit_tiles(const P *__restrict__ X, const P *__restrict__ Y,
           c.k + 1 >= nz)
         continue;
       long long v[8];
-      cic_contrib<P>(c, pw[u] * scale, v);
+      cic_contrib<P, FAST>(c, pw[u] * scale, v);
       const int li = c.i - bx, lj = c.j - by, lk = c.k - bz;
       if (li >= 0 && li + 1 < wx && lj >= 0 && lj + 1 < wy && lk >= 0 && lk + 1 < wz) {
         const int l0 = li + wx * (lj + wy * lk);
@@ -933,6 +950,9 @@ int field_set_domain(izh_field *f, const izh_compdom *dom) {
 int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGrid *kgrid,
                    const int64_t *d_nchunks, int64_t max_chunks) {
   f->fixed_scale = fixed_point_scale(b->wabs_sum);
+  // every contribution is at most |w| * scale: below 2^50 the fused
+  // deposit rounds with the short form
+  const bool fast = b->wabs_max * f->fixed_scale < 0x1p50;
   hipLaunchKernelGGL(k_zero_u64, dim3(grid_for(f->g)), dim3(kBlock), 0, f->stream,
                      f->rho_fx, f->g);
   if (max_chunks > 0) {
@@ -943,20 +963,20 @@ int launch_deposit(izh_bunch *b, izh_field *f, const Chunk *chunks, const KeyGri
       const int s0 = tile_shift(f->nx), s1 = tile_shift(f->ny), s2 = tile_shift(f->nz);
       const int tnx = f->nx >> s0, tny = f->ny >> s1;
       int64_t *spill = b->d_scal ? &b->d_scal->overflow : nullptr;
+      const auto go = [&](auto kern, const auto *tp) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)max_chunks), dim3(kBlock), 0, f->stream, tp,
+                           tp + b->cap, tp + 2 * b->cap, b->orig, b->w0, b->w_value,
+                           b->w_uniform, chunks, d_nchunks, kgrid, f->d_dom, f->nx, f->ny,
+                           f->nz, s0, s1, s2, tnx, tny, f->fixed_scale, f->rho_fx, spill);
+      };
       if (f->precision == IZH_MIXED) {
         const float *tp = (const float *)b->tpos;
-        hipLaunchKernelGGL(k_deposit_tiles<float>, dim3((unsigned)max_chunks),
-                           dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
-                           b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
-                           kgrid, f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
-                           f->fixed_scale, f->rho_fx, spill);
+        if (fast) go(k_deposit_tiles<float, true>, tp);
+        else go(k_deposit_tiles<float, false>, tp);
       } else {
         const double *tp = (const double *)b->tpos;
-        hipLaunchKernelGGL(k_deposit_tiles<double>, dim3((unsigned)max_chunks),
-                           dim3(kBlock), 0, f->stream, tp, tp + b->cap, tp + 2 * b->cap,
-                           b->orig, b->w0, b->w_value, b->w_uniform, chunks, d_nchunks,
-                           kgrid, f->d_dom, f->nx, f->ny, f->nz, s0, s1, s2, tnx, tny,
-                           f->fixed_scale, f->rho_fx, spill);
+        if (fast) go(k_deposit_tiles<double, true>, tp);
+        else go(k_deposit_tiles<double, false>, tp);
       }
     } else {
       // one call's bunch in the t frame, in its own (unsorted) order

@@ -126,6 +126,7 @@ struct izh_bunch {
   izh_const c{};
   int frame = izh::kFrameZ;
   double wabs_sum = 0;   // sum |Pts1(8,:)|, bounds every cell of the grid charge
+  double wabs_max = 0;   // max |Pts1(8,:)|, bounds every particle's contribution
   // moving rows: x, px, y, py, phi, pt and the original index
   double *r[6] = {nullptr};
   double *r_alt[6] = {nullptr};
