@@ -92,6 +92,27 @@ def lookup(n: int, grid, precision: int) -> int:
     return int(ent[k]["resort_period"])
 
 
+def step_ms(n: int, grid, precision: int):
+    """Measured milliseconds per step at the tuned re-sort period, taken
+    from the table's two particle-count decades around n on this grid and
+    form (linear in n between them, proportional to n beyond them); None
+    without an entry."""
+    ent = load().get(_key(precision, tuple(grid)))
+    if not ent:
+        return None
+    # the table's keys are the rounded decades of the measured sizes
+    size_of = {str(int(round(math.log10(sz)))): sz for sz in SIZES}
+    pts = sorted((float(size_of.get(k, 10.0 ** int(k))), min(v["ms_per_step"].values()))
+                 for k, v in ent.items())
+    if n <= pts[0][0]:
+        return pts[0][1]
+    for (n0, t0), (n1, t1) in zip(pts, pts[1:]):
+        if n <= n1:
+            return t0 + (t1 - t0) * (n - n0) / (n1 - n0)
+    n1, t1 = pts[-1]
+    return t1 * n / n1
+
+
 def tuner_for_requests(req) -> int:
     """The tuner hook of impactz.scan: a request's re-sort period."""
     return lookup(req.n, req.grid, req.precision)
