@@ -422,7 +422,7 @@ def scans(args) -> dict:
                 f"Poisson arrivals at 20/s for {args.serve_seconds:g} s, seed 11; particles "
                 "log-uniform in [1e5, 1e7], grids 32^3..128x128x256, 50-400 steps, gradients "
                 "U[5,15] T/m; admission by HBM footprint, continuous batching; shortest "
-                "estimated job first with ageing 0.2 per second waited, at most 16 jobs "
+                "estimated job first with ageing 0.2 per second waited, at most 3 jobs "
                 "running")
     finally:
         pool.close()

@@ -190,7 +190,7 @@ def est_seconds(r: Request) -> float:
 
 def serve(reqs: list[Request], pool: Pool, budget_bytes: int, tuner=None,
           use_graph: bool = True, block: int | None = None, keep_records: bool = False,
-          log=None, policy: str = "sjf", max_live: int | None = 16,
+          log=None, policy: str = "sjf", max_live: int | None = 3,
           aging: float = 0.2) -> dict:
     """Serve requests as they arrive (their `arrival` times, on the wall
     clock from the start): a request waits in the queue until its HBM
@@ -202,8 +202,9 @@ def serve(reqs: list[Request], pool: Pool, budget_bytes: int, tuner=None,
     the waiting request with the least estimated GPU time, `est_seconds`,
     less `aging` times the seconds it has waited, so long requests are not
     held back for ever).  The defaults (shortest first, ageing 0.2, at most
-    16 jobs) were the best of a sweep on the served workload: fewer jobs in
-    flight contend less for the device, and short requests leave first.
+    3 jobs) were the best of a sweep on the served workload: a large bunch
+    fills the device alone, fewer jobs in flight contend less for it, and
+    short requests leave first.
     Latency = finish time - arrival time.  Returns p50/p99 latency and the
     particle-steps/s delivered over the session."""
     if policy not in ("fifo", "sjf"):
