@@ -57,7 +57,7 @@ struct Lines {
   double out_scale;
   // fused z pass: multiply by the half-grid G^ folded in the line's y index
   const void *green;
-  int64_t gkx, gky;        // G^ strides: (nx+1), (nx+1)*(ny+1)
+  int64_t gkx, gky;        // G^ strides of ky and kz: ldx*(nz+1), ldx
   int my;                  // doubled y length, for folding the line's y index
 };
 
@@ -803,11 +803,16 @@ int green_t(izh_field *f, hipStream_t gs) {
     p.df = 1; p.ds = (int64_t)kx * (ny + 1); p.de = kx;
     p.L = my; p.Lin = p.L; p.Lout = ny + 1;
     IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(gs, p, f->tw_y, false)));
-    // z: lines (i<=nx, j<=ny); green_t -> green, scaled by 1/(mx my mz)
+    // z: lines (i<=nx, j<=ny); green_t -> green, scaled by 1/(mx my mz).
+    // G^ is stored [j][k][i] with i rows at the spectra's pitch ldx: the
+    // fused z pass of a block of x lines then reads, for its one y index,
+    // aligned runs of neighbouring i at neighbouring k, one contiguous
+    // region of ldx*(nz+1) values instead of one value run per z plane.
+    const int64_t ld = f->ldx;
     p.src = f->green_t; p.dst = f->green;
     p.nf = kx; p.ns = ny + 1;
     p.sf = 1; p.ss = kx; p.se = (int64_t)kx * (ny + 1);
-    p.df = 1; p.ds = kx; p.de = (int64_t)kx * (ny + 1);
+    p.df = 1; p.ds = ld * (nz + 1); p.de = ld;
     p.L = mz; p.Lin = p.L; p.Lout = nz + 1;
     p.out_scale = 1.0 / ((double)mx * my * mz);
     IZH_CHECK((run_lines<T, kLoadEven, kStoreReal, false, true>(gs, p, f->tw_z, false)));
@@ -848,7 +853,7 @@ int convolve_t(izh_field *f) {
   p.sf = 1; p.ss = ld; p.se = ld * my;
   p.df = 1; p.ds = ld; p.de = ld * my;
   p.L = mz; p.Lin = nz; p.Lout = nz;
-  p.green = f->green; p.gkx = kx; p.gky = (int64_t)kx * (ny + 1); p.my = my;
+  p.green = f->green; p.gkx = ld * (nz + 1); p.gky = ld; p.my = my;
   IZH_CHECK((run_lines<T, kLoadCplx, kStoreCplx, true>(fs, p, f->tw_z, false)));
   // ---- inverse y: B -> A, keep j < ny
   p.green = nullptr;
@@ -917,7 +922,9 @@ int field_alloc(izh_field *f) {
   IZH_HIP(hipMalloc(&f->e, 4 * g * es));
   IZH_HIP(hipMalloc(&f->spec_a, f->ldx * f->ny * f->nz * 2 * es));
   IZH_HIP(hipMalloc(&f->spec_b, f->ldx * 2 * f->ny * f->nz * 2 * es));
-  IZH_HIP(hipMalloc(&f->green, ghalf * es));
+  // G^ at x pitch ldx ([j][k][i], written by its z pass; the x pass's
+  // output at pitch kx fits as well)
+  IZH_HIP(hipMalloc(&f->green, f->ldx * (f->ny + 1) * (f->nz + 1) * es));
   IZH_HIP(hipMalloc(&f->green_t, ghalf * es));
   IZH_HIP(hipMalloc((void **)&f->corner,
                     (size_t)(f->nx + 2) * (f->ny + 2) * (f->nz + 2) * sizeof(double)));

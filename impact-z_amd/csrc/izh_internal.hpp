@@ -179,7 +179,7 @@ struct izh_field {
                                 // to whole 128-byte rows
   void *spec_a = nullptr;       // complex T[ldx*ny*nz]
   void *spec_b = nullptr;       // complex T[ldx*2ny*nz]
-  void *green = nullptr;        // T[(nx+1)*(ny+1)*(nz+1)] transformed Green fn
+  void *green = nullptr;        // T[(ny+1)][(nz+1)][ldx] transformed Green fn
   void *green_t = nullptr;      // T[(nx+1)*(ny+1)*(nz+1)] work
   double *corner = nullptr;     // double[(nx+2)*(ny+2)*(nz+2)] IGF antiderivative
   void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;  // twiddles per length

@@ -92,7 +92,7 @@ def footprint(n: int, grid, precision: int) -> int:
     bunch = n * (2 * 48 + 4 * 4 + 3 * pe + 3 * 8) + 2048 * 52 * 8 \
         + 2 * (n // 16384 + tiles + 1) * 12 + 3 * tiles * 4
     field = g * (8 + 2 * pe + 4 * pe) + ldx * ny * nz * 2 * pe + ldx * 2 * ny * nz * 2 * pe \
-        + 2 * kx * (ny + 1) * (nz + 1) * pe + (nx + 2) * (ny + 2) * (nz + 2) * 8
+        + (kx + ldx) * (ny + 1) * (nz + 1) * pe + (nx + 2) * (ny + 2) * (nz + 2) * 8
     return int(bunch + field)
 
 
