@@ -79,6 +79,8 @@ class Simulation:
         self._left = 0      # segments left in the current element
         self.zedge = 0.0    # z of the current element's entrance
         self._rf = {}       # RF cavity fields on the device library's side
+        # read3_Data has filled the module's Fourier coefficient array (Fcoef)
+        self._fcoef = False
         self.wk = {"on": 0, "read": 0, "btw": 0, "scale": 1.0, "aa": 0.0, "gg": 0.0,
                    "leng": 0.0, "table": None}
         # the loop's CSR bookkeeping (src/Contrl/AccSimulator.f90:722-740,
@@ -192,6 +194,7 @@ class Simulation:
         # read3_Data's coefficients for SolRF and for every cavity under the
         # Lorentz integrator (src/Contrl/AccSimulator.f90:826-847)
         fourier = e.type == 105 or self.flagmap != 1
+        self._fcoef = self._fcoef or fourier
         key = (id(e.field), fourier)
         if key not in self._rf:
             self._rf[key] = (hip.RFData(e.field, fourier=fourier), e.field)
@@ -223,10 +226,22 @@ class Simulation:
         radius 100 m, the CSR wake when v3 > 200, kick1wake) and ISRchicane,
         then the second half step.  Zero current runs under either
         integrator, current under Flagmap=1; ISR takes its deviates from
-        isr_rng (set a seeded numpy Generator)."""
+        isr_rng (set a seeded numpy Generator).
+
+        Under Flagmap=2 the run loop also applies map2_BeamBunch's Lorentz
+        kick in the bend, whose field (getfld_Dipole, src/Appl/Dipole.f90:
+        361-414) takes its pole-face geometry from the Fourier coefficient
+        array that read3_Data fills for RF cavities (src/DataStruct/
+        Data.f90:54, 348-409).  While no cavity has filled it the array is
+        zero and the kick adds nothing, so it is left out; after a cavity
+        the bend is refused."""
         cur = self.beam.current
         if cur > 1.0e-15 and self.flagmap != 1:
             raise hip.IZHError("a bend with current runs under Flagmap=1 only")
+        if self.flagmap != 1 and self._fcoef:
+            raise hip.IZHError("a bend under Flagmap=2 after an RF cavity: its Lorentz "
+                               "kick would read the cavity's Fourier coefficients as "
+                               "pole-face geometry (getfld_Dipole), which is not restated")
         if cur > 1.0e-15 and self.isr_rng is None:
             raise hip.IZHError("a bend with current needs ISR deviates: set "
                                "Simulation.isr_rng to a seeded numpy Generator")

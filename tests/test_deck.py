@@ -299,3 +299,18 @@ def test_deck_refuses_lorentz_bend_after_fourier_coefficients(tmp_path):
                 program_of(deck)
         else:
             program_of(deck)
+
+
+def test_simulation_refuses_lorentz_bend_after_cavity():
+    """The same refusal for a Simulation driven element by element: once a
+    cavity's Fourier table has been built under Flagmap=2, a bend raises
+    before it touches the bunch."""
+    from impactz import beam as B, hip
+    from impactz.sim import Simulation
+    sim = Simulation.__new__(Simulation)
+    sim.beam, sim.flagmap, sim._fcoef = B.Beam(current=0.0), 2, True
+    with pytest.raises(hip.IZHError, match="Fourier coefficients"):
+        sim.bend_step(None)
+    sim.flagmap = 1
+    with pytest.raises(AttributeError):   # past the check, into the bunch
+        sim.bend_step(None)
