@@ -37,7 +37,10 @@ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-constexpr int kFftThreads = 256;
+// Threads per line-kernel block: 512 puts four waves per SIMD on a CU that
+// holds two 4096-point z-pass blocks in LDS (256 left two, and the solve
+// 8% slower).
+constexpr int kFftThreads = 512;
 constexpr int kFftPoints = 2048;  // points per block in LDS
 constexpr int kFftPointsZ = 4096; // the fused z pass: 8 lines of 512 at 128x128x256
 constexpr int kFftMinLine = 8;    // shortest line (a mesh of 4 nodes doubled)
