@@ -64,6 +64,17 @@ struct Lines {
   int my;                  // doubled y length, for folding the line's y index
 };
 
+// Logical block of physical block p of n: the hardware deals blocks to the
+// 8 XCDs in turn (block p to XCD p mod 8), each XCD with its own L2.  XCD x
+// takes the x-th contiguous run of logical blocks, one after another, so
+// blocks whose lines share 128-byte lines of a spectrum or of G^ (FP32 rows,
+// G^'s runs of 8 values) read them through one L2.  A bijection of [0, n)
+// for any n (the first n mod 8 XCDs take one block more).
+__device__ inline int xcd_block(int p, int n) {
+  const int x = p & 7, base = n >> 3, extra = n & 7;
+  return x * base + min(x, extra) + (p >> 3);
+}
+
 __device__ inline int bitrev(int v, int bits) {
   return (int)(__brev((unsigned)v) >> (32 - bits));
 }
@@ -204,7 +215,7 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   }
   const int nf = (int)p.nf;
   const int nlines = nf * (int)p.ns;
-  const int line0 = blockIdx.x * B;
+  const int line0 = xcd_block((int)blockIdx.x, (int)gridDim.x) * B;
   // ---- load (zero padding / even extension / Hermitian completion): all of
   // a thread's loads are issued before the first LDS write
   constexpr int kPer = PTS / kFftThreads;  // B * L == PTS

@@ -136,6 +136,22 @@ def full_row_err(got, ref):
     return max(worst, float(np.max(np.abs(g[idx] - r[idx]) / scale)))
 
 
+def order_row_err(got, ref):
+    """full_row_err for two rows whose sums differ only in the order of their
+    terms.  A near-zero third moment's cube root (fort.29) magnifies a
+    rounding-level change of its sum without bound, so those six values are
+    compared as third moments, cubed, on the scale of the cube of the same
+    coordinate's fourth-moment root (fort.30); every other value as in
+    full_row_err."""
+    g = np.asarray(got, dtype=np.float64)[:41].copy()
+    r = np.asarray(ref, dtype=np.float64)[:41].copy()
+    s3 = np.abs(r[35:41]) ** 3
+    s3[s3 == 0] = 1.0
+    third = float(np.max(np.abs(g[29:35] ** 3 - r[29:35] ** 3) / s3))
+    g[29:35] = r[29:35]
+    return max(full_row_err(g, r), third)
+
+
 def fixture_row_err(got, d, k, prefix=""):
     """Error of a 41-value row against row k of a fixture: every column when
     the fixture holds all the diagnostic files, else fort.18 and 24-26."""

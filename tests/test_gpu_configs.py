@@ -16,7 +16,7 @@ mixed form stays within 1e-4 of FP64 in the rms moments after 400 steps.
 import numpy as np
 import pytest
 
-from conftest import col_err, full_row_err, rel_err
+from conftest import col_err, full_row_err, order_row_err, rel_err
 from test_gpu_parity import TOL, record
 
 pytestmark = pytest.mark.gpu
@@ -147,14 +147,21 @@ def test_c3_split_run_equals_one_run(gpu, c3_input):
     a.close()
     b = Simulation(pts, bm, lat, C3_GRID)
     rb = np.concatenate([b.run_fused(4), b.run_fused(4, start=4)])
-    worst = max(full_row_err(b.diag(rb[k]), b.diag(ra[k])) for k in range(8))
+    worst = max(order_row_err(b.diag(rb[k]), b.diag(ra[k])) for k in range(8))
+    rows = [(np.asarray(b.diag(rb[k]))[:41], np.asarray(b.diag(ra[k]))[:41]) for k in range(8)]
+    own = [[full_row_err(np.where(np.arange(41) == c, g, r), r) for c in range(41)]
+           for g, r in rows]
+    k_own, c_own = np.unravel_index(int(np.argmax(own)), (8, 41))
+    own = {"value": float(np.max(own)), "step": int(k_own), "column": int(c_own)}
     pe = col_err(b.bunch.getpts(), pa)
     b.close()
-    record(test="c3_split_run", steps=8, moments=worst, particles=pe)
+    record(test="c3_split_run", steps=8, moments=worst, moments_own_scale=own, particles=pe)
     # the particles are bitwise the same (fixed-point charge, exact ranges);
     # the moment sums are not, since the atomics of the tile sort order the
-    # particles inside a tile differently from run to run: at 50M particles
-    # the cube roots of the near-zero third moments show that at ~1e-11
+    # particles inside a tile differently from run to run.  Measured on each
+    # value's own magnitude, the cube roots of near-zero third moments turn
+    # that into 1e-11 to 1.5e-10 at 50M particles, so they are compared as
+    # third moments on their coordinate's scale
     assert pe == 0.0
     assert worst < TOL
 
