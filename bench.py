@@ -458,7 +458,7 @@ def main():
                          "mixed-vs-FP64 error report")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the multi-GPU path is a "
+        print("bench.py: measures one GPU; the multi-GPU path is a "
               "design only (no --gpus above 1)", file=sys.stderr)
         sys.exit(2)
     grid = tuple(int(v) for v in args.grid.split(","))

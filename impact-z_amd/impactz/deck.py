@@ -7,7 +7,7 @@ records are whitespace separated, lines whose first non-blank character is
 ``!`` are comments, Fortran ``d`` exponents are accepted and a lattice line
 ends at ``/``.  The element table follows the lattice branch of
 run_AccSimulator (src/Contrl/AccSimulator.f90:237-536, 742-1504) for the
-elements this build runs: drifts (type 0), quadrupoles (type 1: constant
+elements this package runs: drifts (type 0), quadrupoles (type 1: constant
 gradient with file ID <= 1e-5, src/Appl/Quadrupole.f90:265-270, or given by
 K with file ID < -1e-6, src/Appl/BeamBunch.f90:305-307), constant focusing
 (type 2) and solenoids (type 3), under the linear map integrator (Flagmap=1)
@@ -192,7 +192,7 @@ class MapSwitch:
 
 
 def program_of(deck: Deck) -> tuple[list, list[str]]:
-    """The deck's element table as the sequence this build runs: (deck index,
+    """The deck's element table as the sequence this package runs: (deck index,
     Element, FileAction or MapSwitch) in deck order, and notes on the
     diagnostic elements skipped.  Raises DeckError for anything the GPU path
     does not implement under the integrator in effect at that element."""
@@ -293,7 +293,7 @@ def program_of(deck: Deck) -> tuple[list, list[str]]:
 
 
 def lattice_of(deck: Deck) -> tuple[list[Element], list[str]]:
-    """The element table of the deck as Elements of this build, and notes on
+    """The element table of the deck as this package's Elements, and notes on
     the file-writing elements left out of it."""
     prog, notes = program_of(deck)
     out = []

@@ -167,7 +167,7 @@ def test_c3_split_run_equals_one_run(gpu, c3_input):
 
 
 def test_c3_mixed_within_1e4_of_fp64_after_400_steps(gpu, c3_input):
-    """The north star's accuracy bar for the mixed form at full size: rms
+    """The accuracy bar of the mixed form at full size: rms
     moments within 1e-4 relative of the FP64 form over 400 steps (25 FODO
     periods), and the field L2 error at the last step reported."""
     from impactz import beam as B
