@@ -94,6 +94,36 @@ __device__ inline int bitrev(int v, int bits) {
 // is 15 for lines of 64 points or more, else 0 (short lines do no better).
 __device__ inline int lds_pos(int m, int mask) { return m ^ ((m >> 3) & mask); }
 
+// The K stages of one run on a group held in registers: v[j] is position
+// m0 + j * 2^ls of its line, r = m0 mod 2^ls.
+template <typename T, bool DIT, int K>
+__device__ __attribute__((always_inline)) inline void run_regs(cx<T> (&v)[1 << K], int ls,
+                                                               int r, int log2L,
+                                                               const cx<T> *tw, bool inverse) {
+  constexpr int span = 1 << K;
+#pragma unroll
+  for (int st = 0; st < K; st++) {
+    const int e = DIT ? st : K - 1 - st;  // pair distance 2^e in the group
+    const int lt = log2L - 1 - ls - e;    // log2 of the twiddle stride
+#pragma unroll
+    for (int j = 0; j < span; j++) {
+      if ((j >> e) & 1) continue;
+      const int k = r + ((j & ((1 << e) - 1)) << ls);
+      cx<T> w = tw[k << lt];
+      if (inverse) w.im = -w.im;
+      const cx<T> u = v[j], x = v[j + (1 << e)];
+      if (DIT) {
+        const cx<T> y = cmul<T>(x, w);
+        v[j] = {u.re + y.re, u.im + y.im};
+        v[j + (1 << e)] = {u.re - y.re, u.im - y.im};
+      } else {
+        v[j] = {u.re + x.re, u.im + x.im};
+        v[j + (1 << e)] = cmul<T>({u.re - x.re, u.im - x.im}, w);
+      }
+    }
+  }
+}
+
 template <typename T, bool DIT, int K>
 __device__ inline void fft_run(cx<T> *buf, int log2L, int LP, int lines, const cx<T> *tw,
                                bool inverse, int done, int mask) {
@@ -111,39 +141,48 @@ __device__ inline void fft_run(cx<T> *buf, int log2L, int LP, int lines, const c
     cx<T> v[span];
 #pragma unroll
     for (int j = 0; j < span; j++) v[j] = buf[row + lds_pos(m0 + j * s, mask)];
-#pragma unroll
-    for (int st = 0; st < K; st++) {
-      const int e = DIT ? st : K - 1 - st;  // pair distance 2^e in the group
-      const int lt = log2L - 1 - ls - e;    // log2 of the twiddle stride
-#pragma unroll
-      for (int j = 0; j < span; j++) {
-        if ((j >> e) & 1) continue;
-        const int k = r + ((j & ((1 << e) - 1)) << ls);
-        cx<T> w = tw[k << lt];
-        if (inverse) w.im = -w.im;
-        const cx<T> u = v[j], x = v[j + (1 << e)];
-        if (DIT) {
-          const cx<T> y = cmul<T>(x, w);
-          v[j] = {u.re + y.re, u.im + y.im};
-          v[j + (1 << e)] = {u.re - y.re, u.im - y.im};
-        } else {
-          v[j] = {u.re + x.re, u.im + x.im};
-          v[j + (1 << e)] = cmul<T>({u.re - x.re, u.im - x.im}, w);
-        }
-      }
-    }
+    run_regs<T, DIT, K>(v, ls, r, log2L, tw, inverse);
 #pragma unroll
     for (int j = 0; j < span; j++) buf[row + lds_pos(m0 + j * s, mask)] = v[j];
   }
   __syncthreads();
 }
 
+// The fused z pass's middle: the forward transform's last run (stages
+// log2L-3 .. log2L-1, groups of 8 neighbouring positions) and the inverse
+// transform's first run (stages 0 .. 2, the same groups) with the multiply by
+// G^ between them, in registers: two LDS round trips and two barriers fewer
+// than running them apart.  Thread t takes the group of positions 8q .. 8q+7
+// of line lb, lb = t mod B and q = t / B (lanes walk across the block's lines,
+// so their G^ values, gv[j] for position 8q + j, were read in runs of
+// neighbouring x lines); one group per thread (PTS / 8 == kFftThreads).
+template <typename T>
+__device__ __attribute__((always_inline)) inline void fft_conv_middle(
+    cx<T> *buf, int LP, int log2L, int log2B, const cx<T> *tw, bool inverse,
+    const T (&gv)[8], int mask) {
+  const int lb = threadIdx.x & ((1 << log2B) - 1), q = threadIdx.x >> log2B;
+  const int row = lb * LP, m0 = q << 3;
+  cx<T> v[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = buf[row + lds_pos(m0 + j, mask)];
+  run_regs<T, false, 3>(v, 0, 0, log2L, tw, inverse);
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = {v[j].re * gv[j], v[j].im * gv[j]};
+  run_regs<T, true, 3>(v, 0, 0, log2L, tw, true);
+#pragma unroll
+  for (int j = 0; j < 8; j++) buf[row + lds_pos(m0 + j, mask)] = v[j];
+  __syncthreads();
+}
+
+// Stages [from, to) of the transform (all of them by default), in runs of
+// up to three.
 template <typename T, bool DIT>
 __device__ __attribute__((always_inline)) inline void fft_stages(cx<T> *buf, int LP, int log2L, int lines, const cx<T> *tw,
-                           bool inverse, int mask) {
-  int done = 0;
-  while (done < log2L) {
-    const int k = min(3, log2L - done);
+                           bool inverse, int mask, int from = 0, int to = -1) {
+  if (to < 0) to = log2L;
+  int done = from;
+  while (done < to) {
+    const int k = min(3, to - done);
     if (k == 3)
       fft_run<T, DIT, 3>(buf, log2L, LP, lines, tw, inverse, done, mask);
     else if (k == 2)
@@ -291,42 +330,34 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     if (t < (L >> 1)) tw[t] = twv[it];
   }
   __syncthreads();
-  // fused z pass: fetch the line's G^ values now, so their latency hides
-  // behind the forward transform.  Lanes walk across the block's lines
-  // first (neighbouring x indices, neighbouring addresses of G^), then
-  // along the frequencies; the forward spectrum is bit-reversed, so natural
-  // frequency kz sits at position bitrev(kz) of its line.
-  T gv[CONV ? kPer : 1];
+  // fused z pass: fetch the G^ values of the thread's group of the fused
+  // middle run now, so their latency hides behind the forward transform.
+  // Lanes walk across the block's lines first (neighbouring x indices,
+  // neighbouring addresses of G^); the forward spectrum is bit-reversed, so
+  // position m of a line holds natural frequency kz = bitrev(m).
+  static_assert(!CONV || PTS / 8 == kFftThreads, "one middle-run group per thread");
+  T gv[8];
   if (CONV) {
     const T *G = (const T *)p.green;
+    const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
+    const int line = line0 + lb;
+    int i, j2;
+    line_fs(line, p, i, j2);
+    const bool live = line < nlines && i < p.nfv;
+    const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
 #pragma unroll
-    for (int it = 0; it < kPer; it++) {
-      const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t >> log2B, lb = t & (B - 1);
-      const int line = line0 + lb;
-      int i, j2;
-      line_fs(line, p, i, j2);
+    for (int j = 0; j < 8; j++) {
+      const int kz = bitrev((q << 3) + j, log2L);
+      const int kf = kz <= (L >> 1) ? kz : L - kz;
       T g = (T)0;
-      if (line < nlines && i < p.nfv) {
-        const int jf = j2 <= p.my / 2 ? j2 : p.my - j2;
-        const int kf = kz <= (L >> 1) ? kz : L - kz;
-        g = G[i + p.gkx * jf + p.gky * kf];
-      }
-      gv[it] = g;
+      if (live) g = G[i + p.gkx * jf + p.gky * kf];
+      gv[j] = g;
     }
-  }
-  fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask);
-  if (CONV) {
-#pragma unroll
-    for (int it = 0; it < kPer; it++) {
-      const int t = threadIdx.x + it * kFftThreads;
-      const int kz = t >> log2B, lb = t & (B - 1);
-      const int q = lb * LP + lds_pos(bitrev(kz, log2L), mask);
-      const cx<T> v = buf[q];
-      buf[q] = {v.re * gv[it], v.im * gv[it]};
-    }
-    __syncthreads();
-    fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask);
+    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, 0, log2L - 3);
+    fft_conv_middle<T>(buf, LP, log2L, log2B, tw, inverse != 0, gv, mask);
+    fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3);
+  } else {
+    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask);
   }
   // ---- store the kept part of each line
   const int totout = B * L;
