@@ -255,83 +255,8 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   const int nf = (int)p.nf;
   const int nlines = nf * (int)p.ns;
   const int line0 = xcd_block((int)blockIdx.x, (int)gridDim.x) * B;
-  // ---- load (zero padding / even extension / Hermitian completion): all of
-  // a thread's loads are issued before the first LDS write
-  constexpr int kPer = PTS / kFftThreads;  // B * L == PTS
-  cx<T> ld[kPer];
-  int slot[kPer];
-#pragma unroll
-  for (int it = 0; it < kPer; it++) {
-    const int t = threadIdx.x + it * kFftThreads;
-    int lb, m;
-    map_thread(t, p.se, log2L, log2B, lb, m);
-    const int line = line0 + lb;
-    slot[it] = lb * LP + lds_pos(m, mask);
-    T re = (T)0, im = (T)0;
-    if (PAIR) {
-      const int l0 = 2 * line, l1 = l0 + 1;
-      const T *src = (const T *)p.src;
-      const cx<T> *csrc = (const cx<T> *)p.src;
-      if (l0 < nlines) {
-        int f0, s0, f1, s1;
-        line_fs(l0, p, f0, s0);
-        line_fs(l1, p, f1, s1);
-        const int64_t b0 = f0 * p.sf + s0 * p.ss;
-        const bool has1 = l1 < nlines;
-        const int64_t b1 = has1 ? f1 * p.sf + s1 * p.ss : b0;
-        if (LOAD == kLoadEven) {
-          const int mm = m <= (L >> 1) ? m : L - m;
-          re = src[b0 + mm * p.se];
-          im = has1 ? src[b1 + mm * p.se] : (T)0;
-        } else if (LOAD == kLoadReal) {
-          if (m < p.Lin) {
-            re = src[b0 + m * p.se];
-            im = has1 ? src[b1 + m * p.se] : (T)0;
-          }
-        } else {  // kLoadHerm: Z = X + iY of the completed spectra
-          const bool lo = m <= (L >> 1);
-          const int mm = lo ? m : L - m;
-          cx<T> x = csrc[b0 + mm * p.se];
-          cx<T> y = has1 ? csrc[b1 + mm * p.se] : cx<T>{(T)0, (T)0};
-          if (!lo) { x.im = -x.im; y.im = -y.im; }
-          re = x.re - y.im;
-          im = x.im + y.re;
-        }
-      }
-    } else if (line < nlines) {
-      int f, s;
-      line_fs(line, p, f, s);
-      const int64_t base = f * p.sf + s * p.ss;
-      if (f >= p.nfv) {
-        // a padding line: zeros
-      } else if (LOAD == kLoadReal) {
-        if (m < p.Lin) re = ((const T *)p.src)[base + m * p.se];
-      } else if (LOAD == kLoadCplx) {
-        if (m < p.Lin) {
-          const cx<T> v = ((const cx<T> *)p.src)[base + m * p.se];
-          re = v.re; im = v.im;
-        }
-      } else if (LOAD == kLoadEven) {
-        const int mm = m <= (L >> 1) ? m : L - m;
-        re = ((const T *)p.src)[base + mm * p.se];
-      } else {  // kLoadHerm: first L/2+1 values stored
-        const bool lo = m <= (L >> 1);
-        const cx<T> v = ((const cx<T> *)p.src)[base + (lo ? m : L - m) * p.se];
-        re = v.re; im = lo ? v.im : -v.im;
-      }
-    }
-    ld[it] = {re, im};
-  }
-#pragma unroll
-  for (int it = 0; it < kPer; it++) buf[slot[it]] = ld[it];
-#pragma unroll
-  for (int it = 0; it < kTwPer; it++) {
-    const int t = threadIdx.x + it * kFftThreads;
-    if (t < (L >> 1)) tw[t] = twv[it];
-  }
-  __syncthreads();
   // fused z pass: fetch the G^ values of the thread's group of the fused
-  // middle run now, so their latency hides behind the forward transform.
+  // middle run first, so their latency hides behind the forward transform.
   // Lanes walk across the block's lines first (neighbouring x indices,
   // neighbouring addresses of G^); the forward spectrum is bit-reversed, so
   // position m of a line holds natural frequency kz = bitrev(m).
@@ -353,11 +278,133 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       if (live) g = G[i + p.gkx * jf + p.gky * kf];
       gv[j] = g;
     }
-    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, 0, log2L - 3);
+  }
+  // Complex lines walked across (unit line stride, element stride other
+  // than one: the y passes and the fused z pass) skip the load's LDS round
+  // trip: thread t loads the 8 points of its group of the first radix-8
+  // run, positions q + j L/8 of line lb (lb = t mod B, q = t / B: lanes walk
+  // across the block's neighbouring lines, so each run of B points is one
+  // 128-byte line), straight from global memory and runs those three
+  // stages in registers before its first LDS write.
+  constexpr bool kFuseLoad = !PAIR && LOAD == kLoadCplx;
+  const bool fuse = kFuseLoad && p.se != 1 && p.sf == 1 && log2L >= 6;
+  int first = 0;  // stages run before the data reaches LDS
+  if (fuse) {
+    const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
+    const bool act = threadIdx.x < (PTS >> 3);  // one group per thread
+    const int line = line0 + lb;
+    int f = 0, sl = 0;
+    line_fs(line, p, f, sl);
+    const bool live = act && line < nlines && f < p.nfv;
+    const int64_t base = f * p.sf + sl * p.ss;
+    const int s8 = L >> 3;
+    cx<T> v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int m = q + j * s8;
+      T re = (T)0, im = (T)0;
+      if (live && m < p.Lin) {
+        const cx<T> x = ((const cx<T> *)p.src)[base + m * p.se];
+        re = x.re; im = x.im;
+      }
+      v[j] = {re, im};
+    }
+#pragma unroll
+    for (int it = 0; it < kTwPer; it++) {
+      const int t = threadIdx.x + it * kFftThreads;
+      if (t < (L >> 1)) tw[t] = twv[it];
+    }
+    __syncthreads();
+    if (act) {
+      run_regs<T, false, 3>(v, log2L - 3, q, log2L, tw, inverse != 0);
+#pragma unroll
+      for (int j = 0; j < 8; j++) buf[lb * LP + lds_pos(q + j * s8, mask)] = v[j];
+    }
+    __syncthreads();
+    first = 3;
+  } else {
+    // ---- load (zero padding / even extension / Hermitian completion): all of
+    // a thread's loads are issued before the first LDS write
+    constexpr int kPer = PTS / kFftThreads;  // B * L == PTS
+    cx<T> ld[kPer];
+    int slot[kPer];
+#pragma unroll
+    for (int it = 0; it < kPer; it++) {
+      const int t = threadIdx.x + it * kFftThreads;
+      int lb, m;
+      map_thread(t, p.se, log2L, log2B, lb, m);
+      const int line = line0 + lb;
+      slot[it] = lb * LP + lds_pos(m, mask);
+      T re = (T)0, im = (T)0;
+      if (PAIR) {
+        const int l0 = 2 * line, l1 = l0 + 1;
+        const T *src = (const T *)p.src;
+        const cx<T> *csrc = (const cx<T> *)p.src;
+        if (l0 < nlines) {
+          int f0, s0, f1, s1;
+          line_fs(l0, p, f0, s0);
+          line_fs(l1, p, f1, s1);
+          const int64_t b0 = f0 * p.sf + s0 * p.ss;
+          const bool has1 = l1 < nlines;
+          const int64_t b1 = has1 ? f1 * p.sf + s1 * p.ss : b0;
+          if (LOAD == kLoadEven) {
+            const int mm = m <= (L >> 1) ? m : L - m;
+            re = src[b0 + mm * p.se];
+            im = has1 ? src[b1 + mm * p.se] : (T)0;
+          } else if (LOAD == kLoadReal) {
+            if (m < p.Lin) {
+              re = src[b0 + m * p.se];
+              im = has1 ? src[b1 + m * p.se] : (T)0;
+            }
+          } else {  // kLoadHerm: Z = X + iY of the completed spectra
+            const bool lo = m <= (L >> 1);
+            const int mm = lo ? m : L - m;
+            cx<T> x = csrc[b0 + mm * p.se];
+            cx<T> y = has1 ? csrc[b1 + mm * p.se] : cx<T>{(T)0, (T)0};
+            if (!lo) { x.im = -x.im; y.im = -y.im; }
+            re = x.re - y.im;
+            im = x.im + y.re;
+          }
+        }
+      } else if (line < nlines) {
+        int f, s;
+        line_fs(line, p, f, s);
+        const int64_t base = f * p.sf + s * p.ss;
+        if (f >= p.nfv) {
+          // a padding line: zeros
+        } else if (LOAD == kLoadReal) {
+          if (m < p.Lin) re = ((const T *)p.src)[base + m * p.se];
+        } else if (LOAD == kLoadCplx) {
+          if (m < p.Lin) {
+            const cx<T> v = ((const cx<T> *)p.src)[base + m * p.se];
+            re = v.re; im = v.im;
+          }
+        } else if (LOAD == kLoadEven) {
+          const int mm = m <= (L >> 1) ? m : L - m;
+          re = ((const T *)p.src)[base + mm * p.se];
+        } else {  // kLoadHerm: first L/2+1 values stored
+          const bool lo = m <= (L >> 1);
+          const cx<T> v = ((const cx<T> *)p.src)[base + (lo ? m : L - m) * p.se];
+          re = v.re; im = lo ? v.im : -v.im;
+        }
+      }
+      ld[it] = {re, im};
+    }
+#pragma unroll
+    for (int it = 0; it < kPer; it++) buf[slot[it]] = ld[it];
+#pragma unroll
+    for (int it = 0; it < kTwPer; it++) {
+      const int t = threadIdx.x + it * kFftThreads;
+      if (t < (L >> 1)) tw[t] = twv[it];
+    }
+    __syncthreads();
+  }
+  if (CONV) {
+    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, first, log2L - 3);
     fft_conv_middle<T>(buf, LP, log2L, log2B, tw, inverse != 0, gv, mask);
     fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3);
   } else {
-    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask);
+    fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, first);
   }
   // ---- store the kept part of each line
   const int totout = B * L;
