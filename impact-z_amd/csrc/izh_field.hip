@@ -402,6 +402,35 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   if (CONV) {
     fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, first, log2L - 3);
     fft_conv_middle<T>(buf, LP, log2L, log2B, tw, inverse != 0, gv, mask);
+    // The inverse transform's last radix-8 run leaves positions q + j L/8
+    // of each line in natural order: with lanes walking across the lines,
+    // the thread stores its group from registers (runs of B neighbouring x
+    // lines), skipping the store's LDS round trip.
+    if (log2L >= 6 && p.df == 1 && STORE == kStoreCplx) {
+      fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3, log2L - 3);
+      const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
+      const int s8 = L >> 3;
+      cx<T> v[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) v[j] = buf[lb * LP + lds_pos(q + j * s8, mask)];
+      run_regs<T, true, 3>(v, log2L - 3, q, log2L, tw, true);
+      const int line = line0 + lb;
+      if (line < nlines) {
+        int f, sl;
+        line_fs(line, p, f, sl);
+        if (f < p.nfv) {
+          const int64_t o = f * p.df + sl * p.ds;
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int m = q + j * s8;
+            if (m < p.Lout)
+              ((cx<T> *)p.dst)[o + m * p.de] = {(T)(v[j].re * p.out_scale),
+                                                (T)(v[j].im * p.out_scale)};
+          }
+        }
+      }
+      return;
+    }
     fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3);
   } else {
     fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, first);
