@@ -286,8 +286,15 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
   // across the block's neighbouring lines, so each run of B points is one
   // 128-byte line), straight from global memory and runs those three
   // stages in registers before its first LDS write.
+  // The y passes (complex lines in and out, both walked across) run as a
+  // decimation in time: the first radix-8 run takes its groups of
+  // bit-reversed positions 8q .. 8q+7, points bitrev(q) + bitrev3(j) L/8,
+  // straight from global memory, and the last leaves natural positions
+  // q + j L/8, stored from registers, so the data passes through LDS only
+  // between runs.
   constexpr bool kFuseLoad = !PAIR && LOAD == kLoadCplx;
   const bool fuse = kFuseLoad && p.se != 1 && p.sf == 1 && log2L >= 6;
+  const bool dit = fuse && !CONV && STORE == kStoreCplx && p.df == 1;
   int first = 0;  // stages run before the data reaches LDS
   if (fuse) {
     const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
@@ -298,10 +305,11 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     const bool live = act && line < nlines && f < p.nfv;
     const int64_t base = f * p.sf + sl * p.ss;
     const int s8 = L >> 3;
+    const int qr = dit ? bitrev(q, log2L - 3) : q;
     cx<T> v[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const int m = q + j * s8;
+      const int m = qr + (dit ? bitrev(j, 3) : j) * s8;
       T re = (T)0, im = (T)0;
       if (live && m < p.Lin) {
         const cx<T> x = ((const cx<T> *)p.src)[base + m * p.se];
@@ -315,7 +323,11 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
       if (t < (L >> 1)) tw[t] = twv[it];
     }
     __syncthreads();
-    if (act) {
+    if (act && dit) {
+      run_regs<T, true, 3>(v, 0, 0, log2L, tw, inverse != 0);
+#pragma unroll
+      for (int j = 0; j < 8; j++) buf[lb * LP + lds_pos((q << 3) + j, mask)] = v[j];
+    } else if (act) {
       run_regs<T, false, 3>(v, log2L - 3, q, log2L, tw, inverse != 0);
 #pragma unroll
       for (int j = 0; j < 8; j++) buf[lb * LP + lds_pos(q + j * s8, mask)] = v[j];
@@ -399,36 +411,43 @@ k_fft_lines(Lines p, const cx<T> *__restrict__ twg, int inverse) {
     }
     __syncthreads();
   }
+  // The last run of a decimation in time leaves positions q + j L/8 of
+  // each line in natural order: with lanes walking across the lines, the
+  // thread stores its group from registers (runs of B neighbouring x
+  // lines), skipping the store's LDS round trip.
+  auto last_run_store = [&](bool inv) {
+    const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
+    if (threadIdx.x >= (PTS >> 3)) return;  // one group per thread
+    const int s8 = L >> 3;
+    cx<T> v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = buf[lb * LP + lds_pos(q + j * s8, mask)];
+    run_regs<T, true, 3>(v, log2L - 3, q, log2L, tw, inv);
+    const int line = line0 + lb;
+    if (line >= nlines) return;
+    int f, sl;
+    line_fs(line, p, f, sl);
+    if (f >= p.nfv) return;
+    const int64_t o = f * p.df + sl * p.ds;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int m = q + j * s8;
+      if (m < p.Lout)
+        ((cx<T> *)p.dst)[o + m * p.de] = {(T)(v[j].re * p.out_scale),
+                                          (T)(v[j].im * p.out_scale)};
+    }
+  };
+  if (dit) {
+    fft_stages<T, true>(buf, LP, log2L, B, tw, inverse != 0, mask, 3, log2L - 3);
+    last_run_store(inverse != 0);
+    return;
+  }
   if (CONV) {
     fft_stages<T, false>(buf, LP, log2L, B, tw, inverse != 0, mask, first, log2L - 3);
     fft_conv_middle<T>(buf, LP, log2L, log2B, tw, inverse != 0, gv, mask);
-    // The inverse transform's last radix-8 run leaves positions q + j L/8
-    // of each line in natural order: with lanes walking across the lines,
-    // the thread stores its group from registers (runs of B neighbouring x
-    // lines), skipping the store's LDS round trip.
     if (log2L >= 6 && p.df == 1 && STORE == kStoreCplx) {
       fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3, log2L - 3);
-      const int lb = threadIdx.x & (B - 1), q = threadIdx.x >> log2B;
-      const int s8 = L >> 3;
-      cx<T> v[8];
-#pragma unroll
-      for (int j = 0; j < 8; j++) v[j] = buf[lb * LP + lds_pos(q + j * s8, mask)];
-      run_regs<T, true, 3>(v, log2L - 3, q, log2L, tw, true);
-      const int line = line0 + lb;
-      if (line < nlines) {
-        int f, sl;
-        line_fs(line, p, f, sl);
-        if (f < p.nfv) {
-          const int64_t o = f * p.df + sl * p.ds;
-#pragma unroll
-          for (int j = 0; j < 8; j++) {
-            const int m = q + j * s8;
-            if (m < p.Lout)
-              ((cx<T> *)p.dst)[o + m * p.de] = {(T)(v[j].re * p.out_scale),
-                                                (T)(v[j].im * p.out_scale)};
-          }
-        }
-      }
+      last_run_store(true);
       return;
     }
     fft_stages<T, true>(buf, LP, log2L, B, tw, true, mask, 3);
